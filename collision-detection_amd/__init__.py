@@ -1,0 +1,314 @@
+"""Python front end of libcdx.so (include/cdx.h) for tests and benchmarks.
+
+The product is the C ABI; this module only binds it with ctypes and moves
+numpy arrays across.  It never falls back to a CPU path: if libcdx.so is
+missing or no GPU is visible, every call raises.
+
+Load it from the repository root with::
+
+    import importlib.util, sys
+    spec = importlib.util.spec_from_file_location(
+        "collision_detection_amd", "collision-detection_amd/__init__.py",
+        submodule_search_locations=["collision-detection_amd"])
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcdx.so")
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+REAL_F64, REAL_F32 = 0, 1
+NP_NATIVE, NP_MIXED = 0, 1
+STAGES = ["aabb", "bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs",
+          "bp_pairsort", "np_main", "np_resolve", "np_compact"]
+
+# Every extern "C" symbol include/cdx.h declares.
+EXPORTS = [
+    "cdx_version", "cdx_status_string", "cdx_device_count", "cdx_ctx_create", "cdx_ctx_destroy",
+    "cdx_ctx_last_error", "cdx_generate_aabb", "cdx_detect_possible_collisions", "cdx_read_possible_collisions",
+    "cdx_do_narrowphase", "cdx_read_contacts", "cdx_scene_create", "cdx_scene_destroy", "cdx_scene_last_error",
+    "cdx_scene_set_shapes", "cdx_scene_set_bodies", "cdx_scene_update_bodies", "cdx_scene_set_aabbs",
+    "cdx_scene_set_pairs", "cdx_scene_set_bins", "cdx_scene_set_envelope", "cdx_scene_set_narrowphase_mode",
+    "cdx_scene_set_profiling", "cdx_scene_aabb", "cdx_scene_broadphase", "cdx_scene_narrowphase",
+    "cdx_scene_step", "cdx_scene_sync", "cdx_scene_counts", "cdx_scene_read_aabb", "cdx_scene_read_bins",
+    "cdx_scene_read_pairs", "cdx_scene_read_pairs_permuted", "cdx_scene_read_contacts", "cdx_scene_stage_ms",
+    "cdx_scene_device_bytes",
+]
+
+
+class CdxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"cdx status {status}: {msg}")
+        self.status = status
+
+
+class Counts(C.Structure):
+    _fields_ = [
+        ("n_shapes", C.c_uint64), ("bin_entries", C.c_uint64), ("active_bins", C.c_uint64),
+        ("pairs", C.c_uint64), ("contacts", C.c_uint64), ("resolved", C.c_uint64),
+        ("key_max", C.c_uint32), ("flags", C.c_uint32),
+        ("bmin", C.c_float * 3), ("bmax", C.c_float * 3),
+    ]
+
+
+def build(arch: str = "gfx950") -> str:
+    """Compile libcdx.so in-tree with hipcc (cross-compiles without a GPU)."""
+    subprocess.check_call(["make", "-s", "-C", CSRC, f"ARCH={arch}", "-j8"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CdxError(-1, f"{LIB_PATH} is not built: run build() first")
+    L = C.CDLL(LIB_PATH)
+    vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
+    sig = {
+        "cdx_version": ([], i32),
+        "cdx_status_string": ([i32], C.c_char_p),
+        "cdx_device_count": ([vp], i32),
+        "cdx_ctx_create": ([i32, vp], i32),
+        "cdx_ctx_destroy": ([vp], i32),
+        "cdx_ctx_last_error": ([vp], C.c_char_p),
+        "cdx_generate_aabb": ([vp, i32, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp], i32),
+        "cdx_detect_possible_collisions": ([vp, i32, vp, u64, i32, i32, i32, vp], i32),
+        "cdx_read_possible_collisions": ([vp, vp, u64], i32),
+        "cdx_do_narrowphase": ([vp, i32, i32, C.c_double, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp],
+                               i32),
+        "cdx_read_contacts": ([vp, vp, vp, vp, vp, vp, u64], i32),
+        "cdx_scene_create": ([vp, i32, vp], i32),
+        "cdx_scene_destroy": ([vp], i32),
+        "cdx_scene_last_error": ([vp], C.c_char_p),
+        "cdx_scene_set_shapes": ([vp, u64, vp, vp, vp, vp, vp, vp], i32),
+        "cdx_scene_set_bodies": ([vp, u64, vp, vp, vp], i32),
+        "cdx_scene_update_bodies": ([vp, u64, vp, vp, vp], i32),
+        "cdx_scene_set_aabbs": ([vp, u64, vp], i32),
+        "cdx_scene_set_pairs": ([vp, u64, vp], i32),
+        "cdx_scene_set_bins": ([vp, i32, i32, i32], i32),
+        "cdx_scene_set_envelope": ([vp, C.c_double], i32),
+        "cdx_scene_set_narrowphase_mode": ([vp, i32, C.c_double], i32),
+        "cdx_scene_set_profiling": ([vp, i32], i32),
+        "cdx_scene_aabb": ([vp], i32),
+        "cdx_scene_broadphase": ([vp], i32),
+        "cdx_scene_narrowphase": ([vp], i32),
+        "cdx_scene_step": ([vp], i32),
+        "cdx_scene_sync": ([vp], i32),
+        "cdx_scene_counts": ([vp, vp], i32),
+        "cdx_scene_read_aabb": ([vp, i32, vp], i32),
+        "cdx_scene_read_bins": ([vp, vp, vp, vp, vp, vp], i32),
+        "cdx_scene_read_pairs": ([vp, vp], i32),
+        "cdx_scene_read_pairs_permuted": ([vp, vp], i32),
+        "cdx_scene_read_contacts": ([vp, vp, vp, vp, vp, vp], i32),
+        "cdx_scene_stage_ms": ([vp, vp], i32),
+        "cdx_scene_device_bytes": ([vp, vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _kind(dtype):
+    return REAL_F64 if np.dtype(dtype) == np.float64 else REAL_F32
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    lib().cdx_device_count(C.byref(c))
+    return c.value
+
+
+class Context:
+    """One cdx_ctx on a device; also hosts the drop-in one-shot calls."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        st = lib().cdx_ctx_create(device, C.byref(self.h))
+        if st:
+            raise CdxError(st, lib().cdx_status_string(st).decode())
+
+    def close(self):
+        if self.h:
+            lib().cdx_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def _chk(self, st):
+        if st:
+            raise CdxError(st, lib().cdx_ctx_last_error(self.h).decode())
+
+    # -- drop-in calls ------------------------------------------------------
+    def generate_aabb(self, s) -> np.ndarray:
+        out = np.zeros((2 * s.n, 3), dtype=s.dtype)
+        self._chk(lib().cdx_generate_aabb(self.h, _kind(s.dtype), s.n, _p(s.T), _p(s.A), _p(s.B), _p(s.C), _p(s.R),
+                                          _p(s.ID), s.nb, _p(s.pos), _p(s.rot), _p(out)))
+        return out
+
+    def detect_possible_collisions(self, aabb: np.ndarray, bins) -> np.ndarray:
+        """Runs on aabb in place (shifted like the reference) and returns the pair list."""
+        assert aabb.flags.c_contiguous
+        npairs = C.c_uint64(0)
+        self._chk(lib().cdx_detect_possible_collisions(self.h, _kind(aabb.dtype), _p(aabb), aabb.shape[0] // 2,
+                                                       int(bins[0]), int(bins[1]), int(bins[2]), C.byref(npairs)))
+        out = np.zeros(npairs.value, dtype=np.int64)
+        self._chk(lib().cdx_read_possible_collisions(self.h, _p(out), npairs.value))
+        return out
+
+    def do_narrowphase(self, s, pairs: np.ndarray, envelope=0.0, mode=NP_NATIVE):
+        """Returns (norm, cpta, cptb, dpth, bids); pairs is permuted in place."""
+        assert pairs.dtype == np.int64 and pairs.flags.c_contiguous
+        nc = C.c_uint64(0)
+        self._chk(lib().cdx_do_narrowphase(self.h, _kind(s.dtype), mode, float(envelope), s.n, _p(s.T), _p(s.A),
+                                           _p(s.B), _p(s.C), _p(s.R), _p(s.ID), s.nb, _p(s.active), _p(s.pos),
+                                           _p(s.rot), _p(pairs), len(pairs), C.byref(nc)))
+        k = nc.value
+        dt = s.dtype
+        norm, pa, pb = (np.zeros((k, 3), dt) for _ in range(3))
+        d = np.zeros(k, dt)
+        ids = np.zeros((k, 2), np.int32)
+        self._chk(lib().cdx_read_contacts(self.h, _p(norm), _p(pa), _p(pb), _p(d), _p(ids), k))
+        return norm, pa, pb, d, ids
+
+
+class Scene:
+    """A device-resident scene (cdx_scene)."""
+
+    def __init__(self, ctx: Context, dtype=np.float64):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.h = C.c_void_p()
+        ctx._chk(lib().cdx_scene_create(ctx.h, _kind(dtype), C.byref(self.h)))
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            lib().cdx_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def _chk(self, st):
+        if st:
+            raise CdxError(st, lib().cdx_scene_last_error(self.h).decode())
+
+    def load(self, s, bins=None):
+        assert np.dtype(s.dtype) == self.dtype
+        self._keep = s
+        self.n = s.n
+        self._chk(lib().cdx_scene_set_shapes(self.h, s.n, _p(s.T), _p(s.A), _p(s.B), _p(s.C), _p(s.R), _p(s.ID)))
+        self._chk(lib().cdx_scene_set_bodies(self.h, s.nb, _p(s.pos), _p(s.rot), _p(s.active)))
+        g = bins if bins is not None else s.grid
+        self.set_bins(*g)
+        return self
+
+    def set_bins(self, gx, gy, gz):
+        self._chk(lib().cdx_scene_set_bins(self.h, int(gx), int(gy), int(gz)))
+
+    def set_aabbs(self, aabb):
+        aabb = np.ascontiguousarray(aabb, dtype=self.dtype)
+        self.n = aabb.shape[0] // 2
+        self._chk(lib().cdx_scene_set_aabbs(self.h, self.n, _p(aabb)))
+
+    def set_pairs(self, pairs):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int64)
+        self._chk(lib().cdx_scene_set_pairs(self.h, len(pairs), _p(pairs)))
+
+    def update_bodies(self, idx, pos, rot):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        pos = np.ascontiguousarray(pos, dtype=self.dtype)
+        rot = np.ascontiguousarray(rot, dtype=self.dtype)
+        self._chk(lib().cdx_scene_update_bodies(self.h, len(idx), _p(idx), _p(pos), _p(rot)))
+
+    def set_envelope(self, e):
+        self._chk(lib().cdx_scene_set_envelope(self.h, float(e)))
+
+    def set_mode(self, mode, tau=1e-2):
+        self._chk(lib().cdx_scene_set_narrowphase_mode(self.h, int(mode), float(tau)))
+
+    def set_profiling(self, on=True):
+        self._chk(lib().cdx_scene_set_profiling(self.h, 1 if on else 0))
+
+    def aabb(self):
+        self._chk(lib().cdx_scene_aabb(self.h))
+
+    def broadphase(self):
+        self._chk(lib().cdx_scene_broadphase(self.h))
+
+    def narrowphase(self):
+        self._chk(lib().cdx_scene_narrowphase(self.h))
+
+    def step(self):
+        self._chk(lib().cdx_scene_step(self.h))
+
+    def sync(self):
+        self._chk(lib().cdx_scene_sync(self.h))
+
+    def counts(self) -> Counts:
+        c = Counts()
+        self._chk(lib().cdx_scene_counts(self.h, C.byref(c)))
+        return c
+
+    def read_aabb(self, shifted=False):
+        out = np.zeros((2 * self.n, 3), dtype=self.dtype)
+        self._chk(lib().cdx_scene_read_aabb(self.h, 1 if shifted else 0, _p(out)))
+        return out
+
+    def read_bins(self):
+        c = self.counts()
+        bc = np.zeros(self.n, np.uint32)
+        sk = np.zeros(c.bin_entries, np.uint32)
+        sb = np.zeros(c.bin_entries, np.uint32)
+        ak = np.zeros(c.active_bins, np.uint32)
+        ac = np.zeros(c.active_bins, np.uint32)
+        self._chk(lib().cdx_scene_read_bins(self.h, _p(bc), _p(sk), _p(sb), _p(ak), _p(ac)))
+        return bc, sk, sb, ak, ac
+
+    def read_pairs(self):
+        out = np.zeros(self.counts().pairs, np.int64)
+        self._chk(lib().cdx_scene_read_pairs(self.h, _p(out)))
+        return out
+
+    def read_pairs_permuted(self):
+        out = np.zeros(self.counts().pairs, np.int64)
+        self._chk(lib().cdx_scene_read_pairs_permuted(self.h, _p(out)))
+        return out
+
+    def read_contacts(self):
+        k = self.counts().contacts
+        dt = self.dtype
+        norm, pa, pb = (np.zeros((k, 3), dt) for _ in range(3))
+        d = np.zeros(k, dt)
+        ids = np.zeros((k, 2), np.int32)
+        self._chk(lib().cdx_scene_read_contacts(self.h, _p(norm), _p(pa), _p(pb), _p(d), _p(ids)))
+        return norm, pa, pb, d, ids
+
+    def stage_ms(self) -> dict:
+        out = np.zeros(len(STAGES), np.float64)
+        self._chk(lib().cdx_scene_stage_ms(self.h, _p(out)))
+        return dict(zip(STAGES, out.tolist()))
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64(0)
+        self._chk(lib().cdx_scene_device_bytes(self.h, C.byref(b)))
+        return b.value

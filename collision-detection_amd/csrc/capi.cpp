@@ -1,0 +1,272 @@
+// extern "C" layer of libcdx.so (declarations in include/cdx.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/cdx.h"
+#include "cdx_pipeline.hpp"
+#include "cdx_util.hpp"
+
+struct cdx_ctx {
+  int device = 0;
+  std::string err;
+  std::unique_ptr<cdx::SceneBase> aabb_scene[2];
+  std::unique_ptr<cdx::SceneBase> bp_scene[2];
+  std::unique_ptr<cdx::SceneBase> np_scene[2];
+  int last_bp_kind = -1, last_np_kind = -1;
+};
+
+struct cdx_scene {
+  cdx_ctx* ctx = nullptr;
+  std::unique_ptr<cdx::SceneBase> s;
+  std::string err;
+};
+
+namespace {
+
+template <class F> int guarded(std::string* err, F&& f) {
+  try {
+    f();
+    if (err) err->clear();
+    return CDX_OK;
+  } catch (const cdx::Error& e) {
+    if (err) *err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (err) *err = "host allocation failed";
+    return CDX_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    if (err) *err = e.what();
+    return CDX_ERR_INVALID;
+  }
+}
+
+void check_kind(int k) {
+  CDX_REQUIRE(k == CDX_REAL_F64 || k == CDX_REAL_F32, CDX_ERR_INVALID, "real_kind must be CDX_REAL_F64 or CDX_REAL_F32");
+}
+
+cdx::SceneBase& lazy(std::unique_ptr<cdx::SceneBase>& p, int kind, int device) {
+  if (!p) p = cdx::make_scene(kind, device);
+  return *p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdx_version(void) { return CDX_VERSION; }
+
+const char* cdx_status_string(int s) {
+  switch (s) {
+    case CDX_OK: return "ok";
+    case CDX_ERR_INVALID: return "invalid argument";
+    case CDX_ERR_HIP: return "HIP runtime error";
+    case CDX_ERR_UNSUPPORTED_SHAPE: return "unsupported shape type";
+    case CDX_ERR_CAPACITY: return "capacity exceeded";
+    case CDX_ERR_NO_DEVICE: return "no GPU device";
+    case CDX_ERR_OUT_OF_MEMORY: return "out of memory";
+    case CDX_ERR_NOT_READY: return "stage inputs not ready";
+    default: return "unknown status";
+  }
+}
+
+int cdx_device_count(int* count) {
+  return guarded(nullptr, [&] {
+    CDX_REQUIRE(count, CDX_ERR_INVALID, "null count");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *count = (e == hipSuccess) ? c : 0;
+  });
+}
+
+int cdx_ctx_create(int device, cdx_ctx** out) {
+  return guarded(nullptr, [&] {
+    CDX_REQUIRE(out, CDX_ERR_INVALID, "null output");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0) throw cdx::Error(CDX_ERR_NO_DEVICE, "no HIP device visible");
+    CDX_REQUIRE(device >= 0 && device < c, CDX_ERR_INVALID, "device index out of range");
+    CDX_HIP_CHECK(hipSetDevice(device));
+    auto* x = new cdx_ctx();
+    x->device = device;
+    *out = x;
+  });
+}
+
+int cdx_ctx_destroy(cdx_ctx* ctx) {
+  delete ctx;
+  return CDX_OK;
+}
+
+const char* cdx_ctx_last_error(const cdx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cdx_generate_aabb(cdx_ctx* ctx, int real_kind, uint64_t n, const int32_t* T, const void* A, const void* B,
+                      const void* C, const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos,
+                      const void* body_rot, void* aabb_out) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    check_kind(real_kind);
+    (void)C;
+    if (n == 0) return;
+    CDX_REQUIRE(T && A && B && R && ID && body_pos && body_rot && aabb_out, CDX_ERR_INVALID, "null input");
+    cdx::SceneBase& s = lazy(ctx->aabb_scene[real_kind], real_kind, ctx->device);
+    s.set_shapes(n, T, A, B, R, ID);
+    s.set_bodies(nb, body_pos, body_rot, nullptr);
+    s.run_aabb();
+    s.sync();
+    s.read_aabb(aabb_out, false);
+  });
+}
+
+int cdx_detect_possible_collisions(cdx_ctx* ctx, int real_kind, void* aabb_inout, uint64_t n_aabb, int bx, int by,
+                                   int bz, uint64_t* n_pairs) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    check_kind(real_kind);
+    CDX_REQUIRE(n_pairs, CDX_ERR_INVALID, "null n_pairs");
+    cdx::SceneBase& s = lazy(ctx->bp_scene[real_kind], real_kind, ctx->device);
+    ctx->last_bp_kind = real_kind;
+    s.set_grid(bx, by, bz);
+    CDX_REQUIRE(n_aabb == 0 || aabb_inout, CDX_ERR_INVALID, "null aabb_data");
+    s.set_aabbs(n_aabb, aabb_inout);
+    s.run_broadphase();
+    s.sync();
+    if (n_aabb) s.read_aabb(aabb_inout, true);
+    *n_pairs = s.counts().p;
+  });
+}
+
+int cdx_read_possible_collisions(cdx_ctx* ctx, int64_t* out, uint64_t capacity) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    CDX_REQUIRE(ctx->last_bp_kind >= 0, CDX_ERR_NOT_READY, "no broadphase has run on this context");
+    cdx::SceneBase& s = *ctx->bp_scene[ctx->last_bp_kind];
+    uint64_t p = s.counts().p;
+    CDX_REQUIRE(capacity >= p, CDX_ERR_CAPACITY, "pair buffer too small: need " + std::to_string(p));
+    CDX_REQUIRE(p == 0 || out, CDX_ERR_INVALID, "null output");
+    s.read_pairs(out);
+  });
+}
+
+int cdx_do_narrowphase(cdx_ctx* ctx, int real_kind, int np_mode, double envelope, uint64_t n, const int32_t* T,
+                       const void* A, const void* B, const void* C, const void* R, const uint32_t* ID, uint64_t nb,
+                       const uint8_t* obj_active, const void* body_pos, const void* body_rot, int64_t* pairs_inout,
+                       uint64_t n_pairs, uint64_t* n_contacts) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    check_kind(real_kind);
+    (void)C;
+    CDX_REQUIRE(n_contacts, CDX_ERR_INVALID, "null n_contacts");
+    cdx::SceneBase& s = lazy(ctx->np_scene[real_kind], real_kind, ctx->device);
+    ctx->last_np_kind = real_kind;
+    s.set_np_mode(np_mode, 1e-2);
+    s.set_envelope(envelope);
+    s.set_shapes(n, T, A, B, R, ID);
+    s.set_bodies(nb, body_pos, body_rot, obj_active);
+    s.set_pairs(n_pairs, pairs_inout);
+    s.run_narrowphase();
+    s.sync();
+    if (n_pairs) s.read_pairs_permuted(pairs_inout);
+    *n_contacts = s.counts().c;
+  });
+}
+
+int cdx_read_contacts(cdx_ctx* ctx, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids,
+                      uint64_t capacity) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    CDX_REQUIRE(ctx->last_np_kind >= 0, CDX_ERR_NOT_READY, "no narrowphase has run on this context");
+    cdx::SceneBase& s = *ctx->np_scene[ctx->last_np_kind];
+    uint64_t c = s.counts().c;
+    CDX_REQUIRE(capacity >= c, CDX_ERR_CAPACITY, "contact buffers too small: need " + std::to_string(c));
+    s.read_contacts(norm, cpta, cptb, dpth, bids);
+  });
+}
+
+// ---- scenes ---------------------------------------------------------------
+
+int cdx_scene_create(cdx_ctx* ctx, int real_kind, cdx_scene** out) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    check_kind(real_kind);
+    CDX_REQUIRE(out, CDX_ERR_INVALID, "null output");
+    auto* sc = new cdx_scene();
+    sc->ctx = ctx;
+    try {
+      sc->s = cdx::make_scene(real_kind, ctx->device);
+    } catch (...) {
+      delete sc;
+      throw;
+    }
+    *out = sc;
+  });
+}
+
+int cdx_scene_destroy(cdx_scene* scene) {
+  delete scene;
+  return CDX_OK;
+}
+
+const char* cdx_scene_last_error(const cdx_scene* scene) { return scene ? scene->err.c_str() : "null scene"; }
+
+#define SCENE_CALL(body)                                     \
+  if (!scene) return CDX_ERR_INVALID;                        \
+  return guarded(&scene->err, [&] { cdx::SceneBase& s = *scene->s; body; })
+
+int cdx_scene_set_shapes(cdx_scene* scene, uint64_t n, const int32_t* T, const void* A, const void* B, const void* C,
+                         const void* R, const uint32_t* ID) {
+  (void)C;
+  SCENE_CALL(CDX_REQUIRE(n == 0 || (T && A && B && R && ID), CDX_ERR_INVALID, "null input");
+             s.set_shapes(n, T, A, B, R, ID));
+}
+int cdx_scene_set_bodies(cdx_scene* scene, uint64_t nb, const void* pos, const void* rot, const uint8_t* active) {
+  SCENE_CALL(CDX_REQUIRE(nb == 0 || (pos && rot), CDX_ERR_INVALID, "null input"); s.set_bodies(nb, pos, rot, active));
+}
+int cdx_scene_update_bodies(cdx_scene* scene, uint64_t k, const uint32_t* idx, const void* pos, const void* rot) {
+  SCENE_CALL(CDX_REQUIRE(k == 0 || (idx && pos && rot), CDX_ERR_INVALID, "null input");
+             s.update_bodies(k, idx, pos, rot));
+}
+int cdx_scene_set_aabbs(cdx_scene* scene, uint64_t n, const void* aabb) {
+  SCENE_CALL(CDX_REQUIRE(n == 0 || aabb, CDX_ERR_INVALID, "null input"); s.set_aabbs(n, aabb));
+}
+int cdx_scene_set_pairs(cdx_scene* scene, uint64_t p, const int64_t* pairs) {
+  SCENE_CALL(CDX_REQUIRE(p == 0 || pairs, CDX_ERR_INVALID, "null input"); s.set_pairs(p, pairs));
+}
+int cdx_scene_set_bins(cdx_scene* scene, int bx, int by, int bz) { SCENE_CALL(s.set_grid(bx, by, bz)); }
+int cdx_scene_set_envelope(cdx_scene* scene, double e) { SCENE_CALL(s.set_envelope(e)); }
+int cdx_scene_set_narrowphase_mode(cdx_scene* scene, int mode, double tau) {
+  SCENE_CALL(CDX_REQUIRE(tau >= 0.0, CDX_ERR_INVALID, "tau must be non-negative"); s.set_np_mode(mode, tau));
+}
+int cdx_scene_set_profiling(cdx_scene* scene, int on) { SCENE_CALL(s.set_profiling(on != 0)); }
+int cdx_scene_aabb(cdx_scene* scene) { SCENE_CALL(s.run_aabb()); }
+int cdx_scene_broadphase(cdx_scene* scene) { SCENE_CALL(s.run_broadphase()); }
+int cdx_scene_narrowphase(cdx_scene* scene) { SCENE_CALL(s.run_narrowphase()); }
+int cdx_scene_step(cdx_scene* scene) { SCENE_CALL(s.run_aabb(); s.run_broadphase(); s.run_narrowphase()); }
+int cdx_scene_sync(cdx_scene* scene) { SCENE_CALL(s.sync()); }
+int cdx_scene_counts(cdx_scene* scene, cdx_counts* out) {
+  SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); const cdx::StepCounts& c = s.counts();
+             out->n_shapes = c.n; out->bin_entries = c.m; out->active_bins = c.a; out->pairs = c.p;
+             out->contacts = c.c; out->resolved = c.resolved; out->key_max = c.key_max; out->flags = c.bad;
+             for (int i = 0; i < 3; ++i) { out->bmin[i] = c.bmin[i]; out->bmax[i] = c.bmax[i]; });
+}
+int cdx_scene_read_aabb(cdx_scene* scene, int shifted, void* out) {
+  SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); s.read_aabb(out, shifted != 0));
+}
+int cdx_scene_read_bins(cdx_scene* scene, uint32_t* bin_counts, uint32_t* sorted_keys, uint32_t* sorted_bodies,
+                        uint32_t* active_keys, uint32_t* active_counts) {
+  SCENE_CALL(s.read_bins(bin_counts, sorted_keys, sorted_bodies, active_keys, active_counts));
+}
+int cdx_scene_read_pairs(cdx_scene* scene, int64_t* out) { SCENE_CALL(s.read_pairs(out)); }
+int cdx_scene_read_pairs_permuted(cdx_scene* scene, int64_t* out) { SCENE_CALL(s.read_pairs_permuted(out)); }
+int cdx_scene_read_contacts(cdx_scene* scene, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) {
+  SCENE_CALL(s.read_contacts(norm, cpta, cptb, dpth, bids));
+}
+int cdx_scene_stage_ms(cdx_scene* scene, double* ms) {
+  SCENE_CALL(CDX_REQUIRE(ms, CDX_ERR_INVALID, "null output"); s.stage_ms(ms));
+}
+int cdx_scene_device_bytes(cdx_scene* scene, uint64_t* bytes) {
+  SCENE_CALL(CDX_REQUIRE(bytes, CDX_ERR_INVALID, "null output"); *bytes = s.device_bytes());
+}
+
+}  // extern "C"

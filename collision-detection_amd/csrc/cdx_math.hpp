@@ -1,0 +1,153 @@
+// Vector, quaternion and support-map arithmetic for the gfx950 kernels.
+//
+// The reference takes this arithmetic from its gpu_math submodule, which is
+// not vendored.  The conventions used here: real4 quaternions are stored
+// scalar first (w,x,y,z) (AABBGenerator.cuh:53-57); mult(a,b) is the Hamilton
+// product a (x) b; rotations use the matrix of q in the "(e0e0+e1e1)*2-1"
+// form; normalize(0) returns (1,0,0); sign(0) = 0; IsZero compares against
+// FLT_EPSILON for float and DBL_EPSILON for double.
+//
+// Every expression keeps one fixed evaluation order and the library is built
+// with -ffp-contract=off and correctly rounded division and square root, so
+// the kernels reproduce an IEEE host evaluation of the same expressions to
+// the bit.  That is what makes AABBs, bin keys and the pair list bit-exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+namespace cdx {
+
+enum : int { SPHERE = 0, ELLIPSOID = 1, BOX = 2, CYLINDER = 3, RECT = 4, CONE = 5, TRIANGLEMESH = 6, CAPSULE = 7 };
+
+#define CDX_HD __host__ __device__ __forceinline__
+
+template <class R> struct v3 { R x, y, z; };
+template <class R> struct q4 { R w, x, y, z; };
+
+template <class R> CDX_HD v3<R> mk3(R x, R y, R z) { v3<R> r; r.x = x; r.y = y; r.z = z; return r; }
+template <class R> CDX_HD v3<R> operator+(v3<R> a, v3<R> b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R> CDX_HD v3<R> operator-(v3<R> a, v3<R> b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R> CDX_HD v3<R> operator-(v3<R> a) { return mk3(-a.x, -a.y, -a.z); }
+template <class R> CDX_HD v3<R> operator*(v3<R> a, R s) { return mk3(a.x * s, a.y * s, a.z * s); }
+template <class R> CDX_HD v3<R> operator*(R s, v3<R> a) { return mk3(s * a.x, s * a.y, s * a.z); }
+template <class R> CDX_HD v3<R> operator*(v3<R> a, v3<R> b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <class R> CDX_HD v3<R> operator/(v3<R> a, R s) { return mk3(a.x / s, a.y / s, a.z / s); }
+
+template <class R> CDX_HD R dot(v3<R> a, v3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> CDX_HD v3<R> cross(v3<R> a, v3<R> b) {
+  return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+CDX_HD float csqrt(float x) { return sqrtf(x); }
+CDX_HD double csqrt(double x) { return sqrt(x); }
+CDX_HD float cabs(float x) { return fabsf(x); }
+CDX_HD double cabs(double x) { return fabs(x); }
+template <class R> CDX_HD R length(v3<R> a) { return csqrt(dot(a, a)); }
+template <class R> CDX_HD v3<R> normalize(v3<R> a) {
+  R l = length(a);
+  if (l == R(0)) return mk3<R>(1, 0, 0);
+  return a / l;
+}
+template <class R> CDX_HD R sgn(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : R(0)); }
+template <class R> CDX_HD R eps_of();
+template <> CDX_HD float eps_of<float>() { return FLT_EPSILON; }
+template <> CDX_HD double eps_of<double>() { return DBL_EPSILON; }
+template <class R> CDX_HD bool is_zero(R x) { return cabs(x) < eps_of<R>(); }
+template <class R> CDX_HD bool is_zero(v3<R> a) { return is_zero(a.x) && is_zero(a.y) && is_zero(a.z); }
+
+// Hamilton product a (x) b.
+template <class R> CDX_HD q4<R> qmul(q4<R> a, q4<R> b) {
+  q4<R> q;
+  q.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+  q.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+  q.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+  q.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+  return q;
+}
+
+// Rotation matrix of a unit quaternion, row major.
+template <class R> struct mat3 { R a00, a01, a02, a10, a11, a12, a20, a21, a22; };
+template <class R> CDX_HD mat3<R> rotmat(q4<R> q) {
+  R e0e0 = q.w * q.w, e1e1 = q.x * q.x, e2e2 = q.y * q.y, e3e3 = q.z * q.z;
+  R e0e1 = q.w * q.x, e0e2 = q.w * q.y, e0e3 = q.w * q.z;
+  R e1e2 = q.x * q.y, e1e3 = q.x * q.z, e2e3 = q.y * q.z;
+  mat3<R> m;
+  m.a00 = (e0e0 + e1e1) * R(2) - R(1);
+  m.a01 = (e1e2 - e0e3) * R(2);
+  m.a02 = (e1e3 + e0e2) * R(2);
+  m.a10 = (e1e2 + e0e3) * R(2);
+  m.a11 = (e0e0 + e2e2) * R(2) - R(1);
+  m.a12 = (e2e3 - e0e1) * R(2);
+  m.a20 = (e1e3 - e0e2) * R(2);
+  m.a21 = (e2e3 + e0e1) * R(2);
+  m.a22 = (e0e0 + e3e3) * R(2) - R(1);
+  return m;
+}
+template <class R> CDX_HD v3<R> mul(const mat3<R>& m, v3<R> v) {
+  return mk3(m.a00 * v.x + m.a01 * v.y + m.a02 * v.z, m.a10 * v.x + m.a11 * v.y + m.a12 * v.z,
+             m.a20 * v.x + m.a21 * v.y + m.a22 * v.z);
+}
+template <class R> CDX_HD v3<R> mulT(const mat3<R>& m, v3<R> v) {
+  return mk3(m.a00 * v.x + m.a10 * v.y + m.a20 * v.z, m.a01 * v.x + m.a11 * v.y + m.a21 * v.z,
+             m.a02 * v.x + m.a12 * v.y + m.a22 * v.z);
+}
+template <class R> CDX_HD mat3<R> absm(mat3<R> m) {
+  m.a00 = cabs(m.a00); m.a01 = cabs(m.a01); m.a02 = cabs(m.a02);
+  m.a10 = cabs(m.a10); m.a11 = cabs(m.a11); m.a12 = cabs(m.a12);
+  m.a20 = cabs(m.a20); m.a21 = cabs(m.a21); m.a22 = cabs(m.a22);
+  return m;
+}
+template <class R> CDX_HD v3<R> rotate(v3<R> v, q4<R> q) { return mul(rotmat(q), v); }
+
+// Local-frame support maps (SupportFunctions.cuh:11-63), n of unit length.
+template <class R> CDX_HD v3<R> support_local(int type, v3<R> B, v3<R> n) {
+  switch (type) {
+    case SPHERE: return B.x * n;
+    case BOX: return mk3(sgn(n.x) * B.x, sgn(n.y) * B.y, sgn(n.z) * B.z);
+    case ELLIPSOID: {
+      v3<R> nn = normalize(n);
+      return B * B * nn / length(B * nn);
+    }
+    case CYLINDER: {
+      // Axis y.  w = n - (u.n)u with u = (0,1,0) evaluates exactly to
+      // (n.x, 0, n.z), and every zero term of the reference's vector form
+      // adds exactly, so this is the same value to the bit.
+      v3<R> w = mk3(n.x, R(0), n.z);
+      R s = sgn(n.y) * B.y;
+      if (length(w) != R(0)) {
+        v3<R> wn = normalize(w);
+        return mk3(B.x * wn.x, s, B.x * wn.z);
+      }
+      return mk3(R(0) * s, s, R(0) * s);
+    }
+    case CAPSULE: {
+      // Segment y in [-B.y, B.y] swept by a sphere of radius B.x.
+      v3<R> nn = normalize(n);
+      R s = sgn(n.y) * B.y;
+      return mk3(R(0) * s + B.x * nn.x, s + B.x * nn.y, R(0) * s + B.x * nn.z);
+    }
+    default: return mk3<R>(0, 0, 0);
+  }
+}
+
+// World support of a placed shape along b (Narrowphase.cu:19-52):
+// normalize, rotate into the shape frame, support, rotate back, translate.
+template <class R> CDX_HD v3<R> support_world(int type, v3<R> X, v3<R> B, const mat3<R>& m, v3<R> b) {
+  v3<R> n = normalize(b);
+  v3<R> rn = mulT(m, n);
+  return mul(m, support_local(type, B, rn)) + X;
+}
+
+// Order-preserving map float -> uint32 used for atomic min / max of bounds.
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+CDX_HD float ord2f(uint32_t u) {
+  uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  union { uint32_t u; float f; } c;
+  c.u = b;
+  return c.f;
+}
+
+}  // namespace cdx

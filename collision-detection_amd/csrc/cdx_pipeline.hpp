@@ -1,0 +1,83 @@
+// Internal interface between the C ABI (capi.cpp) and the device pipeline
+// (pipeline.hip).  A Scene owns one HIP stream and the device-resident state
+// of one collision scene: shapes and bodies in SoA layout, AABBs, the sorted
+// (bin, body) entries, the pair list and the contacts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace cdx {
+
+enum RealKind : int { REAL_F64 = 0, REAL_F32 = 1 };
+enum NpMode : int { NP_NATIVE = 0, NP_MIXED = 1 };
+
+// Pipeline stages timed with HIP events on the scene's stream.
+enum Stage : int {
+  ST_AABB = 0,
+  ST_BP_BOUNDS,
+  ST_BP_COUNT,
+  ST_BP_SCAN,
+  ST_BP_EMIT,
+  ST_BP_SORT,
+  ST_BP_RUNS,
+  ST_BP_PAIRS,
+  ST_BP_PAIRSORT,
+  ST_NP_MAIN,
+  ST_NP_RESOLVE,
+  ST_NP_COMPACT,
+  ST_COUNT
+};
+
+// Totals a step leaves on the device and copies back at synchronisation.
+struct StepCounts {
+  uint64_t n, m, a, p, c, resolved;
+  uint32_t key_max;
+  uint32_t bad;  // bit 0: unsupported shape type, bit 1: body id out of range
+  float bmin[3], bmax[3];
+};
+
+class SceneBase {
+ public:
+  virtual ~SceneBase() = default;
+  virtual int real_kind() const = 0;
+
+  // Inputs (host pointers, reference AoS layouts).
+  virtual void set_shapes(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R,
+                          const uint32_t* ID) = 0;
+  virtual void set_bodies(uint64_t nb, const void* pos, const void* rot, const uint8_t* active) = 0;
+  virtual void update_bodies(uint64_t k, const uint32_t* idx, const void* pos, const void* rot) = 0;
+  virtual void set_aabbs(uint64_t n, const void* aabb) = 0;
+  virtual void set_pairs(uint64_t p, const int64_t* pairs) = 0;
+
+  // Settings.
+  virtual void set_grid(int gx, int gy, int gz) = 0;
+  virtual void set_envelope(double e) = 0;
+  virtual void set_np_mode(int mode, double tau) = 0;
+  virtual void set_profiling(bool on) = 0;
+
+  // Stages.  Each enqueues on the scene stream; the broadphase reads its
+  // entry and pair totals back to size buffers, as the reference does.
+  virtual void run_aabb() = 0;
+  virtual void run_broadphase() = 0;
+  virtual void run_narrowphase() = 0;
+  virtual void sync() = 0;
+
+  // Results (after sync()).
+  virtual const StepCounts& counts() const = 0;
+  virtual void read_aabb(void* out, bool shifted) = 0;
+  virtual void read_bins(uint32_t* bin_counts, uint32_t* sorted_keys, uint32_t* sorted_bodies, uint32_t* active_keys,
+                         uint32_t* active_counts) = 0;
+  virtual void read_pairs(int64_t* out) = 0;
+  virtual void read_pairs_permuted(int64_t* out) = 0;
+  virtual void read_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) = 0;
+  virtual void stage_ms(double* out) = 0;
+  virtual hipStream_t stream() const = 0;
+  virtual uint64_t device_bytes() const = 0;
+};
+
+std::unique_ptr<SceneBase> make_scene(int real_kind, int device);
+
+}  // namespace cdx

@@ -1,0 +1,70 @@
+// Data-parallel primitives for gfx950 (64-wide wavefronts): wavefront scans
+// built from DPP row shifts / row broadcasts, an LDS-staged block scan, a
+// reduce-then-scan device scan, and a stable LSD radix sort of 64-bit values.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cdx {
+
+constexpr int WAVE = 64;
+
+// Inclusive add-scan across one 64-lane wavefront.  row_shr:1,2,4,8 scans
+// each 16-lane row; row_bcast:15 carries row 0 into row 1 and row 2 into
+// row 3; row_bcast:31 carries rows 0-1 into rows 2-3.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  uint32_t t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, false); v += t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xf, 0xf, false); v += t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, false); v += t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xf, 0xf, false); v += t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xa, 0xf, false); v += t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xc, 0xf, false); v += t;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  uint32_t l = __lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Exclusive block scan of one value per thread; returns the prefix and
+// stores the block total in *total.  scratch holds BS/64 + 1 uint32.
+template <int BS>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratch, uint32_t* total) {
+  constexpr int NW = BS / WAVE;
+  const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  uint32_t inc = wave_incl_scan(v);
+  if (l == WAVE - 1) scratch[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < NW; ++i) { uint32_t c = scratch[i]; scratch[i] = run; run += c; }
+    scratch[NW] = run;
+  }
+  __syncthreads();
+  uint32_t r = scratch[w] + inc - v;
+  *total = scratch[NW];
+  __syncthreads();
+  return r;
+}
+
+// ---- host launchers (prims.hip) ------------------------------------------
+
+// Exclusive scan of n uint32 (in may equal out).  The 64-bit total goes to
+// *d_total (device memory).  tmp must hold scan_tmp_words(n) uint32.
+size_t scan_tmp_words(uint64_t n);
+void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s);
+
+// Stable LSD radix sort of n 64-bit values by the 8-bit digits at the given
+// bit shifts (least significant first).  Result ends in *data (tmp is the
+// ping-pong buffer of n values).  hist_tmp must hold radix_tmp_words(n).
+size_t radix_tmp_words(uint64_t n);
+void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
+                    hipStream_t s);
+
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+}  // namespace cdx

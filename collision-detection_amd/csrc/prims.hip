@@ -1,0 +1,220 @@
+// Scan and sort primitives (declarations in cdx_prims.hpp).
+#include "cdx_prims.hpp"
+#include "cdx_util.hpp"
+
+namespace cdx {
+
+namespace {
+
+constexpr int SCAN_BS = 256;
+constexpr int SCAN_IPT = 16;
+constexpr int SCAN_TILE = SCAN_BS * SCAN_IPT;
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Per-tile sums (64-bit, so a tile of large counts cannot wrap).
+__global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                         uint64_t* __restrict__ part) {
+  __shared__ uint64_t ws[SCAN_BS / WAVE];
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t e = base + (uint64_t)i * SCAN_BS + threadIdx.x;
+    if (e < n) s += in[e];
+  }
+  s = wave_sum_u64(s);
+  if (threadIdx.x % WAVE == 0) ws[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < SCAN_BS / WAVE; ++w) t += ws[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of the tile sums by one block; writes the grand total.
+__global__ __launch_bounds__(1024) void k_scan_partials(uint64_t* __restrict__ part, uint64_t nb,
+                                                        uint64_t* __restrict__ d_total) {
+  __shared__ uint64_t ws[1024 / WAVE + 1];
+  __shared__ uint64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint64_t b0 = 0; b0 < nb; b0 += 1024) {
+    uint64_t i = b0 + threadIdx.x;
+    uint64_t v = i < nb ? part[i] : 0;
+    // wave inclusive scan in 64-bit via shuffles
+    uint64_t inc = v;
+    for (int o = 1; o < WAVE; o <<= 1) {
+      uint64_t t = __shfl_up(inc, o);
+      if ((int)(threadIdx.x % WAVE) >= o) inc += t;
+    }
+    if (threadIdx.x % WAVE == WAVE - 1) ws[threadIdx.x / WAVE] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint64_t run = 0;
+      for (int w = 0; w < 1024 / WAVE; ++w) { uint64_t c = ws[w]; ws[w] = run; run += c; }
+      ws[1024 / WAVE] = run;
+    }
+    __syncthreads();
+    uint64_t c0 = carry;
+    if (i < nb) part[i] = c0 + ws[threadIdx.x / WAVE] + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = c0 + ws[1024 / WAVE];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *d_total = carry;
+}
+
+// Tile scan: each thread owns SCAN_IPT consecutive elements.
+__global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                        uint64_t n, const uint64_t* __restrict__ part) {
+  __shared__ uint32_t scratch[SCAN_BS / WAVE + 1];
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
+  uint32_t v[SCAN_IPT];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t e = base + i;
+    v[i] = e < n ? in[e] : 0u;
+    s += v[i];
+  }
+  uint32_t tot;
+  uint32_t pre = block_excl_scan<SCAN_BS>(s, scratch, &tot);
+  uint32_t run = (uint32_t)part[blockIdx.x] + pre;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t e = base + i;
+    if (e < n) out[e] = run;
+    run += v[i];
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+constexpr int RS_BS = 256;
+constexpr int RS_IPT = 8;
+constexpr int RS_TILE = RS_BS * RS_IPT;
+constexpr int RS_NW = RS_BS / WAVE;
+
+__global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift,
+                                                         uint32_t* __restrict__ hist, uint32_t nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+  for (int i = 0; i < RS_IPT; ++i) {
+    uint64_t e = base + (uint64_t)i * RS_BS + threadIdx.x;
+    if (e < n) atomicAdd(&h[(uint32_t)(in[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter.  Element order inside a tile is (item i, then thread), and
+// ranks among equal digits follow that order: per item, lanes of one
+// wavefront find their equal-digit peers with 8 ballots, and per-wavefront
+// digit counts are prefix-summed across wavefronts in LDS.
+__global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                           uint64_t n, int shift, const uint32_t* __restrict__ offs,
+                                                           uint32_t nb) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[RS_NW][256];
+  const int w = threadIdx.x / WAVE;
+  base[threadIdx.x] = offs[(uint64_t)threadIdx.x * nb + blockIdx.x];
+  uint64_t tile = (uint64_t)blockIdx.x * RS_TILE;
+  const uint64_t lt = lanemask_lt();
+  for (int i = 0; i < RS_IPT; ++i) {
+    uint64_t e = tile + (uint64_t)i * RS_BS + threadIdx.x;
+    bool valid = e < n;
+    uint64_t v = valid ? in[e] : 0ull;
+    uint32_t d = (uint32_t)(v >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    uint32_t cnt = (uint32_t)__popcll(peers);
+#pragma unroll
+    for (int ww = 0; ww < RS_NW; ++ww) wcnt[ww][threadIdx.x] = 0;
+    __syncthreads();
+    if (valid && rank == 0) wcnt[w][d] = cnt;
+    __syncthreads();
+    {
+      uint32_t run = base[threadIdx.x];
+#pragma unroll
+      for (int ww = 0; ww < RS_NW; ++ww) {
+        uint32_t c = wcnt[ww][threadIdx.x];
+        wcnt[ww][threadIdx.x] = run;
+        run += c;
+      }
+      base[threadIdx.x] = run;
+    }
+    __syncthreads();
+    if (valid) out[wcnt[w][d] + rank] = v;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+size_t scan_tmp_words(uint64_t n) {
+  uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  return 2 * (nb + 1);  // uint64 partials
+}
+
+void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s) {
+  uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  uint64_t* part = reinterpret_cast<uint64_t*>(tmp);
+  if (n == 0) {
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, (uint64_t)0, d_total);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, n, part);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, nb, d_total);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, (const uint64_t*)part);
+}
+
+size_t radix_tmp_words(uint64_t n) {
+  uint64_t nb = (n + RS_TILE - 1) / RS_TILE;
+  uint64_t h = 256 * (nb ? nb : 1);
+  return h + scan_tmp_words(h) + 2;
+}
+
+void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
+                    hipStream_t s) {
+  if (n <= 1 || nshift == 0) return;
+  uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  uint64_t h = 256ull * nb;
+  uint32_t* hist = hist_tmp;
+  uint64_t* total = reinterpret_cast<uint64_t*>(hist_tmp + h + scan_tmp_words(h));
+  uint32_t* stmp = hist_tmp + h;
+  uint64_t* src = data;
+  uint64_t* dst = tmp;
+  for (int p = 0; p < nshift; ++p) {
+    hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, n, shifts[p], hist, nb);
+    scan_excl_u32(hist, hist, h, total, stmp, s);
+    hipLaunchKernelGGL(k_radix_downsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, dst, n, shifts[p],
+                       (const uint32_t*)hist, nb);
+    uint64_t* t = src; src = dst; dst = t;
+  }
+  if (src != data) CDX_HIP_CHECK_VOID(hipMemcpyAsync(data, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+}
+
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, v, n);
+}
+
+}  // namespace cdx

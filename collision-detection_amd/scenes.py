@@ -1,0 +1,185 @@
+"""Seeded synthetic scenes for the collision pipeline.
+
+Arrays follow the layout of the reference API's inputs
+(AABBGenerator.cuh:21-30, Narrowphase.cuh:19-34): per shape a type tag T,
+local position A (real3), dimensions B (real3), unused C (real3), local
+rotation R (real4, scalar first) and body id ID; per body a position, a
+rotation and an active flag.  Every workload has one shape per body, local
+position 0 and local rotation identity.
+
+Dimension convention (B): sphere (r, 0, 0); box and ellipsoid semi-axes;
+cylinder (radius, half-height, radius) about local y; capsule (radius,
+half-length of the core segment, radius) about local y.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SPHERE, ELLIPSOID, BOX, CYLINDER, RECT, CONE, TRIANGLEMESH, CAPSULE = range(8)
+TYPE_NAMES = {SPHERE: "sphere", ELLIPSOID: "ellipsoid", BOX: "box", CYLINDER: "cylinder", CAPSULE: "capsule"}
+
+# 1M mixed-body workload: 40 % sphere, 20 % box, 15 % ellipsoid, 15 % capsule, 10 % cylinder.
+MIX_TYPES = np.array([SPHERE, BOX, ELLIPSOID, CAPSULE, CYLINDER], dtype=np.int32)
+MIX_PROBS = np.array([0.40, 0.20, 0.15, 0.15, 0.10])
+
+
+@dataclass
+class Scene:
+    T: np.ndarray        # (n,) int32 shape type
+    A: np.ndarray        # (n,3) local position
+    B: np.ndarray        # (n,3) dimensions
+    C: np.ndarray        # (n,3) unused by the in-scope shapes
+    R: np.ndarray        # (n,4) local rotation (w,x,y,z)
+    ID: np.ndarray       # (n,) uint32 body id
+    pos: np.ndarray      # (nb,3) body position
+    rot: np.ndarray      # (nb,4) body rotation (w,x,y,z)
+    active: np.ndarray   # (nb,) uint8
+    grid: tuple = (16, 16, 16)
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def dtype(self):
+        return self.A.dtype
+
+    @property
+    def n(self):
+        return int(self.T.shape[0])
+
+    @property
+    def nb(self):
+        return int(self.pos.shape[0])
+
+    def astype(self, dtype) -> "Scene":
+        c = lambda a: np.ascontiguousarray(a, dtype=dtype)
+        return Scene(self.T, c(self.A), c(self.B), c(self.C), c(self.R), self.ID, c(self.pos), c(self.rot),
+                     self.active, self.grid, self.name, dict(self.meta))
+
+    def subset(self, idx) -> "Scene":
+        """Bodies idx (one shape per body) as a scene of their own."""
+        idx = np.asarray(idx)
+        n = len(idx)
+        return Scene(self.T[idx].copy(), self.A[idx].copy(), self.B[idx].copy(), self.C[idx].copy(),
+                     self.R[idx].copy(), np.arange(n, dtype=np.uint32), self.pos[idx].copy(), self.rot[idx].copy(),
+                     self.active[idx].copy(), self.grid, self.name + "-subset", dict(self.meta))
+
+
+def random_quats(rng, n):
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q
+
+
+def _one_shape_per_body(T, B, pos, rot, dtype, grid, name, meta):
+    n = len(T)
+    z3 = np.zeros((n, 3))
+    ident = np.zeros((n, 4))
+    ident[:, 0] = 1.0
+    c = lambda a: np.ascontiguousarray(a, dtype=dtype)
+    return Scene(np.ascontiguousarray(T, dtype=np.int32), c(z3), c(B), c(z3), c(ident),
+                 np.arange(n, dtype=np.uint32), c(pos), c(rot), np.ones(n, dtype=np.uint8), tuple(grid), name, meta)
+
+
+def half_extents(T, B, rot):
+    """World AABB half-extents of each shape (|R| times the bounding semi-axes)."""
+    T = np.asarray(T)
+    B = np.asarray(B, dtype=np.float64)
+    sem = B.copy()
+    sph = T == SPHERE
+    sem[sph] = B[sph, :1]
+    cap = T == CAPSULE
+    sem[cap, 1] = B[cap, 1] + B[cap, 0]
+    q = np.asarray(rot, dtype=np.float64)
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    Rm = np.empty((len(q), 3, 3))
+    Rm[:, 0, 0] = 1 - 2 * (y * y + z * z); Rm[:, 0, 1] = 2 * (x * y - w * z); Rm[:, 0, 2] = 2 * (x * z + w * y)
+    Rm[:, 1, 0] = 2 * (x * y + w * z); Rm[:, 1, 1] = 1 - 2 * (x * x + z * z); Rm[:, 1, 2] = 2 * (y * z - w * x)
+    Rm[:, 2, 0] = 2 * (x * z - w * y); Rm[:, 2, 1] = 2 * (y * z + w * x); Rm[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    h = np.einsum("nij,nj->ni", np.abs(Rm), sem)
+    h[sph] = B[sph, :1]
+    return h
+
+
+def mixed_shapes(rng, n, mu=np.log(0.01), sigma=0.3):
+    """Types, dimensions and rotations of the mixed convex workload."""
+    T = rng.choice(MIX_TYPES, size=n, p=MIX_PROBS).astype(np.int32)
+    s = np.exp(rng.normal(mu, sigma, size=n))
+    B = np.zeros((n, 3))
+    m = T == SPHERE
+    B[m, 0] = s[m]
+    for t in (BOX, ELLIPSOID):
+        m = T == t
+        B[m] = s[m, None] * rng.uniform(0.5, 1.5, size=(m.sum(), 3))
+    m = T == CAPSULE
+    B[m, 0] = 0.5 * s[m]
+    B[m, 1] = s[m]
+    B[m, 2] = 0.5 * s[m]
+    m = T == CYLINDER
+    B[m, 0] = s[m]
+    B[m, 1] = s[m] * rng.uniform(0.5, 1.5, size=m.sum())
+    B[m, 2] = s[m]
+    rot = random_quats(rng, n)
+    return T, B, rot
+
+
+def box_side_for_neighbours(h, n, k, rng, samples=400_000):
+    """Cube side L so a body has about k AABB-overlap partners: L^3 = n E[prod(2hA+2hB)] / k."""
+    i = rng.integers(0, len(h), samples)
+    j = rng.integers(0, len(h), samples)
+    e = np.prod(2 * h[i] + 2 * h[j], axis=1).mean()
+    return float((n * e / k) ** (1.0 / 3.0))
+
+
+def grid_for(L_extent, cell):
+    g = max(1, int(np.ceil(L_extent / cell)))
+    return (g, g, g)
+
+
+def small_spheres(n=8192, seed=0x5EED0000, dtype=np.float64, grid=None):
+    """8,192 FP64 spheres, r ~ U[0.01, 0.02], centres ~ U[0,1]^3, identity rotations."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(0.01, 0.02, n)
+    pos = rng.uniform(0.0, 1.0, (n, 3))
+    rot = np.zeros((n, 4)); rot[:, 0] = 1
+    B = np.zeros((n, 3)); B[:, 0] = r
+    g = grid if grid is not None else grid_for(1.04, 0.04)
+    return _one_shape_per_body(np.zeros(n, np.int32), B, pos, rot, dtype, g, "spheres-8k",
+                               {"n": n, "seed": seed})
+
+
+def mixed_bodies(n=1 << 20, seed=0x5EED0001, dtype=np.float64, neighbours=4.0, grid=None):
+    """Mixed convex bodies (log-normal size, uniform rotations) in a cube sized for ~4 AABB partners each."""
+    rng = np.random.default_rng(seed)
+    T, B, rot = mixed_shapes(rng, n)
+    h = half_extents(T, B, rot)
+    L = box_side_for_neighbours(h, n, neighbours, rng)
+    pos = rng.uniform(0.0, L, (n, 3))
+    if grid is None:
+        cell = 2.0 * float(np.mean(2 * h.max(axis=1)))
+        grid = grid_for(L, cell)
+    return _one_shape_per_body(T, B, pos, rot, dtype, grid, "mixed",
+                               {"n": n, "seed": seed, "L": L, "neighbours": neighbours})
+
+
+def lattice_spheres(side=256, seed=0x5EED0002, dtype=np.float32, spacing=0.01, grid=None):
+    """Dense granular pack: side^3 spheres on a jittered cubic lattice, r in {0.005, 0.0075}."""
+    rng = np.random.default_rng(seed)
+    n = side ** 3
+    idx = np.arange(n, dtype=np.int64)
+    ijk = np.stack([idx % side, (idx // side) % side, idx // (side * side)], axis=1).astype(np.float64)
+    r = np.where(rng.random(n) < 0.5, 0.005, 0.0075)
+    pos = ijk * spacing + rng.uniform(-0.1, 0.1, (n, 3)) * r[:, None]
+    del ijk
+    rot = np.zeros((n, 4)); rot[:, 0] = 1
+    B = np.zeros((n, 3)); B[:, 0] = r
+    if grid is None:
+        grid = grid_for(side * spacing + 0.02, 0.015)
+    return _one_shape_per_body(np.zeros(n, np.int32), B, pos, rot, dtype, grid, "lattice",
+                               {"n": n, "seed": seed, "side": side})
+
+
+def mixed_scene_of_size(n, neighbours, seed, dtype=np.float64):
+    """One scene of the batched / served workloads: the mixed shape set at its own density."""
+    return mixed_bodies(n=n, seed=seed, dtype=dtype, neighbours=neighbours)

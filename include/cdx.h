@@ -1,0 +1,186 @@
+/* cdx — MI355X (gfx950) collision detection: the C ABI of libcdx.so.
+ *
+ * Drop-in boundary for the hot path of uwsbel/collision-detection
+ * (Chrono::Collide): shape/pose upload -> AABB generation -> uniform-grid
+ * broadphase -> narrowphase (sphere-sphere + MPR) -> contact readback.
+ * Plain pointers and sizes only; no C++ or HIP types cross this boundary.
+ *
+ * Layouts follow the reference API's containers:
+ *   real3  = 3 packed reals (x,y,z); real4 = quaternion (w,x,y,z);
+ *   shape types are the reference enum (src/collision_detection.h:57)
+ *   SPHERE=0 ELLIPSOID=1 BOX=2 CYLINDER=3 RECT=4 CONE=5 TRIANGLEMESH=6 plus
+ *   CDX_CAPSULE=7 (no reference counterpart).  RECT, CONE and TRIANGLEMESH
+ *   are rejected with CDX_ERR_UNSUPPORTED_SHAPE;
+ *   aabb_data = 2n real3, the n minimum corners followed by the n maximum
+ *   corners (AABBGenerator.cu:114-115);
+ *   pairs = int64 (a << 32 | b), a < b shape indices (Broadphase.cu:302-308);
+ *   obj_active = one byte per body (custom_vector<bool>).
+ * real is double (CDX_REAL_F64) or float (CDX_REAL_F32), chosen per call or
+ * per scene, the reference's compile-time `real` typedef made a parameter.
+ *
+ * Every function returns a status (CDX_OK = 0).  The message of the last
+ * failure is kept per context / per scene.  A context or scene must not be
+ * used from two threads at once; distinct contexts and scenes are
+ * independent (each scene owns its own HIP stream and device buffers).
+ */
+#ifndef CDX_H
+#define CDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CDX_VERSION 1
+
+#define CDX_OK 0
+#define CDX_ERR_INVALID 1
+#define CDX_ERR_HIP 2
+#define CDX_ERR_UNSUPPORTED_SHAPE 3
+#define CDX_ERR_CAPACITY 4
+#define CDX_ERR_NO_DEVICE 5
+#define CDX_ERR_OUT_OF_MEMORY 6
+#define CDX_ERR_NOT_READY 7
+
+#define CDX_REAL_F64 0
+#define CDX_REAL_F32 1
+
+#define CDX_CAPSULE 7
+
+/* Narrowphase precision.  NATIVE runs the MPR in the scene's real type with
+ * the reference's evaluation order.  MIXED runs it in FP32 in the frame of
+ * the first shape and re-solves in FP64 every pair whose FP32 verdict is
+ * within tau x (sum of the two bounding radii) of a decision boundary
+ * (near-zero depth or separation, degenerate portal, iteration cap).  */
+#define CDX_NP_NATIVE 0
+#define CDX_NP_MIXED 1
+
+/* Stages reported by cdx_scene_stage_ms (HIP-event times of the last run). */
+#define CDX_STAGE_AABB 0
+#define CDX_STAGE_BP_BOUNDS 1
+#define CDX_STAGE_BP_COUNT 2
+#define CDX_STAGE_BP_SCAN 3
+#define CDX_STAGE_BP_EMIT 4
+#define CDX_STAGE_BP_SORT 5
+#define CDX_STAGE_BP_RUNS 6
+#define CDX_STAGE_BP_PAIRS 7
+#define CDX_STAGE_BP_PAIRSORT 8
+#define CDX_STAGE_NP_MAIN 9
+#define CDX_STAGE_NP_RESOLVE 10
+#define CDX_STAGE_NP_COMPACT 11
+#define CDX_STAGE_COUNT 12
+
+typedef struct cdx_ctx cdx_ctx;
+typedef struct cdx_scene cdx_scene;
+
+typedef struct cdx_counts {
+  uint64_t n_shapes;
+  uint64_t bin_entries;   /* number_of_bin_intersections (Broadphase.cu:404) */
+  uint64_t active_bins;   /* last_active_bin (Broadphase.cu:440) */
+  uint64_t pairs;         /* number_of_contacts_possible after unique (:528) */
+  uint64_t contacts;      /* number_of_contacts (Narrowphase.cu:580) */
+  uint64_t resolved;      /* pairs re-solved in FP64 (CDX_NP_MIXED) */
+  uint32_t key_max;       /* largest bin key emitted */
+  uint32_t flags;
+  float bmin[3], bmax[3]; /* scene bounds, float (Broadphase.cu:368-376) */
+} cdx_counts;
+
+int cdx_version(void);
+const char* cdx_status_string(int status);
+int cdx_device_count(int* count);
+
+/* ---- contexts --------------------------------------------------------- */
+int cdx_ctx_create(int device, cdx_ctx** out);
+int cdx_ctx_destroy(cdx_ctx* ctx);
+const char* cdx_ctx_last_error(const cdx_ctx* ctx);
+
+/* ---- drop-in calls: one per reference method, host buffers ------------ */
+
+/* Replaces AABBGenerator::GenerateAABB (AABBGenerator.cuh:21-30,
+ * AABBGenerator.cu:135-188).  aabb_out receives 2*n_shapes real3.
+ * C (obj_data_C) is accepted for signature parity; no in-scope shape reads
+ * it.  Unlike the reference, which leaves the AABB slot of an unsupported
+ * shape unwritten, unsupported types fail with CDX_ERR_UNSUPPORTED_SHAPE. */
+int cdx_generate_aabb(cdx_ctx* ctx, int real_kind, uint64_t n_shapes, const int32_t* T, const void* A,
+                      const void* B, const void* C, const void* R, const uint32_t* ID, uint64_t n_bodies,
+                      const void* body_pos, const void* body_rot, void* aabb_out);
+
+/* Replaces Broadphase::detectPossibleCollisions (Broadphase.cuh:17,
+ * Broadphase.cu:344-545) with setBinsPerAxis's grid (Broadphase.cu:57-60).
+ * aabb_inout (2*n_aabb real3) is shifted in place by the scene origin, as
+ * the reference does (Broadphase.cu:382-386).  *n_pairs receives the
+ * number of unique candidate pairs; fetch them with
+ * cdx_read_possible_collisions.  An empty input yields 0 pairs (the
+ * reference reads aabb_data[0] there). */
+int cdx_detect_possible_collisions(cdx_ctx* ctx, int real_kind, void* aabb_inout, uint64_t n_aabb, int bins_x,
+                                   int bins_y, int bins_z, uint64_t* n_pairs);
+int cdx_read_possible_collisions(cdx_ctx* ctx, int64_t* pairs_out, uint64_t capacity);
+
+/* Replaces Narrowphase::DoNarrowphase (Narrowphase.cuh:19-34,
+ * Narrowphase.cu:511-595) and SetCollisionEnvelope (Narrowphase.cuh:53-55).
+ * pairs_inout is permuted in place, contacts first in pair order, as the
+ * reference's sort_by_key leaves it (Narrowphase.cu:584-588).
+ * *n_contacts receives number_of_contacts; fetch the contact arrays with
+ * cdx_read_contacts. */
+int cdx_do_narrowphase(cdx_ctx* ctx, int real_kind, int np_mode, double envelope, uint64_t n_shapes,
+                       const int32_t* T, const void* A, const void* B, const void* C, const void* R,
+                       const uint32_t* ID, uint64_t n_bodies, const uint8_t* obj_active, const void* body_pos,
+                       const void* body_rot, int64_t* pairs_inout, uint64_t n_pairs, uint64_t* n_contacts);
+/* norm, cpta, cptb: real3[n]; dpth: real[n]; bids: int2[n] (body ids).
+ * Any output pointer may be NULL. */
+int cdx_read_contacts(cdx_ctx* ctx, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids,
+                      uint64_t capacity);
+
+/* ---- device-resident scenes (no reference counterpart: the reference
+ * re-uploads through thrust containers on every call) ------------------- */
+int cdx_scene_create(cdx_ctx* ctx, int real_kind, cdx_scene** out);
+int cdx_scene_destroy(cdx_scene* scene);
+const char* cdx_scene_last_error(const cdx_scene* scene);
+
+int cdx_scene_set_shapes(cdx_scene* scene, uint64_t n_shapes, const int32_t* T, const void* A, const void* B,
+                         const void* C, const void* R, const uint32_t* ID);
+int cdx_scene_set_bodies(cdx_scene* scene, uint64_t n_bodies, const void* body_pos, const void* body_rot,
+                         const uint8_t* obj_active);
+/* Incremental pose update of k bodies (indices + new pos/rot). */
+int cdx_scene_update_bodies(cdx_scene* scene, uint64_t k, const uint32_t* body_idx, const void* body_pos,
+                            const void* body_rot);
+/* Caller-supplied AABBs for a broadphase-only scene. */
+int cdx_scene_set_aabbs(cdx_scene* scene, uint64_t n_aabb, const void* aabb_data);
+/* Caller-supplied candidate pairs for a narrowphase-only scene. */
+int cdx_scene_set_pairs(cdx_scene* scene, uint64_t n_pairs, const int64_t* pairs);
+
+int cdx_scene_set_bins(cdx_scene* scene, int bins_x, int bins_y, int bins_z);
+int cdx_scene_set_envelope(cdx_scene* scene, double envelope);
+int cdx_scene_set_narrowphase_mode(cdx_scene* scene, int np_mode, double tau);
+int cdx_scene_set_profiling(cdx_scene* scene, int on);
+
+int cdx_scene_aabb(cdx_scene* scene);
+int cdx_scene_broadphase(cdx_scene* scene);
+int cdx_scene_narrowphase(cdx_scene* scene);
+/* AABB + broadphase + narrowphase. */
+int cdx_scene_step(cdx_scene* scene);
+int cdx_scene_sync(cdx_scene* scene);
+int cdx_scene_counts(cdx_scene* scene, cdx_counts* out);
+
+/* Readback.  shifted = 1 returns the AABBs as the reference leaves them
+ * after detectPossibleCollisions (plus the scene origin). */
+int cdx_scene_read_aabb(cdx_scene* scene, int shifted, void* aabb_out);
+/* Broadphase intermediates the reference keeps internal: per-AABB bin
+ * counts (Bins_Intersected before its scan, Broadphase.cu:393-402), the
+ * sorted (bin, body) entries (after sort_by_key, :428-432) and the active
+ * bins with their occupancy (reduce_by_key, :439-443).  NULL skips one. */
+int cdx_scene_read_bins(cdx_scene* scene, uint32_t* bin_counts, uint32_t* sorted_keys, uint32_t* sorted_bodies,
+                        uint32_t* active_keys, uint32_t* active_counts);
+int cdx_scene_read_pairs(cdx_scene* scene, int64_t* pairs_out);
+int cdx_scene_read_pairs_permuted(cdx_scene* scene, int64_t* pairs_out);
+int cdx_scene_read_contacts(cdx_scene* scene, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids);
+/* Milliseconds per stage of the last run (profiling on), CDX_STAGE_COUNT values. */
+int cdx_scene_stage_ms(cdx_scene* scene, double* ms_out);
+int cdx_scene_device_bytes(cdx_scene* scene, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDX_H */

@@ -1,0 +1,182 @@
+"""GPU parity: libcdx.so (through the C ABI) against the CPU oracle.
+
+Bit-exact: AABBs, per-AABB bin counts, sorted (bin, body) entries, active
+bins and their occupancy, the pair list, the shifted AABBs.
+Contacts (NATIVE mode): the contact set, body ids and the permuted pair list
+exactly; normal, points and depth within 1e-12 x scene length (FP64) or
+1e-5 x scene length (FP32).  MIXED mode: the contact set and ids exactly;
+normal and depth within the tolerances stated in test_gpu_mixed_precision.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene_cases(scenes):
+    return {
+        "spheres8k_f64": scenes.small_spheres(),
+        "mixed20k_f64": scenes.mixed_bodies(n=20_000, seed=11),
+        "mixed20k_f32": scenes.mixed_bodies(n=20_000, seed=12, dtype=np.float32),
+        "lattice16_f32": scenes.lattice_spheres(side=16, seed=13),
+    }
+
+
+@pytest.fixture(scope="module")
+def cases(scenes):
+    return _scene_cases(scenes)
+
+
+CASE_NAMES = ["spheres8k_f64", "mixed20k_f64", "mixed20k_f32", "lattice16_f32"]
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_aabb_bit_exact(cdx, ctx, oracle, cases, name):
+    s = cases[name]
+    print(f"case {name} n={s.n}", flush=True)
+    ref = oracle.aabb(s)
+    got = ctx.generate_aabb(s)
+    assert got.dtype == ref.dtype
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_broadphase_bit_exact(cdx, ctx, oracle, cases, name):
+    s = cases[name]
+    print(f"case {name} n={s.n} grid={s.grid}", flush=True)
+    ab = oracle.aabb(s)
+    ref = oracle.broadphase(ab, s.grid)
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.aabb()
+    sc.broadphase()
+    sc.sync()
+    c = sc.counts()
+    assert c.bin_entries == len(ref.sorted_keys)
+    assert c.active_bins == len(ref.active_keys)
+    assert c.pairs == len(ref.pairs)
+    np.testing.assert_array_equal(np.array(c.bmin), ref.bmin)
+    np.testing.assert_array_equal(np.array(c.bmax), ref.bmax)
+    bc, sk, sb, ak, ac = sc.read_bins()
+    np.testing.assert_array_equal(bc, ref.bin_counts)
+    np.testing.assert_array_equal(sk, ref.sorted_keys)
+    np.testing.assert_array_equal(sb, ref.sorted_bodies)
+    np.testing.assert_array_equal(ak, ref.active_keys)
+    np.testing.assert_array_equal(ac, ref.active_counts)
+    np.testing.assert_array_equal(sc.read_pairs(), ref.pairs)
+    np.testing.assert_array_equal(sc.read_aabb(shifted=True), ref.aabb_shifted)
+    np.testing.assert_array_equal(sc.read_aabb(shifted=False), ab)
+
+
+def test_dropin_broadphase_shifts_in_place(cdx, ctx, oracle, cases):
+    s = cases["spheres8k_f64"]
+    ab = oracle.aabb(s)
+    ref = oracle.broadphase(ab, s.grid)
+    mine = ab.copy()
+    pairs = ctx.detect_possible_collisions(mine, s.grid)
+    np.testing.assert_array_equal(pairs, ref.pairs)
+    np.testing.assert_array_equal(mine, ref.aabb_shifted)
+
+
+def _tol(s):
+    L = float(np.ptp(s.pos, axis=0).max()) + 1.0
+    return (1e-12 if s.dtype == np.float64 else 1e-5) * L
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_narrowphase_native(cdx, ctx, oracle, cases, name):
+    s = cases[name]
+    print(f"case {name}", flush=True)
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    ref = oracle.narrowphase(s, pairs)
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.set_mode(cdx.NP_NATIVE)
+    sc.step()
+    sc.sync()
+    c = sc.counts()
+    assert c.contacts == ref.count
+    norm, pa, pb, d, ids = sc.read_contacts()
+    np.testing.assert_array_equal(ids, ref.bids)
+    np.testing.assert_array_equal(sc.read_pairs_permuted(), ref.pairs_permuted)
+    t = _tol(s)
+    np.testing.assert_allclose(norm, ref.norm, rtol=0, atol=t)
+    np.testing.assert_allclose(pa, ref.cpta, rtol=0, atol=t)
+    np.testing.assert_allclose(pb, ref.cptb, rtol=0, atol=t)
+    np.testing.assert_allclose(d, ref.dpth, rtol=0, atol=t)
+
+
+def test_dropin_narrowphase_envelope_and_inactive(cdx, ctx, oracle, scenes):
+    s = scenes.mixed_bodies(n=8_000, seed=21)
+    rng = np.random.default_rng(5)
+    s.active = (rng.random(s.nb) < 0.7).astype(np.uint8)
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    ref = oracle.narrowphase(s, pairs, envelope=0.001)
+    mine = pairs.copy()
+    norm, pa, pb, d, ids = ctx.do_narrowphase(s, mine, envelope=0.001)
+    np.testing.assert_array_equal(mine, ref.pairs_permuted)
+    np.testing.assert_array_equal(ids, ref.bids)
+    t = _tol(s)
+    np.testing.assert_allclose(norm, ref.norm, atol=t, rtol=0)
+    np.testing.assert_allclose(d, ref.dpth, atol=t, rtol=0)
+
+
+def test_mixed_precision(cdx, ctx, oracle, scenes):
+    """FP32 MPR + FP64 re-solve against the FP64 oracle.
+
+    Stated tolerance: contact set and ids exact; |N32 - N64| <= 1e-5 and
+    |d32 - d64| <= 1e-5 x (rA + rB) (bounding radii) for every contact.
+    """
+    s = scenes.mixed_bodies(n=50_000, seed=31)
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    ref = oracle.narrowphase(s, pairs)
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.set_mode(cdx.NP_MIXED, 1e-2)
+    sc.step()
+    sc.sync()
+    c = sc.counts()
+    print(f"pairs={c.pairs} contacts={c.contacts} resolved={c.resolved}", flush=True)
+    norm, pa, pb, d, ids = sc.read_contacts()
+    assert c.contacts == ref.count
+    np.testing.assert_array_equal(ids, ref.bids)
+    perm = sc.read_pairs_permuted()
+    np.testing.assert_array_equal(perm, ref.pairs_permuted)
+    hits = perm[: c.contacts]
+    a = (hits >> 32).astype(np.int64)
+    b = (hits & 0xffffffff).astype(np.int64)
+    rad = np.linalg.norm(s.B, axis=1)
+    scale = rad[a] + rad[b]
+    en = np.linalg.norm(norm - ref.norm, axis=1)
+    ed = np.abs(d - ref.dpth) / scale
+    print(f"normal err max={en.max():.3e} p99={np.quantile(en, 0.99):.3e}; "
+          f"depth err/scale max={ed.max():.3e} p99={np.quantile(ed, 0.99):.3e}", flush=True)
+    assert en.max() <= 1e-5
+    assert ed.max() <= 1e-5
+
+
+def test_kat_single_bin(cdx, ctx):
+    """test/test1.cu: N identical AABBs [0,1]^3 on a 1^3 grid -> N(N-1)/2 pairs."""
+    for n in (10, 100):
+        ab = np.zeros((2 * n, 3))
+        ab[n:] = 1.0
+        pairs = ctx.detect_possible_collisions(ab, (1, 1, 1))
+        assert len(pairs) == n * (n - 1) // 2
+
+
+def test_kat_duplicated_lattice(cdx, ctx):
+    """test/binTest_Broadphase.cu: two spheres per site of an n^3 lattice -> n^3 pairs (2i, 2i+1)."""
+    n, bpa, r = 10, 10, 0.25
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    c = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.float64)
+    c = np.repeat(c, 2, axis=0)
+    ab = np.concatenate([c - r, c + r])
+    pairs = ctx.detect_possible_collisions(ab, (bpa, bpa, bpa))
+    assert len(pairs) == n ** 3
+    exp = (np.arange(0, 2 * n ** 3, 2, dtype=np.int64) << 32) | np.arange(1, 2 * n ** 3, 2, dtype=np.int64)
+    np.testing.assert_array_equal(pairs, exp)
+
+
+def test_empty_inputs(cdx, ctx):
+    ab = np.zeros((0, 3))
+    assert len(ctx.detect_possible_collisions(ab, (4, 4, 4))) == 0
