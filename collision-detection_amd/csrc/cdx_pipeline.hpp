@@ -12,7 +12,7 @@
 namespace cdx {
 
 enum RealKind : int { REAL_F64 = 0, REAL_F32 = 1 };
-enum NpMode : int { NP_NATIVE = 0, NP_MIXED = 1 };
+enum NpMode : int { NP_NATIVE = 0, NP_MIXED = 1, NP_FAST = 2 };
 
 // Pipeline stages timed with HIP events on the scene's stream.
 enum Stage : int {

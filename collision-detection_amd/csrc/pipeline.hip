@@ -529,8 +529,8 @@ __device__ __forceinline__ float bound_radius(int t, v3<float> b) {
 // it differently from FP64: a separation or penetration within tau of zero,
 // (nearly) coincident centres, a (nearly) degenerate portal, or an iteration
 // cap reached.
-__device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float tau, v3<float>& rnormal,
-                           v3<float>& point) {
+__device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float tau, bool verdict_only,
+                           v3<float>& rnormal, v3<float>& point) {
   typedef float R;
   v3<R> v01, v02, v0, n, v11, v12, v1, v21, v22, v2;
   v01 = A.X;
@@ -588,6 +588,8 @@ __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float
       n = normalize(n);
       R side = dot(n, v1);
       if (side >= R(0) && !hit) hit = true;
+      // The origin is inside the portal by more than tau: a certain hit.
+      if (verdict_only && side > tau) return MPR_HIT;
       v3<R> v41 = sup(A, -n);
       v3<R> v42 = sup(B, n);
       v3<R> v4 = v42 - v41;
@@ -645,7 +647,8 @@ template <class R> __device__ __forceinline__ Placed<float> placed_local(const S
 template <class R>
 __global__ __launch_bounds__(128) void k_np_mixed(uint64_t P, const uint64_t* __restrict__ pairs,
                                                   const ShapeRec<R>* __restrict__ rec, R envelope, float tau_rel,
-                                                  PairOut<R> o, uint32_t* __restrict__ queue, DevStatus* st) {
+                                                  int keep_fp32_hits, PairOut<R> o, uint32_t* __restrict__ queue,
+                                                  DevStatus* st) {
   uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
   if (i >= P) return;
   uint64_t pr = pairs[i];
@@ -661,7 +664,10 @@ __global__ __launch_bounds__(128) void k_np_mixed(uint64_t P, const uint64_t* __
   Placed<float> A = placed_local(ra, ra), B = placed_local(rb, ra);
   float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
   v3<float> N, p0;
-  int r = mpr_guarded(A, B, tau, N, p0);
+  int r = mpr_guarded(A, B, tau, !keep_fp32_hits, N, p0);
+  // MIXED: FP32 settles clear misses; every hit takes the FP64 reference
+  // path, whose capped iteration sequence FP32 cannot reproduce.
+  if (r == MPR_HIT && !keep_fp32_hits) r = MPR_UNSURE;
   if (r == MPR_HIT) {
     v3<float> p1 = dot((sup(A, -N) - p0), N) * N + p0;
     v3<float> p2 = dot((sup(B, N) - p0), N) * N + p0;
@@ -788,7 +794,13 @@ struct DevBuf {
   ~DevBuf() { if (p) (void)hipFree(p); }
   void reserve(size_t b) {
     if (b <= bytes) return;
-    if (p) { CDX_HIP_CHECK(hipFree(p)); p = nullptr; bytes = 0; }
+    if (p) {
+      // Growing: kernels queued on any stream may still read the old block.
+      CDX_HIP_CHECK(hipDeviceSynchronize());
+      CDX_HIP_CHECK(hipFree(p));
+      p = nullptr;
+      bytes = 0;
+    }
     size_t nb = b < 256 ? 256 : b;
     CDX_HIP_CHECK(hipMalloc(&p, nb));
     bytes = nb;
@@ -863,9 +875,10 @@ template <class R> class Scene final : public SceneBase {
 
   void update_bodies(uint64_t k, const uint32_t* idx, const void* pos, const void* rot) override {
     if (k == 0) return;
-    upd_.reserve(k * (4 + 7 * sizeof(R)));
+    const uint64_t idx_bytes = (k * 4 + 15) & ~15ull;
+    upd_.reserve(idx_bytes + k * 7 * sizeof(R));
     uint32_t* didx = upd_.as<uint32_t>();
-    R* dpos = reinterpret_cast<R*>(upd_.as<char>() + k * 4);
+    R* dpos = reinterpret_cast<R*>(upd_.as<char>() + idx_bytes);
     R* drot = dpos + 3 * k;
     CDX_HIP_CHECK(hipMemcpyAsync(didx, idx, k * 4, hipMemcpyHostToDevice, s_));
     CDX_HIP_CHECK(hipMemcpyAsync(dpos, pos, k * 3 * sizeof(R), hipMemcpyHostToDevice, s_));
@@ -904,7 +917,7 @@ template <class R> class Scene final : public SceneBase {
   }
   void set_envelope(double e) override { envelope_ = (R)e; }
   void set_np_mode(int mode, double tau) override {
-    CDX_REQUIRE(mode == NP_NATIVE || mode == NP_MIXED, ERR_INVALID, "unknown narrowphase mode");
+    CDX_REQUIRE(mode == NP_NATIVE || mode == NP_MIXED || mode == NP_FAST, ERR_INVALID, "unknown narrowphase mode");
     mode_ = mode;
     tau_ = (float)tau;
   }
@@ -1060,11 +1073,11 @@ template <class R> class Scene final : public SceneBase {
       else
         hipLaunchKernelGGL(k_np_mixed<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P,
                            (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
-                           tau_, o, queue_.as<uint32_t>(), st);
+                           tau_, mode_ == NP_FAST ? 1 : 0, o, queue_.as<uint32_t>(), st);
     }
     mark(ST_NP_MAIN, false);
     mark(ST_NP_RESOLVE, true);
-    if (P && mode_ == NP_MIXED)
+    if (P && mode_ != NP_NATIVE)
       hipLaunchKernelGGL(k_np_resolve<R>, dim3(std::min<unsigned>(blocks_for(P, 128), 2048u)), dim3(128), 0, s_, P,
                          (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, o,
                          (const uint32_t*)queue_.as<uint32_t>(), (const DevStatus*)st);

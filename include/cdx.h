@@ -48,13 +48,21 @@ extern "C" {
 
 #define CDX_CAPSULE 7
 
-/* Narrowphase precision.  NATIVE runs the MPR in the scene's real type with
- * the reference's evaluation order.  MIXED runs it in FP32 in the frame of
- * the first shape and re-solves in FP64 every pair whose FP32 verdict is
- * within tau x (sum of the two bounding radii) of a decision boundary
- * (near-zero depth or separation, degenerate portal, iteration cap).  */
+/* Narrowphase precision.
+ * NATIVE: the MPR in the scene's real type with the reference's evaluation
+ *   order (Narrowphase.cu:122-423).
+ * MIXED: an FP32 MPR in the frame of the first shape decides clear misses;
+ *   every hit, and every pair whose FP32 verdict lies within
+ *   tau x (sum of the two bounding radii) of a decision boundary (near-zero
+ *   separation, degenerate portal, coincident centres, iteration cap), is
+ *   re-solved in FP64 on the reference path, so contacts equal NATIVE FP64.
+ * FAST: like MIXED, but FP32 contacts are kept for clear hits (depth beyond
+ *   tau); only uncertain pairs are re-solved in FP64.  FP32 resolves the
+ *   MPR portal only to about sqrt(FLT_EPSILON) on curved surfaces, so
+ *   normals differ from FP64 by ~1e-4 (median) to ~1e-3 (p99). */
 #define CDX_NP_NATIVE 0
 #define CDX_NP_MIXED 1
+#define CDX_NP_FAST 2
 
 /* Stages reported by cdx_scene_stage_ms (HIP-event times of the last run). */
 #define CDX_STAGE_AABB 0

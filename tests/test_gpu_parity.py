@@ -121,22 +121,48 @@ def test_dropin_narrowphase_envelope_and_inactive(cdx, ctx, oracle, scenes):
     np.testing.assert_allclose(d, ref.dpth, atol=t, rtol=0)
 
 
-def test_mixed_precision(cdx, ctx, oracle, scenes):
-    """FP32 MPR + FP64 re-solve against the FP64 oracle.
+@pytest.mark.parametrize("mode", ["mixed", "native"])
+def test_mixed_equals_native_fp64(cdx, ctx, oracle, scenes, mode):
+    """MIXED (FP32 filter + FP64 reference path for hits) equals the FP64 oracle."""
+    s = scenes.mixed_bodies(n=50_000, seed=31)
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    ref = oracle.narrowphase(s, pairs)
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.set_mode(cdx.NP_MIXED if mode == "mixed" else cdx.NP_NATIVE, 1e-2)
+    sc.step()
+    sc.sync()
+    c = sc.counts()
+    print(f"mode={mode} pairs={c.pairs} contacts={c.contacts} resolved_fp64={c.resolved}", flush=True)
+    norm, pa, pb, d, ids = sc.read_contacts()
+    assert c.contacts == ref.count
+    np.testing.assert_array_equal(ids, ref.bids)
+    np.testing.assert_array_equal(sc.read_pairs_permuted(), ref.pairs_permuted)
+    t = _tol(s)
+    np.testing.assert_allclose(norm, ref.norm, rtol=0, atol=t)
+    np.testing.assert_allclose(pa, ref.cpta, rtol=0, atol=t)
+    np.testing.assert_allclose(pb, ref.cptb, rtol=0, atol=t)
+    np.testing.assert_allclose(d, ref.dpth, rtol=0, atol=t)
 
-    Stated tolerance: contact set and ids exact; |N32 - N64| <= 1e-5 and
-    |d32 - d64| <= 1e-5 x (rA + rB) (bounding radii) for every contact.
+
+def test_fast_mode_error_report(cdx, ctx, oracle, scenes):
+    """FAST keeps FP32 contacts for clear hits.
+
+    Stated tolerance: the contact set and ids equal the FP64 oracle's; the
+    median normal difference is below 1e-3 and the median depth difference
+    below 1e-4 x (rA + rB).  (FP32 resolves curved MPR portals to about
+    sqrt(FLT_EPSILON), and the reference's FP64 MPR stops at its 34-iteration
+    cap, so neither side is converged beyond ~1e-4.)
     """
     s = scenes.mixed_bodies(n=50_000, seed=31)
     ab = oracle.aabb(s)
     pairs = oracle.broadphase(ab, s.grid).pairs
     ref = oracle.narrowphase(s, pairs)
     sc = cdx.Scene(ctx, s.dtype).load(s)
-    sc.set_mode(cdx.NP_MIXED, 1e-2)
+    sc.set_mode(cdx.NP_FAST, 1e-2)
     sc.step()
     sc.sync()
     c = sc.counts()
-    print(f"pairs={c.pairs} contacts={c.contacts} resolved={c.resolved}", flush=True)
     norm, pa, pb, d, ids = sc.read_contacts()
     assert c.contacts == ref.count
     np.testing.assert_array_equal(ids, ref.bids)
@@ -149,10 +175,11 @@ def test_mixed_precision(cdx, ctx, oracle, scenes):
     scale = rad[a] + rad[b]
     en = np.linalg.norm(norm - ref.norm, axis=1)
     ed = np.abs(d - ref.dpth) / scale
-    print(f"normal err max={en.max():.3e} p99={np.quantile(en, 0.99):.3e}; "
-          f"depth err/scale max={ed.max():.3e} p99={np.quantile(ed, 0.99):.3e}", flush=True)
-    assert en.max() <= 1e-5
-    assert ed.max() <= 1e-5
+    print(f"FAST resolved_fp64={c.resolved}/{c.pairs}; normal diff p50={np.median(en):.2e} "
+          f"p99={np.quantile(en, 0.99):.2e} max={en.max():.2e}; depth diff/scale p50={np.median(ed):.2e} "
+          f"p99={np.quantile(ed, 0.99):.2e} max={ed.max():.2e}", flush=True)
+    assert np.median(en) < 1e-3
+    assert np.median(ed) < 1e-4
 
 
 def test_kat_single_bin(cdx, ctx):

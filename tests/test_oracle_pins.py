@@ -1,0 +1,319 @@
+"""Pins of the oracle by mathematics independent of its code.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a swapped sign, a transposed rotation, a wrong composition order,
+an off-by-one cell) fails it.
+"""
+import numpy as np
+import pytest
+
+SPHERE, ELLIPSOID, BOX, CYLINDER, CAPSULE = 0, 1, 2, 3, 7
+
+
+def quat_to_mat(q):
+    """Textbook rotation matrix of unit quaternion (w,x,y,z)."""
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def hamilton(a, b):
+    aw, av = a[0], np.asarray(a[1:])
+    bw, bv = b[0], np.asarray(b[1:])
+    return np.concatenate([[aw * bw - av @ bv], aw * bv + bw * av + np.cross(av, bv)])
+
+
+def rand_quat(rng):
+    q = rng.standard_normal(4)
+    return q / np.linalg.norm(q)
+
+
+# ------------------------------------------------------------ arithmetic --
+
+def test_rotation_is_q_v_qconj(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        q = rand_quat(rng)
+        v = rng.standard_normal(3)
+        ref = hamilton(hamilton(q, np.concatenate([[0], v])), q * np.array([1, -1, -1, -1]))[1:]
+        np.testing.assert_allclose(oracle.quat_rotate(v, q), ref, atol=1e-14)
+
+
+def test_mult_is_hamilton_product(oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        a, b = rand_quat(rng), rand_quat(rng)
+        np.testing.assert_allclose(oracle.quat_mult(a, b), hamilton(a, b), atol=1e-15)
+        # composition order: rotating by mult(a,b) == rotating by b then by a
+        v = rng.standard_normal(3)
+        np.testing.assert_allclose(oracle.quat_rotate(v, oracle.quat_mult(a, b)),
+                                   oracle.quat_rotate(oracle.quat_rotate(v, b), a), atol=1e-14)
+
+
+# ------------------------------------------------------------------ AABB --
+
+def test_sphere_aabb_closed_form(oracle, scenes):
+    s = scenes.small_spheres(n=2000)
+    ab = oracle.aabb(s)
+    r = s.B[:, :1]
+    np.testing.assert_array_equal(ab[:s.n], s.pos - r)
+    np.testing.assert_array_equal(ab[s.n:], s.pos + r)
+
+
+def _corner_bounds(center, Rm, half):
+    signs = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], dtype=float)
+    pts = center + (signs * half) @ Rm.T
+    return pts.min(axis=0), pts.max(axis=0)
+
+
+@pytest.mark.parametrize("t", [BOX, ELLIPSOID, CYLINDER, CAPSULE])
+def test_box_like_aabb_vs_corners(oracle, scenes, t):
+    """Box-like AABB == bounds of the 8 rotated corners of the bounding box,
+    with a non-trivial local position and local rotation on top of the body pose."""
+    rng = np.random.default_rng(3 + t)
+    n = 300
+    s = scenes.mixed_bodies(n=n, seed=4)
+    s.T[:] = t
+    s.A = rng.uniform(-0.05, 0.05, (n, 3))
+    s.R = np.array([rand_quat(rng) for _ in range(n)])
+    ab = oracle.aabb(s)
+    for i in range(n):
+        Rb = quat_to_mat(s.rot[i])
+        Rl = quat_to_mat(s.R[i])
+        c = Rb @ s.A[i] + s.pos[i]
+        half = s.B[i].copy()
+        if t == CAPSULE:
+            half[1] += half[0]
+        lo, hi = _corner_bounds(c, Rb @ Rl, half)
+        np.testing.assert_allclose(ab[i], lo, atol=1e-14)
+        np.testing.assert_allclose(ab[n + i], hi, atol=1e-14)
+
+
+# ------------------------------------------------------------ broadphase --
+
+def _brute_pairs(shifted):
+    n = len(shifted) // 2
+    mn, mx = shifted[:n], shifted[n:]
+    out = []
+    for i in range(n - 1):
+        ov = np.all((mn[i] <= mx[i + 1:]) & (mn[i + 1:] <= mx[i]), axis=1)
+        j = np.nonzero(ov)[0] + i + 1
+        out.append((np.int64(i) << 32) | j.astype(np.int64))
+    return np.sort(np.concatenate(out)) if out else np.zeros(0, np.int64)
+
+
+def _bin_reference(shifted, inv, grid):
+    """Independent cell enumeration: truncation toward zero, keys in unsigned 32-bit."""
+    n = len(shifted) // 2
+    dt = shifted.dtype
+    g0 = (shifted[:n] * inv.astype(dt)).astype(np.int64)
+    g1 = (shifted[n:] * inv.astype(dt)).astype(np.int64)
+    counts = np.prod(g1 - g0 + 1, axis=1).astype(np.uint32)
+    keys, bodies = [], []
+    gx, gy = np.uint64(grid[0]), np.uint64(grid[1])
+    for i in range(n):
+        xs, ys, zs = (np.arange(g0[i, c], g1[i, c] + 1, dtype=np.uint64) for c in range(3))
+        X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+        k = ((Z * gy * gx + Y * gx + X) & np.uint64(0xffffffff)).ravel()
+        keys.append(k)
+        bodies.append(np.full(k.size, i, np.uint64))
+    keys = np.concatenate(keys)
+    bodies = np.concatenate(bodies)
+    order = np.lexsort((bodies, keys))
+    return counts, keys[order].astype(np.uint32), bodies[order].astype(np.uint32)
+
+
+@pytest.mark.parametrize("case", ["spheres_f64", "mixed_f64", "mixed_f32", "lattice_f32", "small_grid"])
+def test_broadphase_vs_independent(oracle, scenes, case):
+    s = {
+        "spheres_f64": lambda: scenes.small_spheres(),
+        "mixed_f64": lambda: scenes.mixed_bodies(n=6000, seed=5),
+        "mixed_f32": lambda: scenes.mixed_bodies(n=6000, seed=6, dtype=np.float32),
+        "lattice_f32": lambda: scenes.lattice_spheres(side=12, seed=7),
+        "small_grid": lambda: scenes.mixed_bodies(n=3000, seed=8, grid=(3, 5, 2)),
+    }[case]()
+    ab = oracle.aabb(s)
+    r = oracle.broadphase(ab, s.grid)
+    # bounds narrowed to float (Broadphase.cu:368-376) and the in-place shift
+    np.testing.assert_array_equal(r.bmin, ab.min(axis=0).astype(np.float32))
+    np.testing.assert_array_equal(r.bmax, ab.max(axis=0).astype(np.float32))
+    origin = np.abs(r.bmin.astype(s.dtype))
+    np.testing.assert_array_equal(r.aabb_shifted, ab + origin)
+    ext = np.abs(r.bmax.astype(s.dtype) + np.abs(r.bmin.astype(s.dtype)))
+    inv = np.asarray(s.grid, dtype=s.dtype) / ext
+    counts, keys, bodies = _bin_reference(r.aabb_shifted, inv, s.grid)
+    np.testing.assert_array_equal(r.bin_counts, counts)
+    np.testing.assert_array_equal(r.bins_intersected, np.cumsum(counts, dtype=np.uint64).astype(np.uint32))
+    np.testing.assert_array_equal(r.sorted_keys, keys)
+    np.testing.assert_array_equal(r.sorted_bodies, bodies)
+    uk, uc = np.unique(keys, return_counts=True)
+    np.testing.assert_array_equal(r.active_keys, uk)
+    np.testing.assert_array_equal(r.active_counts, uc.astype(np.uint32))
+    np.testing.assert_array_equal(r.pairs, _brute_pairs(r.aabb_shifted))
+    assert r.pdup >= len(r.pairs)
+
+
+def test_broadphase_parallel_flags_agree(oracle, scenes):
+    s = scenes.mixed_bodies(n=20000, seed=9)
+    ab = oracle.aabb(s)
+    a = oracle.broadphase(ab, s.grid, oracle.DEFAULT_FLAGS)
+    b = oracle.broadphase(ab, s.grid, oracle.ALL_PARALLEL)
+    np.testing.assert_array_equal(a.sorted_keys, b.sorted_keys)
+    np.testing.assert_array_equal(a.sorted_bodies, b.sorted_bodies)
+    np.testing.assert_array_equal(a.pairs, b.pairs)
+
+
+# ----------------------------------------------------------- narrowphase --
+
+def test_sphere_sphere_closed_form(oracle, scenes):
+    s = scenes.small_spheres()
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    r = oracle.narrowphase(s, pairs)
+    a = (pairs >> 32)
+    b = pairs & 0xffffffff
+    d = s.pos[b] - s.pos[a]
+    dist = np.linalg.norm(d, axis=1)
+    rs = s.B[a, 0] + s.B[b, 0]
+    hit = np.sum(d * d, axis=1) <= rs * rs
+    np.testing.assert_array_equal(r.hit.astype(bool), hit)
+    a, b, d, dist = a[hit], b[hit], d[hit], dist[hit]
+    N = d / dist[:, None]
+    np.testing.assert_allclose(r.norm, N, atol=1e-13)
+    np.testing.assert_allclose(r.cpta, s.pos[a] + N * s.B[a, :1], atol=1e-13)
+    np.testing.assert_allclose(r.cptb, s.pos[b] - N * s.B[b, :1], atol=1e-13)
+    np.testing.assert_allclose(r.dpth, dist - s.B[a, 0] - s.B[b, 0], atol=1e-13)
+    np.testing.assert_array_equal(r.bids, np.stack([a, b], axis=1))
+
+
+def _support_brute(t, B, Rm, c, d):
+    """Support point by maximising over a dense sample of the shape's surface."""
+    if t == BOX:
+        signs = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], dtype=float)
+        pts = (signs * B) @ Rm.T + c
+        return pts[np.argmax(pts @ d)]
+    dl = Rm.T @ d
+    if t == SPHERE:
+        return c + B[0] * d / np.linalg.norm(d)
+    if t == ELLIPSOID:
+        # argmax over the ellipsoid of <x, dl> = B^2 dl / |B dl|
+        x = B * B * dl / np.linalg.norm(B * dl)
+        return Rm @ x + c
+    if t == CYLINDER:
+        w = np.array([dl[0], 0, dl[2]])
+        x = np.sign(dl[1]) * B[1] * np.array([0, 1, 0]) + B[0] * w / np.linalg.norm(w)
+        return Rm @ x + c
+    if t == CAPSULE:
+        x = np.sign(dl[1]) * B[1] * np.array([0, 1, 0]) + B[0] * dl / np.linalg.norm(dl)
+        return Rm @ x + c
+
+
+@pytest.mark.parametrize("t", [SPHERE, ELLIPSOID, BOX, CYLINDER, CAPSULE])
+def test_support_maps(oracle, t):
+    """World support points: maximal along the direction, and equal to the closed form."""
+    rng = np.random.default_rng(10 + t)
+    for _ in range(100):
+        q = rand_quat(rng)
+        Rm = quat_to_mat(q)
+        B = rng.uniform(0.5, 1.5, 3)
+        c = rng.standard_normal(3)
+        d = rng.standard_normal(3)
+        got = oracle.support(t, c, B, q, d)
+        np.testing.assert_allclose(got, _support_brute(t, B, Rm, c, d), atol=1e-12)
+        # no sampled surface point lies further along d
+        u = rng.standard_normal((400, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        sample = np.array([oracle.support(t, c, B, q, v) for v in u[:40]])
+        assert np.all(sample @ d <= got @ d + 1e-12)
+
+
+IDQ = np.array([1.0, 0, 0, 0])
+
+
+def _mpr_case(tA, BA, XA, qA, tB, BB, XB, qB):
+    import oracle as o
+    hit, n, p, depth = o.mpr(tA, XA, BA, qA, tB, XB, BB, qB)
+    return hit, n, p, depth
+
+
+@pytest.mark.parametrize("dx", [0.9, 0.5, 1.1])
+def test_mpr_box_box_face(oracle, dx):
+    """Axis-aligned unit-half boxes offset along x: overlap depth = 2*0.5 - dx along x."""
+    XA = np.zeros(3)
+    XB = np.array([dx, 0.013, -0.021])
+    B = np.array([0.5, 0.5, 0.5])
+    hit, n, p, depth = _mpr_case(BOX, B, XA, IDQ, BOX, B, XB, IDQ)
+    if dx > 1.0:
+        assert not hit
+        return
+    assert hit
+    np.testing.assert_allclose(np.abs(n), [1, 0, 0], atol=1e-9)
+    np.testing.assert_allclose(depth, 1.0 - dx, atol=1e-9)
+
+
+@pytest.mark.parametrize("t,axis,expect", [
+    (CAPSULE, 1, lambda r, h, rs, d: h + r + rs - d),   # along the capsule axis
+    (CAPSULE, 0, lambda r, h, rs, d: r + rs - d),       # radial
+    (CYLINDER, 1, lambda r, h, rs, d: h + rs - d),      # onto the flat cap
+    (CYLINDER, 0, lambda r, h, rs, d: r + rs - d),      # radial
+    (ELLIPSOID, 0, lambda r, h, rs, d: r + rs - d),     # equal axes == sphere
+])
+def test_mpr_closed_forms_vs_sphere(oracle, t, axis, expect):
+    """Shape at the origin, sphere of radius rs displaced along one axis (slightly off-axis)."""
+    r, h, rs = 0.3, 0.6, 0.25
+    B = np.array([r, h, r]) if t != ELLIPSOID else np.array([r, r, r])
+    for d in (0.2, 0.45, 0.7):
+        # axial cases start from the end of the shape's core (cap plane / segment end)
+        dist = d + h if axis == 1 else d
+        XB = np.zeros(3)
+        XB[axis] = dist
+        XB[(axis + 1) % 3] += 1e-4
+        hit, n, p, depth = _mpr_case(t, B, np.zeros(3), IDQ, SPHERE, np.array([rs, 0, 0]), XB, IDQ)
+        want = expect(r, h, rs, dist)
+        if want <= 0:
+            continue
+        assert hit
+        assert abs(n[axis]) > 0.999
+        np.testing.assert_allclose(depth, want, atol=2e-3 * rs)
+
+
+def test_contact_invariants(oracle, scenes):
+    """|N| = 1, penetration depth <= 0 and depth = N.(p2 - p1) for every contact."""
+    s = scenes.mixed_bodies(n=6000, seed=15)
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    r = oracle.narrowphase(s, pairs)
+    assert r.count > 100
+    np.testing.assert_allclose(np.linalg.norm(r.norm, axis=1), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.sum(r.norm * (r.cptb - r.cpta), axis=1), r.dpth, atol=1e-13)
+    assert np.all(r.dpth <= 1e-12)
+    # contacts are the hits in pair order; misses follow, order kept
+    hits = pairs[r.hit.astype(bool)]
+    np.testing.assert_array_equal(r.pairs_permuted[: r.count], hits)
+    np.testing.assert_array_equal(r.pairs_permuted[r.count:], pairs[~r.hit.astype(bool)])
+
+
+def test_inactive_and_same_body_are_skipped(oracle, scenes):
+    s = scenes.small_spheres(n=3000)
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    full = oracle.narrowphase(s, pairs)
+    s.active[:] = 0
+    s.active[::2] = 1
+    part = oracle.narrowphase(s, pairs)
+    a = pairs >> 32
+    b = pairs & 0xffffffff
+    keep = (s.active[a] == 1) | (s.active[b] == 1)
+    np.testing.assert_array_equal(part.hit.astype(bool), full.hit.astype(bool) & keep)
+    s.ID[:] = 0  # every shape on body 0 -> no contacts at all
+    assert oracle.narrowphase(s, pairs).count == 0
+
+
+def test_envelope_adds_twice_to_depth(oracle, scenes):
+    s = scenes.small_spheres(n=3000)
+    ab = oracle.aabb(s)
+    pairs = oracle.broadphase(ab, s.grid).pairs
+    r0 = oracle.narrowphase(s, pairs)
+    r1 = oracle.narrowphase(s, pairs, envelope=0.002)
+    np.testing.assert_allclose(r1.dpth, r0.dpth + 0.004, atol=1e-13)
