@@ -202,18 +202,36 @@ template <class R> __device__ __forceinline__ bool cells_ok(const Cells<R>& c, i
          c.y1 <= gy && c.z1 <= gz;
 }
 
-// Bins spanned per AABB (Broadphase.cu:150-159), plus the largest key
-// (the max corner's, when the key space does not wrap) to bound the number
-// of radix passes.
+// Per-body broadphase record: shifted AABB and its minimum cell, gathered as
+// one aligned block by the pair search.
+template <class R> struct alignas(16) BpRec {
+  R lo[3], hi[3];
+  int g[3];
+  int pad;
+};
+
+// Bins spanned per AABB (Broadphase.cu:150-159), the per-body record, and
+// the largest key (the max corner's, when the key space does not wrap) to
+// bound the number of radix passes.
 template <class R, int BS>
 __global__ __launch_bounds__(BS) void k_count(uint64_t n, const R* __restrict__ aabb, const Params<R>* __restrict__ prm,
-                                               int gx, int gy, int gz, uint32_t* __restrict__ counts, DevStatus* st) {
+                                               int gx, int gy, int gz, uint32_t* __restrict__ counts,
+                                               BpRec<R>* __restrict__ brec, DevStatus* st) {
   __shared__ uint32_t skm[BS / WAVE];
   uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x;
   uint32_t km = 0;
   if (i < n) {
     Params<R> p = *prm;
     Cells<R> c = cells_of(aabb, n, i, p);
+    BpRec<R> r;
+    r.lo[0] = aabb[i] + p.origin[0];
+    r.lo[1] = aabb[2 * n + i] + p.origin[1];
+    r.lo[2] = aabb[4 * n + i] + p.origin[2];
+    r.hi[0] = aabb[n + i] + p.origin[0];
+    r.hi[1] = aabb[3 * n + i] + p.origin[1];
+    r.hi[2] = aabb[5 * n + i] + p.origin[2];
+    r.g[0] = c.x0; r.g[1] = c.y0; r.g[2] = c.z0; r.pad = 0;
+    brec[i] = r;
     if (cells_ok(c, gx, gy, gz)) {
       counts[i] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, gx, gy);
@@ -271,53 +289,52 @@ __global__ void k_run_write(uint64_t m, const uint64_t* __restrict__ keys, const
   }
 }
 
-// Pair search inside one active bin with de-duplication by owning bin.
-// A pair (a, b) of overlapping AABBs (closed test on shifted coordinates,
-// Broadphase.cu:237 / :296) is reported only from the bin whose key is the
-// hash of the cell max(gmin_a, gmin_b): that cell lies in both AABBs' cell
-// ranges, so both bodies appear in its run, and every other run skips the
+// Pair search with de-duplication by owning bin.  A pair (a, b) of
+// overlapping AABBs (closed test on shifted coordinates, Broadphase.cu:237 /
+// :296) is reported only from the bin whose key is the hash of the cell
+// max(gmin_a, gmin_b): that cell lies in both AABBs' cell ranges, so both
+// bodies appear in its run of sorted entries, and every other run skips the
 // pair.  Repeated entries of one body in a run (hash aliasing at index ==
 // grid size, Broadphase.cu:140-147) are skipped, so each pair is emitted once
 // and the reference's sort + unique (:522-535) reduces to a sort.
+//
+// One thread per sorted entry i: it pairs body(i) with the later entries of
+// the same run (bodies ascend inside a run, so body(i) < body(k)).  Work
+// per thread is the distance to the end of its run, which keeps waves far
+// better balanced than a thread per bin.
 template <class R, bool EMIT>
-__global__ void k_pairs(uint64_t a_bins, const uint32_t* __restrict__ run_start, const uint32_t* __restrict__ run_key,
-                        const uint64_t* __restrict__ keys, const R* __restrict__ aabb, uint64_t n,
-                        const Params<R>* __restrict__ prm, int gx, int gy, uint32_t* __restrict__ pcount,
-                        const uint32_t* __restrict__ poffs, uint64_t* __restrict__ pairs) {
-  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a_bins) return;
-  uint32_t s = run_start[r], e = run_start[r + 1], K = run_key[r];
+__global__ __launch_bounds__(256) void k_pairs(uint64_t m, const uint64_t* __restrict__ keys,
+                                               const BpRec<R>* __restrict__ brec, int gx, int gy,
+                                               uint32_t* __restrict__ pcount, const uint32_t* __restrict__ poffs,
+                                               uint64_t* __restrict__ pairs) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  uint64_t ki = keys[i];
   uint32_t cnt = 0;
-  uint64_t o = EMIT ? poffs[r] : 0;
-  if (e - s >= 2) {
-    Params<R> p = *prm;
-    uint32_t prev_a = 0xffffffffu;
-    for (uint32_t i = s; i < e; ++i) {
-      uint32_t a = (uint32_t)keys[i];
-      if (a == prev_a) continue;
-      prev_a = a;
-      R ax0 = aabb[a] + p.origin[0], ay0 = aabb[2 * n + a] + p.origin[1], az0 = aabb[4 * n + a] + p.origin[2];
-      R ax1 = aabb[n + a] + p.origin[0], ay1 = aabb[3 * n + a] + p.origin[1], az1 = aabb[5 * n + a] + p.origin[2];
-      int gax = cell_of(ax0, p.inv[0]), gay = cell_of(ay0, p.inv[1]), gaz = cell_of(az0, p.inv[2]);
-      uint32_t prev_b = a;
-      for (uint32_t k = i + 1; k < e; ++k) {
-        uint32_t b = (uint32_t)keys[k];
-        if (b == prev_b) continue;
-        prev_b = b;
-        R bx0 = aabb[b] + p.origin[0], by0 = aabb[2 * n + b] + p.origin[1], bz0 = aabb[4 * n + b] + p.origin[2];
-        R bx1 = aabb[n + b] + p.origin[0], by1 = aabb[3 * n + b] + p.origin[1], bz1 = aabb[5 * n + b] + p.origin[2];
-        bool hit = (ax0 <= bx1 && bx0 <= ax1) && (ay0 <= by1 && by0 <= ay1) && (az0 <= bz1 && bz0 <= az1);
-        if (!hit) continue;
-        int ox = max(gax, cell_of(bx0, p.inv[0]));
-        int oy = max(gay, cell_of(by0, p.inv[1]));
-        int oz = max(gaz, cell_of(bz0, p.inv[2]));
-        if (hash_index((uint32_t)ox, (uint32_t)oy, (uint32_t)oz, gx, gy) != K) continue;
-        if (EMIT) pairs[o++] = ((uint64_t)a << 32) | (uint64_t)b;
-        else ++cnt;
-      }
+  if (i == 0 || keys[i - 1] != ki) {
+    const uint32_t K = (uint32_t)(ki >> 32), a = (uint32_t)ki;
+    const BpRec<R> ra = brec[a];
+    uint64_t o = EMIT ? poffs[i] : 0;
+    uint32_t prev = a;
+    for (uint64_t k = i + 1; k < m; ++k) {
+      uint64_t kk = keys[k];
+      if ((uint32_t)(kk >> 32) != K) break;
+      uint32_t b = (uint32_t)kk;
+      if (b == prev) continue;
+      prev = b;
+      const BpRec<R> rb = brec[b];
+      bool hit = (ra.lo[0] <= rb.hi[0] && rb.lo[0] <= ra.hi[0]) && (ra.lo[1] <= rb.hi[1] && rb.lo[1] <= ra.hi[1]) &&
+                 (ra.lo[2] <= rb.hi[2] && rb.lo[2] <= ra.hi[2]);
+      if (!hit) continue;
+      uint32_t ox = (uint32_t)max(ra.g[0], rb.g[0]);
+      uint32_t oy = (uint32_t)max(ra.g[1], rb.g[1]);
+      uint32_t oz = (uint32_t)max(ra.g[2], rb.g[2]);
+      if (hash_index(ox, oy, oz, gx, gy) != K) continue;
+      if (EMIT) pairs[o++] = ((uint64_t)a << 32) | (uint64_t)b;
+      else ++cnt;
     }
   }
-  if (!EMIT) pcount[r] = cnt;
+  if (!EMIT) pcount[i] = cnt;
 }
 
 // ----------------------------------------------------------- narrowphase --
@@ -965,10 +982,12 @@ template <class R> class Scene final : public SceneBase {
     // count + scan (host reads M back, as the reference reads .back())
     counts_buf_.reserve(n * 4);
     offs_.reserve(n * 4);
+    brec_.reserve(n * sizeof(BpRec<R>));
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(n), 16) * 4);
     mark(ST_BP_COUNT, true);
     hipLaunchKernelGGL((k_count<R, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s_, n, (const R*)aabb_.as<R>(),
-                       (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1], g_[2], counts_buf_.as<uint32_t>(), st);
+                       (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1], g_[2], counts_buf_.as<uint32_t>(),
+                       brec_.as<BpRec<R>>(), st);
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
     scan_excl_u32(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), n, &st->M, scan_tmp_.as<uint32_t>(), s_);
@@ -1008,29 +1027,24 @@ template <class R> class Scene final : public SceneBase {
                        (const uint64_t*)keys_.as<uint64_t>(), (const uint32_t*)runidx_.as<uint32_t>(),
                        run_start_.as<uint32_t>(), run_key_.as<uint32_t>(), (const DevStatus*)st);
     mark(ST_BP_RUNS, false);
-    read_status();
-    A_ = hstat_->A;
 
-    // pair count, scan, emit
-    pcount_.reserve((A_ + 1) * 4);
-    poffs_.reserve((A_ + 1) * 4);
+    // pair count (per sorted entry), scan, emit
+    pcount_.reserve((M_ + 1) * 4);
+    poffs_.reserve((M_ + 1) * 4);
     mark(ST_BP_PAIRS, true);
-    hipLaunchKernelGGL((k_pairs<R, false>), dim3(blocks_for(A_, 128)), dim3(128), 0, s_, A_,
-                       (const uint32_t*)run_start_.as<uint32_t>(), (const uint32_t*)run_key_.as<uint32_t>(),
-                       (const uint64_t*)keys_.as<uint64_t>(), (const R*)aabb_.as<R>(), n,
-                       (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1], pcount_.as<uint32_t>(),
-                       (const uint32_t*)nullptr, (uint64_t*)nullptr);
-    scan_excl_u32(pcount_.as<uint32_t>(), poffs_.as<uint32_t>(), A_, &st->P, radix_tmp_.as<uint32_t>(), s_);
+    hipLaunchKernelGGL((k_pairs<R, false>), dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_,
+                       (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(), g_[0], g_[1],
+                       pcount_.as<uint32_t>(), (const uint32_t*)nullptr, (uint64_t*)nullptr);
+    scan_excl_u32(pcount_.as<uint32_t>(), poffs_.as<uint32_t>(), M_, &st->P, radix_tmp_.as<uint32_t>(), s_);
     read_status();
     P_ = hstat_->P;
+    A_ = hstat_->A;
     CDX_REQUIRE(P_ < (1ull << 32), ERR_CAPACITY, "more than 2^32 candidate pairs");
     pairs_.reserve(P_ * 8);
     pairs_tmp_.reserve(P_ * 8);
-    hipLaunchKernelGGL((k_pairs<R, true>), dim3(blocks_for(A_, 128)), dim3(128), 0, s_, A_,
-                       (const uint32_t*)run_start_.as<uint32_t>(), (const uint32_t*)run_key_.as<uint32_t>(),
-                       (const uint64_t*)keys_.as<uint64_t>(), (const R*)aabb_.as<R>(), n,
-                       (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1], (uint32_t*)nullptr,
-                       (const uint32_t*)poffs_.as<uint32_t>(), pairs_.as<uint64_t>());
+    hipLaunchKernelGGL((k_pairs<R, true>), dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_,
+                       (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(), g_[0], g_[1],
+                       (uint32_t*)nullptr, (const uint32_t*)poffs_.as<uint32_t>(), pairs_.as<uint64_t>());
     mark(ST_BP_PAIRS, false);
     int nbits = bits_for((uint32_t)(n - 1));
     std::vector<int> ps;
@@ -1170,7 +1184,7 @@ template <class R> class Scene final : public SceneBase {
   }
 
   uint64_t device_bytes() const override {
-    const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
+    const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_, &brec_,
                            &counts_buf_, &offs_, &keys_, &keys_tmp_, &flags_, &runidx_, &run_start_, &run_key_,
                            &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
                            &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_, &scan_tmp_, &radix_tmp_};
@@ -1239,7 +1253,7 @@ template <class R> class Scene final : public SceneBase {
   StepCounts counts_;
   DevBuf status_, params_, stage_, upd_, tmp_out_;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
-  DevBuf aabb_, rec_;
+  DevBuf aabb_, rec_, brec_;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, flags_, runidx_, run_start_, run_key_, pcount_, poffs_, pairs_, pairs_tmp_;
   DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
