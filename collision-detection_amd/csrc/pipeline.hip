@@ -11,7 +11,7 @@
 //   reduce_by_key + scan #2  :439-480      -> k_run_flags, k_run_write
 //   pair count / store / sort / unique :484-535
 //                                          -> k_pairs (owner-bin dedup), radix_sort_u64
-//   MPR store    Narrowphase.cu:316-504    -> k_np_native / k_np_mixed + k_np_resolve
+//   MPR store    Narrowphase.cu:316-504    -> k_np_front + class sort + k_np_resolve
 //   compaction   Narrowphase.cu:580-593    -> scan + k_np_compact
 #include <algorithm>
 #include <cstring>
@@ -342,11 +342,14 @@ __global__ __launch_bounds__(256) void k_pairs(uint64_t m, const uint64_t* __res
 template <class R> struct Placed {
   int type;
   v3<R> X, B;
-  mat3<R> M;
+  q4<R> q;
 };
 
+// The rotation matrix is rebuilt from the quaternion at every support call:
+// the same rounded values each time, and far fewer live registers than a
+// cached 3x3 per shape (the FP64 MPR is occupancy-bound).
 template <class R> __device__ __forceinline__ v3<R> sup(const Placed<R>& s, v3<R> b) {
-  return support_world(s.type, s.X, s.B, s.M, b);
+  return support_world(s.type, s.X, s.B, rotmat(s.q), b);
 }
 
 template <class R> __device__ __forceinline__ Placed<R> placed(const ShapeRec<R>& r) {
@@ -354,8 +357,7 @@ template <class R> __device__ __forceinline__ Placed<R> placed(const ShapeRec<R>
   s.type = (int)r.type;
   s.X = mk3(r.x, r.y, r.z);
   s.B = mk3(r.bx, r.by, r.bz);
-  q4<R> q; q.w = r.qw; q.x = r.qx; q.y = r.qy; q.z = r.qz;
-  s.M = rotmat(q);
+  s.q.w = r.qw; s.q.x = r.qx; s.q.y = r.qy; s.q.z = r.qz;
   return s;
 }
 
@@ -501,6 +503,7 @@ template <class R> struct PairOut {
   R* v;          // 10 * P: N(3) p1(3) p2(3) depth
   int32_t* ids;  // 2 * P
   uint32_t* hit; // P
+  uint64_t P;
 };
 
 template <class R>
@@ -514,18 +517,6 @@ __device__ __forceinline__ void store_pair(const PairOut<R>& o, uint64_t P, uint
   o.ids[2 * i] = (int32_t)ida;
   o.ids[2 * i + 1] = (int32_t)idb;
   o.hit[i] = 1u;
-}
-
-template <class R>
-__global__ __launch_bounds__(128) void k_np_native(uint64_t P, const uint64_t* __restrict__ pairs,
-                                                   const ShapeRec<R>* __restrict__ rec, R envelope, PairOut<R> o) {
-  uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
-  if (i >= P) return;
-  uint64_t pr = pairs[i];
-  ShapeRec<R> ra = rec[(uint32_t)(pr >> 32)], rb = rec[(uint32_t)(pr & 0xffffffffu)];
-  Contact<R> c;
-  if (contact_native(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
-  else o.hit[i] = 0u;
 }
 
 // ---- mixed precision: FP32 MPR in the frame of shape A, FP64 re-solve ----
@@ -653,65 +644,11 @@ template <class R> __device__ __forceinline__ Placed<float> placed_local(const S
   s.X = mk3((float)((double)r.x - (double)origin.x), (float)((double)r.y - (double)origin.y),
             (float)((double)r.z - (double)origin.z));
   s.B = mk3((float)r.bx, (float)r.by, (float)r.bz);
-  q4<double> q; q.w = r.qw; q.x = r.qx; q.y = r.qy; q.z = r.qz;
-  mat3<double> m = rotmat(q);
-  s.M.a00 = (float)m.a00; s.M.a01 = (float)m.a01; s.M.a02 = (float)m.a02;
-  s.M.a10 = (float)m.a10; s.M.a11 = (float)m.a11; s.M.a12 = (float)m.a12;
-  s.M.a20 = (float)m.a20; s.M.a21 = (float)m.a21; s.M.a22 = (float)m.a22;
+  s.q.w = (float)r.qw; s.q.x = (float)r.qx; s.q.y = (float)r.qy; s.q.z = (float)r.qz;
   return s;
 }
 
-template <class R>
-__global__ __launch_bounds__(128) void k_np_mixed(uint64_t P, const uint64_t* __restrict__ pairs,
-                                                  const ShapeRec<R>* __restrict__ rec, R envelope, float tau_rel,
-                                                  int keep_fp32_hits, PairOut<R> o, uint32_t* __restrict__ queue,
-                                                  DevStatus* st) {
-  uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
-  if (i >= P) return;
-  uint64_t pr = pairs[i];
-  ShapeRec<R> ra = rec[(uint32_t)(pr >> 32)], rb = rec[(uint32_t)(pr & 0xffffffffu)];
-  o.hit[i] = 0u;
-  if (!ra.active && !rb.active) return;
-  if (ra.body == rb.body) return;
-  if (ra.type == SPHERE && rb.type == SPHERE) {
-    Contact<R> c;
-    if (contact_native(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
-    return;
-  }
-  Placed<float> A = placed_local(ra, ra), B = placed_local(rb, ra);
-  float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
-  v3<float> N, p0;
-  int r = mpr_guarded(A, B, tau, !keep_fp32_hits, N, p0);
-  // MIXED: FP32 settles clear misses; every hit takes the FP64 reference
-  // path, whose capped iteration sequence FP32 cannot reproduce.
-  if (r == MPR_HIT && !keep_fp32_hits) r = MPR_UNSURE;
-  if (r == MPR_HIT) {
-    v3<float> p1 = dot((sup(A, -N) - p0), N) * N + p0;
-    v3<float> p2 = dot((sup(B, N) - p0), N) * N + p0;
-    N = -N;
-    float dl = dot(N, p2 - p1);
-    if (fabsf(dl) < tau) r = MPR_UNSURE;
-    else {
-      Contact<R> c;
-      v3<double> Nd = mk3((double)N.x, (double)N.y, (double)N.z);
-      v3<double> o3 = mk3((double)ra.x, (double)ra.y, (double)ra.z);
-      v3<double> q1 = mk3((double)p1.x, (double)p1.y, (double)p1.z) + o3;
-      v3<double> q2 = mk3((double)p2.x, (double)p2.y, (double)p2.z) + o3;
-      q1 = q1 - Nd * (double)envelope;
-      q2 = q2 + Nd * (double)envelope;
-      c.N = mk3((R)Nd.x, (R)Nd.y, (R)Nd.z);
-      c.p1 = mk3((R)q1.x, (R)q1.y, (R)q1.z);
-      c.p2 = mk3((R)q2.x, (R)q2.y, (R)q2.z);
-      c.depth = (R)dot(Nd, q2 - q1);
-      store_pair(o, P, i, c, ra.body, rb.body);
-      return;
-    }
-  }
-  if (r == MPR_UNSURE) {
-    uint64_t slot = atomicAdd((unsigned long long*)&st->Q, 1ull);
-    queue[slot] = (uint32_t)i;
-  }
-}
+__device__ __forceinline__ uint32_t type_slot(uint32_t t) { return t == CAPSULE ? 4u : t; }
 
 template <class R> __device__ __forceinline__ ShapeRec<double> widen(const ShapeRec<R>& r) {
   ShapeRec<double> d;
@@ -719,26 +656,106 @@ template <class R> __device__ __forceinline__ ShapeRec<double> widen(const Shape
   d.bx = r.bx; d.by = r.by; d.bz = r.bz; d.type = r.type; d.body = r.body; d.active = r.active; d.pad = 0;
   return d;
 }
+template <> __device__ __forceinline__ ShapeRec<double> widen<double>(const ShapeRec<double>& r) { return r; }
 
-// FP64 re-solve of the queued pairs with the reference evaluation order.
+template <class CT, class R> __device__ __forceinline__ ShapeRec<CT> as_compute(const ShapeRec<R>& r);
+template <> __device__ __forceinline__ ShapeRec<double> as_compute<double, double>(const ShapeRec<double>& r) { return r; }
+template <> __device__ __forceinline__ ShapeRec<double> as_compute<double, float>(const ShapeRec<float>& r) { return widen(r); }
+template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(const ShapeRec<float>& r) { return r; }
+
+// Front pass over all candidate pairs (function_MPR_Store's early exits and
+// sphere-sphere closed form, Narrowphase.cu:338-396).  Pairs that need the
+// MPR are appended to a queue as (shape-pair class << 32 | pair index), one
+// atomic per wavefront; a one-digit radix pass then groups the queue by
+// class so every wavefront of the MPR kernel runs one pair of support maps.
+// MIXED / FAST first run an FP32 MPR in the frame of shape A: clear misses
+// end here, FAST keeps clear FP32 hits, everything else is queued for the
+// reference-order MPR.
 template <class R>
-__global__ __launch_bounds__(128) void k_np_resolve(uint64_t P, const uint64_t* __restrict__ pairs,
-                                                    const ShapeRec<R>* __restrict__ rec, R envelope, PairOut<R> o,
-                                                    const uint32_t* __restrict__ queue, const DevStatus* st) {
-  uint64_t q = st->Q;
-  for (uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x; j < q; j += (uint64_t)gridDim.x * 128) {
-    uint32_t i = queue[j];
+__global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __restrict__ pairs,
+                                                  const ShapeRec<R>* __restrict__ rec, R envelope, float tau_rel,
+                                                  int mode, PairOut<R> o, uint64_t* __restrict__ queue,
+                                                  DevStatus* st) {
+  uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  bool enq = false;
+  uint32_t cls = 0;
+  if (i < P) {
     uint64_t pr = pairs[i];
-    ShapeRec<double> ra = widen(rec[(uint32_t)(pr >> 32)]), rb = widen(rec[(uint32_t)(pr & 0xffffffffu)]);
-    Contact<double> c;
-    if (contact_native(ra, rb, (double)envelope, c)) {
-      Contact<R> cr;
-      cr.N = mk3((R)c.N.x, (R)c.N.y, (R)c.N.z);
-      cr.p1 = mk3((R)c.p1.x, (R)c.p1.y, (R)c.p1.z);
-      cr.p2 = mk3((R)c.p2.x, (R)c.p2.y, (R)c.p2.z);
-      cr.depth = (R)c.depth;
-      store_pair(o, P, i, cr, ra.body, rb.body);
+    ShapeRec<R> ra = rec[(uint32_t)(pr >> 32)], rb = rec[(uint32_t)(pr & 0xffffffffu)];
+    o.hit[i] = 0u;
+    cls = type_slot(ra.type) * 5u + type_slot(rb.type);
+    if ((ra.active || rb.active) && ra.body != rb.body) {
+      if (ra.type == SPHERE && rb.type == SPHERE) {
+        Contact<R> c;
+        if (contact_native(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
+      } else if (mode == NP_NATIVE) {
+        enq = true;
+      } else {
+        Placed<float> A = placed_local(ra, ra), B = placed_local(rb, ra);
+        float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
+        v3<float> N, p0;
+        const bool keep = (mode == NP_FAST);
+        int r = mpr_guarded(A, B, tau, !keep, N, p0);
+        if (r == MPR_HIT && keep) {
+          v3<float> p1 = dot((sup(A, -N) - p0), N) * N + p0;
+          v3<float> p2 = dot((sup(B, N) - p0), N) * N + p0;
+          N = -N;
+          float dl = dot(N, p2 - p1);
+          if (fabsf(dl) < tau) {
+            r = MPR_UNSURE;
+          } else {
+            Contact<R> c;
+            v3<double> Nd = mk3((double)N.x, (double)N.y, (double)N.z);
+            v3<double> o3 = mk3((double)ra.x, (double)ra.y, (double)ra.z);
+            v3<double> q1 = mk3((double)p1.x, (double)p1.y, (double)p1.z) + o3;
+            v3<double> q2 = mk3((double)p2.x, (double)p2.y, (double)p2.z) + o3;
+            q1 = q1 - Nd * (double)envelope;
+            q2 = q2 + Nd * (double)envelope;
+            c.N = mk3((R)Nd.x, (R)Nd.y, (R)Nd.z);
+            c.p1 = mk3((R)q1.x, (R)q1.y, (R)q1.z);
+            c.p2 = mk3((R)q2.x, (R)q2.y, (R)q2.z);
+            c.depth = (R)dot(Nd, q2 - q1);
+            store_pair(o, P, i, c, ra.body, rb.body);
+          }
+        }
+        // MIXED: FP32 settles clear misses only; hits take the reference
+        // path, whose capped iteration sequence FP32 cannot reproduce.
+        enq = (r == MPR_UNSURE) || (r == MPR_HIT && !keep);
+      }
     }
+  }
+  const uint64_t mask = __ballot(enq);
+  if (mask) {
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+    uint64_t base = 0;
+    if (lane == leader) base = atomicAdd((unsigned long long*)&st->Q, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    if (enq) queue[base + __popcll(mask & lanemask_lt())] = ((uint64_t)cls << 32) | i;
+  }
+}
+
+// MPR on the reference path for the queued pairs, in compute type CT
+// (the scene's real type for NATIVE, FP64 for MIXED / FAST).
+template <class R, class CT>
+__global__ __launch_bounds__(128) void k_np_resolve(uint64_t q, const uint64_t* __restrict__ pairs,
+                                                    const ShapeRec<R>* __restrict__ rec, R envelope, PairOut<R> o,
+                                                    const uint64_t* __restrict__ queue) {
+  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (j >= q) return;
+  uint64_t P = o.P;
+  uint32_t i = (uint32_t)queue[j];
+  uint64_t pr = pairs[i];
+  ShapeRec<CT> ra = as_compute<CT, R>(rec[(uint32_t)(pr >> 32)]);
+  ShapeRec<CT> rb = as_compute<CT, R>(rec[(uint32_t)(pr & 0xffffffffu)]);
+  Contact<CT> c;
+  if (contact_native(ra, rb, (CT)envelope, c)) {
+    Contact<R> cr;
+    cr.N = mk3((R)c.N.x, (R)c.N.y, (R)c.N.z);
+    cr.p1 = mk3((R)c.p1.x, (R)c.p1.y, (R)c.p1.z);
+    cr.p2 = mk3((R)c.p2.x, (R)c.p2.y, (R)c.p2.z);
+    cr.depth = (R)c.depth;
+    store_pair(o, P, i, cr, ra.body, rb.body);
   }
 }
 
@@ -1076,25 +1093,33 @@ template <class R> class Scene final : public SceneBase {
     np_ids_.reserve(std::max<uint64_t>(P, 1) * 8);
     np_hit_.reserve(std::max<uint64_t>(P, 1) * 4);
     np_pos_.reserve(std::max<uint64_t>(P, 1) * 4);
-    queue_.reserve(std::max<uint64_t>(P, 1) * 4);
-    PairOut<R> o{np_v_.as<R>(), np_ids_.as<int32_t>(), np_hit_.as<uint32_t>()};
+    queue_.reserve(std::max<uint64_t>(P, 1) * 8);
+    queue_tmp_.reserve(std::max<uint64_t>(P, 1) * 8);
+    PairOut<R> o{np_v_.as<R>(), np_ids_.as<int32_t>(), np_hit_.as<uint32_t>(), P};
     hipLaunchKernelGGL(k_zero_np, dim3(1), dim3(64), 0, s_, st);
     mark(ST_NP_MAIN, true);
-    if (P) {
-      if (mode_ == NP_NATIVE)
-        hipLaunchKernelGGL(k_np_native<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P,
-                           (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, o);
-      else
-        hipLaunchKernelGGL(k_np_mixed<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P,
-                           (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
-                           tau_, mode_ == NP_FAST ? 1 : 0, o, queue_.as<uint32_t>(), st);
-    }
+    if (P)
+      hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P,
+                         (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
+                         tau_, mode_, o, queue_.as<uint64_t>(), st);
     mark(ST_NP_MAIN, false);
+    read_status();
+    const uint64_t q = hstat_->Q;
     mark(ST_NP_RESOLVE, true);
-    if (P && mode_ != NP_NATIVE)
-      hipLaunchKernelGGL(k_np_resolve<R>, dim3(std::min<unsigned>(blocks_for(P, 128), 2048u)), dim3(128), 0, s_, P,
-                         (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, o,
-                         (const uint32_t*)queue_.as<uint32_t>(), (const DevStatus*)st);
+    if (q) {
+      // group the queue by shape-pair class (one 8-bit digit, stable)
+      const int cls_shift[1] = {32};
+      radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(q), 16) * 4);
+      radix_sort_u64(queue_.as<uint64_t>(), queue_tmp_.as<uint64_t>(), q, cls_shift, 1, radix_tmp_.as<uint32_t>(), s_);
+      if (mode_ == NP_NATIVE)
+        hipLaunchKernelGGL((k_np_resolve<R, R>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q,
+                           (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
+                           o, (const uint64_t*)queue_.as<uint64_t>());
+      else
+        hipLaunchKernelGGL((k_np_resolve<R, double>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q,
+                           (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
+                           o, (const uint64_t*)queue_.as<uint64_t>());
+    }
     mark(ST_NP_RESOLVE, false);
     mark(ST_NP_COMPACT, true);
     scan_excl_u32(np_hit_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st->C,
@@ -1114,7 +1139,7 @@ template <class R> class Scene final : public SceneBase {
                          out_ids_.as<int32_t>(), perm_.as<uint64_t>());
     mark(ST_NP_COMPACT, false);
     counts_.c = C_;
-    counts_.resolved = hstat_->Q;
+    counts_.resolved = mode_ == NP_NATIVE ? 0 : q;
     counts_.p = P;
   }
 
@@ -1186,7 +1211,7 @@ template <class R> class Scene final : public SceneBase {
   uint64_t device_bytes() const override {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_, &brec_,
                            &counts_buf_, &offs_, &keys_, &keys_tmp_, &flags_, &runidx_, &run_start_, &run_key_,
-                           &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
+                           &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_, &queue_tmp_,
                            &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_, &scan_tmp_, &radix_tmp_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
@@ -1255,7 +1280,7 @@ template <class R> class Scene final : public SceneBase {
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, flags_, runidx_, run_start_, run_key_, pcount_, poffs_, pairs_, pairs_tmp_;
-  DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_;
+  DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
   DevBuf scan_tmp_, radix_tmp_;
 };
