@@ -67,4 +67,16 @@ void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 
+// Sorts each segment [off[s], off[s+1]) of data ascending, for s < nseg.
+// Segments of up to 32 elements are insertion-sorted by one thread; longer
+// ones are queued and sorted by a workgroup (bitonic network in LDS up to
+// 4096 elements, in global memory beyond).  If nonempty is not null, the
+// number of non-empty segments is added to it.  tmp holds
+// segsort_tmp_words(nseg) uint32.
+size_t segsort_tmp_words(uint64_t nseg);
+void segsort_u32(uint32_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
+                 hipStream_t s);
+void segsort_u64(uint64_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
+                 hipStream_t s);
+
 }  // namespace cdx

@@ -165,7 +165,116 @@ __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __res
   }
 }
 
+
+constexpr int SEG_SMALL = 32;
+constexpr int SEG_LDS = 4096;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_seg_small(T* __restrict__ data, const uint32_t* __restrict__ off,
+                                                   uint64_t nseg, uint32_t* __restrict__ tmp,
+                                                   unsigned long long* nonempty) {
+  uint64_t s = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  bool ne = false;
+  if (s < nseg) {
+    uint32_t b = off[s], e = off[s + 1];
+    ne = e > b;
+    if (e - b > (uint32_t)SEG_SMALL) {
+      uint32_t q = atomicAdd(&tmp[0], 1u);
+      tmp[1 + q] = (uint32_t)s;
+    } else {
+      for (uint32_t i = b + 1; i < e; ++i) {
+        T v = data[i];
+        uint32_t j = i;
+        while (j > b && data[j - 1] > v) {
+          data[j] = data[j - 1];
+          --j;
+        }
+        data[j] = v;
+      }
+    }
+  }
+  if (nonempty) {
+    uint64_t m = __ballot(ne);
+    if (m && __lane_id() == (uint32_t)(__ffsll((long long)m) - 1)) atomicAdd(nonempty, (unsigned long long)__popcll(m));
+  }
+}
+
+// Ascending bitonic network for any n: a mirrored compare at each block size,
+// then half-cleaners.  Indices >= n act as +infinity and are never swapped.
+template <class T>
+__device__ __forceinline__ void bitonic_block(T* x, uint32_t n) {
+  for (uint32_t p = 1; p < n; p <<= 1) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      if (!(i & p)) {
+        uint32_t j = i ^ (2 * p - 1);
+        if (j < n) {
+          T a = x[i], b = x[j];
+          if (b < a) { x[i] = b; x[j] = a; }
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t q = p >> 1; q > 0; q >>= 1) {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        if (!(i & q)) {
+          uint32_t j = i | q;
+          if (j < n) {
+            T a = x[i], b = x[j];
+            if (b < a) { x[i] = b; x[j] = a; }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_seg_large(T* __restrict__ data, const uint32_t* __restrict__ off,
+                                                   const uint32_t* __restrict__ tmp) {
+  __shared__ T lds[SEG_LDS];
+  const uint32_t nq = tmp[0];
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    uint32_t sid = tmp[1 + q];
+    uint32_t b = off[sid], n = off[sid + 1] - b;
+    if (n <= (uint32_t)SEG_LDS) {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = data[b + i];
+      __syncthreads();
+      bitonic_block(lds, n);
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) data[b + i] = lds[i];
+    } else {
+      bitonic_block(data + b, n);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_word(uint32_t* p) {
+  if (threadIdx.x == 0) p[0] = 0;
+}
+
+template <class T>
+void segsort_impl(T* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty, hipStream_t s) {
+  if (nseg == 0) return;
+  hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, s, tmp);
+  uint64_t blocks = (nseg + 255) / 256;
+  hipLaunchKernelGGL((k_seg_small<T>), dim3((uint32_t)blocks), dim3(256), 0, s, data, off, nseg, tmp,
+                     (unsigned long long*)nonempty);
+  hipLaunchKernelGGL((k_seg_large<T>), dim3(512), dim3(256), 0, s, data, off, (const uint32_t*)tmp);
+}
+
 }  // namespace
+
+size_t segsort_tmp_words(uint64_t nseg) { return nseg + 2; }
+
+void segsort_u32(uint32_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
+                 hipStream_t s) {
+  segsort_impl(data, off, nseg, tmp, nonempty, s);
+}
+void segsort_u64(uint64_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
+                 hipStream_t s) {
+  segsort_impl(data, off, nseg, tmp, nonempty, s);
+}
 
 size_t scan_tmp_words(uint64_t n) {
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
