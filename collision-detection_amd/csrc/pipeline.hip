@@ -5,12 +5,12 @@
 //   AABB         AABBGenerator.cu:84-130   -> k_aabb
 //   bounds       Broadphase.cu:368-381     -> k_aabb (fused) / k_bounds, k_params
 //   count        Broadphase.cu:150-166     -> k_count
-//   scan #1      Broadphase.cu:403-407     -> scan_excl_u32 over the bin histogram
-//   key store    Broadphase.cu:172-209     -> k_emit_bins (counting sort into the dense bin table)
-//   sort_by_key  Broadphase.cu:428-432     -> segsort_u32 inside each bin
-//   reduce_by_key + scan #2  :439-480      -> the dense table's starts; active bins counted in segsort
+//   scan #1      Broadphase.cu:403-407     -> scan_excl_u32
+//   key store    Broadphase.cu:172-209     -> k_emit_keys
+//   sort_by_key  Broadphase.cu:428-432     -> radix_sort_u64 on (key<<32 | body)
+//   reduce_by_key + scan #2  :439-480      -> k_run_flags, k_run_write
 //   pair count / store / sort / unique :484-535
-//                                          -> k_pairs (owner-bin dedup), k_pairs_scatter + segsort_u64
+//                                          -> k_pairs (owner-bin dedup), radix_sort_u64
 //   MPR store    Narrowphase.cu:316-504    -> k_np_front + class sort + k_np_resolve
 //   compaction   Narrowphase.cu:580-593    -> scan + k_np_compact
 #include <algorithm>
@@ -28,7 +28,7 @@ namespace {
 struct DevStatus {
   uint32_t bmin_enc[3], bmax_enc[3];
   uint32_t bad, key_max;
-  uint64_t M, A, P, C, Q, scratch;
+  uint64_t M, A, P, C, Q;
 };
 
 template <class R> struct Params { R origin[3]; R inv[3]; };
@@ -58,7 +58,7 @@ __global__ void k_init_status(DevStatus* st) {
   if (threadIdx.x == 0) {
     for (int c = 0; c < 3; ++c) { st->bmin_enc[c] = 0xffffffffu; st->bmax_enc[c] = 0u; }
     st->bad = 0; st->key_max = 0;
-    st->M = st->A = st->P = st->C = st->Q = st->scratch = 0;
+    st->M = st->A = st->P = st->C = st->Q = 0;
   }
 }
 
@@ -216,8 +216,7 @@ template <class R> struct alignas(16) BpRec {
 template <class R, int BS>
 __global__ __launch_bounds__(BS) void k_count(uint64_t n, const R* __restrict__ aabb, const Params<R>* __restrict__ prm,
                                                int gx, int gy, int gz, uint32_t* __restrict__ counts,
-                                               BpRec<R>* __restrict__ brec, uint32_t* __restrict__ hist,
-                                               DevStatus* st) {
+                                               BpRec<R>* __restrict__ brec, DevStatus* st) {
   __shared__ uint32_t skm[BS / WAVE];
   uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x;
   uint32_t km = 0;
@@ -236,11 +235,6 @@ __global__ __launch_bounds__(BS) void k_count(uint64_t n, const R* __restrict__ 
     if (cells_ok(c, gx, gy, gz)) {
       counts[i] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, gx, gy);
-      // occupancy histogram of the dense bin table (counting sort by bin)
-      for (uint32_t x = (uint32_t)c.x0; x <= (uint32_t)c.x1; ++x)
-        for (uint32_t y = (uint32_t)c.y0; y <= (uint32_t)c.y1; ++y)
-          for (uint32_t z = (uint32_t)c.z0; z <= (uint32_t)c.z1; ++z)
-            atomicAdd(&hist[hash_index(x, y, z, gx, gy)], 1u);
     } else {
       counts[i] = 0;
       atomicOr(&st->bad, 4u);
@@ -256,81 +250,101 @@ __global__ __launch_bounds__(BS) void k_count(uint64_t n, const R* __restrict__ 
   }
 }
 
-// Counting sort of the (bin, body) entries (Broadphase.cu:172-200 store
-// + :428-432 sort_by_key): each entry lands in its bin's slice of the dense
-// bin table (start = exclusive scan of the occupancy histogram); the slot
-// inside the slice comes from an atomic countdown of the histogram, which
-// leaves it zero for the next step.  A segmented sort then restores body
-// order inside each bin, so the result equals the reference's stable
-// sort_by_key of the body-ordered entries.
+// (bin, body) entries in the reference's store order (Broadphase.cu:
+// 172-200), packed as key << 32 | body so one 64-bit sort orders by bin and
+// keeps bodies ascending inside a bin.
 template <class R, int BS>
-__global__ __launch_bounds__(BS) void k_emit_bins(uint64_t n, const BpRec<R>* __restrict__ brec,
+__global__ __launch_bounds__(BS) void k_emit_keys(uint64_t n, const R* __restrict__ aabb,
                                                    const Params<R>* __restrict__ prm, int gx, int gy, int gz,
-                                                   const uint32_t* __restrict__ start, uint32_t* __restrict__ hist,
-                                                   uint32_t* __restrict__ sbody, uint32_t* __restrict__ skey) {
+                                                   const uint32_t* __restrict__ offs, uint64_t* __restrict__ keys) {
   uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  const BpRec<R> r = brec[i];
-  const Params<R> p = *prm;
-  Cells<R> c;
-  c.x0 = r.g[0]; c.y0 = r.g[1]; c.z0 = r.g[2];
-  c.x1 = cell_of(r.hi[0], p.inv[0]);
-  c.y1 = cell_of(r.hi[1], p.inv[1]);
-  c.z1 = cell_of(r.hi[2], p.inv[2]);
+  Params<R> p = *prm;
+  Cells<R> c = cells_of(aabb, n, i, p);
   if (!cells_ok(c, gx, gy, gz)) return;
+  uint64_t o = offs[i];
   for (uint32_t x = (uint32_t)c.x0; x <= (uint32_t)c.x1; ++x)
     for (uint32_t y = (uint32_t)c.y0; y <= (uint32_t)c.y1; ++y)
-      for (uint32_t z = (uint32_t)c.z0; z <= (uint32_t)c.z1; ++z) {
-        uint32_t k = hash_index(x, y, z, gx, gy);
-        uint32_t slot = start[k] + atomicSub(&hist[k], 1u) - 1u;
-        sbody[slot] = (uint32_t)i;
-        skey[slot] = k;
-      }
+      for (uint32_t z = (uint32_t)c.z0; z <= (uint32_t)c.z1; ++z)
+        keys[o++] = ((uint64_t)hash_index(x, y, z, gx, gy) << 32) | (uint64_t)i;
+}
+
+__global__ void k_run_flags(uint64_t m, const uint64_t* __restrict__ keys, uint32_t* __restrict__ flags) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  flags[i] = (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) ? 1u : 0u;
+}
+
+// Active bins (reduce_by_key, Broadphase.cu:439-443) as run starts and keys.
+__global__ void k_run_write(uint64_t m, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ runidx,
+                            uint32_t* __restrict__ run_start, uint32_t* __restrict__ run_key,
+                            const DevStatus* __restrict__ st) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) run_start[st->A] = (uint32_t)m;
+  if (i >= m) return;
+  if (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) {
+    uint32_t r = runidx[i];
+    run_start[r] = (uint32_t)i;
+    run_key[r] = (uint32_t)(keys[i] >> 32);
+  }
 }
 
 // Pair search with de-duplication by owning bin.  A pair (a, b) of
 // overlapping AABBs (closed test on shifted coordinates, Broadphase.cu:237 /
 // :296) is reported only from the bin whose key is the hash of the cell
 // max(gmin_a, gmin_b): that cell lies in both AABBs' cell ranges, so both
-// bodies appear in its slice of sorted entries, and every other slice skips
-// the pair.  Repeated entries of one body in a slice (hash aliasing at index
-// == grid size, Broadphase.cu:140-147) are skipped, so each pair is emitted
-// once and the reference's sort + unique (:522-535) reduces to a sort.
+// bodies appear in its run of sorted entries, and every other run skips the
+// pair.  Repeated entries of one body in a run (hash aliasing at index ==
+// grid size, Broadphase.cu:140-147) are skipped, so each pair is emitted once
+// and the reference's sort + unique (:522-535) reduces to a sort.
 //
-// One thread per sorted entry i pairs body(i) with the later entries of the
-// same bin (bodies ascend inside a bin, so body(i) < body(k)).  Pairs are
-// appended to a staging list (one atomic per wavefront) and counted per
-// smaller body, for the counting sort that orders the final list.
-template <class R>
-__global__ __launch_bounds__(256) void k_pairs(uint64_t m, const uint32_t* __restrict__ sbody,
-                                               const uint32_t* __restrict__ skey, const BpRec<R>* __restrict__ brec,
-                                               int gx, int gy, uint64_t cap, uint64_t* __restrict__ stage,
-                                               uint32_t* __restrict__ acount, DevStatus* st) {
-  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  uint32_t cnt = 0;
-  uint32_t a = 0;
-  uint64_t buf[4];
-  uint32_t nbuf = 0;
-  bool active = i < m;
-  uint32_t K = 0;
-  BpRec<R> ra;
-  uint64_t k = i + 1;
-  if (active) {
-    K = skey[i];
-    a = sbody[i];
-    if (i > 0 && skey[i - 1] == K && sbody[i - 1] == a) active = false;
-    else ra = brec[a];
+// One thread per sorted entry i: it pairs body(i) with the later entries of
+// the same run (bodies ascend inside a run, so body(i) < body(k)).  Work
+// per thread is the distance to the end of its run, which keeps waves far
+// better balanced than a thread per bin.
+constexpr int PAIR_BS = 256;   // sorted entries per workgroup
+constexpr int PAIR_AHEAD = 128; // entries staged past the workgroup's end
+
+template <class R, bool EMIT>
+__global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t m, const uint64_t* __restrict__ keys,
+                                                   const BpRec<R>* __restrict__ brec, int gx, int gy,
+                                                   uint32_t* __restrict__ pcount, const uint32_t* __restrict__ poffs,
+                                                   uint64_t* __restrict__ pairs) {
+  // The workgroup's entries and the first PAIR_AHEAD after them are staged
+  // in LDS with their body records, so the candidate loop reads LDS; only
+  // bins reaching past the staged window fall back to global loads.
+  constexpr int W = PAIR_BS + PAIR_AHEAD;
+  __shared__ BpRec<R> srec[W];
+  __shared__ uint64_t skey[W];
+  const uint64_t b0 = (uint64_t)blockIdx.x * PAIR_BS;
+  for (int t = threadIdx.x; t < W; t += PAIR_BS) {
+    uint64_t e = b0 + t;
+    if (e < m) {
+      uint64_t kk = keys[e];
+      skey[t] = kk;
+      srec[t] = brec[(uint32_t)kk];
+    }
   }
-  uint32_t prev = a;
-  // Each round every lane scans until it has up to 4 pairs or its bin ends;
-  // then the wavefront appends the found pairs with one atomic.
-  while (__ballot(active)) {
-    while (active && nbuf < 4) {
-      if (k >= m || skey[k] != K) { active = false; break; }
-      uint32_t b = sbody[k++];
+  __syncthreads();
+  const uint64_t i = b0 + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t ki = skey[threadIdx.x];
+  uint32_t cnt = 0;
+  const uint64_t kprev = i == 0 ? ~ki : (threadIdx.x > 0 ? skey[threadIdx.x - 1] : keys[i - 1]);
+  if (kprev != ki) {
+    const uint32_t K = (uint32_t)(ki >> 32), a = (uint32_t)ki;
+    const BpRec<R> ra = srec[threadIdx.x];
+    uint64_t o = EMIT ? poffs[i] : 0;
+    uint32_t prev = a;
+    for (uint64_t k = i + 1; k < m; ++k) {
+      const uint64_t t = k - b0;
+      const bool staged = t < (uint64_t)W;
+      const uint64_t kk = staged ? skey[t] : keys[k];
+      if ((uint32_t)(kk >> 32) != K) break;
+      uint32_t b = (uint32_t)kk;
       if (b == prev) continue;
       prev = b;
-      const BpRec<R> rb = brec[b];
+      const BpRec<R> rb = staged ? srec[t] : brec[b];
       bool hit = (ra.lo[0] <= rb.hi[0] && rb.lo[0] <= ra.hi[0]) && (ra.lo[1] <= rb.hi[1] && rb.lo[1] <= ra.hi[1]) &&
                  (ra.lo[2] <= rb.hi[2] && rb.lo[2] <= ra.hi[2]);
       if (!hit) continue;
@@ -338,38 +352,11 @@ __global__ __launch_bounds__(256) void k_pairs(uint64_t m, const uint32_t* __res
       uint32_t oy = (uint32_t)max(ra.g[1], rb.g[1]);
       uint32_t oz = (uint32_t)max(ra.g[2], rb.g[2]);
       if (hash_index(ox, oy, oz, gx, gy) != K) continue;
-      buf[nbuf++] = ((uint64_t)a << 32) | (uint64_t)b;
+      if (EMIT) pairs[o++] = ((uint64_t)a << 32) | (uint64_t)b;
+      else ++cnt;
     }
-    // wave-aggregated append of nbuf pairs per lane
-    uint32_t incl = nbuf;
-    for (int o = 1; o < WAVE; o <<= 1) {
-      uint32_t t = __shfl_up(incl, o);
-      if ((int)__lane_id() >= o) incl += t;
-    }
-    uint32_t total = __shfl(incl, WAVE - 1);
-    if (total) {
-      uint64_t base = 0;
-      if (__lane_id() == 0) base = atomicAdd((unsigned long long*)&st->P, (unsigned long long)total);
-      base = __shfl(base, 0) + (incl - nbuf);
-      for (uint32_t j = 0; j < nbuf; ++j)
-        if (base + j < cap) stage[base + j] = buf[j];
-    }
-    cnt += nbuf;
-    nbuf = 0;
   }
-  if (cnt) atomicAdd(&acount[a], cnt);
-}
-
-// Counting sort of the staged pairs by their smaller body: slot =
-// start of the body's slice + atomic countdown (leaves acount zero).
-__global__ void k_pairs_scatter(uint64_t p, const uint64_t* __restrict__ stage, const uint32_t* __restrict__ aoff,
-                                uint32_t* __restrict__ acount, uint64_t* __restrict__ pairs) {
-  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= p) return;
-  uint64_t v = stage[j];
-  uint32_t a = (uint32_t)(v >> 32);
-  uint32_t slot = aoff[a] + atomicSub(&acount[a], 1u) - 1u;
-  pairs[slot] = v;
+  if (!EMIT) pcount[i] = cnt;
 }
 
 // ----------------------------------------------------------- narrowphase --
@@ -706,11 +693,11 @@ template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(
 // MIXED / FAST first run an FP32 MPR in the frame of shape A: clear misses
 // end here, FAST keeps clear FP32 hits, everything else is queued for the
 // reference-order MPR.
-template <class R>
+template <class R, int MODE>
 __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __restrict__ pairs,
                                                   const ShapeRec<R>* __restrict__ rec, R envelope, float tau_rel,
-                                                  int mode, PairOut<R> o, uint64_t* __restrict__ queue,
-                                                  DevStatus* st) {
+                                                  PairOut<R> o, uint64_t* __restrict__ queue, DevStatus* st) {
+  constexpr int mode = MODE;
   uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
   bool enq = false;
   uint32_t cls = 0;
@@ -823,10 +810,6 @@ __global__ void k_np_compact(uint64_t P, PairOut<R> o, const uint32_t* __restric
 
 __global__ void k_zero_np(DevStatus* st) {
   if (threadIdx.x == 0) { st->Q = 0; st->C = 0; }
-}
-
-__global__ void k_zero_p(DevStatus* st) {
-  if (threadIdx.x == 0) st->P = 0;
 }
 
 template <class R>
@@ -1024,12 +1007,6 @@ template <class R> class Scene final : public SceneBase {
       counts_.n = 0; counts_.m = counts_.a = counts_.p = 0;
       return;
     }
-    // Dense bin table: keys are Hash_Index values, at most
-    // gx*(gy*gz + gy + 1) with max-corner aliasing (Broadphase.cu:140-147).
-    const uint64_t ks = (uint64_t)g_[0] * ((uint64_t)g_[1] * g_[2] + g_[1] + 1) + 1;
-    CDX_REQUIRE(ks <= (1ull << 27), ERR_INVALID,
-                "bins per axis too fine: the dense bin table is limited to 2^27 keys");
-    ks_ = ks;
     mark(ST_BP_BOUNDS, true);
     if (!bounds_fused_) {
       hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, s_, st);
@@ -1041,68 +1018,81 @@ template <class R> class Scene final : public SceneBase {
     mark(ST_BP_BOUNDS, false);
     bounds_fused_ = false;  // a second broadphase on the same AABBs recomputes bounds
 
-    // count: per-body bins spanned, body records, occupancy histogram
+    // count + scan (host reads M back, as the reference reads .back())
     counts_buf_.reserve(n * 4);
+    offs_.reserve(n * 4);
     brec_.reserve(n * sizeof(BpRec<R>));
-    hist_.reserve(ks * 4);
-    start_.reserve((ks + 1) * 4);
-    scan_tmp_.reserve(std::max<size_t>(std::max(scan_tmp_words(ks + 1), scan_tmp_words(n + 1)), 16) * 4);
-    seg_tmp_.reserve(std::max<size_t>(segsort_tmp_words(std::max(ks, n)), 16) * 4);
+    scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(n), 16) * 4);
     mark(ST_BP_COUNT, true);
-    fill_u32(hist_.as<uint32_t>(), 0u, ks + 1, s_);
     hipLaunchKernelGGL((k_count<R, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s_, n, (const R*)aabb_.as<R>(),
                        (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1], g_[2], counts_buf_.as<uint32_t>(),
-                       brec_.as<BpRec<R>>(), hist_.as<uint32_t>(), st);
+                       brec_.as<BpRec<R>>(), st);
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
-    scan_excl_u32(hist_.as<uint32_t>(), start_.as<uint32_t>(), ks + 1, &st->M, scan_tmp_.as<uint32_t>(), s_);
+    scan_excl_u32(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), n, &st->M, scan_tmp_.as<uint32_t>(), s_);
     mark(ST_BP_SCAN, false);
     read_status();
     M_ = hstat_->M;
     CDX_REQUIRE(M_ < (1ull << 32), ERR_CAPACITY, "more than 2^32 bin entries: use fewer bins per axis");
 
-    // scatter entries into their bins, then order bodies inside each bin
-    sbody_.reserve(M_ * 4);
-    skey_.reserve(M_ * 4);
+    // key emit + sort by bin
+    keys_.reserve(M_ * 8);
+    keys_tmp_.reserve(M_ * 8);
     mark(ST_BP_EMIT, true);
-    hipLaunchKernelGGL((k_emit_bins<R, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s_, n,
-                       (const BpRec<R>*)brec_.as<BpRec<R>>(), (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1],
-                       g_[2], (const uint32_t*)start_.as<uint32_t>(), hist_.as<uint32_t>(), sbody_.as<uint32_t>(),
-                       skey_.as<uint32_t>());
+    hipLaunchKernelGGL((k_emit_keys<R, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s_, n, (const R*)aabb_.as<R>(),
+                       (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1], g_[2], (const uint32_t*)offs_.as<uint32_t>(),
+                       keys_.as<uint64_t>());
     mark(ST_BP_EMIT, false);
+    uint64_t key_space = (uint64_t)g_[0] * ((uint64_t)g_[1] * g_[2] + g_[1] + 1) + 1;
+    int key_bits = key_space > 0xffffffffull ? 32 : bits_for(hstat_->key_max);
+    std::vector<int> sh;
+    for (int b = 0; b < key_bits; b += 8) sh.push_back(32 + b);
+    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(M_), scan_tmp_words(M_) + 16) * 4);
     mark(ST_BP_SORT, true);
-    segsort_u32(sbody_.as<uint32_t>(), start_.as<uint32_t>(), ks, seg_tmp_.as<uint32_t>(), &st->A, s_);
+    radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), M_, sh.data(), (int)sh.size(),
+                   radix_tmp_.as<uint32_t>(), s_);
     mark(ST_BP_SORT, false);
 
-    // pair search into a staging list, then counting sort by smaller body
-    acount_.reserve((n + 1) * 4);
-    aoff_.reserve((n + 1) * 4);
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      if (stage_cap_ == 0) stage_cap_ = std::max<uint64_t>(4 * n, 1024);
-      pstage_.reserve(stage_cap_ * 8);
-      mark(ST_BP_PAIRS, true);
-      fill_u32(acount_.as<uint32_t>(), 0u, n + 1, s_);
-      hipLaunchKernelGGL(k_zero_p, dim3(1), dim3(64), 0, s_, st);
-      hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_,
-                         (const uint32_t*)sbody_.as<uint32_t>(), (const uint32_t*)skey_.as<uint32_t>(),
-                         (const BpRec<R>*)brec_.as<BpRec<R>>(), g_[0], g_[1], stage_cap_, pstage_.as<uint64_t>(),
-                         acount_.as<uint32_t>(), st);
-      mark(ST_BP_PAIRS, false);
-      read_status();
-      P_ = hstat_->P;
-      if (P_ <= stage_cap_) break;
-      stage_cap_ = P_ + P_ / 4 + 1024;  // grow and search again
-    }
+    // active bins
+    flags_.reserve(M_ * 4);
+    runidx_.reserve(M_ * 4);
+    run_start_.reserve((M_ + 1) * 4);
+    run_key_.reserve(M_ * 4);
+    mark(ST_BP_RUNS, true);
+    hipLaunchKernelGGL(k_run_flags, dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_, (const uint64_t*)keys_.as<uint64_t>(),
+                       flags_.as<uint32_t>());
+    scan_excl_u32(flags_.as<uint32_t>(), runidx_.as<uint32_t>(), M_, &st->A, radix_tmp_.as<uint32_t>(), s_);
+    hipLaunchKernelGGL(k_run_write, dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_,
+                       (const uint64_t*)keys_.as<uint64_t>(), (const uint32_t*)runidx_.as<uint32_t>(),
+                       run_start_.as<uint32_t>(), run_key_.as<uint32_t>(), (const DevStatus*)st);
+    mark(ST_BP_RUNS, false);
+
+    // pair count (per sorted entry), scan, emit
+    pcount_.reserve((M_ + 1) * 4);
+    poffs_.reserve((M_ + 1) * 4);
+    mark(ST_BP_PAIRS, true);
+    hipLaunchKernelGGL((k_pairs<R, false>), dim3(blocks_for(M_, PAIR_BS)), dim3(PAIR_BS), 0, s_, M_,
+                       (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(), g_[0], g_[1],
+                       pcount_.as<uint32_t>(), (const uint32_t*)nullptr, (uint64_t*)nullptr);
+    scan_excl_u32(pcount_.as<uint32_t>(), poffs_.as<uint32_t>(), M_, &st->P, radix_tmp_.as<uint32_t>(), s_);
+    read_status();
+    P_ = hstat_->P;
     A_ = hstat_->A;
     CDX_REQUIRE(P_ < (1ull << 32), ERR_CAPACITY, "more than 2^32 candidate pairs");
-    pairs_.reserve(std::max<uint64_t>(P_, 1) * 8);
+    pairs_.reserve(P_ * 8);
+    pairs_tmp_.reserve(P_ * 8);
+    hipLaunchKernelGGL((k_pairs<R, true>), dim3(blocks_for(M_, PAIR_BS)), dim3(PAIR_BS), 0, s_, M_,
+                       (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(), g_[0], g_[1],
+                       (uint32_t*)nullptr, (const uint32_t*)poffs_.as<uint32_t>(), pairs_.as<uint64_t>());
+    mark(ST_BP_PAIRS, false);
+    int nbits = bits_for((uint32_t)(n - 1));
+    std::vector<int> ps;
+    for (int b = 0; b < nbits; b += 8) ps.push_back(b);
+    for (int b = 0; b < nbits; b += 8) ps.push_back(32 + b);
+    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(P_), 16) * 4);
     mark(ST_BP_PAIRSORT, true);
-    scan_excl_u32(acount_.as<uint32_t>(), aoff_.as<uint32_t>(), n + 1, &st->scratch, scan_tmp_.as<uint32_t>(), s_);
-    if (P_)
-      hipLaunchKernelGGL(k_pairs_scatter, dim3(blocks_for(P_, 256)), dim3(256), 0, s_, P_,
-                         (const uint64_t*)pstage_.as<uint64_t>(), (const uint32_t*)aoff_.as<uint32_t>(),
-                         acount_.as<uint32_t>(), pairs_.as<uint64_t>());
-    segsort_u64(pairs_.as<uint64_t>(), aoff_.as<uint32_t>(), n, seg_tmp_.as<uint32_t>(), nullptr, s_);
+    radix_sort_u64(pairs_.as<uint64_t>(), pairs_tmp_.as<uint64_t>(), P_, ps.data(), (int)ps.size(),
+                   radix_tmp_.as<uint32_t>(), s_);
     mark(ST_BP_PAIRSORT, false);
     counts_.n = n; counts_.m = M_; counts_.a = A_; counts_.p = P_;
     counts_.key_max = hstat_->key_max;
@@ -1131,9 +1121,15 @@ template <class R> class Scene final : public SceneBase {
     hipLaunchKernelGGL(k_zero_np, dim3(1), dim3(64), 0, s_, st);
     mark(ST_NP_MAIN, true);
     if (P)
-      hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P,
-                         (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
-                         tau_, mode_, o, queue_.as<uint64_t>(), st);
+    {
+      auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, (const uint64_t*)pairs_.as<uint64_t>(),
+                           (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, tau_, o, queue_.as<uint64_t>(), st);
+      };
+      if (mode_ == NP_NATIVE) launch(k_np_front<R, NP_NATIVE>);
+      else if (mode_ == NP_MIXED) launch(k_np_front<R, NP_MIXED>);
+      else launch(k_np_front<R, NP_FAST>);
+    }
     mark(ST_NP_MAIN, false);
     read_status();
     const uint64_t q = hstat_->Q;
@@ -1195,19 +1191,20 @@ template <class R> class Scene final : public SceneBase {
                  uint32_t* active_counts) override {
     sync();
     if (bin_counts && n_) CDX_HIP_CHECK(hipMemcpy(bin_counts, counts_buf_.p, n_ * 4, hipMemcpyDeviceToHost));
-    if (sorted_keys && M_) CDX_HIP_CHECK(hipMemcpy(sorted_keys, skey_.p, M_ * 4, hipMemcpyDeviceToHost));
-    if (sorted_bodies && M_) CDX_HIP_CHECK(hipMemcpy(sorted_bodies, sbody_.p, M_ * 4, hipMemcpyDeviceToHost));
-    if ((active_keys || active_counts) && A_) {
-      std::vector<uint32_t> st(ks_ + 1);
-      CDX_HIP_CHECK(hipMemcpy(st.data(), start_.p, (ks_ + 1) * 4, hipMemcpyDeviceToHost));
-      uint64_t r = 0;
-      for (uint64_t k = 0; k < ks_ && r < A_; ++k) {
-        uint32_t c = st[k + 1] - st[k];
-        if (!c) continue;
-        if (active_keys) active_keys[r] = (uint32_t)k;
-        if (active_counts) active_counts[r] = c;
-        ++r;
+    if ((sorted_keys || sorted_bodies) && M_) {
+      std::vector<uint64_t> k(M_);
+      CDX_HIP_CHECK(hipMemcpy(k.data(), keys_.p, M_ * 8, hipMemcpyDeviceToHost));
+      for (uint64_t i = 0; i < M_; ++i) {
+        if (sorted_keys) sorted_keys[i] = (uint32_t)(k[i] >> 32);
+        if (sorted_bodies) sorted_bodies[i] = (uint32_t)k[i];
       }
+    }
+    if ((active_keys || active_counts) && A_) {
+      std::vector<uint32_t> st(A_ + 1);
+      CDX_HIP_CHECK(hipMemcpy(st.data(), run_start_.p, (A_ + 1) * 4, hipMemcpyDeviceToHost));
+      if (active_keys) CDX_HIP_CHECK(hipMemcpy(active_keys, run_key_.p, A_ * 4, hipMemcpyDeviceToHost));
+      if (active_counts)
+        for (uint64_t r = 0; r < A_; ++r) active_counts[r] = st[r + 1] - st[r];
     }
   }
 
@@ -1241,8 +1238,8 @@ template <class R> class Scene final : public SceneBase {
 
   uint64_t device_bytes() const override {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_, &brec_,
-                           &counts_buf_, &hist_, &start_, &sbody_, &skey_, &acount_, &aoff_, &pstage_, &pairs_,
-                           &seg_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_, &queue_tmp_,
+                           &counts_buf_, &offs_, &keys_, &keys_tmp_, &flags_, &runidx_, &run_start_, &run_key_,
+                           &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_, &queue_tmp_,
                            &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_, &scan_tmp_, &radix_tmp_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
@@ -1310,8 +1307,7 @@ template <class R> class Scene final : public SceneBase {
   DevBuf status_, params_, stage_, upd_, tmp_out_;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_;
-  DevBuf counts_buf_, hist_, start_, sbody_, skey_, acount_, aoff_, pstage_, pairs_, seg_tmp_;
-  uint64_t ks_ = 0, stage_cap_ = 0;
+  DevBuf counts_buf_, offs_, keys_, keys_tmp_, flags_, runidx_, run_start_, run_key_, pcount_, poffs_, pairs_, pairs_tmp_;
   DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
   DevBuf scan_tmp_, radix_tmp_;
