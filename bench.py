@@ -90,6 +90,62 @@ def cpu_baseline(oracle, s, min_seconds=10.0):
     }
 
 
+BP_STAGES = ["bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs", "bp_pairsort"]
+
+
+def lattice_leg(pkg, scenes, ctx, side, steps):
+    """16,777,216-sphere FP32 granular pack (side 256): AABB + broadphase, then the full step.
+
+    HBM fraction uses the broadphase's algorithmic bytes
+    N*(24+4+8+28) + M*(8+16+4+24) + 8A + 8P (count, scan, key emit; key sort,
+    bin starts, pair emit) over the summed broadphase stage time.
+    """
+    log(f"lattice leg: generating {side}^3 spheres")
+    s = scenes.lattice_spheres(side=side)
+    sc = pkg.Scene(ctx, np.float32).load(s)
+    sc.set_profiling(True)
+    sc.aabb()
+    sc.broadphase()
+    sc.sync()
+    c = sc.counts()
+    tot = {k: 0.0 for k in pkg.STAGES}
+    wall = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        sc.aabb()
+        sc.broadphase()
+        sc.sync()
+        wall.append(time.perf_counter() - t0)
+        for k, v in sc.stage_ms().items():
+            tot[k] += v
+    avg = {k: v / steps for k, v in tot.items()}
+    bp_ms = sum(avg[k] for k in BP_STAGES)
+    n, m, a, p = s.n, c.bin_entries, c.active_bins, c.pairs
+    algo = n * (24 + 4 + 8 + 28) + m * (8 + 16 + 4 + 24) + 8 * a + 8 * p
+    gbs = algo / (bp_ms * 1e-3) / 1e9
+    sc.set_mode(pkg.NP_NATIVE)
+    t0 = time.perf_counter()
+    sc.step()
+    sc.sync()
+    step_ms = (time.perf_counter() - t0) * 1e3
+    np_ms = sum(v for k, v in sc.stage_ms().items() if k.startswith("np_"))
+    cc = sc.counts()
+    out = {
+        "workload": f"lattice-{side}^3: {n} FP32 spheres, jittered cubic lattice, r in {{0.005, 0.0075}}",
+        "bins": list(s.grid), "bin_entries": int(m), "active_bins": int(a), "pairs": int(p),
+        "contacts": int(cc.contacts),
+        "aabb_ms": round(avg["aabb"], 4), "broadphase_ms": round(bp_ms, 4),
+        "broadphase_wall_ms_p50": round(float(np.median(wall)) * 1e3, 3),
+        "broadphase_stage_ms": {k: round(avg[k], 4) for k in BP_STAGES},
+        "broadphase_algorithmic_bytes": int(algo), "broadphase_GBs": round(gbs, 1),
+        "broadphase_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "full_step_ms": round(step_ms, 3), "narrowphase_ms": round(np_ms, 3),
+        "bodies_per_s_full_step": round(n / (step_ms * 1e-3), 1),
+    }
+    sc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +154,8 @@ def main():
     ap.add_argument("--mode", choices=["mixed", "native", "fast"], default="mixed")
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lattice-side", type=int, default=256, help="0 skips the 16M-sphere broadphase leg")
+    ap.add_argument("--lattice-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every box has a single MI355X; "
@@ -157,6 +215,11 @@ def main():
         "avg_launch_ms": round(dom_ms, 4),
     }
 
+    lattice = None
+    if args.lattice_side > 0:
+        lattice = lattice_leg(pkg, scenes, ctx, args.lattice_side, args.lattice_steps)
+        log(f"lattice: {lattice}")
+
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -186,6 +249,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "lattice_16M_broadphase": lattice,
     }
     print(json.dumps(out), flush=True)
 

@@ -8,7 +8,7 @@
 //   scan #1      Broadphase.cu:403-407     -> scan_excl_u32
 //   key store    Broadphase.cu:172-209     -> k_emit_keys
 //   sort_by_key  Broadphase.cu:428-432     -> radix_sort_u64 on (key<<32 | body)
-//   reduce_by_key + scan #2  :439-480      -> k_run_flags, k_run_write
+//   reduce_by_key + scan #2  :439-480      -> k_count_runs (run boundaries inside k_pairs)
 //   pair count / store / sort / unique :484-535
 //                                          -> k_pairs (owner-bin dedup), radix_sort_u64
 //   MPR store    Narrowphase.cu:316-504    -> k_np_front + class sort + k_np_resolve
@@ -32,6 +32,21 @@ struct DevStatus {
 };
 
 template <class R> struct Params { R origin[3]; R inv[3]; };
+
+// Compact per-shape header for pair classification: type | active << 8, body.
+struct ShapeHdr {
+  uint32_t ta;
+  uint32_t body;
+};
+
+// Per-shape record of the FP32 filter: FP64 centre (the local frame is
+// formed by an FP64 subtraction), FP32 rotation and dimensions.
+struct alignas(16) ShapeRecF {
+  double x, y, z;
+  float qw, qx, qy, qz;
+  float bx, by, bz;
+  uint32_t type;
+};
 
 template <class R> struct alignas(16) ShapeRec {
   R x, y, z;          // world centre: quatRotate(A, body_rot) + body_pos
@@ -94,7 +109,8 @@ __global__ __launch_bounds__(BS) void k_aabb(uint64_t n, const uint8_t* __restri
                                               const R* __restrict__ dims, const R* __restrict__ lrot,
                                               const R* __restrict__ bpos, const R* __restrict__ brot,
                                               const uint8_t* __restrict__ active, uint64_t nb, R* __restrict__ aabb,
-                                              ShapeRec<R>* __restrict__ rec, DevStatus* st) {
+                                              ShapeRec<R>* __restrict__ rec, ShapeHdr* __restrict__ hdr,
+                                              ShapeRecF* __restrict__ frec, DevStatus* st) {
   uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x;
   uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
   if (i < n) {
@@ -133,6 +149,16 @@ __global__ __launch_bounds__(BS) void k_aabb(uint64_t n, const uint8_t* __restri
       r.bx = b.x; r.by = b.y; r.bz = b.z;
       r.type = (uint32_t)t; r.body = id; r.active = active[id]; r.pad = 0;
       rec[i] = r;
+      ShapeHdr h;
+      h.ta = (uint32_t)t | ((uint32_t)r.active << 8);
+      h.body = id;
+      hdr[i] = h;
+      ShapeRecF fr;
+      fr.x = (double)c.x; fr.y = (double)c.y; fr.z = (double)c.z;
+      fr.qw = (float)q1.w; fr.qx = (float)q1.x; fr.qy = (float)q1.y; fr.qz = (float)q1.z;
+      fr.bx = (float)b.x; fr.by = (float)b.y; fr.bz = (float)b.z;
+      fr.type = (uint32_t)t;
+      frec[i] = fr;
       float f;
       f = (float)lo.x; mn[0] = f2ord(f); mx[0] = f2ord(f);
       f = (float)lo.y; mn[1] = f2ord(f); mx[1] = f2ord(f);
@@ -269,24 +295,15 @@ __global__ __launch_bounds__(BS) void k_emit_keys(uint64_t n, const R* __restric
         keys[o++] = ((uint64_t)hash_index(x, y, z, gx, gy) << 32) | (uint64_t)i;
 }
 
-__global__ void k_run_flags(uint64_t m, const uint64_t* __restrict__ keys, uint32_t* __restrict__ flags) {
+// Active bins (reduce_by_key, Broadphase.cu:439-443): the number of runs of
+// equal keys, counted one atomic per wavefront.  Their keys and occupancy
+// are derived from the sorted entries on readback.
+__global__ void k_count_runs(uint64_t m, const uint64_t* __restrict__ keys, DevStatus* st) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  flags[i] = (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) ? 1u : 0u;
-}
-
-// Active bins (reduce_by_key, Broadphase.cu:439-443) as run starts and keys.
-__global__ void k_run_write(uint64_t m, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ runidx,
-                            uint32_t* __restrict__ run_start, uint32_t* __restrict__ run_key,
-                            const DevStatus* __restrict__ st) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) run_start[st->A] = (uint32_t)m;
-  if (i >= m) return;
-  if (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) {
-    uint32_t r = runidx[i];
-    run_start[r] = (uint32_t)i;
-    run_key[r] = (uint32_t)(keys[i] >> 32);
-  }
+  bool head = i < m && (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32));
+  uint64_t mask = __ballot(head);
+  if (mask && __lane_id() == (uint32_t)(__ffsll((long long)mask) - 1))
+    atomicAdd((unsigned long long*)&st->A, (unsigned long long)__popcll(mask));
 }
 
 // Pair search with de-duplication by owning bin.  A pair (a, b) of
@@ -660,13 +677,12 @@ __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float
   }
 }
 
-template <class R> __device__ __forceinline__ Placed<float> placed_local(const ShapeRec<R>& r, const ShapeRec<R>& origin) {
+__device__ __forceinline__ Placed<float> placed_local(const ShapeRecF& r, const ShapeRecF& origin) {
   Placed<float> s;
   s.type = (int)r.type;
-  s.X = mk3((float)((double)r.x - (double)origin.x), (float)((double)r.y - (double)origin.y),
-            (float)((double)r.z - (double)origin.z));
-  s.B = mk3((float)r.bx, (float)r.by, (float)r.bz);
-  s.q.w = (float)r.qw; s.q.x = (float)r.qx; s.q.y = (float)r.qy; s.q.z = (float)r.qz;
+  s.X = mk3((float)(r.x - origin.x), (float)(r.y - origin.y), (float)(r.z - origin.z));
+  s.B = mk3(r.bx, r.by, r.bz);
+  s.q.w = r.qw; s.q.x = r.qx; s.q.y = r.qy; s.q.z = r.qz;
   return s;
 }
 
@@ -695,7 +711,8 @@ template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(
 // reference-order MPR.
 template <class R, int MODE>
 __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __restrict__ pairs,
-                                                  const ShapeRec<R>* __restrict__ rec, R envelope, float tau_rel,
+                                                  const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
+                                                  const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
                                                   PairOut<R> o, uint64_t* __restrict__ queue, DevStatus* st) {
   constexpr int mode = MODE;
   uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
@@ -703,17 +720,21 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __
   uint32_t cls = 0;
   if (i < P) {
     uint64_t pr = pairs[i];
-    ShapeRec<R> ra = rec[(uint32_t)(pr >> 32)], rb = rec[(uint32_t)(pr & 0xffffffffu)];
+    const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+    const ShapeHdr ha = hdr[ia], hb = hdr[ib];
+    const uint32_t ta = ha.ta & 255u, tb = hb.ta & 255u;
     o.hit[i] = 0u;
-    cls = type_slot(ra.type) * 5u + type_slot(rb.type);
-    if ((ra.active || rb.active) && ra.body != rb.body) {
-      if (ra.type == SPHERE && rb.type == SPHERE) {
+    cls = type_slot(ta) * 5u + type_slot(tb);
+    if (((ha.ta | hb.ta) >> 8) && ha.body != hb.body) {
+      if (ta == SPHERE && tb == SPHERE) {
         Contact<R> c;
+        ShapeRec<R> ra = rec[ia], rb = rec[ib];
         if (contact_native(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
       } else if (mode == NP_NATIVE) {
         enq = true;
       } else {
-        Placed<float> A = placed_local(ra, ra), B = placed_local(rb, ra);
+        const ShapeRecF fa = frec[ia], fb = frec[ib];
+        Placed<float> A = placed_local(fa, fa), B = placed_local(fb, fa);
         float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
         v3<float> N, p0;
         const bool keep = (mode == NP_FAST);
@@ -728,7 +749,7 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __
           } else {
             Contact<R> c;
             v3<double> Nd = mk3((double)N.x, (double)N.y, (double)N.z);
-            v3<double> o3 = mk3((double)ra.x, (double)ra.y, (double)ra.z);
+            v3<double> o3 = mk3(fa.x, fa.y, fa.z);
             v3<double> q1 = mk3((double)p1.x, (double)p1.y, (double)p1.z) + o3;
             v3<double> q2 = mk3((double)p2.x, (double)p2.y, (double)p2.z) + o3;
             q1 = q1 - Nd * (double)envelope;
@@ -737,7 +758,7 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __
             c.p1 = mk3((R)q1.x, (R)q1.y, (R)q1.z);
             c.p2 = mk3((R)q2.x, (R)q2.y, (R)q2.z);
             c.depth = (R)dot(Nd, q2 - q1);
-            store_pair(o, P, i, c, ra.body, rb.body);
+            store_pair(o, P, i, c, ha.body, hb.body);
           }
         }
         // MIXED: FP32 settles clear misses only; hits take the reference
@@ -985,12 +1006,14 @@ template <class R> class Scene final : public SceneBase {
     hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, s_, status_.as<DevStatus>());
     aabb_.reserve(n_ * 6 * sizeof(R));
     rec_.reserve(n_ * sizeof(ShapeRec<R>));
+    hdr_.reserve(n_ * sizeof(ShapeHdr));
+    frec_.reserve(n_ * sizeof(ShapeRecF));
     mark(ST_AABB, true);
     if (n_)
       hipLaunchKernelGGL((k_aabb<R, 256>), dim3(blocks_for(n_, 256)), dim3(256), 0, s_, n_, type_.as<uint8_t>(),
                          body_.as<uint32_t>(), lpos_.as<R>(), dims_.as<R>(), lrot_.as<R>(), bpos_.as<R>(),
                          brot_.as<R>(), active_.as<uint8_t>(), nb_, aabb_.as<R>(), rec_.as<ShapeRec<R>>(),
-                         status_.as<DevStatus>());
+                         hdr_.as<ShapeHdr>(), frec_.as<ShapeRecF>(), status_.as<DevStatus>());
     mark(ST_AABB, false);
     have_aabb_ = true;
     bounds_fused_ = true;
@@ -1054,17 +1077,9 @@ template <class R> class Scene final : public SceneBase {
     mark(ST_BP_SORT, false);
 
     // active bins
-    flags_.reserve(M_ * 4);
-    runidx_.reserve(M_ * 4);
-    run_start_.reserve((M_ + 1) * 4);
-    run_key_.reserve(M_ * 4);
     mark(ST_BP_RUNS, true);
-    hipLaunchKernelGGL(k_run_flags, dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_, (const uint64_t*)keys_.as<uint64_t>(),
-                       flags_.as<uint32_t>());
-    scan_excl_u32(flags_.as<uint32_t>(), runidx_.as<uint32_t>(), M_, &st->A, radix_tmp_.as<uint32_t>(), s_);
-    hipLaunchKernelGGL(k_run_write, dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_,
-                       (const uint64_t*)keys_.as<uint64_t>(), (const uint32_t*)runidx_.as<uint32_t>(),
-                       run_start_.as<uint32_t>(), run_key_.as<uint32_t>(), (const DevStatus*)st);
+    hipLaunchKernelGGL(k_count_runs, dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_,
+                       (const uint64_t*)keys_.as<uint64_t>(), st);
     mark(ST_BP_RUNS, false);
 
     // pair count (per sorted entry), scan, emit
@@ -1124,7 +1139,8 @@ template <class R> class Scene final : public SceneBase {
     {
       auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, (const uint64_t*)pairs_.as<uint64_t>(),
-                           (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, tau_, o, queue_.as<uint64_t>(), st);
+                           (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), (const ShapeHdr*)hdr_.as<ShapeHdr>(),
+                           (const ShapeRecF*)frec_.as<ShapeRecF>(), envelope_, tau_, o, queue_.as<uint64_t>(), st);
       };
       if (mode_ == NP_NATIVE) launch(k_np_front<R, NP_NATIVE>);
       else if (mode_ == NP_MIXED) launch(k_np_front<R, NP_MIXED>);
@@ -1191,20 +1207,21 @@ template <class R> class Scene final : public SceneBase {
                  uint32_t* active_counts) override {
     sync();
     if (bin_counts && n_) CDX_HIP_CHECK(hipMemcpy(bin_counts, counts_buf_.p, n_ * 4, hipMemcpyDeviceToHost));
-    if ((sorted_keys || sorted_bodies) && M_) {
+    if ((sorted_keys || sorted_bodies || active_keys || active_counts) && M_) {
       std::vector<uint64_t> k(M_);
       CDX_HIP_CHECK(hipMemcpy(k.data(), keys_.p, M_ * 8, hipMemcpyDeviceToHost));
+      uint64_t r = 0;
       for (uint64_t i = 0; i < M_; ++i) {
-        if (sorted_keys) sorted_keys[i] = (uint32_t)(k[i] >> 32);
+        uint32_t key = (uint32_t)(k[i] >> 32);
+        if (sorted_keys) sorted_keys[i] = key;
         if (sorted_bodies) sorted_bodies[i] = (uint32_t)k[i];
+        if (i == 0 || key != (uint32_t)(k[i - 1] >> 32)) {
+          if (active_keys) active_keys[r] = key;
+          if (active_counts) active_counts[r] = 0;
+          ++r;
+        }
+        if (active_counts) active_counts[r - 1] += 1;
       }
-    }
-    if ((active_keys || active_counts) && A_) {
-      std::vector<uint32_t> st(A_ + 1);
-      CDX_HIP_CHECK(hipMemcpy(st.data(), run_start_.p, (A_ + 1) * 4, hipMemcpyDeviceToHost));
-      if (active_keys) CDX_HIP_CHECK(hipMemcpy(active_keys, run_key_.p, A_ * 4, hipMemcpyDeviceToHost));
-      if (active_counts)
-        for (uint64_t r = 0; r < A_; ++r) active_counts[r] = st[r + 1] - st[r];
     }
   }
 
@@ -1237,8 +1254,8 @@ template <class R> class Scene final : public SceneBase {
   }
 
   uint64_t device_bytes() const override {
-    const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_, &brec_,
-                           &counts_buf_, &offs_, &keys_, &keys_tmp_, &flags_, &runidx_, &run_start_, &run_key_,
+    const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_, &brec_, &hdr_, &frec_,
+                           &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_, &queue_tmp_,
                            &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_, &scan_tmp_, &radix_tmp_};
     uint64_t t = 0;
@@ -1254,12 +1271,14 @@ template <class R> class Scene final : public SceneBase {
   }
   void run_records_only() {
     rec_.reserve(n_ * sizeof(ShapeRec<R>));
+    hdr_.reserve(n_ * sizeof(ShapeHdr));
+    frec_.reserve(n_ * sizeof(ShapeRecF));
     aabb_.reserve(n_ * 6 * sizeof(R));
     if (n_)
       hipLaunchKernelGGL((k_aabb<R, 256>), dim3(blocks_for(n_, 256)), dim3(256), 0, s_, n_, type_.as<uint8_t>(),
                          body_.as<uint32_t>(), lpos_.as<R>(), dims_.as<R>(), lrot_.as<R>(), bpos_.as<R>(),
                          brot_.as<R>(), active_.as<uint8_t>(), nb_, aabb_.as<R>(), rec_.as<ShapeRec<R>>(),
-                         status_.as<DevStatus>());
+                         hdr_.as<ShapeHdr>(), frec_.as<ShapeRecF>(), status_.as<DevStatus>());
   }
   uint32_t* scan_tmp_for(uint64_t n) {
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(n), 16) * 4);
@@ -1306,8 +1325,8 @@ template <class R> class Scene final : public SceneBase {
   StepCounts counts_;
   DevBuf status_, params_, stage_, upd_, tmp_out_;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
-  DevBuf aabb_, rec_, brec_;
-  DevBuf counts_buf_, offs_, keys_, keys_tmp_, flags_, runidx_, run_start_, run_key_, pcount_, poffs_, pairs_, pairs_tmp_;
+  DevBuf aabb_, rec_, brec_, hdr_, frec_;
+  DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, poffs_, pairs_, pairs_tmp_;
   DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
   DevBuf scan_tmp_, radix_tmp_;
