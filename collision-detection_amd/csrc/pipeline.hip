@@ -77,8 +77,10 @@ __global__ void k_init_status(DevStatus* st) {
   }
 }
 
+// Per-workgroup bounds partials (reduced by k_params): same-address
+// atomics from every workgroup serialise badly across the XCDs.
 template <int BS>
-__device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[3], DevStatus* st) {
+__device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[3], uint32_t* part) {
   __shared__ uint32_t smn[3][BS / WAVE], smx[3][BS / WAVE];
   const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
 #pragma unroll
@@ -95,8 +97,8 @@ __device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[
     int c = threadIdx.x;
     uint32_t a = 0xffffffffu, b = 0u;
     for (int i = 0; i < BS / WAVE; ++i) { a = min(a, smn[c][i]); b = max(b, smx[c][i]); }
-    atomicMin(&st->bmin_enc[c], a);
-    atomicMax(&st->bmax_enc[c], b);
+    part[6 * (uint64_t)blockIdx.x + c] = a;
+    part[6 * (uint64_t)blockIdx.x + 3 + c] = b;
   }
 }
 
@@ -110,7 +112,8 @@ __global__ __launch_bounds__(BS) void k_aabb(uint64_t n, const uint8_t* __restri
                                               const R* __restrict__ bpos, const R* __restrict__ brot,
                                               const uint8_t* __restrict__ active, uint64_t nb, R* __restrict__ aabb,
                                               ShapeRec<R>* __restrict__ rec, ShapeHdr* __restrict__ hdr,
-                                              ShapeRecF* __restrict__ frec, DevStatus* st) {
+                                              ShapeRecF* __restrict__ frec, uint32_t* __restrict__ bpart,
+                                              DevStatus* st) {
   uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x;
   uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
   if (i < n) {
@@ -169,12 +172,12 @@ __global__ __launch_bounds__(BS) void k_aabb(uint64_t n, const uint8_t* __restri
       u = f2ord((float)hi.z); mn[2] = min(mn[2], u); mx[2] = max(mx[2], u);
     }
   }
-  block_bounds_commit<BS>(mn, mx, st);
+  block_bounds_commit<BS>(mn, mx, bpart);
 }
 
 // Bounds of AABBs supplied by the caller (SoA: x[2n], y[2n], z[2n]).
 template <class R, int BS>
-__global__ __launch_bounds__(BS) void k_bounds(uint64_t n2, const R* __restrict__ aabb, DevStatus* st) {
+__global__ __launch_bounds__(BS) void k_bounds(uint64_t n2, const R* __restrict__ aabb, uint32_t* __restrict__ bpart) {
   uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
   for (uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * BS) {
 #pragma unroll
@@ -184,17 +187,36 @@ __global__ __launch_bounds__(BS) void k_bounds(uint64_t n2, const R* __restrict_
       mx[c] = max(mx[c], u);
     }
   }
-  block_bounds_commit<BS>(mn, mx, st);
+  block_bounds_commit<BS>(mn, mx, bpart);
 }
 
-// Origin and inverse bin size (Broadphase.cu:377-381).
+// Scene bounds from the workgroup partials, then origin and inverse bin
+// size (Broadphase.cu:377-381).  One workgroup of 256 threads.
 template <class R>
-__global__ void k_params(const DevStatus* st, int gx, int gy, int gz, Params<R>* prm) {
+__global__ __launch_bounds__(256) void k_params(const uint32_t* __restrict__ bpart, uint64_t npart, DevStatus* st,
+                                                int gx, int gy, int gz, Params<R>* prm) {
+  __shared__ uint32_t red[6][256];
+  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  for (uint64_t b = threadIdx.x; b < npart; b += 256) {
+    for (int c = 0; c < 3; ++c) {
+      v[c] = min(v[c], bpart[6 * b + c]);
+      v[3 + c] = max(v[3 + c], bpart[6 * b + 3 + c]);
+    }
+  }
+  for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = v[c];
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  for (int t = 1; t < 256; ++t)
+    for (int c = 0; c < 3; ++c) {
+      v[c] = min(v[c], red[c][t]);
+      v[3 + c] = max(v[3 + c], red[3 + c][t]);
+    }
   int g[3] = {gx, gy, gz};
   for (int c = 0; c < 3; ++c) {
-    R mn = (R)ord2f(st->bmin_enc[c]);
-    R mx = (R)ord2f(st->bmax_enc[c]);
+    st->bmin_enc[c] = v[c];
+    st->bmax_enc[c] = v[3 + c];
+    R mn = (R)ord2f(v[c]);
+    R mx = (R)ord2f(v[3 + c]);
     R origin = cabs(mn);
     R ext = cabs(mx + cabs(mn));
     prm->origin[c] = origin;
@@ -242,7 +264,8 @@ template <class R> struct alignas(16) BpRec {
 template <class R, int BS>
 __global__ __launch_bounds__(BS) void k_count(uint64_t n, const R* __restrict__ aabb, const Params<R>* __restrict__ prm,
                                                int gx, int gy, int gz, uint32_t* __restrict__ counts,
-                                               BpRec<R>* __restrict__ brec, DevStatus* st) {
+                                               BpRec<R>* __restrict__ brec, uint32_t* __restrict__ kpart,
+                                               DevStatus* st) {
   __shared__ uint32_t skm[BS / WAVE];
   uint64_t i = (uint64_t)blockIdx.x * BS + threadIdx.x;
   uint32_t km = 0;
@@ -272,7 +295,20 @@ __global__ __launch_bounds__(BS) void k_count(uint64_t n, const R* __restrict__ 
   if (threadIdx.x == 0) {
     uint32_t m = 0;
     for (int w = 0; w < BS / WAVE; ++w) m = max(m, skm[w]);
-    atomicMax(&st->key_max, m);
+    kpart[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart, uint64_t npart,
+                                                       DevStatus* st) {
+  __shared__ uint32_t red[256];
+  uint32_t m = 0;
+  for (uint64_t b = threadIdx.x; b < npart; b += 256) m = max(m, kpart[b]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 1; t < 256; ++t) m = max(m, red[t]);
+    st->key_max = m;
   }
 }
 
@@ -298,12 +334,19 @@ __global__ __launch_bounds__(BS) void k_emit_keys(uint64_t n, const R* __restric
 // Active bins (reduce_by_key, Broadphase.cu:439-443): the number of runs of
 // equal keys, counted one atomic per wavefront.  Their keys and occupancy
 // are derived from the sorted entries on readback.
-__global__ void k_count_runs(uint64_t m, const uint64_t* __restrict__ keys, DevStatus* st) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool head = i < m && (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32));
-  uint64_t mask = __ballot(head);
-  if (mask && __lane_id() == (uint32_t)(__ffsll((long long)mask) - 1))
-    atomicAdd((unsigned long long*)&st->A, (unsigned long long)__popcll(mask));
+__global__ __launch_bounds__(256) void k_count_runs(uint64_t m, const uint64_t* __restrict__ keys, DevStatus* st) {
+  __shared__ uint32_t red[256 / WAVE];
+  uint32_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (uint64_t)gridDim.x * 256)
+    c += (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) ? 1u : 0u;
+  for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o);
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 256 / WAVE; ++w) t += red[w];
+    atomicAdd((unsigned long long*)&st->A, (unsigned long long)t);
+  }
 }
 
 // Pair search with de-duplication by owning bin.  A pair (a, b) of
@@ -713,7 +756,7 @@ template <class R, int MODE>
 __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __restrict__ pairs,
                                                   const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
                                                   const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
-                                                  PairOut<R> o, uint64_t* __restrict__ queue, DevStatus* st) {
+                                                  PairOut<R> o, uint32_t* __restrict__ qsel, uint8_t* __restrict__ qcls) {
   constexpr int mode = MODE;
   uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
   bool enq = false;
@@ -767,15 +810,19 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __
       }
     }
   }
-  const uint64_t mask = __ballot(enq);
-  if (mask) {
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
-    uint64_t base = 0;
-    if (lane == leader) base = atomicAdd((unsigned long long*)&st->Q, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    if (enq) queue[base + __popcll(mask & lanemask_lt())] = ((uint64_t)cls << 32) | i;
+  if (i < P) {
+    qsel[i] = enq ? 1u : 0u;
+    qcls[i] = (uint8_t)cls;
   }
+}
+
+// Queue of the pairs that need the MPR, in pair order: slot = exclusive scan
+// of the selection flags (no same-address atomics).
+__global__ void k_queue_compact(uint64_t P, const uint32_t* __restrict__ qsel, const uint32_t* __restrict__ qpos,
+                                const uint8_t* __restrict__ qcls, uint64_t* __restrict__ queue) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P || !qsel[i]) return;
+  queue[qpos[i]] = ((uint64_t)qcls[i] << 32) | i;
 }
 
 // MPR on the reference path for the queued pairs, in compute type CT
@@ -1008,12 +1055,14 @@ template <class R> class Scene final : public SceneBase {
     rec_.reserve(n_ * sizeof(ShapeRec<R>));
     hdr_.reserve(n_ * sizeof(ShapeHdr));
     frec_.reserve(n_ * sizeof(ShapeRecF));
+    bpart_.reserve((uint64_t)blocks_for(n_, 256) * 6 * 4);
+    npart_ = blocks_for(n_, 256);
     mark(ST_AABB, true);
     if (n_)
       hipLaunchKernelGGL((k_aabb<R, 256>), dim3(blocks_for(n_, 256)), dim3(256), 0, s_, n_, type_.as<uint8_t>(),
                          body_.as<uint32_t>(), lpos_.as<R>(), dims_.as<R>(), lrot_.as<R>(), bpos_.as<R>(),
                          brot_.as<R>(), active_.as<uint8_t>(), nb_, aabb_.as<R>(), rec_.as<ShapeRec<R>>(),
-                         hdr_.as<ShapeHdr>(), frec_.as<ShapeRecF>(), status_.as<DevStatus>());
+                         hdr_.as<ShapeHdr>(), frec_.as<ShapeRecF>(), bpart_.as<uint32_t>(), status_.as<DevStatus>());
     mark(ST_AABB, false);
     have_aabb_ = true;
     bounds_fused_ = true;
@@ -1034,9 +1083,13 @@ template <class R> class Scene final : public SceneBase {
     if (!bounds_fused_) {
       hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, s_, st);
       unsigned nbk = std::min<unsigned>(blocks_for(2 * n, 256), 2048u);
-      hipLaunchKernelGGL((k_bounds<R, 256>), dim3(nbk), dim3(256), 0, s_, 2 * n, (const R*)aabb_.as<R>(), st);
+      bpart_.reserve((uint64_t)nbk * 6 * 4);
+      npart_ = nbk;
+      hipLaunchKernelGGL((k_bounds<R, 256>), dim3(nbk), dim3(256), 0, s_, 2 * n, (const R*)aabb_.as<R>(),
+                         bpart_.as<uint32_t>());
     }
-    hipLaunchKernelGGL(k_params<R>, dim3(1), dim3(64), 0, s_, (const DevStatus*)st, g_[0], g_[1], g_[2],
+    hipLaunchKernelGGL(k_params<R>, dim3(1), dim3(256), 0, s_, (const uint32_t*)bpart_.as<uint32_t>(), npart_, st,
+                       g_[0], g_[1], g_[2],
                        params_.as<Params<R>>());
     mark(ST_BP_BOUNDS, false);
     bounds_fused_ = false;  // a second broadphase on the same AABBs recomputes bounds
@@ -1047,9 +1100,12 @@ template <class R> class Scene final : public SceneBase {
     brec_.reserve(n * sizeof(BpRec<R>));
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(n), 16) * 4);
     mark(ST_BP_COUNT, true);
+    kpart_.reserve((uint64_t)blocks_for(n, 256) * 4);
     hipLaunchKernelGGL((k_count<R, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s_, n, (const R*)aabb_.as<R>(),
                        (const Params<R>*)params_.as<Params<R>>(), g_[0], g_[1], g_[2], counts_buf_.as<uint32_t>(),
-                       brec_.as<BpRec<R>>(), st);
+                       brec_.as<BpRec<R>>(), kpart_.as<uint32_t>(), st);
+    hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(),
+                       (uint64_t)blocks_for(n, 256), st);
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
     scan_excl_u32(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), n, &st->M, scan_tmp_.as<uint32_t>(), s_);
@@ -1078,7 +1134,7 @@ template <class R> class Scene final : public SceneBase {
 
     // active bins
     mark(ST_BP_RUNS, true);
-    hipLaunchKernelGGL(k_count_runs, dim3(blocks_for(M_, 256)), dim3(256), 0, s_, M_,
+    hipLaunchKernelGGL(k_count_runs, dim3(std::min<unsigned>(blocks_for(M_, 256), 1024u)), dim3(256), 0, s_, M_,
                        (const uint64_t*)keys_.as<uint64_t>(), st);
     mark(ST_BP_RUNS, false);
 
@@ -1132,6 +1188,8 @@ template <class R> class Scene final : public SceneBase {
     np_pos_.reserve(std::max<uint64_t>(P, 1) * 4);
     queue_.reserve(std::max<uint64_t>(P, 1) * 8);
     queue_tmp_.reserve(std::max<uint64_t>(P, 1) * 8);
+    qsel_.reserve(std::max<uint64_t>(P, 1) * 4);
+    qcls_.reserve(std::max<uint64_t>(P, 1));
     PairOut<R> o{np_v_.as<R>(), np_ids_.as<int32_t>(), np_hit_.as<uint32_t>(), P};
     hipLaunchKernelGGL(k_zero_np, dim3(1), dim3(64), 0, s_, st);
     mark(ST_NP_MAIN, true);
@@ -1140,11 +1198,18 @@ template <class R> class Scene final : public SceneBase {
       auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, (const uint64_t*)pairs_.as<uint64_t>(),
                            (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), (const ShapeHdr*)hdr_.as<ShapeHdr>(),
-                           (const ShapeRecF*)frec_.as<ShapeRecF>(), envelope_, tau_, o, queue_.as<uint64_t>(), st);
+                           (const ShapeRecF*)frec_.as<ShapeRecF>(), envelope_, tau_, o, qsel_.as<uint32_t>(),
+                           qcls_.as<uint8_t>());
       };
       if (mode_ == NP_NATIVE) launch(k_np_front<R, NP_NATIVE>);
       else if (mode_ == NP_MIXED) launch(k_np_front<R, NP_MIXED>);
       else launch(k_np_front<R, NP_FAST>);
+    }
+    if (P) {
+      scan_excl_u32(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st->Q, scan_tmp_for(P), s_);
+      hipLaunchKernelGGL(k_queue_compact, dim3(blocks_for(P, 256)), dim3(256), 0, s_, P,
+                         (const uint32_t*)qsel_.as<uint32_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
+                         (const uint8_t*)qcls_.as<uint8_t>(), queue_.as<uint64_t>());
     }
     mark(ST_NP_MAIN, false);
     read_status();
@@ -1256,7 +1321,7 @@ template <class R> class Scene final : public SceneBase {
   uint64_t device_bytes() const override {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_, &brec_, &hdr_, &frec_,
                            &counts_buf_, &offs_, &keys_, &keys_tmp_,
-                           &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_, &queue_tmp_,
+                           &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_, &queue_tmp_, &qsel_, &qcls_,
                            &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_, &scan_tmp_, &radix_tmp_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
@@ -1273,12 +1338,14 @@ template <class R> class Scene final : public SceneBase {
     rec_.reserve(n_ * sizeof(ShapeRec<R>));
     hdr_.reserve(n_ * sizeof(ShapeHdr));
     frec_.reserve(n_ * sizeof(ShapeRecF));
+    bpart_.reserve((uint64_t)blocks_for(n_, 256) * 6 * 4);
+    npart_ = blocks_for(n_, 256);
     aabb_.reserve(n_ * 6 * sizeof(R));
     if (n_)
       hipLaunchKernelGGL((k_aabb<R, 256>), dim3(blocks_for(n_, 256)), dim3(256), 0, s_, n_, type_.as<uint8_t>(),
                          body_.as<uint32_t>(), lpos_.as<R>(), dims_.as<R>(), lrot_.as<R>(), bpos_.as<R>(),
                          brot_.as<R>(), active_.as<uint8_t>(), nb_, aabb_.as<R>(), rec_.as<ShapeRec<R>>(),
-                         hdr_.as<ShapeHdr>(), frec_.as<ShapeRecF>(), status_.as<DevStatus>());
+                         hdr_.as<ShapeHdr>(), frec_.as<ShapeRecF>(), bpart_.as<uint32_t>(), status_.as<DevStatus>());
   }
   uint32_t* scan_tmp_for(uint64_t n) {
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(n), 16) * 4);
@@ -1325,9 +1392,10 @@ template <class R> class Scene final : public SceneBase {
   StepCounts counts_;
   DevBuf status_, params_, stage_, upd_, tmp_out_;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
-  DevBuf aabb_, rec_, brec_, hdr_, frec_;
+  DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
+  uint64_t npart_ = 0;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, poffs_, pairs_, pairs_tmp_;
-  DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_;
+  DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
   DevBuf scan_tmp_, radix_tmp_;
 };
