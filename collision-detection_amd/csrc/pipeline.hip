@@ -28,7 +28,7 @@ namespace {
 struct DevStatus {
   uint32_t bmin_enc[3], bmax_enc[3];
   uint32_t bad, key_max;
-  uint64_t M, A, P, C, Q;
+  uint64_t M, A, P, C, Q, Q2;
 };
 
 template <class R> struct Params { R origin[3]; R inv[3]; };
@@ -73,7 +73,7 @@ __global__ void k_init_status(DevStatus* st) {
   if (threadIdx.x == 0) {
     for (int c = 0; c < 3; ++c) { st->bmin_enc[c] = 0xffffffffu; st->bmax_enc[c] = 0u; }
     st->bad = 0; st->key_max = 0;
-    st->M = st->A = st->P = st->C = st->Q = 0;
+    st->M = st->A = st->P = st->C = st->Q = st->Q2 = 0;
   }
 }
 
@@ -548,20 +548,35 @@ template <class R> struct Contact {
 
 // function_MPR_Store (Narrowphase.cu:316-423) for two shape records in the
 // scene's own precision.  Returns true on contact.
+// Sphere-sphere closed form (Narrowphase.cu:382-396) plus the envelope and
+// depth of :412-415.
+template <class R>
+__device__ __forceinline__ bool contact_spheres(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R envelope,
+                                                Contact<R>& out) {
+  v3<R> AX = mk3(ra.x, ra.y, ra.z), BX = mk3(rb.x, rb.y, rb.z);
+  v3<R> relpos = BX - AX;
+  R d2 = dot(relpos, relpos);
+  R cd = ra.bx + rb.bx;
+  if (!(d2 <= cd * cd)) return false;
+  v3<R> N = normalize(relpos);
+  v3<R> p1 = AX + N * ra.bx;
+  v3<R> p2 = BX - N * rb.bx;
+  p1 = p1 - N * envelope;
+  p2 = p2 + N * envelope;
+  out.N = N;
+  out.p1 = p1;
+  out.p2 = p2;
+  out.depth = dot(N, p2 - p1);
+  return true;
+}
+
 template <class R>
 __device__ bool contact_native(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R envelope, Contact<R>& out) {
   if (!ra.active && !rb.active) return false;
   if (ra.body == rb.body) return false;
   v3<R> N = mk3<R>(1, 0, 0), p1 = mk3<R>(0, 0, 0), p2 = mk3<R>(0, 0, 0);
   if (ra.type == SPHERE && rb.type == SPHERE) {
-    v3<R> AX = mk3(ra.x, ra.y, ra.z), BX = mk3(rb.x, rb.y, rb.z);
-    v3<R> relpos = BX - AX;
-    R d2 = dot(relpos, relpos);
-    R cd = ra.bx + rb.bx;
-    if (!(d2 <= cd * cd)) return false;
-    N = normalize(relpos);
-    p1 = AX + N * ra.bx;
-    p2 = BX - N * rb.bx;
+    return contact_spheres(ra, rb, envelope, out);
   } else {
     Placed<R> A = placed(ra), B = placed(rb);
     v3<R> p0 = mk3<R>(0, 0, 0);
@@ -752,68 +767,85 @@ template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(
 // MIXED / FAST first run an FP32 MPR in the frame of shape A: clear misses
 // end here, FAST keeps clear FP32 hits, everything else is queued for the
 // reference-order MPR.
-template <class R, int MODE>
+template <class R>
 __global__ __launch_bounds__(128) void k_np_front(uint64_t P, const uint64_t* __restrict__ pairs,
                                                   const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
-                                                  const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
-                                                  PairOut<R> o, uint32_t* __restrict__ qsel, uint8_t* __restrict__ qcls) {
-  constexpr int mode = MODE;
+                                                  R envelope, PairOut<R> o, uint32_t* __restrict__ qsel,
+                                                  uint8_t* __restrict__ qcls) {
   uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (i >= P) return;
+  uint64_t pr = pairs[i];
+  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+  const ShapeHdr ha = hdr[ia], hb = hdr[ib];
+  const uint32_t ta = ha.ta & 255u, tb = hb.ta & 255u;
+  o.hit[i] = 0u;
   bool enq = false;
-  uint32_t cls = 0;
-  if (i < P) {
-    uint64_t pr = pairs[i];
-    const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
-    const ShapeHdr ha = hdr[ia], hb = hdr[ib];
-    const uint32_t ta = ha.ta & 255u, tb = hb.ta & 255u;
-    o.hit[i] = 0u;
-    cls = type_slot(ta) * 5u + type_slot(tb);
-    if (((ha.ta | hb.ta) >> 8) && ha.body != hb.body) {
-      if (ta == SPHERE && tb == SPHERE) {
-        Contact<R> c;
-        ShapeRec<R> ra = rec[ia], rb = rec[ib];
-        if (contact_native(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
-      } else if (mode == NP_NATIVE) {
-        enq = true;
-      } else {
-        const ShapeRecF fa = frec[ia], fb = frec[ib];
-        Placed<float> A = placed_local(fa, fa), B = placed_local(fb, fa);
-        float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
-        v3<float> N, p0;
-        const bool keep = (mode == NP_FAST);
-        int r = mpr_guarded(A, B, tau, !keep, N, p0);
-        if (r == MPR_HIT && keep) {
-          v3<float> p1 = dot((sup(A, -N) - p0), N) * N + p0;
-          v3<float> p2 = dot((sup(B, N) - p0), N) * N + p0;
-          N = -N;
-          float dl = dot(N, p2 - p1);
-          if (fabsf(dl) < tau) {
-            r = MPR_UNSURE;
-          } else {
-            Contact<R> c;
-            v3<double> Nd = mk3((double)N.x, (double)N.y, (double)N.z);
-            v3<double> o3 = mk3(fa.x, fa.y, fa.z);
-            v3<double> q1 = mk3((double)p1.x, (double)p1.y, (double)p1.z) + o3;
-            v3<double> q2 = mk3((double)p2.x, (double)p2.y, (double)p2.z) + o3;
-            q1 = q1 - Nd * (double)envelope;
-            q2 = q2 + Nd * (double)envelope;
-            c.N = mk3((R)Nd.x, (R)Nd.y, (R)Nd.z);
-            c.p1 = mk3((R)q1.x, (R)q1.y, (R)q1.z);
-            c.p2 = mk3((R)q2.x, (R)q2.y, (R)q2.z);
-            c.depth = (R)dot(Nd, q2 - q1);
-            store_pair(o, P, i, c, ha.body, hb.body);
-          }
-        }
-        // MIXED: FP32 settles clear misses only; hits take the reference
-        // path, whose capped iteration sequence FP32 cannot reproduce.
-        enq = (r == MPR_UNSURE) || (r == MPR_HIT && !keep);
-      }
+  if (((ha.ta | hb.ta) >> 8) && ha.body != hb.body) {
+    if (ta == SPHERE && tb == SPHERE) {
+      Contact<R> c;
+      ShapeRec<R> ra = rec[ia], rb = rec[ib];
+      if (contact_spheres(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
+    } else {
+      enq = true;
     }
   }
-  if (i < P) {
-    qsel[i] = enq ? 1u : 0u;
-    qcls[i] = (uint8_t)cls;
+  qsel[i] = enq ? 1u : 0u;
+  qcls[i] = (uint8_t)(type_slot(ta) * 5u + type_slot(tb));
+}
+
+// FP32 MPR over the class-sorted queue (MIXED / FAST), in the frame of
+// shape A: clear misses end here; FAST keeps clear FP32 hits; everything
+// else is selected for the reference-order MPR.
+template <class R, bool KEEP>
+__global__ __launch_bounds__(128) void k_np_filter(uint64_t q, const uint64_t* __restrict__ queue,
+                                                   const uint64_t* __restrict__ pairs,
+                                                   const ShapeHdr* __restrict__ hdr,
+                                                   const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
+                                                   PairOut<R> o, uint32_t* __restrict__ sel) {
+  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (j >= q) return;
+  const uint64_t P = o.P;
+  const uint32_t i = (uint32_t)queue[j];
+  const uint64_t pr = pairs[i];
+  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+  const ShapeRecF fa = frec[ia], fb = frec[ib];
+  Placed<float> A = placed_local(fa, fa), B = placed_local(fb, fa);
+  float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
+  v3<float> N, p0;
+  int r = mpr_guarded(A, B, tau, !KEEP, N, p0);
+  if (KEEP && r == MPR_HIT) {
+    v3<float> p1 = dot((sup(A, -N) - p0), N) * N + p0;
+    v3<float> p2 = dot((sup(B, N) - p0), N) * N + p0;
+    N = -N;
+    float dl = dot(N, p2 - p1);
+    if (fabsf(dl) < tau) {
+      r = MPR_UNSURE;
+    } else {
+      Contact<R> c;
+      v3<double> Nd = mk3((double)N.x, (double)N.y, (double)N.z);
+      v3<double> o3 = mk3(fa.x, fa.y, fa.z);
+      v3<double> q1 = mk3((double)p1.x, (double)p1.y, (double)p1.z) + o3;
+      v3<double> q2 = mk3((double)p2.x, (double)p2.y, (double)p2.z) + o3;
+      q1 = q1 - Nd * (double)envelope;
+      q2 = q2 + Nd * (double)envelope;
+      c.N = mk3((R)Nd.x, (R)Nd.y, (R)Nd.z);
+      c.p1 = mk3((R)q1.x, (R)q1.y, (R)q1.z);
+      c.p2 = mk3((R)q2.x, (R)q2.y, (R)q2.z);
+      c.depth = (R)dot(Nd, q2 - q1);
+      store_pair(o, P, i, c, hdr[ia].body, hdr[ib].body);
+    }
   }
+  // MIXED: FP32 settles clear misses only; hits take the reference path,
+  // whose capped iteration sequence FP32 cannot reproduce.
+  sel[j] = (r == MPR_UNSURE || (r == MPR_HIT && !KEEP)) ? 1u : 0u;
+}
+
+// Stable compaction of the selected queue entries (keeps the class order).
+__global__ void k_queue_select(uint64_t q, const uint64_t* __restrict__ queue, const uint32_t* __restrict__ sel,
+                               const uint32_t* __restrict__ pos, uint64_t* __restrict__ out) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= q || !sel[j]) return;
+  out[pos[j]] = queue[j];
 }
 
 // Queue of the pairs that need the MPR, in pair order: slot = exclusive scan
@@ -1193,41 +1225,52 @@ template <class R> class Scene final : public SceneBase {
     PairOut<R> o{np_v_.as<R>(), np_ids_.as<int32_t>(), np_hit_.as<uint32_t>(), P};
     hipLaunchKernelGGL(k_zero_np, dim3(1), dim3(64), 0, s_, st);
     mark(ST_NP_MAIN, true);
-    if (P)
-    {
-      auto launch = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, (const uint64_t*)pairs_.as<uint64_t>(),
-                           (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), (const ShapeHdr*)hdr_.as<ShapeHdr>(),
-                           (const ShapeRecF*)frec_.as<ShapeRecF>(), envelope_, tau_, o, qsel_.as<uint32_t>(),
-                           qcls_.as<uint8_t>());
-      };
-      if (mode_ == NP_NATIVE) launch(k_np_front<R, NP_NATIVE>);
-      else if (mode_ == NP_MIXED) launch(k_np_front<R, NP_MIXED>);
-      else launch(k_np_front<R, NP_FAST>);
-    }
     if (P) {
+      hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P,
+                         (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(),
+                         (const ShapeHdr*)hdr_.as<ShapeHdr>(), envelope_, o, qsel_.as<uint32_t>(), qcls_.as<uint8_t>());
       scan_excl_u32(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st->Q, scan_tmp_for(P), s_);
       hipLaunchKernelGGL(k_queue_compact, dim3(blocks_for(P, 256)), dim3(256), 0, s_, P,
                          (const uint32_t*)qsel_.as<uint32_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
                          (const uint8_t*)qcls_.as<uint8_t>(), queue_.as<uint64_t>());
     }
-    mark(ST_NP_MAIN, false);
     read_status();
-    const uint64_t q = hstat_->Q;
-    mark(ST_NP_RESOLVE, true);
+    uint64_t q = hstat_->Q;
     if (q) {
-      // group the queue by shape-pair class (one 8-bit digit, stable)
+      // group the MPR queue by shape-pair class (one 8-bit digit, stable)
       const int cls_shift[1] = {32};
       radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(q), 16) * 4);
       radix_sort_u64(queue_.as<uint64_t>(), queue_tmp_.as<uint64_t>(), q, cls_shift, 1, radix_tmp_.as<uint32_t>(), s_);
+    }
+    mark(ST_NP_MAIN, false);
+    mark(ST_NP_RESOLVE, true);
+    uint64_t* rq = queue_.as<uint64_t>();
+    if (q && mode_ != NP_NATIVE) {
+      auto filt = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(blocks_for(q, 128)), dim3(128), 0, s_, q, (const uint64_t*)queue_.as<uint64_t>(),
+                           (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeHdr*)hdr_.as<ShapeHdr>(),
+                           (const ShapeRecF*)frec_.as<ShapeRecF>(), envelope_, tau_, o, qsel_.as<uint32_t>());
+      };
+      if (mode_ == NP_FAST) filt(k_np_filter<R, true>);
+      else filt(k_np_filter<R, false>);
+      scan_excl_u32(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), q, &st->Q2, scan_tmp_for(q), s_);
+      hipLaunchKernelGGL(k_queue_select, dim3(blocks_for(q, 256)), dim3(256), 0, s_, q,
+                         (const uint64_t*)queue_.as<uint64_t>(), (const uint32_t*)qsel_.as<uint32_t>(),
+                         (const uint32_t*)np_pos_.as<uint32_t>(), queue_tmp_.as<uint64_t>());
+      read_status();
+      q = hstat_->Q2;
+      rq = queue_tmp_.as<uint64_t>();
+    }
+    resolved_ = mode_ == NP_NATIVE ? 0 : q;
+    if (q) {
       if (mode_ == NP_NATIVE)
         hipLaunchKernelGGL((k_np_resolve<R, R>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q,
                            (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
-                           o, (const uint64_t*)queue_.as<uint64_t>());
+                           o, (const uint64_t*)rq);
       else
         hipLaunchKernelGGL((k_np_resolve<R, double>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q,
                            (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
-                           o, (const uint64_t*)queue_.as<uint64_t>());
+                           o, (const uint64_t*)rq);
     }
     mark(ST_NP_RESOLVE, false);
     mark(ST_NP_COMPACT, true);
@@ -1248,7 +1291,7 @@ template <class R> class Scene final : public SceneBase {
                          out_ids_.as<int32_t>(), perm_.as<uint64_t>());
     mark(ST_NP_COMPACT, false);
     counts_.c = C_;
-    counts_.resolved = mode_ == NP_NATIVE ? 0 : q;
+    counts_.resolved = resolved_;
     counts_.p = P;
   }
 
@@ -1393,7 +1436,7 @@ template <class R> class Scene final : public SceneBase {
   DevBuf status_, params_, stage_, upd_, tmp_out_;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
-  uint64_t npart_ = 0;
+  uint64_t npart_ = 0, resolved_ = 0;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, poffs_, pairs_, pairs_tmp_;
   DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
