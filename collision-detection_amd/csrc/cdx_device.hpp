@@ -1,0 +1,102 @@
+// Device-side data structures shared by the broadphase and narrowphase
+// kernels.  Included by pipeline.hip only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "cdx_math.hpp"
+#include "cdx_prims.hpp"
+
+namespace cdx {
+namespace {
+
+// Totals of one step, left on the device for the next kernels (and read by
+// the host only at synchronisation).
+struct DevStatus {
+  uint64_t M;        // bin entries
+  uint64_t A;        // active bins
+  uint64_t P;        // unique candidate pairs
+  uint64_t C;        // contacts
+  uint64_t Q, Q2;    // MPR queue, FP64 re-solve queue
+  uint64_t nslots;   // chunk slots of this step (chunks * CHUNK)
+  uint32_t bad;      // 1 unsupported type, 2 body id out of range, 4 AABB outside the grid
+  uint32_t overflow; // 1 bin entries, 2 pairs: capacity exceeded, results incomplete
+  uint32_t key_max;
+  uint32_t nsel;     // scenes in this step
+};
+
+// What the host sets before a step (copied into DevStatus by k_begin_step,
+// so a recorded step reads the current selection from device memory).
+struct StepHdr {
+  uint64_t nslots;
+  uint32_t nsel;
+  uint32_t pad;
+};
+
+constexpr int CHUNK = 256;
+
+// A run of at most CHUNK consecutive shapes of one scene, processed by one
+// workgroup.  Steps over several scenes are lists of chunks.
+struct Chunk {
+  uint32_t scene;
+  uint32_t begin;  // global shape index
+  uint32_t len;
+  uint32_t pad;
+};
+
+// One resident scene: its shape and body ranges in the concatenated arrays,
+// its grid, the base of its keys in the combined key space, and the
+// per-step origin / inverse bin size computed on the device.
+template <class R> struct SceneDesc {
+  uint32_t shape_begin, shape_end, body_begin, body_end;
+  int gx, gy, gz;
+  uint32_t key_base;
+  uint32_t chunk_begin, chunk_end;  // this step's chunk range (empty if not stepped)
+  uint32_t bmin_enc[3], bmax_enc[3];
+  R origin[3], inv[3];
+};
+
+// Compact per-shape header for pair classification: type | active << 8, body.
+struct ShapeHdr {
+  uint32_t ta;
+  uint32_t body;
+};
+
+// Per-shape record of the FP32 filter: FP64 centre (the local frame is
+// formed by an FP64 subtraction), FP32 rotation and dimensions.
+struct alignas(16) ShapeRecF {
+  double x, y, z;
+  float qw, qx, qy, qz;
+  float bx, by, bz;
+  uint32_t type;
+};
+
+template <class R> struct alignas(16) ShapeRec {
+  R x, y, z;          // world centre: quatRotate(A, body_rot) + body_pos
+  R qw, qx, qy, qz;   // world rotation: body_rot (x) local rot
+  R bx, by, bz;       // dimensions
+  uint32_t type, body, active, pad;
+};
+
+// Per-shape broadphase record: shifted AABB and its minimum cell, gathered
+// as one aligned block by the pair search.
+template <class R> struct alignas(16) BpRec {
+  R lo[3], hi[3];
+  int g[3];
+  uint32_t scene;
+};
+
+inline unsigned blocks_for(uint64_t n, unsigned bs) {
+  uint64_t b = (n + bs - 1) / bs;
+  return (unsigned)(b ? b : 1);
+}
+
+__device__ __forceinline__ uint32_t hash_index(uint32_t i, uint32_t j, uint32_t k, int gx, int gy) {
+  // Broadphase.cu:140-147, evaluated in unsigned arithmetic (U3 argument).
+  return ((k * (uint32_t)gy) * (uint32_t)gx) + (j * (uint32_t)gx) + i;
+}
+
+template <class R> __device__ __forceinline__ int cell_of(R x, R inv) { return (int)(x * inv); }
+
+}  // namespace
+}  // namespace cdx

@@ -78,6 +78,40 @@ class SceneBase {
   virtual uint64_t device_bytes() const = 0;
 };
 
+// Many resident scenes stepped together: each step covers a selection of
+// scenes with one launch sequence (recorded as a HIP graph and replayed),
+// buffers sized by capacities, totals read back once at finish().
+class BatchBase {
+ public:
+  virtual ~BatchBase() = default;
+  virtual int real_kind() const = 0;
+  // Appends a scene (host pointers, reference layouts; body ids local to the
+  // scene).  Returns its index.
+  virtual uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R,
+                             const uint32_t* ID, uint64_t nb, const void* pos, const void* rot,
+                             const uint8_t* active, int gx, int gy, int gz) = 0;
+  virtual uint64_t scene_count() const = 0;
+  virtual void update_scene_bodies(uint32_t scene, uint64_t k, const uint32_t* idx, const void* pos,
+                                   const void* rot) = 0;
+  virtual void set_envelope(double e) = 0;
+  virtual void set_np_mode(int mode, double tau) = 0;
+  // Scenes of the following steps (null = all resident scenes).
+  virtual void select(const uint32_t* scenes, uint64_t k) = 0;
+  virtual uint64_t selected_count() const = 0;
+  virtual void step(bool use_graph) = 0;  // enqueue AABB + broadphase + narrowphase
+  virtual void finish() = 0;              // wait, re-run on capacity overflow, read totals
+  virtual const StepCounts& totals() const = 0;
+  // Per selected scene j (position in the selection), after finish().
+  virtual void scene_counts(uint64_t j, StepCounts* out) = 0;
+  virtual void read_scene_pairs(uint64_t j, int64_t* out) = 0;
+  virtual void read_scene_contacts(uint64_t j, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) = 0;
+  virtual double last_step_ms() = 0;
+  virtual uint64_t scene_bytes_estimate(uint64_t n_shapes, uint64_t n_bodies) const = 0;
+  virtual uint64_t device_bytes() const = 0;
+  virtual hipStream_t stream() const = 0;
+};
+
 std::unique_ptr<SceneBase> make_scene(int real_kind, int device);
+std::unique_ptr<BatchBase> make_batch(int real_kind, int device);
 
 }  // namespace cdx

@@ -52,31 +52,23 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratc
 }
 
 // ---- host launchers (prims.hip) ------------------------------------------
+// Element counts are capacities known on the host; when d_n is given, the
+// kernels process min(*d_n, capacity) elements, so a launch sequence can be
+// recorded once (HIP graph) while the data-dependent sizes stay on the GPU.
 
 // Exclusive scan of n uint32 (in may equal out).  The 64-bit total goes to
 // *d_total (device memory).  tmp must hold scan_tmp_words(n) uint32.
 size_t scan_tmp_words(uint64_t n);
-void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s);
+void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s,
+                   const uint64_t* d_n = nullptr);
 
 // Stable LSD radix sort of n 64-bit values by the 8-bit digits at the given
 // bit shifts (least significant first).  Result ends in *data (tmp is the
 // ping-pong buffer of n values).  hist_tmp must hold radix_tmp_words(n).
 size_t radix_tmp_words(uint64_t n);
 void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
-                    hipStream_t s);
+                    hipStream_t s, const uint64_t* d_n = nullptr);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
-
-// Sorts each segment [off[s], off[s+1]) of data ascending, for s < nseg.
-// Segments of up to 32 elements are insertion-sorted by one thread; longer
-// ones are queued and sorted by a workgroup (bitonic network in LDS up to
-// 4096 elements, in global memory beyond).  If nonempty is not null, the
-// number of non-empty segments is added to it.  tmp holds
-// segsort_tmp_words(nseg) uint32.
-size_t segsort_tmp_words(uint64_t nseg);
-void segsort_u32(uint32_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
-                 hipStream_t s);
-void segsort_u64(uint64_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
-                 hipStream_t s);
 
 }  // namespace cdx

@@ -1,0 +1,510 @@
+// Narrowphase kernels (function_MPR_Store, Narrowphase.cu:316-504, and the
+// compaction of Narrowphase.cu:580-593).  Included by pipeline.hip only.
+#pragma once
+#include "cdx_device.hpp"
+
+namespace cdx {
+namespace {
+
+
+template <class R> struct Placed {
+  int type;
+  v3<R> X, B;
+  q4<R> q;
+};
+
+// The rotation matrix is rebuilt from the quaternion at every support call:
+// the same rounded values each time, and far fewer live registers than a
+// cached 3x3 per shape (the FP64 MPR is occupancy-bound).
+template <class R> __device__ __forceinline__ v3<R> sup(const Placed<R>& s, v3<R> b) {
+  return support_world(s.type, s.X, s.B, rotmat(s.q), b);
+}
+
+template <class R> __device__ __forceinline__ Placed<R> placed(const ShapeRec<R>& r) {
+  Placed<R> s;
+  s.type = (int)r.type;
+  s.X = mk3(r.x, r.y, r.z);
+  s.B = mk3(r.bx, r.by, r.bz);
+  s.q.w = r.qw; s.q.x = r.qx; s.q.y = r.qy; s.q.z = r.qz;
+  return s;
+}
+
+template <class R> __device__ __forceinline__ void swap3(v3<R>& a, v3<R>& b) { v3<R> t = a; a = b; b = t; }
+
+// CollideAndFindPoint (Narrowphase.cu:122-314), same evaluation order.
+template <class R>
+__device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rnormal, v3<R>& point, R& depth) {
+  v3<R> v01, v02, v0, n, v11, v12, v1, v21, v22, v2;
+  v01 = mk3<R>(0, 0, 0) + A.X;
+  v02 = mk3<R>(0, 0, 0) + B.X;
+  v0 = v02 - v01;
+  if (is_zero(v0)) v0 = mk3<R>(1, 0, 0);
+  n = -v0;
+  v11 = sup(A, -n);
+  v12 = sup(B, n);
+  v1 = v12 - v11;
+  if (dot(v1, n) <= R(0)) return false;
+  n = cross(v1, v0);
+  if (is_zero(n)) {
+    n = v1 - v0;
+    n = normalize(n);
+    rnormal = n;
+    point = (v11 + v12) * R(.5);
+    depth = dot((v12 - v11), n);
+    return true;
+  }
+  v21 = sup(A, -n);
+  v22 = sup(B, n);
+  v2 = v22 - v21;
+  if (dot(v2, n) <= R(0)) return false;
+  n = cross((v1 - v0), (v2 - v0));
+  if (dot(n, v0) > R(0)) {
+    swap3(v1, v2);
+    swap3(v11, v21);
+    swap3(v12, v22);
+    n = -n;
+  }
+  v3<R> v31, v32, v3_;
+  bool hit = false;
+  int phase1 = 0, phase2 = 0;
+  const int max_iterations = 34;
+  while (true) {
+    if (phase1 > max_iterations) return false;
+    phase1++;
+    v31 = sup(A, -n);
+    v32 = sup(B, n);
+    v3_ = v32 - v31;
+    if (dot(v3_, n) <= R(0)) return false;
+    if (dot(cross(v1, v3_), v0) < R(0)) {
+      v2 = v3_; v21 = v31; v22 = v32;
+      n = cross((v1 - v0), (v3_ - v0));
+      continue;
+    }
+    if (dot(cross(v3_, v2), v0) < R(0)) {
+      v1 = v3_; v11 = v31; v12 = v32;
+      n = cross((v3_ - v0), (v2 - v0));
+      continue;
+    }
+    while (true) {
+      phase2++;
+      n = cross((v2 - v1), (v3_ - v1));
+      n = normalize(n);
+      if (dot(n, v1) >= R(0) && !hit) hit = true;
+      v3<R> v41 = sup(A, -n);
+      v3<R> v42 = sup(B, n);
+      v3<R> v4 = v42 - v41;
+      R delta = dot((v4 - v3_), n);
+      depth = dot(v4, n);
+      if ((double)delta <= 1e-20 || depth <= R(0) || phase2 > max_iterations) {
+        if (hit) {
+          R b0 = dot(cross(v1, v2), v3_);
+          R b1 = dot(cross(v3_, v2), v0);
+          R b2 = dot(cross(v0, v1), v3_);
+          R b3 = dot(cross(v2, v1), v0);
+          R sum = b0 + b1 + b2 + b3;
+          if (sum <= R(0)) {
+            b0 = 0;
+            b1 = dot(cross(v2, v3_), n);
+            b2 = dot(cross(v3_, v1), n);
+            b3 = dot(cross(v1, v2), n);
+            sum = b1 + b2 + b3;
+          }
+          R inv = R(1) / sum;
+          point = (b0 * v01 + b1 * v11 + b2 * v21 + b3 * v31) + (b0 * v02 + b1 * v12 + b2 * v22 + b3 * v32);
+          point = point * inv * R(.5);
+          rnormal = normalize(n);
+        }
+        return hit;
+      }
+      if (dot(cross(v4, v1), v0) < R(0)) {
+        if (dot(cross(v4, v2), v0) < R(0)) { v1 = v4; v11 = v41; v12 = v42; }
+        else { v3_ = v4; v31 = v41; v32 = v42; }
+      } else {
+        if (dot(cross(v4, v3_), v0) < R(0)) { v2 = v4; v21 = v41; v22 = v42; }
+        else { v1 = v4; v11 = v41; v12 = v42; }
+      }
+    }
+  }
+}
+
+template <class R> struct Contact {
+  v3<R> N, p1, p2;
+  R depth;
+};
+
+// function_MPR_Store (Narrowphase.cu:316-423) for two shape records in the
+// scene's own precision.  Returns true on contact.
+// Sphere-sphere closed form (Narrowphase.cu:382-396) plus the envelope and
+// depth of :412-415.
+template <class R>
+__device__ __forceinline__ bool contact_spheres(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R envelope,
+                                                Contact<R>& out) {
+  v3<R> AX = mk3(ra.x, ra.y, ra.z), BX = mk3(rb.x, rb.y, rb.z);
+  v3<R> relpos = BX - AX;
+  R d2 = dot(relpos, relpos);
+  R cd = ra.bx + rb.bx;
+  if (!(d2 <= cd * cd)) return false;
+  v3<R> N = normalize(relpos);
+  v3<R> p1 = AX + N * ra.bx;
+  v3<R> p2 = BX - N * rb.bx;
+  p1 = p1 - N * envelope;
+  p2 = p2 + N * envelope;
+  out.N = N;
+  out.p1 = p1;
+  out.p2 = p2;
+  out.depth = dot(N, p2 - p1);
+  return true;
+}
+
+template <class R>
+__device__ bool contact_native(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R envelope, Contact<R>& out) {
+  if (!ra.active && !rb.active) return false;
+  if (ra.body == rb.body) return false;
+  v3<R> N = mk3<R>(1, 0, 0), p1 = mk3<R>(0, 0, 0), p2 = mk3<R>(0, 0, 0);
+  if (ra.type == SPHERE && rb.type == SPHERE) {
+    return contact_spheres(ra, rb, envelope, out);
+  } else {
+    Placed<R> A = placed(ra), B = placed(rb);
+    v3<R> p0 = mk3<R>(0, 0, 0);
+    R depth = 0;
+    if (!mpr_reference(A, B, N, p0, depth)) return false;
+    p1 = dot((sup(A, -N) - p0), N) * N + p0;
+    p2 = dot((sup(B, N) - p0), N) * N + p0;
+    N = -N;
+  }
+  p1 = p1 - N * envelope;
+  p2 = p2 + N * envelope;
+  out.N = N;
+  out.p1 = p1;
+  out.p2 = p2;
+  out.depth = dot(N, p2 - p1);
+  return true;
+}
+
+// Per-pair results before compaction: SoA of 10 reals, body ids, hit flag.
+template <class R> struct PairOut {
+  R* v;          // 10 * P: N(3) p1(3) p2(3) depth
+  int32_t* ids;  // 2 * P
+  uint32_t* hit; // P
+  uint64_t P;    // capacity: the stride of the SoA arrays
+};
+
+__device__ __forceinline__ uint64_t dev_count(uint64_t cap, const uint64_t* d_n) {
+  uint64_t v = *d_n;
+  return v < cap ? v : cap;
+}
+
+template <class R>
+__device__ __forceinline__ void store_pair(const PairOut<R>& o, uint64_t P, uint64_t i, const Contact<R>& c,
+                                           uint32_t ida, uint32_t idb) {
+  R* v = o.v;
+  v[i] = c.N.x; v[P + i] = c.N.y; v[2 * P + i] = c.N.z;
+  v[3 * P + i] = c.p1.x; v[4 * P + i] = c.p1.y; v[5 * P + i] = c.p1.z;
+  v[6 * P + i] = c.p2.x; v[7 * P + i] = c.p2.y; v[8 * P + i] = c.p2.z;
+  v[9 * P + i] = c.depth;
+  o.ids[2 * i] = (int32_t)ida;
+  o.ids[2 * i + 1] = (int32_t)idb;
+  o.hit[i] = 1u;
+}
+
+// ---- mixed precision: FP32 MPR in the frame of shape A, FP64 re-solve ----
+
+enum : int { MPR_MISS = 0, MPR_HIT = 1, MPR_UNSURE = 2 };
+
+__device__ __forceinline__ float bound_radius(int t, v3<float> b) {
+  switch (t) {
+    case SPHERE: return b.x;
+    case CAPSULE: return b.x + b.y;
+    case CYLINDER: return sqrtf(b.x * b.x + b.y * b.y);
+    default: return sqrtf(b.x * b.x + b.y * b.y + b.z * b.z);
+  }
+}
+
+// MPR of the reference's algorithm in FP32, with shape A at the origin.  It
+// returns MPR_UNSURE instead of a verdict whenever FP32 rounding could decide
+// it differently from FP64: a separation or penetration within tau of zero,
+// (nearly) coincident centres, a (nearly) degenerate portal, or an iteration
+// cap reached.
+__device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float tau, bool verdict_only,
+                           v3<float>& rnormal, v3<float>& point) {
+  typedef float R;
+  v3<R> v01, v02, v0, n, v11, v12, v1, v21, v22, v2;
+  v01 = A.X;
+  v02 = B.X;
+  v0 = v02 - v01;
+  if (length(v0) < 1e-4f * tau) return MPR_UNSURE;
+  n = -v0;
+  v11 = sup(A, -n);
+  v12 = sup(B, n);
+  v1 = v12 - v11;
+  R dn = dot(v1, n);
+  if (dn <= R(0)) return (-dn < tau * length(n)) ? MPR_UNSURE : MPR_MISS;
+  n = cross(v1, v0);
+  {
+    R nn = dot(n, n), s = 1e-6f * dot(v1, v1) * dot(v0, v0);
+    if (nn <= s) return MPR_UNSURE;
+  }
+  v21 = sup(A, -n);
+  v22 = sup(B, n);
+  v2 = v22 - v21;
+  dn = dot(v2, n);
+  if (dn <= R(0)) return (-dn < tau * length(n)) ? MPR_UNSURE : MPR_MISS;
+  n = cross((v1 - v0), (v2 - v0));
+  if (dot(n, v0) > R(0)) {
+    swap3(v1, v2);
+    swap3(v11, v21);
+    swap3(v12, v22);
+    n = -n;
+  }
+  v3<R> v31, v32, v3_;
+  bool hit = false;
+  int phase1 = 0, phase2 = 0;
+  const int max_iterations = 34;
+  while (true) {
+    if (phase1 > max_iterations) return MPR_UNSURE;
+    phase1++;
+    v31 = sup(A, -n);
+    v32 = sup(B, n);
+    v3_ = v32 - v31;
+    dn = dot(v3_, n);
+    if (dn <= R(0)) return (-dn < tau * length(n)) ? MPR_UNSURE : MPR_MISS;
+    if (dot(cross(v1, v3_), v0) < R(0)) {
+      v2 = v3_; v21 = v31; v22 = v32;
+      n = cross((v1 - v0), (v3_ - v0));
+      continue;
+    }
+    if (dot(cross(v3_, v2), v0) < R(0)) {
+      v1 = v3_; v11 = v31; v12 = v32;
+      n = cross((v3_ - v0), (v2 - v0));
+      continue;
+    }
+    while (true) {
+      phase2++;
+      n = cross((v2 - v1), (v3_ - v1));
+      n = normalize(n);
+      R side = dot(n, v1);
+      if (side >= R(0) && !hit) hit = true;
+      // The origin is inside the portal by more than tau: a certain hit.
+      if (verdict_only && side > tau) return MPR_HIT;
+      v3<R> v41 = sup(A, -n);
+      v3<R> v42 = sup(B, n);
+      v3<R> v4 = v42 - v41;
+      R delta = dot((v4 - v3_), n);
+      R depth = dot(v4, n);
+      if ((double)delta <= 1e-20 || depth <= R(0) || phase2 > max_iterations) {
+        if (phase2 > max_iterations) return MPR_UNSURE;
+        if (!hit) {
+          R margin = depth <= R(0) ? -depth : -side;
+          return margin < tau ? MPR_UNSURE : MPR_MISS;
+        }
+        R b0 = dot(cross(v1, v2), v3_);
+        R b1 = dot(cross(v3_, v2), v0);
+        R b2 = dot(cross(v0, v1), v3_);
+        R b3 = dot(cross(v2, v1), v0);
+        R sum = b0 + b1 + b2 + b3;
+        if (sum <= R(0)) {
+          b0 = 0;
+          b1 = dot(cross(v2, v3_), n);
+          b2 = dot(cross(v3_, v1), n);
+          b3 = dot(cross(v1, v2), n);
+          sum = b1 + b2 + b3;
+        }
+        R inv = R(1) / sum;
+        point = (b0 * v01 + b1 * v11 + b2 * v21 + b3 * v31) + (b0 * v02 + b1 * v12 + b2 * v22 + b3 * v32);
+        point = point * inv * R(.5);
+        rnormal = normalize(n);
+        return MPR_HIT;
+      }
+      if (dot(cross(v4, v1), v0) < R(0)) {
+        if (dot(cross(v4, v2), v0) < R(0)) { v1 = v4; v11 = v41; v12 = v42; }
+        else { v3_ = v4; v31 = v41; v32 = v42; }
+      } else {
+        if (dot(cross(v4, v3_), v0) < R(0)) { v2 = v4; v21 = v41; v22 = v42; }
+        else { v1 = v4; v11 = v41; v12 = v42; }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ Placed<float> placed_local(const ShapeRecF& r, const ShapeRecF& origin) {
+  Placed<float> s;
+  s.type = (int)r.type;
+  s.X = mk3((float)(r.x - origin.x), (float)(r.y - origin.y), (float)(r.z - origin.z));
+  s.B = mk3(r.bx, r.by, r.bz);
+  s.q.w = r.qw; s.q.x = r.qx; s.q.y = r.qy; s.q.z = r.qz;
+  return s;
+}
+
+__device__ __forceinline__ uint32_t type_slot(uint32_t t) { return t == CAPSULE ? 4u : t; }
+
+template <class R> __device__ __forceinline__ ShapeRec<double> widen(const ShapeRec<R>& r) {
+  ShapeRec<double> d;
+  d.x = r.x; d.y = r.y; d.z = r.z; d.qw = r.qw; d.qx = r.qx; d.qy = r.qy; d.qz = r.qz;
+  d.bx = r.bx; d.by = r.by; d.bz = r.bz; d.type = r.type; d.body = r.body; d.active = r.active; d.pad = 0;
+  return d;
+}
+template <> __device__ __forceinline__ ShapeRec<double> widen<double>(const ShapeRec<double>& r) { return r; }
+
+template <class CT, class R> __device__ __forceinline__ ShapeRec<CT> as_compute(const ShapeRec<R>& r);
+template <> __device__ __forceinline__ ShapeRec<double> as_compute<double, double>(const ShapeRec<double>& r) { return r; }
+template <> __device__ __forceinline__ ShapeRec<double> as_compute<double, float>(const ShapeRec<float>& r) { return widen(r); }
+template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(const ShapeRec<float>& r) { return r; }
+
+// Front pass over all candidate pairs (function_MPR_Store's early exits and
+// sphere-sphere closed form, Narrowphase.cu:338-396).  Pairs that need the
+// MPR are appended to a queue as (shape-pair class << 32 | pair index), one
+// atomic per wavefront; a one-digit radix pass then groups the queue by
+// class so every wavefront of the MPR kernel runs one pair of support maps.
+// MIXED / FAST first run an FP32 MPR in the frame of shape A: clear misses
+// end here, FAST keeps clear FP32 hits, everything else is queued for the
+// reference-order MPR.
+template <class R>
+__global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* __restrict__ d_np,
+                                                  const uint64_t* __restrict__ pairs,
+                                                  const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
+                                                  R envelope, PairOut<R> o, uint32_t* __restrict__ qsel,
+                                                  uint8_t* __restrict__ qcls) {
+  const uint64_t P = o.P;
+  uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (i >= dev_count(cap, d_np)) return;
+  uint64_t pr = pairs[i];
+  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+  const ShapeHdr ha = hdr[ia], hb = hdr[ib];
+  const uint32_t ta = ha.ta & 255u, tb = hb.ta & 255u;
+  o.hit[i] = 0u;
+  bool enq = false;
+  if (((ha.ta | hb.ta) >> 8) && ha.body != hb.body) {
+    if (ta == SPHERE && tb == SPHERE) {
+      Contact<R> c;
+      ShapeRec<R> ra = rec[ia], rb = rec[ib];
+      if (contact_spheres(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
+    } else {
+      enq = true;
+    }
+  }
+  qsel[i] = enq ? 1u : 0u;
+  qcls[i] = (uint8_t)(type_slot(ta) * 5u + type_slot(tb));
+}
+
+// FP32 MPR over the class-sorted queue (MIXED / FAST), in the frame of
+// shape A: clear misses end here; FAST keeps clear FP32 hits; everything
+// else is selected for the reference-order MPR.
+template <class R, bool KEEP>
+__global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t* __restrict__ d_q,
+                                                   const uint64_t* __restrict__ queue,
+                                                   const uint64_t* __restrict__ pairs,
+                                                   const ShapeHdr* __restrict__ hdr,
+                                                   const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
+                                                   PairOut<R> o, uint32_t* __restrict__ sel) {
+  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (j >= dev_count(cap, d_q)) return;
+  const uint64_t P = o.P;
+  const uint32_t i = (uint32_t)queue[j];
+  const uint64_t pr = pairs[i];
+  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+  const ShapeRecF fa = frec[ia], fb = frec[ib];
+  Placed<float> A = placed_local(fa, fa), B = placed_local(fb, fa);
+  float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
+  v3<float> N, p0;
+  int r = mpr_guarded(A, B, tau, !KEEP, N, p0);
+  if (KEEP && r == MPR_HIT) {
+    v3<float> p1 = dot((sup(A, -N) - p0), N) * N + p0;
+    v3<float> p2 = dot((sup(B, N) - p0), N) * N + p0;
+    N = -N;
+    float dl = dot(N, p2 - p1);
+    if (fabsf(dl) < tau) {
+      r = MPR_UNSURE;
+    } else {
+      Contact<R> c;
+      v3<double> Nd = mk3((double)N.x, (double)N.y, (double)N.z);
+      v3<double> o3 = mk3(fa.x, fa.y, fa.z);
+      v3<double> q1 = mk3((double)p1.x, (double)p1.y, (double)p1.z) + o3;
+      v3<double> q2 = mk3((double)p2.x, (double)p2.y, (double)p2.z) + o3;
+      q1 = q1 - Nd * (double)envelope;
+      q2 = q2 + Nd * (double)envelope;
+      c.N = mk3((R)Nd.x, (R)Nd.y, (R)Nd.z);
+      c.p1 = mk3((R)q1.x, (R)q1.y, (R)q1.z);
+      c.p2 = mk3((R)q2.x, (R)q2.y, (R)q2.z);
+      c.depth = (R)dot(Nd, q2 - q1);
+      store_pair(o, P, i, c, hdr[ia].body, hdr[ib].body);
+    }
+  }
+  // MIXED: FP32 settles clear misses only; hits take the reference path,
+  // whose capped iteration sequence FP32 cannot reproduce.
+  sel[j] = (r == MPR_UNSURE || (r == MPR_HIT && !KEEP)) ? 1u : 0u;
+}
+
+// Stable compaction of the selected queue entries (keeps the class order).
+__global__ void k_queue_select(uint64_t cap, const uint64_t* __restrict__ d_q, const uint64_t* __restrict__ queue,
+                               const uint32_t* __restrict__ sel, const uint32_t* __restrict__ pos,
+                               uint64_t* __restrict__ out) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= dev_count(cap, d_q) || !sel[j]) return;
+  out[pos[j]] = queue[j];
+}
+
+// Queue of the pairs that need the MPR, in pair order: slot = exclusive scan
+// of the selection flags (no same-address atomics).
+__global__ void k_queue_compact(uint64_t cap, const uint64_t* __restrict__ d_np, const uint32_t* __restrict__ qsel,
+                                const uint32_t* __restrict__ qpos, const uint8_t* __restrict__ qcls,
+                                uint64_t* __restrict__ queue) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dev_count(cap, d_np) || !qsel[i]) return;
+  queue[qpos[i]] = ((uint64_t)qcls[i] << 32) | i;
+}
+
+// MPR on the reference path for the queued pairs, in compute type CT
+// (the scene's real type for NATIVE, FP64 for MIXED / FAST).
+template <class R, class CT>
+__global__ __launch_bounds__(128) void k_np_resolve(uint64_t cap, const uint64_t* __restrict__ d_q,
+                                                    const uint64_t* __restrict__ pairs,
+                                                    const ShapeRec<R>* __restrict__ rec, R envelope, PairOut<R> o,
+                                                    const uint64_t* __restrict__ queue) {
+  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (j >= dev_count(cap, d_q)) return;
+  uint64_t P = o.P;
+  uint32_t i = (uint32_t)queue[j];
+  uint64_t pr = pairs[i];
+  ShapeRec<CT> ra = as_compute<CT, R>(rec[(uint32_t)(pr >> 32)]);
+  ShapeRec<CT> rb = as_compute<CT, R>(rec[(uint32_t)(pr & 0xffffffffu)]);
+  Contact<CT> c;
+  if (contact_native(ra, rb, (CT)envelope, c)) {
+    Contact<R> cr;
+    cr.N = mk3((R)c.N.x, (R)c.N.y, (R)c.N.z);
+    cr.p1 = mk3((R)c.p1.x, (R)c.p1.y, (R)c.p1.z);
+    cr.p2 = mk3((R)c.p2.x, (R)c.p2.y, (R)c.p2.z);
+    cr.depth = (R)c.depth;
+    store_pair(o, P, i, cr, ra.body, rb.body);
+  }
+}
+
+// Stable partition of hits to the front (Narrowphase.cu:580-593): contacts
+// in ascending pair order, and the pair list permuted hits first.
+template <class R>
+__global__ void k_np_compact(uint64_t cap, const uint64_t* __restrict__ d_np, PairOut<R> o,
+                             const uint32_t* __restrict__ pos,
+                             const uint64_t* __restrict__ pairs, const DevStatus* __restrict__ st, R* __restrict__ norm,
+                             R* __restrict__ cpta, R* __restrict__ cptb, R* __restrict__ dpth, int32_t* __restrict__ bids,
+                             uint64_t* __restrict__ perm) {
+  const uint64_t P = o.P;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dev_count(cap, d_np)) return;
+  uint64_t C = st->C;
+  uint32_t k = pos[i];
+  if (o.hit[i]) {
+    const R* v = o.v;
+    norm[3 * k] = v[i]; norm[3 * k + 1] = v[P + i]; norm[3 * k + 2] = v[2 * P + i];
+    cpta[3 * k] = v[3 * P + i]; cpta[3 * k + 1] = v[4 * P + i]; cpta[3 * k + 2] = v[5 * P + i];
+    cptb[3 * k] = v[6 * P + i]; cptb[3 * k + 1] = v[7 * P + i]; cptb[3 * k + 2] = v[8 * P + i];
+    dpth[k] = v[9 * P + i];
+    bids[2 * k] = o.ids[2 * i];
+    bids[2 * k + 1] = o.ids[2 * i + 1];
+    perm[k] = pairs[i];
+  } else {
+    perm[C + (i - k)] = pairs[i];
+  }
+}
+
+
+}  // namespace
+}  // namespace cdx

@@ -2,6 +2,8 @@
 #include "cdx_prims.hpp"
 #include "cdx_util.hpp"
 
+#include <algorithm>
+
 namespace cdx {
 
 namespace {
@@ -16,9 +18,16 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 }
 
 // Per-tile sums (64-bit, so a tile of large counts cannot wrap).
-__global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
-                                                         uint64_t* __restrict__ part) {
+__device__ __forceinline__ uint64_t eff_n(uint64_t cap, const uint64_t* d_n) {
+  if (!d_n) return cap;
+  uint64_t v = *d_n;
+  return v < cap ? v : cap;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t cap,
+                                                         const uint64_t* __restrict__ d_n, uint64_t* __restrict__ part) {
   __shared__ uint64_t ws[SCAN_BS / WAVE];
+  const uint64_t n = eff_n(cap, d_n);
   uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
   uint64_t s = 0;
 #pragma unroll
@@ -71,8 +80,10 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t* __restrict__ p
 
 // Tile scan: each thread owns SCAN_IPT consecutive elements.
 __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                        uint64_t n, const uint64_t* __restrict__ part) {
+                                                        uint64_t cap, const uint64_t* __restrict__ d_n,
+                                                        const uint64_t* __restrict__ part) {
   __shared__ uint32_t scratch[SCAN_BS / WAVE + 1];
+  const uint64_t n = eff_n(cap, d_n);
   uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
   uint32_t v[SCAN_IPT];
   uint32_t s = 0;
@@ -93,6 +104,13 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const uint32_t* __restri
   }
 }
 
+__global__ void k_copy_u64(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, uint64_t cap,
+                           const uint64_t* __restrict__ d_n) {
+  const uint64_t n = eff_n(cap, d_n);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
@@ -103,9 +121,11 @@ constexpr int RS_IPT = 8;
 constexpr int RS_TILE = RS_BS * RS_IPT;
 constexpr int RS_NW = RS_BS / WAVE;
 
-__global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restrict__ in, uint64_t n, int shift,
+__global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restrict__ in, uint64_t cap,
+                                                         const uint64_t* __restrict__ d_n, int shift,
                                                          uint32_t* __restrict__ hist, uint32_t nb) {
   __shared__ uint32_t h[256];
+  const uint64_t n = eff_n(cap, d_n);
   h[threadIdx.x] = 0;
   __syncthreads();
   uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
@@ -123,9 +143,10 @@ __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restr
 // wavefront find their equal-digit peers with 8 ballots, and per-wavefront
 // digit counts are prefix-summed across wavefronts in LDS.
 __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
-                                                           uint64_t n, int shift, const uint32_t* __restrict__ offs,
-                                                           uint32_t nb) {
+                                                           uint64_t cap, const uint64_t* __restrict__ d_n, int shift,
+                                                           const uint32_t* __restrict__ offs, uint32_t nb) {
   __shared__ uint32_t base[256];
+  const uint64_t n = eff_n(cap, d_n);
   __shared__ uint32_t wcnt[RS_NW][256];
   const int w = threadIdx.x / WAVE;
   base[threadIdx.x] = offs[(uint64_t)threadIdx.x * nb + blockIdx.x];
@@ -166,131 +187,24 @@ __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __res
 }
 
 
-constexpr int SEG_SMALL = 32;
-constexpr int SEG_LDS = 4096;
-
-template <class T>
-__global__ __launch_bounds__(256) void k_seg_small(T* __restrict__ data, const uint32_t* __restrict__ off,
-                                                   uint64_t nseg, uint32_t* __restrict__ tmp,
-                                                   unsigned long long* nonempty) {
-  uint64_t s = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  bool ne = false;
-  if (s < nseg) {
-    uint32_t b = off[s], e = off[s + 1];
-    ne = e > b;
-    if (e - b > (uint32_t)SEG_SMALL) {
-      uint32_t q = atomicAdd(&tmp[0], 1u);
-      tmp[1 + q] = (uint32_t)s;
-    } else {
-      for (uint32_t i = b + 1; i < e; ++i) {
-        T v = data[i];
-        uint32_t j = i;
-        while (j > b && data[j - 1] > v) {
-          data[j] = data[j - 1];
-          --j;
-        }
-        data[j] = v;
-      }
-    }
-  }
-  if (nonempty) {
-    uint64_t m = __ballot(ne);
-    if (m && __lane_id() == (uint32_t)(__ffsll((long long)m) - 1)) atomicAdd(nonempty, (unsigned long long)__popcll(m));
-  }
-}
-
-// Ascending bitonic network for any n: a mirrored compare at each block size,
-// then half-cleaners.  Indices >= n act as +infinity and are never swapped.
-template <class T>
-__device__ __forceinline__ void bitonic_block(T* x, uint32_t n) {
-  for (uint32_t p = 1; p < n; p <<= 1) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-      if (!(i & p)) {
-        uint32_t j = i ^ (2 * p - 1);
-        if (j < n) {
-          T a = x[i], b = x[j];
-          if (b < a) { x[i] = b; x[j] = a; }
-        }
-      }
-    }
-    __syncthreads();
-    for (uint32_t q = p >> 1; q > 0; q >>= 1) {
-      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        if (!(i & q)) {
-          uint32_t j = i | q;
-          if (j < n) {
-            T a = x[i], b = x[j];
-            if (b < a) { x[i] = b; x[j] = a; }
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_seg_large(T* __restrict__ data, const uint32_t* __restrict__ off,
-                                                   const uint32_t* __restrict__ tmp) {
-  __shared__ T lds[SEG_LDS];
-  const uint32_t nq = tmp[0];
-  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-    uint32_t sid = tmp[1 + q];
-    uint32_t b = off[sid], n = off[sid + 1] - b;
-    if (n <= (uint32_t)SEG_LDS) {
-      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[i] = data[b + i];
-      __syncthreads();
-      bitonic_block(lds, n);
-      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) data[b + i] = lds[i];
-    } else {
-      bitonic_block(data + b, n);
-    }
-    __syncthreads();
-  }
-}
-
-__global__ void k_zero_word(uint32_t* p) {
-  if (threadIdx.x == 0) p[0] = 0;
-}
-
-template <class T>
-void segsort_impl(T* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty, hipStream_t s) {
-  if (nseg == 0) return;
-  hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, s, tmp);
-  uint64_t blocks = (nseg + 255) / 256;
-  hipLaunchKernelGGL((k_seg_small<T>), dim3((uint32_t)blocks), dim3(256), 0, s, data, off, nseg, tmp,
-                     (unsigned long long*)nonempty);
-  hipLaunchKernelGGL((k_seg_large<T>), dim3(512), dim3(256), 0, s, data, off, (const uint32_t*)tmp);
-}
-
 }  // namespace
-
-size_t segsort_tmp_words(uint64_t nseg) { return nseg + 2; }
-
-void segsort_u32(uint32_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
-                 hipStream_t s) {
-  segsort_impl(data, off, nseg, tmp, nonempty, s);
-}
-void segsort_u64(uint64_t* data, const uint32_t* off, uint64_t nseg, uint32_t* tmp, uint64_t* nonempty,
-                 hipStream_t s) {
-  segsort_impl(data, off, nseg, tmp, nonempty, s);
-}
 
 size_t scan_tmp_words(uint64_t n) {
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   return 2 * (nb + 1);  // uint64 partials
 }
 
-void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s) {
+void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s,
+                   const uint64_t* d_n) {
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   uint64_t* part = reinterpret_cast<uint64_t*>(tmp);
   if (n == 0) {
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, (uint64_t)0, d_total);
     return;
   }
-  hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, n, part);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, n, d_n, part);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, nb, d_total);
-  hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, (const uint64_t*)part);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, d_n, (const uint64_t*)part);
 }
 
 size_t radix_tmp_words(uint64_t n) {
@@ -300,7 +214,7 @@ size_t radix_tmp_words(uint64_t n) {
 }
 
 void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
-                    hipStream_t s) {
+                    hipStream_t s, const uint64_t* d_n) {
   if (n <= 1 || nshift == 0) return;
   uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
   uint64_t h = 256ull * nb;
@@ -310,13 +224,16 @@ void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts
   uint64_t* src = data;
   uint64_t* dst = tmp;
   for (int p = 0; p < nshift; ++p) {
-    hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, n, shifts[p], hist, nb);
-    scan_excl_u32(hist, hist, h, total, stmp, s);
-    hipLaunchKernelGGL(k_radix_downsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, dst, n, shifts[p],
+    hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, n, d_n, shifts[p], hist, nb);
+    scan_excl_u32(hist, hist, h, total, stmp, s, nullptr);
+    hipLaunchKernelGGL(k_radix_downsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, dst, n, d_n, shifts[p],
                        (const uint32_t*)hist, nb);
     uint64_t* t = src; src = dst; dst = t;
   }
-  if (src != data) CDX_HIP_CHECK_VOID(hipMemcpyAsync(data, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+  if (src != data) {
+    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy_u64, dim3((uint32_t)blocks), dim3(256), 0, s, (const uint64_t*)src, data, n, d_n);
+  }
 }
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
