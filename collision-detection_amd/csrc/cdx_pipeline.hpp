@@ -98,6 +98,8 @@ class BatchBase {
   // Scenes of the following steps (null = all resident scenes).
   virtual void select(const uint32_t* scenes, uint64_t k) = 0;
   virtual uint64_t selected_count() const = 0;
+  // Bin entries and pairs per selected shape the step buffers are sized for.
+  virtual void set_capacity_factors(double m_per_shape, double p_per_shape) = 0;
   virtual void step(bool use_graph) = 0;  // enqueue AABB + broadphase + narrowphase
   virtual void finish() = 0;              // wait, re-run on capacity overflow, read totals
   virtual const StepCounts& totals() const = 0;

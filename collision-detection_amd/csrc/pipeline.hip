@@ -443,6 +443,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   uint64_t selected_count() const override { return selection_.size(); }
 
+  void set_capacity_factors(double m_per_shape, double p_per_shape) override {
+    CDX_REQUIRE(m_per_shape > 0 && p_per_shape > 0, ERR_INVALID, "capacity factors must be positive");
+    cap_m_per_shape_ = m_per_shape;
+    cap_p_per_shape_ = p_per_shape;
+  }
+
   void step(bool use_graph) override {
     CDX_REQUIRE(!scenes_.empty(), ERR_NOT_READY, "no resident scenes");
     if (selection_dirty_ || !explicit_selection_) {
@@ -455,7 +461,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
     exact_ = false;
     aabb_from_caller_ = false;
-    if (mcap_ == 0 || pcap_ == 0 || caps_for_ != total_selected_shapes()) estimate_capacities();
+    ensure_capacities();
     launch_capacity_step(use_graph);
     pending_step_ = true;
     pending_graph_ = use_graph;
@@ -481,6 +487,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     CDX_REQUIRE(hstat_->overflow == 0 && hstat_->M <= mcap_ && hstat_->P <= pcap_, ERR_CAPACITY,
                 "step totals exceed the buffer capacities");
     M_ = hstat_->M; A_ = hstat_->A; P_ = hstat_->P; C_ = hstat_->C;
+    m_seen_ = std::max(m_seen_, M_);
+    p_seen_ = std::max(p_seen_, P_);
     souts_host_.resize(selection_.size());
     if (!selection_.empty())
       CDX_HIP_CHECK(hipMemcpy(souts_host_.data(), souts_.p, selection_.size() * sizeof(SceneOut),
@@ -692,6 +700,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     nchunks_ = ch.size();
     nslots_ = nchunks_ * CHUNK;
     nsel_ = (uint32_t)sel.size();
+    // radix digit counts from every resident scene, so a recorded step
+    // stays valid whichever scenes a later step selects
+    for (const HostScene& sc : scenes_) {
+      ks_total = std::max<uint64_t>(ks_total, sc.key_base + sc.key_space);
+      nshapes = std::max<uint64_t>(nshapes, sc.shape_begin + sc.n_shapes);
+    }
     key_bits_ = bits_for(ks_total ? ks_total - 1 : 0);
     shape_bits_ = bits_for(nshapes ? nshapes - 1 : 0);
     if (nchunks_ > chunk_cap_ || !chunks_.p) {
@@ -942,14 +956,15 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   }
 
   // ------------------------------------------------------ capacity steps --
-  void estimate_capacities() {
+  // Capacities only grow (a recorded step stays valid while they hold):
+  // entries and pairs per shape from the capacity factors, or from the
+  // largest totals measured so far.
+  void ensure_capacities() {
     const uint64_t n = total_selected_shapes();
-    // bin entries and pairs per shape of the mixed workloads are ~3 and ~2;
-    // start above that and grow from measured totals on overflow
-    mcap_ = std::max<uint64_t>(std::max<uint64_t>(mcap_, 6 * n), 4096);
-    pcap_ = std::max<uint64_t>(std::max<uint64_t>(pcap_, 4 * n), 4096);
-    caps_for_ = n;
-    graph_valid_ = false;
+    uint64_t need_m = std::max<uint64_t>((uint64_t)(cap_m_per_shape_ * n) + 4096, m_seen_ + m_seen_ / 4);
+    uint64_t need_p = std::max<uint64_t>((uint64_t)(cap_p_per_shape_ * n) + 4096, p_seen_ + p_seen_ / 4);
+    if (need_m > mcap_) { mcap_ = need_m; graph_valid_ = false; }
+    if (need_p > pcap_) { pcap_ = need_p; graph_valid_ = false; }
   }
 
   void enqueue_capacity_step() {
@@ -1075,7 +1090,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   int key_bits_ = 0, shape_bits_ = 0;
 
   // capacities and totals
-  uint64_t mcap_ = 0, pcap_ = 0, caps_for_ = 0;
+  uint64_t mcap_ = 0, pcap_ = 0, m_seen_ = 0, p_seen_ = 0;
+  double cap_m_per_shape_ = 6.0, cap_p_per_shape_ = 4.0;
   uint64_t M_ = 0, A_ = 0, P_ = 0, C_ = 0;
   StepCounts counts_;
   std::vector<SceneOut> souts_host_;
