@@ -39,7 +39,11 @@ EXPORTS = [
     "cdx_scene_set_profiling", "cdx_scene_aabb", "cdx_scene_broadphase", "cdx_scene_narrowphase",
     "cdx_scene_step", "cdx_scene_sync", "cdx_scene_counts", "cdx_scene_read_aabb", "cdx_scene_read_bins",
     "cdx_scene_read_pairs", "cdx_scene_read_pairs_permuted", "cdx_scene_read_contacts", "cdx_scene_stage_ms",
-    "cdx_scene_device_bytes",
+    "cdx_scene_device_bytes", "cdx_batch_create", "cdx_batch_destroy", "cdx_batch_last_error",
+    "cdx_batch_add_scene", "cdx_batch_scene_bins", "cdx_batch_scene_count", "cdx_batch_update_bodies",
+    "cdx_batch_set_envelope", "cdx_batch_set_narrowphase_mode", "cdx_batch_select", "cdx_batch_step",
+    "cdx_batch_finish", "cdx_batch_scene_counts", "cdx_batch_read_pairs", "cdx_batch_read_contacts",
+    "cdx_batch_step_ms", "cdx_batch_device_bytes", "cdx_batch_scene_bytes",
 ]
 
 
@@ -113,6 +117,24 @@ def lib():
         "cdx_scene_read_contacts": ([vp, vp, vp, vp, vp, vp], i32),
         "cdx_scene_stage_ms": ([vp, vp], i32),
         "cdx_scene_device_bytes": ([vp, vp], i32),
+        "cdx_batch_create": ([vp, i32, vp], i32),
+        "cdx_batch_destroy": ([vp], i32),
+        "cdx_batch_last_error": ([vp], C.c_char_p),
+        "cdx_batch_add_scene": ([vp, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, i32, i32, i32, vp], i32),
+        "cdx_batch_scene_bins": ([vp, C.c_uint32, vp], i32),
+        "cdx_batch_scene_count": ([vp, vp], i32),
+        "cdx_batch_update_bodies": ([vp, C.c_uint32, u64, vp, vp, vp], i32),
+        "cdx_batch_set_envelope": ([vp, C.c_double], i32),
+        "cdx_batch_set_narrowphase_mode": ([vp, i32, C.c_double], i32),
+        "cdx_batch_select": ([vp, vp, u64], i32),
+        "cdx_batch_step": ([vp, i32], i32),
+        "cdx_batch_finish": ([vp, vp], i32),
+        "cdx_batch_scene_counts": ([vp, u64, vp], i32),
+        "cdx_batch_read_pairs": ([vp, u64, vp, u64], i32),
+        "cdx_batch_read_contacts": ([vp, u64, vp, vp, vp, vp, vp, u64], i32),
+        "cdx_batch_step_ms": ([vp, vp], i32),
+        "cdx_batch_device_bytes": ([vp, vp], i32),
+        "cdx_batch_scene_bytes": ([vp, u64, u64, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -311,4 +333,111 @@ class Scene:
     def device_bytes(self) -> int:
         b = C.c_uint64(0)
         self._chk(lib().cdx_scene_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+
+class Batch:
+    """Many device-resident scenes stepped together (cdx_batch)."""
+
+    def __init__(self, ctx: Context, dtype=np.float64):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.h = C.c_void_p()
+        ctx._chk(lib().cdx_batch_create(ctx.h, _kind(dtype), C.byref(self.h)))
+        self._scenes = []
+        self._selection = None
+
+    def close(self):
+        if self.h:
+            lib().cdx_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def _chk(self, st):
+        if st:
+            raise CdxError(st, lib().cdx_batch_last_error(self.h).decode())
+
+    def add_scene(self, s, bins=None) -> int:
+        """Makes scene `s` resident; bins=None uses s.grid, bins=(0,0,0) autotunes."""
+        assert np.dtype(s.dtype) == self.dtype
+        g = s.grid if bins is None else bins
+        out = C.c_uint32(0)
+        self._chk(lib().cdx_batch_add_scene(self.h, s.n, _p(s.T), _p(s.A), _p(s.B), _p(s.C), _p(s.R), _p(s.ID),
+                                            s.nb, _p(s.pos), _p(s.rot), _p(s.active), int(g[0]), int(g[1]),
+                                            int(g[2]), C.byref(out)))
+        self._scenes.append(s.n)
+        return out.value
+
+    def scene_bins(self, scene):
+        g = np.zeros(3, np.int32)
+        self._chk(lib().cdx_batch_scene_bins(self.h, int(scene), _p(g)))
+        return tuple(int(v) for v in g)
+
+    def scene_count(self) -> int:
+        c = C.c_uint64(0)
+        self._chk(lib().cdx_batch_scene_count(self.h, C.byref(c)))
+        return c.value
+
+    def update_bodies(self, scene, idx, pos, rot):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        pos = np.ascontiguousarray(pos, dtype=self.dtype)
+        rot = np.ascontiguousarray(rot, dtype=self.dtype)
+        self._chk(lib().cdx_batch_update_bodies(self.h, int(scene), len(idx), _p(idx), _p(pos), _p(rot)))
+
+    def set_envelope(self, e):
+        self._chk(lib().cdx_batch_set_envelope(self.h, float(e)))
+
+    def set_mode(self, mode, tau=1e-2):
+        self._chk(lib().cdx_batch_set_narrowphase_mode(self.h, int(mode), float(tau)))
+
+    def select(self, scenes=None):
+        if scenes is None:
+            self._chk(lib().cdx_batch_select(self.h, None, 0))
+            self._selection = None
+        else:
+            a = np.ascontiguousarray(scenes, dtype=np.uint32)
+            self._chk(lib().cdx_batch_select(self.h, _p(a), len(a)))
+            self._selection = a
+
+    def step(self, use_graph=True):
+        self._chk(lib().cdx_batch_step(self.h, 1 if use_graph else 0))
+
+    def finish(self) -> Counts:
+        c = Counts()
+        self._chk(lib().cdx_batch_finish(self.h, C.byref(c)))
+        return c
+
+    def scene_counts(self, j) -> Counts:
+        c = Counts()
+        self._chk(lib().cdx_batch_scene_counts(self.h, int(j), C.byref(c)))
+        return c
+
+    def read_pairs(self, j):
+        out = np.zeros(self.scene_counts(j).pairs, np.int64)
+        self._chk(lib().cdx_batch_read_pairs(self.h, int(j), _p(out), len(out)))
+        return out
+
+    def read_contacts(self, j):
+        k = self.scene_counts(j).contacts
+        dt = self.dtype
+        norm, pa, pb = (np.zeros((k, 3), dt) for _ in range(3))
+        d = np.zeros(k, dt)
+        ids = np.zeros((k, 2), np.int32)
+        self._chk(lib().cdx_batch_read_contacts(self.h, int(j), _p(norm), _p(pa), _p(pb), _p(d), _p(ids), k))
+        return norm, pa, pb, d, ids
+
+    def step_ms(self) -> float:
+        v = C.c_double(0)
+        self._chk(lib().cdx_batch_step_ms(self.h, C.byref(v)))
+        return v.value
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64(0)
+        self._chk(lib().cdx_batch_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def scene_bytes(self, n, nb) -> int:
+        b = C.c_uint64(0)
+        self._chk(lib().cdx_batch_scene_bytes(self.h, int(n), int(nb), C.byref(b)))
         return b.value

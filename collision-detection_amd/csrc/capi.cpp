@@ -1,11 +1,15 @@
 // extern "C" layer of libcdx.so (declarations in include/cdx.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/cdx.h"
+#include "autotune.hpp"
 #include "cdx_pipeline.hpp"
 #include "cdx_util.hpp"
 
@@ -21,6 +25,16 @@ struct cdx_ctx {
 struct cdx_scene {
   cdx_ctx* ctx = nullptr;
   std::unique_ptr<cdx::SceneBase> s;
+  std::string err;
+};
+
+struct cdx_batch {
+  cdx_ctx* ctx = nullptr;
+  int kind = 0;
+  std::unique_ptr<cdx::BatchBase> b;
+  std::unique_ptr<cdx::Autotuner> tuner;
+  std::vector<std::array<int, 3>> bins;
+  double m_per_shape = 0.0, p_per_shape = 0.0;  // largest measured by the autotuner
   std::string err;
 };
 
@@ -41,6 +55,12 @@ template <class F> int guarded(std::string* err, F&& f) {
     if (err) *err = e.what();
     return CDX_ERR_INVALID;
   }
+}
+
+void fill_counts(const cdx::StepCounts& c, cdx_counts* out) {
+  out->n_shapes = c.n; out->bin_entries = c.m; out->active_bins = c.a; out->pairs = c.p;
+  out->contacts = c.c; out->resolved = c.resolved; out->key_max = c.key_max; out->flags = c.bad;
+  for (int i = 0; i < 3; ++i) { out->bmin[i] = c.bmin[i]; out->bmax[i] = c.bmax[i]; }
 }
 
 void check_kind(int k) {
@@ -245,10 +265,7 @@ int cdx_scene_narrowphase(cdx_scene* scene) { SCENE_CALL(s.run_narrowphase()); }
 int cdx_scene_step(cdx_scene* scene) { SCENE_CALL(s.run_aabb(); s.run_broadphase(); s.run_narrowphase()); }
 int cdx_scene_sync(cdx_scene* scene) { SCENE_CALL(s.sync()); }
 int cdx_scene_counts(cdx_scene* scene, cdx_counts* out) {
-  SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); const cdx::StepCounts& c = s.counts();
-             out->n_shapes = c.n; out->bin_entries = c.m; out->active_bins = c.a; out->pairs = c.p;
-             out->contacts = c.c; out->resolved = c.resolved; out->key_max = c.key_max; out->flags = c.bad;
-             for (int i = 0; i < 3; ++i) { out->bmin[i] = c.bmin[i]; out->bmax[i] = c.bmax[i]; });
+  SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); fill_counts(s.counts(), out));
 }
 int cdx_scene_read_aabb(cdx_scene* scene, int shifted, void* out) {
   SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); s.read_aabb(out, shifted != 0));
@@ -267,6 +284,106 @@ int cdx_scene_stage_ms(cdx_scene* scene, double* ms) {
 }
 int cdx_scene_device_bytes(cdx_scene* scene, uint64_t* bytes) {
   SCENE_CALL(CDX_REQUIRE(bytes, CDX_ERR_INVALID, "null output"); *bytes = s.device_bytes());
+}
+
+// ---- batches --------------------------------------------------------------
+
+int cdx_batch_create(cdx_ctx* ctx, int real_kind, cdx_batch** out) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    check_kind(real_kind);
+    CDX_REQUIRE(out, CDX_ERR_INVALID, "null output");
+    auto bt = std::make_unique<cdx_batch>();
+    bt->ctx = ctx;
+    bt->kind = real_kind;
+    bt->b = cdx::make_batch(real_kind, ctx->device);
+    *out = bt.release();
+  });
+}
+
+int cdx_batch_destroy(cdx_batch* batch) {
+  delete batch;
+  return CDX_OK;
+}
+
+const char* cdx_batch_last_error(const cdx_batch* batch) { return batch ? batch->err.c_str() : "null batch"; }
+
+#define BATCH_CALL(...)                                      \
+  if (!batch) return CDX_ERR_INVALID;                        \
+  return guarded(&batch->err, [&] { cdx::BatchBase& b = *batch->b; __VA_ARGS__; })
+
+int cdx_batch_add_scene(cdx_batch* batch, uint64_t n, const int32_t* T, const void* A, const void* B, const void* C,
+                        const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos, const void* body_rot,
+                        const uint8_t* obj_active, int bins_x, int bins_y, int bins_z, uint32_t* scene_out) {
+  (void)C;
+  BATCH_CALL({
+    CDX_REQUIRE(n > 0 && nb > 0, CDX_ERR_INVALID, "a scene needs shapes and bodies");
+    CDX_REQUIRE(T && A && B && R && ID && body_pos && body_rot, CDX_ERR_INVALID, "null input");
+    int g[3] = {bins_x, bins_y, bins_z};
+    if (bins_x == 0 && bins_y == 0 && bins_z == 0) {
+      if (!batch->tuner) batch->tuner = std::make_unique<cdx::Autotuner>(batch->ctx->device, batch->kind);
+      cdx::TuneChoice t = batch->tuner->choose(n, T, A, B, R, ID, nb, body_pos, body_rot, obj_active);
+      for (int k = 0; k < 3; ++k) g[k] = t.grid[k];
+      if (t.m_per_shape > batch->m_per_shape || t.p_per_shape > batch->p_per_shape) {
+        batch->m_per_shape = std::max(batch->m_per_shape, t.m_per_shape);
+        batch->p_per_shape = std::max(batch->p_per_shape, t.p_per_shape);
+        b.set_capacity_factors(batch->m_per_shape * 1.25 + 1.0, batch->p_per_shape * 1.25 + 1.0);
+      }
+    }
+    uint32_t s = b.add_scene(n, T, A, B, R, ID, nb, body_pos, body_rot, obj_active, g[0], g[1], g[2]);
+    if (batch->bins.size() <= s) batch->bins.resize(s + 1);
+    batch->bins[s] = {g[0], g[1], g[2]};
+    if (scene_out) *scene_out = s;
+  });
+}
+
+int cdx_batch_scene_bins(cdx_batch* batch, uint32_t scene, int* bins_xyz) {
+  BATCH_CALL((void)b; CDX_REQUIRE(bins_xyz, CDX_ERR_INVALID, "null output");
+             CDX_REQUIRE(scene < batch->bins.size(), CDX_ERR_INVALID, "scene index out of range");
+             for (int k = 0; k < 3; ++k) bins_xyz[k] = batch->bins[scene][k]);
+}
+int cdx_batch_scene_count(cdx_batch* batch, uint64_t* count) {
+  BATCH_CALL(CDX_REQUIRE(count, CDX_ERR_INVALID, "null output"); *count = b.scene_count());
+}
+int cdx_batch_update_bodies(cdx_batch* batch, uint32_t scene, uint64_t k, const uint32_t* idx, const void* body_pos,
+                            const void* body_rot) {
+  BATCH_CALL(CDX_REQUIRE(k == 0 || (idx && body_pos && body_rot), CDX_ERR_INVALID, "null input");
+             b.update_scene_bodies(scene, k, idx, body_pos, body_rot));
+}
+int cdx_batch_set_envelope(cdx_batch* batch, double envelope) { BATCH_CALL(b.set_envelope(envelope)); }
+int cdx_batch_set_narrowphase_mode(cdx_batch* batch, int np_mode, double tau) {
+  BATCH_CALL(CDX_REQUIRE(tau >= 0.0, CDX_ERR_INVALID, "tau must be non-negative"); b.set_np_mode(np_mode, tau));
+}
+int cdx_batch_select(cdx_batch* batch, const uint32_t* scenes, uint64_t k) {
+  BATCH_CALL(CDX_REQUIRE(scenes || k == 0, CDX_ERR_INVALID, "null scene list"); b.select(scenes, k));
+}
+int cdx_batch_step(cdx_batch* batch, int use_graph) { BATCH_CALL(b.step(use_graph != 0)); }
+int cdx_batch_finish(cdx_batch* batch, cdx_counts* totals) {
+  BATCH_CALL(b.finish(); if (totals) fill_counts(b.totals(), totals));
+}
+int cdx_batch_scene_counts(cdx_batch* batch, uint64_t j, cdx_counts* out) {
+  BATCH_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); cdx::StepCounts c; b.scene_counts(j, &c);
+             fill_counts(c, out));
+}
+int cdx_batch_read_pairs(cdx_batch* batch, uint64_t j, int64_t* pairs_out, uint64_t capacity) {
+  BATCH_CALL(cdx::StepCounts c; b.scene_counts(j, &c);
+             CDX_REQUIRE(capacity >= c.p, CDX_ERR_CAPACITY, "pair buffer too small: need " + std::to_string(c.p));
+             CDX_REQUIRE(c.p == 0 || pairs_out, CDX_ERR_INVALID, "null output"); b.read_scene_pairs(j, pairs_out));
+}
+int cdx_batch_read_contacts(cdx_batch* batch, uint64_t j, void* norm, void* cpta, void* cptb, void* dpth,
+                            int32_t* bids, uint64_t capacity) {
+  BATCH_CALL(cdx::StepCounts c; b.scene_counts(j, &c);
+             CDX_REQUIRE(capacity >= c.c, CDX_ERR_CAPACITY, "contact buffers too small: need " + std::to_string(c.c));
+             b.read_scene_contacts(j, norm, cpta, cptb, dpth, bids));
+}
+int cdx_batch_step_ms(cdx_batch* batch, double* ms) {
+  BATCH_CALL(CDX_REQUIRE(ms, CDX_ERR_INVALID, "null output"); *ms = b.last_step_ms());
+}
+int cdx_batch_device_bytes(cdx_batch* batch, uint64_t* bytes) {
+  BATCH_CALL(CDX_REQUIRE(bytes, CDX_ERR_INVALID, "null output"); *bytes = b.device_bytes());
+}
+int cdx_batch_scene_bytes(cdx_batch* batch, uint64_t n, uint64_t nb, uint64_t* bytes) {
+  BATCH_CALL(CDX_REQUIRE(bytes, CDX_ERR_INVALID, "null output"); *bytes = b.scene_bytes_estimate(n, nb));
 }
 
 }  // extern "C"

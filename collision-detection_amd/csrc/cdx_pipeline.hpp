@@ -108,10 +108,35 @@ class BatchBase {
   virtual void read_scene_pairs(uint64_t j, int64_t* out) = 0;
   virtual void read_scene_contacts(uint64_t j, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) = 0;
   virtual double last_step_ms() = 0;
+  // Serving helpers.  Global body index of a scene's first body.
+  virtual uint64_t scene_body_begin(uint32_t scene) const = 0;
+  // Pose updates of bodies anywhere in the batch (global body indices),
+  // from pinned host memory that stays untouched until finish() returns:
+  // the copies are asynchronous on the batch stream.
+  // The block holds k uint32 ids, then k real3 positions at byte offset
+  // pose_block_pos_offset(k), then k real4 rotations at
+  // pose_block_rot_offset(k).
+  virtual void update_bodies_pinned(uint64_t k, const void* block) = 0;
+  // After finish(): contact range of selected scene j, and every contact of
+  // the step (global body ids) copied into host arrays of totals().c rows.
+  virtual void scene_contact_range(uint64_t j, uint64_t* begin, uint64_t* end) const = 0;
+  virtual void read_all_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) = 0;
   virtual uint64_t scene_bytes_estimate(uint64_t n_shapes, uint64_t n_bodies) const = 0;
+  // Bytes a scene keeps resident, and the step buffers of a step over n shapes.
+  virtual uint64_t resident_bytes_estimate(uint64_t n_shapes, uint64_t n_bodies) const = 0;
+  virtual uint64_t step_bytes_estimate(uint64_t n_shapes) const = 0;
   virtual uint64_t device_bytes() const = 0;
   virtual hipStream_t stream() const = 0;
 };
+
+inline uint64_t pose_block_pos_offset(uint64_t k) { return (k * 4 + 255) & ~255ull; }
+inline uint64_t pose_block_rot_offset(uint64_t k, int real_kind) {
+  const uint64_t r = real_kind == REAL_F64 ? 8 : 4;
+  return pose_block_pos_offset(k) + ((k * 3 * r + 255) & ~255ull);
+}
+inline uint64_t pose_block_bytes(uint64_t k, int real_kind) {
+  return pose_block_rot_offset(k, real_kind) + k * 4 * (real_kind == REAL_F64 ? 8 : 4);
+}
 
 std::unique_ptr<SceneBase> make_scene(int real_kind, int device);
 std::unique_ptr<BatchBase> make_batch(int real_kind, int device);

@@ -538,14 +538,62 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     return ms;
   }
 
+  uint64_t scene_body_begin(uint32_t s) const override {
+    CDX_REQUIRE(s < scenes_.size(), ERR_INVALID, "scene index out of range");
+    return scenes_[s].body_begin;
+  }
+
+  void update_bodies_pinned(uint64_t k, const void* block) override {
+    if (k == 0) return;
+    const uint64_t nbt = scenes_.empty() ? 0 : scenes_.back().body_begin + scenes_.back().n_bodies;
+    const int kind = sizeof(R) == 8 ? REAL_F64 : REAL_F32;
+    const uint64_t bytes = pose_block_bytes(k, kind);
+    upd_.reserve(bytes);
+    CDX_HIP_CHECK(hipMemcpyAsync(upd_.p, block, bytes, hipMemcpyHostToDevice, s_));
+    const char* base = upd_.as<char>();
+    // indices past the resident bodies are dropped by the kernel
+    hipLaunchKernelGGL(k_scatter_bodies<R>, dim3(blocks_for(k, 256)), dim3(256), 0, s_, k, (uint64_t)0, nbt, NBC_,
+                       (const uint32_t*)base, (const R*)(base + pose_block_pos_offset(k)),
+                       (const R*)(base + pose_block_rot_offset(k, kind)), bpos_.as<R>(), brot_.as<R>());
+    aabb_current_ = false;
+    records_valid_ = false;
+  }
+
+  void scene_contact_range(uint64_t j, uint64_t* begin, uint64_t* end) const override {
+    CDX_REQUIRE(j < souts_host_.size(), ERR_INVALID, "selection index out of range");
+    *begin = souts_host_[j].contact_begin;
+    *end = souts_host_[j].contact_end;
+  }
+
+  void read_all_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) override {
+    const uint64_t n = C_;
+    if (!n) return;
+    if (norm) CDX_HIP_CHECK(hipMemcpyAsync(norm, out_norm_.p, n * 3 * sizeof(R), hipMemcpyDeviceToHost, s_));
+    if (cpta) CDX_HIP_CHECK(hipMemcpyAsync(cpta, out_pa_.p, n * 3 * sizeof(R), hipMemcpyDeviceToHost, s_));
+    if (cptb) CDX_HIP_CHECK(hipMemcpyAsync(cptb, out_pb_.p, n * 3 * sizeof(R), hipMemcpyDeviceToHost, s_));
+    if (dpth) CDX_HIP_CHECK(hipMemcpyAsync(dpth, out_d_.p, n * sizeof(R), hipMemcpyDeviceToHost, s_));
+    if (bids) CDX_HIP_CHECK(hipMemcpyAsync(bids, out_ids_.p, n * 8, hipMemcpyDeviceToHost, s_));
+    sync();
+  }
+
+  uint64_t resident_bytes_estimate(uint64_t n, uint64_t nb) const override {
+    // inputs in SoA plus the per-shape records each step rewrites
+    const uint64_t per_shape = 1 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
+                               sizeof(ShapeHdr) + sizeof(ShapeRecF);
+    const uint64_t per_body = 7 * sizeof(R) + 1;
+    return n * per_shape + nb * per_body;
+  }
+
+  uint64_t step_bytes_estimate(uint64_t n) const override {
+    // keys + ping-pong and per-entry pair counts/offsets; pairs, narrowphase
+    // records, queues and contact outputs per pair; slot counts/offsets
+    const double per_m = 2 * 8 + 2 * 4;
+    const double per_p = 2 * 8 + 10 * sizeof(R) + 8 + 4 + 4 + 2 * 8 + 4 + 1 + 10 * sizeof(R) + 8 + 8;
+    return (uint64_t)((double)n * (cap_m_per_shape_ * per_m + cap_p_per_shape_ * per_p + 8.0)) + (1u << 20);
+  }
+
   uint64_t scene_bytes_estimate(uint64_t n, uint64_t nb) const override {
-    // resident inputs and per-shape records, plus step buffers at the
-    // capacity factors (entries and pairs per shape)
-    uint64_t per_shape = 1 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
-                         sizeof(ShapeHdr) + sizeof(ShapeRecF);
-    uint64_t per_body = 7 * sizeof(R) + 1;
-    uint64_t step = n * (6 * (8 * 2 + 8) + 4 * (8 * 8 + 10 * sizeof(R) + 10 * sizeof(R)));
-    return n * per_shape + nb * per_body + step;
+    return resident_bytes_estimate(n, nb) + step_bytes_estimate(n);
   }
 
  private:

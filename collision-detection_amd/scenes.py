@@ -183,3 +183,31 @@ def lattice_spheres(side=256, seed=0x5EED0002, dtype=np.float32, spacing=0.01, g
 def mixed_scene_of_size(n, neighbours, seed, dtype=np.float64):
     """One scene of the batched / served workloads: the mixed shape set at its own density."""
     return mixed_bodies(n=n, seed=seed, dtype=dtype, neighbours=neighbours)
+
+
+def batched_sizes(count=512, lo=1024, hi=65536, seed=0x5EED0003):
+    """Shape counts of the batched workload: log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
+
+
+def batched_scenes(count=512, lo=1024, hi=65536, seed=0x5EED0003, neighbours=4.0, dtype=np.float64):
+    """Independent mixed scenes of log-uniform size (seeds seed+1+i), one per simulation instance."""
+    return [mixed_scene_of_size(int(n), neighbours, seed + 1 + i, dtype)
+            for i, n in enumerate(batched_sizes(count, lo, hi, seed))]
+
+
+def two_shapes_per_body(s: Scene) -> Scene:
+    """Scene s with bodies merged in pairs: shape i rides body i // 2 at a local offset and rotation."""
+    n = s.n
+    nb = (n + 1) // 2
+    rng = np.random.default_rng(n)
+    A = s.pos - s.pos[np.arange(n) // 2 * 2]
+    R = random_quats(rng, n).astype(s.dtype)
+    R[::2] = 0
+    R[::2, 0] = 1  # body rotation only for the even shape
+    c = lambda a: np.ascontiguousarray(a, dtype=s.dtype)
+    ident = np.zeros((nb, 4))
+    ident[:, 0] = 1
+    return Scene(s.T.copy(), c(A), s.B.copy(), s.C.copy(), c(R), (np.arange(n) // 2).astype(np.uint32),
+                 c(s.pos[::2]), c(ident), np.ones(nb, np.uint8), s.grid, s.name + "-2perbody", dict(s.meta))

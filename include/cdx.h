@@ -187,6 +187,54 @@ int cdx_scene_read_contacts(cdx_scene* scene, void* norm, void* cpta, void* cptb
 int cdx_scene_stage_ms(cdx_scene* scene, double* ms_out);
 int cdx_scene_device_bytes(cdx_scene* scene, uint64_t* bytes);
 
+/* ---- batches: many resident scenes stepped by one launch sequence -------
+ * No reference counterpart: the reference steps one scene per
+ * detectPossibleCollisions / DoNarrowphase call (Broadphase.cu:364-455,
+ * Narrowphase.cu:426-489).  A batch holds any number of scenes resident in
+ * HBM, each with its own grid, bounds and key range; cdx_batch_step runs
+ * AABB + broadphase + narrowphase over a selection of them as one sequence
+ * of launches (recorded once as a HIP graph and replayed while the
+ * selection, settings and capacities are unchanged).  Step buffers are
+ * sized by capacities that only grow; a step whose totals overflow them is
+ * re-run at the larger size inside cdx_batch_finish.  Each scene's pairs
+ * and contacts equal what a one-scene call on the same scene returns
+ * (shape and body ids local to the scene). */
+typedef struct cdx_batch cdx_batch;
+
+int cdx_batch_create(cdx_ctx* ctx, int real_kind, cdx_batch** out);
+int cdx_batch_destroy(cdx_batch* batch);
+const char* cdx_batch_last_error(const cdx_batch* batch);
+/* Appends a scene (arrays as for cdx_do_narrowphase; obj_active may be NULL
+ * for all active).  bins_x = bins_y = bins_z = 0 lets the autotuner pick the
+ * grid: the first scene of each size class (octaves of 4 shapes) is timed
+ * at several cell sizes, later scenes of the class reuse the fastest. */
+int cdx_batch_add_scene(cdx_batch* batch, uint64_t n, const int32_t* T, const void* A, const void* B, const void* C,
+                        const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos, const void* body_rot,
+                        const uint8_t* obj_active, int bins_x, int bins_y, int bins_z, uint32_t* scene_out);
+int cdx_batch_scene_bins(cdx_batch* batch, uint32_t scene, int* bins_xyz);
+int cdx_batch_scene_count(cdx_batch* batch, uint64_t* count);
+/* Replaces the poses of bodies idx[0..k) of one scene (local body ids). */
+int cdx_batch_update_bodies(cdx_batch* batch, uint32_t scene, uint64_t k, const uint32_t* idx, const void* body_pos,
+                            const void* body_rot);
+int cdx_batch_set_envelope(cdx_batch* batch, double envelope);
+int cdx_batch_set_narrowphase_mode(cdx_batch* batch, int np_mode, double tau);
+/* Scenes of the following steps (scenes = NULL: every resident scene). */
+int cdx_batch_select(cdx_batch* batch, const uint32_t* scenes, uint64_t k);
+/* Enqueues one step on the batch's stream (use_graph: record / replay). */
+int cdx_batch_step(cdx_batch* batch, int use_graph);
+/* Waits for the step, re-runs it if a capacity overflowed, reads totals. */
+int cdx_batch_finish(cdx_batch* batch, cdx_counts* totals);
+/* Per selected scene j (its position in the selection), after finish. */
+int cdx_batch_scene_counts(cdx_batch* batch, uint64_t j, cdx_counts* out);
+int cdx_batch_read_pairs(cdx_batch* batch, uint64_t j, int64_t* pairs_out, uint64_t capacity);
+int cdx_batch_read_contacts(cdx_batch* batch, uint64_t j, void* norm, void* cpta, void* cptb, void* dpth,
+                            int32_t* bids, uint64_t capacity);
+/* Device milliseconds of the last step (HIP events around its launches). */
+int cdx_batch_step_ms(cdx_batch* batch, double* ms);
+int cdx_batch_device_bytes(cdx_batch* batch, uint64_t* bytes);
+/* Estimated HBM bytes a scene of n shapes and nb bodies adds when resident. */
+int cdx_batch_scene_bytes(cdx_batch* batch, uint64_t n, uint64_t nb, uint64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif
