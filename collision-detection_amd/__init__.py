@@ -43,8 +43,30 @@ EXPORTS = [
     "cdx_batch_add_scene", "cdx_batch_scene_bins", "cdx_batch_scene_count", "cdx_batch_update_bodies",
     "cdx_batch_set_envelope", "cdx_batch_set_narrowphase_mode", "cdx_batch_select", "cdx_batch_step",
     "cdx_batch_finish", "cdx_batch_scene_counts", "cdx_batch_read_pairs", "cdx_batch_read_contacts",
-    "cdx_batch_step_ms", "cdx_batch_device_bytes", "cdx_batch_scene_bytes",
+    "cdx_batch_step_ms", "cdx_batch_device_bytes", "cdx_batch_scene_bytes", "cdx_server_create",
+    "cdx_server_destroy", "cdx_server_last_error", "cdx_server_add_scene", "cdx_server_scene_info",
+    "cdx_server_submit", "cdx_server_poll", "cdx_server_wait", "cdx_server_result", "cdx_server_read_contacts",
+    "cdx_server_release", "cdx_server_stats", "cdx_server_memory",
 ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("real_kind", C.c_int), ("np_mode", C.c_int), ("tau", C.c_double), ("envelope", C.c_double),
+        ("hbm_budget", C.c_uint64), ("max_batch_shapes", C.c_uint64), ("max_pending", C.c_uint64),
+        ("use_graph", C.c_int),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("steps", C.c_uint64),
+        ("stepped_requests", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+        ("max_ms", C.c_double), ("busy_ms", C.c_double), ("window_s", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class CdxError(RuntimeError):
@@ -135,6 +157,19 @@ def lib():
         "cdx_batch_step_ms": ([vp, vp], i32),
         "cdx_batch_device_bytes": ([vp, vp], i32),
         "cdx_batch_scene_bytes": ([vp, u64, u64, vp], i32),
+        "cdx_server_create": ([vp, vp, vp], i32),
+        "cdx_server_destroy": ([vp], i32),
+        "cdx_server_last_error": ([vp], C.c_char_p),
+        "cdx_server_add_scene": ([vp, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, i32, i32, i32, vp], i32),
+        "cdx_server_scene_info": ([vp, C.c_uint32, vp, vp, vp, vp], i32),
+        "cdx_server_submit": ([vp, C.c_uint32, u64, vp, vp, vp, vp], i32),
+        "cdx_server_poll": ([vp, u64, vp], i32),
+        "cdx_server_wait": ([vp, u64, C.c_double, vp], i32),
+        "cdx_server_result": ([vp, u64, vp, vp], i32),
+        "cdx_server_read_contacts": ([vp, u64, vp, vp, vp, vp, vp, u64], i32),
+        "cdx_server_release": ([vp, u64], i32),
+        "cdx_server_stats": ([vp, vp, i32], i32),
+        "cdx_server_memory": ([vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -441,3 +476,88 @@ class Batch:
         b = C.c_uint64(0)
         self._chk(lib().cdx_batch_scene_bytes(self.h, int(n), int(nb), C.byref(b)))
         return b.value
+
+
+class Server:
+    """Served request loop over device-resident scenes (cdx_server)."""
+
+    def __init__(self, ctx: Context, dtype=np.float64, mode=NP_MIXED, tau=1e-2, envelope=0.0, hbm_budget=0,
+                 max_batch_shapes=0, max_pending=0, use_graph=True):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        cfg = ServerConfig(_kind(dtype), int(mode), float(tau), float(envelope), int(hbm_budget),
+                           int(max_batch_shapes), int(max_pending), 1 if use_graph else 0)
+        self.h = C.c_void_p()
+        ctx._chk(lib().cdx_server_create(ctx.h, C.byref(cfg), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().cdx_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def _chk(self, st):
+        if st:
+            raise CdxError(st, lib().cdx_server_last_error(self.h).decode())
+
+    def add_scene(self, s, bins=None) -> int:
+        assert np.dtype(s.dtype) == self.dtype
+        g = s.grid if bins is None else bins
+        out = C.c_uint32(0)
+        self._chk(lib().cdx_server_add_scene(self.h, s.n, _p(s.T), _p(s.A), _p(s.B), _p(s.C), _p(s.R), _p(s.ID),
+                                             s.nb, _p(s.pos), _p(s.rot), _p(s.active), int(g[0]), int(g[1]),
+                                             int(g[2]), C.byref(out)))
+        return out.value
+
+    def scene_info(self, scene):
+        n, nb, cls = C.c_uint64(0), C.c_uint64(0), C.c_int(0)
+        g = np.zeros(3, np.int32)
+        self._chk(lib().cdx_server_scene_info(self.h, int(scene), C.byref(n), C.byref(nb), C.byref(cls), _p(g)))
+        return {"n": n.value, "nb": nb.value, "size_class": cls.value, "bins": tuple(int(v) for v in g)}
+
+    def submit(self, scene, pos, rot, idx=None) -> int:
+        pos = np.ascontiguousarray(pos, dtype=self.dtype)
+        rot = np.ascontiguousarray(rot, dtype=self.dtype)
+        if idx is not None:
+            idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        out = C.c_uint64(0)
+        self._chk(lib().cdx_server_submit(self.h, int(scene), len(pos), _p(idx), _p(pos), _p(rot), C.byref(out)))
+        return out.value
+
+    def poll(self, req) -> bool:
+        d = C.c_int(0)
+        self._chk(lib().cdx_server_poll(self.h, int(req), C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, req, timeout_ms=-1.0) -> bool:
+        d = C.c_int(0)
+        self._chk(lib().cdx_server_wait(self.h, int(req), float(timeout_ms), C.byref(d)))
+        return bool(d.value)
+
+    def result(self, req):
+        n, lat = C.c_uint64(0), C.c_double(0)
+        self._chk(lib().cdx_server_result(self.h, int(req), C.byref(n), C.byref(lat)))
+        return n.value, lat.value
+
+    def read_contacts(self, req):
+        k, _ = self.result(req)
+        dt = self.dtype
+        norm, pa, pb = (np.zeros((k, 3), dt) for _ in range(3))
+        d = np.zeros(k, dt)
+        ids = np.zeros((k, 2), np.int32)
+        self._chk(lib().cdx_server_read_contacts(self.h, int(req), _p(norm), _p(pa), _p(pb), _p(d), _p(ids), k))
+        return norm, pa, pb, d, ids
+
+    def release(self, req):
+        self._chk(lib().cdx_server_release(self.h, int(req)))
+
+    def stats(self, reset=False) -> dict:
+        s = ServerStats()
+        self._chk(lib().cdx_server_stats(self.h, C.byref(s), 1 if reset else 0))
+        return s.as_dict()
+
+    def memory(self):
+        b, c, d = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        self._chk(lib().cdx_server_memory(self.h, C.byref(b), C.byref(c), C.byref(d)))
+        return {"hbm_budget": b.value, "hbm_committed": c.value, "device_bytes": d.value}

@@ -11,6 +11,7 @@
 #include "../../include/cdx.h"
 #include "autotune.hpp"
 #include "cdx_pipeline.hpp"
+#include "server.hpp"
 #include "cdx_util.hpp"
 
 struct cdx_ctx {
@@ -38,7 +39,13 @@ struct cdx_batch {
   std::string err;
 };
 
+struct cdx_server {
+  std::unique_ptr<cdx::Server> srv;
+};
+
 namespace {
+
+thread_local std::string t_server_err;
 
 template <class F> int guarded(std::string* err, F&& f) {
   try {
@@ -384,6 +391,89 @@ int cdx_batch_device_bytes(cdx_batch* batch, uint64_t* bytes) {
 }
 int cdx_batch_scene_bytes(cdx_batch* batch, uint64_t n, uint64_t nb, uint64_t* bytes) {
   BATCH_CALL(CDX_REQUIRE(bytes, CDX_ERR_INVALID, "null output"); *bytes = b.scene_bytes_estimate(n, nb));
+}
+
+// ---- served loop ----------------------------------------------------------
+
+int cdx_server_create(cdx_ctx* ctx, const cdx_server_config* cfg, cdx_server** out) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    CDX_REQUIRE(cfg && out, CDX_ERR_INVALID, "null argument");
+    check_kind(cfg->real_kind);
+    CDX_REQUIRE(cfg->np_mode == CDX_NP_NATIVE || cfg->np_mode == CDX_NP_MIXED || cfg->np_mode == CDX_NP_FAST,
+                CDX_ERR_INVALID, "unknown narrowphase mode");
+    CDX_REQUIRE(cfg->tau >= 0.0, CDX_ERR_INVALID, "tau must be non-negative");
+    cdx::ServerConfig c;
+    c.real_kind = cfg->real_kind;
+    c.np_mode = cfg->np_mode;
+    c.tau = cfg->tau;
+    c.envelope = cfg->envelope;
+    c.hbm_budget = cfg->hbm_budget;
+    if (cfg->max_batch_shapes) c.max_batch_shapes = cfg->max_batch_shapes;
+    if (cfg->max_pending) c.max_pending = cfg->max_pending;
+    c.use_graph = cfg->use_graph != 0;
+    auto sv = std::make_unique<cdx_server>();
+    sv->srv = std::make_unique<cdx::Server>(ctx->device, c);
+    *out = sv.release();
+  });
+}
+
+int cdx_server_destroy(cdx_server* server) {
+  delete server;
+  return CDX_OK;
+}
+
+const char* cdx_server_last_error(const cdx_server* server) {
+  return server ? t_server_err.c_str() : "null server";
+}
+
+#define SERVER_CALL(...)                                                 \
+  if (!server) return CDX_ERR_INVALID;                                   \
+  return guarded(&t_server_err, [&] { cdx::Server& sv = *server->srv; __VA_ARGS__; })
+
+int cdx_server_add_scene(cdx_server* server, uint64_t n, const int32_t* T, const void* A, const void* B,
+                         const void* C, const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos,
+                         const void* body_rot, const uint8_t* obj_active, int bins_x, int bins_y, int bins_z,
+                         uint32_t* scene_out) {
+  (void)C;
+  SERVER_CALL(CDX_REQUIRE(T && A && B && R && ID && body_pos && body_rot, CDX_ERR_INVALID, "null input");
+              uint32_t s = sv.add_scene(n, T, A, B, R, ID, nb, body_pos, body_rot, obj_active, bins_x, bins_y,
+                                        bins_z);
+              if (scene_out) *scene_out = s);
+}
+int cdx_server_scene_info(cdx_server* server, uint32_t scene, uint64_t* n_shapes, uint64_t* n_bodies,
+                          int* size_class, int* bins_xyz) {
+  SERVER_CALL(sv.scene_info(scene, n_shapes, n_bodies, size_class, bins_xyz));
+}
+int cdx_server_submit(cdx_server* server, uint32_t scene, uint64_t k, const uint32_t* idx, const void* body_pos,
+                      const void* body_rot, uint64_t* request_out) {
+  SERVER_CALL(CDX_REQUIRE(request_out, CDX_ERR_INVALID, "null output");
+              *request_out = sv.submit(scene, k, idx, body_pos, body_rot));
+}
+int cdx_server_poll(cdx_server* server, uint64_t request, int* done) {
+  SERVER_CALL(CDX_REQUIRE(done, CDX_ERR_INVALID, "null output"); *done = sv.poll(request));
+}
+int cdx_server_wait(cdx_server* server, uint64_t request, double timeout_ms, int* done) {
+  SERVER_CALL(int d = sv.wait(request, timeout_ms); if (done) *done = d);
+}
+int cdx_server_result(cdx_server* server, uint64_t request, uint64_t* n_contacts, double* latency_ms) {
+  SERVER_CALL(sv.result_counts(request, n_contacts, latency_ms));
+}
+int cdx_server_read_contacts(cdx_server* server, uint64_t request, void* norm, void* cpta, void* cptb, void* dpth,
+                             int32_t* bids, uint64_t capacity) {
+  SERVER_CALL(sv.read_contacts(request, norm, cpta, cptb, dpth, bids, capacity));
+}
+int cdx_server_release(cdx_server* server, uint64_t request) { SERVER_CALL(sv.release(request)); }
+int cdx_server_stats(cdx_server* server, cdx_server_stats_t* out, int reset) {
+  SERVER_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); cdx::ServerStats s = sv.stats();
+              out->submitted = s.submitted; out->completed = s.completed; out->failed = s.failed;
+              out->steps = s.batches; out->stepped_requests = s.batched_requests; out->p50_ms = s.p50_ms;
+              out->p99_ms = s.p99_ms; out->mean_ms = s.mean_ms; out->max_ms = s.max_ms; out->busy_ms = s.busy_ms;
+              out->window_s = s.window_s; if (reset) sv.reset_stats());
+}
+int cdx_server_memory(cdx_server* server, uint64_t* hbm_budget, uint64_t* hbm_committed, uint64_t* device_bytes) {
+  SERVER_CALL(if (hbm_budget) *hbm_budget = sv.hbm_budget(); if (hbm_committed) *hbm_committed = sv.hbm_committed();
+              if (device_bytes) *device_bytes = sv.device_bytes());
 }
 
 }  // extern "C"

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bp_kernels.hpp"
@@ -98,13 +99,26 @@ __global__ void k_aabb_from_aos(uint64_t n, uint64_t NS, const R* __restrict__ a
 
 // ------------------------------------------------------------ host side --
 
+// Device-wide synchronisation (buffer growth) must not run while another
+// thread records a stream into a HIP graph; both take this lock.
+std::recursive_mutex& capture_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  ~DevBuf() { if (p) (void)hipFree(p); }
+  ~DevBuf() {
+    if (p) {
+      std::lock_guard<std::recursive_mutex> lk(capture_mutex());
+      (void)hipFree(p);
+    }
+  }
   // Grows to at least b bytes; contents are not kept.
   void reserve(size_t b) {
     if (b <= bytes) return;
+    std::lock_guard<std::recursive_mutex> lk(capture_mutex());
     release();
     size_t nb = b < 256 ? 256 : b;
     CDX_HIP_CHECK(hipMalloc(&p, nb));
@@ -112,6 +126,7 @@ struct DevBuf {
   }
   void release() {
     if (p) {
+      std::lock_guard<std::recursive_mutex> lk(capture_mutex());
       // kernels queued on any stream may still read the old block
       CDX_HIP_CHECK(hipDeviceSynchronize());
       CDX_HIP_CHECK(hipFree(p));
@@ -1035,8 +1050,15 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
           graph_exec_ = nullptr;
         }
         hipGraph_t g = nullptr;
+        std::lock_guard<std::recursive_mutex> lk(capture_mutex());
         CDX_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-        enqueue_capacity_step();
+        try {
+          enqueue_capacity_step();
+        } catch (...) {
+          // close the capture so the stream stays usable, then report
+          if (hipStreamEndCapture(s_, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+          throw;
+        }
         CDX_HIP_CHECK(hipStreamEndCapture(s_, &g));
         CDX_HIP_CHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
         CDX_HIP_CHECK(hipGraphDestroy(g));

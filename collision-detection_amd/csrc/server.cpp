@@ -1,0 +1,457 @@
+// Served request loop (declarations in server.hpp).
+//
+// Scenes live in one batch engine per size class.  Each class has a worker
+// thread that forms a step from its queue as soon as the previous step of
+// that class has finished (continuous batching): oldest requests first, at
+// most one request per scene, up to max_batch_shapes shapes.  Classes step
+// on their own HIP streams, so steps of different classes overlap on the
+// device.  A request's pose inputs are copied at submit into a pinned
+// block; the worker uploads each block with one asynchronous copy ahead of
+// the step, and reads every contact of the step back with one copy per
+// output array into a pinned result block that the step's requests share.
+#include "server.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_set>
+
+#include "cdx_util.hpp"
+
+namespace cdx {
+
+namespace {
+
+// Size classes by shape count (upper bounds; the last is open).
+constexpr uint64_t kClassHi[] = {4096, 32768, ~0ull};
+constexpr int kNumClasses = 3;
+
+// Pinned host blocks for request inputs, bucketed by power-of-two size.
+class PinnedPool {
+ public:
+  ~PinnedPool() {
+    for (auto& kv : free_)
+      for (void* p : kv.second) (void)hipHostFree(p);
+  }
+  void* get(uint64_t bytes, uint64_t* bucket) {
+    uint64_t b = 4096;
+    while (b < bytes) b <<= 1;
+    *bucket = b;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto& v = free_[b];
+      if (!v.empty()) {
+        void* p = v.back();
+        v.pop_back();
+        return p;
+      }
+    }
+    void* p = nullptr;
+    CDX_HIP_CHECK(hipHostMalloc(&p, b, hipHostMallocDefault));
+    return p;
+  }
+  void put(void* p, uint64_t bucket) {
+    std::lock_guard<std::mutex> lk(mu_);
+    free_[bucket].push_back(p);
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<uint64_t, std::vector<void*>> free_;
+};
+
+PinnedPool& pinned_pool() {
+  static PinnedPool* pool = new PinnedPool();  // lives until exit
+  return *pool;
+}
+
+}  // namespace
+
+ResultBlock::~ResultBlock() {
+  if (base) (void)hipHostFree(base);
+}
+
+struct PoseBlock {
+  void* p = nullptr;
+  uint64_t bucket = 0;
+  ~PoseBlock() {
+    if (p) pinned_pool().put(p, bucket);
+  }
+};
+
+double Server::now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+Server::Server(int device, const ServerConfig& cfg) : dev_(device), cfg_(cfg), tuner_(device, cfg.real_kind) {
+  CDX_REQUIRE(cfg.real_kind == REAL_F64 || cfg.real_kind == REAL_F32, ERR_INVALID, "unknown real kind");
+  CDX_REQUIRE(cfg.max_batch_shapes > 0 && cfg.max_pending > 0, ERR_INVALID, "batch and queue limits must be positive");
+  CDX_HIP_CHECK(hipSetDevice(dev_));
+  size_t free_b = 0, total_b = 0;
+  CDX_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  budget_ = cfg.hbm_budget ? cfg.hbm_budget : (uint64_t)(0.9 * (double)free_b);
+  classes_.resize(kNumClasses);
+  uint64_t lo = 0;
+  for (int c = 0; c < kNumClasses; ++c) {
+    classes_[c] = std::make_unique<Class>();
+    Class& k = *classes_[c];
+    k.lo = lo;
+    k.hi = kClassHi[c];
+    lo = k.hi;
+    k.eng = make_batch(cfg.real_kind, dev_);
+    k.eng->set_np_mode(cfg.np_mode, cfg.tau);
+    k.eng->set_envelope(cfg.envelope);
+  }
+  st_t0_ = now_ms();
+  for (int c = 0; c < kNumClasses; ++c) classes_[c]->worker = std::thread(&Server::worker_loop, this, c);
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_all();
+  for (auto& k : classes_)
+    if (k->worker.joinable()) k->worker.join();
+}
+
+int Server::class_of(uint64_t n) const {
+  for (int c = 0; c < kNumClasses; ++c)
+    if (n < classes_[c]->hi) return c;
+  return kNumClasses - 1;
+}
+
+uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R,
+                           const uint32_t* ID, uint64_t nb, const void* pos, const void* rot, const uint8_t* active,
+                           int bx, int by, int bz) {
+  CDX_REQUIRE(n > 0 && nb > 0, ERR_INVALID, "a scene needs shapes and bodies");
+  const int c = class_of(n);
+  Class& k = *classes_[c];
+  int g[3] = {bx, by, bz};
+  std::lock_guard<std::mutex> elk(k.eng_mu);
+  const bool first_in_class = k.eng->scene_count() == 0;
+  // footprint: the scene's resident arrays, plus the class's step buffers
+  // the first time the class gets a scene
+  uint64_t need = k.eng->resident_bytes_estimate(n, nb);
+  if (first_in_class) need += k.eng->step_bytes_estimate(std::max<uint64_t>(cfg_.max_batch_shapes, n));
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (committed_ + need > budget_)
+      throw Error(ERR_CAPACITY, "scene of " + std::to_string(n) + " shapes needs ~" + std::to_string(need >> 20) +
+                                    " MiB of HBM; " + std::to_string((budget_ - committed_) >> 20) +
+                                    " MiB of the budget are left");
+  }
+  if (bx == 0 && by == 0 && bz == 0) {
+    TuneChoice t = tuner_.choose(n, T, A, B, R, ID, nb, pos, rot, active);
+    for (int i = 0; i < 3; ++i) g[i] = t.grid[i];
+    if (t.m_per_shape > m_per_shape_ || t.p_per_shape > p_per_shape_) {
+      m_per_shape_ = std::max(m_per_shape_, t.m_per_shape);
+      p_per_shape_ = std::max(p_per_shape_, t.p_per_shape);
+      k.eng->set_capacity_factors(m_per_shape_ * 1.25 + 1.0, p_per_shape_ * 1.25 + 1.0);
+    }
+  }
+  const uint32_t local = k.eng->add_scene(n, T, A, B, R, ID, nb, pos, rot, active, g[0], g[1], g[2]);
+  std::lock_guard<std::mutex> lk(mu_);
+  committed_ += need;
+  SceneRef r;
+  r.cls = c;
+  r.local = local;
+  r.n = n;
+  r.nb = nb;
+  r.body_begin = k.eng->scene_body_begin(local);
+  for (int i = 0; i < 3; ++i) r.bins[i] = g[i];
+  scenes_.push_back(r);
+  return (uint32_t)(scenes_.size() - 1);
+}
+
+uint64_t Server::scene_count() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  return scenes_.size();
+}
+
+void Server::scene_info(uint32_t s, uint64_t* n, uint64_t* nb, int* cls, int bins[3]) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  CDX_REQUIRE(s < scenes_.size(), ERR_INVALID, "scene index out of range");
+  const SceneRef& r = scenes_[s];
+  if (n) *n = r.n;
+  if (nb) *nb = r.nb;
+  if (cls) *cls = r.cls;
+  if (bins)
+    for (int i = 0; i < 3; ++i) bins[i] = r.bins[i];
+}
+
+uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void* pos, const void* rot) {
+  SceneRef ref;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    CDX_REQUIRE(s < scenes_.size(), ERR_INVALID, "scene index out of range");
+    CDX_REQUIRE(pending_ < cfg_.max_pending, ERR_CAPACITY, "request queue is full");
+    ref = scenes_[s];
+  }
+  CDX_REQUIRE(k == 0 || (pos && rot), ERR_INVALID, "null pose input");
+  CDX_REQUIRE(idx || k <= ref.nb, ERR_INVALID, "more pose updates than bodies");
+  const int kind = cfg_.real_kind;
+  const uint64_t rs = kind == REAL_F64 ? 8 : 4;
+  auto req = std::make_shared<Request>();
+  req->scene = s;
+  req->local = ref.local;
+  req->n = ref.n;
+  req->body_base = ref.body_begin;
+  req->k = k;
+  if (k) {
+    auto blk = std::make_shared<PoseBlock>();
+    blk->p = pinned_pool().get(pose_block_bytes(k, kind), &blk->bucket);
+    uint32_t* gi = static_cast<uint32_t*>(blk->p);
+    for (uint64_t j = 0; j < k; ++j) {
+      uint64_t b = idx ? idx[j] : j;
+      CDX_REQUIRE(b < ref.nb, ERR_INVALID, "body index out of range in a pose update");
+      gi[j] = (uint32_t)(ref.body_begin + b);
+    }
+    char* base = static_cast<char*>(blk->p);
+    std::memcpy(base + pose_block_pos_offset(k), pos, k * 3 * rs);
+    std::memcpy(base + pose_block_rot_offset(k, kind), rot, k * 4 * rs);
+    req->poses = std::move(blk);
+  }
+  uint64_t id;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    CDX_REQUIRE(pending_ < cfg_.max_pending, ERR_CAPACITY, "request queue is full");
+    id = next_id_++;
+    req->id = id;
+    req->t_submit = now_ms();
+    reqs_[id] = req;
+    classes_[ref.cls]->queue.push_back(req);
+    ++pending_;
+    ++st_sub_;
+  }
+  cv_work_.notify_all();
+  return id;
+}
+
+void Server::worker_loop(int c) {
+  (void)hipSetDevice(dev_);
+  Class& k = *classes_[c];
+  std::vector<std::shared_ptr<Request>> batch;
+  for (;;) {
+    batch.clear();
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_work_.wait(lk, [&] { return stop_ || !k.queue.empty(); });
+      if (stop_) {
+        for (auto& r : k.queue) {
+          r->state = 3;
+          r->err = "server stopped";
+        }
+        k.queue.clear();
+        cv_done_.notify_all();
+        return;
+      }
+      uint64_t shapes = 0;
+      std::unordered_set<uint32_t> taken;
+      for (auto it = k.queue.begin(); it != k.queue.end();) {
+        const auto& r = *it;
+        if (taken.count(r->scene)) { ++it; continue; }  // the scene's next step waits for this one
+        const uint64_t n = r->n;
+        if (!batch.empty() && shapes + n > cfg_.max_batch_shapes) break;
+        shapes += n;
+        taken.insert(r->scene);
+        r->state = 1;
+        batch.push_back(r);
+        it = k.queue.erase(it);
+      }
+    }
+    run_batch(c, batch);
+  }
+}
+
+std::shared_ptr<ResultBlock> Server::get_block(Class& k, uint64_t contacts) {
+  for (auto& b : k.blocks)
+    if (b.use_count() == 1 && b->cap >= contacts) return b;
+  // drop one idle block that is too small before adding a larger one
+  for (auto it = k.blocks.begin(); it != k.blocks.end(); ++it)
+    if (it->use_count() == 1) {
+      k.blocks.erase(it);
+      break;
+    }
+  auto b = std::make_shared<ResultBlock>();
+  const uint64_t cap = std::max<uint64_t>(contacts + contacts / 4, 4096);
+  const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
+  CDX_HIP_CHECK(hipHostMalloc(&b->base, cap * (10 * rs + 8), hipHostMallocDefault));
+  b->cap = cap;
+  k.blocks.push_back(b);
+  return b;
+}
+
+void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
+  Class& k = *classes_[c];
+  std::string err;
+  double step_ms = 0.0;
+  try {
+    std::lock_guard<std::mutex> elk(k.eng_mu);
+    BatchBase& e = *k.eng;
+    std::vector<uint32_t> sel;
+    sel.reserve(batch.size());
+    for (auto& r : batch) {
+      if (r->k) e.update_bodies_pinned(r->k, r->poses->p);
+      sel.push_back(r->local);
+    }
+    e.select(sel.data(), sel.size());
+    e.step(cfg_.use_graph);
+    e.finish();
+    step_ms = e.last_step_ms();
+    const uint64_t C = e.totals().c;
+    auto blk = get_block(k, C);
+    const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
+    char* base = static_cast<char*>(blk->base);
+    const uint64_t cap = blk->cap;
+    e.read_all_contacts(base, base + 3 * rs * cap, base + 6 * rs * cap, base + 9 * rs * cap,
+                        reinterpret_cast<int32_t*>(base + 10 * rs * cap));
+    for (uint64_t j = 0; j < batch.size(); ++j) {
+      uint64_t b0, b1;
+      e.scene_contact_range(j, &b0, &b1);
+      batch[j]->block = blk;
+      batch[j]->c_begin = b0;
+      batch[j]->c_count = b1 - b0;
+    }
+  } catch (const std::exception& ex) {
+    err = ex.what();
+  }
+  const double t = now_ms();
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto& r : batch) {
+      r->poses.reset();
+      --pending_;
+      if (err.empty()) {
+        r->state = 2;
+        r->latency_ms = t - r->t_submit;
+        lat_.push_back(r->latency_ms);
+        ++st_done_;
+      } else {
+        r->state = 3;
+        r->err = err;
+        ++st_fail_;
+      }
+    }
+    ++st_batches_;
+    st_breqs_ += batch.size();
+    st_busy_ += step_ms;
+  }
+  cv_done_.notify_all();
+}
+
+int Server::poll(uint64_t id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = reqs_.find(id);
+  CDX_REQUIRE(it != reqs_.end(), ERR_INVALID, "unknown request id");
+  if (it->second->state == 3) throw Error(ERR_INVALID, "request failed: " + it->second->err);
+  return it->second->state == 2 ? 1 : 0;
+}
+
+int Server::wait(uint64_t id, double timeout_ms) {
+  std::unique_lock<std::mutex> lk(mu_);
+  auto it = reqs_.find(id);
+  CDX_REQUIRE(it != reqs_.end(), ERR_INVALID, "unknown request id");
+  std::shared_ptr<Request> r = it->second;
+  auto ready = [&] { return r->state >= 2; };
+  if (timeout_ms < 0) cv_done_.wait(lk, ready);
+  else if (!cv_done_.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) return 0;
+  if (r->state == 3) throw Error(ERR_INVALID, "request failed: " + r->err);
+  return 1;
+}
+
+void Server::result_counts(uint64_t id, uint64_t* contacts, double* latency_ms) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = reqs_.find(id);
+  CDX_REQUIRE(it != reqs_.end(), ERR_INVALID, "unknown request id");
+  const Request& r = *it->second;
+  CDX_REQUIRE(r.state == 2, ERR_NOT_READY, "request not finished");
+  if (contacts) *contacts = r.c_count;
+  if (latency_ms) *latency_ms = r.latency_ms;
+}
+
+void Server::read_contacts(uint64_t id, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids, uint64_t cap) {
+  std::shared_ptr<Request> r;
+  uint64_t body_base;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = reqs_.find(id);
+    CDX_REQUIRE(it != reqs_.end(), ERR_INVALID, "unknown request id");
+    r = it->second;
+    CDX_REQUIRE(r->state == 2, ERR_NOT_READY, "request not finished");
+    body_base = r->body_base;
+  }
+  const uint64_t n = r->c_count;
+  CDX_REQUIRE(cap >= n, ERR_CAPACITY, "contact buffers too small: need " + std::to_string(n));
+  if (!n) return;
+  const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
+  const char* base = static_cast<const char*>(r->block->base);
+  const uint64_t bc = r->block->cap, b0 = r->c_begin;
+  if (norm) std::memcpy(norm, base + 3 * rs * b0, 3 * rs * n);
+  if (cpta) std::memcpy(cpta, base + 3 * rs * (bc + b0), 3 * rs * n);
+  if (cptb) std::memcpy(cptb, base + 3 * rs * (2 * bc + b0), 3 * rs * n);
+  if (dpth) std::memcpy(dpth, base + 9 * rs * bc + rs * b0, rs * n);
+  if (bids) {
+    const int32_t* ids = reinterpret_cast<const int32_t*>(base + 10 * rs * bc) + 2 * b0;
+    for (uint64_t i = 0; i < 2 * n; ++i) bids[i] = ids[i] - (int32_t)body_base;
+  }
+}
+
+void Server::release(uint64_t id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = reqs_.find(id);
+  CDX_REQUIRE(it != reqs_.end(), ERR_INVALID, "unknown request id");
+  CDX_REQUIRE(it->second->state >= 2, ERR_NOT_READY, "request still queued or running");
+  reqs_.erase(it);
+}
+
+ServerStats Server::stats() {
+  std::lock_guard<std::mutex> lk(mu_);
+  ServerStats s{};
+  s.submitted = st_sub_;
+  s.completed = st_done_;
+  s.failed = st_fail_;
+  s.batches = st_batches_;
+  s.batched_requests = st_breqs_;
+  s.busy_ms = st_busy_;
+  s.window_s = (now_ms() - st_t0_) / 1000.0;
+  if (!lat_.empty()) {
+    std::vector<double> v = lat_;
+    std::sort(v.begin(), v.end());
+    auto q = [&](double f) {
+      size_t i = (size_t)std::ceil(f * (double)v.size()) - 1;
+      return v[std::min(i, v.size() - 1)];
+    };
+    s.p50_ms = q(0.50);
+    s.p99_ms = q(0.99);
+    s.max_ms = v.back();
+    double sum = 0.0;
+    for (double x : v) sum += x;
+    s.mean_ms = sum / (double)v.size();
+  }
+  return s;
+}
+
+void Server::reset_stats() {
+  std::lock_guard<std::mutex> lk(mu_);
+  lat_.clear();
+  st_sub_ = st_done_ = st_fail_ = st_batches_ = st_breqs_ = 0;
+  st_busy_ = 0.0;
+  st_t0_ = now_ms();
+}
+
+uint64_t Server::device_bytes() {
+  uint64_t t = 0;
+  for (auto& k : classes_) {
+    std::lock_guard<std::mutex> elk(k->eng_mu);
+    t += k->eng->device_bytes();
+  }
+  return t;
+}
+
+}  // namespace cdx

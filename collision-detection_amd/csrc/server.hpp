@@ -1,0 +1,142 @@
+// Served request loop over device-resident scenes: continuous batching per
+// scene-size class.  No reference counterpart (the reference API runs one
+// synchronous call per scene); a request carries the pose inputs of
+// AABBGenerator::GenerateAABB (AABBGenerator.cuh:21-30) for one scene and
+// returns that scene's contacts as Narrowphase::DoNarrowphase does
+// (Narrowphase.cuh:19-34).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "autotune.hpp"
+#include "cdx_pipeline.hpp"
+
+namespace cdx {
+
+struct ServerConfig {
+  int real_kind = REAL_F64;
+  int np_mode = NP_MIXED;
+  double tau = 1e-2;
+  double envelope = 0.0;
+  uint64_t hbm_budget = 0;          // bytes of resident scenes + step buffers; 0 = 90 % of free HBM
+  uint64_t max_batch_shapes = 1u << 19;  // shapes per batched step (bounds step latency)
+  uint64_t max_pending = 1u << 16;       // queued requests before submit refuses
+  bool use_graph = true;
+};
+
+struct ServerStats {
+  uint64_t submitted, completed, failed, batches, batched_requests;
+  double p50_ms, p99_ms, mean_ms, max_ms;  // submit -> result ready, over the window
+  double busy_ms;                          // summed device step time of the window
+  double window_s;                         // wall time since the window started
+};
+
+struct PoseBlock;
+
+// Pinned host block holding the contacts of one batched step.
+struct ResultBlock {
+  void* base = nullptr;
+  uint64_t cap = 0;  // contacts
+  ~ResultBlock();
+};
+
+class Server {
+ public:
+  Server(int device, const ServerConfig& cfg);
+  ~Server();
+
+  // Resident scene (host arrays in the reference layouts; body ids local).
+  // bins = 0,0,0 autotunes.  Refuses (ERR_CAPACITY) a scene whose estimated
+  // footprint does not fit the HBM budget.
+  uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R, const uint32_t* ID,
+                     uint64_t nb, const void* pos, const void* rot, const uint8_t* active, int bx, int by, int bz);
+  uint64_t scene_count() const;
+  void scene_info(uint32_t scene, uint64_t* n_shapes, uint64_t* n_bodies, int* size_class, int bins[3]) const;
+
+  // Queues a step request for one scene with k pose updates (idx NULL:
+  // bodies 0..k-1).  The inputs are copied before return.  Returns the
+  // request id.
+  uint64_t submit(uint32_t scene, uint64_t k, const uint32_t* idx, const void* pos, const void* rot);
+  // 1 when the result is ready, 0 while queued or running; throws on a
+  // failed request.
+  int poll(uint64_t req);
+  // Blocks until the request is done (timeout_ms < 0: no limit).  Returns
+  // 1 when done, 0 on timeout.
+  int wait(uint64_t req, double timeout_ms);
+  void result_counts(uint64_t req, uint64_t* contacts, double* latency_ms);
+  void read_contacts(uint64_t req, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids, uint64_t cap);
+  void release(uint64_t req);
+
+  ServerStats stats();
+  void reset_stats();
+  uint64_t device_bytes();
+  uint64_t hbm_budget() const { return budget_; }
+  uint64_t hbm_committed() const { return committed_; }
+
+ private:
+  struct Request {
+    uint64_t id = 0;
+    uint32_t scene = 0, local = 0;  // server scene id; index in its class engine
+    uint64_t n = 0;                 // shapes of the scene
+    uint64_t body_base = 0;         // global body id of the scene's body 0
+    uint64_t k = 0;
+    std::shared_ptr<PoseBlock> poses;  // pinned: global ids, positions, rotations
+    double t_submit = 0.0;
+    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+    std::string err;
+    std::shared_ptr<ResultBlock> block;
+    uint64_t c_begin = 0, c_count = 0;
+    double latency_ms = 0.0;
+  };
+  struct Class {
+    uint64_t lo, hi;  // shape-count range [lo, hi)
+    std::unique_ptr<BatchBase> eng;
+    std::mutex eng_mu;  // the engine is used by the worker and add_scene
+    std::deque<std::shared_ptr<Request>> queue;
+    std::thread worker;
+    std::vector<std::shared_ptr<ResultBlock>> blocks;  // pool, used by the worker only
+  };
+  struct SceneRef {
+    int cls;
+    uint32_t local;
+    uint64_t n, nb, body_begin;
+    int bins[3];
+  };
+
+  void worker_loop(int c);
+  void run_batch(int c, std::vector<std::shared_ptr<Request>>& batch);
+  std::shared_ptr<ResultBlock> get_block(Class& k, uint64_t contacts);
+  int class_of(uint64_t n) const;
+  static double now_ms();
+
+  int dev_;
+  ServerConfig cfg_;
+  uint64_t budget_ = 0, committed_ = 0;
+  std::vector<std::unique_ptr<Class>> classes_;
+  std::vector<SceneRef> scenes_;
+  Autotuner tuner_;
+  double m_per_shape_ = 0.0, p_per_shape_ = 0.0;
+
+  mutable std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  bool stop_ = false;
+  uint64_t next_id_ = 1;
+  uint64_t pending_ = 0;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs_;
+  // window statistics
+  std::vector<double> lat_;
+  uint64_t st_sub_ = 0, st_done_ = 0, st_fail_ = 0, st_batches_ = 0, st_breqs_ = 0;
+  double st_busy_ = 0.0, st_t0_ = 0.0;
+};
+
+}  // namespace cdx

@@ -211,3 +211,33 @@ def two_shapes_per_body(s: Scene) -> Scene:
     ident[:, 0] = 1
     return Scene(s.T.copy(), c(A), s.B.copy(), s.C.copy(), c(R), (np.arange(n) // 2).astype(np.uint32),
                  c(s.pos[::2]), c(ident), np.ones(nb, np.uint8), s.grid, s.name + "-2perbody", dict(s.meta))
+
+
+def quat_mult(a, b):
+    """Hamilton product of (n,4) scalar-first quaternions."""
+    aw, ax, ay, az = a[:, 0], a[:, 1], a[:, 2], a[:, 3]
+    bw, bx, by, bz = b[:, 0], b[:, 1], b[:, 2], b[:, 3]
+    return np.stack([aw * bw - ax * bx - ay * by - az * bz,
+                     aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw], axis=1)
+
+
+def pose_step(s: Scene, rng, sigma=0.05, angle=0.05):
+    """New absolute poses of every body of s after one served step.
+
+    Displacement ~ N(0, (sigma * size)^2) per axis, size = the body's largest
+    dimension; rotation by an angle ~ N(0, angle) rad about a uniform random
+    axis, applied on the left of the current rotation.
+    """
+    nb = s.nb
+    size = np.zeros(nb)
+    np.maximum.at(size, s.ID, np.abs(s.B).max(axis=1).astype(np.float64))
+    pos = s.pos.astype(np.float64) + rng.standard_normal((nb, 3)) * (sigma * size)[:, None]
+    axis = rng.standard_normal((nb, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    th = rng.standard_normal(nb) * angle
+    dq = np.concatenate([np.cos(th / 2)[:, None], np.sin(th / 2)[:, None] * axis], axis=1)
+    rot = quat_mult(dq, s.rot.astype(np.float64))
+    rot /= np.linalg.norm(rot, axis=1, keepdims=True)
+    return pos.astype(s.dtype), rot.astype(s.dtype)

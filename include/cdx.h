@@ -235,6 +235,71 @@ int cdx_batch_device_bytes(cdx_batch* batch, uint64_t* bytes);
 /* Estimated HBM bytes a scene of n shapes and nb bodies adds when resident. */
 int cdx_batch_scene_bytes(cdx_batch* batch, uint64_t n, uint64_t nb, uint64_t* bytes);
 
+/* ---- served request loop ------------------------------------------------
+ * No reference counterpart.  A server keeps scenes resident in HBM (one
+ * batch per size class: < 4096, < 32768 and >= 32768 shapes), admits a new
+ * scene only if its estimated footprint fits the HBM budget, and runs one
+ * worker thread per size class that keeps stepping: each step takes the
+ * oldest queued requests of its class (at most one per scene, up to
+ * max_batch_shapes shapes), so requests that arrive while a step runs join
+ * the next one.  Steps of different classes run on their own streams and
+ * overlap on the device.  A request carries pose updates for one scene
+ * (the body_pos / body_rot inputs of AABBGenerator::GenerateAABB,
+ * AABBGenerator.cuh:21-30) and yields that scene's contacts (the outputs of
+ * Narrowphase::DoNarrowphase, Narrowphase.cuh:28-34), equal to what the
+ * scene gives stepped alone.  Requests for one scene run in submit order.
+ * Thread safety: submit / poll / wait / result / read / release / stats may
+ * be called from any threads; add_scene from one thread at a time.  The
+ * last-error text is kept per calling thread. */
+typedef struct cdx_server cdx_server;
+
+typedef struct cdx_server_config {
+  int real_kind;             /* CDX_REAL_F64 or CDX_REAL_F32 */
+  int np_mode;               /* CDX_NP_* */
+  double tau;                /* narrowphase re-solve band, as cdx_scene_set_narrowphase_mode */
+  double envelope;           /* Narrowphase::SetCollisionEnvelope */
+  uint64_t hbm_budget;       /* bytes for resident scenes + step buffers; 0 = 90 % of free HBM */
+  uint64_t max_batch_shapes; /* shapes per step and class (bounds step latency); 0 = 524288 */
+  uint64_t max_pending;      /* queued requests before submit returns CDX_ERR_CAPACITY; 0 = 65536 */
+  int use_graph;             /* replay recorded HIP graphs */
+} cdx_server_config;
+
+typedef struct cdx_server_stats_t {
+  uint64_t submitted, completed, failed, steps, stepped_requests;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result ready */
+  double busy_ms;                         /* summed device time of the steps */
+  double window_s;                        /* wall time of the statistics window */
+} cdx_server_stats_t;
+
+int cdx_server_create(cdx_ctx* ctx, const cdx_server_config* cfg, cdx_server** out);
+/* Stops the workers (queued requests fail) and frees the scenes. */
+int cdx_server_destroy(cdx_server* server);
+const char* cdx_server_last_error(const cdx_server* server);
+/* bins 0,0,0 autotunes, as cdx_batch_add_scene.  CDX_ERR_CAPACITY when the
+ * scene does not fit the HBM budget. */
+int cdx_server_add_scene(cdx_server* server, uint64_t n, const int32_t* T, const void* A, const void* B,
+                         const void* C, const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos,
+                         const void* body_rot, const uint8_t* obj_active, int bins_x, int bins_y, int bins_z,
+                         uint32_t* scene_out);
+int cdx_server_scene_info(cdx_server* server, uint32_t scene, uint64_t* n_shapes, uint64_t* n_bodies,
+                          int* size_class, int* bins_xyz);
+/* Queues a step of one scene after replacing the poses of bodies idx[0..k)
+ * (idx NULL: bodies 0..k-1).  Inputs are copied before return.
+ * CDX_ERR_CAPACITY when max_pending requests are queued. */
+int cdx_server_submit(cdx_server* server, uint32_t scene, uint64_t k, const uint32_t* idx, const void* body_pos,
+                      const void* body_rot, uint64_t* request_out);
+int cdx_server_poll(cdx_server* server, uint64_t request, int* done);
+/* timeout_ms < 0 waits without limit; *done = 0 on timeout. */
+int cdx_server_wait(cdx_server* server, uint64_t request, double timeout_ms, int* done);
+int cdx_server_result(cdx_server* server, uint64_t request, uint64_t* n_contacts, double* latency_ms);
+int cdx_server_read_contacts(cdx_server* server, uint64_t request, void* norm, void* cpta, void* cptb, void* dpth,
+                             int32_t* bids, uint64_t capacity);
+/* Frees a finished request and its share of the pinned result block. */
+int cdx_server_release(cdx_server* server, uint64_t request);
+/* reset != 0 starts a new statistics window after reading. */
+int cdx_server_stats(cdx_server* server, cdx_server_stats_t* out, int reset);
+int cdx_server_memory(cdx_server* server, uint64_t* hbm_budget, uint64_t* hbm_committed, uint64_t* device_bytes);
+
 #ifdef __cplusplus
 }
 #endif
