@@ -45,19 +45,59 @@ def log(msg):
 
 
 def stage_bytes(s, c):
-    """Algorithmic HBM bytes per launch of each broadphase stage (compulsory reads + writes)."""
-    n, m, a, p = s.n, c.bin_entries, c.active_bins, c.pairs
+    """Algorithmic HBM bytes per launch of the single-kernel stages (compulsory reads + writes, packed sizes).
+
+    r = bytes per real of the AABBs (the input real type).  AABB: per shape
+    type 1 + body id 4 + local pos 3r + dims 3r + local rot 4r + body pos 3r
+    + body rot 4r in, AABB 6r out.  Bin count: AABB 6r in, count 4 out.  Key
+    emit: AABB 6r + offset 4 in, 8 per bin entry out.  Pair count pass: the
+    (bin, body) entry 8 and the AABB 6r per entry in, a count 4 per entry
+    out.  Pair emit pass: the same reads, 8 per pair out.
+    """
+    n, m, p = s.n, c.bin_entries, c.pairs
     r = s.dtype.itemsize
     return {
-        # per shape: type 1 + body id 4 + local pos 3r + dims 3r + local rot 4r, body pos 3r + rot 4r + active 1;
-        # out: AABB 6r + shape record (10r + 16)
-        "aabb": n * (1 + 4 + 17 * r + 1) + n * (6 * r + 10 * r + 16),
+        "aabb": n * (1 + 4 + 17 * r) + n * 6 * r,
         "bp_count": n * 6 * r + n * 4,
-        "bp_scan": n * 4 * 2,
         "bp_emit": n * (6 * r + 4) + m * 8,
-        "bp_runs": m * 8 + a * 8,
-        "bp_pairs": m * 8 + m * 6 * r + p * 8,
+        "bp_pairs": m * (8 + 6 * r) + m * 4,
+        "bp_pairemit": m * (8 + 6 * r) + p * 8,
     }
+
+
+# Kernel behind each single-kernel stage (names as rocprofv3 lists them).
+STAGE_KERNEL = {
+    "aabb": "k_aabb", "bp_count": "k_count", "bp_emit": "k_emit_keys", "bp_pairs": "k_pairs<double, false>",
+    "bp_pairemit": "k_pairs<double, true>",
+}
+
+
+def read_traffic(stage):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3 PMC summary, or None.
+
+    The summary (profiles/pmc_traffic_mixed1M.txt) holds key = value lines:
+    <stage>.fetch_kib / <stage>.write_kib (FETCH_SIZE / WRITE_SIZE per
+    dispatch) and copy.fetch_kib / copy.write_kib / copy.bytes for a plain
+    device copy of known size, which scales the counters to bytes.
+    """
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_mixed1M.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        line = line.split("#", 1)[0].strip()
+        if "=" in line:
+            k, v = line.split("=", 1)
+            try:
+                kv[k.strip()] = float(v)
+            except ValueError:
+                pass
+    need = [f"{stage}.fetch_kib", f"{stage}.write_kib", "copy.fetch_kib", "copy.write_kib", "copy.bytes"]
+    if not all(k in kv for k in need):
+        return None
+    f_scale = kv["copy.bytes"] / (kv["copy.fetch_kib"] * 1024.0)
+    w_scale = kv["copy.bytes"] / (kv["copy.write_kib"] * 1024.0)
+    return int(kv[f"{stage}.fetch_kib"] * 1024.0 * f_scale + kv[f"{stage}.write_kib"] * 1024.0 * w_scale)
 
 
 def cpu_baseline(oracle, s, min_seconds=10.0):
@@ -90,7 +130,8 @@ def cpu_baseline(oracle, s, min_seconds=10.0):
     }
 
 
-BP_STAGES = ["bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs", "bp_pairsort"]
+BP_STAGES = ["bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs", "bp_pairscan",
+             "bp_pairemit", "bp_pairsort"]
 
 
 def lattice_leg(pkg, scenes, ctx, side, steps):
@@ -146,6 +187,236 @@ def lattice_leg(pkg, scenes, ctx, side, steps):
     return out
 
 
+def _gen_threaded(fn, args, workers=16):
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda a: fn(*a), args))
+
+
+def batched_leg(pkg, scenes, ctx, count, steps):
+    """512 independent scenes (log-uniform 1,024-65,536 mixed bodies, U[2,8] neighbours), FP64 input, MIXED.
+
+    One batched step = AABB + broadphase + narrowphase over every scene as one
+    recorded HIP graph, grids from the per-size-class autotuner; it ends when
+    the per-scene contact counts are on the host.  The plain per-scene path
+    steps the same scenes one at a time (one resident cdx_scene each, same
+    grid), each step ending when its contact count is on the host.
+    """
+    log(f"batched leg: generating {count} scenes")
+    n, k = scenes.batched_sizes(count)
+    ss = _gen_threaded(scenes.fleet_scene, [(int(n[i]), float(k[i]), 0x5EED0003 + 1 + i) for i in range(count)])
+    b = pkg.Batch(ctx, np.float64)
+    t0 = time.perf_counter()
+    for s in ss:
+        b.add_scene(s, bins=(0, 0, 0))
+    t_add = time.perf_counter() - t0
+    b.set_mode(pkg.NP_MIXED, 1e-2)
+    for _ in range(2):
+        b.step(True)
+        tot = b.finish()
+    lat = []
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        b.step(True)
+        b.finish()
+        lat.append(time.perf_counter() - t1)
+        dev.append(b.step_ms())
+    total = time.perf_counter() - t0
+    per_scene = [b.scene_counts(j) for j in range(count)]
+    grids = {}
+    for j, s in enumerate(ss):
+        cls = int(np.log2(max(s.n, 1)) // 2)
+        grids.setdefault(cls, []).append(b.scene_bins(j))
+    # plain per-scene path on a sample
+    log("batched leg: plain per-scene path")
+    plain = []
+    mismatch = 0
+    for j, s in enumerate(ss):
+        sc = pkg.Scene(ctx, np.float64).load(s, bins=b.scene_bins(j))
+        sc.set_mode(pkg.NP_MIXED, 1e-2)
+        sc.step()
+        sc.sync()
+        t1 = time.perf_counter()
+        sc.step()
+        sc.sync()
+        plain.append(time.perf_counter() - t1)
+        cc = sc.counts()
+        if cc.pairs != per_scene[j].pairs or cc.contacts != per_scene[j].contacts:
+            mismatch += 1
+        sc.close()
+    lat_ms = np.array(lat) * 1e3
+    plain_sps = count / float(np.sum(plain))
+    sps = count * steps / total
+    out = {
+        "workload": f"{count} scenes, log-uniform 1024-65536 mixed bodies, U[2,8] neighbours, FP64 input, "
+                    "FP32 MPR + FP64 re-solve, autotuned grids",
+        "bodies": int(sum(s.n for s in ss)), "pairs": int(tot.pairs), "contacts": int(tot.contacts),
+        "scenes_per_s": round(sps, 1), "bodies_per_s": round(sps * sum(s.n for s in ss) / count, 1),
+        "p50_step_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "device_step_ms_p50": round(float(np.median(dev)), 3),
+        "plain_scenes_per_s": round(plain_sps, 1),
+        "plain_step_ms_p50": round(float(np.median(plain)) * 1e3, 3),
+        "speedup_vs_plain": round(sps / plain_sps, 2),
+        "per_scene_counts_match_plain": mismatch == 0,
+        "add_scenes_s": round(t_add, 2),
+        "autotuned_grids_by_size_class": {str(c): g[0] for c, g in sorted(grids.items())},
+        "device_bytes": int(b.device_bytes()),
+    }
+    b.close()
+    return out
+
+
+def served_leg(pkg, scenes, ctx, args):
+    """Poisson step requests against 4,096 resident scenes (log-uniform 256-262,144 mixed bodies, FP64, MIXED).
+
+    Each request replaces every body pose of one uniformly chosen scene with
+    the next of two precomputed pose sets (displacement N(0, (0.05 size)^2),
+    rotation by N(0, 0.05) rad about a random axis) and returns its contacts.
+    Latency runs from the submit call to the result being ready on the host.
+    The plain per-scene path runs the same requests one at a time on
+    resident cdx_scene objects (pose upload + step + contact count).
+    """
+    import threading
+    count = args.served_scenes
+    sizes = scenes.served_sizes(count)
+    log(f"served leg: generating {count} scenes ({int(sizes.sum())} bodies)")
+    rng0 = np.random.default_rng(0x5EED0005)
+    seeds = rng0.integers(0, 2**62, count)
+
+    def make(i):
+        s = scenes.fleet_scene(int(sizes[i]), 4.0, int(seeds[i]))
+        p1, r1 = scenes.pose_step(s, np.random.default_rng(int(seeds[i]) + 1))
+        return s, p1, r1
+
+    srv = pkg.Server(ctx, np.float64, mode=pkg.NP_MIXED, max_batch_shapes=args.served_batch_shapes)
+    poses = [None] * count
+    t0 = time.perf_counter()
+    from concurrent.futures import ThreadPoolExecutor
+    refused = 0
+    with ThreadPoolExecutor(16) as ex:
+        for c0 in range(0, count, 256):  # bounded look-ahead keeps host memory in check
+            for i, (s, p1, r1) in zip(range(c0, min(count, c0 + 256)), ex.map(make, range(c0, min(count, c0 + 256)))):
+                try:
+                    sid = srv.add_scene(s, bins=(0, 0, 0))
+                    assert sid == i - refused
+                    poses[i] = ((s.pos, s.rot), (p1, r1))
+                except pkg.CdxError as e:
+                    if e.status != 4:
+                        raise
+                    refused += 1
+            log(f"served leg: {c0 + 256} scenes added")
+    t_setup = time.perf_counter() - t0
+    ids = [i for i in range(count) if poses[i] is not None]
+    sid_of = {i: j for j, i in enumerate(ids)}
+    mem = srv.memory()
+    log(f"served leg: {len(ids)} scenes resident ({refused} refused) in {t_setup:.1f}s, {mem}")
+
+    # plain per-scene path
+    rng = np.random.default_rng(7)
+    sample = rng.choice(ids, min(args.plain_sample, len(ids)), replace=False)
+    plain = []
+    for i in sample:
+        (p0, r0), (p1, r1) = poses[i]
+        s = scenes.fleet_scene(int(sizes[i]), 4.0, int(seeds[i]))
+        sc = pkg.Scene(ctx, np.float64).load(s, bins=srv.scene_info(sid_of[i])["bins"])
+        sc.set_mode(pkg.NP_MIXED, 1e-2)
+        idx = np.arange(s.nb, dtype=np.uint32)
+        sc.step()
+        sc.sync()
+        t1 = time.perf_counter()
+        sc.update_bodies(idx, p1, r1)
+        sc.step()
+        sc.sync()
+        plain.append(time.perf_counter() - t1)
+        sc.close()
+    plain = np.array(plain)
+    # the sample's mean cost, weighted to the whole fleet's mean size
+    plain_sps = len(plain) / float(plain.sum())
+
+    # warm up every class (graph capture, result blocks)
+    state = {i: 0 for i in ids}
+    for i in ids[:64]:
+        srv.wait(srv.submit(sid_of[i], *poses[i][1]), 60_000)
+    srv.stats(reset=True)
+
+    points = []
+    for rate in [float(x) for x in args.served_rates.split(",") if x]:
+        outstanding = []
+        lock = threading.Lock()
+        done_flag = [False]
+
+        def reaper():
+            while True:
+                with lock:
+                    item = outstanding.pop(0) if outstanding else None
+                if item is None:
+                    if done_flag[0]:
+                        return
+                    time.sleep(0.0002)
+                    continue
+                srv.wait(item, 120_000)
+                srv.release(item)
+
+        th = threading.Thread(target=reaper)
+        th.start()
+        srv.stats(reset=True)
+        rr = np.random.default_rng(int(rate))
+        n_req = int(rate * args.served_seconds)
+        arrivals = np.cumsum(rr.exponential(1.0 / rate, n_req))
+        targets = rr.choice(ids, n_req)
+        refused_q = 0
+        t_start = time.perf_counter()
+        for a, i in zip(arrivals, targets):
+            now = time.perf_counter() - t_start
+            if a > now:
+                time.sleep(a - now)
+            state[i] ^= 1
+            pos, rot = poses[i][state[i]]
+            try:
+                rid = srv.submit(sid_of[i], pos, rot)
+            except pkg.CdxError:
+                refused_q += 1
+                continue
+            with lock:
+                outstanding.append(rid)
+        t_submit_end = time.perf_counter() - t_start
+        done_flag[0] = True
+        th.join()
+        st = srv.stats(reset=True)
+        wall = time.perf_counter() - t_start
+        pt = {"rate_rps": rate, "requests": int(st["completed"]), "queue_refused": refused_q,
+              "scenes_per_s": round(st["completed"] / wall, 1),
+              "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+              "steps": int(st["steps"]), "requests_per_step": round(st["stepped_requests"] / max(st["steps"], 1), 2),
+              "device_busy_frac": round(st["busy_ms"] / 1e3 / wall, 3), "submit_lag_s": round(t_submit_end - arrivals[-1], 3)}
+        log(f"served rate {rate}: {pt}")
+        points.append(pt)
+    ok = [p for p in points if p["p99_ms"] <= 2.0]
+    knee = max(ok, key=lambda p: p["scenes_per_s"]) if ok else None
+    best = max(points, key=lambda p: p["scenes_per_s"]) if points else None
+    out = {
+        "workload": f"{count} resident scenes, log-uniform 256-262144 mixed bodies (mean {sizes.mean():.0f}), "
+                    "FP64 input, FP32 MPR + FP64 re-solve, Poisson arrivals, full pose update per request",
+        "resident_scenes": len(ids), "refused_scenes": refused, "resident_bodies": int(sizes[ids].sum()),
+        "hbm_committed_GB": round(mem["hbm_committed"] / 1e9, 2), "device_GB": round(mem["device_bytes"] / 1e9, 2),
+        "setup_s": round(t_setup, 1),
+        "plain_scenes_per_s": round(plain_sps, 1),
+        "plain_p50_ms": round(float(np.median(plain)) * 1e3, 3),
+        "plain_p99_ms": round(float(np.percentile(plain, 99)) * 1e3, 3),
+        "plain_sample": int(len(plain)),
+        "sweep": points,
+        "knee_p99_le_2ms": knee,
+        "speedup_vs_plain_at_knee": round(knee["scenes_per_s"] / plain_sps, 2) if knee else None,
+        "best_scenes_per_s": best["scenes_per_s"] if best else None,
+        "speedup_vs_plain_best": round(best["scenes_per_s"] / plain_sps, 2) if best else None,
+    }
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +427,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lattice-side", type=int, default=256, help="0 skips the 16M-sphere broadphase leg")
     ap.add_argument("--lattice-steps", type=int, default=5)
+    ap.add_argument("--batched-scenes", type=int, default=512, help="0 skips the batched leg")
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--served-scenes", type=int, default=4096, help="0 skips the served leg")
+    ap.add_argument("--served-rates", type=str, default="1000,2000,4000,8000",
+                    help="Poisson arrival rates (requests/s) swept by the served leg")
+    ap.add_argument("--served-seconds", type=float, default=3.0, help="seconds of arrivals per rate")
+    ap.add_argument("--served-batch-shapes", type=int, default=1 << 19)
+    ap.add_argument("--plain-sample", type=int, default=256, help="scenes timed on the plain per-scene path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every box has a single MI355X; "
@@ -196,24 +475,38 @@ def main():
     stage_avg = {k: v / args.steps for k, v in stage_tot.items()}
     log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage_avg.items()))
 
-    # Roofline of the dominant HBM-bound broadphase kernel group and the
-    # narrowphase VALU share.
+    # Roofline of the slowest single-kernel HBM-bound stage (HIP events on
+    # the scene stream around that one launch).  The FP64 MPR re-solve is
+    # VALU/latency-bound and reported with the stage times instead.
     bytes_ = stage_bytes(s, c)
-    bp_stages = ["bp_count", "bp_scan", "bp_emit", "bp_runs", "bp_pairs"]
     dom = max(bytes_, key=lambda k: stage_avg.get(k, 0.0))
     dom_ms = stage_avg[dom]
     achieved = bytes_[dom] / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     roofline = {
-        "kernel": dom,
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": read_traffic(dom),
+        "kernel": STAGE_KERNEL[dom],
+        "stage": dom,
         "algorithmic_bytes": int(bytes_[dom]),
         "avg_launch_ms": round(dom_ms, 4),
+        "all_single_kernel_stages": {
+            k: {"ms": round(stage_avg[k], 4), "GBs": round(b / (stage_avg[k] * 1e-3) / 1e9, 1) if stage_avg[k] else None}
+            for k, b in bytes_.items()},
     }
+
+    batched = None
+    if args.batched_scenes > 0:
+        batched = batched_leg(pkg, scenes, ctx, args.batched_scenes, args.batched_steps)
+        log(f"batched: {json.dumps(batched)}")
+
+    served = None
+    if args.served_scenes > 0:
+        served = served_leg(pkg, scenes, ctx, args)
+        log(f"served: {json.dumps(served)}")
 
     lattice = None
     if args.lattice_side > 0:
@@ -229,7 +522,7 @@ def main():
 
     lat_ms = np.array(lat) * 1000.0
     out = {
-        "metric": "bodies/sec AABB+broad+MPR at 1M mixed shapes",
+        "metric": "bodies/sec AABB+broad+MPR at 1M mixed shapes; scenes/sec + p50/p99 step lat",
         "value": round(s.n * args.steps / total, 1),
         "unit": "bodies/s",
         "n_gpus": 1,
@@ -250,6 +543,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "lattice_16M_broadphase": lattice,
+        "batched_512": batched,
+        "served_4096": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -27,7 +27,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 REAL_F64, REAL_F32 = 0, 1
 NP_NATIVE, NP_MIXED, NP_FAST = 0, 1, 2
 STAGES = ["aabb", "bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs",
-          "bp_pairsort", "np_main", "np_resolve", "np_compact"]
+          "bp_pairsort", "np_main", "np_resolve", "np_compact", "bp_pairscan", "bp_pairemit"]
 
 # Every extern "C" symbol include/cdx.h declares.
 EXPORTS = [

@@ -116,6 +116,7 @@ TuneChoice Autotuner::choose(uint64_t n, const int32_t* T, const void* A, const 
       s->stage_ms(ms.data());
       double bp = 0.0;
       for (int k = ST_BP_BOUNDS; k <= ST_BP_PAIRSORT; ++k) bp += ms[k];
+      bp += ms[ST_BP_PAIRSCAN] + ms[ST_BP_PAIREMIT];
       t = std::min(t, bp);
       m = s->counts().m;
       p = s->counts().p;

@@ -47,9 +47,17 @@ namespace {
 
 thread_local std::string t_server_err;
 
-template <class F> int guarded(std::string* err, F&& f) {
+// Runs f with the HIP gate held shared (gate = true) and turns exceptions
+// into status codes.  Server calls pass gate = false: they may block on
+// other threads' work and take the gate themselves around their HIP calls.
+template <class F> int guarded(std::string* err, F&& f, bool gate = true) {
   try {
-    f();
+    if (gate) {
+      cdx::GateShared g;
+      f();
+    } else {
+      f();
+    }
     if (err) err->clear();
     return CDX_OK;
   } catch (const cdx::Error& e) {
@@ -429,7 +437,7 @@ const char* cdx_server_last_error(const cdx_server* server) {
 
 #define SERVER_CALL(...)                                                 \
   if (!server) return CDX_ERR_INVALID;                                   \
-  return guarded(&t_server_err, [&] { cdx::Server& sv = *server->srv; __VA_ARGS__; })
+  return guarded(&t_server_err, [&] { cdx::Server& sv = *server->srv; __VA_ARGS__; }, false)
 
 int cdx_server_add_scene(cdx_server* server, uint64_t n, const int32_t* T, const void* A, const void* B,
                          const void* C, const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos,

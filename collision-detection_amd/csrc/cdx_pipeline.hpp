@@ -23,11 +23,13 @@ enum Stage : int {
   ST_BP_EMIT,
   ST_BP_SORT,
   ST_BP_RUNS,
-  ST_BP_PAIRS,
+  ST_BP_PAIRS,  // pair counting pass
   ST_BP_PAIRSORT,
   ST_NP_MAIN,
   ST_NP_RESOLVE,
   ST_NP_COMPACT,
+  ST_BP_PAIRSCAN,  // scan of per-entry pair counts (between the two pair passes)
+  ST_BP_PAIREMIT,  // pair emission pass
   ST_COUNT
 };
 
@@ -115,8 +117,9 @@ class BatchBase {
   // the copies are asynchronous on the batch stream.
   // The block holds k uint32 ids, then k real3 positions at byte offset
   // pose_block_pos_offset(k), then k real4 rotations at
-  // pose_block_rot_offset(k).
-  virtual void update_bodies_pinned(uint64_t k, const void* block) = 0;
+  // pose_block_rot_offset(k).  first_body >= 0: the updates cover global
+  // bodies first_body .. first_body+k-1 in order and the id list is unused.
+  virtual void update_bodies_pinned(uint64_t k, const void* block, int64_t first_body) = 0;
   // After finish(): contact range of selected scene j, and every contact of
   // the step (global body ids) copied into host arrays of totals().c rows.
   virtual void scene_contact_range(uint64_t j, uint64_t* begin, uint64_t* end) const = 0;

@@ -66,7 +66,7 @@ __global__ void k_scatter_bodies(uint64_t k, uint64_t base, uint64_t nb_scene, u
                                  const R* __restrict__ rot, R* __restrict__ bpos, R* __restrict__ brot) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k) return;
-  uint32_t b = idx[j];
+  uint64_t b = idx ? idx[j] : j;  // no index list: bodies 0..k-1 of the scene
   if (b >= nb_scene) return;
   uint64_t g = base + b;
   for (int c = 0; c < 3; ++c) bpos[c * stride + g] = pos[3 * j + c];
@@ -99,26 +99,21 @@ __global__ void k_aabb_from_aos(uint64_t n, uint64_t NS, const R* __restrict__ a
 
 // ------------------------------------------------------------ host side --
 
-// Device-wide synchronisation (buffer growth) must not run while another
-// thread records a stream into a HIP graph; both take this lock.
-std::recursive_mutex& capture_mutex() {
-  static std::recursive_mutex m;
-  return m;
-}
-
+// Buffers grow only inside library calls, which hold the HIP gate shared,
+// so no other thread is recording a graph meanwhile.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   ~DevBuf() {
     if (p) {
-      std::lock_guard<std::recursive_mutex> lk(capture_mutex());
+      GateShared g;
       (void)hipFree(p);
     }
   }
   // Grows to at least b bytes; contents are not kept.
   void reserve(size_t b) {
     if (b <= bytes) return;
-    std::lock_guard<std::recursive_mutex> lk(capture_mutex());
+    GateShared g;
     release();
     size_t nb = b < 256 ? 256 : b;
     CDX_HIP_CHECK(hipMalloc(&p, nb));
@@ -126,7 +121,7 @@ struct DevBuf {
   }
   void release() {
     if (p) {
-      std::lock_guard<std::recursive_mutex> lk(capture_mutex());
+      GateShared g;
       // kernels queued on any stream may still read the old block
       CDX_HIP_CHECK(hipDeviceSynchronize());
       CDX_HIP_CHECK(hipFree(p));
@@ -168,6 +163,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     std::memset(&counts_, 0, sizeof(counts_));
   }
   ~Engine() override {
+    GateShared gate;
     (void)hipSetDevice(dev_);
     (void)hipStreamSynchronize(s_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -558,17 +554,22 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     return scenes_[s].body_begin;
   }
 
-  void update_bodies_pinned(uint64_t k, const void* block) override {
+  void update_bodies_pinned(uint64_t k, const void* block, int64_t first_body) override {
     if (k == 0) return;
     const uint64_t nbt = scenes_.empty() ? 0 : scenes_.back().body_begin + scenes_.back().n_bodies;
     const int kind = sizeof(R) == 8 ? REAL_F64 : REAL_F32;
     const uint64_t bytes = pose_block_bytes(k, kind);
     upd_.reserve(bytes);
-    CDX_HIP_CHECK(hipMemcpyAsync(upd_.p, block, bytes, hipMemcpyHostToDevice, s_));
+    const char* hb = static_cast<const char*>(block);
+    const uint64_t from = first_body >= 0 ? pose_block_pos_offset(k) : 0;  // contiguous: no id list
+    CDX_HIP_CHECK(hipMemcpyAsync(upd_.as<char>() + from, hb + from, bytes - from, hipMemcpyHostToDevice, s_));
     const char* base = upd_.as<char>();
-    // indices past the resident bodies are dropped by the kernel
-    hipLaunchKernelGGL(k_scatter_bodies<R>, dim3(blocks_for(k, 256)), dim3(256), 0, s_, k, (uint64_t)0, nbt, NBC_,
-                       (const uint32_t*)base, (const R*)(base + pose_block_pos_offset(k)),
+    const uint32_t* didx = first_body >= 0 ? nullptr : (const uint32_t*)base;
+    const uint64_t b0 = first_body >= 0 ? (uint64_t)first_body : 0;
+    CDX_REQUIRE(first_body < 0 || b0 + k <= nbt, ERR_INVALID, "pose update past the resident bodies");
+    // listed indices past the resident bodies are dropped by the kernel
+    hipLaunchKernelGGL(k_scatter_bodies<R>, dim3(blocks_for(k, 256)), dim3(256), 0, s_, k, b0,
+                       first_body >= 0 ? k : nbt, NBC_, didx, (const R*)(base + pose_block_pos_offset(k)),
                        (const R*)(base + pose_block_rot_offset(k, kind)), bpos_.as<R>(), brot_.as<R>());
     aabb_current_ = false;
     records_valid_ = false;
@@ -889,8 +890,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), pcount_.as<uint32_t>(), (const uint32_t*)nullptr,
                        (uint64_t)0, (uint64_t*)nullptr, st());
+    mark(ST_BP_PAIRS, false);
+    mark(ST_BP_PAIRSCAN, true);
     scan_excl_u32(pcount_.as<uint32_t>(), poffs_.as<uint32_t>(), mcap_, &st()->P, radix_tmp_.as<uint32_t>(), s_,
                   &st()->M);
+    mark(ST_BP_PAIRSCAN, false);
     if (exact_) {
       read_status();
       P_ = hstat_->P;
@@ -900,11 +904,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
     pairs_.reserve(pcap_ * 8);
     pairs_tmp_.reserve(pcap_ * 8);
+    mark(ST_BP_PAIREMIT, true);
     hipLaunchKernelGGL((k_pairs<R, true>), dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (uint32_t*)nullptr,
                        (const uint32_t*)poffs_.as<uint32_t>(), pcap_, pairs_.as<uint64_t>(), st());
-    mark(ST_BP_PAIRS, false);
+    mark(ST_BP_PAIREMIT, false);
     std::vector<int> ps;
     for (int b = 0; b < shape_bits_; b += 8) ps.push_back(b);
     for (int b = 0; b < shape_bits_; b += 8) ps.push_back(32 + b);
@@ -1050,7 +1055,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
           graph_exec_ = nullptr;
         }
         hipGraph_t g = nullptr;
-        std::lock_guard<std::recursive_mutex> lk(capture_mutex());
+        GateExclusive gate;  // no other thread calls HIP while this stream records
         CDX_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         try {
           enqueue_capacity_step();

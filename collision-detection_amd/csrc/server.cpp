@@ -50,6 +50,7 @@ class PinnedPool {
       }
     }
     void* p = nullptr;
+    GateShared g;
     CDX_HIP_CHECK(hipHostMalloc(&p, b, hipHostMallocDefault));
     return p;
   }
@@ -71,7 +72,10 @@ PinnedPool& pinned_pool() {
 }  // namespace
 
 ResultBlock::~ResultBlock() {
-  if (base) (void)hipHostFree(base);
+  if (base) {
+    GateShared g;
+    (void)hipHostFree(base);
+  }
 }
 
 struct PoseBlock {
@@ -88,6 +92,7 @@ double Server::now_ms() {
 }
 
 Server::Server(int device, const ServerConfig& cfg) : dev_(device), cfg_(cfg), tuner_(device, cfg.real_kind) {
+  GateShared gate;
   CDX_REQUIRE(cfg.real_kind == REAL_F64 || cfg.real_kind == REAL_F32, ERR_INVALID, "unknown real kind");
   CDX_REQUIRE(cfg.max_batch_shapes > 0 && cfg.max_pending > 0, ERR_INVALID, "batch and queue limits must be positive");
   CDX_HIP_CHECK(hipSetDevice(dev_));
@@ -134,6 +139,7 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
   Class& k = *classes_[c];
   int g[3] = {bx, by, bz};
   std::lock_guard<std::mutex> elk(k.eng_mu);
+  GateShared gate;  // after the engine lock: a worker holding it may need the gate exclusive
   const bool first_in_class = k.eng->scene_count() == 0;
   // footprint: the scene's resident arrays, plus the class's step buffers
   // the first time the class gets a scene
@@ -186,6 +192,7 @@ void Server::scene_info(uint32_t s, uint64_t* n, uint64_t* nb, int* cls, int bin
 }
 
 uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void* pos, const void* rot) {
+  const double t_enter = now_ms();
   SceneRef ref;
   {
     std::lock_guard<std::mutex> lk(mu_);
@@ -206,12 +213,14 @@ uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void*
   if (k) {
     auto blk = std::make_shared<PoseBlock>();
     blk->p = pinned_pool().get(pose_block_bytes(k, kind), &blk->bucket);
-    uint32_t* gi = static_cast<uint32_t*>(blk->p);
-    for (uint64_t j = 0; j < k; ++j) {
-      uint64_t b = idx ? idx[j] : j;
-      CDX_REQUIRE(b < ref.nb, ERR_INVALID, "body index out of range in a pose update");
-      gi[j] = (uint32_t)(ref.body_begin + b);
+    if (idx) {
+      uint32_t* gi = static_cast<uint32_t*>(blk->p);
+      for (uint64_t j = 0; j < k; ++j) {
+        CDX_REQUIRE(idx[j] < ref.nb, ERR_INVALID, "body index out of range in a pose update");
+        gi[j] = (uint32_t)(ref.body_begin + idx[j]);
+      }
     }
+    req->contiguous = idx == nullptr;
     char* base = static_cast<char*>(blk->p);
     std::memcpy(base + pose_block_pos_offset(k), pos, k * 3 * rs);
     std::memcpy(base + pose_block_rot_offset(k, kind), rot, k * 4 * rs);
@@ -223,7 +232,7 @@ uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void*
     CDX_REQUIRE(pending_ < cfg_.max_pending, ERR_CAPACITY, "request queue is full");
     id = next_id_++;
     req->id = id;
-    req->t_submit = now_ms();
+    req->t_submit = t_enter;  // latency includes the input copy
     reqs_[id] = req;
     classes_[ref.cls]->queue.push_back(req);
     ++pending_;
@@ -234,7 +243,10 @@ uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void*
 }
 
 void Server::worker_loop(int c) {
-  (void)hipSetDevice(dev_);
+  {
+    GateShared gate;
+    (void)hipSetDevice(dev_);
+  }
   Class& k = *classes_[c];
   std::vector<std::shared_ptr<Request>> batch;
   for (;;) {
@@ -293,11 +305,12 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
   double step_ms = 0.0;
   try {
     std::lock_guard<std::mutex> elk(k.eng_mu);
+    GateShared gate;
     BatchBase& e = *k.eng;
     std::vector<uint32_t> sel;
     sel.reserve(batch.size());
     for (auto& r : batch) {
-      if (r->k) e.update_bodies_pinned(r->k, r->poses->p);
+      if (r->k) e.update_bodies_pinned(r->k, r->poses->p, r->contiguous ? (int64_t)r->body_base : -1);
       sel.push_back(r->local);
     }
     e.select(sel.data(), sel.size());
@@ -403,10 +416,12 @@ void Server::read_contacts(uint64_t id, void* norm, void* cpta, void* cptb, void
 }
 
 void Server::release(uint64_t id) {
+  std::shared_ptr<Request> r;  // destroyed after the lock: it may free a pinned block
   std::lock_guard<std::mutex> lk(mu_);
   auto it = reqs_.find(id);
   CDX_REQUIRE(it != reqs_.end(), ERR_INVALID, "unknown request id");
   CDX_REQUIRE(it->second->state >= 2, ERR_NOT_READY, "request still queued or running");
+  r = std::move(it->second);
   reqs_.erase(it);
 }
 

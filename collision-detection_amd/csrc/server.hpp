@@ -90,6 +90,7 @@ class Server {
     uint64_t n = 0;                 // shapes of the scene
     uint64_t body_base = 0;         // global body id of the scene's body 0
     uint64_t k = 0;
+    bool contiguous = false;  // updates cover bodies 0..k-1 (no id list)
     std::shared_ptr<PoseBlock> poses;  // pinned: global ids, positions, rotations
     double t_submit = 0.0;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed

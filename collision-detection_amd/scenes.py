@@ -185,16 +185,53 @@ def mixed_scene_of_size(n, neighbours, seed, dtype=np.float64):
     return mixed_bodies(n=n, seed=seed, dtype=dtype, neighbours=neighbours)
 
 
+_PAIR_VOLUME = None
+
+
+def mix_pair_volume(samples=2_000_000, seed=0x5EED00FF):
+    """E[prod(2hA + 2hB)] over independent bodies of the mixed shape set (cached)."""
+    global _PAIR_VOLUME
+    if _PAIR_VOLUME is None:
+        rng = np.random.default_rng(seed)
+        T, B, rot = mixed_shapes(rng, samples)
+        h = half_extents(T, B, rot)
+        i = rng.permutation(samples)
+        _PAIR_VOLUME = float(np.prod(2 * h + 2 * h[i], axis=1).mean())
+    return _PAIR_VOLUME
+
+
+def fleet_scene(n, neighbours, seed, dtype=np.float64):
+    """One scene of the batched / served workloads: n mixed bodies in a cube of side (n E / k)^(1/3).
+
+    E is the mix's expected AABB-overlap volume, so a body has ~k AABB
+    partners; the grid is left to the autotuner (bins (0, 0, 0)).
+    """
+    rng = np.random.default_rng(seed)
+    T, B, rot = mixed_shapes(rng, n)
+    L = float((n * mix_pair_volume() / neighbours) ** (1.0 / 3.0))
+    pos = rng.uniform(0.0, L, (n, 3))
+    return _one_shape_per_body(T, B, pos, rot, dtype, (0, 0, 0), "fleet",
+                               {"n": n, "seed": seed, "L": L, "neighbours": neighbours})
+
+
 def batched_sizes(count=512, lo=1024, hi=65536, seed=0x5EED0003):
-    """Shape counts of the batched workload: log-uniform in [lo, hi]."""
+    """Shape counts log-uniform in [lo, hi] and densities U[2, 8] neighbours per body."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
+    k = rng.uniform(2.0, 8.0, count)
+    return n, k
+
+
+def batched_scenes(count=512, lo=1024, hi=65536, seed=0x5EED0003, dtype=np.float64):
+    """512 independent mixed scenes, sizes log-uniform in [1024, 65536], U[2, 8] neighbours each."""
+    n, k = batched_sizes(count, lo, hi, seed)
+    return [fleet_scene(int(n[i]), float(k[i]), seed + 1 + i, dtype) for i in range(count)]
+
+
+def served_sizes(count=4096, lo=256, hi=262144, seed=0x5EED0004):
+    """Shape counts of the served workload: log-uniform in [256, 262144]."""
     rng = np.random.default_rng(seed)
     return np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
-
-
-def batched_scenes(count=512, lo=1024, hi=65536, seed=0x5EED0003, neighbours=4.0, dtype=np.float64):
-    """Independent mixed scenes of log-uniform size (seeds seed+1+i), one per simulation instance."""
-    return [mixed_scene_of_size(int(n), neighbours, seed + 1 + i, dtype)
-            for i, n in enumerate(batched_sizes(count, lo, hi, seed))]
 
 
 def two_shapes_per_body(s: Scene) -> Scene:

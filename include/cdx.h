@@ -72,12 +72,14 @@ extern "C" {
 #define CDX_STAGE_BP_EMIT 4
 #define CDX_STAGE_BP_SORT 5
 #define CDX_STAGE_BP_RUNS 6
-#define CDX_STAGE_BP_PAIRS 7
+#define CDX_STAGE_BP_PAIRS 7 /* pair counting pass; the scan and the emission pass are 12 and 13 */
 #define CDX_STAGE_BP_PAIRSORT 8
 #define CDX_STAGE_NP_MAIN 9
 #define CDX_STAGE_NP_RESOLVE 10
 #define CDX_STAGE_NP_COMPACT 11
-#define CDX_STAGE_COUNT 12
+#define CDX_STAGE_BP_PAIRSCAN 12
+#define CDX_STAGE_BP_PAIREMIT 13
+#define CDX_STAGE_COUNT 14
 
 typedef struct cdx_ctx cdx_ctx;
 typedef struct cdx_scene cdx_scene;
