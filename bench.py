@@ -417,6 +417,26 @@ def served_leg(pkg, scenes, ctx, args):
     return out
 
 
+def copy_calibration(nbytes, reps=5):
+    """A plain device copy of nbytes (read nbytes, write nbytes) for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = nbytes // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ms = []
+    for _ in range(reps):
+        ev[0].record()
+        torch.mul(src, 1.0, out=dst)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms.append(ev[0].elapsed_time(ev[1]))
+    best = min(ms)
+    print(json.dumps({"copy_bytes": n * 4, "kernel": "torch elementwise mul by 1 (read n, write n)",
+                      "best_ms": round(best, 4), "GBs": round(2 * n * 4 / (best * 1e-3) / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,7 +455,12 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=3.0, help="seconds of arrivals per rate")
     ap.add_argument("--served-batch-shapes", type=int, default=1 << 19)
     ap.add_argument("--plain-sample", type=int, default=256, help="scenes timed on the plain per-scene path")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        copy_calibration(args.copy_bytes)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every box has a single MI355X; "
                                    "multi-GPU sharding is described in DESIGN.md, not built",
