@@ -1,0 +1,70 @@
+"""Writes the golden fixtures of this directory (run from the repository root).
+
+Each fixture is a small seeded case: its inputs in the reference layouts and
+the oracle's outputs (AABBs, per-AABB bin counts, sorted (bin, body)
+entries, active bins and occupancy, the pair list, the shifted AABBs, the
+contacts and the permuted pair list).  The oracle's broadphase is pinned
+bit-exactly by the reference's own tests and by brute force
+(tests/test_oracle_kat.py, tests/test_oracle_pins.py); these files let the
+GPU parity tests check against fixed data without regenerating it.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def cases(scenes):
+    s = scenes
+    out = {
+        "spheres512_f64": s.small_spheres(n=512, seed=0x5EED0100, grid=(10, 10, 10)),
+        "mixed400_f64": s.mixed_bodies(n=400, seed=0x5EED0101),
+        "mixed400_f32": s.mixed_bodies(n=400, seed=0x5EED0102, dtype=np.float32),
+        "twoshape300_f64": s.two_shapes_per_body(s.mixed_bodies(n=300, seed=0x5EED0103)),
+    }
+    # envelope and inactive bodies
+    e = s.mixed_bodies(n=300, seed=0x5EED0104)
+    e.active = e.active.copy()
+    e.active[::7] = 0
+    out["mixed300_env_inactive_f64"] = e
+    return out
+
+
+ENVELOPE = {"mixed300_env_inactive_f64": 0.002}
+
+
+def write(name, sc, oracle):
+    ab = oracle.aabb(sc)
+    bp = oracle.broadphase(ab, sc.grid)
+    env = ENVELOPE.get(name, 0.0)
+    npr = oracle.narrowphase(sc, bp.pairs, env)
+    np.savez_compressed(
+        os.path.join(HERE, f"{name}.npz"),
+        T=sc.T, A=sc.A, B=sc.B, C=sc.C, R=sc.R, ID=sc.ID, pos=sc.pos, rot=sc.rot, active=sc.active,
+        grid=np.array(sc.grid, np.int32), envelope=np.float64(env),
+        aabb=ab, aabb_shifted=bp.aabb_shifted, bin_counts=bp.bin_counts, sorted_keys=bp.sorted_keys,
+        sorted_bodies=bp.sorted_bodies, active_keys=bp.active_keys, active_counts=bp.active_counts,
+        pairs=bp.pairs, norm=npr.norm, cpta=npr.cpta, cptb=npr.cptb, dpth=npr.dpth, bids=npr.bids,
+        pairs_permuted=npr.pairs_permuted)
+    print(f"{name}: n={sc.n} pairs={len(bp.pairs)} contacts={npr.count}")
+
+
+def main():
+    from conftest import load_package
+    load_package()
+    import collision_detection_amd.scenes as scenes
+    import oracle
+    oracle.lib()
+    for name, sc in cases(scenes).items():
+        write(name, sc, oracle)
+
+
+if __name__ == "__main__":
+    main()
