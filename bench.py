@@ -52,7 +52,8 @@ def stage_bytes(s, c):
     + body rot 4r in, AABB 6r out.  Bin count: AABB 6r in, count 4 out.  Key
     emit: AABB 6r + offset 4 in, 8 per bin entry out.  Pair count pass: the
     (bin, body) entry 8 and the AABB 6r per entry in, a count 4 per entry
-    out.  Pair emit pass: the same reads, 8 per pair out.
+    out.  Pair emit pass: the entry's hit mask 8 and position 4 in, per pair
+    the partner's entry 8 in and its id 4 out.
     """
     n, m, p = s.n, c.bin_entries, c.pairs
     r = s.dtype.itemsize
@@ -61,14 +62,14 @@ def stage_bytes(s, c):
         "bp_count": n * 6 * r + n * 4,
         "bp_emit": n * (6 * r + 4) + m * 8,
         "bp_pairs": m * (8 + 6 * r) + m * 4,
-        "bp_pairemit": m * (8 + 6 * r) + p * 8,
+        "bp_pairemit": m * (8 + 4) + p * (8 + 4),
     }
 
 
 # Kernel behind each single-kernel stage (names as rocprofv3 lists them).
 STAGE_KERNEL = {
-    "aabb": "k_aabb", "bp_count": "k_count", "bp_emit": "k_emit_keys", "bp_pairs": "k_pairs<double, false>",
-    "bp_pairemit": "k_pairs<double, true>",
+    "aabb": "k_aabb<double>", "bp_count": "k_count<double>", "bp_emit": "k_emit_keys<double>",
+    "bp_pairs": "k_pairs<double>", "bp_pairemit": "k_pairs_emit<double>"
 }
 
 

@@ -192,8 +192,8 @@ template <class R>
 __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                  uint64_t NS, const R* __restrict__ aabb,
                                                  const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ counts,
-                                                 BpRec<R>* __restrict__ brec, uint32_t* __restrict__ kpart,
-                                                 DevStatus* st) {
+                                                 BpRec<R>* __restrict__ brec, uint32_t* __restrict__ bscene,
+                                                 uint32_t* __restrict__ kpart, DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   __shared__ uint32_t skm[CHUNK / WAVE];
   const Chunk ch = chunks[blockIdx.x];
@@ -212,9 +212,8 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
     Cells c;
     c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
     c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
-    r.g[0] = c.x0; r.g[1] = c.y0; r.g[2] = c.z0;
-    r.scene = ch.scene;
     brec[i] = r;
+    bscene[i] = ch.scene;
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
       counts[slot] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = d.key_base + hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, d.gx, d.gy);
@@ -257,7 +256,8 @@ template <class R>
 __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                      const SceneDesc<R>* __restrict__ desc,
                                                      const BpRec<R>* __restrict__ brec, const uint32_t* __restrict__ offs,
-                                                     uint64_t mcap, uint64_t* __restrict__ keys, DevStatus* st) {
+                                                     uint64_t mcap, uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ pcount, DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
   if (threadIdx.x >= ch.len) return;
@@ -265,15 +265,19 @@ __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ c
   const SceneDesc<R>& d = desc[ch.scene];
   const BpRec<R> r = brec[i];
   Cells c;
-  c.x0 = r.g[0]; c.y0 = r.g[1]; c.z0 = r.g[2];
+  c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
   c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
   if (!cells_ok(c, d.gx, d.gy, d.gz)) return;
   uint64_t o = offs[(uint64_t)blockIdx.x * CHUNK + threadIdx.x];
   for (uint32_t x = (uint32_t)c.x0; x <= (uint32_t)c.x1; ++x)
     for (uint32_t y = (uint32_t)c.y0; y <= (uint32_t)c.y1; ++y)
       for (uint32_t z = (uint32_t)c.z0; z <= (uint32_t)c.z1; ++z) {
-        if (o < mcap) keys[o] = ((uint64_t)(d.key_base + hash_index(x, y, z, d.gx, d.gy)) << 32) | i;
-        else atomicOr(&st->overflow, 1u);
+        if (o < mcap) {
+          keys[o] = ((uint64_t)(d.key_base + hash_index(x, y, z, d.gx, d.gy)) << 32) | i;
+          pcount[o] = 0;  // pair count of this entry, set by the pair search
+        } else {
+          atomicOr(&st->overflow, 1u);
+        }
         ++o;
       }
 }
@@ -296,6 +300,30 @@ __global__ __launch_bounds__(256) void k_count_runs(uint64_t mcap, const uint64_
   }
 }
 
+// Position, in the emission order of k_emit_keys (x outer, z inner), of the
+// cell with scene-relative key kk among the cells c of one shape.  A max
+// corner at index == grid size makes up to four cells share a key
+// (Broadphase.cu:140-147); the first one inside the shape's range is taken.
+// Returns ~0u if no candidate lies in the range (never for emitted keys).
+__device__ __forceinline__ uint32_t local_cell_index(uint32_t kk, const Cells& c, int gx, int gy) {
+  const uint32_t q = kk / (uint32_t)gx, i0 = kk - q * (uint32_t)gx;
+  for (int ai = 0; ai < 2; ++ai) {
+    uint32_t i, qq;
+    if (ai == 0) { i = i0; qq = q; }
+    else { if (i0 != 0 || q == 0) break; i = (uint32_t)gx; qq = q - 1; }
+    const uint32_t l0 = qq / (uint32_t)gy, j0 = qq - l0 * (uint32_t)gy;
+    for (int aj = 0; aj < 2; ++aj) {
+      uint32_t j, l;
+      if (aj == 0) { j = j0; l = l0; }
+      else { if (j0 != 0 || l0 == 0) break; j = (uint32_t)gy; l = l0 - 1; }
+      if ((int)i >= c.x0 && (int)i <= c.x1 && (int)j >= c.y0 && (int)j <= c.y1 && (int)l >= c.z0 && (int)l <= c.z1)
+        return ((i - (uint32_t)c.x0) * (uint32_t)(c.y1 - c.y0 + 1) + (j - (uint32_t)c.y0)) *
+                   (uint32_t)(c.z1 - c.z0 + 1) + (l - (uint32_t)c.z0);
+    }
+  }
+  return ~0u;
+}
+
 // Pair search with de-duplication by owning bin.  A pair (a, b) of
 // overlapping AABBs (closed test on shifted coordinates, Broadphase.cu:237 /
 // :296) is reported only from the bin whose key is the hash of the cell
@@ -303,21 +331,45 @@ __global__ __launch_bounds__(256) void k_count_runs(uint64_t mcap, const uint64_
 // shapes appear in its run of sorted entries, and every other run skips the
 // pair.  Repeated entries of one shape in a run (hash aliasing at index ==
 // grid size, Broadphase.cu:140-147) are skipped, so each pair is emitted
-// once and the reference's sort + unique (:522-535) reduces to a sort.
+// once and the reference's sort + unique (:522-535) reduces to ordering.
 //
 // One thread per sorted entry i pairs shape(i) with the later entries of
 // the same run (shapes ascend inside a run, so shape(i) < shape(k)).  The
 // workgroup's entries and the next PAIR_AHEAD are staged in LDS with their
 // records; only runs reaching past the window read global memory.
+//
+// Output order: the count pass stores each entry's pair count at the
+// entry's emission position (offs of its shape + its cell's index among the
+// shape's cells), so the scan of those counts groups the pairs by first
+// shape, shapes in step order.  It also leaves a hit mask per entry (bit d:
+// the entry d+1 places later is a partner; bit 63: the run reaches past 63
+// later entries) and the emission position, so the emit pass writes each
+// entry's partners without repeating the tests.  k_pair_groups then sorts
+// each shape's group: no global sort of the pair list is needed.
 constexpr int PAIR_BS = 256;
-constexpr int PAIR_AHEAD = 128;
+constexpr int PAIR_AHEAD = 64;
+constexpr uint64_t MASK_TAIL = 1ull << 63;
 
-template <class R, bool EMIT>
+template <class R>
+__device__ __forceinline__ bool pair_test(const BpRec<R>& ra, const int ga[3], const BpRec<R>& rb, const R inv[3],
+                                          uint32_t K, uint32_t kb, int gx, int gy) {
+  const bool hit = (ra.lo[0] <= rb.hi[0] && rb.lo[0] <= ra.hi[0]) && (ra.lo[1] <= rb.hi[1] && rb.lo[1] <= ra.hi[1]) &&
+                   (ra.lo[2] <= rb.hi[2] && rb.lo[2] <= ra.hi[2]);
+  if (!hit) return false;
+  const uint32_t ox = (uint32_t)max(ga[0], cell_of(rb.lo[0], inv[0]));
+  const uint32_t oy = (uint32_t)max(ga[1], cell_of(rb.lo[1], inv[1]));
+  const uint32_t oz = (uint32_t)max(ga[2], cell_of(rb.lo[2], inv[2]));
+  return kb + hash_index(ox, oy, oz, gx, gy) == K;
+}
+
+template <class R>
 __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t* __restrict__ keys,
                                                    const BpRec<R>* __restrict__ brec,
+                                                   const uint32_t* __restrict__ bscene,
                                                    const SceneDesc<R>* __restrict__ desc,
-                                                   uint32_t* __restrict__ pcount, const uint32_t* __restrict__ poffs,
-                                                   uint64_t pcap, uint64_t* __restrict__ pairs, DevStatus* st) {
+                                                   const uint32_t* __restrict__ offs, uint32_t* __restrict__ pcount,
+                                                   uint64_t* __restrict__ mask, uint32_t* __restrict__ jpos,
+                                                   DevStatus* st) {
   constexpr int W = PAIR_BS + PAIR_AHEAD;
   __shared__ BpRec<R> srec[W];
   __shared__ uint64_t skey[W];
@@ -336,42 +388,233 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   const uint64_t i = b0 + threadIdx.x;
   if (i >= m) return;
   const uint64_t ki = skey[threadIdx.x];
-  uint32_t cnt = 0;
   const uint64_t kprev = i == 0 ? ~ki : (threadIdx.x > 0 ? skey[threadIdx.x - 1] : keys[i - 1]);
-  if (kprev != ki) {
+  if (kprev == ki) {  // repeated entry of one shape in a bin (aliasing)
+    mask[i] = 0;
+    return;
+  }
+  const uint32_t K = (uint32_t)(ki >> 32), a = (uint32_t)ki;
+  const BpRec<R> ra = srec[threadIdx.x];
+  const SceneDesc<R>& d = desc[bscene[a]];
+  const int gx = d.gx, gy = d.gy;
+  const uint32_t kb = d.key_base;
+  const R inv[3] = {d.inv[0], d.inv[1], d.inv[2]};
+  Cells c;
+  c.x0 = cell_of(ra.lo[0], inv[0]); c.y0 = cell_of(ra.lo[1], inv[1]); c.z0 = cell_of(ra.lo[2], inv[2]);
+  c.x1 = cell_of(ra.hi[0], inv[0]); c.y1 = cell_of(ra.hi[1], inv[1]); c.z1 = cell_of(ra.hi[2], inv[2]);
+  const int ga[3] = {c.x0, c.y0, c.z0};
+  // emission position of this entry
+  const uint32_t li = local_cell_index(K - kb, c, gx, gy);
+  const uint64_t slot_a = (uint64_t)d.chunk_begin * CHUNK + (a - d.shape_begin);
+  const uint64_t j = (uint64_t)offs[slot_a] + li;
+  if (li == ~0u || j >= m) {
+    atomicOr(&st->bad, 8u);
+    mask[i] = 0;
+    return;
+  }
+  uint64_t mk = 0;
+  uint32_t cnt = 0;
+  uint32_t prev = a;
+  for (uint64_t k = i + 1; k < m; ++k) {
+    const uint64_t t = k - b0;
+    const bool staged = t < (uint64_t)W;
+    const uint64_t kk = staged ? skey[t] : keys[k];
+    if ((uint32_t)(kk >> 32) != K) break;
+    const uint32_t b = (uint32_t)kk;
+    if (b == prev) continue;
+    prev = b;
+    const uint64_t dd = k - i - 1;
+    if (dd >= 63) mk |= MASK_TAIL;
+    const BpRec<R> rb = staged ? srec[t] : brec[b];
+    if (!pair_test(ra, ga, rb, inv, K, kb, gx, gy)) continue;
+    ++cnt;
+    if (dd < 63) mk |= 1ull << dd;
+  }
+  mask[i] = mk;
+  jpos[i] = (uint32_t)j;
+  pcount[j] = cnt;
+}
+
+// Emit pass: each entry writes its partners' shape ids (ascending) into its
+// range of the grouped pair list, from the hit mask; runs longer than the
+// mask repeat the tests for the remainder.
+template <class R>
+__global__ __launch_bounds__(256) void k_pairs_emit(uint64_t mcap, const uint64_t* __restrict__ keys,
+                                                    const BpRec<R>* __restrict__ brec,
+                                                    const uint32_t* __restrict__ bscene,
+                                                    const SceneDesc<R>* __restrict__ desc,
+                                                    const uint64_t* __restrict__ mask,
+                                                    const uint32_t* __restrict__ jpos,
+                                                    const uint32_t* __restrict__ poffs, uint64_t pcap,
+                                                    uint32_t* __restrict__ second, DevStatus* st) {
+  const uint64_t m = cap_count(mcap, &st->M);
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t mk = mask[i];
+  if (!mk) return;
+  uint64_t o = poffs[jpos[i]];
+  uint64_t bits = mk & ~MASK_TAIL;
+  while (bits) {
+    const int dd = __ffsll((unsigned long long)bits) - 1;
+    bits &= bits - 1;
+    if (o < pcap) second[o] = (uint32_t)keys[i + 1 + dd];
+    else atomicOr(&st->overflow, 2u);
+    ++o;
+  }
+  if (mk & MASK_TAIL) {
+    const uint64_t ki = keys[i];
     const uint32_t K = (uint32_t)(ki >> 32), a = (uint32_t)ki;
-    const BpRec<R> ra = srec[threadIdx.x];
-    const SceneDesc<R>& d = desc[ra.scene];
-    const int gx = d.gx, gy = d.gy;
-    const uint32_t kb = d.key_base;
-    uint64_t o = EMIT ? poffs[i] : 0;
-    uint32_t prev = a;
-    for (uint64_t k = i + 1; k < m; ++k) {
-      const uint64_t t = k - b0;
-      const bool staged = t < (uint64_t)W;
-      const uint64_t kk = staged ? skey[t] : keys[k];
+    const BpRec<R> ra = brec[a];
+    const SceneDesc<R>& d = desc[bscene[a]];
+    const R inv[3] = {d.inv[0], d.inv[1], d.inv[2]};
+    const int ga[3] = {cell_of(ra.lo[0], inv[0]), cell_of(ra.lo[1], inv[1]), cell_of(ra.lo[2], inv[2])};
+    uint32_t prev = (uint32_t)keys[i + 63];
+    for (uint64_t k = i + 64; k < m; ++k) {
+      const uint64_t kk = keys[k];
       if ((uint32_t)(kk >> 32) != K) break;
-      uint32_t b = (uint32_t)kk;
+      const uint32_t b = (uint32_t)kk;
       if (b == prev) continue;
       prev = b;
-      const BpRec<R> rb = staged ? srec[t] : brec[b];
-      bool hit = (ra.lo[0] <= rb.hi[0] && rb.lo[0] <= ra.hi[0]) && (ra.lo[1] <= rb.hi[1] && rb.lo[1] <= ra.hi[1]) &&
-                 (ra.lo[2] <= rb.hi[2] && rb.lo[2] <= ra.hi[2]);
-      if (!hit) continue;
-      uint32_t ox = (uint32_t)max(ra.g[0], rb.g[0]);
-      uint32_t oy = (uint32_t)max(ra.g[1], rb.g[1]);
-      uint32_t oz = (uint32_t)max(ra.g[2], rb.g[2]);
-      if (kb + hash_index(ox, oy, oz, gx, gy) != K) continue;
-      if (EMIT) {
-        if (o < pcap) pairs[o] = ((uint64_t)a << 32) | (uint64_t)b;
-        else atomicOr(&st->overflow, 2u);
-        ++o;
-      } else {
-        ++cnt;
-      }
+      if (!pair_test(ra, ga, brec[b], inv, K, d.key_base, d.gx, d.gy)) continue;
+      if (o < pcap) second[o] = b;
+      else atomicOr(&st->overflow, 2u);
+      ++o;
     }
   }
-  if (!EMIT) pcount[i] = cnt;
+}
+
+// In-register bitonic sort of N values (compile-time indices only).
+template <int N> __device__ __forceinline__ void sort_net(uint32_t (&v)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint32_t x = v[i], y = v[l];
+          const bool up = (i & k) == 0;
+          v[i] = up ? min(x, y) : max(x, y);
+          v[l] = up ? max(x, y) : min(x, y);
+        }
+      }
+}
+
+template <int N>
+__device__ __forceinline__ void sort_group(const uint32_t* __restrict__ second, uint64_t beg, uint32_t len, uint32_t a,
+                                           uint64_t* __restrict__ pairs) {
+  uint32_t v[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) v[t] = (uint32_t)t < len ? second[beg + t] : ~0u;
+  sort_net<N>(v);
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+    if ((uint32_t)t < len) pairs[beg + t] = ((uint64_t)a << 32) | v[t];
+}
+
+// Pair range of the first shape in chunk slot `slot`.
+__device__ __forceinline__ void group_range(uint64_t slot, const uint32_t* __restrict__ counts,
+                                            const uint32_t* __restrict__ offs, const uint32_t* __restrict__ poffs,
+                                            uint64_t m, uint64_t P, uint64_t* beg, uint64_t* end) {
+  const uint64_t e0 = offs[slot], e1 = e0 + counts[slot];
+  *beg = e0 < m ? min((uint64_t)poffs[e0], P) : P;
+  *end = e1 < m ? min((uint64_t)poffs[e1], P) : P;
+}
+
+// Sorts each first shape's group of second shapes and writes the final
+// pairs (a << 32 | b): ascending per scene, scenes in step order.  One
+// thread per shape slot; groups longer than 32 are listed for
+// k_pair_groups_long.
+constexpr int GROUP_LONG = 32;
+constexpr int GROUP_LDS = 4096;
+
+__global__ __launch_bounds__(CHUNK) void k_pair_groups(const Chunk* __restrict__ chunks,
+                                                       const DevStatus* __restrict__ st_in,
+                                                       const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ offs,
+                                                       const uint32_t* __restrict__ poffs, uint64_t mcap, uint64_t pcap,
+                                                       const uint32_t* __restrict__ second,
+                                                       uint64_t* __restrict__ pairs, uint32_t* __restrict__ longs,
+                                                       uint64_t long_cap, DevStatus* st) {
+  if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
+  const Chunk ch = chunks[blockIdx.x];
+  if (threadIdx.x >= ch.len) return;
+  const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
+  uint64_t beg, end;
+  group_range(slot, counts, offs, poffs, cap_count(mcap, &st_in->M), cap_count(pcap, &st_in->P), &beg, &end);
+  if (end <= beg) return;
+  const uint32_t len = (uint32_t)(end - beg);
+  const uint32_t a = ch.begin + threadIdx.x;
+  if (len == 1) {
+    pairs[beg] = ((uint64_t)a << 32) | second[beg];
+  } else if (len <= 4) {
+    sort_group<4>(second, beg, len, a, pairs);
+  } else if (len <= 8) {
+    sort_group<8>(second, beg, len, a, pairs);
+  } else if (len <= 16) {
+    sort_group<16>(second, beg, len, a, pairs);
+  } else if (len <= GROUP_LONG) {
+    sort_group<GROUP_LONG>(second, beg, len, a, pairs);
+  } else {
+    const uint32_t g = atomicAdd(&st->nbig, 1u);  // rare: shapes with more than 32 later partners
+    if (g < long_cap) longs[g] = (uint32_t)slot;
+    else atomicOr(&st->overflow, 4u);
+  }
+}
+
+// Long groups: one workgroup per group, bitonic sort in LDS (groups beyond
+// GROUP_LDS, i.e. one shape overlapping thousands of others, take a serial
+// insertion sort).
+__global__ __launch_bounds__(256) void k_pair_groups_long(const Chunk* __restrict__ chunks,
+                                                          const DevStatus* __restrict__ st,
+                                                          const uint32_t* __restrict__ counts,
+                                                          const uint32_t* __restrict__ offs,
+                                                          const uint32_t* __restrict__ poffs, uint64_t mcap,
+                                                          uint64_t pcap, uint32_t* __restrict__ second,
+                                                          uint64_t* __restrict__ pairs,
+                                                          const uint32_t* __restrict__ longs, uint64_t long_cap) {
+  __shared__ uint32_t v[GROUP_LDS];
+  const uint64_t nlong = min((uint64_t)st->nbig, long_cap);
+  const uint64_t m = cap_count(mcap, &st->M), P = cap_count(pcap, &st->P);
+  for (uint64_t g = blockIdx.x; g < nlong; g += gridDim.x) {
+    const uint32_t slot = longs[g];
+    uint64_t beg, end;
+    group_range(slot, counts, offs, poffs, m, P, &beg, &end);
+    const uint32_t len = (uint32_t)(end - beg);
+    const uint32_t a = chunks[slot / CHUNK].begin + slot % CHUNK;
+    if (len <= GROUP_LDS) {
+      uint32_t n2 = 1;
+      while (n2 < len) n2 <<= 1;
+      for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) v[t] = t < len ? second[beg + t] : ~0u;
+      __syncthreads();
+      for (uint32_t k = 2; k <= n2; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+            const uint32_t l = i ^ j;
+            if (l > i) {
+              const uint32_t x = v[i], y = v[l];
+              const bool up = (i & k) == 0;
+              if ((x > y) == up) { v[i] = y; v[l] = x; }
+            }
+          }
+          __syncthreads();
+        }
+      for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) pairs[beg + t] = ((uint64_t)a << 32) | v[t];
+      __syncthreads();
+    } else if (threadIdx.x == 0) {
+      for (uint32_t t = 1; t < len; ++t) {
+        const uint32_t x = second[beg + t];
+        uint32_t u = t;
+        while (u > 0 && second[beg + u - 1] > x) {
+          second[beg + u] = second[beg + u - 1];
+          --u;
+        }
+        second[beg + u] = x;
+      }
+      for (uint32_t t = 0; t < len; ++t) pairs[beg + t] = ((uint64_t)a << 32) | second[beg + t];
+    }
+  }
 }
 
 // Per stepped scene: its ranges in the step's sorted entries, pair list and
@@ -383,28 +626,22 @@ struct SceneOut {
 
 template <class R>
 __global__ void k_scene_ranges(const uint32_t* __restrict__ sel, const SceneDesc<R>* __restrict__ desc,
-                               const DevStatus* __restrict__ st, uint64_t pcap, const uint32_t* __restrict__ offs,
-                               const uint64_t* __restrict__ pairs, const uint32_t* __restrict__ hitpos,
-                               SceneOut* __restrict__ out) {
+                               const DevStatus* __restrict__ st, uint64_t mcap, uint64_t pcap,
+                               const uint32_t* __restrict__ offs, const uint32_t* __restrict__ poffs,
+                               const uint32_t* __restrict__ hitpos, SceneOut* __restrict__ out) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= st->nsel) return;
   const SceneDesc<R>& d = desc[sel[j]];
   const uint64_t nslot = st->nslots;
+  const uint64_t M = cap_count(mcap, &st->M);
   const uint64_t P = cap_count(pcap, &st->P);
-  auto entry_at = [&](uint64_t slot) -> uint64_t { return slot < nslot ? offs[slot] : st->M; };
-  auto lower = [&](uint64_t key) -> uint64_t {
-    uint64_t lo = 0, hi = P;
-    while (lo < hi) {
-      uint64_t mid = (lo + hi) >> 1;
-      if (pairs[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-  };
+  auto entry_at = [&](uint64_t slot) -> uint64_t { return slot < nslot ? offs[slot] : M; };
+  auto pair_at = [&](uint64_t e) -> uint64_t { return e < M ? min((uint64_t)poffs[e], P) : P; };
   SceneOut o;
   o.m_begin = entry_at((uint64_t)d.chunk_begin * CHUNK);
   o.m_end = entry_at((uint64_t)d.chunk_end * CHUNK);
-  o.pair_begin = lower((uint64_t)d.shape_begin << 32);
-  o.pair_end = lower((uint64_t)d.shape_end << 32);
+  o.pair_begin = pair_at(o.m_begin);
+  o.pair_end = pair_at(o.m_end);
   o.contact_begin = o.pair_begin < P ? hitpos[o.pair_begin] : st->C;
   o.contact_end = o.pair_end < P ? hitpos[o.pair_end] : st->C;
   out[j] = o;

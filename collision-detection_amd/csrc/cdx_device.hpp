@@ -23,6 +23,8 @@ struct DevStatus {
   uint32_t overflow; // 1 bin entries, 2 pairs: capacity exceeded, results incomplete
   uint32_t key_max;
   uint32_t nsel;     // scenes in this step
+  uint32_t nbig;     // pair groups too long for the in-register sort
+  uint32_t pad;
 };
 
 // What the host sets before a step (copied into DevStatus by k_begin_step,
@@ -78,12 +80,11 @@ template <class R> struct alignas(16) ShapeRec {
   uint32_t type, body, active, pad;
 };
 
-// Per-shape broadphase record: shifted AABB and its minimum cell, gathered
-// as one aligned block by the pair search.
-template <class R> struct alignas(16) BpRec {
+// Per-shape broadphase record: the shifted AABB, gathered as one aligned
+// block by the pair search (cells are recomputed from it: one multiply and
+// truncation per axis, bit-identical to k_count's).
+template <class R> struct alignas(sizeof(R) == 4 ? 8 : 16) BpRec {
   R lo[3], hi[3];
-  int g[3];
-  uint32_t scene;
 };
 
 inline unsigned blocks_for(uint64_t n, unsigned bs) {

@@ -48,7 +48,7 @@ __global__ void k_stage_begin(const StepHdr* __restrict__ hdr, DevStatus* st, in
   st->nslots = hdr->nslots;
   st->nsel = hdr->nsel;
   if (what & ZERO_AABB) st->bad = 0;
-  if (what & ZERO_BP) { st->M = st->A = st->P = 0; st->overflow = 0; st->key_max = 0; }
+  if (what & ZERO_BP) { st->M = st->A = st->P = 0; st->overflow = 0; st->key_max = 0; st->nbig = 0; }
   if (what & ZERO_NP) { st->Q = st->Q2 = st->C = 0; }
 }
 
@@ -377,7 +377,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_,
-                           &scan_tmp_, &radix_tmp_, &chunks_, &sel_, &desc_, &souts_};
+                           &scan_tmp_, &radix_tmp_, &chunks_, &sel_, &desc_, &souts_, &longs_, &bscene_, &mask_, &jpos_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -594,7 +594,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   uint64_t resident_bytes_estimate(uint64_t n, uint64_t nb) const override {
     // inputs in SoA plus the per-shape records each step rewrites
-    const uint64_t per_shape = 1 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
+    const uint64_t per_shape = 1 + 4 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
                                sizeof(ShapeHdr) + sizeof(ShapeRecF);
     const uint64_t per_body = 7 * sizeof(R) + 1;
     return n * per_shape + nb * per_body;
@@ -603,7 +603,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t step_bytes_estimate(uint64_t n) const override {
     // keys + ping-pong and per-entry pair counts/offsets; pairs, narrowphase
     // records, queues and contact outputs per pair; slot counts/offsets
-    const double per_m = 2 * 8 + 2 * 4;
+    const double per_m = 2 * 8 + 2 * 4 + 8 + 4;
     const double per_p = 2 * 8 + 10 * sizeof(R) + 8 + 4 + 4 + 2 * 8 + 4 + 1 + 10 * sizeof(R) + 8 + 8;
     return (uint64_t)((double)n * (cap_m_per_shape_ * per_m + cap_p_per_shape_ * per_p + 8.0)) + (1u << 20);
   }
@@ -656,6 +656,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     aabb_current_ = false;
     rec_.reserve(nc * sizeof(ShapeRec<R>));
     brec_.reserve(nc * sizeof(BpRec<R>));
+    bscene_.reserve(nc * 4);
     hdr_.reserve(nc * sizeof(ShapeHdr));
     frec_.reserve(nc * sizeof(ShapeRecF));
     NSC_ = nc;
@@ -848,7 +849,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(), brec_.as<BpRec<R>>(),
-                       kpart_.as<uint32_t>(), st());
+                       bscene_.as<uint32_t>(), kpart_.as<uint32_t>(), st());
     hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
@@ -865,11 +866,16 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
     keys_.reserve(mcap_ * 8);
     keys_tmp_.reserve(mcap_ * 8);
+    pcount_.reserve((mcap_ + 1) * 4);
+    poffs_.reserve((mcap_ + 1) * 4);
+    mask_.reserve(mcap_ * 8);
+    jpos_.reserve(mcap_ * 4);
     mark(ST_BP_EMIT, true);
     hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const uint32_t*)offs_.as<uint32_t>(), mcap_, keys_.as<uint64_t>(), st());
+                       (const uint32_t*)offs_.as<uint32_t>(), mcap_, keys_.as<uint64_t>(), pcount_.as<uint32_t>(),
+                       st());
     mark(ST_BP_EMIT, false);
     std::vector<int> sh;
     for (int b = 0; b < key_bits; b += 8) sh.push_back(32 + b);
@@ -883,13 +889,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        (const uint64_t*)keys_.as<uint64_t>(), st());
     mark(ST_BP_RUNS, false);
 
-    pcount_.reserve((mcap_ + 1) * 4);
-    poffs_.reserve((mcap_ + 1) * 4);
     mark(ST_BP_PAIRS, true);
-    hipLaunchKernelGGL((k_pairs<R, false>), dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
+    hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), pcount_.as<uint32_t>(), (const uint32_t*)nullptr,
-                       (uint64_t)0, (uint64_t*)nullptr, st());
+                       (const uint32_t*)bscene_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
+                       (const uint32_t*)offs_.as<uint32_t>(), pcount_.as<uint32_t>(), mask_.as<uint64_t>(),
+                       jpos_.as<uint32_t>(), st());
     mark(ST_BP_PAIRS, false);
     mark(ST_BP_PAIRSCAN, true);
     scan_excl_u32(pcount_.as<uint32_t>(), poffs_.as<uint32_t>(), mcap_, &st()->P, radix_tmp_.as<uint32_t>(), s_,
@@ -905,18 +910,25 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     pairs_.reserve(pcap_ * 8);
     pairs_tmp_.reserve(pcap_ * 8);
     mark(ST_BP_PAIREMIT, true);
-    hipLaunchKernelGGL((k_pairs<R, true>), dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
+    hipLaunchKernelGGL(k_pairs_emit<R>, dim3(blocks_for(mcap_, 256)), dim3(256), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (uint32_t*)nullptr,
-                       (const uint32_t*)poffs_.as<uint32_t>(), pcap_, pairs_.as<uint64_t>(), st());
+                       (const uint32_t*)bscene_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
+                       (const uint64_t*)mask_.as<uint64_t>(), (const uint32_t*)jpos_.as<uint32_t>(),
+                       (const uint32_t*)poffs_.as<uint32_t>(), pcap_, pairs_tmp_.as<uint32_t>(), st());
     mark(ST_BP_PAIREMIT, false);
-    std::vector<int> ps;
-    for (int b = 0; b < shape_bits_; b += 8) ps.push_back(b);
-    for (int b = 0; b < shape_bits_; b += 8) ps.push_back(32 + b);
-    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(pcap_), 16) * 4);
+    const uint64_t long_cap = pcap_ / (GROUP_LONG + 1) + 1;
+    longs_.reserve(long_cap * 4);
     mark(ST_BP_PAIRSORT, true);
-    radix_sort_u64(pairs_.as<uint64_t>(), pairs_tmp_.as<uint64_t>(), pcap_, ps.data(), (int)ps.size(),
-                   radix_tmp_.as<uint32_t>(), s_, &st()->P);
+    hipLaunchKernelGGL(k_pair_groups, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
+                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
+                       (const uint32_t*)counts_buf_.as<uint32_t>(), (const uint32_t*)offs_.as<uint32_t>(),
+                       (const uint32_t*)poffs_.as<uint32_t>(), mcap_, pcap_, (const uint32_t*)pairs_tmp_.as<uint32_t>(),
+                       pairs_.as<uint64_t>(), longs_.as<uint32_t>(), long_cap, st());
+    hipLaunchKernelGGL(k_pair_groups_long, dim3(256), dim3(256), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
+                       (const DevStatus*)st(), (const uint32_t*)counts_buf_.as<uint32_t>(),
+                       (const uint32_t*)offs_.as<uint32_t>(), (const uint32_t*)poffs_.as<uint32_t>(), mcap_, pcap_,
+                       pairs_tmp_.as<uint32_t>(), pairs_.as<uint64_t>(), (const uint32_t*)longs_.as<uint32_t>(),
+                       long_cap);
     mark(ST_BP_PAIRSORT, false);
     if (exact_) {
       counts_.n = scenes_[0].n_shapes; counts_.m = M_; counts_.a = A_; counts_.p = P_;
@@ -1018,8 +1030,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void stage_scene_ranges() {
     hipLaunchKernelGGL(k_scene_ranges<R>, dim3(blocks_for(sel_grid(), 64)), dim3(64), 0, s_,
                        (const uint32_t*)sel_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
-                       (const DevStatus*)st(), pcap_, (const uint32_t*)offs_.as<uint32_t>(),
-                       (const uint64_t*)pairs_.as<uint64_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
+                       (const DevStatus*)st(), mcap_, pcap_, (const uint32_t*)offs_.as<uint32_t>(),
+                       (const uint32_t*)poffs_.as<uint32_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
                        souts_.as<SceneOut>());
   }
 
@@ -1088,8 +1100,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                                         scan_tmp_words(mcap_) + 16) * 4);
     pcount_.reserve((mcap_ + 1) * 4);
     poffs_.reserve((mcap_ + 1) * 4);
+    mask_.reserve(mcap_ * 8);
+    jpos_.reserve(mcap_ * 4);
     pairs_.reserve(P * 8);
     pairs_tmp_.reserve(P * 8);
+    longs_.reserve((P / (GROUP_LONG + 1) + 1) * 4);
     np_v_.reserve(P * 10 * sizeof(R));
     np_ids_.reserve(P * 8);
     np_hit_.reserve(P * 4);
@@ -1130,6 +1145,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     CDX_REQUIRE(!(bad & 2u), ERR_INVALID, "shape body id out of range");
     CDX_REQUIRE(!(bad & 4u), ERR_INVALID,
                 "an AABB maps outside the grid: non-finite coordinates or a zero scene extent");
+    CDX_REQUIRE(!(bad & 8u), ERR_INVALID, "internal error: a bin entry lies outside its shape's cell range");
   }
   void reset_stage_marks() {
     for (int s = 0; s < ST_COUNT; ++s) used_[s] = false;
@@ -1181,7 +1197,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   int mode_ = NP_NATIVE;
   float tau_ = 1e-2f;
 
-  DevBuf status_, stephdr_, stage_, upd_, tmp_out_;
+  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, bscene_, mask_, jpos_;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
   DevBuf chunks_, sel_, desc_, souts_;
