@@ -252,34 +252,64 @@ __global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restric
 // stable sort orders by bin and keeps shapes ascending inside a bin.  Keys
 // of a scene are offset by its key base, so the bins of all stepped scenes
 // sort into one array, scene after scene.
+constexpr int EMIT_LDS = 4096;  // entries of one chunk staged for coalesced stores
+
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                      const SceneDesc<R>* __restrict__ desc,
                                                      const BpRec<R>* __restrict__ brec, const uint32_t* __restrict__ offs,
-                                                     uint64_t mcap, uint64_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ pcount, DevStatus* st) {
+                                                     const uint32_t* __restrict__ counts, uint64_t mcap,
+                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ pcount,
+                                                     DevStatus* st) {
+  __shared__ uint64_t sk[EMIT_LDS];
+  __shared__ uint64_t s_total;
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
-  if (threadIdx.x >= ch.len) return;
-  const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  const SceneDesc<R>& d = desc[ch.scene];
-  const BpRec<R> r = brec[i];
-  Cells c;
-  c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
-  c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
-  if (!cells_ok(c, d.gx, d.gy, d.gz)) return;
-  uint64_t o = offs[(uint64_t)blockIdx.x * CHUNK + threadIdx.x];
-  for (uint32_t x = (uint32_t)c.x0; x <= (uint32_t)c.x1; ++x)
-    for (uint32_t y = (uint32_t)c.y0; y <= (uint32_t)c.y1; ++y)
-      for (uint32_t z = (uint32_t)c.z0; z <= (uint32_t)c.z1; ++z) {
-        if (o < mcap) {
-          keys[o] = ((uint64_t)(d.key_base + hash_index(x, y, z, d.gx, d.gy)) << 32) | i;
-          pcount[o] = 0;  // pair count of this entry, set by the pair search
-        } else {
-          atomicOr(&st->overflow, 1u);
-        }
-        ++o;
-      }
+  const uint64_t slot0 = (uint64_t)blockIdx.x * CHUNK;
+  const uint64_t e0 = offs[slot0];
+  const bool mine = threadIdx.x < ch.len;
+  const uint64_t o0 = mine ? offs[slot0 + threadIdx.x] : 0;
+  if (threadIdx.x == ch.len - 1) s_total = o0 + counts[slot0 + threadIdx.x] - e0;
+  __syncthreads();
+  const uint64_t total = s_total;
+  const bool staged = total <= EMIT_LDS;
+  if (mine) {
+    const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
+    const SceneDesc<R>& d = desc[ch.scene];
+    const BpRec<R> r = brec[i];
+    Cells c;
+    c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
+    c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
+    if (cells_ok(c, d.gx, d.gy, d.gz)) {
+      uint64_t o = o0;
+      for (uint32_t x = (uint32_t)c.x0; x <= (uint32_t)c.x1; ++x)
+        for (uint32_t y = (uint32_t)c.y0; y <= (uint32_t)c.y1; ++y)
+          for (uint32_t z = (uint32_t)c.z0; z <= (uint32_t)c.z1; ++z) {
+            const uint64_t v = ((uint64_t)(d.key_base + hash_index(x, y, z, d.gx, d.gy)) << 32) | i;
+            if (staged) {
+              sk[o - e0] = v;
+            } else if (o < mcap) {
+              keys[o] = v;
+              pcount[o] = 0;  // pair count of this entry, set by the pair search
+            } else {
+              atomicOr(&st->overflow, 1u);
+            }
+            ++o;
+          }
+    }
+  }
+  if (!staged) return;
+  __syncthreads();
+  // coalesced stores of the chunk's entries (and zeroed pair counts)
+  for (uint64_t t = threadIdx.x; t < total; t += CHUNK) {
+    const uint64_t o = e0 + t;
+    if (o < mcap) {
+      keys[o] = sk[t];
+      pcount[o] = 0;
+    } else {
+      atomicOr(&st->overflow, 1u);
+    }
+  }
 }
 
 // Active bins (reduce_by_key, Broadphase.cu:439-443): the number of runs of

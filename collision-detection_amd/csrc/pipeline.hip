@@ -132,6 +132,12 @@ struct DevBuf {
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// Exchanges two buffers (a sort that ends in its ping-pong buffer).
+inline void swap_bufs(DevBuf& a, DevBuf& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.bytes, b.bytes);
+}
+
 struct HostScene {
   uint64_t shape_begin, n_shapes, body_begin, n_bodies;
   int g[3];
@@ -874,15 +880,16 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const uint32_t*)offs_.as<uint32_t>(), mcap_, keys_.as<uint64_t>(), pcount_.as<uint32_t>(),
-                       st());
+                       (const uint32_t*)offs_.as<uint32_t>(), (const uint32_t*)counts_buf_.as<uint32_t>(), mcap_,
+                       keys_.as<uint64_t>(), pcount_.as<uint32_t>(), st());
     mark(ST_BP_EMIT, false);
     std::vector<int> sh;
     for (int b = 0; b < key_bits; b += 8) sh.push_back(32 + b);
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(std::max(mcap_, slots)), scan_tmp_words(mcap_) + 16) * 4);
     mark(ST_BP_SORT, true);
-    radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, sh.data(), (int)sh.size(),
-                   radix_tmp_.as<uint32_t>(), s_, &st()->M);
+    if (radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, sh.data(), (int)sh.size(),
+                       radix_tmp_.as<uint32_t>(), s_, &st()->M))
+      swap_bufs(keys_, keys_tmp_);
     mark(ST_BP_SORT, false);
     mark(ST_BP_RUNS, true);
     hipLaunchKernelGGL(k_count_runs, dim3(std::min<unsigned>(blocks_for(mcap_, 256), 1024u)), dim3(256), 0, s_, mcap_,
@@ -970,8 +977,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       q = std::max<uint64_t>(hstat_->Q, 1);
     }
     const int cls_shift[1] = {32};
-    radix_sort_u64(queue_.as<uint64_t>(), queue_tmp_.as<uint64_t>(), q, cls_shift, 1, radix_tmp_.as<uint32_t>(), s_,
-                   &st()->Q);
+    if (radix_sort_u64(queue_.as<uint64_t>(), queue_tmp_.as<uint64_t>(), q, cls_shift, 1, radix_tmp_.as<uint32_t>(), s_,
+                       &st()->Q))
+      swap_bufs(queue_, queue_tmp_);
     mark(ST_NP_MAIN, false);
     mark(ST_NP_RESOLVE, true);
     const uint64_t* rq = queue_.as<uint64_t>();

@@ -117,10 +117,25 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
 }
 
 constexpr int RS_BS = 256;
-constexpr int RS_IPT = 8;
+constexpr int RS_IPT = 16;
 constexpr int RS_TILE = RS_BS * RS_IPT;
 constexpr int RS_NW = RS_BS / WAVE;
+constexpr int RS_WTILE = RS_IPT * WAVE;  // elements per wavefront: a contiguous run of the tile
 
+// Lanes of the wavefront holding the same 8-bit digit d (8 ballots).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t bb = __ballot((d >> b) & 1u);
+    peers &= ((d >> b) & 1u) ? bb : ~bb;
+  }
+  return peers;
+}
+
+// Per-tile digit histogram.  Each wavefront owns a contiguous run of the
+// tile; equal digits of a wavefront are counted once by their lowest lane,
+// so LDS atomics do not pile up on runs of equal digits.
 __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restrict__ in, uint64_t cap,
                                                          const uint64_t* __restrict__ d_n, int shift,
                                                          uint32_t* __restrict__ hist, uint32_t nb) {
@@ -128,64 +143,76 @@ __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restr
   const uint64_t n = eff_n(cap, d_n);
   h[threadIdx.x] = 0;
   __syncthreads();
-  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
-#pragma unroll
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * RS_WTILE + lane;
+  const uint64_t lt = lanemask_lt();
+#pragma unroll 4
   for (int i = 0; i < RS_IPT; ++i) {
-    uint64_t e = base + (uint64_t)i * RS_BS + threadIdx.x;
-    if (e < n) atomicAdd(&h[(uint32_t)(in[e] >> shift) & 255u], 1u);
+    const uint64_t e = base + (uint64_t)i * WAVE;
+    const bool valid = e < n;
+    const uint32_t d = valid ? (uint32_t)(in[e] >> shift) & 255u : 0u;
+    const uint64_t peers = digit_peers(d, valid);
+    if (valid && (peers & lt) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
   }
   __syncthreads();
   hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter.  Element order inside a tile is (item i, then thread), and
-// ranks among equal digits follow that order: per item, lanes of one
-// wavefront find their equal-digit peers with 8 ballots, and per-wavefront
-// digit counts are prefix-summed across wavefronts in LDS.
+// Stable scatter.  Tile order is (wavefront, item, lane), each wavefront a
+// contiguous run of the input.  Pass 1 ranks every element among equal
+// digits of its wavefront (ballots + a wavefront-private LDS counter per
+// digit, no barriers); one barrier, a per-digit prefix over wavefronts and
+// the tile's global digit offset; pass 2 writes.
 __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                            uint64_t cap, const uint64_t* __restrict__ d_n, int shift,
                                                            const uint32_t* __restrict__ offs, uint32_t nb) {
-  __shared__ uint32_t base[256];
-  const uint64_t n = eff_n(cap, d_n);
   __shared__ uint32_t wcnt[RS_NW][256];
-  const int w = threadIdx.x / WAVE;
-  base[threadIdx.x] = offs[(uint64_t)threadIdx.x * nb + blockIdx.x];
-  uint64_t tile = (uint64_t)blockIdx.x * RS_TILE;
+  const uint64_t n = eff_n(cap, d_n);
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+#pragma unroll
+  for (int ww = 0; ww < RS_NW; ++ww) wcnt[ww][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * RS_WTILE + lane;
   const uint64_t lt = lanemask_lt();
+  uint64_t v[RS_IPT];
+  uint32_t rk[RS_IPT];
+#pragma unroll
   for (int i = 0; i < RS_IPT; ++i) {
-    uint64_t e = tile + (uint64_t)i * RS_BS + threadIdx.x;
-    bool valid = e < n;
-    uint64_t v = valid ? in[e] : 0ull;
-    uint32_t d = (uint32_t)(v >> shift) & 255u;
-    uint64_t peers = __ballot(valid);
+    const uint64_t e = base + (uint64_t)i * WAVE;
+    const bool valid = e < n;
+    v[i] = valid ? in[e] : 0ull;
+  }
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      uint64_t bb = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? bb : ~bb;
+  for (int i = 0; i < RS_IPT; ++i) {
+    const uint64_t e = base + (uint64_t)i * WAVE;
+    const bool valid = e < n;
+    const uint32_t d = (uint32_t)(v[i] >> shift) & 255u;
+    const uint64_t peers = digit_peers(d, valid);
+    const uint32_t before = wcnt[w][d];  // same value in every lane of the digit (no lane wrote it yet)
+    rk[i] = before + (uint32_t)__popcll(peers & lt);
+    // lanes of one wavefront execute in lockstep: every peer has read the
+    // counter before the leader updates it
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lt) == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  {
+    uint32_t run = offs[(uint64_t)threadIdx.x * nb + blockIdx.x];
+#pragma unroll
+    for (int ww = 0; ww < RS_NW; ++ww) {
+      const uint32_t c = wcnt[ww][threadIdx.x];
+      wcnt[ww][threadIdx.x] = run;
+      run += c;
     }
-    uint32_t rank = (uint32_t)__popcll(peers & lt);
-    uint32_t cnt = (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
 #pragma unroll
-    for (int ww = 0; ww < RS_NW; ++ww) wcnt[ww][threadIdx.x] = 0;
-    __syncthreads();
-    if (valid && rank == 0) wcnt[w][d] = cnt;
-    __syncthreads();
-    {
-      uint32_t run = base[threadIdx.x];
-#pragma unroll
-      for (int ww = 0; ww < RS_NW; ++ww) {
-        uint32_t c = wcnt[ww][threadIdx.x];
-        wcnt[ww][threadIdx.x] = run;
-        run += c;
-      }
-      base[threadIdx.x] = run;
-    }
-    __syncthreads();
-    if (valid) out[wcnt[w][d] + rank] = v;
-    __syncthreads();
+  for (int i = 0; i < RS_IPT; ++i) {
+    const uint64_t e = base + (uint64_t)i * WAVE;
+    if (e < n) out[wcnt[w][(uint32_t)(v[i] >> shift) & 255u] + rk[i]] = v[i];
   }
 }
-
 
 }  // namespace
 
@@ -213,9 +240,9 @@ size_t radix_tmp_words(uint64_t n) {
   return h + scan_tmp_words(h) + 2;
 }
 
-void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
+bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
                     hipStream_t s, const uint64_t* d_n) {
-  if (n <= 1 || nshift == 0) return;
+  if (n <= 1 || nshift == 0) return false;
   uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
   uint64_t h = 256ull * nb;
   uint32_t* hist = hist_tmp;
@@ -230,10 +257,7 @@ void radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts
                        (const uint32_t*)hist, nb);
     uint64_t* t = src; src = dst; dst = t;
   }
-  if (src != data) {
-    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_copy_u64, dim3((uint32_t)blocks), dim3(256), 0, s, (const uint64_t*)src, data, n, d_n);
-  }
+  return src != data;
 }
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
