@@ -52,8 +52,8 @@ def stage_bytes(s, c):
     + body rot 4r in, AABB 6r out.  Bin count: AABB 6r in, count 4 out.  Key
     emit: AABB 6r + offset 4 in, 8 per bin entry out.  Pair count pass: the
     (bin, body) entry 8 and the AABB 6r per entry in, a count 4 per entry
-    out.  Pair emit pass: the entry's hit mask 8 and position 4 in, per pair
-    the partner's entry 8 in and its id 4 out.
+    out.  Pair emit pass: the entry's count 4 in, per pair the partner's
+    entry 8 in and the pair 8 out.
     """
     n, m, p = s.n, c.bin_entries, c.pairs
     r = s.dtype.itemsize
@@ -62,7 +62,7 @@ def stage_bytes(s, c):
         "bp_count": n * 6 * r + n * 4,
         "bp_emit": n * (6 * r + 4) + m * 8,
         "bp_pairs": m * (8 + 6 * r) + m * 4,
-        "bp_pairemit": m * (8 + 4) + p * (8 + 4),
+        "bp_pairemit": m * 4 + p * (8 + 8),
     }
 
 
@@ -482,8 +482,9 @@ def main():
     for _ in range(args.warmup):
         sc.step()
         sc.sync()
-    c = sc.counts()
-    log(f"pairs={c.pairs} contacts={c.contacts} bin_entries={c.bin_entries} resolved={c.resolved}")
+    c = sc.counts() if args.warmup > 0 else None
+    if c is not None:
+        log(f"pairs={c.pairs} contacts={c.contacts} bin_entries={c.bin_entries} resolved={c.resolved}")
     lat = []
     stage_tot = {k: 0.0 for k in pkg.STAGES}
     t_start = time.perf_counter()
@@ -494,6 +495,8 @@ def main():
         lat.append(time.perf_counter() - t0)
         for name, v in sc.stage_ms().items():
             stage_tot[name] += v
+        if c is None:
+            c = sc.counts()
     total = time.perf_counter() - t_start
     c2 = sc.counts()
     assert c2.pairs == c.pairs and c2.contacts == c.contacts, "non-deterministic step"

@@ -363,22 +363,29 @@ __device__ __forceinline__ uint32_t local_cell_index(uint32_t kk, const Cells& c
 // grid size, Broadphase.cu:140-147) are skipped, so each pair is emitted
 // once and the reference's sort + unique (:522-535) reduces to ordering.
 //
-// One thread per sorted entry i pairs shape(i) with the later entries of
-// the same run (shapes ascend inside a run, so shape(i) < shape(k)).  The
-// workgroup's entries and the next PAIR_AHEAD are staged in LDS with their
-// records; only runs reaching past the window read global memory.
-//
-// Output order: the count pass stores each entry's pair count at the
-// entry's emission position (offs of its shape + its cell's index among the
-// shape's cells), so the scan of those counts groups the pairs by first
-// shape, shapes in step order.  It also leaves a hit mask per entry (bit d:
-// the entry d+1 places later is a partner; bit 63: the run reaches past 63
-// later entries) and the emission position, so the emit pass writes each
-// entry's partners without repeating the tests.  k_pair_groups then sorts
-// each shape's group: no global sort of the pair list is needed.
+// Count pass (k_pairs): each sorted entry i pairs shape(i) with the later
+// entries of its run (shapes ascend inside a run, so shape(i) < shape(k)).
+// A workgroup stages its 256 entries and the next PAIR_AHEAD with their
+// records in LDS and splits the (entry, later entry) candidates of its own
+// entries evenly over its lanes, so short and long runs cost the same per
+// lane.  Per entry it stores the pair count and a hit mask (bit d: the
+// entry d+1 places later is a partner; bit 63: partners beyond 62 places)
+// at the entry's emission position j (offs of its shape + its cell's index
+// among the shape's cells), so pairs can be gathered per first shape.
+// k_shape_pairs sums each shape's counts, a scan gives each shape's pair
+// range, and k_pair_emit writes each shape's partners from the masks and
+// sorts them in registers: the pair list comes out ascending per scene
+// without a global sort.
 constexpr int PAIR_BS = 256;
 constexpr int PAIR_AHEAD = 64;
+constexpr int PAIR_W = PAIR_BS + PAIR_AHEAD;
 constexpr uint64_t MASK_TAIL = 1ull << 63;
+
+struct alignas(16) PairMeta {
+  uint64_t mask;
+  uint32_t ipos;  // sorted position of the entry
+  uint32_t pad;
+};
 
 template <class R>
 __device__ __forceinline__ bool pair_test(const BpRec<R>& ra, const int ga[3], const BpRec<R>& rb, const R inv[3],
@@ -392,125 +399,188 @@ __device__ __forceinline__ bool pair_test(const BpRec<R>& ra, const int ga[3], c
   return kb + hash_index(ox, oy, oz, gx, gy) == K;
 }
 
+// Scene parameters and cells of one entry's shape.
+template <class R> struct EntryCtx {
+  BpRec<R> r;
+  R inv[3];
+  int g0[3], g1[3];
+  int gx, gy;
+  uint32_t kb, K, a;
+  uint64_t slot;  // chunk slot of the shape
+};
+
+template <class R>
+__device__ __forceinline__ void entry_ctx(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r,
+                                          const uint32_t* __restrict__ bscene, const SceneDesc<R>* __restrict__ desc) {
+  const SceneDesc<R>& d = desc[bscene[a]];
+  c.r = r;
+  c.K = K;
+  c.a = a;
+  c.gx = d.gx;
+  c.gy = d.gy;
+  c.kb = d.key_base;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    c.inv[k] = d.inv[k];
+    c.g0[k] = cell_of(r.lo[k], c.inv[k]);
+    c.g1[k] = cell_of(r.hi[k], c.inv[k]);
+  }
+  c.slot = (uint64_t)d.chunk_begin * CHUNK + (a - d.shape_begin);
+}
+
 template <class R>
 __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t* __restrict__ keys,
                                                    const BpRec<R>* __restrict__ brec,
                                                    const uint32_t* __restrict__ bscene,
                                                    const SceneDesc<R>* __restrict__ desc,
                                                    const uint32_t* __restrict__ offs, uint32_t* __restrict__ pcount,
-                                                   uint64_t* __restrict__ mask, uint32_t* __restrict__ jpos,
-                                                   DevStatus* st) {
-  constexpr int W = PAIR_BS + PAIR_AHEAD;
-  __shared__ BpRec<R> srec[W];
-  __shared__ uint64_t skey[W];
+                                                   PairMeta* __restrict__ meta, DevStatus* st) {
+  __shared__ BpRec<R> srec[PAIR_W];
+  __shared__ uint32_t skey[PAIR_W], sbody[PAIR_W];
+  __shared__ uint64_t send[PAIR_W / WAVE];
+  __shared__ uint32_t sstart[PAIR_BS + 1];
+  __shared__ uint64_t smask[PAIR_BS];
+  __shared__ uint32_t scnt[PAIR_BS];
+  __shared__ uint32_t scan_scratch[PAIR_BS / WAVE + 1];
   const uint64_t m = cap_count(mcap, &st->M);
   const uint64_t b0 = (uint64_t)blockIdx.x * PAIR_BS;
   if (b0 >= m) return;
-  for (int t = threadIdx.x; t < W; t += PAIR_BS) {
-    uint64_t e = b0 + t;
-    if (e < m) {
-      uint64_t kk = keys[e];
-      skey[t] = kk;
+  const int tid = threadIdx.x, lane = tid % WAVE;
+  const int nloc = (int)min((uint64_t)PAIR_W, m - b0);
+  for (int t = tid; t < PAIR_W; t += PAIR_BS) {
+    if (t < nloc) {
+      const uint64_t kk = keys[b0 + t];
+      skey[t] = (uint32_t)(kk >> 32);
+      sbody[t] = (uint32_t)kk;
       srec[t] = brec[(uint32_t)kk];
     }
   }
   __syncthreads();
-  const uint64_t i = b0 + threadIdx.x;
-  if (i >= m) return;
-  const uint64_t ki = skey[threadIdx.x];
-  const uint64_t kprev = i == 0 ? ~ki : (threadIdx.x > 0 ? skey[threadIdx.x - 1] : keys[i - 1]);
-  if (kprev == ki) {  // repeated entry of one shape in a bin (aliasing)
-    mask[i] = 0;
-    return;
+  // run ends inside the window (the last staged entry always ends one)
+  for (int t = tid; t < PAIR_W; t += PAIR_BS) {
+    const bool f = t < nloc && (t + 1 == nloc || skey[t + 1] != skey[t]);
+    const uint64_t bal = __ballot(f);
+    if (lane == 0) send[t / WAVE] = bal;
   }
-  const uint32_t K = (uint32_t)(ki >> 32), a = (uint32_t)ki;
-  const BpRec<R> ra = srec[threadIdx.x];
-  const SceneDesc<R>& d = desc[bscene[a]];
-  const int gx = d.gx, gy = d.gy;
-  const uint32_t kb = d.key_base;
-  const R inv[3] = {d.inv[0], d.inv[1], d.inv[2]};
-  Cells c;
-  c.x0 = cell_of(ra.lo[0], inv[0]); c.y0 = cell_of(ra.lo[1], inv[1]); c.z0 = cell_of(ra.lo[2], inv[2]);
-  c.x1 = cell_of(ra.hi[0], inv[0]); c.y1 = cell_of(ra.hi[1], inv[1]); c.z1 = cell_of(ra.hi[2], inv[2]);
-  const int ga[3] = {c.x0, c.y0, c.z0};
-  // emission position of this entry
-  const uint32_t li = local_cell_index(K - kb, c, gx, gy);
-  const uint64_t slot_a = (uint64_t)d.chunk_begin * CHUNK + (a - d.shape_begin);
-  const uint64_t j = (uint64_t)offs[slot_a] + li;
-  if (li == ~0u || j >= m) {
-    atomicOr(&st->bad, 8u);
-    mask[i] = 0;
-    return;
+  __syncthreads();
+  // own entry: candidates inside the window
+  const bool valid = tid < nloc;
+  const uint64_t i = b0 + tid;
+  bool dup = false, beyond = false;
+  uint32_t n_i = 0;
+  if (valid) {
+    if (tid > 0) dup = skey[tid - 1] == skey[tid] && sbody[tid - 1] == sbody[tid];
+    else if (i > 0) dup = keys[i - 1] == (((uint64_t)skey[0] << 32) | sbody[0]);
+    int w = tid / WAVE;
+    uint64_t fm = send[w] & (~0ull << (tid % WAVE));
+    while (!fm && ++w < PAIR_W / WAVE) fm = send[w];
+    const int u = fm ? w * WAVE + __ffsll((unsigned long long)fm) - 1 : nloc - 1;
+    if (!dup) {
+      n_i = (uint32_t)(u - tid);
+      beyond = u == nloc - 1 && b0 + nloc < m && (uint32_t)(keys[b0 + nloc] >> 32) == skey[tid];
+    }
   }
-  uint64_t mk = 0;
-  uint32_t cnt = 0;
-  uint32_t prev = a;
-  for (uint64_t k = i + 1; k < m; ++k) {
-    const uint64_t t = k - b0;
-    const bool staged = t < (uint64_t)W;
-    const uint64_t kk = staged ? skey[t] : keys[k];
-    if ((uint32_t)(kk >> 32) != K) break;
-    const uint32_t b = (uint32_t)kk;
-    if (b == prev) continue;
-    prev = b;
-    const uint64_t dd = k - i - 1;
-    if (dd >= 63) mk |= MASK_TAIL;
-    const BpRec<R> rb = staged ? srec[t] : brec[b];
-    if (!pair_test(ra, ga, rb, inv, K, kb, gx, gy)) continue;
-    ++cnt;
-    if (dd < 63) mk |= 1ull << dd;
+  uint32_t total;
+  const uint32_t my_start = block_excl_scan<PAIR_BS>(n_i, scan_scratch, &total);
+  sstart[tid] = my_start;
+  if (tid == PAIR_BS - 1) sstart[PAIR_BS] = total;
+  smask[tid] = 0;
+  scnt[tid] = 0;
+  __syncthreads();
+  // balanced pass over the candidates of the workgroup's entries
+  {
+    uint32_t f = (uint32_t)(((uint64_t)total * tid) / PAIR_BS);
+    const uint32_t fend = (uint32_t)(((uint64_t)total * (tid + 1)) / PAIR_BS);
+    if (f < fend) {
+      int lo = 0, hi = PAIR_BS - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sstart[mid] <= f) lo = mid; else hi = mid - 1;
+      }
+      int e = lo;
+      uint32_t ebeg = sstart[e], eend = sstart[e + 1];
+      EntryCtx<R> c;
+      entry_ctx(c, skey[e], sbody[e], srec[e], bscene, desc);
+      uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
+      uint64_t mk = 0;
+      uint32_t cnt = 0;
+      for (; f < fend; ++f) {
+        if (f >= eend) {
+          if (cnt) { atomicOr((unsigned long long*)&smask[e], (unsigned long long)mk); atomicAdd(&scnt[e], cnt); }
+          mk = 0;
+          cnt = 0;
+          do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
+          entry_ctx(c, skey[e], sbody[e], srec[e], bscene, desc);
+          prev = c.a;
+        }
+        const int k = e + 1 + (int)(f - ebeg);
+        const uint32_t b = sbody[k];
+        if (b == prev) continue;
+        prev = b;
+        if (pair_test(c.r, c.g0, srec[k], c.inv, c.K, c.kb, c.gx, c.gy)) {
+          ++cnt;
+          const int dd = k - e - 1;
+          mk |= dd < 63 ? (1ull << dd) : MASK_TAIL;
+        }
+      }
+      if (cnt) { atomicOr((unsigned long long*)&smask[e], (unsigned long long)mk); atomicAdd(&scnt[e], cnt); }
+    }
   }
-  mask[i] = mk;
-  jpos[i] = (uint32_t)j;
-  pcount[j] = cnt;
-}
-
-// Emit pass: each entry writes its partners' shape ids (ascending) into its
-// range of the grouped pair list, from the hit mask; runs longer than the
-// mask repeat the tests for the remainder.
-template <class R>
-__global__ __launch_bounds__(256) void k_pairs_emit(uint64_t mcap, const uint64_t* __restrict__ keys,
-                                                    const BpRec<R>* __restrict__ brec,
-                                                    const uint32_t* __restrict__ bscene,
-                                                    const SceneDesc<R>* __restrict__ desc,
-                                                    const uint64_t* __restrict__ mask,
-                                                    const uint32_t* __restrict__ jpos,
-                                                    const uint32_t* __restrict__ poffs, uint64_t pcap,
-                                                    uint32_t* __restrict__ second, DevStatus* st) {
-  const uint64_t m = cap_count(mcap, &st->M);
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= m) return;
-  const uint64_t mk = mask[i];
-  if (!mk) return;
-  uint64_t o = poffs[jpos[i]];
-  uint64_t bits = mk & ~MASK_TAIL;
-  while (bits) {
-    const int dd = __ffsll((unsigned long long)bits) - 1;
-    bits &= bits - 1;
-    if (o < pcap) second[o] = (uint32_t)keys[i + 1 + dd];
-    else atomicOr(&st->overflow, 2u);
-    ++o;
-  }
-  if (mk & MASK_TAIL) {
-    const uint64_t ki = keys[i];
-    const uint32_t K = (uint32_t)(ki >> 32), a = (uint32_t)ki;
-    const BpRec<R> ra = brec[a];
-    const SceneDesc<R>& d = desc[bscene[a]];
-    const R inv[3] = {d.inv[0], d.inv[1], d.inv[2]};
-    const int ga[3] = {cell_of(ra.lo[0], inv[0]), cell_of(ra.lo[1], inv[1]), cell_of(ra.lo[2], inv[2])};
-    uint32_t prev = (uint32_t)keys[i + 63];
-    for (uint64_t k = i + 64; k < m; ++k) {
+  __syncthreads();
+  if (!valid || dup) return;
+  uint64_t mk = smask[tid];
+  uint32_t cnt = scnt[tid];
+  EntryCtx<R> c;
+  entry_ctx(c, skey[tid], sbody[tid], srec[tid], bscene, desc);
+  if (beyond) {  // run continues past the window: the rest from global memory (rare)
+    uint32_t prev = sbody[nloc - 1];
+    for (uint64_t k = b0 + nloc; k < m; ++k) {
       const uint64_t kk = keys[k];
-      if ((uint32_t)(kk >> 32) != K) break;
+      if ((uint32_t)(kk >> 32) != c.K) break;
       const uint32_t b = (uint32_t)kk;
       if (b == prev) continue;
       prev = b;
-      if (!pair_test(ra, ga, brec[b], inv, K, d.key_base, d.gx, d.gy)) continue;
-      if (o < pcap) second[o] = b;
-      else atomicOr(&st->overflow, 2u);
-      ++o;
+      if (pair_test(c.r, c.g0, brec[b], c.inv, c.K, c.kb, c.gx, c.gy)) {
+        ++cnt;
+        mk |= MASK_TAIL;
+      }
     }
   }
+  if (!cnt) return;
+  Cells cl;
+  cl.x0 = c.g0[0]; cl.y0 = c.g0[1]; cl.z0 = c.g0[2];
+  cl.x1 = c.g1[0]; cl.y1 = c.g1[1]; cl.z1 = c.g1[2];
+  const uint32_t li = local_cell_index(c.K - c.kb, cl, c.gx, c.gy);
+  const uint64_t j = (uint64_t)offs[c.slot] + li;
+  if (li == ~0u || j >= m) {
+    atomicOr(&st->bad, 8u);
+    return;
+  }
+  pcount[j] = cnt;
+  PairMeta pm;
+  pm.mask = mk;
+  pm.ipos = (uint32_t)i;
+  pm.pad = 0;
+  meta[j] = pm;
+}
+
+// Pairs per shape: the sum of its entries' counts.
+__global__ __launch_bounds__(CHUNK) void k_shape_pairs(const Chunk* __restrict__ chunks,
+                                                       const DevStatus* __restrict__ st,
+                                                       const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ offs,
+                                                       const uint32_t* __restrict__ pcount, uint64_t mcap,
+                                                       uint32_t* __restrict__ scount) {
+  const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
+  if ((uint64_t)blockIdx.x * CHUNK >= st->nslots) return;
+  const Chunk ch = chunks[blockIdx.x];
+  uint32_t sum = 0;
+  if (threadIdx.x < ch.len) {
+    const uint64_t m = cap_count(mcap, &st->M);
+    const uint64_t e0 = offs[slot], e1 = min(e0 + counts[slot], m);
+    for (uint64_t e = e0; e < e1; ++e) sum += pcount[e];
+  }
+  scount[slot] = sum;
 }
 
 // In-register bitonic sort of N values (compile-time indices only).
@@ -532,65 +602,142 @@ template <int N> __device__ __forceinline__ void sort_net(uint32_t (&v)[N]) {
 }
 
 template <int N>
-__device__ __forceinline__ void sort_group(const uint32_t* __restrict__ second, uint64_t beg, uint32_t len, uint32_t a,
-                                           uint64_t* __restrict__ pairs) {
+__device__ __forceinline__ void sort_pairs_in_place(uint64_t* __restrict__ pairs, uint64_t beg, uint32_t len,
+                                                    uint32_t a) {
   uint32_t v[N];
 #pragma unroll
-  for (int t = 0; t < N; ++t) v[t] = (uint32_t)t < len ? second[beg + t] : ~0u;
+  for (int t = 0; t < N; ++t) v[t] = (uint32_t)t < len ? (uint32_t)pairs[beg + t] : ~0u;
   sort_net<N>(v);
 #pragma unroll
   for (int t = 0; t < N; ++t)
     if ((uint32_t)t < len) pairs[beg + t] = ((uint64_t)a << 32) | v[t];
 }
 
-// Pair range of the first shape in chunk slot `slot`.
-__device__ __forceinline__ void group_range(uint64_t slot, const uint32_t* __restrict__ counts,
-                                            const uint32_t* __restrict__ offs, const uint32_t* __restrict__ poffs,
-                                            uint64_t m, uint64_t P, uint64_t* beg, uint64_t* end) {
-  const uint64_t e0 = offs[slot], e1 = e0 + counts[slot];
-  *beg = e0 < m ? min((uint64_t)poffs[e0], P) : P;
-  *end = e1 < m ? min((uint64_t)poffs[e1], P) : P;
-}
-
-// Sorts each first shape's group of second shapes and writes the final
-// pairs (a << 32 | b): ascending per scene, scenes in step order.  One
-// thread per shape slot; groups longer than 32 are listed for
-// k_pair_groups_long.
 constexpr int GROUP_LONG = 32;
 constexpr int GROUP_LDS = 4096;
 
-__global__ __launch_bounds__(CHUNK) void k_pair_groups(const Chunk* __restrict__ chunks,
-                                                       const DevStatus* __restrict__ st_in,
-                                                       const uint32_t* __restrict__ counts,
-                                                       const uint32_t* __restrict__ offs,
-                                                       const uint32_t* __restrict__ poffs, uint64_t mcap, uint64_t pcap,
-                                                       const uint32_t* __restrict__ second,
-                                                       uint64_t* __restrict__ pairs, uint32_t* __restrict__ longs,
-                                                       uint64_t long_cap, DevStatus* st) {
+template <int N>
+__device__ __forceinline__ void sort_lds_group(uint32_t* __restrict__ v, uint32_t len) {
+  uint32_t r[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) r[t] = (uint32_t)t < len ? v[t] : ~0u;
+  sort_net<N>(r);
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+    if ((uint32_t)t < len) v[t] = r[t];
+}
+
+// Partners of one shape from the hit masks of its entries (the rest of runs
+// longer than the mask by repeating the tests), appended through put(b).
+template <class R, class Put>
+__device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_t a, const uint32_t* __restrict__ pcount,
+                                               const PairMeta* __restrict__ meta, const uint64_t* __restrict__ keys,
+                                               const BpRec<R>* __restrict__ brec, const uint32_t* __restrict__ bscene,
+                                               const SceneDesc<R>* __restrict__ desc, uint64_t m, Put put) {
+  for (uint64_t e = e0; e < e1; ++e) {
+    if (!pcount[e]) continue;
+    const PairMeta pm = meta[e];
+    const uint64_t ip = pm.ipos;
+    uint64_t bits = pm.mask & ~MASK_TAIL;
+    while (bits) {
+      const int dd = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      put((uint32_t)keys[ip + 1 + dd]);
+    }
+    if (pm.mask & MASK_TAIL) {
+      const uint64_t ki = keys[ip];
+      EntryCtx<R> c;
+      entry_ctx(c, (uint32_t)(ki >> 32), a, brec[a], bscene, desc);
+      uint32_t prev = (uint32_t)keys[ip + 63];
+      for (uint64_t k = ip + 64; k < m; ++k) {
+        const uint64_t kk = keys[k];
+        if ((uint32_t)(kk >> 32) != c.K) break;
+        const uint32_t b = (uint32_t)kk;
+        if (b == prev) continue;
+        prev = b;
+        if (pair_test(c.r, c.g0, brec[b], c.inv, c.K, c.kb, c.gx, c.gy)) put(b);
+      }
+    }
+  }
+}
+
+constexpr int EMIT_PAIRS_LDS = 4096;
+
+// Writes each shape's pairs (a << 32 | b) into its range, sorted by b.  A
+// workgroup handles one chunk: partners go to LDS, each shape's group is
+// sorted in registers, and the chunk's pairs leave in one coalesced store.
+// Chunks with more pairs than fit in LDS write in place; groups longer than
+// GROUP_LONG are listed for k_pair_groups_long.
+template <class R>
+__global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
+                                                     const uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offs,
+                                                     const uint32_t* __restrict__ pcount,
+                                                     const PairMeta* __restrict__ meta,
+                                                     const uint32_t* __restrict__ scount,
+                                                     const uint32_t* __restrict__ soff, const uint64_t* __restrict__ keys,
+                                                     const BpRec<R>* __restrict__ brec,
+                                                     const uint32_t* __restrict__ bscene,
+                                                     const SceneDesc<R>* __restrict__ desc, uint64_t mcap,
+                                                     uint64_t pcap, uint64_t* __restrict__ pairs,
+                                                     uint32_t* __restrict__ longs, uint64_t long_cap, DevStatus* st) {
+  __shared__ uint32_t sb[EMIT_PAIRS_LDS];
+  __shared__ uint8_t sa[EMIT_PAIRS_LDS];
+  __shared__ uint64_t s_span;
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
-  if (threadIdx.x >= ch.len) return;
-  const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
-  uint64_t beg, end;
-  group_range(slot, counts, offs, poffs, cap_count(mcap, &st_in->M), cap_count(pcap, &st_in->P), &beg, &end);
-  if (end <= beg) return;
-  const uint32_t len = (uint32_t)(end - beg);
-  const uint32_t a = ch.begin + threadIdx.x;
-  if (len == 1) {
-    pairs[beg] = ((uint64_t)a << 32) | second[beg];
-  } else if (len <= 4) {
-    sort_group<4>(second, beg, len, a, pairs);
-  } else if (len <= 8) {
-    sort_group<8>(second, beg, len, a, pairs);
-  } else if (len <= 16) {
-    sort_group<16>(second, beg, len, a, pairs);
-  } else if (len <= GROUP_LONG) {
-    sort_group<GROUP_LONG>(second, beg, len, a, pairs);
-  } else {
-    const uint32_t g = atomicAdd(&st->nbig, 1u);  // rare: shapes with more than 32 later partners
-    if (g < long_cap) longs[g] = (uint32_t)slot;
-    else atomicOr(&st->overflow, 4u);
+  const uint64_t slot0 = (uint64_t)blockIdx.x * CHUNK;
+  const uint64_t p0 = soff[slot0];
+  const bool mine = threadIdx.x < ch.len;
+  const uint64_t slot = slot0 + threadIdx.x;
+  const uint32_t tot = mine ? scount[slot] : 0;
+  const uint64_t beg = mine ? soff[slot] : 0;
+  if (threadIdx.x == ch.len - 1) s_span = beg + tot - p0;
+  __syncthreads();
+  const uint64_t span = s_span;
+  if (p0 + span > pcap) {
+    if (threadIdx.x == 0) atomicOr(&st->overflow, 2u);
+    return;
   }
+  if (span == 0) return;
+  const bool staged = span <= EMIT_PAIRS_LDS;
+  const uint32_t a = ch.begin + threadIdx.x;
+  const uint64_t m = cap_count(mcap, &st_in->M);
+  if (mine && tot) {
+    const uint64_t e0 = offs[slot], e1 = min(e0 + counts[slot], m);
+    uint64_t o = beg;
+    if (staged) {
+      uint32_t* v = sb + (beg - p0);
+      uint32_t q = 0;
+      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, bscene, desc, m, [&](uint32_t b) { v[q++] = b; });
+      for (uint32_t t = 0; t < tot; ++t) sa[beg - p0 + t] = (uint8_t)threadIdx.x;
+      if (tot <= 4) sort_lds_group<4>(v, tot);
+      else if (tot <= 8) sort_lds_group<8>(v, tot);
+      else if (tot <= 16) sort_lds_group<16>(v, tot);
+      else if (tot <= GROUP_LONG) sort_lds_group<GROUP_LONG>(v, tot);
+      else {
+        const uint32_t g = atomicAdd(&st->nbig, 1u);
+        if (g < long_cap) longs[g] = (uint32_t)slot;
+        else atomicOr(&st->overflow, 4u);
+      }
+    } else {
+      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, bscene, desc, m,
+                        [&](uint32_t b) { pairs[o++] = ((uint64_t)a << 32) | b; });
+      if (tot <= 4) sort_pairs_in_place<4>(pairs, beg, tot, a);
+      else if (tot <= 8) sort_pairs_in_place<8>(pairs, beg, tot, a);
+      else if (tot <= 16) sort_pairs_in_place<16>(pairs, beg, tot, a);
+      else if (tot <= GROUP_LONG) sort_pairs_in_place<GROUP_LONG>(pairs, beg, tot, a);
+      else {
+        const uint32_t g = atomicAdd(&st->nbig, 1u);
+        if (g < long_cap) longs[g] = (uint32_t)slot;
+        else atomicOr(&st->overflow, 4u);
+      }
+    }
+  }
+  if (!staged) return;
+  __syncthreads();
+  for (uint64_t q = threadIdx.x; q < span; q += CHUNK)
+    pairs[p0 + q] = ((uint64_t)(ch.begin + sa[q]) << 32) | sb[q];
 }
 
 // Long groups: one workgroup per group, bitonic sort in LDS (groups beyond
@@ -598,25 +745,21 @@ __global__ __launch_bounds__(CHUNK) void k_pair_groups(const Chunk* __restrict__
 // insertion sort).
 __global__ __launch_bounds__(256) void k_pair_groups_long(const Chunk* __restrict__ chunks,
                                                           const DevStatus* __restrict__ st,
-                                                          const uint32_t* __restrict__ counts,
-                                                          const uint32_t* __restrict__ offs,
-                                                          const uint32_t* __restrict__ poffs, uint64_t mcap,
-                                                          uint64_t pcap, uint32_t* __restrict__ second,
+                                                          const uint32_t* __restrict__ scount,
+                                                          const uint32_t* __restrict__ soff,
                                                           uint64_t* __restrict__ pairs,
                                                           const uint32_t* __restrict__ longs, uint64_t long_cap) {
   __shared__ uint32_t v[GROUP_LDS];
   const uint64_t nlong = min((uint64_t)st->nbig, long_cap);
-  const uint64_t m = cap_count(mcap, &st->M), P = cap_count(pcap, &st->P);
   for (uint64_t g = blockIdx.x; g < nlong; g += gridDim.x) {
     const uint32_t slot = longs[g];
-    uint64_t beg, end;
-    group_range(slot, counts, offs, poffs, m, P, &beg, &end);
-    const uint32_t len = (uint32_t)(end - beg);
+    const uint64_t beg = soff[slot];
+    const uint32_t len = scount[slot];
     const uint32_t a = chunks[slot / CHUNK].begin + slot % CHUNK;
     if (len <= GROUP_LDS) {
       uint32_t n2 = 1;
       while (n2 < len) n2 <<= 1;
-      for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) v[t] = t < len ? second[beg + t] : ~0u;
+      for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) v[t] = t < len ? (uint32_t)pairs[beg + t] : ~0u;
       __syncthreads();
       for (uint32_t k = 2; k <= n2; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -634,15 +777,14 @@ __global__ __launch_bounds__(256) void k_pair_groups_long(const Chunk* __restric
       __syncthreads();
     } else if (threadIdx.x == 0) {
       for (uint32_t t = 1; t < len; ++t) {
-        const uint32_t x = second[beg + t];
+        const uint64_t x = pairs[beg + t];
         uint32_t u = t;
-        while (u > 0 && second[beg + u - 1] > x) {
-          second[beg + u] = second[beg + u - 1];
+        while (u > 0 && pairs[beg + u - 1] > x) {
+          pairs[beg + u] = pairs[beg + u - 1];
           --u;
         }
-        second[beg + u] = x;
+        pairs[beg + u] = x;
       }
-      for (uint32_t t = 0; t < len; ++t) pairs[beg + t] = ((uint64_t)a << 32) | second[beg + t];
     }
   }
 }
@@ -657,7 +799,7 @@ struct SceneOut {
 template <class R>
 __global__ void k_scene_ranges(const uint32_t* __restrict__ sel, const SceneDesc<R>* __restrict__ desc,
                                const DevStatus* __restrict__ st, uint64_t mcap, uint64_t pcap,
-                               const uint32_t* __restrict__ offs, const uint32_t* __restrict__ poffs,
+                               const uint32_t* __restrict__ offs, const uint32_t* __restrict__ soff,
                                const uint32_t* __restrict__ hitpos, SceneOut* __restrict__ out) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= st->nsel) return;
@@ -665,13 +807,13 @@ __global__ void k_scene_ranges(const uint32_t* __restrict__ sel, const SceneDesc
   const uint64_t nslot = st->nslots;
   const uint64_t M = cap_count(mcap, &st->M);
   const uint64_t P = cap_count(pcap, &st->P);
-  auto entry_at = [&](uint64_t slot) -> uint64_t { return slot < nslot ? offs[slot] : M; };
-  auto pair_at = [&](uint64_t e) -> uint64_t { return e < M ? min((uint64_t)poffs[e], P) : P; };
+  auto entry_at = [&](uint64_t slot) -> uint64_t { return slot < nslot ? min((uint64_t)offs[slot], M) : M; };
+  auto pair_at = [&](uint64_t slot) -> uint64_t { return slot < nslot ? min((uint64_t)soff[slot], P) : P; };
   SceneOut o;
   o.m_begin = entry_at((uint64_t)d.chunk_begin * CHUNK);
   o.m_end = entry_at((uint64_t)d.chunk_end * CHUNK);
-  o.pair_begin = pair_at(o.m_begin);
-  o.pair_end = pair_at(o.m_end);
+  o.pair_begin = pair_at((uint64_t)d.chunk_begin * CHUNK);
+  o.pair_end = pair_at((uint64_t)d.chunk_end * CHUNK);
   o.contact_begin = o.pair_begin < P ? hitpos[o.pair_begin] : st->C;
   o.contact_end = o.pair_end < P ? hitpos[o.pair_end] : st->C;
   out[j] = o;

@@ -381,9 +381,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t device_bytes() const override {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
-                           &pcount_, &poffs_, &pairs_, &pairs_tmp_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
+                           &pcount_, &pairs_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_,
-                           &scan_tmp_, &radix_tmp_, &chunks_, &sel_, &desc_, &souts_, &longs_, &bscene_, &mask_, &jpos_};
+                           &scan_tmp_, &radix_tmp_, &chunks_, &sel_, &desc_, &souts_, &longs_, &bscene_, &meta_, &scount_, &soff_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -609,7 +609,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t step_bytes_estimate(uint64_t n) const override {
     // keys + ping-pong and per-entry pair counts/offsets; pairs, narrowphase
     // records, queues and contact outputs per pair; slot counts/offsets
-    const double per_m = 2 * 8 + 2 * 4 + 8 + 4;
+    const double per_m = 2 * 8 + 4 + 16;
     const double per_p = 2 * 8 + 10 * sizeof(R) + 8 + 4 + 4 + 2 * 8 + 4 + 1 + 10 * sizeof(R) + 8 + 8;
     return (uint64_t)((double)n * (cap_m_per_shape_ * per_m + cap_p_per_shape_ * per_p + 8.0)) + (1u << 20);
   }
@@ -786,6 +786,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       kpart_.reserve(chunk_cap_ * 4);
       counts_buf_.reserve(chunk_cap_ * CHUNK * 4);
       offs_.reserve(chunk_cap_ * CHUNK * 4);
+      scount_.reserve(chunk_cap_ * CHUNK * 4);
+      soff_.reserve(chunk_cap_ * CHUNK * 4);
       graph_valid_ = false;
     }
     if (nsel_ > sel_cap_ || !sel_.p) {
@@ -873,9 +875,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     keys_.reserve(mcap_ * 8);
     keys_tmp_.reserve(mcap_ * 8);
     pcount_.reserve((mcap_ + 1) * 4);
-    poffs_.reserve((mcap_ + 1) * 4);
-    mask_.reserve(mcap_ * 8);
-    jpos_.reserve(mcap_ * 4);
+    meta_.reserve(mcap_ * sizeof(PairMeta));
     mark(ST_BP_EMIT, true);
     hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
@@ -900,12 +900,15 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
                        (const uint32_t*)bscene_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
-                       (const uint32_t*)offs_.as<uint32_t>(), pcount_.as<uint32_t>(), mask_.as<uint64_t>(),
-                       jpos_.as<uint32_t>(), st());
+                       (const uint32_t*)offs_.as<uint32_t>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(), st());
     mark(ST_BP_PAIRS, false);
     mark(ST_BP_PAIRSCAN, true);
-    scan_excl_u32(pcount_.as<uint32_t>(), poffs_.as<uint32_t>(), mcap_, &st()->P, radix_tmp_.as<uint32_t>(), s_,
-                  &st()->M);
+    hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
+                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
+                       (const uint32_t*)counts_buf_.as<uint32_t>(), (const uint32_t*)offs_.as<uint32_t>(),
+                       (const uint32_t*)pcount_.as<uint32_t>(), mcap_, scount_.as<uint32_t>());
+    scan_excl_u32(scount_.as<uint32_t>(), soff_.as<uint32_t>(), slots, &st()->P, scan_tmp_.as<uint32_t>(), s_,
+                  &st()->nslots);
     mark(ST_BP_PAIRSCAN, false);
     if (exact_) {
       read_status();
@@ -915,27 +918,23 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       pcap_ = std::max<uint64_t>(P_, 1);
     }
     pairs_.reserve(pcap_ * 8);
-    pairs_tmp_.reserve(pcap_ * 8);
-    mark(ST_BP_PAIREMIT, true);
-    hipLaunchKernelGGL(k_pairs_emit<R>, dim3(blocks_for(mcap_, 256)), dim3(256), 0, s_, mcap_,
-                       (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const uint32_t*)bscene_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
-                       (const uint64_t*)mask_.as<uint64_t>(), (const uint32_t*)jpos_.as<uint32_t>(),
-                       (const uint32_t*)poffs_.as<uint32_t>(), pcap_, pairs_tmp_.as<uint32_t>(), st());
-    mark(ST_BP_PAIREMIT, false);
     const uint64_t long_cap = pcap_ / (GROUP_LONG + 1) + 1;
     longs_.reserve(long_cap * 4);
-    mark(ST_BP_PAIRSORT, true);
-    hipLaunchKernelGGL(k_pair_groups, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
+    mark(ST_BP_PAIREMIT, true);
+    hipLaunchKernelGGL(k_pair_emit<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
                        (const uint32_t*)counts_buf_.as<uint32_t>(), (const uint32_t*)offs_.as<uint32_t>(),
-                       (const uint32_t*)poffs_.as<uint32_t>(), mcap_, pcap_, (const uint32_t*)pairs_tmp_.as<uint32_t>(),
-                       pairs_.as<uint64_t>(), longs_.as<uint32_t>(), long_cap, st());
+                       (const uint32_t*)pcount_.as<uint32_t>(), (const PairMeta*)meta_.as<PairMeta>(),
+                       (const uint32_t*)scount_.as<uint32_t>(), (const uint32_t*)soff_.as<uint32_t>(),
+                       (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
+                       (const uint32_t*)bscene_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), mcap_,
+                       pcap_, pairs_.as<uint64_t>(), longs_.as<uint32_t>(), long_cap, st());
+    mark(ST_BP_PAIREMIT, false);
+    mark(ST_BP_PAIRSORT, true);
     hipLaunchKernelGGL(k_pair_groups_long, dim3(256), dim3(256), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
-                       (const DevStatus*)st(), (const uint32_t*)counts_buf_.as<uint32_t>(),
-                       (const uint32_t*)offs_.as<uint32_t>(), (const uint32_t*)poffs_.as<uint32_t>(), mcap_, pcap_,
-                       pairs_tmp_.as<uint32_t>(), pairs_.as<uint64_t>(), (const uint32_t*)longs_.as<uint32_t>(),
-                       long_cap);
+                       (const DevStatus*)st(), (const uint32_t*)scount_.as<uint32_t>(),
+                       (const uint32_t*)soff_.as<uint32_t>(), pairs_.as<uint64_t>(),
+                       (const uint32_t*)longs_.as<uint32_t>(), long_cap);
     mark(ST_BP_PAIRSORT, false);
     if (exact_) {
       counts_.n = scenes_[0].n_shapes; counts_.m = M_; counts_.a = A_; counts_.p = P_;
@@ -1039,7 +1038,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_scene_ranges<R>, dim3(blocks_for(sel_grid(), 64)), dim3(64), 0, s_,
                        (const uint32_t*)sel_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
                        (const DevStatus*)st(), mcap_, pcap_, (const uint32_t*)offs_.as<uint32_t>(),
-                       (const uint32_t*)poffs_.as<uint32_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
+                       (const uint32_t*)soff_.as<uint32_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
                        souts_.as<SceneOut>());
   }
 
@@ -1107,11 +1106,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     radix_tmp_.reserve(std::max<size_t>(std::max(radix_tmp_words(std::max(mcap_, slots)), radix_tmp_words(P)),
                                         scan_tmp_words(mcap_) + 16) * 4);
     pcount_.reserve((mcap_ + 1) * 4);
-    poffs_.reserve((mcap_ + 1) * 4);
-    mask_.reserve(mcap_ * 8);
-    jpos_.reserve(mcap_ * 4);
+    meta_.reserve(mcap_ * sizeof(PairMeta));
     pairs_.reserve(P * 8);
-    pairs_tmp_.reserve(P * 8);
     longs_.reserve((P / (GROUP_LONG + 1) + 1) * 4);
     np_v_.reserve(P * 10 * sizeof(R));
     np_ids_.reserve(P * 8);
@@ -1205,11 +1201,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   int mode_ = NP_NATIVE;
   float tau_ = 1e-2f;
 
-  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, bscene_, mask_, jpos_;
+  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, bscene_, meta_, scount_, soff_;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
   DevBuf chunks_, sel_, desc_, souts_;
-  DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, poffs_, pairs_, pairs_tmp_;
+  DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;
   DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
   DevBuf scan_tmp_, radix_tmp_;
