@@ -224,7 +224,12 @@ __device__ __forceinline__ float bound_radius(int t, v3<float> b) {
 // returns MPR_UNSURE instead of a verdict whenever FP32 rounding could decide
 // it differently from FP64: a separation or penetration within tau of zero,
 // (nearly) coincident centres, a (nearly) degenerate portal, or an iteration
-// cap reached.
+// cap reached.  With verdict_only (the pair is re-solved in FP64 unless it
+// is a clear miss) the cap is VERDICT_ITERATIONS per phase: a pair that has
+// not separated by then goes to FP64, so one slow lane does not hold its
+// whole wavefront for the full 34 iterations.
+constexpr int VERDICT_ITERATIONS = 6;
+
 __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float tau, bool verdict_only,
                            v3<float>& rnormal, v3<float>& point) {
   typedef float R;
@@ -259,7 +264,7 @@ __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float
   v3<R> v31, v32, v3_;
   bool hit = false;
   int phase1 = 0, phase2 = 0;
-  const int max_iterations = 34;
+  const int max_iterations = verdict_only ? VERDICT_ITERATIONS : 34;
   while (true) {
     if (phase1 > max_iterations) return MPR_UNSURE;
     phase1++;
