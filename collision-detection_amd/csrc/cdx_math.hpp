@@ -138,6 +138,49 @@ template <class R> CDX_HD v3<R> support_world(int type, v3<R> X, v3<R> B, const 
   return mul(m, support_local(type, B, rn)) + X;
 }
 
+// Approximate FP32 variants (hardware reciprocal / reciprocal square root,
+// about 1 ulp) for the FP32 filter, whose verdicts keep a margin of tau far
+// above that error.  A zero vector normalises to (1, 0, 0) as above.
+__device__ __forceinline__ v3<float> normalize_fast(v3<float> v) {
+  const float d = dot(v, v);
+  if (d == 0.f) return mk3(1.f, 0.f, 0.f);
+  return v * __builtin_amdgcn_rsqf(d);
+}
+__device__ __forceinline__ float length_fast(v3<float> v) { return __builtin_amdgcn_sqrtf(dot(v, v)); }
+
+__device__ __forceinline__ v3<float> support_local_fast(int type, v3<float> B, v3<float> n) {
+  switch (type) {
+    case SPHERE: return B.x * n;
+    case BOX: return mk3(sgn(n.x) * B.x, sgn(n.y) * B.y, sgn(n.z) * B.z);
+    case ELLIPSOID: {
+      v3<float> nn = normalize_fast(n);
+      return B * B * nn * __builtin_amdgcn_rcpf(length_fast(B * nn));
+    }
+    case CYLINDER: {
+      const float l2 = n.x * n.x + n.z * n.z;
+      const float s = sgn(n.y) * B.y;
+      if (l2 != 0.f) {
+        const float r = B.x * __builtin_amdgcn_rsqf(l2);
+        return mk3(n.x * r, s, n.z * r);
+      }
+      return mk3(0.f, s, 0.f);
+    }
+    case CAPSULE: {
+      v3<float> nn = normalize_fast(n);
+      const float s = sgn(n.y) * B.y;
+      return mk3(B.x * nn.x, s + B.x * nn.y, B.x * nn.z);
+    }
+    default: return mk3(0.f, 0.f, 0.f);
+  }
+}
+
+__device__ __forceinline__ v3<float> support_world_fast(int type, v3<float> X, v3<float> B, const mat3<float>& m,
+                                                        v3<float> b) {
+  v3<float> n = normalize_fast(b);
+  v3<float> rn = mulT(m, n);
+  return mul(m, support_local_fast(type, B, rn)) + X;
+}
+
 // Order-preserving map float -> uint32 used for atomic min / max of bounds.
 __device__ __forceinline__ uint32_t f2ord(float f) {
   uint32_t u = __float_as_uint(f);

@@ -7,15 +7,17 @@ namespace cdx {
 namespace {
 
 
+// A shape placed in the world for the reference-order MPR: type, centre,
+// dimensions, rotation.  The rotation matrix is rebuilt from the quaternion
+// at every support call: the same rounded values each time, and far fewer
+// live registers than a cached 3x3 per shape (the FP64 MPR is
+// occupancy-bound: caching both matrices takes it to 250 VGPRs).
 template <class R> struct Placed {
   int type;
   v3<R> X, B;
   q4<R> q;
 };
 
-// The rotation matrix is rebuilt from the quaternion at every support call:
-// the same rounded values each time, and far fewer live registers than a
-// cached 3x3 per shape (the FP64 MPR is occupancy-bound).
 template <class R> __device__ __forceinline__ v3<R> sup(const Placed<R>& s, v3<R> b) {
   return support_world(s.type, s.X, s.B, rotmat(s.q), b);
 }
@@ -27,6 +29,18 @@ template <class R> __device__ __forceinline__ Placed<R> placed(const ShapeRec<R>
   s.B = mk3(r.bx, r.by, r.bz);
   s.q.w = r.qw; s.q.x = r.qx; s.q.y = r.qy; s.q.z = r.qz;
   return s;
+}
+
+// A shape for the FP32 filter: rotation matrix built once per pair (cheap
+// in FP32 registers), approximate normalisation in the support maps.
+struct PlacedF {
+  int type;
+  v3<float> X, B;
+  mat3<float> m;
+};
+
+__device__ __forceinline__ v3<float> sup_fast(const PlacedF& s, v3<float> b) {
+  return support_world_fast(s.type, s.X, s.B, s.m, b);
 }
 
 template <class R> __device__ __forceinline__ void swap3(v3<R>& a, v3<R>& b) { v3<R> t = a; a = b; b = t; }
@@ -230,30 +244,30 @@ __device__ __forceinline__ float bound_radius(int t, v3<float> b) {
 // whole wavefront for the full 34 iterations.
 constexpr int VERDICT_ITERATIONS = 6;
 
-__device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float tau, bool verdict_only,
+__device__ int mpr_guarded(const PlacedF& A, const PlacedF& B, float tau, bool verdict_only,
                            v3<float>& rnormal, v3<float>& point) {
   typedef float R;
   v3<R> v01, v02, v0, n, v11, v12, v1, v21, v22, v2;
   v01 = A.X;
   v02 = B.X;
   v0 = v02 - v01;
-  if (length(v0) < 1e-4f * tau) return MPR_UNSURE;
+  if (length_fast(v0) < 1e-4f * tau) return MPR_UNSURE;
   n = -v0;
-  v11 = sup(A, -n);
-  v12 = sup(B, n);
+  v11 = sup_fast(A, -n);
+  v12 = sup_fast(B, n);
   v1 = v12 - v11;
   R dn = dot(v1, n);
-  if (dn <= R(0)) return (-dn < tau * length(n)) ? MPR_UNSURE : MPR_MISS;
+  if (dn <= R(0)) return (-dn < tau * length_fast(n)) ? MPR_UNSURE : MPR_MISS;
   n = cross(v1, v0);
   {
     R nn = dot(n, n), s = 1e-6f * dot(v1, v1) * dot(v0, v0);
     if (nn <= s) return MPR_UNSURE;
   }
-  v21 = sup(A, -n);
-  v22 = sup(B, n);
+  v21 = sup_fast(A, -n);
+  v22 = sup_fast(B, n);
   v2 = v22 - v21;
   dn = dot(v2, n);
-  if (dn <= R(0)) return (-dn < tau * length(n)) ? MPR_UNSURE : MPR_MISS;
+  if (dn <= R(0)) return (-dn < tau * length_fast(n)) ? MPR_UNSURE : MPR_MISS;
   n = cross((v1 - v0), (v2 - v0));
   if (dot(n, v0) > R(0)) {
     swap3(v1, v2);
@@ -268,11 +282,11 @@ __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float
   while (true) {
     if (phase1 > max_iterations) return MPR_UNSURE;
     phase1++;
-    v31 = sup(A, -n);
-    v32 = sup(B, n);
+    v31 = sup_fast(A, -n);
+    v32 = sup_fast(B, n);
     v3_ = v32 - v31;
     dn = dot(v3_, n);
-    if (dn <= R(0)) return (-dn < tau * length(n)) ? MPR_UNSURE : MPR_MISS;
+    if (dn <= R(0)) return (-dn < tau * length_fast(n)) ? MPR_UNSURE : MPR_MISS;
     if (dot(cross(v1, v3_), v0) < R(0)) {
       v2 = v3_; v21 = v31; v22 = v32;
       n = cross((v1 - v0), (v3_ - v0));
@@ -286,13 +300,13 @@ __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float
     while (true) {
       phase2++;
       n = cross((v2 - v1), (v3_ - v1));
-      n = normalize(n);
+      n = normalize_fast(n);
       R side = dot(n, v1);
       if (side >= R(0) && !hit) hit = true;
       // The origin is inside the portal by more than tau: a certain hit.
       if (verdict_only && side > tau) return MPR_HIT;
-      v3<R> v41 = sup(A, -n);
-      v3<R> v42 = sup(B, n);
+      v3<R> v41 = sup_fast(A, -n);
+      v3<R> v42 = sup_fast(B, n);
       v3<R> v4 = v42 - v41;
       R delta = dot((v4 - v3_), n);
       R depth = dot(v4, n);
@@ -317,7 +331,7 @@ __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float
         R inv = R(1) / sum;
         point = (b0 * v01 + b1 * v11 + b2 * v21 + b3 * v31) + (b0 * v02 + b1 * v12 + b2 * v22 + b3 * v32);
         point = point * inv * R(.5);
-        rnormal = normalize(n);
+        rnormal = normalize_fast(n);
         return MPR_HIT;
       }
       if (dot(cross(v4, v1), v0) < R(0)) {
@@ -331,12 +345,14 @@ __device__ int mpr_guarded(const Placed<float>& A, const Placed<float>& B, float
   }
 }
 
-__device__ __forceinline__ Placed<float> placed_local(const ShapeRecF& r, const ShapeRecF& origin) {
-  Placed<float> s;
+__device__ __forceinline__ PlacedF placed_local(const ShapeRecF& r, const ShapeRecF& origin) {
+  PlacedF s;
   s.type = (int)r.type;
   s.X = mk3((float)(r.x - origin.x), (float)(r.y - origin.y), (float)(r.z - origin.z));
   s.B = mk3(r.bx, r.by, r.bz);
-  s.q.w = r.qw; s.q.x = r.qx; s.q.y = r.qy; s.q.z = r.qz;
+  q4<float> q;
+  q.w = r.qw; q.x = r.qx; q.y = r.qy; q.z = r.qz;
+  s.m = rotmat(q);
   return s;
 }
 
@@ -408,13 +424,13 @@ __global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t*
   const uint64_t pr = pairs[i];
   const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
   const ShapeRecF fa = frec[ia], fb = frec[ib];
-  Placed<float> A = placed_local(fa, fa), B = placed_local(fb, fa);
+  PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
   float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
   v3<float> N, p0;
   int r = mpr_guarded(A, B, tau, !KEEP, N, p0);
   if (KEEP && r == MPR_HIT) {
-    v3<float> p1 = dot((sup(A, -N) - p0), N) * N + p0;
-    v3<float> p2 = dot((sup(B, N) - p0), N) * N + p0;
+    v3<float> p1 = dot((sup_fast(A, -N) - p0), N) * N + p0;
+    v3<float> p2 = dot((sup_fast(B, N) - p0), N) * N + p0;
     N = -N;
     float dl = dot(N, p2 - p1);
     if (fabsf(dl) < tau) {
