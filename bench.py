@@ -276,7 +276,10 @@ def served_leg(pkg, scenes, ctx, args):
     Each request replaces every body pose of one uniformly chosen scene with
     the next of two precomputed pose sets (displacement N(0, (0.05 size)^2),
     rotation by N(0, 0.05) rad about a random axis) and returns its contacts.
-    Latency runs from the submit call to the result being ready on the host.
+    Producer threads write each request's poses into a pinned buffer of the
+    server ahead of its arrival (as a caller producing poses in place would);
+    latency runs from the submit call at the arrival time to the result being
+    ready on the host.
     The plain per-scene path runs the same requests one at a time on
     resident cdx_scene objects (pose upload + step + contact count).
     """
@@ -292,7 +295,8 @@ def served_leg(pkg, scenes, ctx, args):
         p1, r1 = scenes.pose_step(s, np.random.default_rng(int(seeds[i]) + 1))
         return s, p1, r1
 
-    srv = pkg.Server(ctx, np.float64, mode=pkg.NP_MIXED, max_batch_shapes=args.served_batch_shapes)
+    srv = pkg.Server(ctx, np.float64, mode=pkg.NP_MIXED, max_batch_shapes=args.served_batch_shapes,
+                     max_pending=args.served_max_pending, use_graph=not args.served_eager)
     poses = [None] * count
     t0 = time.perf_counter()
     from concurrent.futures import ThreadPoolExecutor
@@ -337,17 +341,56 @@ def served_leg(pkg, scenes, ctx, args):
     # the sample's mean cost, weighted to the whole fleet's mean size
     plain_sps = len(plain) / float(plain.sum())
 
-    # warm up every class (graph capture, result blocks)
+    # warm up every class (graph capture, pinned pose and result blocks of
+    # every size bucket) with one request per scene
     state = {i: 0 for i in ids}
-    for i in ids[:64]:
-        srv.wait(srv.submit(sid_of[i], *poses[i][1]), 60_000)
+    warm = []
+    for i in ids:
+        warm.append(srv.submit(sid_of[i], *poses[i][1]))
+        if len(warm) >= 64:
+            for r in warm:
+                srv.wait(r, 60_000)
+                srv.release(r)
+            warm = []
+    for r in warm:
+        srv.wait(r, 60_000)
+        srv.release(r)
     srv.stats(reset=True)
 
     points = []
     for rate in [float(x) for x in args.served_rates.split(",") if x]:
+        rr = np.random.default_rng(int(rate))
+        n_req = int(rate * args.served_seconds)
+        arrivals = np.cumsum(rr.exponential(1.0 / rate, n_req))
+        targets = rr.choice(ids, n_req)
+        which = np.zeros(n_req, np.int8)
+        for r, i in enumerate(targets):  # each request of a scene takes the other pose set
+            state[i] ^= 1
+            which[r] = state[i]
+        # Producers fill pinned pose buffers ahead of the arrivals (the caller
+        # writes its new poses in place); a request is submitted at its
+        # arrival time, and its latency runs from that submit call.
+        ready = {}
+        cond = threading.Condition()
+        ahead = 256
+
+        def producer(start, step):
+            for r in range(start, n_req, step):
+                with cond:
+                    cond.wait_for(lambda: r < progress[0] + ahead)
+                i = targets[r]
+                pos, rot = poses[i][which[r]]
+                h, bp, br = srv.pose_buffer(sid_of[i], len(pos))
+                np.copyto(bp, pos)
+                np.copyto(br, rot)
+                with cond:
+                    ready[r] = h
+                    cond.notify_all()
+
         outstanding = []
         lock = threading.Lock()
         done_flag = [False]
+        progress = [0]
 
         def reaper():
             while True:
@@ -361,38 +404,50 @@ def served_leg(pkg, scenes, ctx, args):
                 srv.wait(item, 120_000)
                 srv.release(item)
 
+        prods = [threading.Thread(target=producer, args=(k, 8)) for k in range(8)]
+        for t in prods:
+            t.start()
         th = threading.Thread(target=reaper)
         th.start()
+        with cond:  # let the producers fill the first buffers before the clock starts
+            cond.wait_for(lambda: len(ready) >= min(ahead // 2, n_req), timeout=30)
         srv.stats(reset=True)
-        rr = np.random.default_rng(int(rate))
-        n_req = int(rate * args.served_seconds)
-        arrivals = np.cumsum(rr.exponential(1.0 / rate, n_req))
-        targets = rr.choice(ids, n_req)
         refused_q = 0
+        lag = []
         t_start = time.perf_counter()
-        for a, i in zip(arrivals, targets):
+        for r in range(n_req):
             now = time.perf_counter() - t_start
-            if a > now:
-                time.sleep(a - now)
-            state[i] ^= 1
-            pos, rot = poses[i][state[i]]
+            if arrivals[r] > now:
+                time.sleep(arrivals[r] - now)
+            with cond:
+                cond.wait_for(lambda: r in ready)
+                h = ready.pop(r)
+                progress[0] = r + 1
+                cond.notify_all()
+            lag.append(time.perf_counter() - t_start - arrivals[r])
             try:
-                rid = srv.submit(sid_of[i], pos, rot)
+                rid = srv.submit_buffer(h)
             except pkg.CdxError:
                 refused_q += 1
+                srv.drop_buffer(h)
                 continue
             with lock:
                 outstanding.append(rid)
         t_submit_end = time.perf_counter() - t_start
         done_flag[0] = True
         th.join()
+        for t in prods:
+            t.join()
         st = srv.stats(reset=True)
         wall = time.perf_counter() - t_start
         pt = {"rate_rps": rate, "requests": int(st["completed"]), "queue_refused": refused_q,
               "scenes_per_s": round(st["completed"] / wall, 1),
               "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
               "steps": int(st["steps"]), "requests_per_step": round(st["stepped_requests"] / max(st["steps"], 1), 2),
-              "device_busy_frac": round(st["busy_ms"] / 1e3 / wall, 3), "submit_lag_s": round(t_submit_end - arrivals[-1], 3)}
+              "device_busy_frac": round(st["busy_ms"] / 1e3 / wall, 3),
+              "worker_host_ms_per_step": {k: round(v / max(st["steps"], 1), 3) for k, v in st["host_ms"].items()},
+              "submit_lag_p99_ms": round(float(np.percentile(lag, 99)) * 1e3, 3),
+              "submit_end_lag_s": round(t_submit_end - float(arrivals[-1]), 3)}
         log(f"served rate {rate}: {pt}")
         points.append(pt)
     ok = [p for p in points if p["p99_ms"] <= 2.0]
@@ -451,10 +506,13 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=512, help="0 skips the batched leg")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--served-scenes", type=int, default=4096, help="0 skips the served leg")
-    ap.add_argument("--served-rates", type=str, default="1000,2000,4000,8000",
+    ap.add_argument("--served-rates", type=str, default="500,1000,2000,4000,6000",
                     help="Poisson arrival rates (requests/s) swept by the served leg")
     ap.add_argument("--served-seconds", type=float, default=3.0, help="seconds of arrivals per rate")
     ap.add_argument("--served-batch-shapes", type=int, default=1 << 19)
+    ap.add_argument("--served-max-pending", type=int, default=4096,
+                    help="queued requests before the server refuses new ones")
+    ap.add_argument("--served-eager", action="store_true", help="launch served steps without HIP graphs")
     ap.add_argument("--plain-sample", type=int, default=256, help="scenes timed on the plain per-scene path")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (calibrates HBM byte counters)")

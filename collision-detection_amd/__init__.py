@@ -46,7 +46,8 @@ EXPORTS = [
     "cdx_batch_step_ms", "cdx_batch_device_bytes", "cdx_batch_scene_bytes", "cdx_server_create",
     "cdx_server_destroy", "cdx_server_last_error", "cdx_server_add_scene", "cdx_server_scene_info",
     "cdx_server_submit", "cdx_server_poll", "cdx_server_wait", "cdx_server_result", "cdx_server_read_contacts",
-    "cdx_server_release", "cdx_server_stats", "cdx_server_memory",
+    "cdx_server_release", "cdx_server_stats", "cdx_server_memory", "cdx_server_pose_buffer",
+    "cdx_server_submit_buffer", "cdx_server_drop_buffer",
 ]
 
 
@@ -62,11 +63,13 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("steps", C.c_uint64),
         ("stepped_requests", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
-        ("max_ms", C.c_double), ("busy_ms", C.c_double), ("window_s", C.c_double),
+        ("max_ms", C.c_double), ("busy_ms", C.c_double), ("window_s", C.c_double), ("host_ms", C.c_double * 6),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "host_ms"}
+        d["host_ms"] = dict(zip(["upload", "select", "launch", "wait", "block", "readback"], list(self.host_ms)))
+        return d
 
 
 class CdxError(RuntimeError):
@@ -170,6 +173,9 @@ def lib():
         "cdx_server_release": ([vp, u64], i32),
         "cdx_server_stats": ([vp, vp, i32], i32),
         "cdx_server_memory": ([vp, vp, vp, vp], i32),
+        "cdx_server_pose_buffer": ([vp, C.c_uint32, u64, vp, vp, vp], i32),
+        "cdx_server_submit_buffer": ([vp, u64, vp], i32),
+        "cdx_server_drop_buffer": ([vp, u64], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -524,6 +530,23 @@ class Server:
         out = C.c_uint64(0)
         self._chk(lib().cdx_server_submit(self.h, int(scene), len(pos), _p(idx), _p(pos), _p(rot), C.byref(out)))
         return out.value
+
+    def pose_buffer(self, scene, k):
+        """Pinned (positions (k,3), rotations (k,4)) arrays for bodies 0..k-1, filled in place, and their handle."""
+        p, r, h = C.c_void_p(), C.c_void_p(), C.c_uint64(0)
+        self._chk(lib().cdx_server_pose_buffer(self.h, int(scene), int(k), C.byref(p), C.byref(r), C.byref(h)))
+        ct = C.c_double if self.dtype == np.float64 else C.c_float
+        pos = np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(int(k), 3))
+        rot = np.ctypeslib.as_array(C.cast(r, C.POINTER(ct)), shape=(int(k), 4))
+        return h.value, pos, rot
+
+    def submit_buffer(self, handle) -> int:
+        out = C.c_uint64(0)
+        self._chk(lib().cdx_server_submit_buffer(self.h, int(handle), C.byref(out)))
+        return out.value
+
+    def drop_buffer(self, handle):
+        self._chk(lib().cdx_server_drop_buffer(self.h, int(handle)))
 
     def poll(self, req) -> bool:
         d = C.c_int(0)

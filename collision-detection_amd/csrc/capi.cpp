@@ -458,6 +458,15 @@ int cdx_server_submit(cdx_server* server, uint32_t scene, uint64_t k, const uint
   SERVER_CALL(CDX_REQUIRE(request_out, CDX_ERR_INVALID, "null output");
               *request_out = sv.submit(scene, k, idx, body_pos, body_rot));
 }
+int cdx_server_pose_buffer(cdx_server* server, uint32_t scene, uint64_t k, void** body_pos, void** body_rot,
+                           uint64_t* handle) {
+  SERVER_CALL(CDX_REQUIRE(handle, CDX_ERR_INVALID, "null output");
+              *handle = sv.pose_buffer(scene, k, body_pos, body_rot));
+}
+int cdx_server_submit_buffer(cdx_server* server, uint64_t handle, uint64_t* request_out) {
+  SERVER_CALL(CDX_REQUIRE(request_out, CDX_ERR_INVALID, "null output"); *request_out = sv.submit_buffer(handle));
+}
+int cdx_server_drop_buffer(cdx_server* server, uint64_t handle) { SERVER_CALL(sv.drop_buffer(handle)); }
 int cdx_server_poll(cdx_server* server, uint64_t request, int* done) {
   SERVER_CALL(CDX_REQUIRE(done, CDX_ERR_INVALID, "null output"); *done = sv.poll(request));
 }
@@ -477,7 +486,8 @@ int cdx_server_stats(cdx_server* server, cdx_server_stats_t* out, int reset) {
               out->submitted = s.submitted; out->completed = s.completed; out->failed = s.failed;
               out->steps = s.batches; out->stepped_requests = s.batched_requests; out->p50_ms = s.p50_ms;
               out->p99_ms = s.p99_ms; out->mean_ms = s.mean_ms; out->max_ms = s.max_ms; out->busy_ms = s.busy_ms;
-              out->window_s = s.window_s; if (reset) sv.reset_stats());
+              out->window_s = s.window_s; for (int q = 0; q < 6; ++q) out->host_ms[q] = s.host_ms[q];
+              if (reset) sv.reset_stats());
 }
 int cdx_server_memory(cdx_server* server, uint64_t* hbm_budget, uint64_t* hbm_committed, uint64_t* device_bytes) {
   SERVER_CALL(if (hbm_budget) *hbm_budget = sv.hbm_budget(); if (hbm_committed) *hbm_committed = sv.hbm_committed();

@@ -52,6 +52,16 @@ __global__ void k_stage_begin(const StepHdr* __restrict__ hdr, DevStatus* st, in
   if (what & ZERO_NP) { st->Q = st->Q2 = st->C = 0; }
 }
 
+// Chunk ranges of the selected scenes, written into their descriptors.
+template <class R>
+__global__ void k_apply_selection(uint64_t nsel, const uint32_t* __restrict__ rng, SceneDesc<R>* __restrict__ desc) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsel) return;
+  SceneDesc<R>& d = desc[rng[3 * j]];
+  d.chunk_begin = rng[3 * j + 1];
+  d.chunk_end = rng[3 * j + 2];
+}
+
 template <class R>
 __global__ void k_deinterleave(uint64_t n, int k, const R* __restrict__ aos, R* __restrict__ soa, uint64_t stride,
                                uint64_t offset) {
@@ -176,6 +186,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     for (int i = 0; i < 2 * ST_COUNT + 2; ++i) (void)hipEventDestroy(ev_[i]);
     (void)hipHostFree(hstat_);
     (void)hipHostFree(hhdr_);
+    if (stage_host_) (void)hipHostFree(stage_host_);
+    if (souts_pinned_) (void)hipHostFree(souts_pinned_);
     (void)hipStreamDestroy(s_);
   }
 
@@ -507,9 +519,17 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     m_seen_ = std::max(m_seen_, M_);
     p_seen_ = std::max(p_seen_, P_);
     souts_host_.resize(selection_.size());
-    if (!selection_.empty())
-      CDX_HIP_CHECK(hipMemcpy(souts_host_.data(), souts_.p, selection_.size() * sizeof(SceneOut),
-                              hipMemcpyDeviceToHost));
+    if (!selection_.empty()) {
+      const size_t bytes = selection_.size() * sizeof(SceneOut);
+      if (bytes > souts_pinned_bytes_) {
+        if (souts_pinned_) CDX_HIP_CHECK(hipHostFree(souts_pinned_));
+        souts_pinned_bytes_ = std::max<size_t>(bytes * 2, 4096);
+        CDX_HIP_CHECK(hipHostMalloc(&souts_pinned_, souts_pinned_bytes_, hipHostMallocDefault));
+      }
+      CDX_HIP_CHECK(hipMemcpyAsync(souts_pinned_, souts_.p, bytes, hipMemcpyDeviceToHost, s_));
+      CDX_HIP_CHECK(hipStreamSynchronize(s_));
+      std::memcpy(souts_host_.data(), souts_pinned_, bytes);
+    }
     counts_.n = total_selected_shapes();
     counts_.m = M_; counts_.a = A_; counts_.p = P_; counts_.c = C_;
     counts_.resolved = mode_ == NP_NATIVE ? 0 : hstat_->Q2;
@@ -742,6 +762,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     d.gx = sc.g[0]; d.gy = sc.g[1]; d.gz = sc.g[2];
     d.key_base = (uint32_t)sc.key_base;
     selection_dirty_ = true;
+    desc_dirty_ = true;
   }
 
   // Chunk table, per-scene chunk ranges and the step header for a set of
@@ -800,15 +821,51 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       desc_cap_ = std::max<uint64_t>(std::max<uint64_t>(desc_host_.size(), 1), desc_cap_ * 2);
       desc_.reserve(desc_cap_ * sizeof(SceneDesc<R>));
       graph_valid_ = false;
+      desc_dirty_ = true;
     }
-    if (!ch.empty()) CDX_HIP_CHECK(hipMemcpy(chunks_.p, ch.data(), ch.size() * sizeof(Chunk), hipMemcpyHostToDevice));
-    if (!sel.empty()) CDX_HIP_CHECK(hipMemcpy(sel_.p, sel.data(), sel.size() * 4, hipMemcpyHostToDevice));
-    CDX_HIP_CHECK(hipMemcpy(desc_.p, desc_host_.data(), desc_host_.size() * sizeof(SceneDesc<R>),
-                            hipMemcpyHostToDevice));
-    hhdr_->nslots = nslots_;
-    hhdr_->nsel = nsel_;
-    hhdr_->pad = 0;
-    CDX_HIP_CHECK(hipMemcpy(stephdr_.p, hhdr_, sizeof(StepHdr), hipMemcpyHostToDevice));
+    // Uploads through one pinned staging block, ordered on the scene stream:
+    // the step header, the selection, the chunk table, and the selected
+    // scenes' chunk ranges (applied to their descriptors by a small kernel;
+    // the full descriptor table is uploaded only after scenes change).
+    if (desc_dirty_) {
+      CDX_HIP_CHECK(hipMemcpyAsync(desc_.p, desc_host_.data(), desc_host_.size() * sizeof(SceneDesc<R>),
+                                   hipMemcpyHostToDevice, s_));
+      CDX_HIP_CHECK(hipStreamSynchronize(s_));  // desc_host_ is pageable
+      desc_dirty_ = false;
+    }
+    const size_t sel_bytes = ((nsel_ * 4 + 15) & ~size_t(15));
+    const size_t rng_bytes = ((nsel_ * 12 + 15) & ~size_t(15));
+    const size_t need = sizeof(StepHdr) + sel_bytes + rng_bytes + nchunks_ * sizeof(Chunk) + 64;
+    if (need > stage_host_bytes_) {
+      if (stage_host_) CDX_HIP_CHECK(hipHostFree(stage_host_));
+      stage_host_bytes_ = std::max<size_t>(need * 2, 4096);
+      CDX_HIP_CHECK(hipHostMalloc(&stage_host_, stage_host_bytes_, hipHostMallocDefault));
+    }
+    char* hs = static_cast<char*>(stage_host_);
+    StepHdr* shdr = reinterpret_cast<StepHdr*>(hs);
+    shdr->nslots = nslots_;
+    shdr->nsel = nsel_;
+    shdr->pad = 0;
+    uint32_t* ssel = reinterpret_cast<uint32_t*>(hs + sizeof(StepHdr));
+    uint32_t* srng = reinterpret_cast<uint32_t*>(hs + sizeof(StepHdr) + sel_bytes);
+    Chunk* sch = reinterpret_cast<Chunk*>(hs + sizeof(StepHdr) + sel_bytes + rng_bytes);
+    for (uint32_t j = 0; j < nsel_; ++j) {
+      ssel[j] = sel[j];
+      srng[3 * j] = sel[j];
+      srng[3 * j + 1] = desc_host_[sel[j]].chunk_begin;
+      srng[3 * j + 2] = desc_host_[sel[j]].chunk_end;
+    }
+    if (!ch.empty()) std::memcpy(sch, ch.data(), ch.size() * sizeof(Chunk));
+    selrng_.reserve(std::max<size_t>(rng_bytes, 16));
+    CDX_HIP_CHECK(hipMemcpyAsync(stephdr_.p, shdr, sizeof(StepHdr), hipMemcpyHostToDevice, s_));
+    if (nsel_) {
+      CDX_HIP_CHECK(hipMemcpyAsync(sel_.p, ssel, nsel_ * 4, hipMemcpyHostToDevice, s_));
+      CDX_HIP_CHECK(hipMemcpyAsync(selrng_.p, srng, nsel_ * 12, hipMemcpyHostToDevice, s_));
+      hipLaunchKernelGGL(k_apply_selection<R>, dim3(blocks_for(nsel_, 64)), dim3(64), 0, s_, (uint64_t)nsel_,
+                         (const uint32_t*)selrng_.as<uint32_t>(), desc_.as<SceneDesc<R>>());
+    }
+    if (!ch.empty())
+      CDX_HIP_CHECK(hipMemcpyAsync(chunks_.p, sch, ch.size() * sizeof(Chunk), hipMemcpyHostToDevice, s_));
     selection_dirty_ = false;
     if (key_bits_ != graph_key_bits_ || shape_bits_ != graph_shape_bits_) graph_valid_ = false;
   }
@@ -1201,7 +1258,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   int mode_ = NP_NATIVE;
   float tau_ = 1e-2f;
 
-  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, bscene_, meta_, scount_, soff_;
+  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, bscene_, meta_, scount_, soff_, selrng_;
+  void* stage_host_ = nullptr;
+  size_t stage_host_bytes_ = 0;
+  void* souts_pinned_ = nullptr;
+  size_t souts_pinned_bytes_ = 0;
+  bool desc_dirty_ = true;
   DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
   DevBuf chunks_, sel_, desc_, souts_;

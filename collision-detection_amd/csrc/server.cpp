@@ -25,9 +25,10 @@ namespace cdx {
 
 namespace {
 
-// Size classes by shape count (upper bounds; the last is open).
-constexpr uint64_t kClassHi[] = {4096, 32768, ~0ull};
-constexpr int kNumClasses = 3;
+// Size classes by shape count (upper bounds; the last is open).  Each class
+// steps on its own stream; four match the device's hardware queues.
+constexpr uint64_t kClassHi[] = {8192, 49152, 131072, ~0ull};
+constexpr int kNumClasses = 4;
 
 // Pinned host blocks for request inputs, bucketed by power-of-two size.
 class PinnedPool {
@@ -226,20 +227,82 @@ uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void*
     std::memcpy(base + pose_block_rot_offset(k, kind), rot, k * 4 * rs);
     req->poses = std::move(blk);
   }
+  return enqueue(std::move(req), ref.cls, t_enter);
+}
+
+uint64_t Server::enqueue(std::shared_ptr<Request> req, int cls, double t_enter) {
   uint64_t id;
   {
     std::lock_guard<std::mutex> lk(mu_);
     CDX_REQUIRE(pending_ < cfg_.max_pending, ERR_CAPACITY, "request queue is full");
     id = next_id_++;
     req->id = id;
-    req->t_submit = t_enter;  // latency includes the input copy
+    req->t_submit = t_enter;  // latency includes the input copy, if any
     reqs_[id] = req;
-    classes_[ref.cls]->queue.push_back(req);
+    classes_[cls]->queue.push_back(req);
     ++pending_;
     ++st_sub_;
   }
   cv_work_.notify_all();
   return id;
+}
+
+uint64_t Server::pose_buffer(uint32_t s, uint64_t k, void** pos, void** rot) {
+  SceneRef ref;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    CDX_REQUIRE(s < scenes_.size(), ERR_INVALID, "scene index out of range");
+    ref = scenes_[s];
+  }
+  CDX_REQUIRE(k > 0 && k <= ref.nb, ERR_INVALID, "pose buffer size must be in 1..bodies of the scene");
+  CDX_REQUIRE(pos && rot, ERR_INVALID, "null output");
+  const int kind = cfg_.real_kind;
+  auto req = std::make_shared<Request>();
+  req->scene = s;
+  req->local = ref.local;
+  req->n = ref.n;
+  req->body_base = ref.body_begin;
+  req->k = k;
+  req->contiguous = true;
+  auto blk = std::make_shared<PoseBlock>();
+  blk->p = pinned_pool().get(pose_block_bytes(k, kind), &blk->bucket);
+  char* base = static_cast<char*>(blk->p);
+  *pos = base + pose_block_pos_offset(k);
+  *rot = base + pose_block_rot_offset(k, kind);
+  req->poses = std::move(blk);
+  std::lock_guard<std::mutex> lk(mu_);
+  const uint64_t h = next_buffer_++;
+  buffers_[h] = std::move(req);
+  return h;
+}
+
+uint64_t Server::submit_buffer(uint64_t h) {
+  const double t_enter = now_ms();
+  std::shared_ptr<Request> req;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = buffers_.find(h);
+    CDX_REQUIRE(it != buffers_.end(), ERR_INVALID, "unknown pose buffer");
+    CDX_REQUIRE(pending_ < cfg_.max_pending, ERR_CAPACITY, "request queue is full");
+    req = std::move(it->second);
+    buffers_.erase(it);
+  }
+  const int cls = scenes_cls(req->scene);
+  return enqueue(std::move(req), cls, t_enter);
+}
+
+void Server::drop_buffer(uint64_t h) {
+  std::shared_ptr<Request> req;
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = buffers_.find(h);
+  CDX_REQUIRE(it != buffers_.end(), ERR_INVALID, "unknown pose buffer");
+  req = std::move(it->second);
+  buffers_.erase(it);
+}
+
+int Server::scenes_cls(uint32_t s) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  return scenes_[s].cls;
 }
 
 void Server::worker_loop(int c) {
@@ -284,16 +347,21 @@ void Server::worker_loop(int c) {
 std::shared_ptr<ResultBlock> Server::get_block(Class& k, uint64_t contacts) {
   for (auto& b : k.blocks)
     if (b.use_count() == 1 && b->cap >= contacts) return b;
-  // drop one idle block that is too small before adding a larger one
+  // Replace an idle block that is too small, or add one: capacity for twice
+  // the largest step seen, so growth stops after the first large steps.
   for (auto it = k.blocks.begin(); it != k.blocks.end(); ++it)
     if (it->use_count() == 1) {
       k.blocks.erase(it);
       break;
     }
+  k.max_contacts = std::max(k.max_contacts, contacts);
   auto b = std::make_shared<ResultBlock>();
-  const uint64_t cap = std::max<uint64_t>(contacts + contacts / 4, 4096);
+  const uint64_t cap = std::max<uint64_t>(2 * k.max_contacts, 1 << 16);
   const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
-  CDX_HIP_CHECK(hipHostMalloc(&b->base, cap * (10 * rs + 8), hipHostMallocDefault));
+  {
+    GateShared g;
+    CDX_HIP_CHECK(hipHostMalloc(&b->base, cap * (10 * rs + 8), hipHostMallocDefault));
+  }
   b->cap = cap;
   k.blocks.push_back(b);
   return b;
@@ -303,27 +371,36 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
   Class& k = *classes_[c];
   std::string err;
   double step_ms = 0.0;
+  double ph[6] = {0, 0, 0, 0, 0, 0};
   try {
     std::lock_guard<std::mutex> elk(k.eng_mu);
     GateShared gate;
     BatchBase& e = *k.eng;
     std::vector<uint32_t> sel;
     sel.reserve(batch.size());
+    double t0 = now_ms();
     for (auto& r : batch) {
       if (r->k) e.update_bodies_pinned(r->k, r->poses->p, r->contiguous ? (int64_t)r->body_base : -1);
       sel.push_back(r->local);
     }
+    double t1 = now_ms();
     e.select(sel.data(), sel.size());
+    double t2 = now_ms();
     e.step(cfg_.use_graph);
+    double t3 = now_ms();
     e.finish();
+    double t4 = now_ms();
     step_ms = e.last_step_ms();
     const uint64_t C = e.totals().c;
     auto blk = get_block(k, C);
+    double t5 = now_ms();
     const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
     char* base = static_cast<char*>(blk->base);
     const uint64_t cap = blk->cap;
     e.read_all_contacts(base, base + 3 * rs * cap, base + 6 * rs * cap, base + 9 * rs * cap,
                         reinterpret_cast<int32_t*>(base + 10 * rs * cap));
+    double t6 = now_ms();
+    ph[0] = t1 - t0; ph[1] = t2 - t1; ph[2] = t3 - t2; ph[3] = t4 - t3; ph[4] = t5 - t4; ph[5] = t6 - t5;
     for (uint64_t j = 0; j < batch.size(); ++j) {
       uint64_t b0, b1;
       e.scene_contact_range(j, &b0, &b1);
@@ -354,6 +431,7 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
     ++st_batches_;
     st_breqs_ += batch.size();
     st_busy_ += step_ms;
+    for (int q = 0; q < 6; ++q) st_phase_[q] += ph[q];
   }
   cv_done_.notify_all();
 }
@@ -434,6 +512,7 @@ ServerStats Server::stats() {
   s.batches = st_batches_;
   s.batched_requests = st_breqs_;
   s.busy_ms = st_busy_;
+  for (int q = 0; q < 6; ++q) s.host_ms[q] = st_phase_[q];
   s.window_s = (now_ms() - st_t0_) / 1000.0;
   if (!lat_.empty()) {
     std::vector<double> v = lat_;
@@ -457,6 +536,7 @@ void Server::reset_stats() {
   lat_.clear();
   st_sub_ = st_done_ = st_fail_ = st_batches_ = st_breqs_ = 0;
   st_busy_ = 0.0;
+  for (int q = 0; q < 6; ++q) st_phase_[q] = 0.0;
   st_t0_ = now_ms();
 }
 

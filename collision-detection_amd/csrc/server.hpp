@@ -39,6 +39,9 @@ struct ServerStats {
   double p50_ms, p99_ms, mean_ms, max_ms;  // submit -> result ready, over the window
   double busy_ms;                          // summed device step time of the window
   double window_s;                         // wall time since the window started
+  // host time of the workers' step phases, summed: pose uploads, selection,
+  // launch, wait for the step, result block, contact readback
+  double host_ms[6];
 };
 
 struct PoseBlock;
@@ -67,6 +70,12 @@ class Server {
   // bodies 0..k-1).  The inputs are copied before return.  Returns the
   // request id.
   uint64_t submit(uint32_t scene, uint64_t k, const uint32_t* idx, const void* pos, const void* rot);
+  // Zero-copy submission: a pinned block for the poses of bodies 0..k-1 of
+  // a scene, filled by the caller in place (k real3 positions, k real4
+  // rotations), then queued by submit_buffer (which takes it over).
+  uint64_t pose_buffer(uint32_t scene, uint64_t k, void** pos, void** rot);
+  uint64_t submit_buffer(uint64_t handle);
+  void drop_buffer(uint64_t handle);
   // 1 when the result is ready, 0 while queued or running; throws on a
   // failed request.
   int poll(uint64_t req);
@@ -106,6 +115,7 @@ class Server {
     std::deque<std::shared_ptr<Request>> queue;
     std::thread worker;
     std::vector<std::shared_ptr<ResultBlock>> blocks;  // pool, used by the worker only
+    uint64_t max_contacts = 0;
   };
   struct SceneRef {
     int cls;
@@ -114,10 +124,12 @@ class Server {
     int bins[3];
   };
 
+  uint64_t enqueue(std::shared_ptr<Request> req, int cls, double t_enter);
   void worker_loop(int c);
   void run_batch(int c, std::vector<std::shared_ptr<Request>>& batch);
   std::shared_ptr<ResultBlock> get_block(Class& k, uint64_t contacts);
   int class_of(uint64_t n) const;
+  int scenes_cls(uint32_t s) const;
   static double now_ms();
 
   int dev_;
@@ -134,10 +146,13 @@ class Server {
   uint64_t next_id_ = 1;
   uint64_t pending_ = 0;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs_;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> buffers_;  // filled by the caller, not yet queued
+  uint64_t next_buffer_ = 1;
   // window statistics
   std::vector<double> lat_;
   uint64_t st_sub_ = 0, st_done_ = 0, st_fail_ = 0, st_batches_ = 0, st_breqs_ = 0;
   double st_busy_ = 0.0, st_t0_ = 0.0;
+  double st_phase_[6] = {0, 0, 0, 0, 0, 0};
 };
 
 }  // namespace cdx

@@ -239,7 +239,8 @@ int cdx_batch_scene_bytes(cdx_batch* batch, uint64_t n, uint64_t nb, uint64_t* b
 
 /* ---- served request loop ------------------------------------------------
  * No reference counterpart.  A server keeps scenes resident in HBM (one
- * batch per size class: < 4096, < 32768 and >= 32768 shapes), admits a new
+ * batch per size class: < 8192, < 49152, < 131072 and >= 131072 shapes),
+ * admits a new
  * scene only if its estimated footprint fits the HBM budget, and runs one
  * worker thread per size class that keeps stepping: each step takes the
  * oldest queued requests of its class (at most one per scene, up to
@@ -271,6 +272,7 @@ typedef struct cdx_server_stats_t {
   double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result ready */
   double busy_ms;                         /* summed device time of the steps */
   double window_s;                        /* wall time of the statistics window */
+  double host_ms[6]; /* workers' host time per phase: pose upload, selection, launch, wait, result block, readback */
 } cdx_server_stats_t;
 
 int cdx_server_create(cdx_ctx* ctx, const cdx_server_config* cfg, cdx_server** out);
@@ -290,6 +292,15 @@ int cdx_server_scene_info(cdx_server* server, uint32_t scene, uint64_t* n_shapes
  * CDX_ERR_CAPACITY when max_pending requests are queued. */
 int cdx_server_submit(cdx_server* server, uint32_t scene, uint64_t k, const uint32_t* idx, const void* body_pos,
                       const void* body_rot, uint64_t* request_out);
+/* Zero-copy submission: a pinned block for the poses of bodies 0..k-1 of
+ * one scene (k real3 positions at *body_pos, k real4 rotations at
+ * *body_rot) that the caller fills in place; cdx_server_submit_buffer
+ * queues it (the server takes the block over), cdx_server_drop_buffer
+ * discards it unsubmitted. */
+int cdx_server_pose_buffer(cdx_server* server, uint32_t scene, uint64_t k, void** body_pos, void** body_rot,
+                           uint64_t* handle);
+int cdx_server_submit_buffer(cdx_server* server, uint64_t handle, uint64_t* request_out);
+int cdx_server_drop_buffer(cdx_server* server, uint64_t handle);
 int cdx_server_poll(cdx_server* server, uint64_t request, int* done);
 /* timeout_ms < 0 waits without limit; *done = 0 on timeout. */
 int cdx_server_wait(cdx_server* server, uint64_t request, double timeout_ms, int* done);

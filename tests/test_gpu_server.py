@@ -12,8 +12,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# scenes in all three size classes (< 4096, < 32768, >= 32768 shapes)
-SIZES = [600, 2500, 3900, 5000, 12000, 30000, 40000, 1200, 9000, 70000]
+# scenes in all four size classes (< 8192, < 49152, < 131072, >= 131072 shapes)
+SIZES = [600, 2500, 3900, 5000, 12000, 30000, 60000, 1200, 9000, 140000]
 
 
 def _single_contacts(cdx, ctx, s, mode):
@@ -179,3 +179,23 @@ def test_server_errors(cdx, ctx, scenes, served):
         srv.submit(ids[0], ss[0].pos[:2], ss[0].rot[:2], idx=np.array([0, ss[0].nb], np.uint32))
     with pytest.raises(cdx.CdxError):
         srv.result(123456789)
+
+
+def test_pose_buffer_submission(cdx, ctx, scenes, served):
+    srv, ss, ids = served
+    rng = np.random.default_rng(14)
+    j = 2
+    s = ss[j]
+    p, q = scenes.pose_step(s, rng)
+    h, pos, rot = srv.pose_buffer(ids[j], s.nb)
+    pos[:] = p
+    rot[:] = q
+    rid = srv.submit_buffer(h)
+    assert srv.wait(rid, 30_000)
+    s.pos[:], s.rot[:] = p, q
+    _equal(srv.read_contacts(rid), _single_contacts(cdx, ctx, s, cdx.NP_MIXED))
+    srv.release(rid)
+    h2, _, _ = srv.pose_buffer(ids[j], 10)
+    srv.drop_buffer(h2)
+    with pytest.raises(cdx.CdxError):
+        srv.submit_buffer(h2)
