@@ -381,10 +381,15 @@ constexpr int PAIR_AHEAD = 64;
 constexpr int PAIR_W = PAIR_BS + PAIR_AHEAD;
 constexpr uint64_t MASK_TAIL = 1ull << 63;
 
+// Per entry with pairs: the hit mask, the entry's sorted position, and the
+// partners themselves when there are at most META_IDS of them (the emit
+// pass then reads no sorted entries; in any order, groups are sorted after).
+constexpr int META_IDS = 4;
 struct alignas(16) PairMeta {
   uint64_t mask;
   uint32_t ipos;  // sorted position of the entry
-  uint32_t pad;
+  uint32_t n;     // pairs of the entry
+  uint32_t ids[META_IDS];
 };
 
 template <class R>
@@ -441,6 +446,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   __shared__ uint32_t sstart[PAIR_BS + 1];
   __shared__ uint64_t smask[PAIR_BS];
   __shared__ uint32_t scnt[PAIR_BS];
+  __shared__ uint32_t sids[PAIR_BS][META_IDS];
   __shared__ uint32_t scan_scratch[PAIR_BS / WAVE + 1];
   const uint64_t m = cap_count(mcap, &st->M);
   const uint64_t b0 = (uint64_t)blockIdx.x * PAIR_BS;
@@ -504,9 +510,18 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
       uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
       uint64_t mk = 0;
       uint32_t cnt = 0;
+      uint32_t loc[META_IDS];
+      auto flush = [&]() {
+        if (!cnt) return;
+        atomicOr((unsigned long long*)&smask[e], (unsigned long long)mk);
+        const uint32_t at = atomicAdd(&scnt[e], cnt);
+#pragma unroll
+        for (int q = 0; q < META_IDS; ++q)
+          if ((uint32_t)q < cnt && at + q < (uint32_t)META_IDS) sids[e][at + q] = loc[q];
+      };
       for (; f < fend; ++f) {
         if (f >= eend) {
-          if (cnt) { atomicOr((unsigned long long*)&smask[e], (unsigned long long)mk); atomicAdd(&scnt[e], cnt); }
+          flush();
           mk = 0;
           cnt = 0;
           do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
@@ -518,12 +533,15 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
         if (b == prev) continue;
         prev = b;
         if (pair_test(c.r, c.g0, srec[k], c.inv, c.K, c.kb, c.gx, c.gy)) {
+#pragma unroll
+          for (int q = 0; q < META_IDS; ++q)
+            if ((uint32_t)q == cnt) loc[q] = b;
           ++cnt;
           const int dd = k - e - 1;
           mk |= dd < 63 ? (1ull << dd) : MASK_TAIL;
         }
       }
-      if (cnt) { atomicOr((unsigned long long*)&smask[e], (unsigned long long)mk); atomicAdd(&scnt[e], cnt); }
+      flush();
     }
   }
   __syncthreads();
@@ -560,7 +578,9 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   PairMeta pm;
   pm.mask = mk;
   pm.ipos = (uint32_t)i;
-  pm.pad = 0;
+  pm.n = cnt;
+#pragma unroll
+  for (int q = 0; q < META_IDS; ++q) pm.ids[q] = sids[tid][q];
   meta[j] = pm;
 }
 
@@ -637,6 +657,12 @@ __device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_
   for (uint64_t e = e0; e < e1; ++e) {
     if (!pcount[e]) continue;
     const PairMeta pm = meta[e];
+    if (pm.n <= (uint32_t)META_IDS && !(pm.mask & MASK_TAIL)) {
+#pragma unroll
+      for (int q = 0; q < META_IDS; ++q)
+        if ((uint32_t)q < pm.n) put(pm.ids[q]);
+      continue;
+    }
     const uint64_t ip = pm.ipos;
     uint64_t bits = pm.mask & ~MASK_TAIL;
     while (bits) {
