@@ -192,8 +192,8 @@ template <class R>
 __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                  uint64_t NS, const R* __restrict__ aabb,
                                                  const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ counts,
-                                                 BpRec<R>* __restrict__ brec, uint32_t* __restrict__ bscene,
-                                                 uint32_t* __restrict__ kpart, DevStatus* st) {
+                                                 BpRec<R>* __restrict__ brec, uint32_t* __restrict__ kpart,
+                                                 DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   __shared__ uint32_t skm[CHUNK / WAVE];
   const Chunk ch = chunks[blockIdx.x];
@@ -212,8 +212,8 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
     Cells c;
     c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
     c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
+    r.scene = ch.scene;
     brec[i] = r;
-    bscene[i] = ch.scene;
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
       counts[slot] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = d.key_base + hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, d.gx, d.gy);
@@ -416,8 +416,8 @@ template <class R> struct EntryCtx {
 
 template <class R>
 __device__ __forceinline__ void entry_ctx(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r,
-                                          const uint32_t* __restrict__ bscene, const SceneDesc<R>* __restrict__ desc) {
-  const SceneDesc<R>& d = desc[bscene[a]];
+                                          const SceneDesc<R>* __restrict__ desc) {
+  const SceneDesc<R>& d = desc[r.scene];
   c.r = r;
   c.K = K;
   c.a = a;
@@ -436,7 +436,6 @@ __device__ __forceinline__ void entry_ctx(EntryCtx<R>& c, uint32_t K, uint32_t a
 template <class R>
 __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t* __restrict__ keys,
                                                    const BpRec<R>* __restrict__ brec,
-                                                   const uint32_t* __restrict__ bscene,
                                                    const SceneDesc<R>* __restrict__ desc,
                                                    const uint32_t* __restrict__ offs, uint32_t* __restrict__ pcount,
                                                    PairMeta* __restrict__ meta, DevStatus* st) {
@@ -506,7 +505,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
       int e = lo;
       uint32_t ebeg = sstart[e], eend = sstart[e + 1];
       EntryCtx<R> c;
-      entry_ctx(c, skey[e], sbody[e], srec[e], bscene, desc);
+      entry_ctx(c, skey[e], sbody[e], srec[e], desc);
       uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
       uint64_t mk = 0;
       uint32_t cnt = 0;
@@ -525,7 +524,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
           mk = 0;
           cnt = 0;
           do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
-          entry_ctx(c, skey[e], sbody[e], srec[e], bscene, desc);
+          entry_ctx(c, skey[e], sbody[e], srec[e], desc);
           prev = c.a;
         }
         const int k = e + 1 + (int)(f - ebeg);
@@ -549,7 +548,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   uint64_t mk = smask[tid];
   uint32_t cnt = scnt[tid];
   EntryCtx<R> c;
-  entry_ctx(c, skey[tid], sbody[tid], srec[tid], bscene, desc);
+  entry_ctx(c, skey[tid], sbody[tid], srec[tid], desc);
   if (beyond) {  // run continues past the window: the rest from global memory (rare)
     uint32_t prev = sbody[nloc - 1];
     for (uint64_t k = b0 + nloc; k < m; ++k) {
@@ -652,8 +651,7 @@ __device__ __forceinline__ void sort_lds_group(uint32_t* __restrict__ v, uint32_
 template <class R, class Put>
 __device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_t a, const uint32_t* __restrict__ pcount,
                                                const PairMeta* __restrict__ meta, const uint64_t* __restrict__ keys,
-                                               const BpRec<R>* __restrict__ brec, const uint32_t* __restrict__ bscene,
-                                               const SceneDesc<R>* __restrict__ desc, uint64_t m, Put put) {
+                                               const BpRec<R>* __restrict__ brec, const SceneDesc<R>* __restrict__ desc, uint64_t m, Put put) {
   for (uint64_t e = e0; e < e1; ++e) {
     if (!pcount[e]) continue;
     const PairMeta pm = meta[e];
@@ -673,7 +671,7 @@ __device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_
     if (pm.mask & MASK_TAIL) {
       const uint64_t ki = keys[ip];
       EntryCtx<R> c;
-      entry_ctx(c, (uint32_t)(ki >> 32), a, brec[a], bscene, desc);
+      entry_ctx(c, (uint32_t)(ki >> 32), a, brec[a], desc);
       uint32_t prev = (uint32_t)keys[ip + 63];
       for (uint64_t k = ip + 64; k < m; ++k) {
         const uint64_t kk = keys[k];
@@ -703,7 +701,6 @@ __global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ c
                                                      const uint32_t* __restrict__ scount,
                                                      const uint32_t* __restrict__ soff, const uint64_t* __restrict__ keys,
                                                      const BpRec<R>* __restrict__ brec,
-                                                     const uint32_t* __restrict__ bscene,
                                                      const SceneDesc<R>* __restrict__ desc, uint64_t mcap,
                                                      uint64_t pcap, uint64_t* __restrict__ pairs,
                                                      uint32_t* __restrict__ longs, uint64_t long_cap, DevStatus* st) {
@@ -735,7 +732,7 @@ __global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ c
     if (staged) {
       uint32_t* v = sb + (beg - p0);
       uint32_t q = 0;
-      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, bscene, desc, m, [&](uint32_t b) { v[q++] = b; });
+      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, desc, m, [&](uint32_t b) { v[q++] = b; });
       for (uint32_t t = 0; t < tot; ++t) sa[beg - p0 + t] = (uint8_t)threadIdx.x;
       if (tot <= 4) sort_lds_group<4>(v, tot);
       else if (tot <= 8) sort_lds_group<8>(v, tot);
@@ -747,7 +744,7 @@ __global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ c
         else atomicOr(&st->overflow, 4u);
       }
     } else {
-      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, bscene, desc, m,
+      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, desc, m,
                         [&](uint32_t b) { pairs[o++] = ((uint64_t)a << 32) | b; });
       if (tot <= 4) sort_pairs_in_place<4>(pairs, beg, tot, a);
       else if (tot <= 8) sort_pairs_in_place<8>(pairs, beg, tot, a);

@@ -395,7 +395,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_,
-                           &scan_tmp_, &radix_tmp_, &chunks_, &sel_, &desc_, &souts_, &longs_, &bscene_, &meta_, &scount_, &soff_};
+                           &scan_tmp_, &radix_tmp_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -620,7 +620,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   uint64_t resident_bytes_estimate(uint64_t n, uint64_t nb) const override {
     // inputs in SoA plus the per-shape records each step rewrites
-    const uint64_t per_shape = 1 + 4 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
+    const uint64_t per_shape = 1 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
                                sizeof(ShapeHdr) + sizeof(ShapeRecF);
     const uint64_t per_body = 7 * sizeof(R) + 1;
     return n * per_shape + nb * per_body;
@@ -682,7 +682,6 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     aabb_current_ = false;
     rec_.reserve(nc * sizeof(ShapeRec<R>));
     brec_.reserve(nc * sizeof(BpRec<R>));
-    bscene_.reserve(nc * 4);
     hdr_.reserve(nc * sizeof(ShapeHdr));
     frec_.reserve(nc * sizeof(ShapeRecF));
     NSC_ = nc;
@@ -914,7 +913,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(), brec_.as<BpRec<R>>(),
-                       bscene_.as<uint32_t>(), kpart_.as<uint32_t>(), st());
+                       kpart_.as<uint32_t>(), st());
     hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
@@ -956,7 +955,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_BP_PAIRS, true);
     hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const uint32_t*)bscene_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
                        (const uint32_t*)offs_.as<uint32_t>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(), st());
     mark(ST_BP_PAIRS, false);
     mark(ST_BP_PAIRSCAN, true);
@@ -984,7 +983,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        (const uint32_t*)pcount_.as<uint32_t>(), (const PairMeta*)meta_.as<PairMeta>(),
                        (const uint32_t*)scount_.as<uint32_t>(), (const uint32_t*)soff_.as<uint32_t>(),
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const uint32_t*)bscene_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), mcap_,
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), mcap_,
                        pcap_, pairs_.as<uint64_t>(), longs_.as<uint32_t>(), long_cap, st());
     mark(ST_BP_PAIREMIT, false);
     mark(ST_BP_PAIRSORT, true);
@@ -1258,7 +1257,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   int mode_ = NP_NATIVE;
   float tau_ = 1e-2f;
 
-  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, bscene_, meta_, scount_, soff_, selrng_;
+  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, meta_, scount_, soff_, selrng_;
   void* stage_host_ = nullptr;
   size_t stage_host_bytes_ = 0;
   void* souts_pinned_ = nullptr;
