@@ -6,7 +6,11 @@ Workload: 1,048,576 mixed convex bodies, FP64 input (40 % sphere, 20 % box,
 uniform random rotations; centres uniform in a cube sized for ~4 AABB
 neighbours per body).  A step is one full pass (AABB generation, uniform-grid
 broadphase, MPR narrowphase, contact compaction) with the scene resident in
-HBM; the step ends when the contact count is known on the host.
+HBM; the step ends when the contact count is known on the host.  `value` is
+the step recorded once as a HIP graph and replayed (the scene as a one-scene
+batch); `scene_path` is the same step through the stage-by-stage scene API,
+which reads totals back between stages as the reference does, and provides
+the per-stage HIP-event times.
 
 Prints ONE JSON line.  The CPU baseline is the oracle (our C++/OpenMP
 restatement of the reference's shipped OpenMP path) timed on a bounded
@@ -562,6 +566,31 @@ def main():
     stage_avg = {k: v / args.steps for k, v in stage_tot.items()}
     log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage_avg.items()))
 
+    # The same step as a one-scene batch: buffers sized by capacity, the
+    # launch sequence recorded once as a HIP graph and replayed, totals read
+    # back once per step (no host round trips between stages).
+    b1 = pkg.Batch(ctx, s.dtype)
+    b1.add_scene(s)
+    b1.set_mode(mode, 1e-2)
+    for _ in range(max(args.warmup, 2)):
+        b1.step(True)
+        gc = b1.finish()
+    glat = []
+    g0 = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        b1.step(True)
+        gc = b1.finish()
+        glat.append(time.perf_counter() - t0)
+    gtotal = time.perf_counter() - g0
+    assert gc.pairs == c.pairs and gc.contacts == c.contacts, "graph step disagrees with the scene step"
+    graph_path = {"ms_per_step": round(1000.0 * gtotal / args.steps, 4),
+                  "bodies_per_s": round(s.n * args.steps / gtotal, 1),
+                  "p50_step_ms": round(float(np.percentile(np.array(glat) * 1e3, 50)), 4),
+                  "p99_step_ms": round(float(np.percentile(np.array(glat) * 1e3, 99)), 4)}
+    log(f"graph path: {graph_path}")
+    b1.close()
+
     # Roofline of the slowest single-kernel HBM-bound stage (HIP events on
     # the scene stream around that one launch).  The FP64 MPR re-solve is
     # VALU/latency-bound and reported with the stage times instead.
@@ -608,14 +637,17 @@ def main():
         cpu = cpu_baseline(oracle, s)
 
     lat_ms = np.array(lat) * 1000.0
+    scene_path = {"ms_per_step": round(ms_step, 4), "bodies_per_s": round(s.n * args.steps / total, 1),
+                  "p50_step_ms": round(float(np.percentile(lat_ms, 50)), 4),
+                  "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 4)}
     out = {
         "metric": "bodies/sec AABB+broad+MPR at 1M mixed shapes; scenes/sec + p50/p99 step lat",
-        "value": round(s.n * args.steps / total, 1),
+        "value": graph_path["bodies_per_s"],
         "unit": "bodies/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": graph_path["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -624,9 +656,10 @@ def main():
         "config": {"workload": "mixed-1M: 1,048,576 mixed convex bodies, FP64 input, ~4 AABB neighbours/body",
                    "n_bodies": s.n, "bins": list(s.grid), "narrowphase": args.mode,
                    "pairs": int(c.pairs), "contacts": int(c.contacts), "fp64_resolved": int(c.resolved)},
-        "p50_step_ms": round(float(np.percentile(lat_ms, 50)), 4),
-        "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 4),
+        "p50_step_ms": graph_path["p50_step_ms"],
+        "p99_step_ms": graph_path["p99_step_ms"],
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "scene_path": scene_path,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "lattice_16M_broadphase": lattice,

@@ -102,6 +102,9 @@ class BatchBase {
   virtual uint64_t selected_count() const = 0;
   // Bin entries and pairs per selected shape the step buffers are sized for.
   virtual void set_capacity_factors(double m_per_shape, double p_per_shape) = 0;
+  // Sizes every step buffer for steps of up to `shapes` shapes in `scenes`
+  // scenes now, so that later steps neither allocate nor re-record.
+  virtual void reserve_step(uint64_t shapes, uint64_t scenes) = 0;
   virtual void step(bool use_graph) = 0;  // enqueue AABB + broadphase + narrowphase
   virtual void finish() = 0;              // wait, re-run on capacity overflow, read totals
   virtual const StepCounts& totals() const = 0;

@@ -618,6 +618,18 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     sync();
   }
 
+  void reserve_step(uint64_t shapes, uint64_t scenes) override {
+    sync_internal();
+    const uint64_t need_m = (uint64_t)(cap_m_per_shape_ * (double)shapes) + 4096;
+    const uint64_t need_p = (uint64_t)(cap_p_per_shape_ * (double)shapes) + 4096;
+    if (need_m > mcap_) { mcap_ = need_m; graph_valid_ = false; }
+    if (need_p > pcap_) { pcap_ = need_p; graph_valid_ = false; }
+    const uint64_t chunks = shapes / CHUNK + scenes + 1;
+    if (chunks > chunk_cap_) grow_chunk_cap(chunks);
+    if (scenes > sel_cap_) grow_sel_cap(scenes);
+    prepare_capacity_buffers();
+  }
+
   uint64_t resident_bytes_estimate(uint64_t n, uint64_t nb) const override {
     // inputs in SoA plus the per-shape records each step rewrites
     const uint64_t per_shape = 1 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
@@ -799,23 +811,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
     key_bits_ = bits_for(ks_total ? ks_total - 1 : 0);
     shape_bits_ = bits_for(nshapes ? nshapes - 1 : 0);
-    if (nchunks_ > chunk_cap_ || !chunks_.p) {
-      chunk_cap_ = std::max<uint64_t>(std::max<uint64_t>(nchunks_, 1), chunk_cap_ * 2);
-      chunks_.reserve(chunk_cap_ * sizeof(Chunk));
-      bpart_.reserve(chunk_cap_ * 6 * 4);
-      kpart_.reserve(chunk_cap_ * 4);
-      counts_buf_.reserve(chunk_cap_ * CHUNK * 4);
-      offs_.reserve(chunk_cap_ * CHUNK * 4);
-      scount_.reserve(chunk_cap_ * CHUNK * 4);
-      soff_.reserve(chunk_cap_ * CHUNK * 4);
-      graph_valid_ = false;
-    }
-    if (nsel_ > sel_cap_ || !sel_.p) {
-      sel_cap_ = std::max<uint64_t>(std::max<uint64_t>(nsel_, 1), sel_cap_ * 2);
-      sel_.reserve(sel_cap_ * 4);
-      souts_.reserve(sel_cap_ * sizeof(SceneOut));
-      graph_valid_ = false;
-    }
+    if (nchunks_ > chunk_cap_ || !chunks_.p)
+      grow_chunk_cap(std::max<uint64_t>(std::max<uint64_t>(nchunks_, 1), chunk_cap_ * 2));
+    if (nsel_ > sel_cap_ || !sel_.p) grow_sel_cap(std::max<uint64_t>(std::max<uint64_t>(nsel_, 1), sel_cap_ * 2));
     if (desc_host_.size() > desc_cap_ || !desc_.p) {
       desc_cap_ = std::max<uint64_t>(std::max<uint64_t>(desc_host_.size(), 1), desc_cap_ * 2);
       desc_.reserve(desc_cap_ * sizeof(SceneDesc<R>));
@@ -1102,6 +1100,24 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   // Capacities only grow (a recorded step stays valid while they hold):
   // entries and pairs per shape from the capacity factors, or from the
   // largest totals measured so far.
+  void grow_chunk_cap(uint64_t cap) {
+    chunk_cap_ = cap;
+    chunks_.reserve(chunk_cap_ * sizeof(Chunk));
+    bpart_.reserve(chunk_cap_ * 6 * 4);
+    kpart_.reserve(chunk_cap_ * 4);
+    counts_buf_.reserve(chunk_cap_ * CHUNK * 4);
+    offs_.reserve(chunk_cap_ * CHUNK * 4);
+    scount_.reserve(chunk_cap_ * CHUNK * 4);
+    soff_.reserve(chunk_cap_ * CHUNK * 4);
+    graph_valid_ = false;
+  }
+  void grow_sel_cap(uint64_t cap) {
+    sel_cap_ = cap;
+    sel_.reserve(sel_cap_ * 4);
+    souts_.reserve(sel_cap_ * sizeof(SceneOut));
+    graph_valid_ = false;
+  }
+
   void ensure_capacities() {
     const uint64_t n = total_selected_shapes();
     uint64_t need_m = std::max<uint64_t>((uint64_t)(cap_m_per_shape_ * n) + 4096, m_seen_ + m_seen_ / 4);

@@ -163,6 +163,10 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
     }
   }
   const uint32_t local = k.eng->add_scene(n, T, A, B, R, ID, nb, pos, rot, active, g[0], g[1], g[2]);
+  // step buffers for the largest step this class can form, sized now
+  const uint64_t nsc = k.eng->scene_count();
+  const uint64_t per_step = std::min<uint64_t>(nsc, std::max<uint64_t>(1, cfg_.max_batch_shapes / std::max<uint64_t>(k.lo, 1)));
+  k.eng->reserve_step(std::max<uint64_t>(cfg_.max_batch_shapes, n), per_step);
   std::lock_guard<std::mutex> lk(mu_);
   committed_ += need;
   SceneRef r;
