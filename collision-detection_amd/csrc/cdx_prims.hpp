@@ -1,6 +1,6 @@
 // Data-parallel primitives for gfx950 (64-wide wavefronts): wavefront scans
 // built from DPP row shifts / row broadcasts, an LDS-staged block scan, a
-// reduce-then-scan device scan, and a stable LSD radix sort of 64-bit values.
+// reduce-then-scan device scan (two kernels up to 2048 tiles of 4096), and a stable LSD radix sort of 64-bit values.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>

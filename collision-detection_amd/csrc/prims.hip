@@ -17,13 +17,13 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
-// Per-tile sums (64-bit, so a tile of large counts cannot wrap).
 __device__ __forceinline__ uint64_t eff_n(uint64_t cap, const uint64_t* d_n) {
   if (!d_n) return cap;
   uint64_t v = *d_n;
   return v < cap ? v : cap;
 }
 
+// Per-tile sums (64-bit, so a tile of large counts cannot wrap).
 __global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t cap,
                                                          const uint64_t* __restrict__ d_n, uint64_t* __restrict__ part) {
   __shared__ uint64_t ws[SCAN_BS / WAVE];
@@ -102,6 +102,54 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const uint32_t* __restri
     if (e < n) out[e] = run;
     run += v[i];
   }
+}
+
+// Tile scan that also forms its own tile offset: the sum of the preceding
+// tile sums, read by the whole workgroup (a few KB from L2 while the tile
+// count is small), so no separate kernel scans the tile sums.  The last
+// tile writes the grand total.
+constexpr uint64_t SCAN_DIRECT_TILES = 2048;
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* __restrict__ in,
+                                                               uint32_t* __restrict__ out, uint64_t cap,
+                                                               const uint64_t* __restrict__ d_n,
+                                                               const uint64_t* __restrict__ part,
+                                                               uint64_t* __restrict__ d_total) {
+  __shared__ uint32_t scratch[SCAN_BS / WAVE + 1];
+  __shared__ uint64_t ws[SCAN_BS / WAVE];
+  const uint64_t n = eff_n(cap, d_n);
+  const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (n == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_total = 0;
+    return;
+  }
+  if (blockIdx.x >= nb) return;
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
+  uint32_t v[SCAN_IPT];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t e = base + i;
+    v[i] = e < n ? in[e] : 0u;
+    s += v[i];
+  }
+  uint64_t pre_tiles = 0;
+  for (uint64_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BS) pre_tiles += part[i];
+  pre_tiles = wave_sum_u64(pre_tiles);
+  if (threadIdx.x % WAVE == 0) ws[threadIdx.x / WAVE] = pre_tiles;
+  uint32_t tot;
+  uint32_t pre = block_excl_scan<SCAN_BS>(s, scratch, &tot);  // synchronises: ws is complete
+  uint64_t offset = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_BS / WAVE; ++w) offset += ws[w];
+  uint32_t run = (uint32_t)offset + pre;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t e = base + i;
+    if (e < n) out[e] = run;
+    run += v[i];
+  }
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) *d_total = offset + part[blockIdx.x];
 }
 
 __global__ void k_copy_u64(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, uint64_t cap,
@@ -230,6 +278,11 @@ void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_to
     return;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, n, d_n, part);
+  if (nb <= SCAN_DIRECT_TILES) {
+    hipLaunchKernelGGL(k_scan_tiles_direct, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, d_n,
+                       (const uint64_t*)part, d_total);
+    return;
+  }
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, nb, d_total);
   hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, d_n, (const uint64_t*)part);
 }
