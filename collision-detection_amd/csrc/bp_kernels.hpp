@@ -139,23 +139,35 @@ __global__ __launch_bounds__(CHUNK) void k_bounds(const Chunk* __restrict__ chun
 
 // Per stepped scene: bounds from its chunks' partials, then origin and
 // inverse bin size (Broadphase.cu:377-381).  One workgroup per scene.
+constexpr int PARAMS_BS = 1024;
+
 template <class R>
-__global__ __launch_bounds__(256) void k_params(const uint32_t* __restrict__ sel, const DevStatus* __restrict__ st,
-                                                const uint32_t* __restrict__ bpart, SceneDesc<R>* __restrict__ desc) {
+__global__ __launch_bounds__(PARAMS_BS) void k_params(const uint32_t* __restrict__ sel,
+                                                      const DevStatus* __restrict__ st,
+                                                      const uint32_t* __restrict__ bpart,
+                                                      SceneDesc<R>* __restrict__ desc) {
   if (blockIdx.x >= st->nsel) return;
-  __shared__ uint32_t red[6][256];
+  __shared__ uint32_t red[6][PARAMS_BS / WAVE];
   SceneDesc<R>& d = desc[sel[blockIdx.x]];
   uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-  for (uint64_t b = d.chunk_begin + threadIdx.x; b < d.chunk_end; b += 256) {
+#pragma unroll 4
+  for (uint64_t b = d.chunk_begin + threadIdx.x; b < d.chunk_end; b += PARAMS_BS) {
     for (int c = 0; c < 3; ++c) {
       v[c] = min(v[c], bpart[6 * b + c]);
       v[3 + c] = max(v[3 + c], bpart[6 * b + 3 + c]);
     }
   }
-  for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = v[c];
+  const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  for (int o = 32; o > 0; o >>= 1)
+    for (int c = 0; c < 3; ++c) {
+      v[c] = min(v[c], (uint32_t)__shfl_xor((int)v[c], o));
+      v[3 + c] = max(v[3 + c], (uint32_t)__shfl_xor((int)v[3 + c], o));
+    }
+  if (l == 0)
+    for (int c = 0; c < 6; ++c) red[c][w] = v[c];
   __syncthreads();
   if (threadIdx.x != 0) return;
-  for (int t = 1; t < 256; ++t)
+  for (int t = 1; t < PARAMS_BS / WAVE; ++t)
     for (int c = 0; c < 3; ++c) {
       v[c] = min(v[c], red[c][t]);
       v[3 + c] = max(v[3 + c], red[3 + c][t]);
@@ -313,19 +325,39 @@ __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ c
 }
 
 // Active bins (reduce_by_key, Broadphase.cu:439-443): the number of runs of
-// equal keys, one atomic per workgroup of a capped grid.
-__global__ __launch_bounds__(256) void k_count_runs(uint64_t mcap, const uint64_t* __restrict__ keys, DevStatus* st) {
-  __shared__ uint32_t red[256 / WAVE];
+// equal keys.  Each thread counts the run starts among RUNS_IPT consecutive
+// entries per pass (independent loads in flight); a fixed grid of RUNS_GRID
+// workgroups strides over the entries, so only RUNS_GRID atomics meet at the
+// one counter (same-address atomics are serialised device-wide).
+constexpr int RUNS_BS = 256;
+constexpr int RUNS_IPT = 16;
+constexpr unsigned RUNS_GRID = 256;
+
+__global__ __launch_bounds__(RUNS_BS) void k_count_runs(uint64_t mcap, const uint64_t* __restrict__ keys,
+                                                        DevStatus* st) {
+  __shared__ uint32_t red[RUNS_BS / WAVE];
   const uint64_t m = cap_count(mcap, &st->M);
   uint32_t c = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (uint64_t)gridDim.x * 256)
-    c += (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) ? 1u : 0u;
+  for (uint64_t b0 = ((uint64_t)blockIdx.x * RUNS_BS + threadIdx.x) * RUNS_IPT; b0 < m;
+       b0 += (uint64_t)gridDim.x * RUNS_BS * RUNS_IPT) {
+    uint32_t k[RUNS_IPT + 1];
+#pragma unroll
+    for (int i = 0; i <= RUNS_IPT; ++i) {
+      const uint64_t e = b0 + (uint64_t)i - 1;  // e = b0 - 1 for i = 0
+      k[i] = (i == 0 && b0 == 0) || e >= m ? ~0u : (uint32_t)(keys[e] >> 32);
+    }
+#pragma unroll
+    for (int i = 1; i <= RUNS_IPT; ++i) {
+      const uint64_t e = b0 + (uint64_t)i - 1;
+      c += (e < m && (e == 0 || k[i] != k[i - 1])) ? 1u : 0u;
+    }
+  }
   for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o);
   if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t = 0;
-    for (int w = 0; w < 256 / WAVE; ++w) t += red[w];
+    for (int w = 0; w < RUNS_BS / WAVE; ++w) t += red[w];
     if (t) atomicAdd((unsigned long long*)&st->A, (unsigned long long)t);
   }
 }
@@ -392,16 +424,29 @@ struct alignas(16) PairMeta {
   uint32_t ids[META_IDS];
 };
 
+// Closed overlap test of two shifted AABBs (Broadphase.cu:237 / :296).  The
+// comparisons are combined without short-circuit evaluation: a && chain
+// would issue six dependent LDS reads, each waited for before the next.
+template <class R> __device__ __forceinline__ bool overlaps(const BpRec<R>& a, const BpRec<R>& b) {
+  return (a.lo[0] <= b.hi[0]) & (b.lo[0] <= a.hi[0]) & (a.lo[1] <= b.hi[1]) & (b.lo[1] <= a.hi[1]) &
+         (a.lo[2] <= b.hi[2]) & (b.lo[2] <= a.hi[2]);
+}
+
+// Whether the overlapping pair belongs to the run of key K: the hash of the
+// cell max(gmin_a, gmin_b) (both AABBs cover it) must be K.
 template <class R>
-__device__ __forceinline__ bool pair_test(const BpRec<R>& ra, const int ga[3], const BpRec<R>& rb, const R inv[3],
-                                          uint32_t K, uint32_t kb, int gx, int gy) {
-  const bool hit = (ra.lo[0] <= rb.hi[0] && rb.lo[0] <= ra.hi[0]) && (ra.lo[1] <= rb.hi[1] && rb.lo[1] <= ra.hi[1]) &&
-                   (ra.lo[2] <= rb.hi[2] && rb.lo[2] <= ra.hi[2]);
-  if (!hit) return false;
+__device__ __forceinline__ bool owner_bin(const int ga[3], const BpRec<R>& rb, const R inv[3], uint32_t K, uint32_t kb,
+                                          int gx, int gy) {
   const uint32_t ox = (uint32_t)max(ga[0], cell_of(rb.lo[0], inv[0]));
   const uint32_t oy = (uint32_t)max(ga[1], cell_of(rb.lo[1], inv[1]));
   const uint32_t oz = (uint32_t)max(ga[2], cell_of(rb.lo[2], inv[2]));
   return kb + hash_index(ox, oy, oz, gx, gy) == K;
+}
+
+template <class R>
+__device__ __forceinline__ bool pair_test(const BpRec<R>& ra, const int ga[3], const BpRec<R>& rb, const R inv[3],
+                                          uint32_t K, uint32_t kb, int gx, int gy) {
+  return overlaps(ra, rb) && owner_bin(ga, rb, inv, K, kb, gx, gy);
 }
 
 // Scene parameters and cells of one entry's shape.
@@ -529,9 +574,12 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
         }
         const int k = e + 1 + (int)(f - ebeg);
         const uint32_t b = sbody[k];
-        if (b == prev) continue;
+        const BpRec<R> rk = srec[k];
+        // repeat entries of one shape (b == prev) are skipped; the record is
+        // read and tested with the body id rather than after it
+        const bool fresh = b != prev;
         prev = b;
-        if (pair_test(c.r, c.g0, srec[k], c.inv, c.K, c.kb, c.gx, c.gy)) {
+        if ((fresh & overlaps(c.r, rk)) && owner_bin(c.g0, rk, c.inv, c.K, c.kb, c.gx, c.gy)) {
 #pragma unroll
           for (int q = 0; q < META_IDS; ++q)
             if ((uint32_t)q == cnt) loc[q] = b;

@@ -881,8 +881,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t sel_grid() const { return exact_ ? std::max<uint64_t>(nsel_, 1) : std::max<uint64_t>(sel_cap_, 1); }
 
   void stage_aabb() {
-    hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
-                       (int)ZERO_AABB);
+    if (!opened_)
+      hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                         (int)ZERO_AABB);
     mark(ST_AABB, true);
     hipLaunchKernelGGL(k_aabb<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, type_.as<uint8_t>(), body_.as<uint32_t>(), lpos_.as<R>(),
@@ -894,14 +895,15 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   // Broadphase from the AABBs (exact mode reads totals back between steps).
   void stage_broadphase() {
-    hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
-                       (int)ZERO_BP);
+    if (!opened_)
+      hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                         (int)ZERO_BP);
     mark(ST_BP_BOUNDS, true);
     if (aabb_from_caller_)
       hipLaunchKernelGGL(k_bounds<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                          (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                          bpart_.as<uint32_t>());
-    hipLaunchKernelGGL(k_params<R>, dim3((unsigned)sel_grid()), dim3(256), 0, s_, (const uint32_t*)sel_.as<uint32_t>(),
+    hipLaunchKernelGGL(k_params<R>, dim3((unsigned)sel_grid()), dim3(PARAMS_BS), 0, s_, (const uint32_t*)sel_.as<uint32_t>(),
                        (const DevStatus*)st(), (const uint32_t*)bpart_.as<uint32_t>(), desc_.as<SceneDesc<R>>());
     mark(ST_BP_BOUNDS, false);
 
@@ -946,7 +948,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       swap_bufs(keys_, keys_tmp_);
     mark(ST_BP_SORT, false);
     mark(ST_BP_RUNS, true);
-    hipLaunchKernelGGL(k_count_runs, dim3(std::min<unsigned>(blocks_for(mcap_, 256), 1024u)), dim3(256), 0, s_, mcap_,
+    hipLaunchKernelGGL(k_count_runs, dim3(std::min<unsigned>(blocks_for(mcap_, RUNS_BS * RUNS_IPT), RUNS_GRID)),
+                       dim3(RUNS_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), st());
     mark(ST_BP_RUNS, false);
 
@@ -1001,8 +1004,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   }
 
   void stage_narrowphase() {
-    hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
-                       (int)ZERO_NP);
+    if (!opened_)
+      hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                         (int)ZERO_NP);
     const uint64_t P = std::max<uint64_t>(pcap_, 1);
     np_v_.reserve(P * 10 * sizeof(R));
     np_ids_.reserve(P * 8);
@@ -1127,13 +1131,20 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   }
 
   void enqueue_capacity_step() {
-    bool saved_prof = prof_;
+    struct Restore {  // also on an exception
+      Engine* e;
+      bool prof;
+      ~Restore() { e->prof_ = prof; e->opened_ = false; }
+    } restore{this, prof_};
     prof_ = false;  // no event records inside a recorded step
+    // one stage opening for the whole step (the stages' totals are disjoint)
+    hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                       (int)(ZERO_AABB | ZERO_BP | ZERO_NP));
+    opened_ = true;
     stage_aabb();
     stage_broadphase();
     stage_narrowphase();
     stage_scene_ranges();
-    prof_ = saved_prof;
   }
 
   void launch_capacity_step(bool use_graph) {
@@ -1286,6 +1297,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
   DevBuf scan_tmp_, radix_tmp_;
+  bool opened_ = false;  // the step's totals were zeroed by one k_stage_begin
 };
 
 }  // namespace
