@@ -358,6 +358,18 @@ __device__ __forceinline__ PlacedF placed_local(const ShapeRecF& r, const ShapeR
 
 __device__ __forceinline__ uint32_t type_slot(uint32_t t) { return t == CAPSULE ? 4u : t; }
 
+// The first portal test of mpr_guarded on its own (same operations, so the
+// same verdict): true when the pair is a clear miss along the centre line.
+// The front pass drops such pairs before the queue is built and sorted.
+__device__ __forceinline__ bool mpr_first_miss(const PlacedF& A, const PlacedF& B, float tau) {
+  const v3<float> v0 = B.X - A.X;
+  if (length_fast(v0) < 1e-4f * tau) return false;
+  const v3<float> n = -v0;
+  const v3<float> v1 = sup_fast(B, n) - sup_fast(A, -n);
+  const float dn = dot(v1, n);
+  return dn <= 0.f && !(-dn < tau * length_fast(n));
+}
+
 template <class R> __device__ __forceinline__ ShapeRec<double> widen(const ShapeRec<R>& r) {
   ShapeRec<double> d;
   d.x = r.x; d.y = r.y; d.z = r.z; d.qw = r.qw; d.qx = r.qx; d.qy = r.qy; d.qz = r.qz;
@@ -378,11 +390,14 @@ template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(
 // class so every wavefront of the MPR kernel runs one pair of support maps.
 // MIXED / FAST first run an FP32 MPR in the frame of shape A: clear misses
 // end here, FAST keeps clear FP32 hits, everything else is queued for the
-// reference-order MPR.
+// reference-order MPR.  In those modes the front pass already applies the
+// FP32 MPR's first portal test, so pairs separated along the centre line
+// (most candidate pairs) never enter the queue.
 template <class R>
 __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* __restrict__ d_np,
                                                   const uint64_t* __restrict__ pairs,
                                                   const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
+                                                  const ShapeRecF* __restrict__ frec, float tau_rel, int prefilter,
                                                   R envelope, PairOut<R> o, uint32_t* __restrict__ qsel,
                                                   uint8_t* __restrict__ qcls) {
   const uint64_t P = o.P;
@@ -399,6 +414,11 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
       Contact<R> c;
       ShapeRec<R> ra = rec[ia], rb = rec[ib];
       if (contact_spheres(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
+    } else if (prefilter) {
+      const ShapeRecF fa = frec[ia], fb = frec[ib];
+      const PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
+      const float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
+      enq = !mpr_first_miss(A, B, tau);
     } else {
       enq = true;
     }

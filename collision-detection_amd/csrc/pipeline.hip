@@ -1023,7 +1023,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_NP_MAIN, true);
     hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, dP,
                        (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(),
-                       (const ShapeHdr*)hdr_.as<ShapeHdr>(), envelope_, o, qsel_.as<uint32_t>(), qcls_.as<uint8_t>());
+                       (const ShapeHdr*)hdr_.as<ShapeHdr>(), (const ShapeRecF*)frec_.as<ShapeRecF>(), tau_,
+                       (int)(mode_ != NP_NATIVE), envelope_, o, qsel_.as<uint32_t>(), qcls_.as<uint8_t>());
     scan_excl_u32(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->Q, scan_tmp_.as<uint32_t>(), s_, dP);
     hipLaunchKernelGGL(k_queue_compact, dim3(blocks_for(P, 256)), dim3(256), 0, s_, P, dP,
                        (const uint32_t*)qsel_.as<uint32_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
