@@ -62,13 +62,13 @@ size_t scan_tmp_words(uint64_t n);
 void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s,
                    const uint64_t* d_n = nullptr);
 
-// Stable LSD radix sort of n 64-bit values by the 8-bit digits at the given
-// bit shifts (least significant first), ping-ponging between data and tmp.
-// Returns true when the result ends in tmp (an odd number of passes): the
-// caller swaps the buffers instead of copying.  hist_tmp must hold
-// radix_tmp_words(n).
-size_t radix_tmp_words(uint64_t n);
-bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
+// Stable LSD radix sort of n 64-bit values by bits [lo_bit, lo_bit + nbits),
+// in the fewest passes of 8 to 11-bit digits whose per-tile digit counts
+// stay small, ping-ponging between data and tmp.  Returns true when the
+// result ends in tmp (an odd number of passes): the caller swaps the buffers
+// instead of copying.  hist_tmp must hold radix_tmp_words(n, nbits).
+size_t radix_tmp_words(uint64_t n, int nbits);
+bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, int lo_bit, int nbits, uint32_t* hist_tmp,
                     hipStream_t s, const uint64_t* d_n = nullptr);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);

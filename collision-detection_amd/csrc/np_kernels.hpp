@@ -357,6 +357,7 @@ __device__ __forceinline__ PlacedF placed_local(const ShapeRecF& r, const ShapeR
 }
 
 __device__ __forceinline__ uint32_t type_slot(uint32_t t) { return t == CAPSULE ? 4u : t; }
+constexpr int CLASS_BITS = 8;  // sort digit of the queue class (type_slot(a) * 5 + type_slot(b) < 25)
 
 // The first portal test of mpr_guarded on its own (same operations, so the
 // same verdict): true when the pair is a clear miss along the centre line.

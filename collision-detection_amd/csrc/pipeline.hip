@@ -939,12 +939,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        (const uint32_t*)offs_.as<uint32_t>(), (const uint32_t*)counts_buf_.as<uint32_t>(), mcap_,
                        keys_.as<uint64_t>(), pcount_.as<uint32_t>(), st());
     mark(ST_BP_EMIT, false);
-    std::vector<int> sh;
-    for (int b = 0; b < key_bits; b += 8) sh.push_back(32 + b);
-    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(std::max(mcap_, slots)), scan_tmp_words(mcap_) + 16) * 4);
+    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits), 16) * 4);
     mark(ST_BP_SORT, true);
-    if (radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, sh.data(), (int)sh.size(),
-                       radix_tmp_.as<uint32_t>(), s_, &st()->M))
+    if (radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, 32, key_bits, radix_tmp_.as<uint32_t>(),
+                       s_, &st()->M))
       swap_bufs(keys_, keys_tmp_);
     mark(ST_BP_SORT, false);
     mark(ST_BP_RUNS, true);
@@ -1017,7 +1015,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     qsel_.reserve(P * 4);
     qcls_.reserve(P);
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(P), 16) * 4);
-    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(P), 16) * 4);
+    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(P, CLASS_BITS), 16) * 4);
     PairOut<R> o{np_v_.as<R>(), np_ids_.as<int32_t>(), np_hit_.as<uint32_t>(), P};
     const uint64_t* dP = &st()->P;
     mark(ST_NP_MAIN, true);
@@ -1034,8 +1032,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       read_status();
       q = std::max<uint64_t>(hstat_->Q, 1);
     }
-    const int cls_shift[1] = {32};
-    if (radix_sort_u64(queue_.as<uint64_t>(), queue_tmp_.as<uint64_t>(), q, cls_shift, 1, radix_tmp_.as<uint32_t>(), s_,
+    if (radix_sort_u64(queue_.as<uint64_t>(), queue_tmp_.as<uint64_t>(), q, 32, CLASS_BITS, radix_tmp_.as<uint32_t>(), s_,
                        &st()->Q))
       swap_bufs(queue_, queue_tmp_);
     mark(ST_NP_MAIN, false);
@@ -1187,8 +1184,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     scan_tmp_.reserve(std::max<size_t>(std::max(scan_tmp_words(slots), scan_tmp_words(P)), 16) * 4);
     keys_.reserve(mcap_ * 8);
     keys_tmp_.reserve(mcap_ * 8);
-    radix_tmp_.reserve(std::max<size_t>(std::max(radix_tmp_words(std::max(mcap_, slots)), radix_tmp_words(P)),
-                                        scan_tmp_words(mcap_) + 16) * 4);
+    radix_tmp_.reserve(std::max<size_t>(std::max(radix_tmp_words(mcap_, key_bits_), radix_tmp_words(P, CLASS_BITS)), 16) *
+                       4);
     pcount_.reserve((mcap_ + 1) * 4);
     meta_.reserve(mcap_ * sizeof(PairMeta));
     pairs_.reserve(P * 8);

@@ -170,11 +170,11 @@ constexpr int RS_TILE = RS_BS * RS_IPT;
 constexpr int RS_NW = RS_BS / WAVE;
 constexpr int RS_WTILE = RS_IPT * WAVE;  // elements per wavefront: a contiguous run of the tile
 
-// Lanes of the wavefront holding the same 8-bit digit d (8 ballots).
-__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+// Lanes of the wavefront holding the same DB-bit digit d (DB ballots).
+template <int DB> __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
   uint64_t peers = __ballot(valid);
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
+  for (int b = 0; b < DB; ++b) {
     const uint64_t bb = __ballot((d >> b) & 1u);
     peers &= ((d >> b) & 1u) ? bb : ~bb;
   }
@@ -184,12 +184,14 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 // Per-tile digit histogram.  Each wavefront owns a contiguous run of the
 // tile; equal digits of a wavefront are counted once by their lowest lane,
 // so LDS atomics do not pile up on runs of equal digits.
+template <int DB>
 __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restrict__ in, uint64_t cap,
                                                          const uint64_t* __restrict__ d_n, int shift,
                                                          uint32_t* __restrict__ hist, uint32_t nb) {
-  __shared__ uint32_t h[256];
+  constexpr uint32_t ND = 1u << DB;
+  __shared__ uint32_t h[ND];
   const uint64_t n = eff_n(cap, d_n);
-  h[threadIdx.x] = 0;
+  for (uint32_t d = threadIdx.x; d < ND; d += RS_BS) h[d] = 0;
   __syncthreads();
   const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * RS_WTILE + lane;
@@ -198,12 +200,12 @@ __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restr
   for (int i = 0; i < RS_IPT; ++i) {
     const uint64_t e = base + (uint64_t)i * WAVE;
     const bool valid = e < n;
-    const uint32_t d = valid ? (uint32_t)(in[e] >> shift) & 255u : 0u;
-    const uint64_t peers = digit_peers(d, valid);
+    const uint32_t d = valid ? (uint32_t)(in[e] >> shift) & (ND - 1) : 0u;
+    const uint64_t peers = digit_peers<DB>(d, valid);
     if (valid && (peers & lt) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
   }
   __syncthreads();
-  hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+  for (uint32_t d = threadIdx.x; d < ND; d += RS_BS) hist[(uint64_t)d * nb + blockIdx.x] = h[d];
 }
 
 // Stable scatter.  Tile order is (wavefront, item, lane), each wavefront a
@@ -211,14 +213,17 @@ __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restr
 // digits of its wavefront (ballots + a wavefront-private LDS counter per
 // digit, no barriers); one barrier, a per-digit prefix over wavefronts and
 // the tile's global digit offset; pass 2 writes.
+template <int DB>
 __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                            uint64_t cap, const uint64_t* __restrict__ d_n, int shift,
                                                            const uint32_t* __restrict__ offs, uint32_t nb) {
-  __shared__ uint32_t wcnt[RS_NW][256];
+  constexpr uint32_t ND = 1u << DB;
+  __shared__ uint32_t wcnt[RS_NW][ND];
   const uint64_t n = eff_n(cap, d_n);
   const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  for (uint32_t d = threadIdx.x; d < ND; d += RS_BS)
 #pragma unroll
-  for (int ww = 0; ww < RS_NW; ++ww) wcnt[ww][threadIdx.x] = 0;
+    for (int ww = 0; ww < RS_NW; ++ww) wcnt[ww][d] = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * RS_WTILE + lane;
   const uint64_t lt = lanemask_lt();
@@ -234,8 +239,8 @@ __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __res
   for (int i = 0; i < RS_IPT; ++i) {
     const uint64_t e = base + (uint64_t)i * WAVE;
     const bool valid = e < n;
-    const uint32_t d = (uint32_t)(v[i] >> shift) & 255u;
-    const uint64_t peers = digit_peers(d, valid);
+    const uint32_t d = (uint32_t)(v[i] >> shift) & (ND - 1);
+    const uint64_t peers = digit_peers<DB>(d, valid);
     const uint32_t before = wcnt[w][d];  // same value in every lane of the digit (no lane wrote it yet)
     rk[i] = before + (uint32_t)__popcll(peers & lt);
     // lanes of one wavefront execute in lockstep: every peer has read the
@@ -245,12 +250,12 @@ __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __res
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  {
-    uint32_t run = offs[(uint64_t)threadIdx.x * nb + blockIdx.x];
+  for (uint32_t d = threadIdx.x; d < ND; d += RS_BS) {
+    uint32_t run = offs[(uint64_t)d * nb + blockIdx.x];
 #pragma unroll
     for (int ww = 0; ww < RS_NW; ++ww) {
-      const uint32_t c = wcnt[ww][threadIdx.x];
-      wcnt[ww][threadIdx.x] = run;
+      const uint32_t c = wcnt[ww][d];
+      wcnt[ww][d] = run;
       run += c;
     }
   }
@@ -258,8 +263,38 @@ __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __res
 #pragma unroll
   for (int i = 0; i < RS_IPT; ++i) {
     const uint64_t e = base + (uint64_t)i * WAVE;
-    if (e < n) out[wcnt[w][(uint32_t)(v[i] >> shift) & 255u] + rk[i]] = v[i];
+    if (e < n) out[wcnt[w][(uint32_t)(v[i] >> shift) & (ND - 1)] + rk[i]] = v[i];
   }
+}
+
+// Digit width per pass: the fewest passes whose digits have at most 11 bits
+// and whose per-tile histograms stay within RS_HIST_CAP counts.
+constexpr int RS_MIN_DB = 8, RS_MAX_DB = 11;
+constexpr uint64_t RS_HIST_CAP = 8ull << 20;
+
+struct RadixPlan {
+  int passes, db;
+};
+
+RadixPlan radix_plan(uint64_t n, int nbits) {
+  if (nbits <= 0) return {0, RS_MIN_DB};
+  const uint64_t nb = std::max<uint64_t>((n + RS_TILE - 1) / RS_TILE, 1);
+  const int p8 = (nbits + RS_MIN_DB - 1) / RS_MIN_DB;
+  for (int p = 1; p < p8; ++p) {
+    const int db = (nbits + p - 1) / p;
+    if (db <= RS_MAX_DB && (1ull << db) * nb <= RS_HIST_CAP) return {p, db};
+  }
+  return {p8, RS_MIN_DB};
+}
+
+template <int DB>
+void radix_pass(const uint64_t* src, uint64_t* dst, uint64_t n, const uint64_t* d_n, int shift, uint32_t* hist,
+                uint64_t* total, uint32_t* stmp, uint32_t nb, hipStream_t s) {
+  const uint64_t h = (uint64_t)(1u << DB) * nb;
+  hipLaunchKernelGGL(k_radix_upsweep<DB>, dim3(nb), dim3(RS_BS), 0, s, src, n, d_n, shift, hist, nb);
+  scan_excl_u32(hist, hist, h, total, stmp, s, nullptr);
+  hipLaunchKernelGGL(k_radix_downsweep<DB>, dim3(nb), dim3(RS_BS), 0, s, src, dst, n, d_n, shift,
+                     (const uint32_t*)hist, nb);
 }
 
 }  // namespace
@@ -287,27 +322,32 @@ void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_to
   hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, d_n, (const uint64_t*)part);
 }
 
-size_t radix_tmp_words(uint64_t n) {
-  uint64_t nb = (n + RS_TILE - 1) / RS_TILE;
-  uint64_t h = 256 * (nb ? nb : 1);
+size_t radix_tmp_words(uint64_t n, int nbits) {
+  const RadixPlan pl = radix_plan(n, nbits);
+  const uint64_t nb = std::max<uint64_t>((n + RS_TILE - 1) / RS_TILE, 1);
+  const uint64_t h = (uint64_t)(1u << pl.db) * nb;
   return h + scan_tmp_words(h) + 2;
 }
 
-bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, const int* shifts, int nshift, uint32_t* hist_tmp,
+bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, int lo_bit, int nbits, uint32_t* hist_tmp,
                     hipStream_t s, const uint64_t* d_n) {
-  if (n <= 1 || nshift == 0) return false;
-  uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
-  uint64_t h = 256ull * nb;
+  const RadixPlan pl = radix_plan(n, nbits);
+  if (n <= 1 || pl.passes == 0) return false;
+  const uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  const uint64_t h = (uint64_t)(1u << pl.db) * nb;
   uint32_t* hist = hist_tmp;
-  uint64_t* total = reinterpret_cast<uint64_t*>(hist_tmp + h + scan_tmp_words(h));
   uint32_t* stmp = hist_tmp + h;
+  uint64_t* total = reinterpret_cast<uint64_t*>(hist_tmp + h + scan_tmp_words(h));
   uint64_t* src = data;
   uint64_t* dst = tmp;
-  for (int p = 0; p < nshift; ++p) {
-    hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, n, d_n, shifts[p], hist, nb);
-    scan_excl_u32(hist, hist, h, total, stmp, s, nullptr);
-    hipLaunchKernelGGL(k_radix_downsweep, dim3(nb), dim3(RS_BS), 0, s, (const uint64_t*)src, dst, n, d_n, shifts[p],
-                       (const uint32_t*)hist, nb);
+  for (int p = 0; p < pl.passes; ++p) {
+    const int shift = lo_bit + p * pl.db;
+    switch (pl.db) {
+      case 8: radix_pass<8>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
+      case 9: radix_pass<9>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
+      case 10: radix_pass<10>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
+      default: radix_pass<11>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
+    }
     uint64_t* t = src; src = dst; dst = t;
   }
   return src != data;
