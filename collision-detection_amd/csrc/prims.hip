@@ -211,21 +211,28 @@ __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restr
 // Stable scatter.  Tile order is (wavefront, item, lane), each wavefront a
 // contiguous run of the input.  Pass 1 ranks every element among equal
 // digits of its wavefront (ballots + a wavefront-private LDS counter per
-// digit, no barriers); one barrier, a per-digit prefix over wavefronts and
-// the tile's global digit offset; pass 2 writes.
+// digit, no barriers).  The tile is then ordered by digit in LDS (a scan of
+// the tile's digit counts gives each digit's place) and written out in LDS
+// order: consecutive threads store consecutive positions of one digit's
+// run, so the stores coalesce instead of scattering one element per lane.
 template <int DB>
 __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                            uint64_t cap, const uint64_t* __restrict__ d_n, int shift,
                                                            const uint32_t* __restrict__ offs, uint32_t nb) {
   constexpr uint32_t ND = 1u << DB;
+  constexpr int DPT = ND / RS_BS;  // digits per thread in the digit scan
   __shared__ uint32_t wcnt[RS_NW][ND];
+  __shared__ uint32_t gbase[ND];  // global position of the digit's run minus its place in the tile
+  __shared__ uint64_t sk[RS_TILE];
+  __shared__ uint32_t scratch[RS_BS / WAVE + 1];
   const uint64_t n = eff_n(cap, d_n);
   const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   for (uint32_t d = threadIdx.x; d < ND; d += RS_BS)
 #pragma unroll
     for (int ww = 0; ww < RS_NW; ++ww) wcnt[ww][d] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * RS_WTILE + lane;
+  const uint64_t t0 = (uint64_t)blockIdx.x * RS_TILE;
+  const uint64_t base = t0 + (uint64_t)w * RS_WTILE + lane;
   const uint64_t lt = lanemask_lt();
   uint64_t v[RS_IPT];
   uint32_t rk[RS_IPT];
@@ -250,20 +257,48 @@ __global__ __launch_bounds__(RS_BS) void k_radix_downsweep(const uint64_t* __res
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  for (uint32_t d = threadIdx.x; d < ND; d += RS_BS) {
-    uint32_t run = offs[(uint64_t)d * nb + blockIdx.x];
+  // place of each digit in the tile (exclusive scan of the tile's digit
+  // counts, DPT consecutive digits per thread), per-wavefront places inside
+  // each digit, and the digit's global offset
+  {
+    uint32_t tot[DPT];
+    uint32_t sum = 0;
 #pragma unroll
-    for (int ww = 0; ww < RS_NW; ++ww) {
-      const uint32_t c = wcnt[ww][d];
-      wcnt[ww][d] = run;
-      run += c;
+    for (int j = 0; j < DPT; ++j) {
+      const uint32_t d = threadIdx.x * DPT + j;
+      uint32_t c = 0;
+#pragma unroll
+      for (int ww = 0; ww < RS_NW; ++ww) c += wcnt[ww][d];
+      tot[j] = c;
+      sum += c;
+    }
+    uint32_t all;
+    uint32_t place = block_excl_scan<RS_BS>(sum, scratch, &all);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+      const uint32_t d = threadIdx.x * DPT + j;
+      gbase[d] = offs[(uint64_t)d * nb + blockIdx.x] - place;
+      uint32_t run = place;
+#pragma unroll
+      for (int ww = 0; ww < RS_NW; ++ww) {
+        const uint32_t c = wcnt[ww][d];
+        wcnt[ww][d] = run;
+        run += c;
+      }
+      place += tot[j];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < RS_IPT; ++i) {
     const uint64_t e = base + (uint64_t)i * WAVE;
-    if (e < n) out[wcnt[w][(uint32_t)(v[i] >> shift) & (ND - 1)] + rk[i]] = v[i];
+    if (e < n) sk[wcnt[w][(uint32_t)(v[i] >> shift) & (ND - 1)] + rk[i]] = v[i];
+  }
+  __syncthreads();
+  const uint32_t cnt = (uint32_t)min((uint64_t)RS_TILE, n > t0 ? n - t0 : 0);
+  for (uint32_t t = threadIdx.x; t < cnt; t += RS_BS) {
+    const uint64_t x = sk[t];
+    out[gbase[(uint32_t)(x >> shift) & (ND - 1)] + t] = x;
   }
 }
 
