@@ -139,6 +139,25 @@ BP_STAGES = ["bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs"
              "bp_pairemit", "bp_pairsort"]
 
 
+def dropin_leg(pkg, ctx, s, mode, steps):
+    """The reference's call sequence through the drop-in calls: host arrays in,
+    host arrays out, every call re-uploading what it reads (PCIe-inclusive)."""
+    def one():
+        aabb = ctx.generate_aabb(s)
+        pairs = ctx.detect_possible_collisions(aabb, s.grid)
+        out = ctx.do_narrowphase(s, pairs, mode=mode)
+        return len(pairs), len(out[3])
+    counts = one()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    dt = (time.perf_counter() - t0) / steps
+    return {"ms_per_pass": round(dt * 1e3, 3), "bodies_per_s": round(s.n / dt, 1), "passes": steps,
+            "pairs": counts[0], "contacts": counts[1],
+            "calls": "cdx_generate_aabb -> cdx_detect_possible_collisions + read -> cdx_do_narrowphase + read, "
+                     "numpy host buffers"}
+
+
 def lattice_leg(pkg, scenes, ctx, side, steps):
     """16,777,216-sphere FP32 granular pack (side 256): AABB + broadphase, then the full step.
 
@@ -505,6 +524,7 @@ def main():
     ap.add_argument("--mode", choices=["mixed", "native", "fast"], default="mixed")
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dropin-passes", type=int, default=3, help="passes through the drop-in calls (0 skips)")
     ap.add_argument("--lattice-side", type=int, default=256, help="0 skips the 16M-sphere broadphase leg")
     ap.add_argument("--lattice-steps", type=int, default=5)
     ap.add_argument("--batched-scenes", type=int, default=512, help="0 skips the batched leg")
@@ -614,6 +634,12 @@ def main():
             for k, b in bytes_.items()},
     }
 
+    dropin = None
+    if args.dropin_passes > 0:
+        dropin = dropin_leg(pkg, ctx, s, mode, args.dropin_passes)
+        assert dropin["pairs"] == c.pairs and dropin["contacts"] == c.contacts, "drop-in calls disagree with the step"
+        log(f"drop-in calls: {dropin}")
+
     batched = None
     if args.batched_scenes > 0:
         batched = batched_leg(pkg, scenes, ctx, args.batched_scenes, args.batched_steps)
@@ -660,6 +686,7 @@ def main():
         "p99_step_ms": graph_path["p99_step_ms"],
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "scene_path": scene_path,
+        "dropin_path": dropin,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "lattice_16M_broadphase": lattice,

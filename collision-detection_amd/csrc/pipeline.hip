@@ -175,6 +175,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     CDX_HIP_CHECK(hipHostMalloc(&hstat_, sizeof(DevStatus), hipHostMallocDefault));
     CDX_HIP_CHECK(hipHostMalloc(&hhdr_, sizeof(StepHdr), hipHostMallocDefault));
     std::memset(hstat_, 0, sizeof(DevStatus));
+    CDX_HIP_CHECK(hipMemsetAsync(status_.p, 0, sizeof(DevStatus), s_));
+    CDX_HIP_CHECK(hipStreamSynchronize(s_));
     for (int i = 0; i < 2 * ST_COUNT + 2; ++i) CDX_HIP_CHECK(hipEventCreate(&ev_[i]));
     std::memset(&counts_, 0, sizeof(counts_));
   }
@@ -895,9 +897,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   // Broadphase from the AABBs (exact mode reads totals back between steps).
   void stage_broadphase() {
-    if (!opened_)
+    if (!opened_)  // AABBs from the caller: no AABB stage cleared the error flags
       hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
-                         (int)ZERO_BP);
+                         (int)(ZERO_BP | (aabb_from_caller_ ? ZERO_AABB : 0)));
     mark(ST_BP_BOUNDS, true);
     if (aabb_from_caller_)
       hipLaunchKernelGGL(k_bounds<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,

@@ -77,6 +77,18 @@ def test_dropin_broadphase_shifts_in_place(cdx, ctx, oracle, cases):
     np.testing.assert_array_equal(mine, ref.aabb_shifted)
 
 
+def test_dropin_error_does_not_stick(cdx, ctx, oracle, cases):
+    """An AABB with max < min is reported (no bins, CDX_ERR_INVALID); the next
+    drop-in broadphase on the same context starts with clear error flags."""
+    bad = np.array([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [0.5, 0.5, 0.5], [0.2, 0.2, 0.2]])
+    with pytest.raises(cdx.CdxError):
+        ctx.detect_possible_collisions(bad, (4, 4, 4))
+    s = cases["spheres8k_f64"]
+    ab = oracle.aabb(s)
+    ref = oracle.broadphase(ab, s.grid)
+    np.testing.assert_array_equal(ctx.detect_possible_collisions(ab.copy(), s.grid), ref.pairs)
+
+
 def _tol(s):
     L = float(np.ptp(s.pos, axis=0).max()) + 1.0
     return (1e-12 if s.dtype == np.float64 else 1e-5) * L
