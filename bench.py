@@ -105,15 +105,27 @@ def read_traffic(stage):
     return int(kv[f"{stage}.fetch_kib"] * 1024.0 * f_scale + kv[f"{stage}.write_kib"] * 1024.0 * w_scale)
 
 
-def cpu_baseline(oracle, s, min_seconds=10.0):
-    """The oracle pipeline (AABB + broadphase + narrowphase, FP64) on the whole workload, repeated for ~10 s."""
+def _host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND", "")}
+
+
+def _oracle_passes(oracle, s, flags, min_seconds):
     reps, t_total, parts = 0, 0.0, [0.0, 0.0, 0.0]
     pairs = contacts = 0
     while t_total < min_seconds:
         t0 = time.perf_counter()
         ab = oracle.aabb(s)
         t1 = time.perf_counter()
-        bp = oracle.broadphase(ab, s.grid)
+        bp = oracle.broadphase(ab, s.grid, flags)
         t2 = time.perf_counter()
         npr = oracle.narrowphase(s, bp.pairs)
         t3 = time.perf_counter()
@@ -123,7 +135,14 @@ def cpu_baseline(oracle, s, min_seconds=10.0):
         t_total += t3 - t0
         reps += 1
         pairs, contacts = len(bp.pairs), npr.count
-    return {
+    return reps, t_total, parts, pairs, contacts, bp.pairs
+
+
+def cpu_baseline(oracle, s, min_seconds=10.0, all_parallel_seconds=5.0):
+    """The oracle pipeline (AABB + broadphase + narrowphase, FP64) on the whole workload, repeated for ~10 s with
+    the reference's default parallel flags (the faithful baseline), then ~5 s with every broadphase step parallel."""
+    reps, t_total, parts, pairs, contacts, pair_list = _oracle_passes(oracle, s, oracle.DEFAULT_FLAGS, min_seconds)
+    out = {
         "value": round(s.n * reps / t_total, 1),
         "unit": "bodies/s",
         "cores": oracle.max_threads(),
@@ -132,7 +151,27 @@ def cpu_baseline(oracle, s, min_seconds=10.0):
                    f"host threads, reference default parallel flags (1,0,1,0,0,0,0,0,1,0); per pass aabb "
                    f"{parts[0] / reps:.3f}s broadphase {parts[1] / reps:.3f}s narrowphase {parts[2] / reps:.3f}s; "
                    f"{pairs} pairs, {contacts} contacts"),
+        "host": _host_info(),
     }
+    if all_parallel_seconds > 0:
+        r2, t2, p2, _, _, _ = _oracle_passes(oracle, s, oracle.ALL_PARALLEL, all_parallel_seconds)
+        out["all_parallel_flags"] = {
+            "value": round(s.n * r2 / t2, 1), "unit": "bodies/s",
+            "sample": (f"{r2} passes ({t2:.1f} s), every broadphase step parallel; per pass aabb {p2[0] / r2:.3f}s "
+                       f"broadphase {p2[1] / r2:.3f}s narrowphase {p2[2] / r2:.3f}s")}
+    out["mpr_steps"] = mpr_step_histogram(oracle, s, pair_list)
+    return out
+
+
+def mpr_step_histogram(oracle, s, pairs):
+    """Work per candidate pair of the reference narrowphase, in support steps
+    (pairs of support-map calls, contact points included), from the oracle."""
+    st = oracle.mpr_steps(s, pairs)
+    m = st[st > 0]
+    edges = [(1, 1), (2, 3), (4, 8), (9, 16), (17, 32), (33, 38), (39, 73)]
+    return {"pairs": int(len(st)), "mpr_pairs": int(len(m)), "mean_steps": round(float(m.mean()), 2) if len(m) else 0.0,
+            "hist": {f"{a}-{b}" if a != b else str(a): int(((m >= a) & (m <= b)).sum()) for a, b in edges},
+            "note": "39+ steps: the refinement ran to its 34-iteration cap"}
 
 
 BP_STAGES = ["bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs", "bp_pairscan",
@@ -685,6 +724,8 @@ def main():
         "p50_step_ms": graph_path["p50_step_ms"],
         "p99_step_ms": graph_path["p99_step_ms"],
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "narrowphase_pairs_per_s": round(c.pairs / (1e-3 * (stage_avg["np_main"] + stage_avg["np_resolve"] +
+                                                             stage_avg["np_compact"])), 1),
         "scene_path": scene_path,
         "dropin_path": dropin,
         "roofline": roofline,

@@ -328,8 +328,12 @@ template <class R> inline r3<R> sup_capsule(r3<R> B, r3<R> n) {
   return sign(n.y) * B.y * u + B.x * normalize(n);
 }
 
+// Support-map evaluations made by this thread (for the MPR step histogram).
+thread_local uint64_t t_support_calls = 0;
+
 // Narrowphase.cu:19-52 (in-scope shapes)
 template <class R> inline r3<R> TransformSupportVert(int type, r3<R> A, r3<R> B, r4<R> Rq, r3<R> b) {
+  ++t_support_calls;
   r3<R> n = normalize(b);
   r3<R> rn = quatRotateMatT(n, Rq);
   r3<R> ls;
@@ -646,6 +650,24 @@ void oracle_np_free(oracle_np_result* r) {
   if (!r) return;
   std::free(r->norm); std::free(r->cpta); std::free(r->cptb); std::free(r->dpth); std::free(r->bids);
   std::free(r->pairs_permuted); std::free(r->hit); std::free(r);
+}
+
+// Support steps (pairs of support-map evaluations, the MPR's unit of work,
+// including the two that place the contact points) per candidate pair of
+// function_MPR_Store (Narrowphase.cu:316-423): 0 for pairs that end before
+// the MPR (inactive, same body, sphere-sphere).
+void oracle_mpr_steps_f64(uint64_t P, const int64_t* pairs, const int32_t* T, const double* A, const double* B,
+                          const double* Rq, const uint32_t* ID, const uint8_t* active, const double* bpos,
+                          const double* brot, int32_t* steps) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)P; ++i) {
+    r3<double> N, p1, p2;
+    double d;
+    int32_t id2[2];
+    t_support_calls = 0;
+    mpr_store<double>(pairs[i], T, A, B, Rq, ID, active, bpos, brot, 0.0, N, p1, p2, d, id2);
+    steps[i] = (int32_t)(t_support_calls / 2);
+  }
 }
 
 // One MPR query on two world-placed shapes (CollideAndFindPoint,

@@ -8,6 +8,7 @@ The oracle restates the reference's OpenMP path:
   aabb          AABBGenerator.cu:84-130
   broadphase    Broadphase.cu:344-545
   narrowphase   Narrowphase.cu:316-595
+  mpr_steps     the narrowphase's support steps per pair (MPR work histogram)
 """
 from __future__ import annotations
 
@@ -78,6 +79,7 @@ def lib():
         L.oracle_support_f64.argtypes = [C.c_int, vp, vp, vp, vp, vp]
         L.oracle_quat_rotate_f64.argtypes = [vp, vp, vp]
         L.oracle_quat_mult_f64.argtypes = [vp, vp, vp]
+        L.oracle_mpr_steps_f64.argtypes = [C.c_uint64] + [vp] * 10
         L.oracle_max_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -197,6 +199,19 @@ def narrowphase(scene, pairs: np.ndarray, envelope: float = 0.0) -> NPResult:
     finally:
         L.oracle_np_free(r)
     return res
+
+
+def mpr_steps(scene, pairs: np.ndarray) -> np.ndarray:
+    """Support steps (pairs of support-map calls) the reference narrowphase
+    spends on each candidate pair, contact points included; 0 where it ends
+    before the MPR.  FP64 scenes only."""
+    s = scene
+    assert s.dtype == np.float64
+    pr = np.ascontiguousarray(pairs, dtype=np.int64)
+    out = np.zeros(len(pr), dtype=np.int32)
+    lib().oracle_mpr_steps_f64(len(pr), _p(pr), _p(s.T), _p(s.A), _p(s.B), _p(s.R), _p(s.ID), _p(s.active),
+                               _p(s.pos), _p(s.rot), _p(out))
+    return out
 
 
 def mpr(tA, AX, AY, AR, tB, BX, BY, BR):

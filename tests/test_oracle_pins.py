@@ -317,3 +317,38 @@ def test_envelope_adds_twice_to_depth(oracle, scenes):
     r0 = oracle.narrowphase(s, pairs)
     r1 = oracle.narrowphase(s, pairs, envelope=0.002)
     np.testing.assert_allclose(r1.dpth, r0.dpth + 0.004, atol=1e-13)
+
+
+def _pair_scene(scenes, ta, tb, xb, dims=0.5):
+    T = np.array([ta, tb], np.int32)
+    z3 = np.zeros((2, 3))
+    B = np.full((2, 3), dims)
+    R = np.tile([1.0, 0.0, 0.0, 0.0], (2, 1))
+    pos = np.array([[0.0, 0.0, 0.0], list(xb)])
+    return scenes.Scene(T, z3.copy(), B, z3.copy(), R, np.arange(2, dtype=np.uint32), pos, R.copy(),
+                        np.ones(2, np.uint8), (4, 4, 4))
+
+
+def test_mpr_steps_counts(oracle, scenes):
+    """Support steps per pair: none for sphere pairs (closed form), one for a
+    pair separated along the centre line (the first portal test ends it),
+    two for a hit whose first support lies on the centre line (the
+    degenerate early hit, Narrowphase.cu:161-171, plus the contact points),
+    and otherwise the discovery steps (2), at least one refinement step and
+    the contact points (1), at most the caps (2 + 35 + 35 + 1)."""
+    pair = np.array([(0 << 32) | 1], np.int64)
+    assert oracle.mpr_steps(_pair_scene(scenes, SPHERE, SPHERE, (0.5, 0, 0)), pair)[0] == 0
+    assert oracle.mpr_steps(_pair_scene(scenes, BOX, BOX, (3.0, 0, 0)), pair)[0] == 1
+    assert oracle.mpr_steps(_pair_scene(scenes, SPHERE, BOX, (0.7, 0, 0)), pair)[0] == 2
+    k = int(oracle.mpr_steps(_pair_scene(scenes, SPHERE, BOX, (0.7, 0.2, 0.1)), pair)[0])
+    assert 4 <= k <= 73
+    # the count is the work of the narrowphase itself: a hit costs more than a miss
+    s = scenes.mixed_bodies(n=4096, seed=3)
+    pairs = oracle.broadphase(oracle.aabb(s), s.grid).pairs
+    steps = oracle.mpr_steps(s, pairs)
+    hit = oracle.narrowphase(s, pairs).hit.astype(bool)
+    a, b = pairs >> 32, pairs & 0xffffffff
+    mpr = ~((s.T[a] == SPHERE) & (s.T[b] == SPHERE))
+    assert (steps[~mpr] == 0).all()
+    assert (steps[mpr] >= 1).all() and steps.max() <= 73
+    assert steps[mpr & hit].min() >= 2 and np.median(steps[mpr & hit]) >= 4
