@@ -460,9 +460,8 @@ template <class R> struct EntryCtx {
 };
 
 template <class R>
-__device__ __forceinline__ void entry_ctx(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r,
-                                          const SceneDesc<R>* __restrict__ desc) {
-  const SceneDesc<R>& d = desc[r.scene];
+__device__ __forceinline__ void entry_ctx_d(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r,
+                                            const SceneDesc<R>& d) {
   c.r = r;
   c.K = K;
   c.a = a;
@@ -479,6 +478,21 @@ __device__ __forceinline__ void entry_ctx(EntryCtx<R>& c, uint32_t K, uint32_t a
 }
 
 template <class R>
+__device__ __forceinline__ void entry_ctx(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r,
+                                          const SceneDesc<R>* __restrict__ desc) {
+  entry_ctx_d(c, K, a, r, desc[r.scene]);
+}
+
+// entry_ctx with the scene descriptor from LDS when the whole staged window
+// belongs to one scene (block-uniform branch), else from global memory.
+template <class R>
+__device__ __forceinline__ void entry_ctx_w(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r, bool one,
+                                            const SceneDesc<R>& sd, const SceneDesc<R>* __restrict__ desc) {
+  if (one) entry_ctx_d(c, K, a, r, sd);
+  else entry_ctx_d(c, K, a, r, desc[r.scene]);
+}
+
+template <class R>
 __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t* __restrict__ keys,
                                                    const BpRec<R>* __restrict__ brec,
                                                    const SceneDesc<R>* __restrict__ desc,
@@ -492,6 +506,8 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   __shared__ uint32_t scnt[PAIR_BS];
   __shared__ uint32_t sids[PAIR_BS][META_IDS];
   __shared__ uint32_t scan_scratch[PAIR_BS / WAVE + 1];
+  __shared__ SceneDesc<R> sdesc;  // the window's scene, when it has only one
+  __shared__ int sone;
   const uint64_t m = cap_count(mcap, &st->M);
   const uint64_t b0 = (uint64_t)blockIdx.x * PAIR_BS;
   if (b0 >= m) return;
@@ -506,6 +522,13 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     }
   }
   __syncthreads();
+  if (tid == 0) {
+    // keys ascend and scenes own disjoint key ranges: one scene at both ends
+    // means one scene throughout
+    const uint32_t s0 = srec[0].scene;
+    sone = s0 == srec[nloc - 1].scene ? 1 : 0;
+    if (sone) sdesc = desc[s0];
+  }
   // run ends inside the window (the last staged entry always ends one)
   for (int t = tid; t < PAIR_W; t += PAIR_BS) {
     const bool f = t < nloc && (t + 1 == nloc || skey[t + 1] != skey[t]);
@@ -513,6 +536,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     if (lane == 0) send[t / WAVE] = bal;
   }
   __syncthreads();
+  const bool one = sone != 0;
   // own entry: candidates inside the window
   const bool valid = tid < nloc;
   const uint64_t i = b0 + tid;
@@ -550,7 +574,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
       int e = lo;
       uint32_t ebeg = sstart[e], eend = sstart[e + 1];
       EntryCtx<R> c;
-      entry_ctx(c, skey[e], sbody[e], srec[e], desc);
+      entry_ctx_w(c, skey[e], sbody[e], srec[e], one, sdesc, desc);
       uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
       uint64_t mk = 0;
       uint32_t cnt = 0;
@@ -569,7 +593,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
           mk = 0;
           cnt = 0;
           do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
-          entry_ctx(c, skey[e], sbody[e], srec[e], desc);
+          entry_ctx_w(c, skey[e], sbody[e], srec[e], one, sdesc, desc);
           prev = c.a;
         }
         const int k = e + 1 + (int)(f - ebeg);
@@ -596,7 +620,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   uint64_t mk = smask[tid];
   uint32_t cnt = scnt[tid];
   EntryCtx<R> c;
-  entry_ctx(c, skey[tid], sbody[tid], srec[tid], desc);
+  entry_ctx_w(c, skey[tid], sbody[tid], srec[tid], one, sdesc, desc);
   if (beyond) {  // run continues past the window: the rest from global memory (rare)
     uint32_t prev = sbody[nloc - 1];
     for (uint64_t k = b0 + nloc; k < m; ++k) {
