@@ -367,13 +367,14 @@ __global__ __launch_bounds__(RUNS_BS) void k_count_runs(uint64_t mcap, const uin
 // corner at index == grid size makes up to four cells share a key
 // (Broadphase.cu:140-147); the first one inside the shape's range is taken.
 // Returns ~0u if no candidate lies in the range (never for emitted keys).
-__device__ __forceinline__ uint32_t local_cell_index(uint32_t kk, const Cells& c, int gx, int gy) {
-  const uint32_t q = kk / (uint32_t)gx, i0 = kk - q * (uint32_t)gx;
+__device__ __forceinline__ uint32_t local_cell_index(uint32_t kk, const Cells& c, int gx, int gy, uint64_t rgx,
+                                                     uint64_t rgy) {
+  const uint32_t q = div_by_recip(kk, (uint32_t)gx, rgx), i0 = kk - q * (uint32_t)gx;
   for (int ai = 0; ai < 2; ++ai) {
     uint32_t i, qq;
     if (ai == 0) { i = i0; qq = q; }
     else { if (i0 != 0 || q == 0) break; i = (uint32_t)gx; qq = q - 1; }
-    const uint32_t l0 = qq / (uint32_t)gy, j0 = qq - l0 * (uint32_t)gy;
+    const uint32_t l0 = div_by_recip(qq, (uint32_t)gy, rgy), j0 = qq - l0 * (uint32_t)gy;
     for (int aj = 0; aj < 2; ++aj) {
       uint32_t j, l;
       if (aj == 0) { j = j0; l = l0; }
@@ -455,6 +456,7 @@ template <class R> struct EntryCtx {
   R inv[3];
   int g0[3], g1[3];
   int gx, gy;
+  uint64_t rgx, rgy;
   uint32_t kb, K, a;
   uint64_t slot;  // chunk slot of the shape
 };
@@ -467,6 +469,8 @@ __device__ __forceinline__ void entry_ctx_d(EntryCtx<R>& c, uint32_t K, uint32_t
   c.a = a;
   c.gx = d.gx;
   c.gy = d.gy;
+  c.rgx = d.rgx;
+  c.rgy = d.rgy;
   c.kb = d.key_base;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -639,7 +643,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   Cells cl;
   cl.x0 = c.g0[0]; cl.y0 = c.g0[1]; cl.z0 = c.g0[2];
   cl.x1 = c.g1[0]; cl.y1 = c.g1[1]; cl.z1 = c.g1[2];
-  const uint32_t li = local_cell_index(c.K - c.kb, cl, c.gx, c.gy);
+  const uint32_t li = local_cell_index(c.K - c.kb, cl, c.gx, c.gy, c.rgx, c.rgy);
   const uint64_t j = (uint64_t)offs[c.slot] + li;
   if (li == ~0u || j >= m) {
     atomicOr(&st->bad, 8u);

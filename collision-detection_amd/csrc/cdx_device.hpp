@@ -56,7 +56,16 @@ template <class R> struct SceneDesc {
   uint32_t chunk_begin, chunk_end;  // this step's chunk range (empty if not stepped)
   uint32_t bmin_enc[3], bmax_enc[3];
   R origin[3], inv[3];
+  uint64_t rgx, rgy;  // floor((2^64 - 1) / g) + 1 for g = gx, gy: exact 32-bit division by multiply (0 for g = 1)
 };
+
+// n / d for n < 2^32 from r = floor((2^64 - 1) / d) + 1 (d >= 2): r exceeds
+// 2^64 / d by less than 1, so n * r / 2^64 exceeds n / d by less than 2^-32,
+// less than the gap 1/d between n / d and the next integer.
+__device__ __forceinline__ uint32_t div_by_recip(uint32_t n, uint32_t d, uint64_t r) {
+  return d == 1u ? n : (uint32_t)__umul64hi((uint64_t)n, r);
+}
+inline uint64_t recip_of(uint32_t d) { return d <= 1u ? 0ull : ~0ull / d + 1ull; }
 
 // Compact per-shape header for pair classification: type | active << 8, body.
 struct ShapeHdr {

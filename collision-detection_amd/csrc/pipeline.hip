@@ -773,6 +773,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     d.body_begin = (uint32_t)sc.body_begin;
     d.body_end = (uint32_t)(sc.body_begin + sc.n_bodies);
     d.gx = sc.g[0]; d.gy = sc.g[1]; d.gz = sc.g[2];
+    d.rgx = recip_of((uint32_t)sc.g[0]);
+    d.rgy = recip_of((uint32_t)sc.g[1]);
     d.key_base = (uint32_t)sc.key_base;
     selection_dirty_ = true;
     desc_dirty_ = true;
