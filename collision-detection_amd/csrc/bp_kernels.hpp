@@ -414,20 +414,18 @@ constexpr int PAIR_AHEAD = 64;
 constexpr int PAIR_W = PAIR_BS + PAIR_AHEAD;
 constexpr uint64_t MASK_TAIL = 1ull << 63;
 
-// Per entry with pairs: the hit mask, the entry's sorted position, and the
-// partners themselves when there are at most META_IDS of them (the emit
-// pass then reads no sorted entries; in any order, groups are sorted after).
-constexpr int META_IDS = 4;
+// Per entry with pairs, 16 bytes: either the partners themselves (at most
+// META_IDS of them and none beyond the window: the emit pass then reads no
+// sorted entries; in any order, groups are sorted after), or the hit mask
+// and the entry's sorted position.  w0 = pair count, bit 31 set for the
+// mask form; ids form: w1..w3 = partner ids; mask form: w1 = sorted
+// position, w2 / w3 = low / high half of the hit mask.
+constexpr int META_IDS = 3;
+constexpr uint32_t META_MASK_FORM = 0x80000000u;
 struct alignas(16) PairMeta {
-  uint64_t mask;
-  uint32_t ipos;  // sorted position of the entry
-  uint32_t n;     // pairs of the entry
-  uint32_t ids[META_IDS];
+  uint32_t w0, w1, w2, w3;
 };
 
-// Closed overlap test of two shifted AABBs (Broadphase.cu:237 / :296).  The
-// comparisons are combined without short-circuit evaluation: a && chain
-// would issue six dependent LDS reads, each waited for before the next.
 template <class R> __device__ __forceinline__ bool overlaps(const BpRec<R>& a, const BpRec<R>& b) {
   return (a.lo[0] <= b.hi[0]) & (b.lo[0] <= a.hi[0]) & (a.lo[1] <= b.hi[1]) & (b.lo[1] <= a.hi[1]) &
          (a.lo[2] <= b.hi[2]) & (b.lo[2] <= a.hi[2]);
@@ -651,11 +649,17 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   }
   pcount[j] = cnt;
   PairMeta pm;
-  pm.mask = mk;
-  pm.ipos = (uint32_t)i;
-  pm.n = cnt;
-#pragma unroll
-  for (int q = 0; q < META_IDS; ++q) pm.ids[q] = sids[tid][q];
+  if (cnt <= (uint32_t)META_IDS && !(mk & MASK_TAIL)) {
+    pm.w0 = cnt;
+    pm.w1 = sids[tid][0];
+    pm.w2 = sids[tid][1];
+    pm.w3 = sids[tid][2];
+  } else {
+    pm.w0 = cnt | META_MASK_FORM;
+    pm.w1 = (uint32_t)i;
+    pm.w2 = (uint32_t)mk;
+    pm.w3 = (uint32_t)(mk >> 32);
+  }
   meta[j] = pm;
 }
 
@@ -731,20 +735,21 @@ __device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_
   for (uint64_t e = e0; e < e1; ++e) {
     if (!pcount[e]) continue;
     const PairMeta pm = meta[e];
-    if (pm.n <= (uint32_t)META_IDS && !(pm.mask & MASK_TAIL)) {
-#pragma unroll
-      for (int q = 0; q < META_IDS; ++q)
-        if ((uint32_t)q < pm.n) put(pm.ids[q]);
+    if (!(pm.w0 & META_MASK_FORM)) {
+      if (pm.w0 > 0) put(pm.w1);
+      if (pm.w0 > 1) put(pm.w2);
+      if (pm.w0 > 2) put(pm.w3);
       continue;
     }
-    const uint64_t ip = pm.ipos;
-    uint64_t bits = pm.mask & ~MASK_TAIL;
+    const uint64_t ip = pm.w1;
+    const uint64_t mask = ((uint64_t)pm.w3 << 32) | pm.w2;
+    uint64_t bits = mask & ~MASK_TAIL;
     while (bits) {
       const int dd = __ffsll((unsigned long long)bits) - 1;
       bits &= bits - 1;
       put((uint32_t)keys[ip + 1 + dd]);
     }
-    if (pm.mask & MASK_TAIL) {
+    if (mask & MASK_TAIL) {
       const uint64_t ki = keys[ip];
       EntryCtx<R> c;
       entry_ctx(c, (uint32_t)(ki >> 32), a, brec[a], desc);
