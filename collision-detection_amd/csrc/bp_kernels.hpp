@@ -324,44 +324,6 @@ __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ c
   }
 }
 
-// Active bins (reduce_by_key, Broadphase.cu:439-443): the number of runs of
-// equal keys.  Each thread counts the run starts among RUNS_IPT consecutive
-// entries per pass (independent loads in flight); a fixed grid of RUNS_GRID
-// workgroups strides over the entries, so only RUNS_GRID atomics meet at the
-// one counter (same-address atomics are serialised device-wide).
-constexpr int RUNS_BS = 256;
-constexpr int RUNS_IPT = 16;
-constexpr unsigned RUNS_GRID = 256;
-
-__global__ __launch_bounds__(RUNS_BS) void k_count_runs(uint64_t mcap, const uint64_t* __restrict__ keys,
-                                                        DevStatus* st) {
-  __shared__ uint32_t red[RUNS_BS / WAVE];
-  const uint64_t m = cap_count(mcap, &st->M);
-  uint32_t c = 0;
-  for (uint64_t b0 = ((uint64_t)blockIdx.x * RUNS_BS + threadIdx.x) * RUNS_IPT; b0 < m;
-       b0 += (uint64_t)gridDim.x * RUNS_BS * RUNS_IPT) {
-    uint32_t k[RUNS_IPT + 1];
-#pragma unroll
-    for (int i = 0; i <= RUNS_IPT; ++i) {
-      const uint64_t e = b0 + (uint64_t)i - 1;  // e = b0 - 1 for i = 0
-      k[i] = (i == 0 && b0 == 0) || e >= m ? ~0u : (uint32_t)(keys[e] >> 32);
-    }
-#pragma unroll
-    for (int i = 1; i <= RUNS_IPT; ++i) {
-      const uint64_t e = b0 + (uint64_t)i - 1;
-      c += (e < m && (e == 0 || k[i] != k[i - 1])) ? 1u : 0u;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o);
-  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < RUNS_BS / WAVE; ++w) t += red[w];
-    if (t) atomicAdd((unsigned long long*)&st->A, (unsigned long long)t);
-  }
-}
-
 // Position, in the emission order of k_emit_keys (x outer, z inner), of the
 // cell with scene-relative key kk among the cells c of one shape.  A max
 // corner at index == grid size makes up to four cells share a key
@@ -499,7 +461,8 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
                                                    const BpRec<R>* __restrict__ brec,
                                                    const SceneDesc<R>* __restrict__ desc,
                                                    const uint32_t* __restrict__ offs, uint32_t* __restrict__ pcount,
-                                                   PairMeta* __restrict__ meta, DevStatus* st) {
+                                                   PairMeta* __restrict__ meta, uint32_t* __restrict__ runs_part,
+                                                   DevStatus* st) {
   __shared__ BpRec<R> srec[PAIR_W];
   __shared__ uint32_t skey[PAIR_W], sbody[PAIR_W];
   __shared__ uint64_t send[PAIR_W / WAVE];
@@ -510,6 +473,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   __shared__ uint32_t scan_scratch[PAIR_BS / WAVE + 1];
   __shared__ SceneDesc<R> sdesc;  // the window's scene, when it has only one
   __shared__ int sone;
+  __shared__ uint32_t sruns[PAIR_BS / WAVE];
   const uint64_t m = cap_count(mcap, &st->M);
   const uint64_t b0 = (uint64_t)blockIdx.x * PAIR_BS;
   if (b0 >= m) return;
@@ -544,9 +508,18 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   const uint64_t i = b0 + tid;
   bool dup = false, beyond = false;
   uint32_t n_i = 0;
+  bool run_start = false;  // the entry opens a run of equal keys (an active bin)
   if (valid) {
-    if (tid > 0) dup = skey[tid - 1] == skey[tid] && sbody[tid - 1] == sbody[tid];
-    else if (i > 0) dup = keys[i - 1] == (((uint64_t)skey[0] << 32) | sbody[0]);
+    if (tid > 0) {
+      dup = skey[tid - 1] == skey[tid] && sbody[tid - 1] == sbody[tid];
+      run_start = skey[tid - 1] != skey[tid];
+    } else if (i > 0) {
+      const uint64_t prev = keys[i - 1];
+      dup = prev == (((uint64_t)skey[0] << 32) | sbody[0]);
+      run_start = (uint32_t)(prev >> 32) != skey[0];
+    } else {
+      run_start = true;
+    }
     int w = tid / WAVE;
     uint64_t fm = send[w] & (~0ull << (tid % WAVE));
     while (!fm && ++w < PAIR_W / WAVE) fm = send[w];
@@ -556,10 +529,20 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
       beyond = u == nloc - 1 && b0 + nloc < m && (uint32_t)(keys[b0 + nloc] >> 32) == skey[tid];
     }
   }
+  {
+    const uint64_t rb = __ballot(run_start);
+    if (lane == 0) sruns[tid / WAVE] = (uint32_t)__popcll(rb);
+  }
   uint32_t total;
-  const uint32_t my_start = block_excl_scan<PAIR_BS>(n_i, scan_scratch, &total);
+  const uint32_t my_start = block_excl_scan<PAIR_BS>(n_i, scan_scratch, &total);  // synchronises: sruns complete
   sstart[tid] = my_start;
   if (tid == PAIR_BS - 1) sstart[PAIR_BS] = total;
+  if (tid == 0) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int w = 0; w < PAIR_BS / WAVE; ++w) r += sruns[w];
+    runs_part[blockIdx.x] = r;
+  }
   smask[tid] = 0;
   scnt[tid] = 0;
   __syncthreads();
@@ -661,6 +644,32 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     pm.w3 = (uint32_t)(mk >> 32);
   }
   meta[j] = pm;
+}
+
+// Active bins (reduce_by_key, Broadphase.cu:439-443): the pair count pass
+// counts the run starts among each workgroup's own entries; this sums those
+// counts.  A fixed grid of RUNS_GRID workgroups strides over them, so only
+// RUNS_GRID atomics meet at the one counter (same-address atomics are
+// serialised device-wide).
+constexpr int RUNS_BS = 256;
+constexpr unsigned RUNS_GRID = 64;
+
+__global__ __launch_bounds__(RUNS_BS) void k_sum_runs(uint64_t mcap, const uint32_t* __restrict__ runs_part,
+                                                      DevStatus* st) {
+  __shared__ uint32_t red[RUNS_BS / WAVE];
+  const uint64_t m = cap_count(mcap, &st->M);
+  const uint64_t nb = (m + PAIR_BS - 1) / PAIR_BS;
+  uint64_t c = 0;
+  for (uint64_t b = (uint64_t)blockIdx.x * RUNS_BS + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * RUNS_BS)
+    c += runs_part[b];
+  for (int o = 32; o > 0; o >>= 1) c += (uint64_t)__shfl_xor((long long)c, o);
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = (uint32_t)c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < RUNS_BS / WAVE; ++w) t += red[w];
+    if (t) atomicAdd((unsigned long long*)&st->A, (unsigned long long)t);
+  }
 }
 
 // Pairs per shape: the sum of its entries' counts.

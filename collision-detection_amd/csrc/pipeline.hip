@@ -8,7 +8,7 @@
 //   scan #1      Broadphase.cu:403-407     -> scan_excl_u32 over chunk slots
 //   key store    Broadphase.cu:172-209     -> k_emit_keys
 //   sort_by_key  Broadphase.cu:428-432     -> radix_sort_u64 on (key<<32 | shape)
-//   reduce_by_key + scan #2  :439-480      -> k_count_runs (run boundaries read inside k_pairs)
+//   reduce_by_key + scan #2  :439-480      -> run starts counted in k_pairs, summed by k_sum_runs
 //   pair count / store / sort / unique :484-535
 //                                          -> k_pairs (owner-bin dedup), radix_sort_u64
 //   MPR store    Narrowphase.cu:316-504    -> k_np_front, class sort, k_np_filter, k_np_resolve
@@ -397,7 +397,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_,
-                           &scan_tmp_, &radix_tmp_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_};
+                           &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -936,6 +936,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     keys_tmp_.reserve(mcap_ * 8);
     pcount_.reserve((mcap_ + 1) * 4);
     meta_.reserve(mcap_ * sizeof(PairMeta));
+    runs_part_.reserve((mcap_ / PAIR_BS + 1) * 4);
     mark(ST_BP_EMIT, true);
     hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
@@ -949,18 +950,18 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        s_, &st()->M))
       swap_bufs(keys_, keys_tmp_);
     mark(ST_BP_SORT, false);
-    mark(ST_BP_RUNS, true);
-    hipLaunchKernelGGL(k_count_runs, dim3(std::min<unsigned>(blocks_for(mcap_, RUNS_BS * RUNS_IPT), RUNS_GRID)),
-                       dim3(RUNS_BS), 0, s_, mcap_,
-                       (const uint64_t*)keys_.as<uint64_t>(), st());
-    mark(ST_BP_RUNS, false);
 
     mark(ST_BP_PAIRS, true);
     hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
-                       (const uint32_t*)offs_.as<uint32_t>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(), st());
+                       (const uint32_t*)offs_.as<uint32_t>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(),
+                       runs_part_.as<uint32_t>(), st());
     mark(ST_BP_PAIRS, false);
+    mark(ST_BP_RUNS, true);
+    hipLaunchKernelGGL(k_sum_runs, dim3(RUNS_GRID), dim3(RUNS_BS), 0, s_, mcap_,
+                       (const uint32_t*)runs_part_.as<uint32_t>(), st());
+    mark(ST_BP_RUNS, false);
     mark(ST_BP_PAIRSCAN, true);
     hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
@@ -1192,6 +1193,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        4);
     pcount_.reserve((mcap_ + 1) * 4);
     meta_.reserve(mcap_ * sizeof(PairMeta));
+    runs_part_.reserve((mcap_ / PAIR_BS + 1) * 4);
     pairs_.reserve(P * 8);
     longs_.reserve((P / (GROUP_LONG + 1) + 1) * 4);
     np_v_.reserve(P * 10 * sizeof(R));
@@ -1298,7 +1300,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;
   DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
-  DevBuf scan_tmp_, radix_tmp_;
+  DevBuf scan_tmp_, radix_tmp_, runs_part_;
   bool opened_ = false;  // the step's totals were zeroed by one k_stage_begin
 };
 
