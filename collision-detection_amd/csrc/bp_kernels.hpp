@@ -672,20 +672,40 @@ __global__ __launch_bounds__(RUNS_BS) void k_sum_runs(uint64_t mcap, const uint3
   }
 }
 
-// Pairs per shape: the sum of its entries' counts.
+// Pairs per shape: the sum of its entries' counts.  A chunk's entries are
+// contiguous, so the workgroup first reads their counts in one coalesced
+// sweep into LDS (one memory latency), then each shape sums its own.
+constexpr int SHAPE_PAIRS_LDS = 4096;
+
 __global__ __launch_bounds__(CHUNK) void k_shape_pairs(const Chunk* __restrict__ chunks,
                                                        const DevStatus* __restrict__ st,
                                                        const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ offs,
                                                        const uint32_t* __restrict__ pcount, uint64_t mcap,
                                                        uint32_t* __restrict__ scount) {
+  __shared__ uint32_t sp[SHAPE_PAIRS_LDS];
+  __shared__ uint64_t s_e0, s_e1;
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
   if ((uint64_t)blockIdx.x * CHUNK >= st->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
+  if (ch.len == 0) {
+    scount[slot] = 0;
+    return;
+  }
+  const uint64_t m = cap_count(mcap, &st->M);
+  const bool mine = threadIdx.x < ch.len;
+  const uint64_t e0 = mine ? offs[slot] : 0, e1 = mine ? min(e0 + counts[slot], m) : 0;
+  if (threadIdx.x == 0) s_e0 = min(e0, m);
+  if (threadIdx.x == ch.len - 1) s_e1 = e1;
+  __syncthreads();
+  const uint64_t c0 = s_e0, n = s_e1 > c0 ? s_e1 - c0 : 0;
   uint32_t sum = 0;
-  if (threadIdx.x < ch.len) {
-    const uint64_t m = cap_count(mcap, &st->M);
-    const uint64_t e0 = offs[slot], e1 = min(e0 + counts[slot], m);
+  if (n <= (uint64_t)SHAPE_PAIRS_LDS) {
+    for (uint64_t t = threadIdx.x; t < n; t += CHUNK) sp[t] = pcount[c0 + t];
+    __syncthreads();
+    if (mine)
+      for (uint64_t e = e0; e < e1; ++e) sum += sp[e - c0];
+  } else if (mine) {
     for (uint64_t e = e0; e < e1; ++e) sum += pcount[e];
   }
   scount[slot] = sum;
