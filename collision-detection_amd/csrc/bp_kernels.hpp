@@ -757,12 +757,12 @@ __device__ __forceinline__ void sort_lds_group(uint32_t* __restrict__ v, uint32_
 
 // Partners of one shape from the hit masks of its entries (the rest of runs
 // longer than the mask by repeating the tests), appended through put(b).
-template <class R, class Put>
-__device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_t a, const uint32_t* __restrict__ pcount,
+template <class R, class PC, class Put>
+__device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_t a, PC pcount,
                                                const PairMeta* __restrict__ meta, const uint64_t* __restrict__ keys,
                                                const BpRec<R>* __restrict__ brec, const SceneDesc<R>* __restrict__ desc, uint64_t m, Put put) {
   for (uint64_t e = e0; e < e1; ++e) {
-    if (!pcount[e]) continue;
+    if (!pcount(e)) continue;
     const PairMeta pm = meta[e];
     if (!(pm.w0 & META_MASK_FORM)) {
       if (pm.w0 > 0) put(pm.w1);
@@ -796,6 +796,7 @@ __device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_
 }
 
 constexpr int EMIT_PAIRS_LDS = 4096;
+constexpr int EMIT_COUNTS_LDS = 2048;
 
 // Writes each shape's pairs (a << 32 | b) into its range, sorted by b.  A
 // workgroup handles one chunk: partners go to LDS, each shape's group is
@@ -816,16 +817,24 @@ __global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ c
                                                      uint32_t* __restrict__ longs, uint64_t long_cap, DevStatus* st) {
   __shared__ uint32_t sb[EMIT_PAIRS_LDS];
   __shared__ uint8_t sa[EMIT_PAIRS_LDS];
-  __shared__ uint64_t s_span;
+  __shared__ uint32_t spc[EMIT_COUNTS_LDS];  // the chunk's entry counts, read in one coalesced sweep
+  __shared__ uint64_t s_span, s_e0, s_e1;
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
+  if (ch.len == 0) return;
   const uint64_t slot0 = (uint64_t)blockIdx.x * CHUNK;
   const uint64_t p0 = soff[slot0];
   const bool mine = threadIdx.x < ch.len;
   const uint64_t slot = slot0 + threadIdx.x;
   const uint32_t tot = mine ? scount[slot] : 0;
   const uint64_t beg = mine ? soff[slot] : 0;
-  if (threadIdx.x == ch.len - 1) s_span = beg + tot - p0;
+  const uint64_t m = cap_count(mcap, &st_in->M);
+  const uint64_t e0 = mine ? offs[slot] : 0, e1 = mine ? min(e0 + counts[slot], m) : 0;
+  if (threadIdx.x == 0) s_e0 = min(e0, m);
+  if (threadIdx.x == ch.len - 1) {
+    s_span = beg + tot - p0;
+    s_e1 = e1;
+  }
   __syncthreads();
   const uint64_t span = s_span;
   if (p0 + span > pcap) {
@@ -835,14 +844,22 @@ __global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ c
   if (span == 0) return;
   const bool staged = span <= EMIT_PAIRS_LDS;
   const uint32_t a = ch.begin + threadIdx.x;
-  const uint64_t m = cap_count(mcap, &st_in->M);
+  const uint64_t c0 = s_e0, nc = s_e1 > c0 ? s_e1 - c0 : 0;
+  const bool counts_staged = staged && nc <= (uint64_t)EMIT_COUNTS_LDS;
+  if (counts_staged) {
+    for (uint64_t t = threadIdx.x; t < nc; t += CHUNK) spc[t] = pcount[c0 + t];
+    __syncthreads();
+  }
   if (mine && tot) {
-    const uint64_t e0 = offs[slot], e1 = min(e0 + counts[slot], m);
     uint64_t o = beg;
     if (staged) {
       uint32_t* v = sb + (beg - p0);
       uint32_t q = 0;
-      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, desc, m, [&](uint32_t b) { v[q++] = b; });
+      auto put = [&](uint32_t b) { v[q++] = b; };
+      if (counts_staged)
+        shape_partners<R>(e0, e1, a, [&](uint64_t e) { return spc[e - c0]; }, meta, keys, brec, desc, m, put);
+      else
+        shape_partners<R>(e0, e1, a, [&](uint64_t e) { return pcount[e]; }, meta, keys, brec, desc, m, put);
       for (uint32_t t = 0; t < tot; ++t) sa[beg - p0 + t] = (uint8_t)threadIdx.x;
       if (tot <= 4) sort_lds_group<4>(v, tot);
       else if (tot <= 8) sort_lds_group<8>(v, tot);
@@ -854,7 +871,7 @@ __global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ c
         else atomicOr(&st->overflow, 4u);
       }
     } else {
-      shape_partners<R>(e0, e1, a, pcount, meta, keys, brec, desc, m,
+      shape_partners<R>(e0, e1, a, [&](uint64_t e) { return pcount[e]; }, meta, keys, brec, desc, m,
                         [&](uint32_t b) { pairs[o++] = ((uint64_t)a << 32) | b; });
       if (tot <= 4) sort_pairs_in_place<4>(pairs, beg, tot, a);
       else if (tot <= 8) sort_pairs_in_place<8>(pairs, beg, tot, a);
