@@ -23,6 +23,36 @@ __device__ __forceinline__ uint64_t eff_n(uint64_t cap, const uint64_t* d_n) {
   return v < cap ? v : cap;
 }
 
+// SCAN_IPT consecutive uint32 of one thread: four 16-byte accesses when the
+// run is whole and aligned, else one element at a time (the tile's tail).
+__device__ __forceinline__ void load_run(const uint32_t* __restrict__ in, uint64_t base, uint64_t n,
+                                         uint32_t (&v)[SCAN_IPT]) {
+  if (base + SCAN_IPT <= n && (((uintptr_t)(in + base)) & 15) == 0) {
+    const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT / 4; ++q) {
+      const uint4 x = p[q];
+      v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) v[i] = base + i < n ? in[base + i] : 0u;
+  }
+}
+
+__device__ __forceinline__ void store_run(uint32_t* __restrict__ out, uint64_t base, uint64_t n,
+                                          const uint32_t (&r)[SCAN_IPT]) {
+  if (base + SCAN_IPT <= n && (((uintptr_t)(out + base)) & 15) == 0) {
+    uint4* p = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT / 4; ++q) p[q] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i)
+      if (base + i < n) out[base + i] = r[i];
+  }
+}
+
 // Per-tile sums (64-bit, so a tile of large counts cannot wrap).
 __global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t cap,
                                                          const uint64_t* __restrict__ d_n, uint64_t* __restrict__ part) {
@@ -86,22 +116,20 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const uint32_t* __restri
   const uint64_t n = eff_n(cap, d_n);
   uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
   uint32_t v[SCAN_IPT];
+  load_run(in, base, n, v);
   uint32_t s = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN_IPT; ++i) {
-    uint64_t e = base + i;
-    v[i] = e < n ? in[e] : 0u;
-    s += v[i];
-  }
+  for (int i = 0; i < SCAN_IPT; ++i) s += v[i];
   uint32_t tot;
   uint32_t pre = block_excl_scan<SCAN_BS>(s, scratch, &tot);
   uint32_t run = (uint32_t)part[blockIdx.x] + pre;
 #pragma unroll
   for (int i = 0; i < SCAN_IPT; ++i) {
-    uint64_t e = base + i;
-    if (e < n) out[e] = run;
-    run += v[i];
+    const uint32_t x = v[i];
+    v[i] = run;
+    run += x;
   }
+  store_run(out, base, n, v);
 }
 
 // Tile scan that also forms its own tile offset: the sum of the preceding
@@ -126,13 +154,10 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* _
   if (blockIdx.x >= nb) return;
   uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
   uint32_t v[SCAN_IPT];
+  load_run(in, base, n, v);
   uint32_t s = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN_IPT; ++i) {
-    uint64_t e = base + i;
-    v[i] = e < n ? in[e] : 0u;
-    s += v[i];
-  }
+  for (int i = 0; i < SCAN_IPT; ++i) s += v[i];
   uint64_t pre_tiles = 0;
   for (uint64_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BS) pre_tiles += part[i];
   pre_tiles = wave_sum_u64(pre_tiles);
@@ -145,10 +170,11 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* _
   uint32_t run = (uint32_t)offset + pre;
 #pragma unroll
   for (int i = 0; i < SCAN_IPT; ++i) {
-    uint64_t e = base + i;
-    if (e < n) out[e] = run;
-    run += v[i];
+    const uint32_t x = v[i];
+    v[i] = run;
+    run += x;
   }
+  store_run(out, base, n, v);
   if (blockIdx.x == nb - 1 && threadIdx.x == 0) *d_total = offset + part[blockIdx.x];
 }
 
