@@ -60,10 +60,19 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restr
   const uint64_t n = eff_n(cap, d_n);
   uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
   uint64_t s = 0;
+  if (base + SCAN_TILE <= n && (((uintptr_t)(in + base)) & 15) == 0) {
+    const uint4* p = reinterpret_cast<const uint4*>(in + base);
 #pragma unroll
-  for (int i = 0; i < SCAN_IPT; ++i) {
-    uint64_t e = base + (uint64_t)i * SCAN_BS + threadIdx.x;
-    if (e < n) s += in[e];
+    for (int q = 0; q < SCAN_IPT / 4; ++q) {
+      const uint4 x = p[q * SCAN_BS + threadIdx.x];
+      s += (uint64_t)x.x + x.y + x.z + x.w;
+    }
+  } else if (base < n) {
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) {
+      uint64_t e = base + (uint64_t)i * SCAN_BS + threadIdx.x;
+      if (e < n) s += in[e];
+    }
   }
   s = wave_sum_u64(s);
   if (threadIdx.x % WAVE == 0) ws[threadIdx.x / WAVE] = s;
