@@ -85,36 +85,36 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restr
 }
 
 // Exclusive scan of the tile sums by one block; writes the grand total.
+// Each of the 1024 threads owns a contiguous slice of the sums: it adds its
+// slice, one block scan gives the slice offsets, and it rewrites its slice.
 __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t* __restrict__ part, uint64_t nb,
                                                         uint64_t* __restrict__ d_total) {
   __shared__ uint64_t ws[1024 / WAVE + 1];
-  __shared__ uint64_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint64_t b0 = 0; b0 < nb; b0 += 1024) {
-    uint64_t i = b0 + threadIdx.x;
-    uint64_t v = i < nb ? part[i] : 0;
-    // wave inclusive scan in 64-bit via shuffles
-    uint64_t inc = v;
-    for (int o = 1; o < WAVE; o <<= 1) {
-      uint64_t t = __shfl_up(inc, o);
-      if ((int)(threadIdx.x % WAVE) >= o) inc += t;
-    }
-    if (threadIdx.x % WAVE == WAVE - 1) ws[threadIdx.x / WAVE] = inc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint64_t run = 0;
-      for (int w = 0; w < 1024 / WAVE; ++w) { uint64_t c = ws[w]; ws[w] = run; run += c; }
-      ws[1024 / WAVE] = run;
-    }
-    __syncthreads();
-    uint64_t c0 = carry;
-    if (i < nb) part[i] = c0 + ws[threadIdx.x / WAVE] + inc - v;
-    __syncthreads();
-    if (threadIdx.x == 0) carry = c0 + ws[1024 / WAVE];
-    __syncthreads();
+  const uint64_t per = (nb + 1023) / 1024;
+  const uint64_t b0 = min((uint64_t)threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+  uint64_t sum = 0;
+  for (uint64_t i = b0; i < b1; ++i) sum += part[i];
+  // block exclusive scan of the slice sums (64-bit)
+  uint64_t inc = sum;
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint64_t t = __shfl_up(inc, o);
+    if ((int)(threadIdx.x % WAVE) >= o) inc += t;
   }
-  if (threadIdx.x == 0) *d_total = carry;
+  if (threadIdx.x % WAVE == WAVE - 1) ws[threadIdx.x / WAVE] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t run = 0;
+    for (int w = 0; w < 1024 / WAVE; ++w) { const uint64_t c = ws[w]; ws[w] = run; run += c; }
+    ws[1024 / WAVE] = run;
+  }
+  __syncthreads();
+  uint64_t run = ws[threadIdx.x / WAVE] + inc - sum;
+  for (uint64_t i = b0; i < b1; ++i) {
+    const uint64_t c = part[i];
+    part[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == 0) *d_total = ws[1024 / WAVE];
 }
 
 // Tile scan: each thread owns SCAN_IPT consecutive elements.
