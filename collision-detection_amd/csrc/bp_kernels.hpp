@@ -65,6 +65,10 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
                                                 ShapeRec<R>* __restrict__ rec, ShapeHdr* __restrict__ hdr,
                                                 ShapeRecF* __restrict__ frec, uint32_t* __restrict__ bpart,
                                                 const SceneDesc<R>* __restrict__ desc, DevStatus* st) {
+  // the chunk's narrowphase records are built in LDS and leave in coalesced
+  // 16-byte stores (a record per thread would store at a 64-112 B stride)
+  __shared__ ShapeRec<R> lrec[CHUNK];
+  __shared__ ShapeRecF lfrec[CHUNK];
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
       r.qw = q1.w; r.qx = q1.x; r.qy = q1.y; r.qz = q1.z;
       r.bx = b.x; r.by = b.y; r.bz = b.z;
       r.type = (uint32_t)t; r.body = id; r.active = active[id]; r.pad = 0;
-      rec[i] = r;
+      lrec[threadIdx.x] = r;
       ShapeHdr h;
       h.ta = (uint32_t)t | ((uint32_t)r.active << 8);
       h.body = id;
@@ -114,11 +118,21 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
       fr.qw = (float)q1.w; fr.qx = (float)q1.x; fr.qy = (float)q1.y; fr.qz = (float)q1.z;
       fr.bx = (float)b.x; fr.by = (float)b.y; fr.bz = (float)b.z;
       fr.type = (uint32_t)t;
-      frec[i] = fr;
+      lfrec[threadIdx.x] = fr;
       bounds_fold(mn, mx, (float)lo.x, (float)lo.y, (float)lo.z, (float)hi.x, (float)hi.y, (float)hi.z);
     }
   }
-  block_bounds_commit(mn, mx, bpart);
+  block_bounds_commit(mn, mx, bpart);  // synchronises: the LDS records are complete
+  {
+    const uint32_t nr = ch.len * (uint32_t)(sizeof(ShapeRec<R>) / 16);
+    const uint4* src = reinterpret_cast<const uint4*>(lrec);
+    uint4* dst = reinterpret_cast<uint4*>(rec + ch.begin);
+    for (uint32_t t = threadIdx.x; t < nr; t += CHUNK) dst[t] = src[t];
+    const uint32_t nf = ch.len * (uint32_t)(sizeof(ShapeRecF) / 16);
+    const uint4* fsrc = reinterpret_cast<const uint4*>(lfrec);
+    uint4* fdst = reinterpret_cast<uint4*>(frec + ch.begin);
+    for (uint32_t t = threadIdx.x; t < nf; t += CHUNK) fdst[t] = fsrc[t];
+  }
 }
 
 // Bounds of AABBs supplied by the caller (Broadphase::detectPossibleCollisions
