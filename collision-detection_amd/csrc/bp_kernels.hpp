@@ -222,6 +222,7 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
                                                  DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   __shared__ uint32_t skm[CHUNK / WAVE];
+  __shared__ BpRec<R> lrec[CHUNK];  // the chunk's records, stored coalesced after the barrier
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
     c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
     c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
     r.scene = ch.scene;
-    brec[i] = r;
+    lrec[threadIdx.x] = r;
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
       counts[slot] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = d.key_base + hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, d.gx, d.gy);
@@ -258,6 +259,10 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
     for (int w = 0; w < CHUNK / WAVE; ++w) m = max(m, skm[w]);
     kpart[blockIdx.x] = m;
   }
+  const uint32_t nr = ch.len * (uint32_t)(sizeof(BpRec<R>) / 16);
+  const uint4* src = reinterpret_cast<const uint4*>(lrec);
+  uint4* dst = reinterpret_cast<uint4*>(brec + ch.begin);
+  for (uint32_t t = threadIdx.x; t < nr; t += CHUNK) dst[t] = src[t];
 }
 
 __global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
