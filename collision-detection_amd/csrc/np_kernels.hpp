@@ -521,30 +521,56 @@ __global__ __launch_bounds__(128) void k_np_resolve(uint64_t cap, const uint64_t
 }
 
 // Stable partition of hits to the front (Narrowphase.cu:580-593): contacts
-// in ascending pair order, and the pair list permuted hits first.
+// in ascending pair order, and the pair list permuted hits first.  A
+// workgroup's hits are consecutive contacts: they are gathered in LDS and the
+// interleaved contact arrays leave in coalesced stores.
+constexpr int COMPACT_BS = 256;
+
 template <class R>
-__global__ void k_np_compact(uint64_t cap, const uint64_t* __restrict__ d_np, PairOut<R> o,
-                             const uint32_t* __restrict__ pos,
-                             const uint64_t* __restrict__ pairs, const DevStatus* __restrict__ st, R* __restrict__ norm,
-                             R* __restrict__ cpta, R* __restrict__ cptb, R* __restrict__ dpth, int32_t* __restrict__ bids,
-                             uint64_t* __restrict__ perm) {
+__global__ __launch_bounds__(COMPACT_BS) void k_np_compact(uint64_t cap, const uint64_t* __restrict__ d_np,
+                                                           PairOut<R> o, const uint32_t* __restrict__ pos,
+                                                           const uint64_t* __restrict__ pairs,
+                                                           const DevStatus* __restrict__ st, R* __restrict__ norm,
+                                                           R* __restrict__ cpta, R* __restrict__ cptb,
+                                                           R* __restrict__ dpth, int32_t* __restrict__ bids,
+                                                           uint64_t* __restrict__ perm) {
+  __shared__ R lv[10][COMPACT_BS];
+  __shared__ int32_t lid[2 * COMPACT_BS];
+  __shared__ uint32_t s_k0, s_nh;
   const uint64_t P = o.P;
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= dev_count(cap, d_np)) return;
-  uint64_t C = st->C;
-  uint32_t k = pos[i];
-  if (o.hit[i]) {
-    const R* v = o.v;
-    norm[3 * k] = v[i]; norm[3 * k + 1] = v[P + i]; norm[3 * k + 2] = v[2 * P + i];
-    cpta[3 * k] = v[3 * P + i]; cpta[3 * k + 1] = v[4 * P + i]; cpta[3 * k + 2] = v[5 * P + i];
-    cptb[3 * k] = v[6 * P + i]; cptb[3 * k + 1] = v[7 * P + i]; cptb[3 * k + 2] = v[8 * P + i];
-    dpth[k] = v[9 * P + i];
-    bids[2 * k] = o.ids[2 * i];
-    bids[2 * k + 1] = o.ids[2 * i + 1];
-    perm[k] = pairs[i];
-  } else {
-    perm[C + (i - k)] = pairs[i];
+  const uint64_t n = dev_count(cap, d_np);
+  const uint64_t i0 = (uint64_t)blockIdx.x * COMPACT_BS;
+  if (i0 >= n) return;
+  const uint64_t i = i0 + threadIdx.x;
+  const uint64_t last = min(i0 + COMPACT_BS, n) - 1;
+  const uint64_t C = st->C;
+  if (threadIdx.x == 0) s_k0 = pos[i0];
+  if (i == last) s_nh = pos[i] + (o.hit[i] ? 1u : 0u);
+  __syncthreads();
+  const uint32_t k0 = s_k0, nh = s_nh - k0;
+  if (i < n) {
+    const uint32_t k = pos[i];
+    if (o.hit[i]) {
+      const uint32_t sl = k - k0;
+      const R* v = o.v;
+#pragma unroll
+      for (int c = 0; c < 10; ++c) lv[c][sl] = v[(uint64_t)c * P + i];
+      lid[2 * sl] = o.ids[2 * i];
+      lid[2 * sl + 1] = o.ids[2 * i + 1];
+      perm[k] = pairs[i];
+    } else {
+      perm[C + (i - k)] = pairs[i];
+    }
   }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < 3 * nh; t += COMPACT_BS) {
+    const uint32_t sl = t / 3, c = t - 3 * sl;
+    norm[3 * (uint64_t)k0 + t] = lv[c][sl];
+    cpta[3 * (uint64_t)k0 + t] = lv[3 + c][sl];
+    cptb[3 * (uint64_t)k0 + t] = lv[6 + c][sl];
+  }
+  for (uint32_t t = threadIdx.x; t < nh; t += COMPACT_BS) dpth[k0 + t] = lv[9][t];
+  for (uint32_t t = threadIdx.x; t < 2 * nh; t += COMPACT_BS) bids[2 * (uint64_t)k0 + t] = lid[t];
 }
 
 

@@ -1081,7 +1081,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     out_d_.reserve(P * sizeof(R));
     out_ids_.reserve(P * 8);
     perm_.reserve(P * 8);
-    hipLaunchKernelGGL(k_np_compact<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s_, P, dP, o,
+    hipLaunchKernelGGL(k_np_compact<R>, dim3(blocks_for(P, COMPACT_BS)), dim3(COMPACT_BS), 0, s_, P, dP, o,
                        (const uint32_t*)np_pos_.as<uint32_t>(), (const uint64_t*)pairs_.as<uint64_t>(),
                        (const DevStatus*)st(), out_norm_.as<R>(), out_pa_.as<R>(), out_pb_.as<R>(), out_d_.as<R>(),
                        out_ids_.as<int32_t>(), perm_.as<uint64_t>());
