@@ -10,7 +10,8 @@
 //   sort_by_key  Broadphase.cu:428-432     -> radix_sort_u64 on (key<<32 | shape)
 //   reduce_by_key + scan #2  :439-480      -> run starts counted in k_pairs, summed by k_sum_runs
 //   pair count / store / sort / unique :484-535
-//                                          -> k_pairs (owner-bin dedup), radix_sort_u64
+//                                          -> k_pairs (owner-bin dedup), k_shape_pairs + scan,
+//                                             k_pair_emit (per-shape sorted groups), k_pair_groups_long
 //   MPR store    Narrowphase.cu:316-504    -> k_np_front, class sort, k_np_filter, k_np_resolve
 //   compaction   Narrowphase.cu:580-593    -> scan + k_np_compact
 //
