@@ -118,6 +118,8 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
       fr.qw = (float)q1.w; fr.qx = (float)q1.x; fr.qy = (float)q1.y; fr.qz = (float)q1.z;
       fr.bx = (float)b.x; fr.by = (float)b.y; fr.bz = (float)b.z;
       fr.type = (uint32_t)t;
+      fr.body = id;
+      fr.active = r.active;
       lfrec[threadIdx.x] = fr;
       bounds_fold(mn, mx, (float)lo.x, (float)lo.y, (float)lo.z, (float)hi.x, (float)hi.y, (float)hi.z);
     }

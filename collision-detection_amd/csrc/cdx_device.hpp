@@ -80,6 +80,7 @@ struct alignas(16) ShapeRecF {
   float qw, qx, qy, qz;
   float bx, by, bz;
   uint32_t type;
+  uint32_t body, active;  // as in ShapeRec: the front pass of MIXED / FAST reads only this record
 };
 
 template <class R> struct alignas(16) ShapeRec {

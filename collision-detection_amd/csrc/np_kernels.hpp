@@ -406,22 +406,35 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
   if (i >= dev_count(cap, d_np)) return;
   uint64_t pr = pairs[i];
   const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
-  const ShapeHdr ha = hdr[ia], hb = hdr[ib];
-  const uint32_t ta = ha.ta & 255u, tb = hb.ta & 255u;
   o.hit[i] = 0u;
   bool enq = false;
-  if (((ha.ta | hb.ta) >> 8) && ha.body != hb.body) {
-    if (ta == SPHERE && tb == SPHERE) {
-      Contact<R> c;
-      ShapeRec<R> ra = rec[ia], rb = rec[ib];
-      if (contact_spheres(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
-    } else if (prefilter) {
-      const ShapeRecF fa = frec[ia], fb = frec[ib];
-      const PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
-      const float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
-      enq = !mpr_first_miss(A, B, tau);
-    } else {
-      enq = true;
+  uint32_t ta, tb;
+  auto spheres = [&]() {
+    Contact<R> c;
+    ShapeRec<R> ra = rec[ia], rb = rec[ib];
+    if (contact_spheres(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
+  };
+  if (prefilter) {
+    // the FP32 records carry type, body and activity: two gathers per pair
+    const ShapeRecF fa = frec[ia], fb = frec[ib];
+    ta = fa.type;
+    tb = fb.type;
+    if ((fa.active | fb.active) && fa.body != fb.body) {
+      if (ta == SPHERE && tb == SPHERE) {
+        spheres();
+      } else {
+        const PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
+        const float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
+        enq = !mpr_first_miss(A, B, tau);
+      }
+    }
+  } else {
+    const ShapeHdr ha = hdr[ia], hb = hdr[ib];
+    ta = ha.ta & 255u;
+    tb = hb.ta & 255u;
+    if (((ha.ta | hb.ta) >> 8) && ha.body != hb.body) {
+      if (ta == SPHERE && tb == SPHERE) spheres();
+      else enq = true;
     }
   }
   qsel[i] = enq ? 1u : 0u;
@@ -468,7 +481,7 @@ __global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t*
       c.p1 = mk3((R)q1.x, (R)q1.y, (R)q1.z);
       c.p2 = mk3((R)q2.x, (R)q2.y, (R)q2.z);
       c.depth = (R)dot(Nd, q2 - q1);
-      store_pair(o, P, i, c, hdr[ia].body, hdr[ib].body);
+      store_pair(o, P, i, c, fa.body, fb.body);
     }
   }
   // MIXED: FP32 settles clear misses only; hits take the reference path,
