@@ -212,37 +212,49 @@ __device__ __forceinline__ bool cells_ok(const Cells& c, int gx, int gy, int gz)
          c.y1 <= gy && c.z1 <= gz;
 }
 
+// Shifted AABB of shape i (coordinates plus the scene origin,
+// Broadphase.cu:382-386).  k_count and k_emit_keys both build it with the
+// same single additions, so their cells agree bit for bit.
+template <class R>
+__device__ __forceinline__ BpRec<R> shifted_rec(const R* __restrict__ aabb, uint64_t NS, uint64_t i,
+                                                const SceneDesc<R>& d, uint32_t scene) {
+  BpRec<R> r;
+  r.lo[0] = aabb[i] + d.origin[0];
+  r.lo[1] = aabb[2 * NS + i] + d.origin[1];
+  r.lo[2] = aabb[4 * NS + i] + d.origin[2];
+  r.hi[0] = aabb[NS + i] + d.origin[0];
+  r.hi[1] = aabb[3 * NS + i] + d.origin[1];
+  r.hi[2] = aabb[5 * NS + i] + d.origin[2];
+  r.scene = scene;
+  r.offs = 0;
+  return r;
+}
+
+template <class R> __device__ __forceinline__ Cells cells_of(const BpRec<R>& r, const SceneDesc<R>& d) {
+  Cells c;
+  c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
+  c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
+  return c;
+}
+
 // Bins spanned per AABB (Broadphase.cu:150-159), on coordinates shifted by
 // the scene origin (Broadphase.cu:382-386) and hashed by truncation
-// (HashMin, :125-133); plus the per-shape broadphase record and the largest
-// key (the max corner's, keys do not wrap in the dense key range).
+// (HashMin, :125-133); plus the largest key (the max corner's, keys do not
+// wrap in the dense key range).
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                  uint64_t NS, const R* __restrict__ aabb,
                                                  const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ counts,
-                                                 BpRec<R>* __restrict__ brec, uint32_t* __restrict__ kpart,
-                                                 DevStatus* st) {
+                                                 uint32_t* __restrict__ kpart, DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   __shared__ uint32_t skm[CHUNK / WAVE];
-  __shared__ BpRec<R> lrec[CHUNK];  // the chunk's records, stored coalesced after the barrier
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
   uint32_t km = 0;
   if (threadIdx.x < ch.len) {
     const SceneDesc<R>& d = desc[ch.scene];
-    BpRec<R> r;
-    r.lo[0] = aabb[i] + d.origin[0];
-    r.lo[1] = aabb[2 * NS + i] + d.origin[1];
-    r.lo[2] = aabb[4 * NS + i] + d.origin[2];
-    r.hi[0] = aabb[NS + i] + d.origin[0];
-    r.hi[1] = aabb[3 * NS + i] + d.origin[1];
-    r.hi[2] = aabb[5 * NS + i] + d.origin[2];
-    Cells c;
-    c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
-    c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
-    r.scene = ch.scene;
-    lrec[threadIdx.x] = r;
+    const Cells c = cells_of(shifted_rec(aabb, NS, i, d, ch.scene), d);
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
       counts[slot] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = d.key_base + hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, d.gx, d.gy);
@@ -261,10 +273,6 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
     for (int w = 0; w < CHUNK / WAVE; ++w) m = max(m, skm[w]);
     kpart[blockIdx.x] = m;
   }
-  const uint32_t nr = ch.len * (uint32_t)(sizeof(BpRec<R>) / 16);
-  const uint4* src = reinterpret_cast<const uint4*>(lrec);
-  uint4* dst = reinterpret_cast<uint4*>(brec + ch.begin);
-  for (uint32_t t = threadIdx.x; t < nr; t += CHUNK) dst[t] = src[t];
 }
 
 __global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
@@ -284,20 +292,26 @@ __global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restric
 // 172-200: x, then y, then z), packed as key << 32 | shape so one 64-bit
 // stable sort orders by bin and keeps shapes ascending inside a bin.  Keys
 // of a scene are offset by its key base, so the bins of all stepped scenes
-// sort into one array, scene after scene.
+// sort into one array, scene after scene.  The pass also writes each
+// shape's broadphase record (shifted AABB, scene, first entry position),
+// built in LDS and stored in coalesced 16-byte runs, so the pair search
+// finds a shape's emission range in the record it gathers anyway.
 constexpr int EMIT_LDS = 4096;  // entries of one chunk staged for coalesced stores
 
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
+                                                     uint64_t NS, const R* __restrict__ aabb,
                                                      const SceneDesc<R>* __restrict__ desc,
-                                                     const BpRec<R>* __restrict__ brec, const uint32_t* __restrict__ offs,
+                                                     const uint32_t* __restrict__ offs,
                                                      const uint32_t* __restrict__ counts, uint64_t mcap,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ pcount,
-                                                     DevStatus* st) {
-  __shared__ uint64_t sk[EMIT_LDS];
+                                                     BpRec<R>* __restrict__ brec, DevStatus* st) {
+  static_assert(sizeof(BpRec<R>) * CHUNK <= sizeof(uint64_t) * EMIT_LDS, "records fit the key staging area");
+  __shared__ alignas(64) uint64_t sk[EMIT_LDS];  // the chunk's entries, then its records
   __shared__ uint64_t s_total;
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
+  if (ch.len == 0) return;
   const uint64_t slot0 = (uint64_t)blockIdx.x * CHUNK;
   const uint64_t e0 = offs[slot0];
   const bool mine = threadIdx.x < ch.len;
@@ -306,13 +320,13 @@ __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ c
   __syncthreads();
   const uint64_t total = s_total;
   const bool staged = total <= EMIT_LDS;
+  const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
+  BpRec<R> r;
   if (mine) {
-    const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
     const SceneDesc<R>& d = desc[ch.scene];
-    const BpRec<R> r = brec[i];
-    Cells c;
-    c.x0 = cell_of(r.lo[0], d.inv[0]); c.y0 = cell_of(r.lo[1], d.inv[1]); c.z0 = cell_of(r.lo[2], d.inv[2]);
-    c.x1 = cell_of(r.hi[0], d.inv[0]); c.y1 = cell_of(r.hi[1], d.inv[1]); c.z1 = cell_of(r.hi[2], d.inv[2]);
+    r = shifted_rec(aabb, NS, i, d, ch.scene);
+    r.offs = (uint32_t)o0;
+    const Cells c = cells_of(r, d);
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
       uint64_t o = o0;
       for (uint32_t x = (uint32_t)c.x0; x <= (uint32_t)c.x1; ++x)
@@ -331,18 +345,27 @@ __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ c
           }
     }
   }
-  if (!staged) return;
-  __syncthreads();
-  // coalesced stores of the chunk's entries (and zeroed pair counts)
-  for (uint64_t t = threadIdx.x; t < total; t += CHUNK) {
-    const uint64_t o = e0 + t;
-    if (o < mcap) {
-      keys[o] = sk[t];
-      pcount[o] = 0;
-    } else {
-      atomicOr(&st->overflow, 1u);
+  if (staged) {
+    __syncthreads();
+    // coalesced stores of the chunk's entries (and zeroed pair counts)
+    for (uint64_t t = threadIdx.x; t < total; t += CHUNK) {
+      const uint64_t o = e0 + t;
+      if (o < mcap) {
+        keys[o] = sk[t];
+        pcount[o] = 0;
+      } else {
+        atomicOr(&st->overflow, 1u);
+      }
     }
   }
+  __syncthreads();  // the staging area now takes the records
+  BpRec<R>* lrec = reinterpret_cast<BpRec<R>*>(sk);
+  if (mine) lrec[threadIdx.x] = r;
+  __syncthreads();
+  const uint32_t nr = ch.len * (uint32_t)(sizeof(BpRec<R>) / 16);
+  const uint4* src = reinterpret_cast<const uint4*>(lrec);
+  uint4* dst = reinterpret_cast<uint4*>(brec + ch.begin);
+  for (uint32_t t = threadIdx.x; t < nr; t += CHUNK) dst[t] = src[t];
 }
 
 // Position, in the emission order of k_emit_keys (x outer, z inner), of the
@@ -381,13 +404,15 @@ __device__ __forceinline__ uint32_t local_cell_index(uint32_t kk, const Cells& c
 //
 // Count pass (k_pairs): each sorted entry i pairs shape(i) with the later
 // entries of its run (shapes ascend inside a run, so shape(i) < shape(k)).
-// A workgroup stages its 256 entries and the next PAIR_AHEAD with their
-// records in LDS and splits the (entry, later entry) candidates of its own
-// entries evenly over its lanes, so short and long runs cost the same per
-// lane.  Per entry it stores the pair count and a hit mask (bit d: the
-// entry d+1 places later is a partner; bit 63: partners beyond 62 places)
-// at the entry's emission position j (offs of its shape + its cell's index
-// among the shape's cells), so pairs can be gathered per first shape.
+// A workgroup stages its 256 entries and the next PAIR_AHEAD keys in LDS,
+// with the records of its own entries and of the look-ahead entries that
+// continue its last run, and splits the (entry, later entry) candidates of
+// its own entries evenly over its lanes, so short and long runs cost the
+// same per lane.  Per entry it stores the pair count and a hit mask (bit d:
+// the entry d+1 places later is a partner; bit 63: partners beyond 62
+// places) at the entry's emission position j (the first entry of its shape,
+// carried in the shape's record, + its cell's index among the shape's
+// cells), so pairs can be gathered per first shape.
 // k_shape_pairs sums each shape's counts, a scan gives each shape's pair
 // range, and k_pair_emit writes each shape's partners from the masks and
 // sorts them in registers: the pair list comes out ascending per scene
@@ -439,7 +464,6 @@ template <class R> struct EntryCtx {
   int gx, gy;
   uint64_t rgx, rgy;
   uint32_t kb, K, a;
-  uint64_t slot;  // chunk slot of the shape
 };
 
 template <class R>
@@ -459,7 +483,6 @@ __device__ __forceinline__ void entry_ctx_d(EntryCtx<R>& c, uint32_t K, uint32_t
     c.g0[k] = cell_of(r.lo[k], c.inv[k]);
     c.g1[k] = cell_of(r.hi[k], c.inv[k]);
   }
-  c.slot = (uint64_t)d.chunk_begin * CHUNK + (a - d.shape_begin);
 }
 
 template <class R>
@@ -480,8 +503,7 @@ __device__ __forceinline__ void entry_ctx_w(EntryCtx<R>& c, uint32_t K, uint32_t
 template <class R>
 __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t* __restrict__ keys,
                                                    const BpRec<R>* __restrict__ brec,
-                                                   const SceneDesc<R>* __restrict__ desc,
-                                                   const uint32_t* __restrict__ offs, uint32_t* __restrict__ pcount,
+                                                   const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ pcount,
                                                    PairMeta* __restrict__ meta, uint32_t* __restrict__ runs_part,
                                                    DevStatus* st) {
   __shared__ BpRec<R> srec[PAIR_W];
@@ -500,20 +522,31 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   if (b0 >= m) return;
   const int tid = threadIdx.x, lane = tid % WAVE;
   const int nloc = (int)min((uint64_t)PAIR_W, m - b0);
-  for (int t = tid; t < PAIR_W; t += PAIR_BS) {
-    if (t < nloc) {
-      const uint64_t kk = keys[b0 + t];
-      skey[t] = (uint32_t)(kk >> 32);
-      sbody[t] = (uint32_t)kk;
-      srec[t] = brec[(uint32_t)kk];
-    }
+  const int nown = min(nloc, PAIR_BS);
+  if (tid < nloc) {
+    const uint64_t kk = keys[b0 + tid];
+    skey[tid] = (uint32_t)(kk >> 32);
+    sbody[tid] = (uint32_t)kk;
+    srec[tid] = brec[(uint32_t)kk];
+  }
+  static_assert(PAIR_AHEAD <= PAIR_BS, "one look-ahead entry per thread");
+  if (tid < PAIR_AHEAD && PAIR_BS + tid < nloc) {
+    // A look-ahead entry is a candidate of an own entry only if it continues
+    // the run of the last own entry (keys ascend, so those entries come
+    // first); only their records are gathered.
+    const int t = PAIR_BS + tid;
+    const uint64_t kk = keys[b0 + t];
+    const uint32_t klast = (uint32_t)(keys[b0 + PAIR_BS - 1] >> 32);
+    skey[t] = (uint32_t)(kk >> 32);
+    sbody[t] = (uint32_t)kk;
+    if ((uint32_t)(kk >> 32) == klast) srec[t] = brec[(uint32_t)kk];
   }
   __syncthreads();
   if (tid == 0) {
     // keys ascend and scenes own disjoint key ranges: one scene at both ends
-    // means one scene throughout
+    // of the own entries means one scene for every staged record
     const uint32_t s0 = srec[0].scene;
-    sone = s0 == srec[nloc - 1].scene ? 1 : 0;
+    sone = s0 == srec[nown - 1].scene ? 1 : 0;
     if (sone) sdesc = desc[s0];
   }
   // run ends inside the window (the last staged entry always ends one)
@@ -646,7 +679,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   cl.x0 = c.g0[0]; cl.y0 = c.g0[1]; cl.z0 = c.g0[2];
   cl.x1 = c.g1[0]; cl.y1 = c.g1[1]; cl.z1 = c.g1[2];
   const uint32_t li = local_cell_index(c.K - c.kb, cl, c.gx, c.gy, c.rgx, c.rgy);
-  const uint64_t j = (uint64_t)offs[c.slot] + li;
+  const uint64_t j = (uint64_t)c.r.offs + li;
   if (li == ~0u || j >= m) {
     atomicOr(&st->bad, 8u);
     return;

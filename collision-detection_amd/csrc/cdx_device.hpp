@@ -90,15 +90,17 @@ template <class R> struct alignas(16) ShapeRec {
   uint32_t type, body, active, pad;
 };
 
-// Per-shape broadphase record: the shifted AABB and the shape's scene,
-// gathered as one aligned block by the pair search (32 B in FP32, 64 B in
-// FP64: one gather never straddles more lines than it must; cells are
-// recomputed from it, one multiply and truncation per axis, bit-identical
-// to k_count's).
+// Per-shape broadphase record: the shifted AABB, the shape's scene and the
+// position of its first bin entry in emission order, gathered as one
+// aligned block by the pair search (32 B in FP32, 64 B in FP64: one gather
+// never straddles more lines than it must; cells are recomputed from it,
+// one multiply and truncation per axis, bit-identical to k_count's).
 template <class R> struct alignas(sizeof(R) == 4 ? 32 : 64) BpRec {
   R lo[3], hi[3];
   uint32_t scene;
+  uint32_t offs;
 };
+static_assert(sizeof(BpRec<float>) == 32 && sizeof(BpRec<double>) == 64, "record sizes");
 
 inline unsigned blocks_for(uint64_t n, unsigned bs) {
   uint64_t b = (n + bs - 1) / bs;

@@ -917,7 +917,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_BP_COUNT, true);
     hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(), brec_.as<BpRec<R>>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(),
                        kpart_.as<uint32_t>(), st());
     hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
     mark(ST_BP_COUNT, false);
@@ -940,10 +940,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     runs_part_.reserve((mcap_ / PAIR_BS + 1) * 4);
     mark(ST_BP_EMIT, true);
     hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
-                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const uint32_t*)offs_.as<uint32_t>(), (const uint32_t*)counts_buf_.as<uint32_t>(), mcap_,
-                       keys_.as<uint64_t>(), pcount_.as<uint32_t>(), st());
+                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (const uint32_t*)offs_.as<uint32_t>(),
+                       (const uint32_t*)counts_buf_.as<uint32_t>(), mcap_, keys_.as<uint64_t>(),
+                       pcount_.as<uint32_t>(), brec_.as<BpRec<R>>(), st());
     mark(ST_BP_EMIT, false);
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits), 16) * 4);
     mark(ST_BP_SORT, true);
@@ -955,8 +955,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_BP_PAIRS, true);
     hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(),
-                       (const uint32_t*)offs_.as<uint32_t>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(),
                        runs_part_.as<uint32_t>(), st());
     mark(ST_BP_PAIRS, false);
     mark(ST_BP_RUNS, true);
