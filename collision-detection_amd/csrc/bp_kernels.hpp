@@ -510,9 +510,8 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   __shared__ uint32_t skey[PAIR_W], sbody[PAIR_W];
   __shared__ uint64_t send[PAIR_W / WAVE];
   __shared__ uint32_t sstart[PAIR_BS + 1];
-  __shared__ uint64_t smask[PAIR_BS];
-  __shared__ uint32_t scnt[PAIR_BS];
-  __shared__ uint32_t sids[PAIR_BS][META_IDS];
+  __shared__ uint64_t smask[PAIR_BS];  // hit mask per own entry, set by LDS atomics
+  __shared__ uint32_t scnt[PAIR_BS];   // partners beyond 62 places per own entry
   __shared__ uint32_t scan_scratch[PAIR_BS / WAVE + 1];
   __shared__ SceneDesc<R> sdesc;  // the window's scene, when it has only one
   __shared__ int sone;
@@ -615,22 +614,8 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
       EntryCtx<R> c;
       entry_ctx_w(c, skey[e], sbody[e], srec[e], one, sdesc, desc);
       uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
-      uint64_t mk = 0;
-      uint32_t cnt = 0;
-      uint32_t loc[META_IDS];
-      auto flush = [&]() {
-        if (!cnt) return;
-        atomicOr((unsigned long long*)&smask[e], (unsigned long long)mk);
-        const uint32_t at = atomicAdd(&scnt[e], cnt);
-#pragma unroll
-        for (int q = 0; q < META_IDS; ++q)
-          if ((uint32_t)q < cnt && at + q < (uint32_t)META_IDS) sids[e][at + q] = loc[q];
-      };
       for (; f < fend; ++f) {
         if (f >= eend) {
-          flush();
-          mk = 0;
-          cnt = 0;
           do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
           entry_ctx_w(c, skey[e], sbody[e], srec[e], one, sdesc, desc);
           prev = c.a;
@@ -643,21 +628,24 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
         const bool fresh = b != prev;
         prev = b;
         if ((fresh & overlaps(c.r, rk)) && owner_bin(c.g0, rk, c.inv, c.K, c.kb, c.gx, c.gy)) {
-#pragma unroll
-          for (int q = 0; q < META_IDS; ++q)
-            if ((uint32_t)q == cnt) loc[q] = b;
-          ++cnt;
+          // a hit goes straight to the entry's mask: lanes sharing an entry
+          // need no merge step, and the partners are read back from the
+          // mask bits and the staged bodies
           const int dd = k - e - 1;
-          mk |= dd < 63 ? (1ull << dd) : MASK_TAIL;
+          if (dd < 63) {
+            atomicOr((unsigned long long*)&smask[e], 1ull << dd);
+          } else {
+            atomicOr((unsigned long long*)&smask[e], (unsigned long long)MASK_TAIL);
+            atomicAdd(&scnt[e], 1u);
+          }
         }
       }
-      flush();
     }
   }
   __syncthreads();
   if (!valid || dup) return;
   uint64_t mk = smask[tid];
-  uint32_t cnt = scnt[tid];
+  uint32_t cnt = (uint32_t)__popcll(mk & ~MASK_TAIL) + scnt[tid];
   EntryCtx<R> c;
   entry_ctx_w(c, skey[tid], sbody[tid], srec[tid], one, sdesc, desc);
   if (beyond) {  // run continues past the window: the rest from global memory (rare)
@@ -687,10 +675,18 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   pcount[j] = cnt;
   PairMeta pm;
   if (cnt <= (uint32_t)META_IDS && !(mk & MASK_TAIL)) {
+    // partners in ascending position: the lowest set bits of the mask
+    uint32_t id[META_IDS];
+    uint64_t bits = mk;
+#pragma unroll
+    for (int q = 0; q < META_IDS; ++q) {
+      id[q] = bits ? sbody[tid + __ffsll((unsigned long long)bits)] : 0u;
+      bits &= bits - 1;
+    }
     pm.w0 = cnt;
-    pm.w1 = sids[tid][0];
-    pm.w2 = sids[tid][1];
-    pm.w3 = sids[tid][2];
+    pm.w1 = id[0];
+    pm.w2 = id[1];
+    pm.w3 = id[2];
   } else {
     pm.w0 = cnt | META_MASK_FORM;
     pm.w1 = (uint32_t)i;
