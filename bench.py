@@ -207,25 +207,10 @@ def lattice_leg(pkg, scenes, ctx, side, steps):
     log(f"lattice leg: generating {side}^3 spheres")
     s = scenes.lattice_spheres(side=side)
     sc = pkg.Scene(ctx, np.float32).load(s)
-    sc.set_profiling(True)
-    sc.aabb()
-    sc.broadphase()
-    sc.sync()
-    c = sc.counts()
-    tot = {k: 0.0 for k in pkg.STAGES}
-    wall = []
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        sc.aabb()
-        sc.broadphase()
-        sc.sync()
-        wall.append(time.perf_counter() - t0)
-        for k, v in sc.stage_ms().items():
-            tot[k] += v
-    avg = {k: v / steps for k, v in tot.items()}
+    avg, wall, c = _bp_measure(sc, steps)
     bp_ms = sum(avg[k] for k in BP_STAGES)
     n, m, a, p = s.n, c.bin_entries, c.active_bins, c.pairs
-    algo = n * (24 + 4 + 8 + 28) + m * (8 + 16 + 4 + 24) + 8 * a + 8 * p
+    algo = _bp_algo_bytes(n, m, a, p)
     gbs = algo / (bp_ms * 1e-3) / 1e9
     sc.set_mode(pkg.NP_NATIVE)
     t0 = time.perf_counter()
@@ -247,7 +232,53 @@ def lattice_leg(pkg, scenes, ctx, side, steps):
         "bodies_per_s_full_step": round(n / (step_ms * 1e-3), 1),
     }
     sc.close()
+    # The same scene at the grid the per-size-class autotuner picks (the pair
+    # list does not depend on the grid; bin counts and keys do).
+    log("lattice leg: autotuned grid")
+    bt = pkg.Batch(ctx, np.float32)
+    tuned = bt.scene_bins(bt.add_scene(s, bins=(0, 0, 0)))
+    bt.close()
+    sc = pkg.Scene(ctx, np.float32).load(s, bins=tuned)
+    tavg, twall, tc = _bp_measure(sc, steps)
+    sc.close()
+    if tc.pairs != p:
+        raise RuntimeError(f"autotuned grid changed the pair count: {tc.pairs} != {p}")
+    t_ms = sum(tavg[k] for k in BP_STAGES)
+    t_algo = _bp_algo_bytes(n, tc.bin_entries, tc.active_bins, tc.pairs)
+    out["autotuned_grid"] = {
+        "bins": list(tuned), "bin_entries": int(tc.bin_entries), "active_bins": int(tc.active_bins),
+        "pairs": int(tc.pairs), "broadphase_ms": round(t_ms, 4),
+        "broadphase_wall_ms_p50": round(float(np.median(twall)) * 1e3, 3),
+        "broadphase_stage_ms": {k: round(tavg[k], 4) for k in BP_STAGES},
+        "broadphase_algorithmic_bytes": int(t_algo),
+        "broadphase_GBs": round(t_algo / (t_ms * 1e-3) / 1e9, 1),
+        "broadphase_hbm_frac": round(t_algo / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
     return out
+
+
+def _bp_algo_bytes(n, m, a, p):
+    return n * (24 + 4 + 8 + 28) + m * (8 + 16 + 4 + 24) + 8 * a + 8 * p
+
+
+def _bp_measure(sc, steps):
+    """AABB + broadphase `steps` times after one untimed pass: mean stage times (HIP events), wall times, counts."""
+    sc.set_profiling(True)
+    sc.aabb()
+    sc.broadphase()
+    sc.sync()
+    c = sc.counts()
+    tot = {}
+    wall = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        sc.aabb()
+        sc.broadphase()
+        sc.sync()
+        wall.append(time.perf_counter() - t0)
+        for k, v in sc.stage_ms().items():
+            tot[k] = tot.get(k, 0.0) + v
+    return {k: v / steps for k, v in tot.items()}, wall, c
 
 
 def _gen_threaded(fn, args, workers=16):
