@@ -216,9 +216,10 @@ template <int DB> __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bo
   return peers;
 }
 
-// Per-tile digit histogram.  Each wavefront owns a contiguous run of the
-// tile; equal digits of a wavefront are counted once by their lowest lane,
-// so LDS atomics do not pile up on runs of equal digits.
+// Per-tile digit histogram: one LDS atomic per element.  (Counting equal
+// digits of a wavefront once, by ballots over the digit bits, costs about
+// six VALU instructions per digit bit per element and made this pass
+// VALU-bound; a full 64-way same-address conflict costs less than that.)
 template <int DB>
 __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restrict__ in, uint64_t cap,
                                                          const uint64_t* __restrict__ d_n, int shift,
@@ -230,14 +231,10 @@ __global__ __launch_bounds__(RS_BS) void k_radix_upsweep(const uint64_t* __restr
   __syncthreads();
   const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)w * RS_WTILE + lane;
-  const uint64_t lt = lanemask_lt();
 #pragma unroll 4
   for (int i = 0; i < RS_IPT; ++i) {
     const uint64_t e = base + (uint64_t)i * WAVE;
-    const bool valid = e < n;
-    const uint32_t d = valid ? (uint32_t)(in[e] >> shift) & (ND - 1) : 0u;
-    const uint64_t peers = digit_peers<DB>(d, valid);
-    if (valid && (peers & lt) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    if (e < n) atomicAdd(&h[(uint32_t)(in[e] >> shift) & (ND - 1)], 1u);
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < ND; d += RS_BS) hist[(uint64_t)d * nb + blockIdx.x] = h[d];
