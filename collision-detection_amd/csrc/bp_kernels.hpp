@@ -434,15 +434,31 @@ struct alignas(16) PairMeta {
   uint32_t w0, w1, w2, w3;
 };
 
-template <class R> __device__ __forceinline__ bool overlaps(const BpRec<R>& a, const BpRec<R>& b) {
+// Shifted AABB alone, as the pair search stages it in LDS (the scene and
+// first entry are kept only for the workgroup's own entries).
+template <class R> struct alignas(sizeof(R) == 4 ? 8 : 16) BpBox {
+  R lo[3], hi[3];
+};
+
+template <class R> __device__ __forceinline__ BpBox<R> box_of(const BpRec<R>& r) {
+  BpBox<R> b;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = r.lo[k];
+    b.hi[k] = r.hi[k];
+  }
+  return b;
+}
+
+template <class A, class B> __device__ __forceinline__ bool overlaps(const A& a, const B& b) {
   return (a.lo[0] <= b.hi[0]) & (b.lo[0] <= a.hi[0]) & (a.lo[1] <= b.hi[1]) & (b.lo[1] <= a.hi[1]) &
          (a.lo[2] <= b.hi[2]) & (b.lo[2] <= a.hi[2]);
 }
 
 // Whether the overlapping pair belongs to the run of key K: the hash of the
 // cell max(gmin_a, gmin_b) (both AABBs cover it) must be K.
-template <class R>
-__device__ __forceinline__ bool owner_bin(const int ga[3], const BpRec<R>& rb, const R inv[3], uint32_t K, uint32_t kb,
+template <class R, class B>
+__device__ __forceinline__ bool owner_bin(const int ga[3], const B& rb, const R inv[3], uint32_t K, uint32_t kb,
                                           int gx, int gy) {
   const uint32_t ox = (uint32_t)max(ga[0], cell_of(rb.lo[0], inv[0]));
   const uint32_t oy = (uint32_t)max(ga[1], cell_of(rb.lo[1], inv[1]));
@@ -450,15 +466,15 @@ __device__ __forceinline__ bool owner_bin(const int ga[3], const BpRec<R>& rb, c
   return kb + hash_index(ox, oy, oz, gx, gy) == K;
 }
 
-template <class R>
-__device__ __forceinline__ bool pair_test(const BpRec<R>& ra, const int ga[3], const BpRec<R>& rb, const R inv[3],
-                                          uint32_t K, uint32_t kb, int gx, int gy) {
+template <class R, class A, class B>
+__device__ __forceinline__ bool pair_test(const A& ra, const int ga[3], const B& rb, const R inv[3], uint32_t K,
+                                          uint32_t kb, int gx, int gy) {
   return overlaps(ra, rb) && owner_bin(ga, rb, inv, K, kb, gx, gy);
 }
 
 // Scene parameters and cells of one entry's shape.
 template <class R> struct EntryCtx {
-  BpRec<R> r;
+  BpBox<R> r;
   R inv[3];
   int g0[3], g1[3];
   int gx, gy;
@@ -466,10 +482,9 @@ template <class R> struct EntryCtx {
   uint32_t kb, K, a;
 };
 
-template <class R>
-__device__ __forceinline__ void entry_ctx_d(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r,
+template <class R, class B>
+__device__ __forceinline__ void entry_ctx_d(EntryCtx<R>& c, uint32_t K, uint32_t a, const B& r,
                                             const SceneDesc<R>& d) {
-  c.r = r;
   c.K = K;
   c.a = a;
   c.gx = d.gx;
@@ -479,6 +494,8 @@ __device__ __forceinline__ void entry_ctx_d(EntryCtx<R>& c, uint32_t K, uint32_t
   c.kb = d.key_base;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
+    c.r.lo[k] = r.lo[k];
+    c.r.hi[k] = r.hi[k];
     c.inv[k] = d.inv[k];
     c.g0[k] = cell_of(r.lo[k], c.inv[k]);
     c.g1[k] = cell_of(r.hi[k], c.inv[k]);
@@ -494,10 +511,11 @@ __device__ __forceinline__ void entry_ctx(EntryCtx<R>& c, uint32_t K, uint32_t a
 // entry_ctx with the scene descriptor from LDS when the whole staged window
 // belongs to one scene (block-uniform branch), else from global memory.
 template <class R>
-__device__ __forceinline__ void entry_ctx_w(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpRec<R>& r, bool one,
-                                            const SceneDesc<R>& sd, const SceneDesc<R>* __restrict__ desc) {
+__device__ __forceinline__ void entry_ctx_w(EntryCtx<R>& c, uint32_t K, uint32_t a, const BpBox<R>& r,
+                                            uint32_t scene, bool one, const SceneDesc<R>& sd,
+                                            const SceneDesc<R>* __restrict__ desc) {
   if (one) entry_ctx_d(c, K, a, r, sd);
-  else entry_ctx_d(c, K, a, r, desc[r.scene]);
+  else entry_ctx_d(c, K, a, r, desc[scene]);
 }
 
 template <class R>
@@ -506,12 +524,12 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
                                                    const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ pcount,
                                                    PairMeta* __restrict__ meta, uint32_t* __restrict__ runs_part,
                                                    DevStatus* st) {
-  __shared__ BpRec<R> srec[PAIR_W];
+  __shared__ BpBox<R> sbox[PAIR_W];
   __shared__ uint32_t skey[PAIR_W], sbody[PAIR_W];
+  __shared__ uint32_t sscene[PAIR_BS];  // scenes of the own entries
   __shared__ uint64_t send[PAIR_W / WAVE];
   __shared__ uint32_t sstart[PAIR_BS + 1];
   __shared__ uint64_t smask[PAIR_BS];  // hit mask per own entry, set by LDS atomics
-  __shared__ uint32_t scnt[PAIR_BS];   // partners beyond 62 places per own entry
   __shared__ uint32_t scan_scratch[PAIR_BS / WAVE + 1];
   __shared__ SceneDesc<R> sdesc;  // the window's scene, when it has only one
   __shared__ int sone;
@@ -522,11 +540,15 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   const int tid = threadIdx.x, lane = tid % WAVE;
   const int nloc = (int)min((uint64_t)PAIR_W, m - b0);
   const int nown = min(nloc, PAIR_BS);
+  uint32_t my_offs = 0;  // first entry of the own entry's shape
   if (tid < nloc) {
     const uint64_t kk = keys[b0 + tid];
     skey[tid] = (uint32_t)(kk >> 32);
     sbody[tid] = (uint32_t)kk;
-    srec[tid] = brec[(uint32_t)kk];
+    const BpRec<R> r = brec[(uint32_t)kk];
+    sbox[tid] = box_of(r);
+    sscene[tid] = r.scene;
+    my_offs = r.offs;
   }
   static_assert(PAIR_AHEAD <= PAIR_BS, "one look-ahead entry per thread");
   if (tid < PAIR_AHEAD && PAIR_BS + tid < nloc) {
@@ -538,14 +560,14 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     const uint32_t klast = (uint32_t)(keys[b0 + PAIR_BS - 1] >> 32);
     skey[t] = (uint32_t)(kk >> 32);
     sbody[t] = (uint32_t)kk;
-    if ((uint32_t)(kk >> 32) == klast) srec[t] = brec[(uint32_t)kk];
+    if ((uint32_t)(kk >> 32) == klast) sbox[t] = box_of(brec[(uint32_t)kk]);
   }
   __syncthreads();
   if (tid == 0) {
     // keys ascend and scenes own disjoint key ranges: one scene at both ends
     // of the own entries means one scene for every staged record
-    const uint32_t s0 = srec[0].scene;
-    sone = s0 == srec[nown - 1].scene ? 1 : 0;
+    const uint32_t s0 = sscene[0];
+    sone = s0 == sscene[nown - 1] ? 1 : 0;
     if (sone) sdesc = desc[s0];
   }
   // run ends inside the window (the last staged entry always ends one)
@@ -561,6 +583,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   const uint64_t i = b0 + tid;
   bool dup = false, beyond = false;
   uint32_t n_i = 0;
+  int u = tid;  // last entry of the own entry's run inside the window
   bool run_start = false;  // the entry opens a run of equal keys (an active bin)
   if (valid) {
     if (tid > 0) {
@@ -576,7 +599,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     int w = tid / WAVE;
     uint64_t fm = send[w] & (~0ull << (tid % WAVE));
     while (!fm && ++w < PAIR_W / WAVE) fm = send[w];
-    const int u = fm ? w * WAVE + __ffsll((unsigned long long)fm) - 1 : nloc - 1;
+    u = fm ? w * WAVE + __ffsll((unsigned long long)fm) - 1 : nloc - 1;
     if (!dup) {
       n_i = (uint32_t)(u - tid);
       beyond = u == nloc - 1 && b0 + nloc < m && (uint32_t)(keys[b0 + nloc] >> 32) == skey[tid];
@@ -597,7 +620,6 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     runs_part[blockIdx.x] = r;
   }
   smask[tid] = 0;
-  scnt[tid] = 0;
   __syncthreads();
   // balanced pass over the candidates of the workgroup's entries
   {
@@ -612,17 +634,17 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
       int e = lo;
       uint32_t ebeg = sstart[e], eend = sstart[e + 1];
       EntryCtx<R> c;
-      entry_ctx_w(c, skey[e], sbody[e], srec[e], one, sdesc, desc);
+      entry_ctx_w(c, skey[e], sbody[e], sbox[e], sscene[e], one, sdesc, desc);
       uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
       for (; f < fend; ++f) {
         if (f >= eend) {
           do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
-          entry_ctx_w(c, skey[e], sbody[e], srec[e], one, sdesc, desc);
+          entry_ctx_w(c, skey[e], sbody[e], sbox[e], sscene[e], one, sdesc, desc);
           prev = c.a;
         }
         const int k = e + 1 + (int)(f - ebeg);
         const uint32_t b = sbody[k];
-        const BpRec<R> rk = srec[k];
+        const BpBox<R> rk = sbox[k];
         // repeat entries of one shape (b == prev) are skipped; the record is
         // read and tested with the body id rather than after it
         const bool fresh = b != prev;
@@ -630,14 +652,10 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
         if ((fresh & overlaps(c.r, rk)) && owner_bin(c.g0, rk, c.inv, c.K, c.kb, c.gx, c.gy)) {
           // a hit goes straight to the entry's mask: lanes sharing an entry
           // need no merge step, and the partners are read back from the
-          // mask bits and the staged bodies
+          // mask bits and the staged bodies (partners 63 or more places on
+          // are only flagged, and counted again below)
           const int dd = k - e - 1;
-          if (dd < 63) {
-            atomicOr((unsigned long long*)&smask[e], 1ull << dd);
-          } else {
-            atomicOr((unsigned long long*)&smask[e], (unsigned long long)MASK_TAIL);
-            atomicAdd(&scnt[e], 1u);
-          }
+          atomicOr((unsigned long long*)&smask[e], dd < 63 ? (1ull << dd) : (unsigned long long)MASK_TAIL);
         }
       }
     }
@@ -645,9 +663,15 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   __syncthreads();
   if (!valid || dup) return;
   uint64_t mk = smask[tid];
-  uint32_t cnt = (uint32_t)__popcll(mk & ~MASK_TAIL) + scnt[tid];
+  uint32_t cnt = (uint32_t)__popcll(mk & ~MASK_TAIL);
   EntryCtx<R> c;
-  entry_ctx_w(c, skey[tid], sbody[tid], srec[tid], one, sdesc, desc);
+  entry_ctx_w(c, skey[tid], sbody[tid], sbox[tid], sscene[tid], one, sdesc, desc);
+  if (mk & MASK_TAIL) {  // partners 63 or more places on inside the window (long runs only)
+    for (int k = tid + 64; k <= u; ++k) {
+      const uint32_t b = sbody[k];
+      if (b != sbody[k - 1] && pair_test(c.r, c.g0, sbox[k], c.inv, c.K, c.kb, c.gx, c.gy)) ++cnt;
+    }
+  }
   if (beyond) {  // run continues past the window: the rest from global memory (rare)
     uint32_t prev = sbody[nloc - 1];
     for (uint64_t k = b0 + nloc; k < m; ++k) {
@@ -667,7 +691,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   cl.x0 = c.g0[0]; cl.y0 = c.g0[1]; cl.z0 = c.g0[2];
   cl.x1 = c.g1[0]; cl.y1 = c.g1[1]; cl.z1 = c.g1[2];
   const uint32_t li = local_cell_index(c.K - c.kb, cl, c.gx, c.gy, c.rgx, c.rgy);
-  const uint64_t j = (uint64_t)c.r.offs + li;
+  const uint64_t j = (uint64_t)my_offs + li;
   if (li == ~0u || j >= m) {
     atomicOr(&st->bad, 8u);
     return;

@@ -44,6 +44,28 @@ def test_aabb_bit_exact(cdx, ctx, oracle, cases, name):
 def test_broadphase_bit_exact(cdx, ctx, oracle, cases, name):
     s = cases[name]
     print(f"case {name} n={s.n} grid={s.grid}", flush=True)
+    _check_broadphase(cdx, ctx, oracle, s)
+
+
+LONG_RUN_CASES = {
+    # runs of several hundred entries per bin: the pair search's look-ahead
+    # window, its continuation past the window and hit masks with partners
+    # 63 or more places on
+    "mixed3k_grid2_f64": dict(n=3000, seed=21, grid=(2, 2, 2)),
+    "mixed3k_grid2_f32": dict(n=3000, seed=22, dtype=np.float32, grid=(2, 2, 2)),
+    # ~60 AABB partners per shape: partner groups longer than the register sorts
+    "dense600_grid3_f64": dict(n=600, seed=23, neighbours=60.0, grid=(3, 3, 3)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LONG_RUN_CASES))
+def test_broadphase_long_runs(cdx, ctx, oracle, scenes, name):
+    s = scenes.mixed_bodies(**LONG_RUN_CASES[name])
+    print(f"case {name} n={s.n} grid={s.grid}", flush=True)
+    _check_broadphase(cdx, ctx, oracle, s)
+
+
+def _check_broadphase(cdx, ctx, oracle, s):
     ab = oracle.aabb(s)
     ref = oracle.broadphase(ab, s.grid)
     sc = cdx.Scene(ctx, s.dtype).load(s)
