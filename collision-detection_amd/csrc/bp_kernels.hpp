@@ -869,7 +869,7 @@ __device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_
   }
 }
 
-constexpr int EMIT_PAIRS_LDS = 4096;
+constexpr int EMIT_PAIRS_LDS = 3584;
 constexpr int EMIT_COUNTS_LDS = 2048;
 
 // Writes each shape's pairs (a << 32 | b) into its range, sorted by b.  A
@@ -878,7 +878,7 @@ constexpr int EMIT_COUNTS_LDS = 2048;
 // Chunks with more pairs than fit in LDS write in place; groups longer than
 // GROUP_LONG are listed for k_pair_groups_long.
 template <class R>
-__global__ __launch_bounds__(CHUNK) void k_pair_emit(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
+__global__ __launch_bounds__(CHUNK, 6) void k_pair_emit(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                      const uint32_t* __restrict__ counts,
                                                      const uint32_t* __restrict__ offs,
                                                      const uint32_t* __restrict__ pcount,
