@@ -7,6 +7,8 @@
 // the resident capacity NS; per-step arrays (bin counts, offsets) live in
 // "slot" space: slot = chunk * CHUNK + lane.
 #pragma once
+#include <type_traits>
+
 #include "cdx_device.hpp"
 
 namespace cdx {
@@ -435,8 +437,10 @@ struct alignas(16) PairMeta {
 };
 
 // Shifted AABB alone, as the pair search stages it in LDS (the scene and
-// first entry are kept only for the workgroup's own entries).
-template <class R> struct alignas(sizeof(R) == 4 ? 8 : 16) BpBox {
+// first entry are kept only for the workgroup's own entries).  FP32 boxes
+// are padded to 32 bytes so an entry's LDS address is a shift, not a
+// quarter-rate multiply.
+template <class R> struct alignas(sizeof(R) == 4 ? 32 : 16) BpBox {
   R lo[3], hi[3];
 };
 
@@ -457,13 +461,25 @@ template <class A, class B> __device__ __forceinline__ bool overlaps(const A& a,
 
 // Whether the overlapping pair belongs to the run of key K: the hash of the
 // cell max(gmin_a, gmin_b) (both AABBs cover it) must be K.
-template <class R, class B>
+// SMALL: the scene's grid satisfies grid_fits_u24, so the hash can use
+// full-rate 24-bit multiplies (same low 32 bits as the 32-bit products).
+template <class R, class B, bool SMALL = false>
 __device__ __forceinline__ bool owner_bin(const int ga[3], const B& rb, const R inv[3], uint32_t K, uint32_t kb,
                                           int gx, int gy) {
   const uint32_t ox = (uint32_t)max(ga[0], cell_of(rb.lo[0], inv[0]));
   const uint32_t oy = (uint32_t)max(ga[1], cell_of(rb.lo[1], inv[1]));
   const uint32_t oz = (uint32_t)max(ga[2], cell_of(rb.lo[2], inv[2]));
-  return kb + hash_index(ox, oy, oz, gx, gy) == K;
+  if constexpr (SMALL)
+    return kb + mad_u24(mad_u24(oz, (uint32_t)gy, oy), (uint32_t)gx, ox) == K;
+  else
+    return kb + hash_index(ox, oy, oz, gx, gy) == K;
+}
+
+// Cells run 0..grid size per axis (the max corner may sit on index == grid
+// size), so with these bounds every factor of the hash, and z * gy + y,
+// stays below 2^24.
+__device__ __forceinline__ bool grid_fits_u24(int gx, int gy, int gz) {
+  return (uint64_t)gx < (1u << 24) && (uint64_t)(gz + 1) * (uint64_t)(gy + 1) < (1u << 24);
 }
 
 template <class R, class A, class B>
@@ -622,44 +638,51 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
   smask[tid] = 0;
   __syncthreads();
   // balanced pass over the candidates of the workgroup's entries
-  {
+  auto balanced_pass = [&](auto small_grid) {
+    constexpr bool SMALL = decltype(small_grid)::value;
     uint32_t f = (uint32_t)(((uint64_t)total * tid) / PAIR_BS);
     const uint32_t fend = (uint32_t)(((uint64_t)total * (tid + 1)) / PAIR_BS);
-    if (f < fend) {
-      int lo = 0, hi = PAIR_BS - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (sstart[mid] <= f) lo = mid; else hi = mid - 1;
+    if (f >= fend) return;
+    int lo = 0, hi = PAIR_BS - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (sstart[mid] <= f) lo = mid; else hi = mid - 1;
+    }
+    int e = lo;
+    uint32_t ebeg = sstart[e], eend = sstart[e + 1];
+    EntryCtx<R> c;
+    entry_ctx_w(c, skey[e], sbody[e], sbox[e], sscene[e], one, sdesc, desc);
+    uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
+    for (; f < fend; ++f) {
+      if (f >= eend) {
+        do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
+        e &= PAIR_BS - 1;  // e < PAIR_BS: as for k below
+        entry_ctx_w(c, skey[e], sbody[e], sbox[e], sscene[e], one, sdesc, desc);
+        prev = c.a;
       }
-      int e = lo;
-      uint32_t ebeg = sstart[e], eend = sstart[e + 1];
-      EntryCtx<R> c;
-      entry_ctx_w(c, skey[e], sbody[e], sbox[e], sscene[e], one, sdesc, desc);
-      uint32_t prev = f == ebeg ? c.a : sbody[e + (int)(f - ebeg)];
-      for (; f < fend; ++f) {
-        if (f >= eend) {
-          do { ++e; ebeg = sstart[e]; eend = sstart[e + 1]; } while (eend == ebeg);
-          entry_ctx_w(c, skey[e], sbody[e], sbox[e], sscene[e], one, sdesc, desc);
-          prev = c.a;
-        }
-        const int k = e + 1 + (int)(f - ebeg);
-        const uint32_t b = sbody[k];
-        const BpBox<R> rk = sbox[k];
-        // repeat entries of one shape (b == prev) are skipped; the record is
-        // read and tested with the body id rather than after it
-        const bool fresh = b != prev;
-        prev = b;
-        if ((fresh & overlaps(c.r, rk)) && owner_bin(c.g0, rk, c.inv, c.K, c.kb, c.gx, c.gy)) {
-          // a hit goes straight to the entry's mask: lanes sharing an entry
-          // need no merge step, and the partners are read back from the
-          // mask bits and the staged bodies (partners 63 or more places on
-          // are only flagged, and counted again below)
-          const int dd = k - e - 1;
-          atomicOr((unsigned long long*)&smask[e], dd < 63 ? (1ull << dd) : (unsigned long long)MASK_TAIL);
-        }
+      // k < PAIR_W <= 1024: the mask changes nothing but tells the compiler
+      // the index is small, so the record address uses a full-rate 24-bit
+      // multiply
+      const int k = (e + 1 + (int)(f - ebeg)) & 1023;
+      const uint32_t b = sbody[k];
+      const BpBox<R> rk = sbox[k];
+      // repeat entries of one shape (b == prev) are skipped; the record is
+      // read and tested with the body id rather than after it
+      const bool fresh = b != prev;
+      prev = b;
+      if ((fresh & overlaps(c.r, rk)) &&
+          owner_bin<R, BpBox<R>, SMALL>(c.g0, rk, c.inv, c.K, c.kb, c.gx, c.gy)) {
+        // a hit goes straight to the entry's mask: lanes sharing an entry
+        // need no merge step, and the partners are read back from the
+        // mask bits and the staged bodies (partners 63 or more places on
+        // are only flagged, and counted again below)
+        const int dd = k - e - 1;
+        atomicOr((unsigned long long*)&smask[e], dd < 63 ? (1ull << dd) : (unsigned long long)MASK_TAIL);
       }
     }
-  }
+  };
+  if (one && grid_fits_u24(sdesc.gx, sdesc.gy, sdesc.gz)) balanced_pass(std::true_type{});
+  else balanced_pass(std::false_type{});
   __syncthreads();
   if (!valid || dup) return;
   uint64_t mk = smask[tid];

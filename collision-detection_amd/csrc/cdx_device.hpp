@@ -114,5 +114,14 @@ __device__ __forceinline__ uint32_t hash_index(uint32_t i, uint32_t j, uint32_t 
 
 template <class R> __device__ __forceinline__ int cell_of(R x, R inv) { return (int)(x * inv); }
 
+// a * b + c for a, b < 2^24: the low 32 bits of the product, one full-rate
+// instruction (a 32-bit multiply is quarter rate, and the compiler cannot
+// prove the operand range itself).
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t d;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
 }  // namespace
 }  // namespace cdx
