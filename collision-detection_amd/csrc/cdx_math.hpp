@@ -132,10 +132,27 @@ template <class R> CDX_HD v3<R> support_local(int type, v3<R> B, v3<R> n) {
 
 // World support of a placed shape along b (Narrowphase.cu:19-52):
 // normalize, rotate into the shape frame, support, rotate back, translate.
-template <class R> CDX_HD v3<R> support_world(int type, v3<R> X, v3<R> B, const mat3<R>& m, v3<R> b) {
-  v3<R> n = normalize(b);
+template <class R> CDX_HD v3<R> support_world_unit(int type, v3<R> X, v3<R> B, const mat3<R>& m, v3<R> n) {
   v3<R> rn = mulT(m, n);
   return mul(m, support_local(type, B, rn)) + X;
+}
+template <class R> CDX_HD v3<R> support_world(int type, v3<R> X, v3<R> B, const mat3<R>& m, v3<R> b) {
+  return support_world_unit(type, X, B, m, normalize(b));
+}
+
+// normalize(-b) and normalize(b) from one square root and one set of
+// divisions: -b has the same squared length, and IEEE division commutes
+// with negation, so normalize(-b) == -normalize(b) bit for bit, except for
+// b = 0, which normalises to (1, 0, 0) either way.  The MPR always asks for
+// the support of one shape along -b and of the other along b.
+template <class R> CDX_HD void normalize_pm(v3<R> b, v3<R>& neg, v3<R>& pos) {
+  const R l = length(b);
+  if (l == R(0)) {
+    neg = pos = mk3<R>(1, 0, 0);
+    return;
+  }
+  pos = b / l;
+  neg = -pos;
 }
 
 // Approximate FP32 variants (hardware reciprocal / reciprocal square root,
@@ -174,11 +191,26 @@ __device__ __forceinline__ v3<float> support_local_fast(int type, v3<float> B, v
   }
 }
 
-__device__ __forceinline__ v3<float> support_world_fast(int type, v3<float> X, v3<float> B, const mat3<float>& m,
-                                                        v3<float> b) {
-  v3<float> n = normalize_fast(b);
+__device__ __forceinline__ v3<float> support_world_fast_unit(int type, v3<float> X, v3<float> B,
+                                                             const mat3<float>& m, v3<float> n) {
   v3<float> rn = mulT(m, n);
   return mul(m, support_local_fast(type, B, rn)) + X;
+}
+__device__ __forceinline__ v3<float> support_world_fast(int type, v3<float> X, v3<float> B, const mat3<float>& m,
+                                                        v3<float> b) {
+  return support_world_fast_unit(type, X, B, m, normalize_fast(b));
+}
+
+// normalize_fast of -b and of b from one reciprocal square root ((-b) * r
+// is -(b * r) exactly), as normalize_pm.
+__device__ __forceinline__ void normalize_fast_pm(v3<float> b, v3<float>& neg, v3<float>& pos) {
+  const float d = dot(b, b);
+  if (d == 0.f) {
+    neg = pos = mk3(1.f, 0.f, 0.f);
+    return;
+  }
+  pos = b * __builtin_amdgcn_rsqf(d);
+  neg = -pos;
 }
 
 // Order-preserving map float -> uint32 used for atomic min / max of bounds.

@@ -22,6 +22,15 @@ template <class R> __device__ __forceinline__ v3<R> sup(const Placed<R>& s, v3<R
   return support_world(s.type, s.X, s.B, rotmat(s.q), b);
 }
 
+// sa = sup(A, -n), sb = sup(B, n) with one normalisation of n (normalize_pm).
+template <class R>
+__device__ __forceinline__ void sup_pair(const Placed<R>& A, const Placed<R>& B, v3<R> n, v3<R>& sa, v3<R>& sb) {
+  v3<R> na, nb;
+  normalize_pm(n, na, nb);
+  sa = support_world_unit(A.type, A.X, A.B, rotmat(A.q), na);
+  sb = support_world_unit(B.type, B.X, B.B, rotmat(B.q), nb);
+}
+
 template <class R> __device__ __forceinline__ Placed<R> placed(const ShapeRec<R>& r) {
   Placed<R> s;
   s.type = (int)r.type;
@@ -43,6 +52,15 @@ __device__ __forceinline__ v3<float> sup_fast(const PlacedF& s, v3<float> b) {
   return support_world_fast(s.type, s.X, s.B, s.m, b);
 }
 
+// sa = sup_fast(A, -n), sb = sup_fast(B, n) with one normalisation of n.
+__device__ __forceinline__ void sup_fast_pair(const PlacedF& A, const PlacedF& B, v3<float> n, v3<float>& sa,
+                                              v3<float>& sb) {
+  v3<float> na, nb;
+  normalize_fast_pm(n, na, nb);
+  sa = support_world_fast_unit(A.type, A.X, A.B, A.m, na);
+  sb = support_world_fast_unit(B.type, B.X, B.B, B.m, nb);
+}
+
 template <class R> __device__ __forceinline__ void swap3(v3<R>& a, v3<R>& b) { v3<R> t = a; a = b; b = t; }
 
 // CollideAndFindPoint (Narrowphase.cu:122-314), same evaluation order.
@@ -54,8 +72,7 @@ __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rno
   v0 = v02 - v01;
   if (is_zero(v0)) v0 = mk3<R>(1, 0, 0);
   n = -v0;
-  v11 = sup(A, -n);
-  v12 = sup(B, n);
+  sup_pair(A, B, n, v11, v12);
   v1 = v12 - v11;
   if (dot(v1, n) <= R(0)) return false;
   n = cross(v1, v0);
@@ -67,8 +84,7 @@ __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rno
     depth = dot((v12 - v11), n);
     return true;
   }
-  v21 = sup(A, -n);
-  v22 = sup(B, n);
+  sup_pair(A, B, n, v21, v22);
   v2 = v22 - v21;
   if (dot(v2, n) <= R(0)) return false;
   n = cross((v1 - v0), (v2 - v0));
@@ -85,8 +101,7 @@ __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rno
   while (true) {
     if (phase1 > max_iterations) return false;
     phase1++;
-    v31 = sup(A, -n);
-    v32 = sup(B, n);
+    sup_pair(A, B, n, v31, v32);
     v3_ = v32 - v31;
     if (dot(v3_, n) <= R(0)) return false;
     if (dot(cross(v1, v3_), v0) < R(0)) {
@@ -104,8 +119,8 @@ __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rno
       n = cross((v2 - v1), (v3_ - v1));
       n = normalize(n);
       if (dot(n, v1) >= R(0) && !hit) hit = true;
-      v3<R> v41 = sup(A, -n);
-      v3<R> v42 = sup(B, n);
+      v3<R> v41, v42;
+      sup_pair(A, B, n, v41, v42);
       v3<R> v4 = v42 - v41;
       R delta = dot((v4 - v3_), n);
       depth = dot(v4, n);
@@ -182,8 +197,10 @@ __device__ bool contact_native(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R e
     v3<R> p0 = mk3<R>(0, 0, 0);
     R depth = 0;
     if (!mpr_reference(A, B, N, p0, depth)) return false;
-    p1 = dot((sup(A, -N) - p0), N) * N + p0;
-    p2 = dot((sup(B, N) - p0), N) * N + p0;
+    v3<R> sa, sb;
+    sup_pair(A, B, N, sa, sb);
+    p1 = dot((sa - p0), N) * N + p0;
+    p2 = dot((sb - p0), N) * N + p0;
     N = -N;
   }
   p1 = p1 - N * envelope;
@@ -253,8 +270,7 @@ __device__ int mpr_guarded(const PlacedF& A, const PlacedF& B, float tau, bool v
   v0 = v02 - v01;
   if (length_fast(v0) < 1e-4f * tau) return MPR_UNSURE;
   n = -v0;
-  v11 = sup_fast(A, -n);
-  v12 = sup_fast(B, n);
+  sup_fast_pair(A, B, n, v11, v12);
   v1 = v12 - v11;
   R dn = dot(v1, n);
   if (dn <= R(0)) return (-dn < tau * length_fast(n)) ? MPR_UNSURE : MPR_MISS;
@@ -263,8 +279,7 @@ __device__ int mpr_guarded(const PlacedF& A, const PlacedF& B, float tau, bool v
     R nn = dot(n, n), s = 1e-6f * dot(v1, v1) * dot(v0, v0);
     if (nn <= s) return MPR_UNSURE;
   }
-  v21 = sup_fast(A, -n);
-  v22 = sup_fast(B, n);
+  sup_fast_pair(A, B, n, v21, v22);
   v2 = v22 - v21;
   dn = dot(v2, n);
   if (dn <= R(0)) return (-dn < tau * length_fast(n)) ? MPR_UNSURE : MPR_MISS;
@@ -282,8 +297,7 @@ __device__ int mpr_guarded(const PlacedF& A, const PlacedF& B, float tau, bool v
   while (true) {
     if (phase1 > max_iterations) return MPR_UNSURE;
     phase1++;
-    v31 = sup_fast(A, -n);
-    v32 = sup_fast(B, n);
+    sup_fast_pair(A, B, n, v31, v32);
     v3_ = v32 - v31;
     dn = dot(v3_, n);
     if (dn <= R(0)) return (-dn < tau * length_fast(n)) ? MPR_UNSURE : MPR_MISS;
@@ -305,8 +319,8 @@ __device__ int mpr_guarded(const PlacedF& A, const PlacedF& B, float tau, bool v
       if (side >= R(0) && !hit) hit = true;
       // The origin is inside the portal by more than tau: a certain hit.
       if (verdict_only && side > tau) return MPR_HIT;
-      v3<R> v41 = sup_fast(A, -n);
-      v3<R> v42 = sup_fast(B, n);
+      v3<float> v41, v42;
+      sup_fast_pair(A, B, n, v41, v42);
       v3<R> v4 = v42 - v41;
       R delta = dot((v4 - v3_), n);
       R depth = dot(v4, n);
@@ -366,7 +380,9 @@ __device__ __forceinline__ bool mpr_first_miss(const PlacedF& A, const PlacedF& 
   const v3<float> v0 = B.X - A.X;
   if (length_fast(v0) < 1e-4f * tau) return false;
   const v3<float> n = -v0;
-  const v3<float> v1 = sup_fast(B, n) - sup_fast(A, -n);
+  v3<float> sa, sb;
+  sup_fast_pair(A, B, n, sa, sb);
+  const v3<float> v1 = sb - sa;
   const float dn = dot(v1, n);
   return dn <= 0.f && !(-dn < tau * length_fast(n));
 }
@@ -463,8 +479,10 @@ __global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t*
   v3<float> N, p0;
   int r = mpr_guarded(A, B, tau, !KEEP, N, p0);
   if (KEEP && r == MPR_HIT) {
-    v3<float> p1 = dot((sup_fast(A, -N) - p0), N) * N + p0;
-    v3<float> p2 = dot((sup_fast(B, N) - p0), N) * N + p0;
+    v3<float> sa, sb;
+    sup_fast_pair(A, B, N, sa, sb);
+    v3<float> p1 = dot((sa - p0), N) * N + p0;
+    v3<float> p2 = dot((sb - p0), N) * N + p0;
     N = -N;
     float dl = dot(N, p2 - p1);
     if (fabsf(dl) < tau) {
