@@ -9,9 +9,9 @@ namespace {
 
 // A shape placed in the world for the reference-order MPR: type, centre,
 // dimensions, rotation.  The rotation matrix is rebuilt from the quaternion
-// at every support call: the same rounded values each time, and far fewer
-// live registers than a cached 3x3 per shape (the FP64 MPR is
-// occupancy-bound: caching both matrices takes it to 250 VGPRs).
+// at every support call: the same rounded values each time, with fewer live
+// registers than a cached 3x3 per shape (caching both matrices per pair
+// measured no faster: the kernel stays at 2 waves per SIMD either way).
 template <class R> struct Placed {
   int type;
   v3<R> X, B;
