@@ -238,6 +238,27 @@ def test_kat_duplicated_lattice(cdx, ctx):
     np.testing.assert_array_equal(pairs, exp)
 
 
+@pytest.mark.parametrize("axis", [0, 2])
+def test_broadphase_large_grid(cdx, ctx, oracle, axis):
+    """2^24 bins along one axis: the owner-bin hash takes its 32-bit path
+    (the 24-bit one needs every hash factor below 2^24); pairs vs the oracle."""
+    rng = np.random.default_rng(31)
+    n2 = 1000
+    c = np.stack([rng.uniform(0, 1e-3, n2), rng.uniform(0, 1e-3, n2), rng.uniform(0.0, 100.0, n2)], axis=1)
+    c = np.repeat(c, 2, axis=0)
+    c[1::2, 2] += 1e-5  # each twin overlaps its partner: pairs (2i, 2i+1)
+    if axis == 0:
+        c = c[:, [2, 1, 0]].copy()
+    ab = np.concatenate([c - 2e-5, c + 2e-5])
+    grid = [1, 1, 1]
+    grid[axis] = 1 << 24
+    print(f"large grid {grid}", flush=True)
+    ref = oracle.broadphase(ab, grid)
+    pairs = ctx.detect_possible_collisions(ab, tuple(grid))
+    assert len(ref.pairs) >= n2
+    np.testing.assert_array_equal(pairs, ref.pairs)
+
+
 def test_empty_inputs(cdx, ctx):
     ab = np.zeros((0, 3))
     assert len(ctx.detect_possible_collisions(ab, (4, 4, 4))) == 0
