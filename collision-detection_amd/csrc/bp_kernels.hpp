@@ -676,8 +676,9 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
         // need no merge step, and the partners are read back from the
         // mask bits and the staged bodies (partners 63 or more places on
         // are only flagged, and counted again below)
-        const int dd = k - e - 1;
-        atomicOr((unsigned long long*)&smask[e], dd < 63 ? (1ull << dd) : (unsigned long long)MASK_TAIL);
+        // bit dd for partners up to 62 places on, MASK_TAIL (bit 63) beyond
+        const int dd = min(k - e - 1, 63);
+        atomicOr((unsigned long long*)&smask[e], 1ull << dd);
       }
     }
   };
