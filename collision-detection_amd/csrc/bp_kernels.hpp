@@ -900,7 +900,9 @@ constexpr int EMIT_COUNTS_LDS = 2048;
 // workgroup handles one chunk: partners go to LDS, each shape's group is
 // sorted in registers, and the chunk's pairs leave in one coalesced store.
 // Chunks with more pairs than fit in LDS write in place; groups longer than
-// GROUP_LONG are listed for k_pair_groups_long.
+// GROUP_LONG are listed for k_pair_groups_long.  The kernel waits on
+// dependent loads (entry metadata, then sorted entries), so it is held to
+// 80 VGPRs and 26 KB of LDS: 6 waves per SIMD.
 template <class R>
 __global__ __launch_bounds__(CHUNK, 6) void k_pair_emit(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                      const uint32_t* __restrict__ counts,
