@@ -859,37 +859,61 @@ template <class R, class PC, class Put>
 __device__ __forceinline__ void shape_partners(uint64_t e0, uint64_t e1, uint32_t a, PC pcount,
                                                const PairMeta* __restrict__ meta, const uint64_t* __restrict__ keys,
                                                const BpRec<R>* __restrict__ brec, const SceneDesc<R>* __restrict__ desc, uint64_t m, Put put) {
-  for (uint64_t e = e0; e < e1; ++e) {
-    if (!pcount(e)) continue;
-    const PairMeta pm = meta[e];
+  // The next entry's metadata is loaded before this one is unpacked, and
+  // mask-form partners are read four at a time, so the dependent loads of
+  // one shape overlap.
+  auto next = [&](uint64_t e) {
+    while (e < e1 && !pcount(e)) ++e;
+    return e;
+  };
+  uint64_t e = next(e0);
+  PairMeta pm = {0u, 0u, 0u, 0u};
+  if (e < e1) pm = meta[e];
+  while (e < e1) {
+    const uint64_t en = next(e + 1);
+    PairMeta pn = {0u, 0u, 0u, 0u};
+    if (en < e1) pn = meta[en];
     if (!(pm.w0 & META_MASK_FORM)) {
       if (pm.w0 > 0) put(pm.w1);
       if (pm.w0 > 1) put(pm.w2);
       if (pm.w0 > 2) put(pm.w3);
-      continue;
-    }
-    const uint64_t ip = pm.w1;
-    const uint64_t mask = ((uint64_t)pm.w3 << 32) | pm.w2;
-    uint64_t bits = mask & ~MASK_TAIL;
-    while (bits) {
-      const int dd = __ffsll((unsigned long long)bits) - 1;
-      bits &= bits - 1;
-      put((uint32_t)keys[ip + 1 + dd]);
-    }
-    if (mask & MASK_TAIL) {
-      const uint64_t ki = keys[ip];
-      EntryCtx<R> c;
-      entry_ctx(c, (uint32_t)(ki >> 32), a, brec[a], desc);
-      uint32_t prev = (uint32_t)keys[ip + 63];
-      for (uint64_t k = ip + 64; k < m; ++k) {
-        const uint64_t kk = keys[k];
-        if ((uint32_t)(kk >> 32) != c.K) break;
-        const uint32_t b = (uint32_t)kk;
-        if (b == prev) continue;
-        prev = b;
-        if (pair_test(c.r, c.g0, brec[b], c.inv, c.K, c.kb, c.gx, c.gy)) put(b);
+    } else {
+      const uint64_t ip = pm.w1;
+      const uint64_t mask = ((uint64_t)pm.w3 << 32) | pm.w2;
+      uint64_t bits = mask & ~MASK_TAIL;
+      while (bits) {
+        uint32_t id[4];
+        int got = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (bits) {
+            const int dd = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            id[q] = (uint32_t)keys[ip + 1 + dd];
+            got = q + 1;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < got) put(id[q]);
+      }
+      if (mask & MASK_TAIL) {
+        const uint64_t ki = keys[ip];
+        EntryCtx<R> c;
+        entry_ctx(c, (uint32_t)(ki >> 32), a, brec[a], desc);
+        uint32_t prev = (uint32_t)keys[ip + 63];
+        for (uint64_t k = ip + 64; k < m; ++k) {
+          const uint64_t kk = keys[k];
+          if ((uint32_t)(kk >> 32) != c.K) break;
+          const uint32_t b = (uint32_t)kk;
+          if (b == prev) continue;
+          prev = b;
+          if (pair_test(c.r, c.g0, brec[b], c.inv, c.K, c.kb, c.gx, c.gy)) put(b);
+        }
       }
     }
+    e = en;
+    pm = pn;
   }
 }
 
