@@ -213,11 +213,18 @@ __device__ bool contact_native(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R e
 }
 
 // Per-pair results before compaction: SoA of 10 reals, body ids, hit flag.
+// One contact per pair index, written only for hits: a record per pair, so
+// the compaction reads each hit's own bytes (component arrays strided by the
+// pair count made it read nearly every line of all ten of them).
+template <class R> struct alignas(16) PairRec {
+  R v[10];         // N(3) p1(3) p2(3) depth
+  int32_t ids[2];  // body ids
+};
+
 template <class R> struct PairOut {
-  R* v;          // 10 * P: N(3) p1(3) p2(3) depth
-  int32_t* ids;  // 2 * P
-  uint32_t* hit; // P
-  uint64_t P;    // capacity: the stride of the SoA arrays
+  PairRec<R>* rec;  // P records
+  uint32_t* hit;    // P hit flags
+  uint64_t P;       // capacity
 };
 
 __device__ __forceinline__ uint64_t dev_count(uint64_t cap, const uint64_t* d_n) {
@@ -228,13 +235,15 @@ __device__ __forceinline__ uint64_t dev_count(uint64_t cap, const uint64_t* d_n)
 template <class R>
 __device__ __forceinline__ void store_pair(const PairOut<R>& o, uint64_t P, uint64_t i, const Contact<R>& c,
                                            uint32_t ida, uint32_t idb) {
-  R* v = o.v;
-  v[i] = c.N.x; v[P + i] = c.N.y; v[2 * P + i] = c.N.z;
-  v[3 * P + i] = c.p1.x; v[4 * P + i] = c.p1.y; v[5 * P + i] = c.p1.z;
-  v[6 * P + i] = c.p2.x; v[7 * P + i] = c.p2.y; v[8 * P + i] = c.p2.z;
-  v[9 * P + i] = c.depth;
-  o.ids[2 * i] = (int32_t)ida;
-  o.ids[2 * i + 1] = (int32_t)idb;
+  (void)P;
+  PairRec<R> r;
+  r.v[0] = c.N.x; r.v[1] = c.N.y; r.v[2] = c.N.z;
+  r.v[3] = c.p1.x; r.v[4] = c.p1.y; r.v[5] = c.p1.z;
+  r.v[6] = c.p2.x; r.v[7] = c.p2.y; r.v[8] = c.p2.z;
+  r.v[9] = c.depth;
+  r.ids[0] = (int32_t)ida;
+  r.ids[1] = (int32_t)idb;
+  o.rec[i] = r;
   o.hit[i] = 1u;
 }
 
@@ -583,11 +592,11 @@ __global__ __launch_bounds__(COMPACT_BS) void k_np_compact(uint64_t cap, const u
     const uint32_t k = pos[i];
     if (o.hit[i]) {
       const uint32_t sl = k - k0;
-      const R* v = o.v;
+      const PairRec<R> r = o.rec[i];
 #pragma unroll
-      for (int c = 0; c < 10; ++c) lv[c][sl] = v[(uint64_t)c * P + i];
-      lid[2 * sl] = o.ids[2 * i];
-      lid[2 * sl + 1] = o.ids[2 * i + 1];
+      for (int c = 0; c < 10; ++c) lv[c][sl] = r.v[c];
+      lid[2 * sl] = r.ids[0];
+      lid[2 * sl + 1] = r.ids[1];
       perm[k] = pairs[i];
     } else {
       perm[C + (i - k)] = pairs[i];

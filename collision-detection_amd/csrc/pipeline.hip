@@ -396,7 +396,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t device_bytes() const override {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
-                           &pcount_, &pairs_, &np_v_, &np_ids_, &np_hit_, &np_pos_, &queue_,
+                           &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_,
                            &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_};
     uint64_t t = 0;
@@ -645,7 +645,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     // keys + ping-pong and per-entry pair counts/offsets; pairs, narrowphase
     // records, queues and contact outputs per pair; slot counts/offsets
     const double per_m = 2 * 8 + 4 + 16;
-    const double per_p = 2 * 8 + 10 * sizeof(R) + 8 + 4 + 4 + 2 * 8 + 4 + 1 + 10 * sizeof(R) + 8 + 8;
+    const double per_p = 2 * 8 + sizeof(PairRec<R>) + 4 + 4 + 2 * 8 + 4 + 1 + 10 * sizeof(R) + 8 + 8;
     return (uint64_t)((double)n * (cap_m_per_shape_ * per_m + cap_p_per_shape_ * per_p + 8.0)) + (1u << 20);
   }
 
@@ -1011,8 +1011,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
                          (int)ZERO_NP);
     const uint64_t P = std::max<uint64_t>(pcap_, 1);
-    np_v_.reserve(P * 10 * sizeof(R));
-    np_ids_.reserve(P * 8);
+    np_v_.reserve(P * sizeof(PairRec<R>));
     np_hit_.reserve(P * 4);
     np_pos_.reserve(P * 4);
     queue_.reserve(P * 8);
@@ -1021,7 +1020,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     qcls_.reserve(P);
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(P), 16) * 4);
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(P, CLASS_BITS), 16) * 4);
-    PairOut<R> o{np_v_.as<R>(), np_ids_.as<int32_t>(), np_hit_.as<uint32_t>(), P};
+    PairOut<R> o{np_v_.as<PairRec<R>>(), np_hit_.as<uint32_t>(), P};
     const uint64_t* dP = &st()->P;
     mark(ST_NP_MAIN, true);
     hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, dP,
@@ -1196,8 +1195,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     runs_part_.reserve((mcap_ / PAIR_BS + 1) * 4);
     pairs_.reserve(P * 8);
     longs_.reserve((P / (GROUP_LONG + 1) + 1) * 4);
-    np_v_.reserve(P * 10 * sizeof(R));
-    np_ids_.reserve(P * 8);
+    np_v_.reserve(P * sizeof(PairRec<R>));
     np_hit_.reserve(P * 4);
     np_pos_.reserve(P * 4);
     queue_.reserve(P * 8);
@@ -1298,7 +1296,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
   DevBuf chunks_, sel_, desc_, souts_;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;
-  DevBuf np_v_, np_ids_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
+  DevBuf np_v_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
   DevBuf scan_tmp_, radix_tmp_, runs_part_;
   bool opened_ = false;  // the step's totals were zeroed by one k_stage_begin
