@@ -124,9 +124,12 @@ class BatchBase {
   // bodies first_body .. first_body+k-1 in order and the id list is unused.
   virtual void update_bodies_pinned(uint64_t k, const void* block, int64_t first_body) = 0;
   // After finish(): contact range of selected scene j, and every contact of
-  // the step (global body ids) copied into host arrays of totals().c rows.
+  // the step (global body ids) in one copy of contacts_packed_bytes() bytes:
+  // with C = totals().c and R the real type, normals (3C R), points A (3C R),
+  // points B (3C R), depths (C R), then body id pairs (2C int32).
   virtual void scene_contact_range(uint64_t j, uint64_t* begin, uint64_t* end) const = 0;
-  virtual void read_all_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) = 0;
+  virtual uint64_t contacts_packed_bytes() const = 0;
+  virtual void read_all_contacts_packed(void* dst) = 0;
   virtual uint64_t scene_bytes_estimate(uint64_t n_shapes, uint64_t n_bodies) const = 0;
   // Bytes a scene keeps resident, and the step buffers of a step over n shapes.
   virtual uint64_t resident_bytes_estimate(uint64_t n_shapes, uint64_t n_bodies) const = 0;

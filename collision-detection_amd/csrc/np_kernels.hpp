@@ -570,9 +570,7 @@ template <class R>
 __global__ __launch_bounds__(COMPACT_BS) void k_np_compact(uint64_t cap, const uint64_t* __restrict__ d_np,
                                                            PairOut<R> o, const uint32_t* __restrict__ pos,
                                                            const uint64_t* __restrict__ pairs,
-                                                           const DevStatus* __restrict__ st, R* __restrict__ norm,
-                                                           R* __restrict__ cpta, R* __restrict__ cptb,
-                                                           R* __restrict__ dpth, int32_t* __restrict__ bids,
+                                                           const DevStatus* __restrict__ st, R* __restrict__ out,
                                                            uint64_t* __restrict__ perm) {
   __shared__ R lv[10][COMPACT_BS];
   __shared__ int32_t lid[2 * COMPACT_BS];
@@ -584,6 +582,12 @@ __global__ __launch_bounds__(COMPACT_BS) void k_np_compact(uint64_t cap, const u
   const uint64_t i = i0 + threadIdx.x;
   const uint64_t last = min(i0 + COMPACT_BS, n) - 1;
   const uint64_t C = st->C;
+  // the contact arrays, one after the other in one buffer, each C rows long
+  R* __restrict__ norm = out;
+  R* __restrict__ cpta = out + 3 * C;
+  R* __restrict__ cptb = out + 6 * C;
+  R* __restrict__ dpth = out + 9 * C;
+  int32_t* __restrict__ bids = reinterpret_cast<int32_t*>(out + 10 * C);
   if (threadIdx.x == 0) s_k0 = pos[i0];
   if (i == last) s_nh = pos[i] + (o.hit[i] ? 1u : 0u);
   __syncthreads();

@@ -397,7 +397,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
-                           &queue_tmp_, &qsel_, &qcls_, &out_norm_, &out_pa_, &out_pb_, &out_d_, &out_ids_, &perm_,
+                           &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
                            &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
@@ -610,14 +610,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     *end = souts_host_[j].contact_end;
   }
 
-  void read_all_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) override {
-    const uint64_t n = C_;
-    if (!n) return;
-    if (norm) CDX_HIP_CHECK(hipMemcpyAsync(norm, out_norm_.p, n * 3 * sizeof(R), hipMemcpyDeviceToHost, s_));
-    if (cpta) CDX_HIP_CHECK(hipMemcpyAsync(cpta, out_pa_.p, n * 3 * sizeof(R), hipMemcpyDeviceToHost, s_));
-    if (cptb) CDX_HIP_CHECK(hipMemcpyAsync(cptb, out_pb_.p, n * 3 * sizeof(R), hipMemcpyDeviceToHost, s_));
-    if (dpth) CDX_HIP_CHECK(hipMemcpyAsync(dpth, out_d_.p, n * sizeof(R), hipMemcpyDeviceToHost, s_));
-    if (bids) CDX_HIP_CHECK(hipMemcpyAsync(bids, out_ids_.p, n * 8, hipMemcpyDeviceToHost, s_));
+  uint64_t contacts_packed_bytes() const override { return C_ * (10 * sizeof(R) + 8); }
+
+  void read_all_contacts_packed(void* dst) override {
+    if (!C_) return;
+    CDX_HIP_CHECK(hipMemcpyAsync(dst, out_.p, contacts_packed_bytes(), hipMemcpyDeviceToHost, s_));
     sync();
   }
 
@@ -1074,16 +1071,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_NP_COMPACT, true);
     scan_excl_u32(np_hit_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->C, scan_tmp_.as<uint32_t>(), s_, dP);
     // contacts are at most the pairs: the outputs are sized by the pair capacity
-    out_norm_.reserve(P * 3 * sizeof(R));
-    out_pa_.reserve(P * 3 * sizeof(R));
-    out_pb_.reserve(P * 3 * sizeof(R));
-    out_d_.reserve(P * sizeof(R));
-    out_ids_.reserve(P * 8);
+    out_.reserve(P * (10 * sizeof(R) + 8));
     perm_.reserve(P * 8);
     hipLaunchKernelGGL(k_np_compact<R>, dim3(blocks_for(P, COMPACT_BS)), dim3(COMPACT_BS), 0, s_, P, dP, o,
                        (const uint32_t*)np_pos_.as<uint32_t>(), (const uint64_t*)pairs_.as<uint64_t>(),
-                       (const DevStatus*)st(), out_norm_.as<R>(), out_pa_.as<R>(), out_pb_.as<R>(), out_d_.as<R>(),
-                       out_ids_.as<int32_t>(), perm_.as<uint64_t>());
+                       (const DevStatus*)st(), out_.as<R>(), perm_.as<uint64_t>());
     mark(ST_NP_COMPACT, false);
     if (exact_) {
       read_status();
@@ -1202,11 +1194,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     queue_tmp_.reserve(P * 8);
     qsel_.reserve(P * 4);
     qcls_.reserve(P);
-    out_norm_.reserve(P * 3 * sizeof(R));
-    out_pa_.reserve(P * 3 * sizeof(R));
-    out_pb_.reserve(P * 3 * sizeof(R));
-    out_d_.reserve(P * sizeof(R));
-    out_ids_.reserve(P * 8);
+    out_.reserve(P * (10 * sizeof(R) + 8));
     perm_.reserve(P * 8);
   }
 
@@ -1214,12 +1202,16 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void copy_contacts(uint64_t begin, uint64_t n, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids,
                      uint64_t body_base) {
     if (!n) return;
-    if (norm) CDX_HIP_CHECK(hipMemcpy(norm, out_norm_.as<R>() + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
-    if (cpta) CDX_HIP_CHECK(hipMemcpy(cpta, out_pa_.as<R>() + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
-    if (cptb) CDX_HIP_CHECK(hipMemcpy(cptb, out_pb_.as<R>() + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
-    if (dpth) CDX_HIP_CHECK(hipMemcpy(dpth, out_d_.as<R>() + begin, n * sizeof(R), hipMemcpyDeviceToHost));
+    // the step's contact arrays lie one after the other, each C_ rows long
+    const R* o = out_.as<R>();
+    const uint64_t C = C_;
+    if (norm) CDX_HIP_CHECK(hipMemcpy(norm, o + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
+    if (cpta) CDX_HIP_CHECK(hipMemcpy(cpta, o + 3 * C + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
+    if (cptb) CDX_HIP_CHECK(hipMemcpy(cptb, o + 6 * C + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
+    if (dpth) CDX_HIP_CHECK(hipMemcpy(dpth, o + 9 * C + begin, n * sizeof(R), hipMemcpyDeviceToHost));
     if (bids) {
-      CDX_HIP_CHECK(hipMemcpy(bids, out_ids_.as<int32_t>() + 2 * begin, n * 8, hipMemcpyDeviceToHost));
+      CDX_HIP_CHECK(hipMemcpy(bids, reinterpret_cast<const int32_t*>(o + 10 * C) + 2 * begin, n * 8,
+                              hipMemcpyDeviceToHost));
       if (body_base)
         for (uint64_t i = 0; i < 2 * n; ++i) bids[i] -= (int32_t)body_base;
     }
@@ -1297,7 +1289,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf chunks_, sel_, desc_, souts_;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;
   DevBuf np_v_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
-  DevBuf out_norm_, out_pa_, out_pb_, out_d_, out_ids_, perm_;
+  DevBuf out_, perm_;  // out_: the step's contact arrays back to back, C rows each
   DevBuf scan_tmp_, radix_tmp_, runs_part_;
   bool opened_ = false;  // the step's totals were zeroed by one k_stage_begin
 };

@@ -398,11 +398,8 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
     const uint64_t C = e.totals().c;
     auto blk = get_block(k, C);
     double t5 = now_ms();
-    const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
-    char* base = static_cast<char*>(blk->base);
-    const uint64_t cap = blk->cap;
-    e.read_all_contacts(base, base + 3 * rs * cap, base + 6 * rs * cap, base + 9 * rs * cap,
-                        reinterpret_cast<int32_t*>(base + 10 * rs * cap));
+    e.read_all_contacts_packed(blk->base);  // one copy: the arrays are C rows each
+    blk->rows = C;
     double t6 = now_ms();
     ph[0] = t1 - t0; ph[1] = t2 - t1; ph[2] = t3 - t2; ph[3] = t4 - t3; ph[4] = t5 - t4; ph[5] = t6 - t5;
     for (uint64_t j = 0; j < batch.size(); ++j) {
@@ -486,7 +483,7 @@ void Server::read_contacts(uint64_t id, void* norm, void* cpta, void* cptb, void
   if (!n) return;
   const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
   const char* base = static_cast<const char*>(r->block->base);
-  const uint64_t bc = r->block->cap, b0 = r->c_begin;
+  const uint64_t bc = r->block->rows, b0 = r->c_begin;
   if (norm) std::memcpy(norm, base + 3 * rs * b0, 3 * rs * n);
   if (cpta) std::memcpy(cpta, base + 3 * rs * (bc + b0), 3 * rs * n);
   if (cptb) std::memcpy(cptb, base + 3 * rs * (2 * bc + b0), 3 * rs * n);

@@ -47,9 +47,12 @@ struct ServerStats {
 struct PoseBlock;
 
 // Pinned host block holding the contacts of one batched step.
+// The step's contact arrays lie back to back, each `rows` (the step's
+// contact count) long: normals, points A, points B, depths, body id pairs.
 struct ResultBlock {
   void* base = nullptr;
-  uint64_t cap = 0;  // contacts
+  uint64_t cap = 0;   // contacts
+  uint64_t rows = 0;  // contacts of the step that filled it
   ~ResultBlock();
 };
 
