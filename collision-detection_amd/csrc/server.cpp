@@ -7,8 +7,9 @@
 // on their own HIP streams, so steps of different classes overlap on the
 // device.  A request's pose inputs are copied at submit into a pinned
 // block; the worker uploads each block with one asynchronous copy ahead of
-// the step, and reads every contact of the step back with one copy per
-// output array into a pinned result block that the step's requests share.
+// the step, and reads every contact of the step back with one copy (the
+// step's contact arrays lie back to back on the device) into a pinned
+// result block that the step's requests share.
 #include "server.hpp"
 
 #include <algorithm>
