@@ -123,18 +123,6 @@ class BatchBase {
   // pose_block_rot_offset(k).  first_body >= 0: the updates cover global
   // bodies first_body .. first_body+k-1 in order and the id list is unused.
   virtual void update_bodies_pinned(uint64_t k, const void* block, int64_t first_body) = 0;
-  // The same update in two halves, so the copy of the next step's poses
-  // overlaps the current step: prefetch_copy puts a pose block (same layout)
-  // into staging slot `slot` (0 or 1) at byte `offset` (a multiple of 256)
-  // on a second stream; prefetch_mark closes the slot's copies;
-  // update_bodies_prefetched then makes the batch stream wait for them and
-  // applies the block.  A slot must not be refilled before the step that
-  // applied it has finished.  prefetch_reserve grows a slot (no copy of
-  // that slot may be in flight).
-  virtual void prefetch_reserve(int slot, uint64_t bytes) = 0;
-  virtual void prefetch_copy(int slot, uint64_t offset, uint64_t k, const void* block, int64_t first_body) = 0;
-  virtual void prefetch_mark(int slot) = 0;
-  virtual void update_bodies_prefetched(int slot, uint64_t offset, uint64_t k, int64_t first_body) = 0;
   // After finish(): contact range of selected scene j, and every contact of
   // the step (global body ids) in one copy of contacts_packed_bytes() bytes:
   // with C = totals().c and R the real type, normals (3C R), points A (3C R),

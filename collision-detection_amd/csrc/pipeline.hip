@@ -191,11 +191,6 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     (void)hipHostFree(hhdr_);
     if (stage_host_) (void)hipHostFree(stage_host_);
     if (souts_pinned_) (void)hipHostFree(souts_pinned_);
-    if (cs_) {
-      (void)hipStreamSynchronize(cs_);
-      for (int i = 0; i < 2; ++i) (void)hipEventDestroy(ev_pre_[i]);
-      (void)hipStreamDestroy(cs_);
-    }
     (void)hipStreamDestroy(s_);
   }
 
@@ -403,7 +398,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
-                           &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &pre_[0], &pre_[1]};
+                           &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -590,55 +585,14 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   void update_bodies_pinned(uint64_t k, const void* block, int64_t first_body) override {
     if (k == 0) return;
+    const uint64_t nbt = scenes_.empty() ? 0 : scenes_.back().body_begin + scenes_.back().n_bodies;
     const int kind = sizeof(R) == 8 ? REAL_F64 : REAL_F32;
     const uint64_t bytes = pose_block_bytes(k, kind);
     upd_.reserve(bytes);
     const char* hb = static_cast<const char*>(block);
     const uint64_t from = first_body >= 0 ? pose_block_pos_offset(k) : 0;  // contiguous: no id list
     CDX_HIP_CHECK(hipMemcpyAsync(upd_.as<char>() + from, hb + from, bytes - from, hipMemcpyHostToDevice, s_));
-    scatter_bodies(upd_.as<char>(), k, first_body);
-  }
-
-  void prefetch_reserve(int slot, uint64_t bytes) override {
-    CDX_REQUIRE(slot == 0 || slot == 1, ERR_INVALID, "prefetch slot must be 0 or 1");
-    pre_[slot].reserve(bytes);  // grows only (waits for the device when it does)
-  }
-
-  void prefetch_copy(int slot, uint64_t offset, uint64_t k, const void* block, int64_t first_body) override {
-    if (k == 0) return;
-    CDX_REQUIRE(slot == 0 || slot == 1, ERR_INVALID, "prefetch slot must be 0 or 1");
-    const uint64_t bytes = pose_block_bytes(k, sizeof(R) == 8 ? REAL_F64 : REAL_F32);
-    CDX_REQUIRE(offset % 256 == 0 && offset + bytes <= pre_[slot].bytes, ERR_INVALID,
-                "pose block outside the prefetch slot");
-    if (!cs_) {
-      CDX_HIP_CHECK(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking));
-      for (int i = 0; i < 2; ++i) CDX_HIP_CHECK(hipEventCreateWithFlags(&ev_pre_[i], hipEventDisableTiming));
-    }
-    const uint64_t from = first_body >= 0 ? pose_block_pos_offset(k) : 0;
-    CDX_HIP_CHECK(hipMemcpyAsync(pre_[slot].as<char>() + offset + from, static_cast<const char*>(block) + from,
-                                 bytes - from, hipMemcpyHostToDevice, cs_));
-  }
-
-  void prefetch_mark(int slot) override {
-    CDX_REQUIRE(slot == 0 || slot == 1, ERR_INVALID, "prefetch slot must be 0 or 1");
-    if (cs_) CDX_HIP_CHECK(hipEventRecord(ev_pre_[slot], cs_));
-  }
-
-  void update_bodies_prefetched(int slot, uint64_t offset, uint64_t k, int64_t first_body) override {
-    if (k == 0) return;
-    CDX_REQUIRE((slot == 0 || slot == 1) && cs_, ERR_INVALID, "nothing prefetched into this slot");
-    const uint64_t bytes = pose_block_bytes(k, sizeof(R) == 8 ? REAL_F64 : REAL_F32);
-    CDX_REQUIRE(offset % 256 == 0 && offset + bytes <= pre_[slot].bytes, ERR_INVALID,
-                "pose block outside the prefetch slot");
-    CDX_HIP_CHECK(hipStreamWaitEvent(s_, ev_pre_[slot], 0));
-    scatter_bodies(pre_[slot].as<char>() + offset, k, first_body);
-  }
-
-  // Applies a pose block already in device memory at `base` (layout of
-  // update_bodies_pinned) on the batch stream.
-  void scatter_bodies(const char* base, uint64_t k, int64_t first_body) {
-    const uint64_t nbt = scenes_.empty() ? 0 : scenes_.back().body_begin + scenes_.back().n_bodies;
-    const int kind = sizeof(R) == 8 ? REAL_F64 : REAL_F32;
+    const char* base = upd_.as<char>();
     const uint32_t* didx = first_body >= 0 ? nullptr : (const uint32_t*)base;
     const uint64_t b0 = first_body >= 0 ? (uint64_t)first_body : 0;
     CDX_REQUIRE(first_body < 0 || b0 + k <= nbt, ERR_INVALID, "pose update past the resident bodies");
@@ -1325,9 +1279,6 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   float tau_ = 1e-2f;
 
   DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, meta_, scount_, soff_, selrng_;
-  DevBuf pre_[2];                          // pose blocks of the next served step
-  hipStream_t cs_ = nullptr;               // their copies (created on first use)
-  hipEvent_t ev_pre_[2] = {nullptr, nullptr};
   void* stage_host_ = nullptr;
   size_t stage_host_bytes_ = 0;
   void* souts_pinned_ = nullptr;

@@ -9,9 +9,7 @@
 // block; the worker uploads each block with one asynchronous copy ahead of
 // the step, and reads every contact of the step back with one copy (the
 // step's contact arrays lie back to back on the device) into a pinned
-// result block that the step's requests share.  Requests that arrive while
-// a step runs are taken as the next batch right after its launch, and
-// their poses are copied on the engine's second stream meanwhile.
+// result block that the step's requests share.
 #include "server.hpp"
 
 #include <algorithm>
@@ -312,83 +310,16 @@ int Server::scenes_cls(uint32_t s) const {
   return scenes_[s].cls;
 }
 
-void Server::take_batch(Class& k, const std::vector<std::shared_ptr<Request>>* busy,
-                        std::vector<std::shared_ptr<Request>>& out) {
-  uint64_t shapes = 0;
-  std::unordered_set<uint32_t> taken;
-  if (busy)
-    for (auto& r : *busy) taken.insert(r->scene);  // their next requests wait for the running step
-  for (auto it = k.queue.begin(); it != k.queue.end();) {
-    const auto& r = *it;
-    if (taken.count(r->scene)) { ++it; continue; }  // the scene's next step waits for this one
-    const uint64_t n = r->n;
-    if (!out.empty() && shapes + n > cfg_.max_batch_shapes) break;
-    shapes += n;
-    taken.insert(r->scene);
-    r->state = 1;
-    out.push_back(r);
-    it = k.queue.erase(it);
-  }
-}
-
-void Server::prefetch_next(Class& k, BatchBase& e, const std::vector<std::shared_ptr<Request>>& running,
-                           std::vector<std::shared_ptr<Request>>& next, std::vector<uint64_t>& offs, int slot) {
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    if (stop_ || k.queue.empty()) return;
-    take_batch(k, &running, next);
-  }
-  if (next.empty()) return;
-  try {
-    const int kind = cfg_.real_kind;
-    uint64_t off = 0;
-    offs.assign(next.size(), 0);
-    for (uint64_t j = 0; j < next.size(); ++j) {
-      offs[j] = off;
-      if (next[j]->k) off += (pose_block_bytes(next[j]->k, kind) + 255) & ~255ull;
-    }
-    e.prefetch_reserve(slot, std::max<uint64_t>(off, 256));
-    for (uint64_t j = 0; j < next.size(); ++j) {
-      const auto& r = next[j];
-      if (r->k) e.prefetch_copy(slot, offs[j], r->k, r->poses->p, r->contiguous ? (int64_t)r->body_base : -1);
-    }
-    e.prefetch_mark(slot);
-  } catch (const std::exception& ex) {
-    // a batch never starts from a half-filled slot: its requests fail here
-    std::lock_guard<std::mutex> lk(mu_);
-    for (auto& r : next) {
-      r->poses.reset();
-      --pending_;
-      r->state = 3;
-      r->err = ex.what();
-      ++st_fail_;
-    }
-    next.clear();
-    offs.clear();
-    cv_done_.notify_all();
-  }
-}
-
 void Server::worker_loop(int c) {
   {
     GateShared gate;
     (void)hipSetDevice(dev_);
   }
   Class& k = *classes_[c];
-  std::vector<std::shared_ptr<Request>> batch, next;
-  std::vector<uint64_t> offs, next_offs;
-  int slot = -1;  // engine slot holding the poses of `batch` (-1: not prefetched)
+  std::vector<std::shared_ptr<Request>> batch;
   for (;;) {
-    if (!next.empty()) {
-      batch.swap(next);
-      offs.swap(next_offs);
-      next.clear();
-      next_offs.clear();
-      slot = slot < 0 ? 0 : 1 - slot;  // the slot the prefetch used
-    } else {
-      batch.clear();
-      offs.clear();
-      slot = -1;
+    batch.clear();
+    {
       std::unique_lock<std::mutex> lk(mu_);
       cv_work_.wait(lk, [&] { return stop_ || !k.queue.empty(); });
       if (stop_) {
@@ -400,10 +331,21 @@ void Server::worker_loop(int c) {
         cv_done_.notify_all();
         return;
       }
-      take_batch(k, nullptr, batch);
+      uint64_t shapes = 0;
+      std::unordered_set<uint32_t> taken;
+      for (auto it = k.queue.begin(); it != k.queue.end();) {
+        const auto& r = *it;
+        if (taken.count(r->scene)) { ++it; continue; }  // the scene's next step waits for this one
+        const uint64_t n = r->n;
+        if (!batch.empty() && shapes + n > cfg_.max_batch_shapes) break;
+        shapes += n;
+        taken.insert(r->scene);
+        r->state = 1;
+        batch.push_back(r);
+        it = k.queue.erase(it);
+      }
     }
-    // the next batch's poses go to the slot this batch does not use
-    run_batch(c, batch, slot, offs, &next, &next_offs, slot < 0 ? 0 : 1 - slot);
+    run_batch(c, batch);
   }
 }
 
@@ -430,9 +372,7 @@ std::shared_ptr<ResultBlock> Server::get_block(Class& k, uint64_t contacts) {
   return b;
 }
 
-void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch, int slot,
-                       const std::vector<uint64_t>& offs, std::vector<std::shared_ptr<Request>>* next,
-                       std::vector<uint64_t>* next_offs, int next_slot) {
+void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
   Class& k = *classes_[c];
   std::string err;
   double step_ms = 0.0;
@@ -444,13 +384,8 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch, int 
     std::vector<uint32_t> sel;
     sel.reserve(batch.size());
     double t0 = now_ms();
-    for (uint64_t j = 0; j < batch.size(); ++j) {
-      auto& r = batch[j];
-      const int64_t first = r->contiguous ? (int64_t)r->body_base : -1;
-      if (r->k) {
-        if (slot >= 0) e.update_bodies_prefetched(slot, offs[j], r->k, first);
-        else e.update_bodies_pinned(r->k, r->poses->p, first);
-      }
+    for (auto& r : batch) {
+      if (r->k) e.update_bodies_pinned(r->k, r->poses->p, r->contiguous ? (int64_t)r->body_base : -1);
       sel.push_back(r->local);
     }
     double t1 = now_ms();
@@ -458,9 +393,6 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch, int 
     double t2 = now_ms();
     e.step(cfg_.use_graph);
     double t3 = now_ms();
-    if (next) prefetch_next(k, e, batch, *next, *next_offs, next_slot);
-    ph[0] += now_ms() - t3;
-    t3 = now_ms();
     e.finish();
     double t4 = now_ms();
     step_ms = e.last_step_ms();
@@ -470,7 +402,7 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch, int 
     e.read_all_contacts_packed(blk->base);  // one copy: the arrays are C rows each
     blk->rows = C;
     double t6 = now_ms();
-    ph[0] += t1 - t0; ph[1] = t2 - t1; ph[2] = t3 - t2; ph[3] = t4 - t3; ph[4] = t5 - t4; ph[5] = t6 - t5;
+    ph[0] = t1 - t0; ph[1] = t2 - t1; ph[2] = t3 - t2; ph[3] = t4 - t3; ph[4] = t5 - t4; ph[5] = t6 - t5;
     for (uint64_t j = 0; j < batch.size(); ++j) {
       uint64_t b0, b1;
       e.scene_contact_range(j, &b0, &b1);

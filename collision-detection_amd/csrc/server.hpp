@@ -129,20 +129,7 @@ class Server {
 
   uint64_t enqueue(std::shared_ptr<Request> req, int cls, double t_enter);
   void worker_loop(int c);
-  // Runs one step.  slot >= 0: the batch's poses were prefetched into that
-  // engine slot at offs.  next != nullptr: while the step runs, the next
-  // batch is taken from the queue and its poses prefetched into next_slot.
-  void run_batch(int c, std::vector<std::shared_ptr<Request>>& batch, int slot, const std::vector<uint64_t>& offs,
-                 std::vector<std::shared_ptr<Request>>* next, std::vector<uint64_t>* next_offs, int next_slot);
-  // Takes the next batch from class c's queue (oldest first, one request per
-  // scene, scenes in `busy` skipped, at most max_batch_shapes shapes);
-  // caller holds mu_.
-  void take_batch(Class& k, const std::vector<std::shared_ptr<Request>>* busy,
-                  std::vector<std::shared_ptr<Request>>& out);
-  // While a step of class k runs: takes the next batch and copies its poses
-  // into engine slot `slot` on the engine's second stream.
-  void prefetch_next(Class& k, BatchBase& e, const std::vector<std::shared_ptr<Request>>& running,
-                     std::vector<std::shared_ptr<Request>>& next, std::vector<uint64_t>& offs, int slot);
+  void run_batch(int c, std::vector<std::shared_ptr<Request>>& batch);
   std::shared_ptr<ResultBlock> get_block(Class& k, uint64_t contacts);
   int class_of(uint64_t n) const;
   int scenes_cls(uint32_t s) const;
