@@ -123,6 +123,10 @@ class BatchBase {
   // pose_block_rot_offset(k).  first_body >= 0: the updates cover global
   // bodies first_body .. first_body+k-1 in order and the id list is unused.
   virtual void update_bodies_pinned(uint64_t k, const void* block, int64_t first_body) = 0;
+  // The same update from a block already copied to device memory (same
+  // layout) by another stream: the batch stream waits for `ready`, recorded
+  // after that copy.  The block stays untouched until finish() returns.
+  virtual void update_bodies_device(uint64_t k, const void* dev_block, int64_t first_body, hipEvent_t ready) = 0;
   // After finish(): contact range of selected scene j, and every contact of
   // the step (global body ids) in one copy of contacts_packed_bytes() bytes:
   // with C = totals().c and R the real type, normals (3C R), points A (3C R),

@@ -585,14 +585,27 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   void update_bodies_pinned(uint64_t k, const void* block, int64_t first_body) override {
     if (k == 0) return;
-    const uint64_t nbt = scenes_.empty() ? 0 : scenes_.back().body_begin + scenes_.back().n_bodies;
     const int kind = sizeof(R) == 8 ? REAL_F64 : REAL_F32;
     const uint64_t bytes = pose_block_bytes(k, kind);
     upd_.reserve(bytes);
     const char* hb = static_cast<const char*>(block);
     const uint64_t from = first_body >= 0 ? pose_block_pos_offset(k) : 0;  // contiguous: no id list
     CDX_HIP_CHECK(hipMemcpyAsync(upd_.as<char>() + from, hb + from, bytes - from, hipMemcpyHostToDevice, s_));
-    const char* base = upd_.as<char>();
+    scatter_bodies(upd_.as<char>(), k, first_body);
+  }
+
+  void update_bodies_device(uint64_t k, const void* dev_block, int64_t first_body, hipEvent_t ready) override {
+    if (k == 0) return;
+    CDX_REQUIRE(dev_block && ready, ERR_INVALID, "device pose block without its copy event");
+    CDX_HIP_CHECK(hipStreamWaitEvent(s_, ready, 0));
+    scatter_bodies(static_cast<const char*>(dev_block), k, first_body);
+  }
+
+  // Applies a pose block in device memory at `base` (layout of
+  // update_bodies_pinned) on the batch stream.
+  void scatter_bodies(const char* base, uint64_t k, int64_t first_body) {
+    const uint64_t nbt = scenes_.empty() ? 0 : scenes_.back().body_begin + scenes_.back().n_bodies;
+    const int kind = sizeof(R) == 8 ? REAL_F64 : REAL_F32;
     const uint32_t* didx = first_body >= 0 ? nullptr : (const uint32_t*)base;
     const uint64_t b0 = first_body >= 0 ? (uint64_t)first_body : 0;
     CDX_REQUIRE(first_body < 0 || b0 + k <= nbt, ERR_INVALID, "pose update past the resident bodies");

@@ -6,8 +6,10 @@
 // most one request per scene, up to max_batch_shapes shapes.  Classes step
 // on their own HIP streams, so steps of different classes overlap on the
 // device.  A request's pose inputs are copied at submit into a pinned
-// block; the worker uploads each block with one asynchronous copy ahead of
-// the step, and reads every contact of the step back with one copy (the
+// block, and from there to a device block by an asynchronous copy started
+// right away on the server's upload stream; the step waits for that copy
+// and applies the block.  The worker reads every contact of the step back
+// with one copy (the
 // step's contact arrays lie back to back on the device) into a pinned
 // result block that the step's requests share.
 #include "server.hpp"
@@ -80,10 +82,69 @@ ResultBlock::~ResultBlock() {
   }
 }
 
+// Device pose blocks (bucketed by power-of-two size) and copy events,
+// reused across requests; owned jointly by the server and the blocks in use.
+struct DevPools {
+  std::mutex mu;
+  std::map<uint64_t, std::vector<void*>> blocks;
+  std::vector<hipEvent_t> events;
+  ~DevPools() {
+    GateShared g;
+    for (auto& kv : blocks)
+      for (void* p : kv.second) (void)hipFree(p);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
+  void* get_block(uint64_t bytes, uint64_t* bucket) {
+    uint64_t b = 4096;
+    while (b < bytes) b <<= 1;
+    *bucket = b;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto& v = blocks[b];
+      if (!v.empty()) {
+        void* p = v.back();
+        v.pop_back();
+        return p;
+      }
+    }
+    void* p = nullptr;
+    CDX_HIP_CHECK(hipMalloc(&p, b));
+    return p;
+  }
+  hipEvent_t get_event() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!events.empty()) {
+        hipEvent_t e = events.back();
+        events.pop_back();
+        return e;
+      }
+    }
+    hipEvent_t e = nullptr;
+    CDX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+  }
+  void put(void* p, uint64_t bucket, hipEvent_t e) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (p) blocks[bucket].push_back(p);
+    if (e) events.push_back(e);
+  }
+};
+
 struct PoseBlock {
-  void* p = nullptr;
+  void* p = nullptr;  // pinned host block
   uint64_t bucket = 0;
+  void* dev = nullptr;  // device copy, started at submit
+  uint64_t dev_bucket = 0;
+  hipEvent_t ev = nullptr;  // recorded after that copy
+  std::shared_ptr<DevPools> pools;
+  // No wait here: a request that ran has had its copy waited for by its
+  // step; the copy of one that never ran may still be in flight, but every
+  // later copy into the recycled device block is ordered after it on the
+  // same stream, and what it reads from the recycled host block only ends
+  // up in that device block.
   ~PoseBlock() {
+    if (pools) pools->put(dev, dev_bucket, ev);
     if (p) pinned_pool().put(p, bucket);
   }
 };
@@ -98,6 +159,8 @@ Server::Server(int device, const ServerConfig& cfg) : dev_(device), cfg_(cfg), t
   CDX_REQUIRE(cfg.real_kind == REAL_F64 || cfg.real_kind == REAL_F32, ERR_INVALID, "unknown real kind");
   CDX_REQUIRE(cfg.max_batch_shapes > 0 && cfg.max_pending > 0, ERR_INVALID, "batch and queue limits must be positive");
   CDX_HIP_CHECK(hipSetDevice(dev_));
+  CDX_HIP_CHECK(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+  pools_ = std::make_shared<DevPools>();
   size_t free_b = 0, total_b = 0;
   CDX_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   budget_ = cfg.hbm_budget ? cfg.hbm_budget : (uint64_t)(0.9 * (double)free_b);
@@ -125,6 +188,15 @@ Server::~Server() {
   cv_work_.notify_all();
   for (auto& k : classes_)
     if (k->worker.joinable()) k->worker.join();
+  {
+    GateShared g;
+    (void)hipStreamSynchronize(up_);
+  }
+  reqs_.clear();
+  buffers_.clear();
+  pools_.reset();
+  GateShared g;
+  (void)hipStreamDestroy(up_);
 }
 
 int Server::class_of(uint64_t n) const {
@@ -235,7 +307,22 @@ uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void*
   return enqueue(std::move(req), ref.cls, t_enter);
 }
 
+void Server::upload_poses(Request& r) {
+  if (!r.k || !r.poses) return;
+  PoseBlock& pb = *r.poses;
+  const uint64_t bytes = pose_block_bytes(r.k, cfg_.real_kind);
+  const uint64_t from = r.contiguous ? pose_block_pos_offset(r.k) : 0;  // contiguous: no id list
+  GateShared g;
+  pb.pools = pools_;
+  pb.dev = pools_->get_block(bytes, &pb.dev_bucket);
+  pb.ev = pools_->get_event();
+  CDX_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(pb.dev) + from, static_cast<const char*>(pb.p) + from,
+                               bytes - from, hipMemcpyHostToDevice, up_));
+  CDX_HIP_CHECK(hipEventRecord(pb.ev, up_));
+}
+
 uint64_t Server::enqueue(std::shared_ptr<Request> req, int cls, double t_enter) {
+  upload_poses(*req);
   uint64_t id;
   {
     std::lock_guard<std::mutex> lk(mu_);
@@ -385,7 +472,9 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
     sel.reserve(batch.size());
     double t0 = now_ms();
     for (auto& r : batch) {
-      if (r->k) e.update_bodies_pinned(r->k, r->poses->p, r->contiguous ? (int64_t)r->body_base : -1);
+      const int64_t first = r->contiguous ? (int64_t)r->body_base : -1;
+      if (r->k && r->poses->dev) e.update_bodies_device(r->k, r->poses->dev, first, r->poses->ev);
+      else if (r->k) e.update_bodies_pinned(r->k, r->poses->p, first);
       sel.push_back(r->local);
     }
     double t1 = now_ms();

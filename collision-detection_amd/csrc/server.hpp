@@ -45,6 +45,7 @@ struct ServerStats {
 };
 
 struct PoseBlock;
+struct DevPools;
 
 // Pinned host block holding the contacts of one batched step.
 // The step's contact arrays lie back to back, each `rows` (the step's
@@ -103,7 +104,7 @@ class Server {
     uint64_t body_base = 0;         // global body id of the scene's body 0
     uint64_t k = 0;
     bool contiguous = false;  // updates cover bodies 0..k-1 (no id list)
-    std::shared_ptr<PoseBlock> poses;  // pinned: global ids, positions, rotations
+    std::shared_ptr<PoseBlock> poses;  // pinned: global ids, positions, rotations (+ device copy)
     double t_submit = 0.0;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
     std::string err;
@@ -134,8 +135,13 @@ class Server {
   int class_of(uint64_t n) const;
   int scenes_cls(uint32_t s) const;
   static double now_ms();
+  // Starts the copy of a request's poses to device memory on up_ (at submit,
+  // so the copy overlaps the queueing and the steps before it).
+  void upload_poses(Request& r);
 
   int dev_;
+  hipStream_t up_ = nullptr;        // pose copies at submit
+  std::shared_ptr<DevPools> pools_;  // device pose blocks and their copy events
   ServerConfig cfg_;
   uint64_t budget_ = 0, committed_ = 0;
   std::vector<std::unique_ptr<Class>> classes_;
