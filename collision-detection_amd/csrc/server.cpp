@@ -3,7 +3,7 @@
 // Scenes live in one batch engine per size class.  Each class has a worker
 // thread that forms a step from its queue as soon as the previous step of
 // that class has finished (continuous batching): oldest requests first, at
-// most one request per scene, up to max_batch_shapes shapes.  Classes step
+// most one request per scene, up to the class's step size.  Classes step
 // on their own HIP streams, so steps of different classes overlap on the
 // device.  A request's pose inputs are copied at submit into a pinned
 // block, and from there to a device block by an asynchronous copy started
@@ -32,6 +32,11 @@ namespace {
 // steps on its own stream; four match the device's hardware queues.
 constexpr uint64_t kClassHi[] = {8192, 49152, 131072, ~0ull};
 constexpr int kNumClasses = 4;
+// A class's steps hold at most this many scenes of its largest size (and
+// never more than max_batch_shapes shapes): step buffers, and so every
+// launch grid of a recorded step, are sized by it, and a step of a few
+// small scenes should not launch workgroups for half a million shapes.
+constexpr uint64_t kScenesPerStep = 8;
 
 // Pinned host blocks for request inputs, bucketed by power-of-two size.
 class PinnedPool {
@@ -172,6 +177,7 @@ Server::Server(int device, const ServerConfig& cfg) : dev_(device), cfg_(cfg), t
     k.lo = lo;
     k.hi = kClassHi[c];
     lo = k.hi;
+    k.max_shapes = k.hi == ~0ull ? cfg.max_batch_shapes : std::min(cfg.max_batch_shapes, kScenesPerStep * k.hi);
     k.eng = make_batch(cfg.real_kind, dev_);
     k.eng->set_np_mode(cfg.np_mode, cfg.tau);
     k.eng->set_envelope(cfg.envelope);
@@ -218,7 +224,7 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
   // footprint: the scene's resident arrays, plus the class's step buffers
   // the first time the class gets a scene
   uint64_t need = k.eng->resident_bytes_estimate(n, nb);
-  if (first_in_class) need += k.eng->step_bytes_estimate(std::max<uint64_t>(cfg_.max_batch_shapes, n));
+  if (first_in_class) need += k.eng->step_bytes_estimate(std::max<uint64_t>(k.max_shapes, n));
   {
     std::lock_guard<std::mutex> lk(mu_);
     if (committed_ + need > budget_)
@@ -238,8 +244,9 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
   const uint32_t local = k.eng->add_scene(n, T, A, B, R, ID, nb, pos, rot, active, g[0], g[1], g[2]);
   // step buffers for the largest step this class can form, sized now
   const uint64_t nsc = k.eng->scene_count();
-  const uint64_t per_step = std::min<uint64_t>(nsc, std::max<uint64_t>(1, cfg_.max_batch_shapes / std::max<uint64_t>(k.lo, 1)));
-  k.eng->reserve_step(std::max<uint64_t>(cfg_.max_batch_shapes, n), per_step);
+  k.min_n = std::min(k.min_n, n);  // a step holds at most max_shapes / min_n scenes
+  const uint64_t per_step = std::min<uint64_t>(nsc, std::max<uint64_t>(1, k.max_shapes / k.min_n));
+  k.eng->reserve_step(std::max<uint64_t>(k.max_shapes, n), per_step);
   std::lock_guard<std::mutex> lk(mu_);
   committed_ += need;
   SceneRef r;
@@ -424,7 +431,7 @@ void Server::worker_loop(int c) {
         const auto& r = *it;
         if (taken.count(r->scene)) { ++it; continue; }  // the scene's next step waits for this one
         const uint64_t n = r->n;
-        if (!batch.empty() && shapes + n > cfg_.max_batch_shapes) break;
+        if (!batch.empty() && shapes + n > k.max_shapes) break;
         shapes += n;
         taken.insert(r->scene);
         r->state = 1;

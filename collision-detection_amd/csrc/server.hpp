@@ -114,6 +114,8 @@ class Server {
   };
   struct Class {
     uint64_t lo, hi;  // shape-count range [lo, hi)
+    uint64_t max_shapes;  // shapes per step of this class
+    uint64_t min_n = ~0ull;  // smallest resident scene (bounds the scenes per step)
     std::unique_ptr<BatchBase> eng;
     std::mutex eng_mu;  // the engine is used by the worker and add_scene
     std::deque<std::shared_ptr<Request>> queue;
