@@ -189,6 +189,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     for (int i = 0; i < 2 * ST_COUNT + 2; ++i) (void)hipEventDestroy(ev_[i]);
     (void)hipHostFree(hstat_);
     (void)hipHostFree(hhdr_);
+    if (ev_stage_) (void)hipEventDestroy(ev_stage_);
     if (stage_host_) (void)hipHostFree(stage_host_);
     if (souts_pinned_) (void)hipHostFree(souts_pinned_);
     (void)hipStreamDestroy(s_);
@@ -468,7 +469,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     if (scenes) sel.assign(scenes, scenes + k);
     else for (uint32_t s = 0; s < scenes_.size(); ++s) sel.push_back(s);
     for (uint32_t s : sel) CDX_REQUIRE(s < scenes_.size(), ERR_INVALID, "scene index out of range");
-    sync_internal();
+    // no stream-wide wait: set_selection waits for the last copies out of
+    // its staging block, and everything else it writes is ordered on the
+    // stream (pose updates queued just before keep running meanwhile)
     set_selection(sel);
     explicit_selection_ = true;
   }
@@ -845,6 +848,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       CDX_HIP_CHECK(hipStreamSynchronize(s_));  // desc_host_ is pageable
       desc_dirty_ = false;
     }
+    if (stage_pending_) {  // the previous selection's copies read the staging block
+      CDX_HIP_CHECK(hipEventSynchronize(ev_stage_));
+      stage_pending_ = false;
+    }
     const size_t sel_bytes = ((nsel_ * 4 + 15) & ~size_t(15));
     const size_t rng_bytes = ((nsel_ * 12 + 15) & ~size_t(15));
     const size_t need = sizeof(StepHdr) + sel_bytes + rng_bytes + nchunks_ * sizeof(Chunk) + 64;
@@ -878,6 +885,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
     if (!ch.empty())
       CDX_HIP_CHECK(hipMemcpyAsync(chunks_.p, sch, ch.size() * sizeof(Chunk), hipMemcpyHostToDevice, s_));
+    if (!ev_stage_) CDX_HIP_CHECK(hipEventCreateWithFlags(&ev_stage_, hipEventDisableTiming));
+    CDX_HIP_CHECK(hipEventRecord(ev_stage_, s_));
+    stage_pending_ = true;
     selection_dirty_ = false;
     if (key_bits_ != graph_key_bits_ || shape_bits_ != graph_shape_bits_) graph_valid_ = false;
   }
@@ -1294,6 +1304,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, meta_, scount_, soff_, selrng_;
   void* stage_host_ = nullptr;
   size_t stage_host_bytes_ = 0;
+  hipEvent_t ev_stage_ = nullptr;  // after the last copies out of stage_host_
+  bool stage_pending_ = false;
   void* souts_pinned_ = nullptr;
   size_t souts_pinned_bytes_ = 0;
   bool desc_dirty_ = true;
