@@ -73,7 +73,7 @@ def stage_bytes(s, c):
 # Kernel behind each single-kernel stage (names as rocprofv3 lists them).
 STAGE_KERNEL = {
     "aabb": "k_aabb<double>", "bp_count": "k_count<double>", "bp_emit": "k_emit_keys<double>",
-    "bp_pairs": "k_pairs<double>", "bp_pairemit": "k_pairs_emit<double>"
+    "bp_pairs": "k_pairs<double>", "bp_pairemit": "k_pair_emit<double>"
 }
 
 
@@ -103,6 +103,37 @@ def read_traffic(stage):
     f_scale = kv["copy.bytes"] / (kv["copy.fetch_kib"] * 1024.0)
     w_scale = kv["copy.bytes"] / (kv["copy.write_kib"] * 1024.0)
     return int(kv[f"{stage}.fetch_kib"] * 1024.0 * f_scale + kv[f"{stage}.write_kib"] * 1024.0 * w_scale)
+
+
+def read_resolve_fp64():
+    """FP64 rate of the MPR re-solve kernel from the committed rocprofv3 summary, or None.
+
+    profiles/pmc_fp64_resolve_mixed1M.txt holds the kernel's FP64 instruction
+    counts per dispatch (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, per wave) and
+    its average duration from the kernel-trace summary of the same tree.
+    Issued FLOP = 64 x (ADD + MUL + TRANS + 2 FMA), inactive lanes included.
+    """
+    path = os.path.join(ROOT, "profiles", "pmc_fp64_resolve_mixed1M.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        line = line.split("#", 1)[0].strip()
+        if "=" in line:
+            k, v = line.split("=", 1)
+            try:
+                kv[k.strip()] = float(v)
+            except ValueError:
+                pass
+    need = ["resolve.add_f64", "resolve.mul_f64", "resolve.fma_f64", "resolve.trans_f64", "resolve.kernel_us"]
+    if not all(k in kv for k in need):
+        return None
+    flop = 64.0 * (kv["resolve.add_f64"] + kv["resolve.mul_f64"] + kv["resolve.trans_f64"] + 2.0 * kv["resolve.fma_f64"])
+    tflops = flop / (kv["resolve.kernel_us"] * 1e-6) / 1e12
+    return {"bound": "valu_fp64", "kernel": "k_np_resolve<double, double>", "achieved": round(tflops, 2),
+            "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / FP64_VECTOR_PEAK_TFLOPS, 4),
+            "flop_issued_per_launch": int(flop), "avg_launch_ms": round(kv["resolve.kernel_us"] / 1000.0, 4),
+            "source": "profiles/pmc_fp64_resolve_mixed1M.txt (rocprofv3 counters and kernel time)"}
 
 
 def _host_info():
@@ -760,6 +791,7 @@ def main():
         "scene_path": scene_path,
         "dropin_path": dropin,
         "roofline": roofline,
+        "resolve_fp64": read_resolve_fp64(),
         "cpu_baseline": cpu,
         "lattice_16M_broadphase": lattice,
         "batched_512": batched,
