@@ -166,16 +166,14 @@ template <class R> struct Contact {
 // Sphere-sphere closed form (Narrowphase.cu:382-396) plus the envelope and
 // depth of :412-415.
 template <class R>
-__device__ __forceinline__ bool contact_spheres(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R envelope,
-                                                Contact<R>& out) {
-  v3<R> AX = mk3(ra.x, ra.y, ra.z), BX = mk3(rb.x, rb.y, rb.z);
+__device__ __forceinline__ bool contact_spheres(v3<R> AX, R rad_a, v3<R> BX, R rad_b, R envelope, Contact<R>& out) {
   v3<R> relpos = BX - AX;
   R d2 = dot(relpos, relpos);
-  R cd = ra.bx + rb.bx;
+  R cd = rad_a + rad_b;
   if (!(d2 <= cd * cd)) return false;
   v3<R> N = normalize(relpos);
-  v3<R> p1 = AX + N * ra.bx;
-  v3<R> p2 = BX - N * rb.bx;
+  v3<R> p1 = AX + N * rad_a;
+  v3<R> p2 = BX - N * rad_b;
   p1 = p1 - N * envelope;
   p2 = p2 + N * envelope;
   out.N = N;
@@ -183,6 +181,11 @@ __device__ __forceinline__ bool contact_spheres(const ShapeRec<R>& ra, const Sha
   out.p2 = p2;
   out.depth = dot(N, p2 - p1);
   return true;
+}
+template <class R>
+__device__ __forceinline__ bool contact_spheres(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R envelope,
+                                                Contact<R>& out) {
+  return contact_spheres(mk3(ra.x, ra.y, ra.z), ra.bx, mk3(rb.x, rb.y, rb.z), rb.bx, envelope, out);
 }
 
 template <class R>
@@ -446,7 +449,20 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
     tb = fb.type;
     if ((fa.active | fb.active) && fa.body != fb.body) {
       if (ta == SPHERE && tb == SPHERE) {
-        spheres();
+        // The FP32 records hold the scene-precision centres exactly (widened
+        // to FP64) and, in an FP32 scene, the radii too: only an FP64 scene
+        // reads its radii from the full records.
+        R rad_a, rad_b;
+        if constexpr (sizeof(R) == 4) {
+          rad_a = fa.bx;
+          rad_b = fb.bx;
+        } else {
+          rad_a = rec[ia].bx;
+          rad_b = rec[ib].bx;
+        }
+        Contact<R> c;
+        if (contact_spheres(mk3((R)fa.x, (R)fa.y, (R)fa.z), rad_a, mk3((R)fb.x, (R)fb.y, (R)fb.z), rad_b, envelope, c))
+          store_pair(o, P, i, c, fa.body, fb.body);
       } else {
         const PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
         const float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
