@@ -87,13 +87,25 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const uint32_t* __restr
 // Exclusive scan of the tile sums by one block; writes the grand total.
 // Each of the 1024 threads owns a contiguous slice of the sums: it adds its
 // slice, one block scan gives the slice offsets, and it rewrites its slice.
+// Slices are read eight sums at a time, so a long slice costs a few memory
+// round trips rather than one per sum.
+constexpr int PART_UNROLL = 8;
+
 __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t* __restrict__ part, uint64_t nb,
                                                         uint64_t* __restrict__ d_total) {
   __shared__ uint64_t ws[1024 / WAVE + 1];
   const uint64_t per = (nb + 1023) / 1024;
   const uint64_t b0 = min((uint64_t)threadIdx.x * per, nb), b1 = min(b0 + per, nb);
   uint64_t sum = 0;
-  for (uint64_t i = b0; i < b1; ++i) sum += part[i];
+  uint64_t i = b0;
+  for (; i + PART_UNROLL <= b1; i += PART_UNROLL) {
+    uint64_t x[PART_UNROLL];
+#pragma unroll
+    for (int k = 0; k < PART_UNROLL; ++k) x[k] = part[i + k];
+#pragma unroll
+    for (int k = 0; k < PART_UNROLL; ++k) sum += x[k];
+  }
+  for (; i < b1; ++i) sum += part[i];
   // block exclusive scan of the slice sums (64-bit)
   uint64_t inc = sum;
   for (int o = 1; o < WAVE; o <<= 1) {
@@ -109,7 +121,18 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t* __restrict__ p
   }
   __syncthreads();
   uint64_t run = ws[threadIdx.x / WAVE] + inc - sum;
-  for (uint64_t i = b0; i < b1; ++i) {
+  i = b0;
+  for (; i + PART_UNROLL <= b1; i += PART_UNROLL) {
+    uint64_t x[PART_UNROLL];
+#pragma unroll
+    for (int k = 0; k < PART_UNROLL; ++k) x[k] = part[i + k];
+#pragma unroll
+    for (int k = 0; k < PART_UNROLL; ++k) {
+      part[i + k] = run;
+      run += x[k];
+    }
+  }
+  for (; i < b1; ++i) {
     const uint64_t c = part[i];
     part[i] = run;
     run += c;
