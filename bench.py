@@ -646,8 +646,8 @@ def main():
         copy_calibration(args.copy_bytes)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: every box has a single MI355X; "
-                                   "multi-GPU sharding is described in DESIGN.md, not built",
+        print(json.dumps({"error": "this pipeline runs on one GPU (one MI355X per process); "
+                                   "multi-GPU sharding of scenes is not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
