@@ -414,14 +414,15 @@ template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(
 
 // Front pass over all candidate pairs (function_MPR_Store's early exits and
 // sphere-sphere closed form, Narrowphase.cu:338-396).  Pairs that need the
-// MPR are appended to a queue as (shape-pair class << 32 | pair index), one
-// atomic per wavefront; a one-digit radix pass then groups the queue by
-// class so every wavefront of the MPR kernel runs one pair of support maps.
-// MIXED / FAST first run an FP32 MPR in the frame of shape A: clear misses
-// end here, FAST keeps clear FP32 hits, everything else is queued for the
-// reference-order MPR.  In those modes the front pass already applies the
-// FP32 MPR's first portal test, so pairs separated along the centre line
-// (most candidate pairs) never enter the queue.
+// MPR are flagged (qsel) with their shape-pair class (qcls); a scan of the
+// flags and k_queue_compact turn them into a queue of (class << 32 | pair
+// index) in pair order, and a one-digit radix pass then groups the queue by
+// class so every wavefront of the MPR kernels runs one pair of support maps.
+// MIXED / FAST then run an FP32 MPR in the frame of shape A (k_np_filter):
+// clear misses end there, FAST keeps clear FP32 hits, everything else goes
+// to the reference-order MPR.  In those modes the front pass already applies
+// the FP32 MPR's first portal test, so pairs separated along the centre
+// line (most candidate pairs) never enter the queue.
 template <class R>
 __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* __restrict__ d_np,
                                                   const uint64_t* __restrict__ pairs,
