@@ -23,7 +23,11 @@
 //   narrowphase — sphere-sphere closed form; MPR sphere pairs vs closed form;
 //                 box face contacts vs separating-axis depth; equal-axis
 //                 ellipsoid == sphere; capsule / cylinder axial and radial
-//                 closed forms; |N|=1, swap symmetry.
+//                 closed forms; |N|=1; the barycentric contact point vs a
+//                 numpy solve of the barycentric system on the final portal
+//                 and inside both shapes (planted slips ORACLE_PLANT 1-3
+//                 must fail it); p1 / p2 on the shapes' supporting planes;
+//                 swap symmetry (a,b) -> (b,a).
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
@@ -348,11 +352,30 @@ template <class R> inline r3<R> TransformSupportVert(int type, r3<R> A, r3<R> B,
   return quatRotateMat(ls, Rq) + A;
 }
 
+// Final portal of a hit that reaches the barycentric contact point
+// (Narrowphase.cu:265-289): the tetrahedron v0..v3 of Minkowski-difference
+// points, the support points of A (v*1) and B (v*2) that make each vertex,
+// the weights as computed, and whether the sum <= 0 fallback was taken.
+// Filled only when a caller asks, for the pin tests; it changes nothing.
+template <class R> struct MprPortal {
+  r3<R> v[4], a[4], b[4], n;
+  R w[4];
+  int fallback = 0, filled = 0;
+};
+
+// Planted slips (ORACLE_PLANT, compiled only into the test variants
+// liboracle_plant<k>.so): 1 drops the halving of the contact point,
+// 2 swaps the weights of v2 and v3, 3 pairs v1's weight with v2's A point.
+// The pin tests build each variant and check that it fails them.
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 // Narrowphase.cu:122-314 (XenoCollide-derived MPR).  GetCenter (:11-17) is
 // the shape position for every in-scope type.
 template <class R>
 bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r4<R> AR, int tB, r3<R> BX, r3<R> BY, r4<R> BR, r3<R>& rnormal,
-                         r3<R>& point, R& depth) {
+                         r3<R>& point, R& depth, MprPortal<R>* portal = nullptr) {
   r3<R> v01, v02, v0, n, v11, v12, v1, v21, v22, v2;
   v01 = R3<R>(0, 0, 0) + AX;
   v02 = R3<R>(0, 0, 0) + BX;
@@ -421,7 +444,9 @@ bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r4<R> AR, int tB, r3<R> BX,
           R b2 = dot(cross(v0, v1), v3);
           R b3 = dot(cross(v2, v1), v0);
           R sum = b0 + b1 + b2 + b3;
+          int fallback = 0;
           if (sum <= 0.) {
+            fallback = 1;
             b0 = 0;
             b1 = dot(cross(v2, v3), n);
             b2 = dot(cross(v3, v1), n);
@@ -429,9 +454,30 @@ bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r4<R> AR, int tB, r3<R> BX,
             sum = b1 + b2 + b3;
           }
           R inv = R(1.0f) / sum;
+#if ORACLE_PLANT == 2
+          std::swap(b2, b3);
+#endif
+#if ORACLE_PLANT == 3
+          point = (b0 * v01 + b1 * v21 + b2 * v21 + b3 * v31) + (b0 * v02 + b1 * v12 + b2 * v22 + b3 * v32);
+#else
           point = (b0 * v01 + b1 * v11 + b2 * v21 + b3 * v31) + (b0 * v02 + b1 * v12 + b2 * v22 + b3 * v32);
+#endif
+#if ORACLE_PLANT == 1
+          point = point * inv;
+#else
           point = point * inv * R(.5);
+#endif
           rnormal = normalize(n);
+          (void)fallback;
+          if (portal) {
+            portal->v[0] = v0; portal->v[1] = v1; portal->v[2] = v2; portal->v[3] = v3;
+            portal->a[0] = v01; portal->a[1] = v11; portal->a[2] = v21; portal->a[3] = v31;
+            portal->b[0] = v02; portal->b[1] = v12; portal->b[2] = v22; portal->b[3] = v32;
+            portal->w[0] = b0; portal->w[1] = b1; portal->w[2] = b2; portal->w[3] = b3;
+            portal->n = n;
+            portal->fallback = fallback;
+            portal->filled = 1;
+          }
         }
         return hit;
       }
@@ -681,6 +727,33 @@ int oracle_mpr_f64(int tA, const double* AX, const double* AY, const double* AR,
   st3(normal, 0, n);
   st3(point, 0, p);
   *depth = d;
+  return hit ? 1 : 0;
+}
+
+// The same query, also returning the final portal when the hit reaches the
+// barycentric contact point (Narrowphase.cu:265-289): portal = 12 v, 12 A
+// points, 12 B points (4 vertices x 3), 4 weights, the portal normal (3),
+// then the fallback flag and whether the portal was filled (as doubles).
+int oracle_mpr_portal_f64(int tA, const double* AX, const double* AY, const double* AR, int tB, const double* BX,
+                          const double* BY, const double* BR, double* normal, double* point, double* depth,
+                          double* portal) {
+  r3<double> n{0, 0, 0}, p{0, 0, 0};
+  double d = 0;
+  MprPortal<double> pt;
+  bool hit = CollideAndFindPoint<double>(tA, ld3(AX, 0), ld3(AY, 0), ld4(AR, 0), tB, ld3(BX, 0), ld3(BY, 0), ld4(BR, 0),
+                                         n, p, d, &pt);
+  st3(normal, 0, n);
+  st3(point, 0, p);
+  *depth = d;
+  for (int i = 0; i < 4; ++i) {
+    st3(portal, i, pt.v[i]);
+    st3(portal + 12, i, pt.a[i]);
+    st3(portal + 24, i, pt.b[i]);
+    portal[36 + i] = pt.w[i];
+  }
+  st3(portal + 40, 0, pt.n);
+  portal[43] = pt.fallback;
+  portal[44] = pt.filled;
   return hit ? 1 : 0;
 }
 

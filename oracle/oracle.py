@@ -62,7 +62,21 @@ def lib():
     global _lib
     if _lib is None:
         build()
-        L = C.CDLL(LIB_PATH)
+        _lib = _configure(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def planted(k: int):
+    """The oracle with planted slip k in the MPR contact point (ORACLE_PLANT
+    in oracle.cpp), for tests that check the pins catch it."""
+    path = os.path.join(HERE, "_build", f"liboracle_plant{k}.so")
+    if not os.path.exists(path):
+        subprocess.check_call(["make", "-s", "-C", HERE, "planted"])
+    return _configure(C.CDLL(path))
+
+
+def _configure(L):
+    if True:
         vp = C.c_void_p
         for suf in ("f64", "f32"):
             getattr(L, f"oracle_aabb_{suf}").argtypes = [C.c_uint64] + [vp] * 9
@@ -81,8 +95,9 @@ def lib():
         L.oracle_quat_mult_f64.argtypes = [vp, vp, vp]
         L.oracle_mpr_steps_f64.argtypes = [C.c_uint64] + [vp] * 10
         L.oracle_max_threads.restype = C.c_int
-        _lib = L
-    return _lib
+        L.oracle_mpr_portal_f64.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_mpr_portal_f64.restype = C.c_int
+    return L
 
 
 def _p(a):
@@ -223,6 +238,39 @@ def mpr(tA, AX, AY, AR, tB, BX, BY, BR):
     hit = lib().oracle_mpr_f64(int(tA), _p(f(AX)), _p(f(AY)), _p(f(AR)), int(tB), _p(f(BX)), _p(f(BY)), _p(f(BR)),
                                _p(n), _p(p), _p(d))
     return bool(hit), n, p, float(d[0])
+
+
+@dataclass
+class Portal:
+    """Final MPR portal of a hit that reached the barycentric contact point
+    (Narrowphase.cu:265-289): vertices v (4x3, Minkowski-difference points,
+    v[0] the centre difference), the support points of A and B making each
+    vertex (a, b: 4x3), the weights w as computed, the portal normal n, and
+    whether the sum <= 0 fallback was taken.  filled is False when the MPR
+    ended elsewhere (miss, degenerate early hit)."""
+    v: np.ndarray
+    a: np.ndarray
+    b: np.ndarray
+    w: np.ndarray
+    n: np.ndarray
+    fallback: bool
+    filled: bool
+
+
+def mpr_portal(tA, AX, AY, AR, tB, BX, BY, BR, L=None):
+    """mpr() plus the final portal; L selects another build of the oracle
+    (planted())."""
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+    n = np.zeros(3)
+    p = np.zeros(3)
+    d = np.zeros(1)
+    pt = np.zeros(45)
+    L = L or lib()
+    hit = L.oracle_mpr_portal_f64(int(tA), _p(f(AX)), _p(f(AY)), _p(f(AR)), int(tB), _p(f(BX)), _p(f(BY)), _p(f(BR)),
+                                  _p(n), _p(p), _p(d), _p(pt))
+    portal = Portal(v=pt[0:12].reshape(4, 3), a=pt[12:24].reshape(4, 3), b=pt[24:36].reshape(4, 3), w=pt[36:40].copy(),
+                    n=pt[40:43].copy(), fallback=bool(pt[43]), filled=bool(pt[44]))
+    return bool(hit), n, p, float(d[0]), portal
 
 
 def support(t, X, Y, Rq, direction):

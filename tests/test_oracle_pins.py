@@ -352,3 +352,216 @@ def test_mpr_steps_counts(oracle, scenes):
     assert (steps[~mpr] == 0).all()
     assert (steps[mpr] >= 1).all() and steps.max() <= 73
     assert steps[mpr & hit].min() >= 2 and np.median(steps[mpr & hit]) >= 4
+
+
+# ------------------------------------------- MPR contact point (barycentric) --
+
+def _dims(rng, t):
+    if t == SPHERE:
+        return np.array([rng.uniform(0.2, 0.5), 0.0, 0.0])
+    if t in (BOX, ELLIPSOID):
+        return rng.uniform(0.2, 0.6, 3)
+    r = rng.uniform(0.15, 0.5)
+    return np.array([r, rng.uniform(0.2, 0.6), r])  # cylinder / capsule: (r, half length, r)
+
+
+def _barycentric(portal):
+    """Barycentric coordinates of the origin, by a linear solve: in the
+    tetrahedron v0..v3, or (fallback) of its projection along the portal
+    normal onto the triangle v1 v2 v3."""
+    if not portal.fallback:
+        M = np.vstack([portal.v.T, np.ones(4)])
+        return np.linalg.solve(M, [0.0, 0.0, 0.0, 1.0])
+    M = np.zeros((4, 4))
+    M[:3, :3] = portal.v[1:].T
+    M[:3, 3] = -portal.n
+    M[3, :3] = 1.0
+    return np.concatenate([[0.0], np.linalg.solve(M, [0.0, 0.0, 0.0, 1.0])[:3]])
+
+
+def _outside_by(t, B, q, c, p, extra_dirs=()):
+    """How far p lies outside the shape: max over directions u of
+    u.p - h(u), with h the support function (numpy closed forms)."""
+    rng = np.random.default_rng(99)
+    u = rng.standard_normal((400, 3))
+    u = np.vstack([u, np.eye(3), -np.eye(3)] + [np.atleast_2d(d) for d in extra_dirs])
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    Rm = quat_to_mat(q)
+    h = np.array([_support_brute(t, B, Rm, c, d) @ d for d in u])
+    return float(np.max(u @ p - h))
+
+
+_CURVED_TYPES = [SPHERE, ELLIPSOID, BOX, CYLINDER, CAPSULE]
+
+
+def _random_barycentric_hits(oracle_lib, oracle, count, seed):
+    """Penetrating pairs of every shape-type pair (sphere-sphere excluded:
+    the narrowphase solves it in closed form) whose MPR ends in the
+    barycentric contact point (Narrowphase.cu:265-289)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        tA, tB = rng.choice(_CURVED_TYPES, 2)
+        if tA == SPHERE and tB == SPHERE:
+            continue
+        BA, BB = _dims(rng, tA), _dims(rng, tB)
+        qA, qB = rand_quat(rng), rand_quat(rng)
+        XA = rng.uniform(-1, 1, 3)
+        d = rng.standard_normal(3)
+        XB = XA + d * rng.uniform(0.2, 0.9) / np.linalg.norm(d)
+        hit, n, p, depth, portal = oracle.mpr_portal(tA, XA, BA, qA, tB, XB, BB, qB, L=oracle_lib)
+        if hit and portal.filled:
+            out.append(((tA, BA, XA, qA), (tB, BB, XB, qB), n, p, depth, portal))
+    return out
+
+
+def _check_barycentric_point(case, tol=1e-6):
+    """The contact point is the mean of the two halves sum(l_i a_i) and
+    sum(l_i b_i), l the barycentric coordinates of the origin in the final
+    portal; with the origin inside (l >= 0) the halves coincide, so the
+    point lies in both shapes."""
+    (tA, BA, XA, qA), (tB, BB, XB, qB), n, p, depth, portal = case
+    lam = _barycentric(portal)
+    assert lam.min() > -1e-9, f"origin outside the final portal: {lam}"
+    want = (lam[:, None] * (portal.a + portal.b)).sum(axis=0) / 2
+    np.testing.assert_allclose(p, want, rtol=0, atol=tol)
+    # the halves of a correct point coincide (sum l v = 0), so it lies in A and in B
+    assert _outside_by(tA, BA, qA, XA, p, [n, -n]) < tol
+    assert _outside_by(tB, BB, qB, XB, p, [n, -n]) < tol
+
+
+def test_mpr_contact_point_barycentric(oracle):
+    """Pins Narrowphase.cu:265-289 on 400 random penetrating pairs of every
+    shape-type pair that reach the barycentric branch: the oracle's point
+    equals the point built from a numpy solve of the barycentric system on
+    its own final portal, and lies inside both shapes."""
+    cases = _random_barycentric_hits(None, oracle, 400, seed=21)
+    kinds = {(c[0][0], c[1][0]) for c in cases}
+    assert len(kinds) >= 20  # every ordered type pair but sphere-sphere
+    for case in cases:
+        _check_barycentric_point(case)
+
+
+@pytest.mark.parametrize("name,tA,BA,qA,tB,BB,XB,qB", [
+    ("box-box off-axis", BOX, [0.5, 0.4, 0.3], [0.99, 0.0, 0.0, 0.141], BOX, [0.2, 0.25, 0.15], [0.62, 0.13, -0.07],
+     [0.955, 0.296, 0.0, 0.0]),
+    ("sphere-box off-centre face", BOX, [0.5, 0.5, 0.5], [1, 0, 0, 0], SPHERE, [0.3, 0, 0], [0.7, 0.2, 0.1],
+     [1, 0, 0, 0]),
+    ("capsule-box", CAPSULE, [0.2, 0.4, 0.2], [0.92, 0.39, 0.0, 0.0], BOX, [0.3, 0.3, 0.3], [0.1, 0.45, 0.38],
+     [0.98, 0.0, 0.2, 0.0]),
+    ("ellipsoid (unequal axes)-sphere", ELLIPSOID, [0.5, 0.3, 0.2], [0.9, 0.3, 0.3, 0.1], SPHERE, [0.25, 0, 0],
+     [0.35, 0.3, 0.2], [1, 0, 0, 0]),
+])
+def test_mpr_contact_point_named_cases(oracle, name, tA, BA, qA, tB, BB, XB, qB):
+    qA = np.asarray(qA, float) / np.linalg.norm(qA)
+    qB = np.asarray(qB, float) / np.linalg.norm(qB)
+    XA = np.zeros(3)
+    hit, n, p, depth, portal = oracle.mpr_portal(tA, XA, np.asarray(BA, float), qA, tB, np.asarray(XB, float),
+                                                 np.asarray(BB, float), qB)
+    assert hit and portal.filled, name
+    _check_barycentric_point(((tA, np.asarray(BA, float), XA, qA), (tB, np.asarray(BB, float), np.asarray(XB, float),
+                                                                      qB), n, p, depth, portal))
+    if name.startswith("sphere-box"):
+        # sphere centre c over the face x = h of a box at the origin, portal on
+        # the face: the halves meet at x = h D / (h + r) on the sphere's axis
+        h, r, D = 0.5, 0.3, XB[0]
+        np.testing.assert_allclose(p, [h * D / (h + r), XB[1], XB[2]], atol=1e-12)
+
+
+_NAMED_FLAT = [  # face contacts: the final portal's vertices stay apart, so every term counts
+    (BOX, [0.5, 0.4, 0.3], [0.99, 0.0, 0.0, 0.141], BOX, [0.2, 0.25, 0.15], [0.62, 0.13, -0.07], [0.955, 0.296, 0, 0]),
+    (BOX, [0.5, 0.5, 0.5], [1, 0, 0, 0], SPHERE, [0.3, 0, 0], [0.7, 0.2, 0.1], [1, 0, 0, 0]),
+]
+
+
+@pytest.mark.parametrize("slip,min_frac", [(1, 0.9), (2, 0.3), (3, 0.3)])
+def test_mpr_pins_catch_planted_slips(oracle, slip, min_frac):
+    """Oracle builds with a planted slip in the contact point (1: the halving
+    dropped, 2: the v2/v3 weights swapped, 3: v1's weight on v2's A point)
+    fail the barycentric pin on both face-contact cases and on most random
+    cases.  Slips 2 and 3 change nothing where the refinement has shrunk
+    the portal onto one point of a curved surface (the swapped support
+    points coincide there), hence the lower fraction."""
+    L = oracle.planted(slip)
+    for tA, BA, qA, tB, BB, XB, qB in _NAMED_FLAT:
+        qA = np.asarray(qA, float) / np.linalg.norm(qA)
+        qB = np.asarray(qB, float) / np.linalg.norm(qB)
+        BA, BB, XB, XA = np.asarray(BA, float), np.asarray(BB, float), np.asarray(XB, float), np.zeros(3)
+        hit, n, p, depth, portal = oracle.mpr_portal(tA, XA, BA, qA, tB, XB, BB, qB, L=L)
+        with pytest.raises(AssertionError):
+            _check_barycentric_point(((tA, BA, XA, qA), (tB, BB, XB, qB), n, p, depth, portal))
+    cases = _random_barycentric_hits(L, oracle, 60, seed=22)
+    failed = 0
+    for case in cases:
+        try:
+            _check_barycentric_point(case)
+        except AssertionError:
+            failed += 1
+    assert failed >= min_frac * len(cases), f"slip {slip} failed only {failed} of {len(cases)} cases"
+
+
+def _np_two(scenes, shapes, envelope=0.0):
+    """Narrowphase of one pair of world-placed shapes (one body each)."""
+    T = np.array([s[0] for s in shapes], np.int32)
+    B = np.array([np.asarray(s[1], float) for s in shapes])
+    pos = np.array([np.asarray(s[2], float) for s in shapes])
+    rot = np.array([np.asarray(s[3], float) for s in shapes])
+    z3 = np.zeros((2, 3))
+    R = np.tile([1.0, 0.0, 0.0, 0.0], (2, 1))
+    sc = scenes.Scene(T, z3.copy(), B, z3.copy(), R, np.arange(2, dtype=np.uint32), pos, rot, np.ones(2, np.uint8),
+                      (4, 4, 4))
+    return sc
+
+
+def test_contact_points_on_support_planes(oracle, scenes):
+    """function_MPR_Store (Narrowphase.cu:404-415): p1 and p2 lie on the line
+    through the MPR point along N, p1 on A's supporting plane of normal N
+    and p2 on B's of normal -N (numpy support functions), so depth is the
+    negated width of the Minkowski difference along N."""
+    cases = _random_barycentric_hits(None, oracle, 150, seed=23)
+    pair = np.array([(0 << 32) | 1], np.int64)
+    for (tA, BA, XA, qA), (tB, BB, XB, qB), n_mpr, p0, _, _ in cases:
+        sc = _np_two(scenes, [(tA, BA, XA, qA), (tB, BB, XB, qB)])
+        r = oracle.narrowphase(sc, pair)
+        assert r.count == 1
+        N, p1, p2 = r.norm[0], r.cpta[0], r.cptb[0]
+        np.testing.assert_allclose(N, -n_mpr, atol=1e-15)  # N points from A to B
+        hA = _support_brute(tA, BA, quat_to_mat(qA), XA, N) @ N
+        hB = _support_brute(tB, BB, quat_to_mat(qB), XB, -N) @ N
+        np.testing.assert_allclose(p1 @ N, hA, atol=1e-12)
+        np.testing.assert_allclose(p2 @ N, hB, atol=1e-12)
+        np.testing.assert_allclose(np.cross(p1 - p0, N), 0, atol=1e-12)
+        np.testing.assert_allclose(np.cross(p2 - p0, N), 0, atol=1e-12)
+        assert r.dpth[0] <= 0
+
+
+def test_swap_symmetry(oracle, scenes):
+    """(a, b) -> (b, a) flips N, swaps p1 and p2 and keeps the depth.
+    Exact to rounding where the contact is between flat faces (box-box,
+    where the MPR stops on the face); on curved surfaces the reference's
+    34-iteration cap (Narrowphase.cu:263) leaves the portal, and so the
+    normal, dependent on the order of exploration, which the swapped pair
+    changes: measured up to ~6e-3 in the normal, median ~1e-6."""
+    pair = np.array([(0 << 32) | 1], np.int64)
+    rng = np.random.default_rng(24)
+    flat, curved = [], []
+    for _ in range(300):
+        tA, tB = rng.choice(_CURVED_TYPES, 2)
+        if tA == SPHERE and tB == SPHERE:
+            continue
+        a = (tA, _dims(rng, tA), np.zeros(3), rand_quat(rng))
+        d = rng.standard_normal(3)
+        b = (tB, _dims(rng, tB), d * rng.uniform(0.2, 0.9) / np.linalg.norm(d), rand_quat(rng))
+        r1 = oracle.narrowphase(_np_two(scenes, [a, b]), pair)
+        r2 = oracle.narrowphase(_np_two(scenes, [b, a]), pair)
+        if r1.count != 1 or r2.count != 1:
+            continue
+        err = (np.abs(r1.norm[0] + r2.norm[0]).max(), abs(r1.dpth[0] - r2.dpth[0]),
+               np.abs(r1.cpta[0] - r2.cptb[0]).max(), np.abs(r1.cptb[0] - r2.cpta[0]).max())
+        (flat if tA == BOX and tB == BOX else curved).append(err)
+    flat, curved = np.array(flat), np.array(curved)
+    assert len(flat) >= 5 and len(curved) >= 100
+    assert flat.max() < 1e-12
+    assert np.median(curved[:, 0]) < 1e-5 and curved[:, 0].max() < 2e-2
+    assert np.median(curved[:, 1]) < 1e-6 and curved[:, 1].max() < 2e-3
+    assert np.median(curved[:, 2:]) < 1e-5 and curved[:, 2:].max() < 2e-2
