@@ -33,7 +33,7 @@ STAGES = ["aabb", "bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_
 EXPORTS = [
     "cdx_version", "cdx_status_string", "cdx_device_count", "cdx_ctx_create", "cdx_ctx_destroy",
     "cdx_ctx_last_error", "cdx_generate_aabb", "cdx_detect_possible_collisions", "cdx_read_possible_collisions",
-    "cdx_do_narrowphase", "cdx_read_contacts", "cdx_scene_create", "cdx_scene_destroy", "cdx_scene_last_error",
+    "cdx_do_narrowphase", "cdx_read_contacts", "cdx_mpr_store", "cdx_scene_create", "cdx_scene_destroy", "cdx_scene_last_error",
     "cdx_scene_set_shapes", "cdx_scene_set_bodies", "cdx_scene_update_bodies", "cdx_scene_set_aabbs",
     "cdx_scene_set_pairs", "cdx_scene_set_bins", "cdx_scene_set_envelope", "cdx_scene_set_narrowphase_mode",
     "cdx_scene_set_profiling", "cdx_scene_aabb", "cdx_scene_broadphase", "cdx_scene_narrowphase",
@@ -117,6 +117,8 @@ def lib():
         "cdx_do_narrowphase": ([vp, i32, i32, C.c_double, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp],
                                i32),
         "cdx_read_contacts": ([vp, vp, vp, vp, vp, vp, u64], i32),
+        "cdx_mpr_store": ([vp, i32, i32, C.c_double, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp, vp,
+                           vp, vp, vp, vp], i32),
         "cdx_scene_create": ([vp, i32, vp], i32),
         "cdx_scene_destroy": ([vp], i32),
         "cdx_scene_last_error": ([vp], C.c_char_p),
@@ -252,6 +254,22 @@ class Context:
         ids = np.zeros((k, 2), np.int32)
         self._chk(lib().cdx_read_contacts(self.h, _p(norm), _p(pa), _p(pb), _p(d), _p(ids), k))
         return norm, pa, pb, d, ids
+
+    def mpr_store(self, s, pairs: np.ndarray, envelope=0.0, mode=NP_NATIVE, fill=0.0, active_fill=1):
+        """Narrowphase::host_MPR_Store: per-pair arrays at the pair index,
+        written only for contacts (others keep `fill` / `active_fill`).
+        Returns (contact_active, norm, cpta, cptb, dpth, bids)."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.int64)
+        P = len(pairs)
+        dt = s.dtype
+        act = np.full(P, active_fill, np.uint32)
+        norm, pa, pb = (np.full((P, 3), fill, dt) for _ in range(3))
+        d = np.full(P, fill, dt)
+        ids = np.full((P, 2), -1, np.int32)
+        self._chk(lib().cdx_mpr_store(self.h, _kind(dt), mode, float(envelope), s.n, _p(s.T), _p(s.A), _p(s.B),
+                                      _p(s.C), _p(s.R), _p(s.ID), s.nb, _p(s.active), _p(s.pos), _p(s.rot), _p(pairs),
+                                      P, _p(act), _p(norm), _p(pa), _p(pb), _p(d), _p(ids)))
+        return act, norm, pa, pb, d, ids
 
 
 class Scene:

@@ -770,6 +770,43 @@ __global__ __launch_bounds__(RUNS_BS) void k_sum_runs(uint64_t mcap, const uint3
   }
 }
 
+// reduce_by_key materialised for a caller that reads the bins
+// (Broadphase.cu:428-443, :476-480): the sorted (bin, body) entries split
+// into their two arrays, one head flag per run of equal keys; after an
+// exclusive scan of the heads, each head writes its key and start to its
+// run's slot, and each run's occupancy is the distance to the next start.
+// The step itself never runs these (the pair search finds runs on its own).
+__global__ __launch_bounds__(256) void k_bins_split(uint64_t m, const uint64_t* __restrict__ entries,
+                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ bodies,
+                                                    uint32_t* __restrict__ heads) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t e = entries[i];
+  const uint32_t k = (uint32_t)(e >> 32);
+  keys[i] = k;
+  bodies[i] = (uint32_t)e;
+  heads[i] = (i == 0 || (uint32_t)(entries[i - 1] >> 32) != k) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_bins_runs(uint64_t m, const uint32_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ heads,
+                                                   const uint32_t* __restrict__ run_of,
+                                                   uint32_t* __restrict__ active_keys,
+                                                   uint32_t* __restrict__ run_start) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !heads[i]) return;
+  active_keys[run_of[i]] = keys[i];
+  run_start[run_of[i]] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_bins_occupancy(uint64_t a, uint64_t m, const uint32_t* __restrict__ run_start,
+                                                        uint32_t* __restrict__ counts) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a) return;
+  const uint64_t end = r + 1 < a ? run_start[r + 1] : m;
+  counts[r] = (uint32_t)(end - run_start[r]);
+}
+
 // Pairs per shape: the sum of its entries' counts.  A chunk's entries are
 // contiguous, so the workgroup first reads their counts in one coalesced
 // sweep into LDS (one memory latency), then each shape sums its own.

@@ -207,6 +207,50 @@ int cdx_do_narrowphase(cdx_ctx* ctx, int real_kind, int np_mode, double envelope
   });
 }
 
+int cdx_mpr_store(cdx_ctx* ctx, int real_kind, int np_mode, double envelope, uint64_t n, const int32_t* T,
+                  const void* A, const void* B, const void* C, const void* R, const uint32_t* ID, uint64_t nb,
+                  const uint8_t* obj_active, const void* body_pos, const void* body_rot, const int64_t* pairs,
+                  uint64_t n_pairs, uint32_t* contact_active, void* norm, void* ptA, void* ptB, void* depth,
+                  int32_t* ids) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    check_kind(real_kind);
+    (void)C;
+    if (n_pairs == 0) return;
+    CDX_REQUIRE(pairs, CDX_ERR_INVALID, "null pairs");
+    cdx::SceneBase& s = lazy(ctx->np_scene[real_kind], real_kind, ctx->device);
+    ctx->last_np_kind = real_kind;
+    s.set_np_mode(np_mode, 1e-2);
+    s.set_envelope(envelope);
+    s.set_shapes(n, T, A, B, R, ID);
+    s.set_bodies(nb, body_pos, body_rot, obj_active);
+    s.set_pairs(n_pairs, pairs);
+    s.run_narrowphase();
+    s.sync();
+    // contacts come compacted in pair order; put each back at its pair
+    std::vector<uint32_t> hit(n_pairs);
+    s.read_pair_hits(hit.data());
+    const uint64_t c = s.counts().c;
+    const size_t rs = real_kind == CDX_REAL_F64 ? 8 : 4;
+    std::vector<char> nrm(c * 3 * rs), pa(c * 3 * rs), pb(c * 3 * rs), dp(c * rs);
+    std::vector<int32_t> bi(2 * c);
+    s.read_contacts(nrm.data(), pa.data(), pb.data(), dp.data(), bi.data());
+    uint64_t j = 0;
+    for (uint64_t i = 0; i < n_pairs; ++i) {
+      if (!hit[i]) continue;
+      CDX_REQUIRE(j < c, CDX_ERR_HIP, "internal error: more hit flags than contacts");
+      if (norm) std::memcpy(static_cast<char*>(norm) + i * 3 * rs, nrm.data() + j * 3 * rs, 3 * rs);
+      if (ptA) std::memcpy(static_cast<char*>(ptA) + i * 3 * rs, pa.data() + j * 3 * rs, 3 * rs);
+      if (ptB) std::memcpy(static_cast<char*>(ptB) + i * 3 * rs, pb.data() + j * 3 * rs, 3 * rs);
+      if (depth) std::memcpy(static_cast<char*>(depth) + i * rs, dp.data() + j * rs, rs);
+      if (ids) { ids[2 * i] = bi[2 * j]; ids[2 * i + 1] = bi[2 * j + 1]; }
+      if (contact_active) contact_active[i] = 0;
+      ++j;
+    }
+    CDX_REQUIRE(j == c, CDX_ERR_HIP, "internal error: hit flags and contact count disagree");
+  });
+}
+
 int cdx_read_contacts(cdx_ctx* ctx, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids,
                       uint64_t capacity) {
   if (!ctx) return CDX_ERR_INVALID;

@@ -74,6 +74,8 @@ class SceneBase {
                          uint32_t* active_counts) = 0;
   virtual void read_pairs(int64_t* out) = 0;
   virtual void read_pairs_permuted(int64_t* out) = 0;
+  // Per input pair (narrowphase order before compaction): 1 for a contact.
+  virtual void read_pair_hits(uint32_t* out) = 0;
   virtual void read_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) = 0;
   virtual void stage_ms(double* out) = 0;
   virtual hipStream_t stream() const = 0;

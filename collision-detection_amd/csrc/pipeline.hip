@@ -354,20 +354,37 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     // scene 0 starts at shape 0, so its slots are its shape indices
     if (bin_counts && n) CDX_HIP_CHECK(hipMemcpy(bin_counts, counts_buf_.p, n * 4, hipMemcpyDeviceToHost));
     if ((sorted_keys || sorted_bodies || active_keys || active_counts) && M_) {
-      std::vector<uint64_t> k(M_);
-      CDX_HIP_CHECK(hipMemcpy(k.data(), keys_.p, M_ * 8, hipMemcpyDeviceToHost));
-      uint64_t r = 0;
-      for (uint64_t i = 0; i < M_; ++i) {
-        uint32_t key = (uint32_t)(k[i] >> 32);
-        if (sorted_keys) sorted_keys[i] = key;
-        if (sorted_bodies) sorted_bodies[i] = (uint32_t)k[i];
-        if (i == 0 || key != (uint32_t)(k[i - 1] >> 32)) {
-          if (active_keys) active_keys[r] = key;
-          if (active_counts) active_counts[r] = 0;
-          ++r;
-        }
-        if (active_counts) active_counts[r - 1] += 1;
-      }
+      // reduce_by_key on the device (Broadphase.cu:439-443), only for callers
+      // that read the bins; the host copies the results
+      const uint64_t m = M_;
+      const size_t tmp_words = std::max<size_t>(scan_tmp_words(m), 16);
+      bins_.reserve(m * 4 * 6 + 64 + tmp_words * 4);
+      uint32_t* k = bins_.as<uint32_t>();
+      uint32_t* b = k + m;
+      uint32_t* heads = b + m;
+      uint32_t* run_of = heads + m;
+      uint32_t* akeys = run_of + m;
+      uint32_t* rstart = akeys + m;
+      uint64_t* d_runs = reinterpret_cast<uint64_t*>(rstart + m);  // 24m bytes in: 8-byte aligned
+      uint32_t* tmp = reinterpret_cast<uint32_t*>(d_runs + 8);
+      hipLaunchKernelGGL(k_bins_split, dim3(blocks_for(m, 256)), dim3(256), 0, s_, m,
+                         (const uint64_t*)keys_.as<uint64_t>(), k, b, heads);
+      scan_excl_u32(heads, run_of, m, d_runs, tmp, s_);
+      hipLaunchKernelGGL(k_bins_runs, dim3(blocks_for(m, 256)), dim3(256), 0, s_, m, (const uint32_t*)k,
+                         (const uint32_t*)heads, (const uint32_t*)run_of, akeys, rstart);
+      uint64_t runs = 0;
+      CDX_HIP_CHECK(hipMemcpyAsync(&runs, d_runs, 8, hipMemcpyDeviceToHost, s_));
+      sync();
+      CDX_REQUIRE(runs == A_, ERR_HIP, "internal error: active bins of the pair search and reduce_by_key differ");
+      if (runs)
+        hipLaunchKernelGGL(k_bins_occupancy, dim3(blocks_for(runs, 256)), dim3(256), 0, s_, runs, m,
+                           (const uint32_t*)rstart, heads);  // heads no longer needed: occupancy goes there
+      if (sorted_keys) CDX_HIP_CHECK(hipMemcpyAsync(sorted_keys, k, m * 4, hipMemcpyDeviceToHost, s_));
+      if (sorted_bodies) CDX_HIP_CHECK(hipMemcpyAsync(sorted_bodies, b, m * 4, hipMemcpyDeviceToHost, s_));
+      if (active_keys && runs) CDX_HIP_CHECK(hipMemcpyAsync(active_keys, akeys, runs * 4, hipMemcpyDeviceToHost, s_));
+      if (active_counts && runs)
+        CDX_HIP_CHECK(hipMemcpyAsync(active_counts, heads, runs * 4, hipMemcpyDeviceToHost, s_));
+      sync();
     }
   }
 
@@ -378,6 +395,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void read_pairs_permuted(int64_t* out) override {
     sync();
     if (P_) CDX_HIP_CHECK(hipMemcpy(out, perm_.p, P_ * 8, hipMemcpyDeviceToHost));
+  }
+  void read_pair_hits(uint32_t* out) override {
+    sync();
+    if (P_) CDX_HIP_CHECK(hipMemcpy(out, np_hit_.p, P_ * 4, hipMemcpyDeviceToHost));
   }
   void read_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) override {
     sync();
@@ -399,7 +420,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
-                           &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_};
+                           &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &bins_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -1301,7 +1322,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   int mode_ = NP_NATIVE;
   float tau_ = 1e-2f;
 
-  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, meta_, scount_, soff_, selrng_;
+  DevBuf status_, stephdr_, stage_, upd_, tmp_out_, longs_, meta_, scount_, soff_, selrng_, bins_;
   void* stage_host_ = nullptr;
   size_t stage_host_bytes_ = 0;
   hipEvent_t ev_stage_ = nullptr;  // after the last copies out of stage_host_

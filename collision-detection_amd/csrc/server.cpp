@@ -226,19 +226,32 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
   uint64_t need = k.eng->resident_bytes_estimate(n, nb);
   if (first_in_class) need += k.eng->step_bytes_estimate(std::max<uint64_t>(k.max_shapes, n));
   {
+    // check and reserve in one step, so concurrent admissions (of other
+    // classes) cannot both pass the check; given back if admission fails
     std::lock_guard<std::mutex> lk(mu_);
     if (committed_ + need > budget_)
       throw Error(ERR_CAPACITY, "scene of " + std::to_string(n) + " shapes needs ~" + std::to_string(need >> 20) +
                                     " MiB of HBM; " + std::to_string((budget_ - committed_) >> 20) +
                                     " MiB of the budget are left");
+    committed_ += need;
   }
+  struct Reservation {
+    Server* s;
+    uint64_t bytes;
+    ~Reservation() {
+      if (bytes) {
+        std::lock_guard<std::mutex> lk(s->mu_);
+        s->committed_ -= bytes;
+      }
+    }
+  } reservation{this, need};
   if (bx == 0 && by == 0 && bz == 0) {
     TuneChoice t = tuner_.choose(n, T, A, B, R, ID, nb, pos, rot, active);
     for (int i = 0; i < 3; ++i) g[i] = t.grid[i];
-    if (t.m_per_shape > m_per_shape_ || t.p_per_shape > p_per_shape_) {
-      m_per_shape_ = std::max(m_per_shape_, t.m_per_shape);
-      p_per_shape_ = std::max(p_per_shape_, t.p_per_shape);
-      k.eng->set_capacity_factors(m_per_shape_ * 1.25 + 1.0, p_per_shape_ * 1.25 + 1.0);
+    if (t.m_per_shape > k.m_per_shape || t.p_per_shape > k.p_per_shape) {
+      k.m_per_shape = std::max(k.m_per_shape, t.m_per_shape);
+      k.p_per_shape = std::max(k.p_per_shape, t.p_per_shape);
+      k.eng->set_capacity_factors(k.m_per_shape * 1.25 + 1.0, k.p_per_shape * 1.25 + 1.0);
     }
   }
   const uint32_t local = k.eng->add_scene(n, T, A, B, R, ID, nb, pos, rot, active, g[0], g[1], g[2]);
@@ -247,8 +260,8 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
   k.min_n = std::min(k.min_n, n);  // a step holds at most max_shapes / min_n scenes
   const uint64_t per_step = std::min<uint64_t>(nsc, std::max<uint64_t>(1, k.max_shapes / k.min_n));
   k.eng->reserve_step(std::max<uint64_t>(k.max_shapes, n), per_step);
+  reservation.bytes = 0;  // admitted: the footprint stays committed
   std::lock_guard<std::mutex> lk(mu_);
-  committed_ += need;
   SceneRef r;
   r.cls = c;
   r.local = local;

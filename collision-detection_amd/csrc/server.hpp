@@ -116,6 +116,9 @@ class Server {
     uint64_t lo, hi;  // shape-count range [lo, hi)
     uint64_t max_shapes;  // shapes per step of this class
     uint64_t min_n = ~0ull;  // smallest resident scene (bounds the scenes per step)
+    // largest bin entries / pairs per shape the autotuner measured in this
+    // class (guarded by eng_mu, like the engine they size)
+    double m_per_shape = 0.0, p_per_shape = 0.0;
     std::unique_ptr<BatchBase> eng;
     std::mutex eng_mu;  // the engine is used by the worker and add_scene
     std::deque<std::shared_ptr<Request>> queue;
@@ -149,8 +152,6 @@ class Server {
   std::vector<std::unique_ptr<Class>> classes_;
   std::vector<SceneRef> scenes_;
   Autotuner tuner_;
-  double m_per_shape_ = 0.0, p_per_shape_ = 0.0;
-
   mutable std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   bool stop_ = false;

@@ -142,6 +142,20 @@ int cdx_do_narrowphase(cdx_ctx* ctx, int real_kind, int np_mode, double envelope
 int cdx_read_contacts(cdx_ctx* ctx, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids,
                       uint64_t capacity);
 
+/* Replaces Narrowphase::host_MPR_Store (Narrowphase.cuh:36-52, body
+ * Narrowphase.cu:464-504 over function_MPR_Store :316-423): the per-pair
+ * store without compaction.  For each pair i that makes a contact, norm[i],
+ * ptA[i], ptB[i] (real3), depth[i] (real) and ids[2i..2i+1] (body ids) are
+ * written and contact_active[i] is set to 0; for every other pair nothing
+ * is written, so the caller's values stay (the reference leaves them, and
+ * DoNarrowphase pre-fills contact_active with 1).  pairs is not modified.
+ * Any output pointer may be NULL. */
+int cdx_mpr_store(cdx_ctx* ctx, int real_kind, int np_mode, double envelope, uint64_t n_shapes, const int32_t* T,
+                  const void* A, const void* B, const void* C, const void* R, const uint32_t* ID, uint64_t n_bodies,
+                  const uint8_t* obj_active, const void* body_pos, const void* body_rot, const int64_t* pairs,
+                  uint64_t n_pairs, uint32_t* contact_active, void* norm, void* ptA, void* ptB, void* depth,
+                  int32_t* ids);
+
 /* ---- device-resident scenes (no reference counterpart: the reference
  * re-uploads through thrust containers on every call) ------------------- */
 int cdx_scene_create(cdx_ctx* ctx, int real_kind, cdx_scene** out);

@@ -21,6 +21,7 @@
 #ifndef CDX_COLLISION_DETECTION_H
 #define CDX_COLLISION_DETECTION_H
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <iostream>
@@ -199,6 +200,44 @@ class Narrowphase {
     cdx_detail::check(cdx_read_contacts(cdx_detail::context(), norm_data.data(), cpta_data.data(), cptb_data.data(),
                                         dpth_data.data(), reinterpret_cast<int32_t*>(bids_data.data()), nc));
     number_of_contacts = (uint)nc;
+    total_possible_contacts = (uint)potentialCollisions.size();
+  }
+  // Narrowphase.cuh:36-52, Narrowphase.cu:464-504: function_MPR_Store over
+  // the first total_possible_contacts pairs (the count of the last
+  // DoNarrowphase call, 0 before one, as the reference's member), without
+  // compaction: a contact's outputs go to its pair index and clear
+  // contact_active there; other entries are left as they are.  The shape
+  // and body counts the GPU upload needs are the largest index the pairs
+  // reach, as the reference reads nothing beyond them.
+  void host_MPR_Store(const shape_type* obj_data_T, const real3* obj_data_A, const real3* obj_data_B,
+                      const real3* obj_data_C, const real4* obj_data_R, const uint* obj_data_ID,
+                      const bool* obj_active, const real3* body_pos, const real4* body_rot, long long* contact_pair,
+                      uint* contact_active, real3* norm, real3* ptA, real3* ptB, real* contactDepth, int2* ids) {
+    const uint64_t np = total_possible_contacts;
+    if (np == 0) return;
+    uint64_t ns = 0, nb = 0;
+    for (uint64_t i = 0; i < np; ++i) {
+      const uint64_t a = (uint64_t)contact_pair[i] >> 32, b = (uint64_t)contact_pair[i] & 0xffffffffu;
+      ns = std::max<uint64_t>(ns, std::max(a, b) + 1);
+    }
+    for (uint64_t i = 0; i < ns; ++i) nb = std::max<uint64_t>(nb, (uint64_t)obj_data_ID[i] + 1);
+    static_assert(sizeof(bool) == 1, "obj_active is read as one byte per body");
+    cdx_detail::check(cdx_mpr_store(
+        cdx_detail::context(), CDX_REAL_KIND, mode_, (double)collision_envelope, ns,
+        reinterpret_cast<const int32_t*>(obj_data_T), obj_data_A, obj_data_B, obj_data_C, obj_data_R, obj_data_ID,
+        nb, reinterpret_cast<const uint8_t*>(obj_active), body_pos, body_rot,
+        reinterpret_cast<const int64_t*>(contact_pair), np, contact_active, norm, ptA, ptB, contactDepth,
+        reinterpret_cast<int32_t*>(ids)));
+  }
+  // custom_vector<bool> holds bytes here: its data() comes as unsigned char.
+  void host_MPR_Store(const shape_type* obj_data_T, const real3* obj_data_A, const real3* obj_data_B,
+                      const real3* obj_data_C, const real4* obj_data_R, const uint* obj_data_ID,
+                      const unsigned char* obj_active, const real3* body_pos, const real4* body_rot,
+                      long long* contact_pair, uint* contact_active, real3* norm, real3* ptA, real3* ptB,
+                      real* contactDepth, int2* ids) {
+    host_MPR_Store(obj_data_T, obj_data_A, obj_data_B, obj_data_C, obj_data_R, obj_data_ID,
+                   reinterpret_cast<const bool*>(obj_active), body_pos, body_rot, contact_pair, contact_active, norm,
+                   ptA, ptB, contactDepth, ids);
   }
   void SetCollisionEnvelope(const real& envelope) { collision_envelope = envelope; }
   real GetCollisionEnvelope() { return collision_envelope; }
@@ -209,6 +248,7 @@ class Narrowphase {
  private:
   real collision_envelope = 0;
   int mode_ = CDX_NP_NATIVE;
+  uint total_possible_contacts = 0;
 };
 
 #endif  // CDX_COLLISION_DETECTION_H

@@ -1,7 +1,8 @@
 // The full reference call chain through the drop-in classes:
 // GenerateAABB -> detectPossibleCollisions -> DoNarrowphase, on a seeded
-// scene of mixed shapes.  Inputs and outputs are written to argv[1] so the
-// test driver can compare them with the CPU oracle.
+// scene of mixed shapes, then host_MPR_Store on the same pairs.  Inputs and
+// outputs are written to argv[1] so the test driver can compare them with
+// the CPU oracle.
 #include <cmath>
 #include <cstdio>
 #include "collision_detection.h"
@@ -64,10 +65,20 @@ int main(int argc, char** argv) {
   custom_vector<int2> bids;
   uint nc = 0;
   np.DoNarrowphase(T, A, B, C, R, ID, active, pos, rot, pairs, norm, cpta, cptb, dpth, bids, nc);
+  // Narrowphase::host_MPR_Store over the unpermuted pair list: per-pair
+  // outputs at the pair index, contact_active cleared for contacts only.
+  const size_t P = pairs_sorted.size();
+  custom_vector<uint> cact(P, 1);
+  custom_vector<real3> pn(P, R3(NAN)), ppa(P, R3(NAN)), ppb(P, R3(NAN));
+  custom_vector<real> pd(P, NAN);
+  custom_vector<int2> pid(P, I2(-1, -1));
+  np.host_MPR_Store(T.data(), A.data(), B.data(), C.data(), R.data(), ID.data(), active.data(), pos.data(), rot.data(),
+                    pairs_sorted.data(), cact.data(), pn.data(), ppa.data(), ppb.data(), pd.data(), pid.data());
   FILE* f = fopen(argv[1], "wb");
   put(f, T); put(f, A); put(f, B); put(f, R); put(f, ID); put(f, pos); put(f, rot); put(f, active);
   put(f, aabb_unshifted); put(f, aabb); put(f, pairs_sorted); put(f, pairs);
   put(f, norm); put(f, cpta); put(f, cptb); put(f, dpth); put(f, bids);
+  put(f, cact); put(f, pn); put(f, ppa); put(f, ppb); put(f, pd); put(f, pid);
   fclose(f);
   printf("%d shapes, %zu pairs, %u contacts\n", n, pairs.size(), nc);
   return 0;

@@ -72,7 +72,9 @@ def _read_dump(path):
                 ("ID", np.uint32, 1), ("pos", np.float64, 3), ("rot", np.float64, 4), ("active", np.uint8, 1),
                 ("aabb", np.float64, 3), ("aabb_shifted", np.float64, 3), ("pairs", np.int64, 1),
                 ("pairs_permuted", np.int64, 1), ("norm", np.float64, 3), ("cpta", np.float64, 3),
-                ("cptb", np.float64, 3), ("dpth", np.float64, 1), ("bids", np.int32, 2)]
+                ("cptb", np.float64, 3), ("dpth", np.float64, 1), ("bids", np.int32, 2),
+                ("st_active", np.uint32, 1), ("st_norm", np.float64, 3), ("st_cpta", np.float64, 3),
+                ("st_cptb", np.float64, 3), ("st_dpth", np.float64, 1), ("st_bids", np.int32, 2)]
         d = {}
         for name, dt, k in spec:
             n = int(np.frombuffer(f.read(8), np.uint64)[0])
@@ -102,3 +104,29 @@ def test_pipeline_main_vs_oracle(exes, tmp_path, oracle, scenes):
     np.testing.assert_allclose(d["cpta"], npr.cpta, atol=1e-12, rtol=0)
     np.testing.assert_allclose(d["cptb"], npr.cptb, atol=1e-12, rtol=0)
     np.testing.assert_allclose(d["dpth"], npr.dpth, atol=1e-12, rtol=0)
+
+
+@pytest.mark.gpu
+def test_host_mpr_store_main_vs_oracle(exes, tmp_path, oracle, scenes):
+    """Narrowphase::host_MPR_Store (Narrowphase.cu:464-504): per-pair arrays
+    at the pair index; contacts clear contact_active, every other entry
+    keeps the caller's value."""
+    dump = str(tmp_path / "pipeline.bin")
+    r = _run(exes["pipeline"], dump, 6000)
+    assert r.returncode == 0
+    d = _read_dump(dump)
+    n = len(d["T"])
+    s = scenes.Scene(d["T"], d["A"], d["B"], np.zeros((n, 3)), d["R"], d["ID"], d["pos"], d["rot"], d["active"],
+                     (12, 12, 12))
+    pairs = d["pairs"]
+    npr = oracle.narrowphase(s, pairs, envelope=0.0005)
+    hit = npr.hit.astype(bool)
+    assert hit.sum() == npr.count > 100 and (~hit).sum() > 100
+    np.testing.assert_array_equal(d["st_active"], np.where(hit, 0, 1).astype(np.uint32))
+    for k, ref in (("st_norm", npr.norm), ("st_cpta", npr.cpta), ("st_cptb", npr.cptb)):
+        np.testing.assert_allclose(d[k][hit], ref, atol=1e-12, rtol=0)
+        assert np.isnan(d[k][~hit]).all()
+    np.testing.assert_allclose(d["st_dpth"][hit], npr.dpth, atol=1e-12, rtol=0)
+    assert np.isnan(d["st_dpth"][~hit]).all()
+    np.testing.assert_array_equal(d["st_bids"][hit], npr.bids)
+    assert (d["st_bids"][~hit] == -1).all()

@@ -155,6 +155,27 @@ def test_dropin_narrowphase_envelope_and_inactive(cdx, ctx, oracle, scenes):
     np.testing.assert_allclose(d, ref.dpth, atol=t, rtol=0)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dropin_mpr_store_per_pair(cdx, ctx, oracle, scenes, dtype):
+    """host_MPR_Store (Narrowphase.cu:464-504) through cdx_mpr_store: contacts
+    at their pair index, contact_active cleared there, other entries kept."""
+    s = scenes.mixed_bodies(n=8_000, seed=25, dtype=dtype)
+    rng = np.random.default_rng(6)
+    s.active = (rng.random(s.nb) < 0.7).astype(np.uint8)
+    pairs = oracle.broadphase(oracle.aabb(s), s.grid).pairs
+    print(f"mpr_store n={s.n} pairs={len(pairs)} dtype={np.dtype(dtype).name}", flush=True)
+    ref = oracle.narrowphase(s, pairs, envelope=0.001)
+    hit = ref.hit.astype(bool)
+    act, norm, pa, pb, d, ids = ctx.mpr_store(s, pairs, envelope=0.001, fill=np.nan, active_fill=1)
+    np.testing.assert_array_equal(act, np.where(hit, 0, 1))
+    np.testing.assert_array_equal(ids[hit], ref.bids)
+    assert (ids[~hit] == -1).all() and np.isnan(norm[~hit]).all() and np.isnan(d[~hit]).all()
+    t = _tol(s)
+    for got, want in ((norm, ref.norm), (pa, ref.cpta), (pb, ref.cptb)):
+        np.testing.assert_allclose(got[hit], want, atol=t, rtol=0)
+    np.testing.assert_allclose(d[hit], ref.dpth, atol=t, rtol=0)
+
+
 @pytest.mark.parametrize("mode", ["mixed", "native"])
 def test_mixed_equals_native_fp64(cdx, ctx, oracle, scenes, mode):
     """MIXED (FP32 filter + FP64 reference path for hits) equals the FP64 oracle."""
