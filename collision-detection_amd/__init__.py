@@ -87,9 +87,10 @@ class Counts(C.Structure):
     ]
 
 
-def build(arch: str = "gfx950") -> str:
-    """Compile libcdx.so in-tree with hipcc (cross-compiles without a GPU)."""
-    subprocess.check_call(["make", "-s", "-C", CSRC, f"ARCH={arch}", "-j8"])
+def build(arch: str = "gfx950", force: bool = False) -> str:
+    """Compile libcdx.so in-tree with hipcc (cross-compiles without a GPU);
+    force recompiles every source even when the library is up to date."""
+    subprocess.check_call(["make", "-s", "-C", CSRC, f"ARCH={arch}", "-j8"] + (["-B"] if force else []))
     return LIB_PATH
 
 
