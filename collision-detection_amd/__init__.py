@@ -33,6 +33,7 @@ STAGES = ["aabb", "bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_
 EXPORTS = [
     "cdx_version", "cdx_status_string", "cdx_device_count", "cdx_ctx_create", "cdx_ctx_destroy",
     "cdx_ctx_last_error", "cdx_generate_aabb", "cdx_detect_possible_collisions", "cdx_read_possible_collisions",
+    "cdx_detect_possible_collisions_noalloc",
     "cdx_do_narrowphase", "cdx_read_contacts", "cdx_mpr_store", "cdx_scene_create", "cdx_scene_destroy", "cdx_scene_last_error",
     "cdx_scene_set_shapes", "cdx_scene_set_bodies", "cdx_scene_update_bodies", "cdx_scene_set_aabbs",
     "cdx_scene_set_pairs", "cdx_scene_set_bins", "cdx_scene_set_envelope", "cdx_scene_set_narrowphase_mode",
@@ -115,6 +116,7 @@ def lib():
         "cdx_generate_aabb": ([vp, i32, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp], i32),
         "cdx_detect_possible_collisions": ([vp, i32, vp, u64, i32, i32, i32, vp], i32),
         "cdx_read_possible_collisions": ([vp, vp, u64], i32),
+        "cdx_detect_possible_collisions_noalloc": ([vp, i32, vp, u64, i32, i32, i32, vp, u64, vp, vp], i32),
         "cdx_do_narrowphase": ([vp, i32, i32, C.c_double, u64, vp, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, u64, vp],
                                i32),
         "cdx_read_contacts": ([vp, vp, vp, vp, vp, vp, u64], i32),

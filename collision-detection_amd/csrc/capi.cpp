@@ -10,6 +10,7 @@
 
 #include "../../include/cdx.h"
 #include "autotune.hpp"
+#include "cdx_noalloc.hpp"
 #include "cdx_pipeline.hpp"
 #include "server.hpp"
 #include "cdx_util.hpp"
@@ -169,6 +170,19 @@ int cdx_detect_possible_collisions(cdx_ctx* ctx, int real_kind, void* aabb_inout
     s.sync();
     if (n_aabb) s.read_aabb(aabb_inout, true);
     *n_pairs = s.counts().p;
+  });
+}
+
+int cdx_detect_possible_collisions_noalloc(cdx_ctx* ctx, int real_kind, void* aabb_inout, uint64_t n_aabb, int bx,
+                                           int by, int bz, int64_t* pairs_out, uint64_t pair_capacity,
+                                           uint64_t* n_pairs, void* stream) {
+  if (!ctx) return CDX_ERR_INVALID;
+  return guarded(&ctx->err, [&] {
+    check_kind(real_kind);
+    CDX_REQUIRE(n_pairs, CDX_ERR_INVALID, "null n_pairs");
+    *n_pairs = 0;
+    *n_pairs = cdx::noalloc_broadphase(ctx->device, real_kind, aabb_inout, n_aabb, bx, by, bz, pairs_out,
+                                       pair_capacity, stream);
   });
 }
 

@@ -156,6 +156,24 @@ int cdx_mpr_store(cdx_ctx* ctx, int real_kind, int np_mode, double envelope, uin
                   uint64_t n_pairs, uint32_t* contact_active, void* norm, void* ptA, void* ptB, void* depth,
                   int32_t* ids);
 
+/* No counterpart in the reference: detectPossibleCollisions for callers
+ * that cannot allocate (code under HIP graph capture set-up, tight serving
+ * loops, harnesses that forbid allocation).  The same kernels as
+ * cdx_detect_possible_collisions, on DEVICE buffers: aabb_inout (2*n_aabb
+ * real3 in device memory) is shifted in place, the sorted unique pairs go to
+ * pairs_out (device memory, pair_capacity int64), their count to *n_pairs
+ * (host).  Every intermediate lives in one fixed device array of the library
+ * (48 MiB, taken once per device and held until the process ends); no
+ * device or pinned memory is allocated.  Limits: n_aabb <= 65536, and the
+ * bin entries must fit what the array holds at that size (about 1.2M for
+ * FP64 at the limit): beyond either, or with more pairs than pair_capacity,
+ * the call fails with CDX_ERR_CAPACITY.  Synchronous on `stream` (a
+ * hipStream_t; NULL = the default stream).  Calls on one device are
+ * serialised (they share its array). */
+int cdx_detect_possible_collisions_noalloc(cdx_ctx* ctx, int real_kind, void* aabb_inout, uint64_t n_aabb,
+                                           int bins_x, int bins_y, int bins_z, int64_t* pairs_out,
+                                           uint64_t pair_capacity, uint64_t* n_pairs, void* stream);
+
 /* ---- device-resident scenes (no reference counterpart: the reference
  * re-uploads through thrust containers on every call) ------------------- */
 int cdx_scene_create(cdx_ctx* ctx, int real_kind, cdx_scene** out);
