@@ -25,6 +25,12 @@ import time
 
 import numpy as np
 
+# The CPU baseline binds its OpenMP threads to neighbouring cores (the plan
+# of record for the reference's OpenMP path); set before any OpenMP runtime
+# loads in this process.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "collision-detection_amd")
 
@@ -136,6 +142,31 @@ def read_resolve_fp64():
             "source": "profiles/pmc_fp64_resolve_mixed1M.txt (rocprofv3 counters and kernel time)"}
 
 
+def contact_error(got, ref, scale):
+    """Contacts of the GPU against the FP64 oracle's, matched by body-id pair.
+
+    normal_err: max-norm of N_gpu - N_oracle (unit vectors); depth_rel_err:
+    |d_gpu - d_oracle| / |d_oracle| over contacts deeper than 1e-3 x scale
+    (relative error is meaningless at grazing depth), depth_abs_err over all
+    matched contacts in units of scale (the typical shape size)."""
+    gn, ga, gb, gd, gi = got
+    key = lambda ids: (ids[:, 0].astype(np.int64) << 32) | ids[:, 1].astype(np.int64)
+    kg, kr = key(gi), key(ref.bids)
+    both, ig, ir = np.intersect1d(kg, kr, assume_unique=True, return_indices=True)
+    out = {"contacts": int(len(both)), "only_gpu": int(len(kg) - len(both)), "only_oracle": int(len(kr) - len(both))}
+    if len(both) == 0:
+        return out
+    ne = np.abs(gn[ig].astype(np.float64) - ref.norm[ir]).max(axis=1)
+    dg, dr = gd[ig].astype(np.float64), ref.dpth[ir].astype(np.float64)
+    deep = np.abs(dr) > 1e-3 * scale
+    de = np.abs(dg - dr)
+    rel = de[deep] / np.abs(dr[deep]) if deep.any() else np.zeros(1)
+    out.update({"normal_err_max": float(ne.max()), "normal_err_p99": float(np.percentile(ne, 99)),
+                "depth_rel_err_max": float(rel.max()), "depth_rel_err_p99": float(np.percentile(rel, 99)),
+                "depth_abs_err_max_over_scale": float(de.max() / scale)})
+    return out
+
+
 def _host_info():
     model = ""
     try:
@@ -169,9 +200,34 @@ def _oracle_passes(oracle, s, flags, min_seconds):
     return reps, t_total, parts, pairs, contacts, bp.pairs
 
 
-def cpu_baseline(oracle, s, min_seconds=10.0, all_parallel_seconds=5.0):
-    """The oracle pipeline (AABB + broadphase + narrowphase, FP64) on the whole workload, repeated for ~10 s with
-    the reference's default parallel flags (the faithful baseline), then ~5 s with every broadphase step parallel."""
+def _oracle_scene_rate(oracle, ss, grids, min_seconds):
+    """Full oracle passes over a fixed list of scenes, one at a time: scenes/s."""
+    done, t0 = 0, time.perf_counter()
+    while True:
+        for s, g in zip(ss, grids):
+            ab = oracle.aabb(s)
+            bp = oracle.broadphase(ab, g, oracle.DEFAULT_FLAGS)
+            oracle.narrowphase(s, bp.pairs)
+            done += 1
+        if time.perf_counter() - t0 >= min_seconds:
+            break
+    dt = time.perf_counter() - t0
+    return done / dt, done, dt
+
+
+def cpu_leg(oracle, scenes, s, checks, min_seconds=10.0, all_parallel_seconds=5.0):
+    """The CPU side of the run, after every GPU leg.
+
+    Baselines: the oracle (C++/OpenMP restatement of the reference's OpenMP
+    path), OpenMP on all host threads bound close, on bounded samples: the
+    mixed-1M workload (~10 s with the reference's default parallel flags,
+    ~5 s all-parallel), the 8,192-sphere workload, the broadphase of a
+    2,097,152-sphere block of the 16M lattice, and per-scene loops over
+    fixed samples of the batched and served scene sets.
+    Checks: the GPU's contacts of each workload against the FP64 oracle
+    (normal / depth error), and the FLOP census of the pairs the GPU's
+    reference-order MPR launch ran on (the roofline's algorithmic FLOPs).
+    """
     reps, t_total, parts, pairs, contacts, pair_list = _oracle_passes(oracle, s, oracle.DEFAULT_FLAGS, min_seconds)
     out = {
         "value": round(s.n * reps / t_total, 1),
@@ -191,6 +247,72 @@ def cpu_baseline(oracle, s, min_seconds=10.0, all_parallel_seconds=5.0):
             "sample": (f"{r2} passes ({t2:.1f} s), every broadphase step parallel; per pass aabb {p2[0] / r2:.3f}s "
                        f"broadphase {p2[1] / r2:.3f}s narrowphase {p2[2] / r2:.3f}s")}
     out["mpr_steps"] = mpr_step_histogram(oracle, s, pair_list)
+    ref1 = oracle.narrowphase(s, pair_list)
+    configs = {}
+    precision = {}
+    c1 = checks.get("C1")
+    if c1 is not None:
+        precision["C1_mixed_1M"] = dict(mode=c1["mode"], **contact_error(c1["contacts"], ref1, 0.01))
+        census = oracle.mpr_flops(s, pair_list)
+        q = c1["mpr_queue"]
+        out["mpr_flop_census"] = {
+            "pairs": int(len(pair_list)), "flops_all_pairs": int(census.sum()),
+            "mpr_launch_pairs": int(len(q)), "flops_mpr_launch": int(census[q].sum()),
+            "note": "FP64 +, -, *, /, sqrt of function_MPR_Store per pair after world placement (oracle count)"}
+    c0 = checks.get("C0")
+    if c0 is not None:
+        s0 = c0["scene"]
+        t0 = time.perf_counter()
+        r0 = 0
+        while time.perf_counter() - t0 < 3.0:
+            ab = oracle.aabb(s0)
+            bp = oracle.broadphase(ab, s0.grid, oracle.DEFAULT_FLAGS)
+            ref0 = oracle.narrowphase(s0, bp.pairs)
+            r0 += 1
+        dt = time.perf_counter() - t0
+        configs["C0_spheres_8k"] = {"value": round(s0.n * r0 / dt, 1), "unit": "bodies/s",
+                                    "sample": f"{r0} full passes over the 8,192-sphere FP64 workload ({dt:.1f} s)",
+                                    "gpu_bodies_per_s": c0.get("gpu_bodies_per_s")}
+        precision["C0_spheres_8k"] = dict(mode=c0["mode"], **contact_error(c0["contacts"], ref0, 0.01))
+    c2 = checks.get("C2")
+    if c2 is not None:
+        blk = scenes.lattice_spheres(side=128)
+        t0 = time.perf_counter()
+        bp = oracle.broadphase(oracle.aabb(blk), blk.grid, oracle.DEFAULT_FLAGS)
+        dt = time.perf_counter() - t0
+        configs["C2_lattice_broadphase"] = {
+            "value": round(blk.n / dt, 1), "unit": "bodies/s",
+            "sample": (f"AABB + broadphase of a 128^3 block (2,097,152 FP32 spheres, same spacing and radii) of the "
+                       f"16M lattice, one pass ({dt:.1f} s), {len(bp.pairs)} pairs"),
+            "gpu_bodies_per_s": c2.get("gpu_broadphase_bodies_per_s")}
+        sub = c2["scene"]
+        s64 = sub.astype(np.float64)
+        ref2 = oracle.narrowphase(s64, oracle.broadphase(oracle.aabb(s64), sub.grid).pairs)
+        precision["C2_lattice_subset"] = dict(mode=c2["mode"], workload=c2["workload"],
+                                              **contact_error(c2["contacts"], ref2, 0.005))
+    c3 = checks.get("C3")
+    if c3 is not None:
+        ss, grids = c3["scenes"], c3["grids"]
+        rate, done, dt = _oracle_scene_rate(oracle, ss, grids, 3.0)
+        configs["C3_batched"] = {"value": round(rate, 2), "unit": "scenes/s",
+                                 "sample": f"{len(ss)} fixed scenes of the batched set, full pass each, one at a "
+                                           f"time ({done} scene passes, {dt:.1f} s)"}
+        errs = [contact_error(got, oracle.narrowphase(sc, oracle.broadphase(oracle.aabb(sc), g).pairs), 0.01)
+                for sc, g, got in zip(ss, grids, c3["contacts"])]
+        precision["C3_batched_sample"] = {
+            "mode": c3["mode"], "scenes": len(errs), "contacts": sum(e["contacts"] for e in errs),
+            "only_gpu": sum(e["only_gpu"] for e in errs), "only_oracle": sum(e["only_oracle"] for e in errs),
+            "normal_err_max": max(e.get("normal_err_max", 0.0) for e in errs),
+            "depth_rel_err_max": max(e.get("depth_rel_err_max", 0.0) for e in errs)}
+    c4 = checks.get("C4")
+    if c4 is not None:
+        ss = c4["scenes"]
+        rate, done, dt = _oracle_scene_rate(oracle, ss, c4["grids"], 3.0)
+        configs["C4_served"] = {"value": round(rate, 2), "unit": "scenes/s",
+                                "sample": f"{len(ss)} scenes drawn from the served size distribution, full pass "
+                                          f"each, one at a time ({done} scene passes, {dt:.1f} s)"}
+    out["configs"] = configs
+    out["precision_vs_fp64_oracle"] = precision
     return out
 
 
@@ -272,6 +394,18 @@ def lattice_leg(pkg, scenes, ctx, side, steps):
     sc = pkg.Scene(ctx, np.float32).load(s, bins=tuned)
     tavg, twall, tc = _bp_measure(sc, steps)
     sc.close()
+    del s
+    # a 64^3 block of the same lattice, full step in NATIVE FP32, for the
+    # FP32 contact error against the FP64 oracle
+    sub = scenes.lattice_spheres(side=64)
+    sc = pkg.Scene(ctx, np.float32).load(sub)
+    sc.set_mode(pkg.NP_NATIVE)
+    sc.step()
+    sc.sync()
+    check = {"mode": "native-fp32", "scene": sub, "contacts": sc.read_contacts(),
+             "workload": "lattice-64^3 (262,144 FP32 spheres, same spacing and radii)",
+             "gpu_broadphase_bodies_per_s": round(n / (bp_ms * 1e-3), 1)}
+    sc.close()
     if tc.pairs != p:
         raise RuntimeError(f"autotuned grid changed the pair count: {tc.pairs} != {p}")
     t_ms = sum(tavg[k] for k in BP_STAGES)
@@ -285,7 +419,7 @@ def lattice_leg(pkg, scenes, ctx, side, steps):
         "broadphase_GBs": round(t_algo / (t_ms * 1e-3) / 1e9, 1),
         "broadphase_hbm_frac": round(t_algo / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
-    return out
+    return out, check
 
 
 def _bp_algo_bytes(n, m, a, p):
@@ -371,6 +505,9 @@ def batched_leg(pkg, scenes, ctx, count, steps):
         if cc.pairs != per_scene[j].pairs or cc.contacts != per_scene[j].contacts:
             mismatch += 1
         sc.close()
+    sample = list(range(0, count, max(1, count // 16)))[:16]
+    check = {"mode": "mixed", "scenes": [ss[j] for j in sample], "grids": [b.scene_bins(j) for j in sample],
+             "contacts": [b.read_contacts(j) for j in sample]}
     lat_ms = np.array(lat) * 1e3
     plain_sps = count / float(np.sum(plain))
     sps = count * steps / total
@@ -391,7 +528,7 @@ def batched_leg(pkg, scenes, ctx, count, steps):
         "device_bytes": int(b.device_bytes()),
     }
     b.close()
-    return out
+    return out, check
 
 
 def served_leg(pkg, scenes, ctx, args):
@@ -464,6 +601,9 @@ def served_leg(pkg, scenes, ctx, args):
     plain = np.array(plain)
     # the sample's mean cost, weighted to the whole fleet's mean size
     plain_sps = len(plain) / float(plain.sum())
+    cpu_ids = [int(i) for i in sample[:16]]
+    check = {"scenes": [scenes.fleet_scene(int(sizes[i]), 4.0, int(seeds[i])) for i in cpu_ids],
+             "grids": [srv.scene_info(sid_of[i])["bins"] for i in cpu_ids]}
 
     # warm up every class (graph capture, pinned pose and result blocks of
     # every size bucket) with one request per scene
@@ -594,7 +734,7 @@ def served_leg(pkg, scenes, ctx, args):
         "speedup_vs_plain_best": round(best["scenes_per_s"] / plain_sps, 2) if best else None,
     }
     srv.close()
-    return out
+    return out, check
 
 
 def copy_calibration(nbytes, reps=5):
@@ -712,28 +852,38 @@ def main():
     log(f"graph path: {graph_path}")
     b1.close()
 
-    # Roofline of the slowest single-kernel HBM-bound stage (HIP events on
-    # the scene stream around that one launch).  The FP64 MPR re-solve is
-    # VALU/latency-bound and reported with the stage times instead.
+    checks = {"C1": {"mode": args.mode, "contacts": sc.read_contacts(), "mpr_queue": sc.read_mpr_queue()}}
+
+    # The HBM-bound single-kernel stages: algorithmic bytes over their
+    # HIP-event times on the scene stream (the slowest one leads).
     bytes_ = stage_bytes(s, c)
-    dom = max(bytes_, key=lambda k: stage_avg.get(k, 0.0))
-    dom_ms = stage_avg[dom]
-    achieved = bytes_[dom] / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": read_traffic(dom),
-        "kernel": STAGE_KERNEL[dom],
-        "stage": dom,
-        "algorithmic_bytes": int(bytes_[dom]),
-        "avg_launch_ms": round(dom_ms, 4),
+    hbm = max(bytes_, key=lambda k: stage_avg.get(k, 0.0))
+    hbm_ms = stage_avg[hbm]
+    hbm_gbs = bytes_[hbm] / (hbm_ms * 1e-3) / 1e9 if hbm_ms > 0 else 0.0
+    hbm_stage = {
+        "bound": "hbm", "kernel": STAGE_KERNEL[hbm], "stage": hbm, "achieved": round(hbm_gbs, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
+        "traffic": read_traffic(hbm), "algorithmic_bytes": int(bytes_[hbm]), "avg_launch_ms": round(hbm_ms, 4),
         "all_single_kernel_stages": {
             k: {"ms": round(stage_avg[k], 4), "GBs": round(b / (stage_avg[k] * 1e-3) / 1e9, 1) if stage_avg[k] else None}
             for k, b in bytes_.items()},
     }
+
+    # C0: the 8,192-sphere FP64 workload, full step (precision check and
+    # GPU rate beside its CPU baseline).
+    s0 = scenes.small_spheres()
+    sc0 = pkg.Scene(ctx, s0.dtype).load(s0)
+    sc0.set_mode(mode, 1e-2)
+    sc0.step()
+    sc0.sync()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        sc0.step()
+        sc0.sync()
+    c0_rate = s0.n * 20 / (time.perf_counter() - t0)
+    checks["C0"] = {"mode": args.mode, "scene": s0, "contacts": sc0.read_contacts(),
+                    "gpu_bodies_per_s": round(c0_rate, 1)}
+    sc0.close()
 
     dropin = None
     if args.dropin_passes > 0:
@@ -743,25 +893,52 @@ def main():
 
     batched = None
     if args.batched_scenes > 0:
-        batched = batched_leg(pkg, scenes, ctx, args.batched_scenes, args.batched_steps)
+        batched, checks["C3"] = batched_leg(pkg, scenes, ctx, args.batched_scenes, args.batched_steps)
         log(f"batched: {json.dumps(batched)}")
 
     served = None
     if args.served_scenes > 0:
-        served = served_leg(pkg, scenes, ctx, args)
+        served, checks["C4"] = served_leg(pkg, scenes, ctx, args)
         log(f"served: {json.dumps(served)}")
 
     lattice = None
     if args.lattice_side > 0:
-        lattice = lattice_leg(pkg, scenes, ctx, args.lattice_side, args.lattice_steps)
+        lattice, checks["C2"] = lattice_leg(pkg, scenes, ctx, args.lattice_side, args.lattice_steps)
         log(f"lattice: {lattice}")
 
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        log("timing the CPU baseline")
-        cpu = cpu_baseline(oracle, s)
+        log("timing the CPU baseline and checking contacts against the FP64 oracle")
+        cpu = cpu_leg(oracle, scenes, s, checks)
+
+    # Roofline of the step's dominant kernel: the reference-order FP64 MPR
+    # (k_np_resolve), timed alone with HIP events on the scene stream; its
+    # algorithmic FLOPs are the oracle's census of the reference algorithm's
+    # FP64 operations on exactly the pairs that launch ran on.
+    mpr_ms = stage_avg.get("np_mpr", 0.0)
+    census = cpu.get("mpr_flop_census") if cpu else None
+    flops = census["flops_mpr_launch"] if census else None
+    tflops = flops / (mpr_ms * 1e-3) / 1e12 if flops and mpr_ms > 0 else None
+    roofline = {
+        "bound": "mfma",
+        "compute_unit": ("FP64 vector ALU; the kernel issues no MFMA instruction: MI355X's FP64 vector peak equals "
+                         "its FP64 matrix peak, 78.6 TFLOP/s"),
+        "achieved": round(tflops, 3) if tflops is not None else None,
+        "peak": FP64_VECTOR_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(tflops / FP64_VECTOR_PEAK_TFLOPS, 4) if tflops is not None else None,
+        "traffic": read_traffic("np_mpr"),
+        "kernel": "k_np_resolve<double, double>",
+        "stage": "np_mpr",
+        "avg_launch_ms": round(mpr_ms, 4),
+        "share_of_step": round(mpr_ms / sum(stage_avg[k] for k in pkg.STAGES if k != "np_mpr"), 3),
+        "pairs_per_launch": int(len(checks["C1"]["mpr_queue"])),
+        "algorithmic_flops_per_launch": int(flops) if flops else None,
+        "flops_unit": "FP64 +, -, *, /, sqrt of the reference MPR per pair (oracle census), summed over the launch",
+        "issued_fp64_counters": read_resolve_fp64(),
+    }
 
     lat_ms = np.array(lat) * 1000.0
     scene_path = {"ms_per_step": round(ms_step, 4), "bodies_per_s": round(s.n * args.steps / total, 1),
@@ -791,7 +968,7 @@ def main():
         "scene_path": scene_path,
         "dropin_path": dropin,
         "roofline": roofline,
-        "resolve_fp64": read_resolve_fp64(),
+        "hbm_stage_roofline": hbm_stage,
         "cpu_baseline": cpu,
         "lattice_16M_broadphase": lattice,
         "batched_512": batched,

@@ -27,7 +27,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 REAL_F64, REAL_F32 = 0, 1
 NP_NATIVE, NP_MIXED, NP_FAST = 0, 1, 2
 STAGES = ["aabb", "bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs",
-          "bp_pairsort", "np_main", "np_resolve", "np_compact", "bp_pairscan", "bp_pairemit"]
+          "bp_pairsort", "np_main", "np_resolve", "np_compact", "bp_pairscan", "bp_pairemit", "np_mpr"]
 
 # Every extern "C" symbol include/cdx.h declares.
 EXPORTS = [
@@ -39,7 +39,8 @@ EXPORTS = [
     "cdx_scene_set_pairs", "cdx_scene_set_bins", "cdx_scene_set_envelope", "cdx_scene_set_narrowphase_mode",
     "cdx_scene_set_profiling", "cdx_scene_aabb", "cdx_scene_broadphase", "cdx_scene_narrowphase",
     "cdx_scene_step", "cdx_scene_sync", "cdx_scene_counts", "cdx_scene_read_aabb", "cdx_scene_read_bins",
-    "cdx_scene_read_pairs", "cdx_scene_read_pairs_permuted", "cdx_scene_read_contacts", "cdx_scene_stage_ms",
+    "cdx_scene_read_pairs", "cdx_scene_read_pairs_permuted", "cdx_scene_read_mpr_queue", "cdx_scene_read_contacts",
+    "cdx_scene_stage_ms",
     "cdx_scene_device_bytes", "cdx_batch_create", "cdx_batch_destroy", "cdx_batch_last_error",
     "cdx_batch_add_scene", "cdx_batch_scene_bins", "cdx_batch_scene_count", "cdx_batch_update_bodies",
     "cdx_batch_set_envelope", "cdx_batch_set_narrowphase_mode", "cdx_batch_select", "cdx_batch_step",
@@ -144,6 +145,7 @@ def lib():
         "cdx_scene_read_bins": ([vp, vp, vp, vp, vp, vp], i32),
         "cdx_scene_read_pairs": ([vp, vp], i32),
         "cdx_scene_read_pairs_permuted": ([vp, vp], i32),
+        "cdx_scene_read_mpr_queue": ([vp, vp, u64, vp], i32),
         "cdx_scene_read_contacts": ([vp, vp, vp, vp, vp, vp], i32),
         "cdx_scene_stage_ms": ([vp, vp], i32),
         "cdx_scene_device_bytes": ([vp, vp], i32),
@@ -376,6 +378,15 @@ class Scene:
     def read_pairs_permuted(self):
         out = np.zeros(self.counts().pairs, np.int64)
         self._chk(lib().cdx_scene_read_pairs_permuted(self.h, _p(out)))
+        return out
+
+    def read_mpr_queue(self):
+        """Indices (into read_pairs()) of the pairs the last narrowphase's
+        reference-order MPR launch ran on."""
+        n = C.c_uint64(0)
+        self._chk(lib().cdx_scene_read_mpr_queue(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint32)
+        self._chk(lib().cdx_scene_read_mpr_queue(self.h, _p(out), n.value, C.byref(n)))
         return out
 
     def read_contacts(self):

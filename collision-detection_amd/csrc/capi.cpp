@@ -352,6 +352,12 @@ int cdx_scene_read_pairs_permuted(cdx_scene* scene, int64_t* out) { SCENE_CALL(s
 int cdx_scene_read_contacts(cdx_scene* scene, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) {
   SCENE_CALL(s.read_contacts(norm, cpta, cptb, dpth, bids));
 }
+int cdx_scene_read_mpr_queue(cdx_scene* scene, uint32_t* pair_index, uint64_t capacity, uint64_t* count) {
+  SCENE_CALL(CDX_REQUIRE(count, CDX_ERR_INVALID, "null count"); *count = s.read_mpr_queue(pair_index, capacity));
+}
+static_assert(cdx::ST_COUNT == CDX_STAGE_COUNT, "stage list of cdx.h and the pipeline differ");
+static_assert(cdx::ST_NP_MPR == CDX_STAGE_NP_MPR && cdx::ST_BP_PAIREMIT == CDX_STAGE_BP_PAIREMIT, "stage numbers");
+
 int cdx_scene_stage_ms(cdx_scene* scene, double* ms) {
   SCENE_CALL(CDX_REQUIRE(ms, CDX_ERR_INVALID, "null output"); s.stage_ms(ms));
 }

@@ -30,6 +30,7 @@ enum Stage : int {
   ST_NP_COMPACT,
   ST_BP_PAIRSCAN,  // scan of per-entry pair counts (between the two pair passes)
   ST_BP_PAIREMIT,  // pair emission pass
+  ST_NP_MPR,       // the reference-order MPR launch alone (inside ST_NP_RESOLVE)
   ST_COUNT
 };
 
@@ -76,6 +77,10 @@ class SceneBase {
   virtual void read_pairs_permuted(int64_t* out) = 0;
   // Per input pair (narrowphase order before compaction): 1 for a contact.
   virtual void read_pair_hits(uint32_t* out) = 0;
+  // Pairs (indices into the pair list) the reference-order MPR launch of the
+  // last narrowphase ran on, in queue order; returns their count and writes
+  // at most cap of them (out may be null).
+  virtual uint64_t read_mpr_queue(uint32_t* out, uint64_t cap) = 0;
   virtual void read_contacts(void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids) = 0;
   virtual void stage_ms(double* out) = 0;
   virtual hipStream_t stream() const = 0;

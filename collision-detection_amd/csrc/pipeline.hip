@@ -396,6 +396,17 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     sync();
     if (P_) CDX_HIP_CHECK(hipMemcpy(out, perm_.p, P_ * 8, hipMemcpyDeviceToHost));
   }
+  uint64_t read_mpr_queue(uint32_t* out, uint64_t cap) override {
+    sync();
+    const uint64_t q = mpr_q_;  // set by exact-mode (scene) narrowphases only
+    if (!out || !q) return q;
+    const uint64_t k = std::min(q, cap);
+    std::vector<uint64_t> e(k);
+    const DevBuf& b = mpr_in_tmp_ ? queue_tmp_ : queue_;
+    CDX_HIP_CHECK(hipMemcpy(e.data(), b.p, k * 8, hipMemcpyDeviceToHost));
+    for (uint64_t j = 0; j < k; ++j) out[j] = (uint32_t)e[j];
+    return q;
+  }
   void read_pair_hits(uint32_t* out) override {
     sync();
     if (P_) CDX_HIP_CHECK(hipMemcpy(out, np_hit_.p, P_ * 4, hipMemcpyDeviceToHost));
@@ -1103,6 +1114,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
         q = std::max<uint64_t>(hstat_->Q2, 1);
       }
     }
+    mpr_in_tmp_ = mode_ != NP_NATIVE;  // the filter's selection went to queue_tmp_
+    mpr_q_ = exact_ ? (mode_ != NP_NATIVE ? hstat_->Q2 : hstat_->Q) : 0;
+    mark(ST_NP_MPR, true);
     if (mode_ == NP_NATIVE)
       hipLaunchKernelGGL((k_np_resolve<R, R>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q, dq,
                          (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
@@ -1111,6 +1125,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       hipLaunchKernelGGL((k_np_resolve<R, double>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q, dq,
                          (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
                          o, rq);
+    mark(ST_NP_MPR, false);
     mark(ST_NP_RESOLVE, false);
     mark(ST_NP_COMPACT, true);
     scan_excl_u32(np_hit_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->C, scan_tmp_.as<uint32_t>(), s_, dP);
@@ -1338,6 +1353,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf out_, perm_;  // out_: the step's contact arrays back to back, C rows each
   DevBuf scan_tmp_, radix_tmp_, runs_part_;
   bool opened_ = false;  // the step's totals were zeroed by one k_stage_begin
+  bool mpr_in_tmp_ = false;  // the last reference-order MPR launch read queue_tmp_ (else queue_)
+  uint64_t mpr_q_ = 0;       // and this many pairs of it
 };
 
 }  // namespace

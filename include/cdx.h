@@ -79,7 +79,8 @@ extern "C" {
 #define CDX_STAGE_NP_COMPACT 11
 #define CDX_STAGE_BP_PAIRSCAN 12
 #define CDX_STAGE_BP_PAIREMIT 13
-#define CDX_STAGE_COUNT 14
+#define CDX_STAGE_NP_MPR 14 /* the reference-order MPR launch alone (part of NP_RESOLVE) */
+#define CDX_STAGE_COUNT 15
 
 typedef struct cdx_ctx cdx_ctx;
 typedef struct cdx_scene cdx_scene;
@@ -216,6 +217,11 @@ int cdx_scene_read_bins(cdx_scene* scene, uint32_t* bin_counts, uint32_t* sorted
                         uint32_t* active_keys, uint32_t* active_counts);
 int cdx_scene_read_pairs(cdx_scene* scene, int64_t* pairs_out);
 int cdx_scene_read_pairs_permuted(cdx_scene* scene, int64_t* pairs_out);
+/* The pairs (indices into the cdx_scene_read_pairs order) that the
+ * reference-order MPR launch of the last narrowphase ran on: every queued
+ * pair in NATIVE, the FP64 re-solves in MIXED / FAST.  *count receives how
+ * many; at most capacity indices are written (pair_index may be NULL). */
+int cdx_scene_read_mpr_queue(cdx_scene* scene, uint32_t* pair_index, uint64_t capacity, uint64_t* count);
 int cdx_scene_read_contacts(cdx_scene* scene, void* norm, void* cpta, void* cptb, void* dpth, int32_t* bids);
 /* Milliseconds per stage of the last run (profiling on), CDX_STAGE_COUNT values. */
 int cdx_scene_stage_ms(cdx_scene* scene, double* ms_out);

@@ -42,6 +42,41 @@
 
 using namespace orc;
 
+namespace orc {
+
+// Operation-counting real for the FLOP census of the narrowphase
+// (oracle_mpr_flops_f64): a double whose +, -, *, / and sqrt each add one
+// to a per-thread counter (negation, fabs and comparisons are free, as on
+// the GPU where they fold into operand modifiers and compares).
+struct Fl {
+  double v;
+  Fl() = default;
+  Fl(double x) : v(x) {}
+};
+inline thread_local uint64_t t_flops = 0;
+inline Fl operator+(Fl a, Fl b) { ++t_flops; return Fl(a.v + b.v); }
+inline Fl operator-(Fl a, Fl b) { ++t_flops; return Fl(a.v - b.v); }
+inline Fl operator*(Fl a, Fl b) { ++t_flops; return Fl(a.v * b.v); }
+inline Fl operator/(Fl a, Fl b) { ++t_flops; return Fl(a.v / b.v); }
+inline Fl operator-(Fl a) { return Fl(-a.v); }
+inline bool operator<(Fl a, Fl b) { return a.v < b.v; }
+inline bool operator>(Fl a, Fl b) { return a.v > b.v; }
+inline bool operator<=(Fl a, Fl b) { return a.v <= b.v; }
+inline bool operator>=(Fl a, Fl b) { return a.v >= b.v; }
+inline bool operator==(Fl a, Fl b) { return a.v == b.v; }
+inline bool operator!=(Fl a, Fl b) { return a.v != b.v; }
+inline Fl sqrt(Fl a) { ++t_flops; return Fl(std::sqrt(a.v)); }
+inline Fl fabs(Fl a) { return Fl(std::fabs(a.v)); }
+template <> inline Fl eps_of<Fl>() { return Fl(DBL_EPSILON); }
+
+// Counter value when a pair's world placement is done (function_MPR_Store
+// past Narrowphase.cu:377), for the census; a no-op for float / double.
+inline thread_local uint64_t t_flops_placed = 0;
+template <class R> inline void mark_placed() {}
+template <> inline void mark_placed<Fl>() { t_flops_placed = t_flops; }
+
+}  // namespace orc
+
 namespace {
 
 enum : int { SPHERE = 0, ELLIPSOID = 1, BOX = 2, CYLINDER = 3, RECT = 4, CONE = 5, TRIANGLEMESH = 6, CAPSULE = 7 };
@@ -519,6 +554,7 @@ bool mpr_store(int64_t p, const int32_t* T, const R* A, const R* B, const R* Rq,
   r4<R> BR = mult(rotB, ld4(Rq, ib));
   if (body_attached(tA)) AX = quatRotate(AX, rotA) + posA;
   if (body_attached(tB)) BX = quatRotate(BX, rotB) + posB;
+  mark_placed<R>();
   N = R3<R>(1, 0, 0);
   p1 = R3<R>(0, 0, 0);
   p2 = R3<R>(0, 0, 0);
@@ -713,6 +749,42 @@ void oracle_mpr_steps_f64(uint64_t P, const int64_t* pairs, const int32_t* T, co
     t_support_calls = 0;
     mpr_store<double>(pairs[i], T, A, B, Rq, ID, active, bpos, brot, 0.0, N, p1, p2, d, id2);
     steps[i] = (int32_t)(t_support_calls / 2);
+  }
+}
+
+// FLOP census of function_MPR_Store (Narrowphase.cu:316-423) per candidate
+// pair in FP64: the +, -, *, / and sqrt operations the reference algorithm
+// performs for the pair once both shapes are placed in the world (their
+// world centres and rotations are per-shape work), through the MPR and the
+// contact points; 0 for pairs that end before the placement (inactive,
+// same body).  The pair runs on a two-shape copy of its inputs.
+void oracle_mpr_flops_f64(uint64_t P, const int64_t* pairs, const int32_t* T, const double* A, const double* B,
+                          const double* Rq, const uint32_t* ID, const uint8_t* active, const double* bpos,
+                          const double* brot, double envelope, uint64_t* flops) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)P; ++i) {
+    const uint64_t ia = (uint64_t)pairs[i] >> 32, ib = (uint64_t)pairs[i] & 0xffffffffu;
+    const uint64_t sh[2] = {ia, ib};
+    int32_t t2[2];
+    uint32_t id2[2] = {0, ID[ia] == ID[ib] ? 0u : 1u};
+    uint8_t act2[2];
+    Fl a2[6], b2[6], r2[8], p2[6], q2[8];
+    for (int k = 0; k < 2; ++k) {
+      const uint64_t s = sh[k], b = ID[s];
+      t2[k] = T[s];
+      act2[id2[k]] = active[b];
+      for (int c = 0; c < 3; ++c) { a2[3 * k + c] = A[3 * s + c]; b2[3 * k + c] = B[3 * s + c]; }
+      for (int c = 0; c < 4; ++c) r2[4 * k + c] = Rq[4 * s + c];
+      for (int c = 0; c < 3; ++c) p2[3 * id2[k] + c] = bpos[3 * b + c];
+      for (int c = 0; c < 4; ++c) q2[4 * id2[k] + c] = brot[4 * b + c];
+    }
+    r3<Fl> N, pa, pb;
+    Fl d;
+    int32_t ids[2];
+    t_flops = 0;
+    t_flops_placed = 0;
+    mpr_store<Fl>((int64_t)1, t2, a2, b2, r2, id2, act2, p2, q2, Fl(envelope), N, pa, pb, d, ids);
+    flops[i] = t_flops_placed ? t_flops - t_flops_placed : 0;
   }
 }
 

@@ -9,6 +9,7 @@ The oracle restates the reference's OpenMP path:
   broadphase    Broadphase.cu:344-545
   narrowphase   Narrowphase.cu:316-595
   mpr_steps     the narrowphase's support steps per pair (MPR work histogram)
+  mpr_flops     the narrowphase's FP64 operations per pair (FLOP census)
 """
 from __future__ import annotations
 
@@ -70,8 +71,7 @@ def planted(k: int):
     """The oracle with planted slip k in the MPR contact point (ORACLE_PLANT
     in oracle.cpp), for tests that check the pins catch it."""
     path = os.path.join(HERE, "_build", f"liboracle_plant{k}.so")
-    if not os.path.exists(path):
-        subprocess.check_call(["make", "-s", "-C", HERE, "planted"])
+    subprocess.check_call(["make", "-s", "-C", HERE, "planted"])  # rebuilt when oracle.cpp changed
     return _configure(C.CDLL(path))
 
 
@@ -95,6 +95,7 @@ def _configure(L):
         L.oracle_quat_mult_f64.argtypes = [vp, vp, vp]
         L.oracle_mpr_steps_f64.argtypes = [C.c_uint64] + [vp] * 10
         L.oracle_max_threads.restype = C.c_int
+        L.oracle_mpr_flops_f64.argtypes = [C.c_uint64] + [vp] * 9 + [C.c_double, vp]
         L.oracle_mpr_portal_f64.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_mpr_portal_f64.restype = C.c_int
     return L
@@ -226,6 +227,20 @@ def mpr_steps(scene, pairs: np.ndarray) -> np.ndarray:
     out = np.zeros(len(pr), dtype=np.int32)
     lib().oracle_mpr_steps_f64(len(pr), _p(pr), _p(s.T), _p(s.A), _p(s.B), _p(s.R), _p(s.ID), _p(s.active),
                                _p(s.pos), _p(s.rot), _p(out))
+    return out
+
+
+def mpr_flops(scene, pairs: np.ndarray, envelope: float = 0.0) -> np.ndarray:
+    """FP64 operations (+, -, *, /, sqrt) function_MPR_Store performs per
+    candidate pair once the two shapes are placed in the world (MPR and
+    contact points; per-shape placement excluded); 0 where it ends earlier.
+    FP64 scenes only."""
+    s = scene
+    assert s.dtype == np.float64
+    pr = np.ascontiguousarray(pairs, dtype=np.int64)
+    out = np.zeros(len(pr), dtype=np.uint64)
+    lib().oracle_mpr_flops_f64(len(pr), _p(pr), _p(s.T), _p(s.A), _p(s.B), _p(s.R), _p(s.ID), _p(s.active),
+                               _p(s.pos), _p(s.rot), float(envelope), _p(out))
     return out
 
 

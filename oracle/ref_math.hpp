@@ -26,6 +26,12 @@
 
 namespace orc {
 
+// Unqualified below, so that the operation-counting real of the flop
+// census (oracle.cpp) finds its own overloads; for float / double these are
+// the std functions.
+using std::fabs;
+using std::sqrt;
+
 template <class R> struct r3 { R x, y, z; };
 template <class R> struct r4 { R w, x, y, z; };
 struct i3 { int x, y, z; };
@@ -44,18 +50,18 @@ template <class R> inline R dot(r3<R> a, r3<R> b) { return a.x * b.x + a.y * b.y
 template <class R> inline r3<R> cross(r3<R> a, r3<R> b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-template <class R> inline R length(r3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline R length(r3<R> a) { return sqrt(dot(a, a)); }
 template <class R> inline r3<R> normalize(r3<R> a) {
   R l = length(a);
   if (l == R(0)) return {R(1), R(0), R(0)};
   return a / l;
 }
-template <class R> inline r3<R> fabs3(r3<R> a) { return {std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; }
+template <class R> inline r3<R> fabs3(r3<R> a) { return {fabs(a.x), fabs(a.y), fabs(a.z)}; }
 template <class R> inline R sign(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : R(0)); }
 template <class R> inline R eps_of();
 template <> inline float eps_of<float>() { return FLT_EPSILON; }
 template <> inline double eps_of<double>() { return DBL_EPSILON; }
-template <class R> inline bool IsZero(R x) { return std::fabs(x) < eps_of<R>(); }
+template <class R> inline bool IsZero(R x) { return fabs(x) < eps_of<R>(); }
 template <class R> inline bool IsZero(r3<R> a) { return IsZero(a.x) && IsZero(a.y) && IsZero(a.z); }
 
 template <class R> inline r4<R> mult(r4<R> a, r4<R> b) {
@@ -87,7 +93,7 @@ template <class R> inline m33<R> AMat(r4<R> q) {
 }
 template <class R> inline m33<R> AbsMat(m33<R> a) {
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) a.m[i][j] = std::fabs(a.m[i][j]);
+    for (int j = 0; j < 3; ++j) a.m[i][j] = fabs(a.m[i][j]);
   return a;
 }
 template <class R> inline r3<R> MatMult(const m33<R>& a, r3<R> v) {

@@ -283,3 +283,27 @@ def test_broadphase_large_grid(cdx, ctx, oracle, axis):
 def test_empty_inputs(cdx, ctx):
     ab = np.zeros((0, 3))
     assert len(ctx.detect_possible_collisions(ab, (4, 4, 4))) == 0
+
+
+def test_mpr_queue_is_the_mpr_work(cdx, ctx, oracle, scenes):
+    """The reference-order MPR launch runs on exactly the pairs the reference
+    sends into CollideAndFindPoint (NATIVE: every pair with MPR steps in the
+    oracle), and on the FP64 re-solves in MIXED."""
+    s = scenes.mixed_bodies(n=20_000, seed=33)
+    pairs = oracle.broadphase(oracle.aabb(s), s.grid).pairs
+    want = np.nonzero(oracle.mpr_steps(s, pairs) > 0)[0]
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.set_mode(cdx.NP_NATIVE)
+    sc.set_profiling(True)
+    sc.step()
+    sc.sync()
+    q = sc.read_mpr_queue()
+    np.testing.assert_array_equal(np.sort(q), want)
+    assert sc.stage_ms()["np_mpr"] > 0
+    sc.set_mode(cdx.NP_MIXED, 1e-2)
+    sc.step()
+    sc.sync()
+    q = sc.read_mpr_queue()
+    assert len(q) == sc.counts().resolved and len(np.unique(q)) == len(q)
+    assert np.isin(q, want).all()
+    sc.close()

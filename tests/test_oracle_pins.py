@@ -565,3 +565,28 @@ def test_swap_symmetry(oracle, scenes):
     assert np.median(curved[:, 0]) < 1e-5 and curved[:, 0].max() < 2e-2
     assert np.median(curved[:, 1]) < 1e-6 and curved[:, 1].max() < 2e-3
     assert np.median(curved[:, 2:]) < 1e-5 and curved[:, 2:].max() < 2e-2
+
+
+def test_flop_census_sphere_pair_closed_form(oracle, scenes):
+    """The FLOP census counts each +, -, *, / and sqrt of function_MPR_Store
+    after placement: a touching sphere pair (Narrowphase.cu:382-396, then the
+    envelope and depth :412-415) is 51 operations, a separated one 10, and a
+    pair of one body 0."""
+    pair = np.array([(0 << 32) | 1], np.int64)
+    hit = _pair_scene(scenes, SPHERE, SPHERE, (0.7, 0.1, 0.0))
+    miss = _pair_scene(scenes, SPHERE, SPHERE, (1.7, 0.1, 0.0))
+    assert oracle.narrowphase(hit, pair).count == 1 and oracle.narrowphase(miss, pair).count == 0
+    assert oracle.mpr_flops(hit, pair)[0] == 51
+    assert oracle.mpr_flops(miss, pair)[0] == 10
+    same = _pair_scene(scenes, SPHERE, BOX, (0.7, 0.1, 0.0))
+    same.ID[:] = 0
+    assert oracle.mpr_flops(same, pair)[0] == 0
+    # MPR pairs: the census grows with the support steps the pair takes
+    s = scenes.mixed_bodies(n=4096, seed=3)
+    pairs = oracle.broadphase(oracle.aabb(s), s.grid).pairs
+    f = oracle.mpr_flops(s, pairs).astype(np.int64)
+    st = oracle.mpr_steps(s, pairs)
+    m = st > 0
+    assert (f[m] > 0).all()
+    per_step = f[m] / st[m]
+    assert 150 < np.median(per_step) < 600
