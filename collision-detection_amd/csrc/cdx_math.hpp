@@ -34,19 +34,27 @@ template <class R> CDX_HD v3<R> operator*(R s, v3<R> a) { return mk3(s * a.x, s 
 template <class R> CDX_HD v3<R> operator*(v3<R> a, v3<R> b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
 template <class R> CDX_HD v3<R> operator/(v3<R> a, R s) { return mk3(a.x / s, a.y / s, a.z / s); }
 
-template <class R> CDX_HD R dot(v3<R> a, v3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// Fused multiply-add, one rounding (v_fma_f32 / v_fma_f64).  The products
+// and sums of dot, cross, the rotation matrix and matrix-vector products are
+// written as explicit fused operations in a fixed order (the oracle states
+// the same ones), so the build's -ffp-contract=off changes none of them.
+CDX_HD float cfma(float a, float b, float c) { return fmaf(a, b, c); }
+CDX_HD double cfma(double a, double b, double c) { return fma(a, b, c); }
+
+template <class R> CDX_HD R dot(v3<R> a, v3<R> b) { return cfma(a.z, b.z, cfma(a.y, b.y, a.x * b.x)); }
 template <class R> CDX_HD v3<R> cross(v3<R> a, v3<R> b) {
-  return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+  return mk3(cfma(a.y, b.z, -(a.z * b.y)), cfma(a.z, b.x, -(a.x * b.z)), cfma(a.x, b.y, -(a.y * b.x)));
 }
 CDX_HD float csqrt(float x) { return sqrtf(x); }
 CDX_HD double csqrt(double x) { return sqrt(x); }
 CDX_HD float cabs(float x) { return fabsf(x); }
 CDX_HD double cabs(double x) { return fabs(x); }
 template <class R> CDX_HD R length(v3<R> a) { return csqrt(dot(a, a)); }
+// a / |a| as a times one reciprocal of the length.
 template <class R> CDX_HD v3<R> normalize(v3<R> a) {
   R l = length(a);
   if (l == R(0)) return mk3<R>(1, 0, 0);
-  return a / l;
+  return a * (R(1) / l);
 }
 template <class R> CDX_HD R sgn(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : R(0)); }
 template <class R> CDX_HD R eps_of();
@@ -67,29 +75,28 @@ template <class R> CDX_HD q4<R> qmul(q4<R> a, q4<R> b) {
 
 // Rotation matrix of a unit quaternion, row major.
 template <class R> struct mat3 { R a00, a01, a02, a10, a11, a12, a20, a21, a22; };
+// Entries in the "(e0e0+e1e1)*2-1" form, each product pair fused: the
+// doubling is exact, so (s * 2) - 1 is the single rounding of fma(s, 2, -1).
 template <class R> CDX_HD mat3<R> rotmat(q4<R> q) {
-  R e0e0 = q.w * q.w, e1e1 = q.x * q.x, e2e2 = q.y * q.y, e3e3 = q.z * q.z;
-  R e0e1 = q.w * q.x, e0e2 = q.w * q.y, e0e3 = q.w * q.z;
-  R e1e2 = q.x * q.y, e1e3 = q.x * q.z, e2e3 = q.y * q.z;
   mat3<R> m;
-  m.a00 = (e0e0 + e1e1) * R(2) - R(1);
-  m.a01 = (e1e2 - e0e3) * R(2);
-  m.a02 = (e1e3 + e0e2) * R(2);
-  m.a10 = (e1e2 + e0e3) * R(2);
-  m.a11 = (e0e0 + e2e2) * R(2) - R(1);
-  m.a12 = (e2e3 - e0e1) * R(2);
-  m.a20 = (e1e3 - e0e2) * R(2);
-  m.a21 = (e2e3 + e0e1) * R(2);
-  m.a22 = (e0e0 + e3e3) * R(2) - R(1);
+  m.a00 = cfma(cfma(q.w, q.w, q.x * q.x), R(2), R(-1));
+  m.a01 = cfma(q.x, q.y, -(q.w * q.z)) * R(2);
+  m.a02 = cfma(q.x, q.z, q.w * q.y) * R(2);
+  m.a10 = cfma(q.x, q.y, q.w * q.z) * R(2);
+  m.a11 = cfma(cfma(q.w, q.w, q.y * q.y), R(2), R(-1));
+  m.a12 = cfma(q.y, q.z, -(q.w * q.x)) * R(2);
+  m.a20 = cfma(q.x, q.z, -(q.w * q.y)) * R(2);
+  m.a21 = cfma(q.y, q.z, q.w * q.x) * R(2);
+  m.a22 = cfma(cfma(q.w, q.w, q.z * q.z), R(2), R(-1));
   return m;
 }
 template <class R> CDX_HD v3<R> mul(const mat3<R>& m, v3<R> v) {
-  return mk3(m.a00 * v.x + m.a01 * v.y + m.a02 * v.z, m.a10 * v.x + m.a11 * v.y + m.a12 * v.z,
-             m.a20 * v.x + m.a21 * v.y + m.a22 * v.z);
+  return mk3(cfma(m.a02, v.z, cfma(m.a01, v.y, m.a00 * v.x)), cfma(m.a12, v.z, cfma(m.a11, v.y, m.a10 * v.x)),
+             cfma(m.a22, v.z, cfma(m.a21, v.y, m.a20 * v.x)));
 }
 template <class R> CDX_HD v3<R> mulT(const mat3<R>& m, v3<R> v) {
-  return mk3(m.a00 * v.x + m.a10 * v.y + m.a20 * v.z, m.a01 * v.x + m.a11 * v.y + m.a21 * v.z,
-             m.a02 * v.x + m.a12 * v.y + m.a22 * v.z);
+  return mk3(cfma(m.a20, v.z, cfma(m.a10, v.y, m.a00 * v.x)), cfma(m.a21, v.z, cfma(m.a11, v.y, m.a01 * v.x)),
+             cfma(m.a22, v.z, cfma(m.a12, v.y, m.a02 * v.x)));
 }
 template <class R> CDX_HD mat3<R> absm(mat3<R> m) {
   m.a00 = cabs(m.a00); m.a01 = cabs(m.a01); m.a02 = cabs(m.a02);
@@ -106,7 +113,7 @@ template <class R> CDX_HD v3<R> support_local(int type, v3<R> B, v3<R> n) {
     case BOX: return mk3(sgn(n.x) * B.x, sgn(n.y) * B.y, sgn(n.z) * B.z);
     case ELLIPSOID: {
       v3<R> nn = normalize(n);
-      return B * B * nn / length(B * nn);
+      return B * B * nn * (R(1) / length(B * nn));
     }
     case CYLINDER: {
       // Axis y.  w = n - (u.n)u with u = (0,1,0) evaluates exactly to
@@ -140,9 +147,9 @@ template <class R> CDX_HD v3<R> support_world(int type, v3<R> X, v3<R> B, const 
   return support_world_unit(type, X, B, m, normalize(b));
 }
 
-// normalize(-b) and normalize(b) from one square root and one set of
-// divisions: -b has the same squared length, and IEEE division commutes
-// with negation, so normalize(-b) == -normalize(b) bit for bit, except for
+// normalize(-b) and normalize(b) from one square root and one reciprocal:
+// -b has the same squared length, and rounding commutes with negation, so
+// normalize(-b) == -normalize(b) bit for bit, except for
 // b = 0, which normalises to (1, 0, 0) either way.  The MPR always asks for
 // the support of one shape along -b and of the other along b.
 template <class R> CDX_HD void normalize_pm(v3<R> b, v3<R>& neg, v3<R>& pos) {
@@ -151,7 +158,7 @@ template <class R> CDX_HD void normalize_pm(v3<R> b, v3<R>& neg, v3<R>& pos) {
     neg = pos = mk3<R>(1, 0, 0);
     return;
   }
-  pos = b / l;
+  pos = b * (R(1) / l);
   neg = -pos;
 }
 

@@ -46,8 +46,9 @@ namespace orc {
 
 // Operation-counting real for the FLOP census of the narrowphase
 // (oracle_mpr_flops_f64): a double whose +, -, *, / and sqrt each add one
-// to a per-thread counter (negation, fabs and comparisons are free, as on
-// the GPU where they fold into operand modifiers and compares).
+// to a per-thread counter and a fused multiply-add two (negation, fabs and
+// comparisons are free, as on the GPU where they fold into operand
+// modifiers and compares).
 struct Fl {
   double v;
   Fl() = default;
@@ -66,6 +67,7 @@ inline bool operator>=(Fl a, Fl b) { return a.v >= b.v; }
 inline bool operator==(Fl a, Fl b) { return a.v == b.v; }
 inline bool operator!=(Fl a, Fl b) { return a.v != b.v; }
 inline Fl sqrt(Fl a) { ++t_flops; return Fl(std::sqrt(a.v)); }
+inline Fl fma(Fl a, Fl b, Fl c) { t_flops += 2; return Fl(std::fma(a.v, b.v, c.v)); }
 inline Fl fabs(Fl a) { return Fl(std::fabs(a.v)); }
 template <> inline Fl eps_of<Fl>() { return Fl(DBL_EPSILON); }
 
@@ -351,7 +353,7 @@ template <class R> inline r3<R> sup_box(r3<R> B, r3<R> n) {
 // SupportFunctions.cuh:37-50
 template <class R> inline r3<R> sup_ellipsoid(r3<R> B, r3<R> n) {
   r3<R> nn = normalize(n);
-  return B * B * nn / length(B * nn);
+  return B * B * nn * (R(1) / length(B * nn));
 }
 // SupportFunctions.cuh:52-63
 template <class R> inline r3<R> sup_cylinder(r3<R> B, r3<R> n) {

@@ -14,12 +14,16 @@
 //   AMat(q)          = rotation matrix of unit q, entries in the
 //                      "(e0e0+e1e1)*2-1" form.
 //   quatRotate(v,q)  = quatRotateMat(v,q) = AMat(q)*v ; quatRotateMatT = AMat(q)^T*v.
-//   normalize(v)     = v / sqrt(dot(v,v)); zero input returns (1,0,0).
+//   dot, cross, AMat and the matrix-vector products fuse each multiply-add
+//     (one rounding, fixed order; as nvcc contracts the reference's own
+//     device build by default).
+//   normalize(v)     = v * (1 / sqrt(dot(v,v))); zero input returns (1,0,0).
 //   sign(x)          = -1 / 0 / +1.
 //   IsZero(x)        = |x| < FLT_EPSILON (float) or DBL_EPSILON (double);
 //                      IsZero(real3) = all three components.
 // Every expression keeps a fixed evaluation order and the build uses
-// -ffp-contract=off, so results are reproducible to the bit.
+// -ffp-contract=off (only the explicit fma calls fuse), so results are
+// reproducible to the bit.
 #pragma once
 #include <cfloat>
 #include <cmath>
@@ -30,6 +34,7 @@ namespace orc {
 // census (oracle.cpp) finds its own overloads; for float / double these are
 // the std functions.
 using std::fabs;
+using std::fma;
 using std::sqrt;
 
 template <class R> struct r3 { R x, y, z; };
@@ -46,15 +51,17 @@ template <class R> inline r3<R> operator*(r3<R> a, r3<R> b) { return {a.x * b.x,
 template <class R> inline r3<R> operator/(r3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
 template <class R> inline r3<R> operator/(r3<R> a, r3<R> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
 
-template <class R> inline R dot(r3<R> a, r3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// Each multiply-add of dot, cross, AMat and the matrix-vector products is
+// one fused operation (one rounding) in the order written.
+template <class R> inline R dot(r3<R> a, r3<R> b) { return fma(a.z, b.z, fma(a.y, b.y, a.x * b.x)); }
 template <class R> inline r3<R> cross(r3<R> a, r3<R> b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  return {fma(a.y, b.z, -(a.z * b.y)), fma(a.z, b.x, -(a.x * b.z)), fma(a.x, b.y, -(a.y * b.x))};
 }
 template <class R> inline R length(r3<R> a) { return sqrt(dot(a, a)); }
 template <class R> inline r3<R> normalize(r3<R> a) {
   R l = length(a);
   if (l == R(0)) return {R(1), R(0), R(0)};
-  return a / l;
+  return a * (R(1) / l);
 }
 template <class R> inline r3<R> fabs3(r3<R> a) { return {fabs(a.x), fabs(a.y), fabs(a.z)}; }
 template <class R> inline R sign(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : R(0)); }
@@ -75,20 +82,19 @@ template <class R> inline r4<R> mult(r4<R> a, r4<R> b) {
 
 template <class R> struct m33 { R m[3][3]; };
 
+// (e0e0 + e1e1) * 2 - 1 with e0e0 + e1e1 fused; the doubling is exact, so
+// the "- 1" is the single rounding of fma(s, 2, -1).
 template <class R> inline m33<R> AMat(r4<R> q) {
-  R e0e0 = q.w * q.w, e1e1 = q.x * q.x, e2e2 = q.y * q.y, e3e3 = q.z * q.z;
-  R e0e1 = q.w * q.x, e0e2 = q.w * q.y, e0e3 = q.w * q.z;
-  R e1e2 = q.x * q.y, e1e3 = q.x * q.z, e2e3 = q.y * q.z;
   m33<R> r;
-  r.m[0][0] = (e0e0 + e1e1) * R(2) - R(1);
-  r.m[0][1] = (e1e2 - e0e3) * R(2);
-  r.m[0][2] = (e1e3 + e0e2) * R(2);
-  r.m[1][0] = (e1e2 + e0e3) * R(2);
-  r.m[1][1] = (e0e0 + e2e2) * R(2) - R(1);
-  r.m[1][2] = (e2e3 - e0e1) * R(2);
-  r.m[2][0] = (e1e3 - e0e2) * R(2);
-  r.m[2][1] = (e2e3 + e0e1) * R(2);
-  r.m[2][2] = (e0e0 + e3e3) * R(2) - R(1);
+  r.m[0][0] = fma(fma(q.w, q.w, q.x * q.x), R(2), R(-1));
+  r.m[0][1] = fma(q.x, q.y, -(q.w * q.z)) * R(2);
+  r.m[0][2] = fma(q.x, q.z, q.w * q.y) * R(2);
+  r.m[1][0] = fma(q.x, q.y, q.w * q.z) * R(2);
+  r.m[1][1] = fma(fma(q.w, q.w, q.y * q.y), R(2), R(-1));
+  r.m[1][2] = fma(q.y, q.z, -(q.w * q.x)) * R(2);
+  r.m[2][0] = fma(q.x, q.z, -(q.w * q.y)) * R(2);
+  r.m[2][1] = fma(q.y, q.z, q.w * q.x) * R(2);
+  r.m[2][2] = fma(fma(q.w, q.w, q.z * q.z), R(2), R(-1));
   return r;
 }
 template <class R> inline m33<R> AbsMat(m33<R> a) {
@@ -97,14 +103,14 @@ template <class R> inline m33<R> AbsMat(m33<R> a) {
   return a;
 }
 template <class R> inline r3<R> MatMult(const m33<R>& a, r3<R> v) {
-  return {a.m[0][0] * v.x + a.m[0][1] * v.y + a.m[0][2] * v.z,
-          a.m[1][0] * v.x + a.m[1][1] * v.y + a.m[1][2] * v.z,
-          a.m[2][0] * v.x + a.m[2][1] * v.y + a.m[2][2] * v.z};
+  return {fma(a.m[0][2], v.z, fma(a.m[0][1], v.y, a.m[0][0] * v.x)),
+          fma(a.m[1][2], v.z, fma(a.m[1][1], v.y, a.m[1][0] * v.x)),
+          fma(a.m[2][2], v.z, fma(a.m[2][1], v.y, a.m[2][0] * v.x))};
 }
 template <class R> inline r3<R> MatTMult(const m33<R>& a, r3<R> v) {
-  return {a.m[0][0] * v.x + a.m[1][0] * v.y + a.m[2][0] * v.z,
-          a.m[0][1] * v.x + a.m[1][1] * v.y + a.m[2][1] * v.z,
-          a.m[0][2] * v.x + a.m[1][2] * v.y + a.m[2][2] * v.z};
+  return {fma(a.m[2][0], v.z, fma(a.m[1][0], v.y, a.m[0][0] * v.x)),
+          fma(a.m[2][1], v.z, fma(a.m[1][1], v.y, a.m[0][1] * v.x)),
+          fma(a.m[2][2], v.z, fma(a.m[1][2], v.y, a.m[0][2] * v.x))};
 }
 template <class R> inline r3<R> quatRotate(r3<R> v, r4<R> q) { return MatMult(AMat(q), v); }
 template <class R> inline r3<R> quatRotateMat(r3<R> v, r4<R> q) { return MatMult(AMat(q), v); }

@@ -569,14 +569,14 @@ def test_swap_symmetry(oracle, scenes):
 
 def test_flop_census_sphere_pair_closed_form(oracle, scenes):
     """The FLOP census counts each +, -, *, / and sqrt of function_MPR_Store
-    after placement: a touching sphere pair (Narrowphase.cu:382-396, then the
-    envelope and depth :412-415) is 51 operations, a separated one 10, and a
-    pair of one body 0."""
+    after placement (a fused multiply-add as two): a touching sphere pair
+    (Narrowphase.cu:382-396, then the envelope and depth :412-415) is 52
+    operations, a separated one 10, and a pair of one body 0."""
     pair = np.array([(0 << 32) | 1], np.int64)
     hit = _pair_scene(scenes, SPHERE, SPHERE, (0.7, 0.1, 0.0))
     miss = _pair_scene(scenes, SPHERE, SPHERE, (1.7, 0.1, 0.0))
     assert oracle.narrowphase(hit, pair).count == 1 and oracle.narrowphase(miss, pair).count == 0
-    assert oracle.mpr_flops(hit, pair)[0] == 51
+    assert oracle.mpr_flops(hit, pair)[0] == 52
     assert oracle.mpr_flops(miss, pair)[0] == 10
     same = _pair_scene(scenes, SPHERE, BOX, (0.7, 0.1, 0.0))
     same.ID[:] = 0
