@@ -64,12 +64,12 @@ void Autotuner::grid_for(const double lo[3], const double hi[3], double cell, ui
   }
 }
 
-TuneChoice Autotuner::choose(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R,
+TuneChoice Autotuner::choose(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* R,
                              const uint32_t* ID, uint64_t nb, const void* pos, const void* rot,
                              const uint8_t* active) {
   CDX_REQUIRE(n > 0, ERR_INVALID, "a scene needs shapes");
   std::unique_ptr<SceneBase> s = make_scene(kind_, dev_);
-  s->set_shapes(n, T, A, B, R, ID);
+  s->set_shapes(n, T, A, B, C, R, ID);
   s->set_bodies(nb, pos, rot, active);
   s->run_aabb();
   s->sync();

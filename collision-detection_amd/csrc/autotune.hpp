@@ -30,8 +30,8 @@ class Autotuner {
   // computed on the GPU; the first scene of a size class is measured: the
   // broadphase runs at each candidate cell factor and the fastest wins; the
   // class's later scenes reuse that factor with their own bounds.
-  TuneChoice choose(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R, const uint32_t* ID,
-                    uint64_t nb, const void* pos, const void* rot, const uint8_t* active);
+  TuneChoice choose(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* R,
+                    const uint32_t* ID, uint64_t nb, const void* pos, const void* rot, const uint8_t* active);
 
  private:
   struct ClassInfo {

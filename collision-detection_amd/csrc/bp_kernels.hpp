@@ -61,7 +61,8 @@ template <class R>
 __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                 uint64_t NS, const uint8_t* __restrict__ type,
                                                 const uint32_t* __restrict__ body, const R* __restrict__ lpos,
-                                                const R* __restrict__ dims, const R* __restrict__ lrot, uint64_t NB,
+                                                const R* __restrict__ dims, const R* __restrict__ cpos,
+                                                const R* __restrict__ lrot, uint64_t NB,
                                                 const R* __restrict__ bpos, const R* __restrict__ brot,
                                                 const uint8_t* __restrict__ active, R* __restrict__ aabb,
                                                 ShapeRec<R>* __restrict__ rec, ShapeHdr* __restrict__ hdr,
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
     uint32_t id = body[i];
     if (id < desc[ch.scene].body_begin || id >= desc[ch.scene].body_end) {
       atomicOr(&st->bad, 2u);
-    } else if (!(t == SPHERE || t == ELLIPSOID || t == BOX || t == CYLINDER || t == CAPSULE)) {
+    } else if (t < SPHERE || t > CAPSULE) {
       atomicOr(&st->bad, 1u);
     } else {
       v3<R> a = mk3(lpos[i], lpos[NS + i], lpos[2 * NS + i]);
@@ -88,14 +89,40 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
       q4<R> lr; lr.w = lrot[i]; lr.x = lrot[NS + i]; lr.y = lrot[2 * NS + i]; lr.z = lrot[3 * NS + i];
       v3<R> p = mk3(bpos[id], bpos[NB + id], bpos[2 * NB + id]);
       q4<R> rq; rq.w = brot[id]; rq.x = brot[NB + id]; rq.y = brot[2 * NB + id]; rq.z = brot[3 * NB + id];
-      v3<R> c = rotate(a, rq) + p;
       q4<R> q1 = qmul(rq, lr);
-      v3<R> lo, hi;
+      v3<R> lo, hi, c;
+      // the narrowphase record: centre, rotation, dimensions; a triangle
+      // keeps its three world vertices there instead (x,y,z | bx,by,bz |
+      // qw,qx,qy), the narrowphase's support of it
+      v3<R> rb = b;
+      q4<R> rq1 = q1;
       if (t == SPHERE) {
+        c = rotate(a, rq) + p;
         v3<R> r3 = mk3(b.x, b.x, b.x);
         lo = c - r3;
         hi = c + r3;
+      } else if (t == TRIANGLEMESH) {
+        // function_MPR_Store's placement (Narrowphase.cu:363-367) and the
+        // reference's triangle bounds (AABBGenerator.cu:16-23, :103-107)
+        const v3<R> cc = mk3(cpos[i], cpos[NS + i], cpos[2 * NS + i]);
+        const v3<R> va = rotate(a + p, q1), vb = rotate(b + p, q1), vc = rotate(cc + p, q1);
+        lo = mk3(min(va.x, min(vb.x, vc.x)), min(va.y, min(vb.y, vc.y)), min(va.z, min(vb.z, vc.z)));
+        hi = mk3(max(va.x, max(vb.x, vc.x)), max(va.y, max(vb.y, vc.y)), max(va.z, max(vb.z, vc.z)));
+        c = va;
+        rb = vb;
+        rq1.w = vc.x; rq1.x = vc.y; rq1.y = vc.z; rq1.z = R(0);
+      } else if (t == RECT || t == CONE) {
+        // placed at the local position, rotated by body (x) local, as
+        // function_MPR_Store does for these types (it applies no body pose);
+        // bounded by the rectangle at height B.z / the cone's (r, h/2, r) box
+        c = a;
+        const mat3<R> m = rotmat(q1);
+        const v3<R> ctr = t == RECT ? a + mul(m, mk3(R(0), R(0), b.z)) : a;
+        const v3<R> h = mul(absm(m), t == RECT ? mk3(b.x, b.y, R(0)) : mk3(b.x, b.y * R(.5), b.x));
+        lo = ctr - h;
+        hi = ctr + h;
       } else {
+        c = rotate(a, rq) + p;
         // Loose box of the semi-extents for ellipsoid and cylinder, as the
         // reference does; the capsule's is (r, h + r, r).
         v3<R> d = (t == CAPSULE) ? mk3(b.x, b.y + b.x, b.z) : b;
@@ -107,8 +134,8 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
       aabb[NS + i] = hi.x; aabb[3 * NS + i] = hi.y; aabb[5 * NS + i] = hi.z;
       ShapeRec<R> r;
       r.x = c.x; r.y = c.y; r.z = c.z;
-      r.qw = q1.w; r.qx = q1.x; r.qy = q1.y; r.qz = q1.z;
-      r.bx = b.x; r.by = b.y; r.bz = b.z;
+      r.qw = rq1.w; r.qx = rq1.x; r.qy = rq1.y; r.qz = rq1.z;
+      r.bx = rb.x; r.by = rb.y; r.bz = rb.z;
       r.type = (uint32_t)t; r.body = id; r.active = active[id]; r.pad = 0;
       lrec[threadIdx.x] = r;
       ShapeHdr h;

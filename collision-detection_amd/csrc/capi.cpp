@@ -143,11 +143,10 @@ int cdx_generate_aabb(cdx_ctx* ctx, int real_kind, uint64_t n, const int32_t* T,
   if (!ctx) return CDX_ERR_INVALID;
   return guarded(&ctx->err, [&] {
     check_kind(real_kind);
-    (void)C;
     if (n == 0) return;
     CDX_REQUIRE(T && A && B && R && ID && body_pos && body_rot && aabb_out, CDX_ERR_INVALID, "null input");
     cdx::SceneBase& s = lazy(ctx->aabb_scene[real_kind], real_kind, ctx->device);
-    s.set_shapes(n, T, A, B, R, ID);
+    s.set_shapes(n, T, A, B, C, R, ID);
     s.set_bodies(nb, body_pos, body_rot, nullptr);
     s.run_aabb();
     s.sync();
@@ -205,13 +204,12 @@ int cdx_do_narrowphase(cdx_ctx* ctx, int real_kind, int np_mode, double envelope
   if (!ctx) return CDX_ERR_INVALID;
   return guarded(&ctx->err, [&] {
     check_kind(real_kind);
-    (void)C;
     CDX_REQUIRE(n_contacts, CDX_ERR_INVALID, "null n_contacts");
     cdx::SceneBase& s = lazy(ctx->np_scene[real_kind], real_kind, ctx->device);
     ctx->last_np_kind = real_kind;
     s.set_np_mode(np_mode, 1e-2);
     s.set_envelope(envelope);
-    s.set_shapes(n, T, A, B, R, ID);
+    s.set_shapes(n, T, A, B, C, R, ID);
     s.set_bodies(nb, body_pos, body_rot, obj_active);
     s.set_pairs(n_pairs, pairs_inout);
     s.run_narrowphase();
@@ -229,14 +227,13 @@ int cdx_mpr_store(cdx_ctx* ctx, int real_kind, int np_mode, double envelope, uin
   if (!ctx) return CDX_ERR_INVALID;
   return guarded(&ctx->err, [&] {
     check_kind(real_kind);
-    (void)C;
     if (n_pairs == 0) return;
     CDX_REQUIRE(pairs, CDX_ERR_INVALID, "null pairs");
     cdx::SceneBase& s = lazy(ctx->np_scene[real_kind], real_kind, ctx->device);
     ctx->last_np_kind = real_kind;
     s.set_np_mode(np_mode, 1e-2);
     s.set_envelope(envelope);
-    s.set_shapes(n, T, A, B, R, ID);
+    s.set_shapes(n, T, A, B, C, R, ID);
     s.set_bodies(nb, body_pos, body_rot, obj_active);
     s.set_pairs(n_pairs, pairs);
     s.run_narrowphase();
@@ -309,9 +306,8 @@ const char* cdx_scene_last_error(const cdx_scene* scene) { return scene ? scene-
 
 int cdx_scene_set_shapes(cdx_scene* scene, uint64_t n, const int32_t* T, const void* A, const void* B, const void* C,
                          const void* R, const uint32_t* ID) {
-  (void)C;
   SCENE_CALL(CDX_REQUIRE(n == 0 || (T && A && B && R && ID), CDX_ERR_INVALID, "null input");
-             s.set_shapes(n, T, A, B, R, ID));
+             s.set_shapes(n, T, A, B, C, R, ID));
 }
 int cdx_scene_set_bodies(cdx_scene* scene, uint64_t nb, const void* pos, const void* rot, const uint8_t* active) {
   SCENE_CALL(CDX_REQUIRE(nb == 0 || (pos && rot), CDX_ERR_INVALID, "null input"); s.set_bodies(nb, pos, rot, active));
@@ -394,14 +390,13 @@ const char* cdx_batch_last_error(const cdx_batch* batch) { return batch ? batch-
 int cdx_batch_add_scene(cdx_batch* batch, uint64_t n, const int32_t* T, const void* A, const void* B, const void* C,
                         const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos, const void* body_rot,
                         const uint8_t* obj_active, int bins_x, int bins_y, int bins_z, uint32_t* scene_out) {
-  (void)C;
   BATCH_CALL({
     CDX_REQUIRE(n > 0 && nb > 0, CDX_ERR_INVALID, "a scene needs shapes and bodies");
     CDX_REQUIRE(T && A && B && R && ID && body_pos && body_rot, CDX_ERR_INVALID, "null input");
     int g[3] = {bins_x, bins_y, bins_z};
     if (bins_x == 0 && bins_y == 0 && bins_z == 0) {
       if (!batch->tuner) batch->tuner = std::make_unique<cdx::Autotuner>(batch->ctx->device, batch->kind);
-      cdx::TuneChoice t = batch->tuner->choose(n, T, A, B, R, ID, nb, body_pos, body_rot, obj_active);
+      cdx::TuneChoice t = batch->tuner->choose(n, T, A, B, C, R, ID, nb, body_pos, body_rot, obj_active);
       for (int k = 0; k < 3; ++k) g[k] = t.grid[k];
       if (t.m_per_shape > batch->m_per_shape || t.p_per_shape > batch->p_per_shape) {
         batch->m_per_shape = std::max(batch->m_per_shape, t.m_per_shape);
@@ -409,7 +404,7 @@ int cdx_batch_add_scene(cdx_batch* batch, uint64_t n, const int32_t* T, const vo
         b.set_capacity_factors(batch->m_per_shape * 1.25 + 1.0, batch->p_per_shape * 1.25 + 1.0);
       }
     }
-    uint32_t s = b.add_scene(n, T, A, B, R, ID, nb, body_pos, body_rot, obj_active, g[0], g[1], g[2]);
+    uint32_t s = b.add_scene(n, T, A, B, C, R, ID, nb, body_pos, body_rot, obj_active, g[0], g[1], g[2]);
     if (batch->bins.size() <= s) batch->bins.resize(s + 1);
     batch->bins[s] = {g[0], g[1], g[2]};
     if (scene_out) *scene_out = s;
@@ -507,9 +502,8 @@ int cdx_server_add_scene(cdx_server* server, uint64_t n, const int32_t* T, const
                          const void* C, const void* R, const uint32_t* ID, uint64_t nb, const void* body_pos,
                          const void* body_rot, const uint8_t* obj_active, int bins_x, int bins_y, int bins_z,
                          uint32_t* scene_out) {
-  (void)C;
   SERVER_CALL(CDX_REQUIRE(T && A && B && R && ID && body_pos && body_rot, CDX_ERR_INVALID, "null input");
-              uint32_t s = sv.add_scene(n, T, A, B, R, ID, nb, body_pos, body_rot, obj_active, bins_x, bins_y,
+              uint32_t s = sv.add_scene(n, T, A, B, C, R, ID, nb, body_pos, body_rot, obj_active, bins_x, bins_y,
                                         bins_z);
               if (scene_out) *scene_out = s);
 }

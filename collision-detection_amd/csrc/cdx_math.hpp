@@ -106,8 +106,11 @@ template <class R> CDX_HD mat3<R> absm(mat3<R> m) {
 }
 template <class R> CDX_HD v3<R> rotate(v3<R> v, q4<R> q) { return mul(rotmat(q), v); }
 
-// Local-frame support maps (SupportFunctions.cuh:11-63), n of unit length.
-template <class R> CDX_HD v3<R> support_local(int type, v3<R> B, v3<R> n) {
+// Local-frame support maps (SupportFunctions.cuh:11-99), n of unit length.
+// EXT compiles in the rectangle and the cone; the kernels that never meet
+// them leave them out (a kernel's register allocation covers every case it
+// holds).
+template <class R, bool EXT = true> CDX_HD v3<R> support_local(int type, v3<R> B, v3<R> n) {
   switch (type) {
     case SPHERE: return B.x * n;
     case BOX: return mk3(sgn(n.x) * B.x, sgn(n.y) * B.y, sgn(n.z) * B.z);
@@ -133,15 +136,44 @@ template <class R> CDX_HD v3<R> support_local(int type, v3<R> B, v3<R> n) {
       R s = sgn(n.y) * B.y;
       return mk3(R(0) * s + B.x * nn.x, s + B.x * nn.y, R(0) * s + B.x * nn.z);
     }
+    case RECT:  // SupportFunctions.cuh:65-73: |x| <= B.x, |y| <= B.y at z = B.z
+      if constexpr (!EXT) return mk3<R>(0, 0, 0);
+      return mk3(n.x < R(0) ? -B.x : B.x, n.y < R(0) ? -B.y : B.y, B.z);
+    case CONE: {  // SupportFunctions.cuh:74-99: radius B.x, height B.y, apex at +B.y/2
+      if constexpr (!EXT) return mk3<R>(0, 0, 0);
+      const R radius = B.x, height = B.y;
+      const R sin_angle = radius / csqrt(radius * radius + height * height);
+      if (n.y > length(n) * sin_angle) return mk3(R(0), height / R(2.0), R(0));
+      const R s = csqrt(n.x * n.x + n.z * n.z);
+      if ((double)s > 1e-9) {  // the reference's double literal: compared in double
+        const R d = radius / s;
+        return mk3(n.x * d, -height / R(2.0), n.z * d);
+      }
+      return mk3(R(0), -height / R(2.0), R(0));
+    }
     default: return mk3<R>(0, 0, 0);
   }
 }
 
+// SupportFunctions.cuh:14-29: the vertex furthest along n, the first of
+// equals (world-placed vertices; TRIANGLEMESH is not rotated per query).
+template <class R> CDX_HD v3<R> support_triangle(v3<R> A, v3<R> B, v3<R> C, v3<R> n) {
+  R dist = dot(A, n);
+  v3<R> point = A;
+  if (dot(B, n) > dist) {
+    dist = dot(B, n);
+    point = B;
+  }
+  if (dot(C, n) > dist) point = C;
+  return point;
+}
+
 // World support of a placed shape along b (Narrowphase.cu:19-52):
 // normalize, rotate into the shape frame, support, rotate back, translate.
-template <class R> CDX_HD v3<R> support_world_unit(int type, v3<R> X, v3<R> B, const mat3<R>& m, v3<R> n) {
+template <class R, bool EXT = true>
+CDX_HD v3<R> support_world_unit(int type, v3<R> X, v3<R> B, const mat3<R>& m, v3<R> n) {
   v3<R> rn = mulT(m, n);
-  return mul(m, support_local(type, B, rn)) + X;
+  return mul(m, support_local<R, EXT>(type, B, rn)) + X;
 }
 template <class R> CDX_HD v3<R> support_world(int type, v3<R> X, v3<R> B, const mat3<R>& m, v3<R> b) {
   return support_world_unit(type, X, B, m, normalize(b));

@@ -48,7 +48,8 @@ class SceneBase {
   virtual int real_kind() const = 0;
 
   // Inputs (host pointers, reference AoS layouts).
-  virtual void set_shapes(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R,
+  // C (obj_data_C, the third triangle vertex) may be null: zeros.
+  virtual void set_shapes(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* R,
                           const uint32_t* ID) = 0;
   virtual void set_bodies(uint64_t nb, const void* pos, const void* rot, const uint8_t* active) = 0;
   virtual void update_bodies(uint64_t k, const uint32_t* idx, const void* pos, const void* rot) = 0;
@@ -96,8 +97,8 @@ class BatchBase {
   virtual int real_kind() const = 0;
   // Appends a scene (host pointers, reference layouts; body ids local to the
   // scene).  Returns its index.
-  virtual uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R,
-                             const uint32_t* ID, uint64_t nb, const void* pos, const void* rot,
+  virtual uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C,
+                             const void* R, const uint32_t* ID, uint64_t nb, const void* pos, const void* rot,
                              const uint8_t* active, int gx, int gy, int gz) = 0;
   virtual uint64_t scene_count() const = 0;
   virtual void update_scene_bodies(uint32_t scene, uint64_t k, const uint32_t* idx, const void* pos,

@@ -12,23 +12,37 @@ namespace {
 // at every support call: the same rounded values each time, with fewer live
 // registers than a cached 3x3 per shape (caching both matrices per pair
 // measured no faster: the kernel stays at 2 waves per SIMD either way).
+// A triangle holds its three world vertices in X, B and q.w/x/y instead.
 template <class R> struct Placed {
   int type;
   v3<R> X, B;
   q4<R> q;
 };
 
-template <class R> __device__ __forceinline__ v3<R> sup(const Placed<R>& s, v3<R> b) {
-  return support_world(s.type, s.X, s.B, rotmat(s.q), b);
+// World support along a unit direction (TransformSupportVert,
+// Narrowphase.cu:19-52).
+// EXT: rectangles, cones and triangles possible (see support_local).
+template <bool EXT, class R> __device__ __forceinline__ v3<R> support_placed(const Placed<R>& s, v3<R> n) {
+  if constexpr (EXT)
+    if (s.type == TRIANGLEMESH) return support_triangle(s.X, s.B, mk3(s.q.w, s.q.x, s.q.y), n);
+  return support_world_unit<R, EXT>(s.type, s.X, s.B, rotmat(s.q), n);
+}
+
+// GetCenter (Narrowphase.cu:11-17): the triangle's centroid
+// (SupportFunctions.cuh:119-121), else 0 + the centre.
+template <bool EXT, class R> __device__ __forceinline__ v3<R> center_of(const Placed<R>& s) {
+  if (EXT && s.type == TRIANGLEMESH)
+    return mk3((s.X.x + s.B.x + s.q.w) / R(3.0f), (s.X.y + s.B.y + s.q.x) / R(3.0f), (s.X.z + s.B.z + s.q.y) / R(3.0f));
+  return mk3<R>(0, 0, 0) + s.X;
 }
 
 // sa = sup(A, -n), sb = sup(B, n) with one normalisation of n (normalize_pm).
-template <class R>
+template <bool EXT, class R>
 __device__ __forceinline__ void sup_pair(const Placed<R>& A, const Placed<R>& B, v3<R> n, v3<R>& sa, v3<R>& sb) {
   v3<R> na, nb;
   normalize_pm(n, na, nb);
-  sa = support_world_unit(A.type, A.X, A.B, rotmat(A.q), na);
-  sb = support_world_unit(B.type, B.X, B.B, rotmat(B.q), nb);
+  sa = support_placed<EXT>(A, na);
+  sb = support_placed<EXT>(B, nb);
 }
 
 template <class R> __device__ __forceinline__ Placed<R> placed(const ShapeRec<R>& r) {
@@ -64,15 +78,15 @@ __device__ __forceinline__ void sup_fast_pair(const PlacedF& A, const PlacedF& B
 template <class R> __device__ __forceinline__ void swap3(v3<R>& a, v3<R>& b) { v3<R> t = a; a = b; b = t; }
 
 // CollideAndFindPoint (Narrowphase.cu:122-314), same evaluation order.
-template <class R>
+template <bool EXT, class R>
 __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rnormal, v3<R>& point, R& depth) {
   v3<R> v01, v02, v0, n, v11, v12, v1, v21, v22, v2;
-  v01 = mk3<R>(0, 0, 0) + A.X;
-  v02 = mk3<R>(0, 0, 0) + B.X;
+  v01 = center_of<EXT>(A);
+  v02 = center_of<EXT>(B);
   v0 = v02 - v01;
   if (is_zero(v0)) v0 = mk3<R>(1, 0, 0);
   n = -v0;
-  sup_pair(A, B, n, v11, v12);
+  sup_pair<EXT>(A, B, n, v11, v12);
   v1 = v12 - v11;
   if (dot(v1, n) <= R(0)) return false;
   n = cross(v1, v0);
@@ -84,7 +98,7 @@ __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rno
     depth = dot((v12 - v11), n);
     return true;
   }
-  sup_pair(A, B, n, v21, v22);
+  sup_pair<EXT>(A, B, n, v21, v22);
   v2 = v22 - v21;
   if (dot(v2, n) <= R(0)) return false;
   n = cross((v1 - v0), (v2 - v0));
@@ -101,7 +115,7 @@ __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rno
   while (true) {
     if (phase1 > max_iterations) return false;
     phase1++;
-    sup_pair(A, B, n, v31, v32);
+    sup_pair<EXT>(A, B, n, v31, v32);
     v3_ = v32 - v31;
     if (dot(v3_, n) <= R(0)) return false;
     if (dot(cross(v1, v3_), v0) < R(0)) {
@@ -120,7 +134,7 @@ __device__ bool mpr_reference(const Placed<R>& A, const Placed<R>& B, v3<R>& rno
       n = normalize(n);
       if (dot(n, v1) >= R(0) && !hit) hit = true;
       v3<R> v41, v42;
-      sup_pair(A, B, n, v41, v42);
+      sup_pair<EXT>(A, B, n, v41, v42);
       v3<R> v4 = v42 - v41;
       R delta = dot((v4 - v3_), n);
       depth = dot(v4, n);
@@ -188,10 +202,11 @@ __device__ __forceinline__ bool contact_spheres(const ShapeRec<R>& ra, const Sha
   return contact_spheres(mk3(ra.x, ra.y, ra.z), ra.bx, mk3(rb.x, rb.y, rb.z), rb.bx, envelope, out);
 }
 
-template <class R>
+template <bool EXT, class R>
 __device__ bool contact_native(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R envelope, Contact<R>& out) {
   if (!ra.active && !rb.active) return false;
   if (ra.body == rb.body) return false;
+  if (EXT && (ra.type == TRIANGLEMESH || rb.type == TRIANGLEMESH)) envelope = 0;  // Narrowphase.cu:364, :373
   v3<R> N = mk3<R>(1, 0, 0), p1 = mk3<R>(0, 0, 0), p2 = mk3<R>(0, 0, 0);
   if (ra.type == SPHERE && rb.type == SPHERE) {
     return contact_spheres(ra, rb, envelope, out);
@@ -199,9 +214,9 @@ __device__ bool contact_native(const ShapeRec<R>& ra, const ShapeRec<R>& rb, R e
     Placed<R> A = placed(ra), B = placed(rb);
     v3<R> p0 = mk3<R>(0, 0, 0);
     R depth = 0;
-    if (!mpr_reference(A, B, N, p0, depth)) return false;
+    if (!mpr_reference<EXT>(A, B, N, p0, depth)) return false;
     v3<R> sa, sb;
-    sup_pair(A, B, N, sa, sb);
+    sup_pair<EXT>(A, B, N, sa, sb);
     p1 = dot((sa - p0), N) * N + p0;
     p2 = dot((sb - p0), N) * N + p0;
     N = -N;
@@ -382,8 +397,20 @@ __device__ __forceinline__ PlacedF placed_local(const ShapeRecF& r, const ShapeR
   return s;
 }
 
-__device__ __forceinline__ uint32_t type_slot(uint32_t t) { return t == CAPSULE ? 4u : t; }
-constexpr int CLASS_BITS = 8;  // sort digit of the queue class (type_slot(a) * 5 + type_slot(b) < 25)
+// Queue class slot of a shape type: the FP32-filtered types first
+// (sphere, ellipsoid, box, cylinder, capsule), then rectangle, cone and
+// triangle, which go to the reference-order MPR directly.
+__device__ __forceinline__ uint32_t type_slot(uint32_t t) { return t == CAPSULE ? 4u : (t >= RECT && t <= TRIANGLEMESH ? t + 1u : t); }
+__device__ __forceinline__ bool fp32_filtered(uint32_t t) { return t <= CYLINDER || t == CAPSULE; }
+// Queue class of a pair: type_slot(a) * 8 + type_slot(b), plus EXT_CLASS
+// when a rectangle, cone or triangle takes part, so the sorted queue keeps
+// those pairs at its end for the resolve kernel that supports them.
+constexpr uint32_t EXT_CLASS = 64;
+constexpr int CLASS_BITS = 8;  // sort digit of the queue class (< 128)
+__device__ __forceinline__ uint32_t pair_class(uint32_t ta, uint32_t tb) {
+  const uint32_t c = type_slot(ta) * 8u + type_slot(tb);
+  return fp32_filtered(ta) && fp32_filtered(tb) ? c : EXT_CLASS + c;
+}
 
 // The first portal test of mpr_guarded on its own (same operations, so the
 // same verdict): true when the pair is a clear miss along the centre line.
@@ -464,6 +491,8 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
         Contact<R> c;
         if (contact_spheres(mk3((R)fa.x, (R)fa.y, (R)fa.z), rad_a, mk3((R)fb.x, (R)fb.y, (R)fb.z), rad_b, envelope, c))
           store_pair(o, P, i, c, fa.body, fb.body);
+      } else if (!fp32_filtered(ta) || !fp32_filtered(tb)) {
+        enq = true;  // no FP32 model of rectangles, cones and triangles: FP64 decides
       } else {
         const PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
         const float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
@@ -480,7 +509,7 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
     }
   }
   qsel[i] = enq ? 1u : 0u;
-  qcls[i] = (uint8_t)(type_slot(ta) * 5u + type_slot(tb));
+  qcls[i] = (uint8_t)pair_class(ta, tb);
 }
 
 // FP32 MPR over the class-sorted queue (MIXED / FAST), in the frame of
@@ -500,6 +529,10 @@ __global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t*
   const uint64_t pr = pairs[i];
   const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
   const ShapeRecF fa = frec[ia], fb = frec[ib];
+  if (!fp32_filtered(fa.type) || !fp32_filtered(fb.type)) {
+    sel[j] = 1u;  // rectangles, cones and triangles: the reference-order MPR decides
+    return;
+  }
   PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
   float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
   v3<float> N, p0;
@@ -554,28 +587,51 @@ __global__ void k_queue_compact(uint64_t cap, const uint64_t* __restrict__ d_np,
 
 // MPR on the reference path for the queued pairs, in compute type CT
 // (the scene's real type for NATIVE, FP64 for MIXED / FAST).
-template <class R, class CT>
+// EXT: the launch for the pairs of class EXT_CLASS and above (a rectangle,
+// cone or triangle takes part); the other launch takes the rest, one entry
+// per thread, skipping the extended ones, which the class sort keeps at the
+// end of the queue.
+template <class R, class CT, bool EXT>
 __global__ __launch_bounds__(128) void k_np_resolve(uint64_t cap, const uint64_t* __restrict__ d_q,
                                                     const uint64_t* __restrict__ pairs,
                                                     const ShapeRec<R>* __restrict__ rec, R envelope, PairOut<R> o,
                                                     const uint64_t* __restrict__ queue) {
-  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
-  if (j >= dev_count(cap, d_q)) return;
-  uint64_t P = o.P;
-  uint32_t i = (uint32_t)queue[j];
-  uint64_t pr = pairs[i];
-  ShapeRec<CT> ra = as_compute<CT, R>(rec[(uint32_t)(pr >> 32)]);
-  ShapeRec<CT> rb = as_compute<CT, R>(rec[(uint32_t)(pr & 0xffffffffu)]);
-  Contact<CT> c;
-  if (contact_native(ra, rb, (CT)envelope, c)) {
-    Contact<R> cr;
-    cr.N = mk3((R)c.N.x, (R)c.N.y, (R)c.N.z);
-    cr.p1 = mk3((R)c.p1.x, (R)c.p1.y, (R)c.p1.z);
-    cr.p2 = mk3((R)c.p2.x, (R)c.p2.y, (R)c.p2.z);
-    cr.depth = (R)c.depth;
-    store_pair(o, P, i, cr, ra.body, rb.body);
+  const uint64_t q = dev_count(cap, d_q);
+  const uint64_t P = o.P;
+  auto one = [&](uint64_t e) {
+    const uint32_t i = (uint32_t)e;
+    const uint64_t pr = pairs[i];
+    ShapeRec<CT> ra = as_compute<CT, R>(rec[(uint32_t)(pr >> 32)]);
+    ShapeRec<CT> rb = as_compute<CT, R>(rec[(uint32_t)(pr & 0xffffffffu)]);
+    Contact<CT> c;
+    if (contact_native<EXT>(ra, rb, (CT)envelope, c)) {
+      Contact<R> cr;
+      cr.N = mk3((R)c.N.x, (R)c.N.y, (R)c.N.z);
+      cr.p1 = mk3((R)c.p1.x, (R)c.p1.y, (R)c.p1.z);
+      cr.p2 = mk3((R)c.p2.x, (R)c.p2.y, (R)c.p2.z);
+      cr.depth = (R)c.depth;
+      store_pair(o, P, i, cr, ra.body, rb.body);
+    }
+  };
+  const uint64_t t = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if constexpr (!EXT) {
+    if (t >= q) return;
+    const uint64_t e = queue[t];
+    if ((uint32_t)(e >> 32) < EXT_CLASS) one(e);
+  } else {
+    // the extended classes sort last: walk the queue from its end and stop
+    // at the first entry of a filtered class
+    const uint64_t stride = (uint64_t)gridDim.x * 128;
+    for (uint64_t k = t; k < q; k += stride) {
+      const uint64_t e = queue[q - 1 - k];
+      if ((uint32_t)(e >> 32) < EXT_CLASS) return;
+      one(e);
+    }
   }
 }
+
+// Grid of the extended-class resolve launch (it strides over its entries).
+constexpr unsigned EXT_RESOLVE_GRID = 1024;
 
 // Stable partition of hits to the front (Narrowphase.cu:580-593): contacts
 // in ascending pair order, and the pair list permuted hits first.  A

@@ -200,14 +200,14 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   // ---------------------------------------------------- single scene API --
 
-  void set_shapes(uint64_t n, const int32_t* T, const void* A, const void* B, const void* Rq,
+  void set_shapes(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* Rq,
                   const uint32_t* ID) override {
     check_types(n, T);
     sync_internal();
     single_ensure_scene0();
     ensure_shape_capacity(n);
     scenes_[0].n_shapes = n;
-    upload_shapes(0, n, T, A, B, Rq, ID, 0);
+    upload_shapes(0, n, T, A, B, C, Rq, ID, 0);
     shapes_set_ = true;
     aabb_current_ = false;
     records_valid_ = false;
@@ -427,7 +427,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   }
 
   uint64_t device_bytes() const override {
-    const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
+    const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &cpos_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
@@ -439,7 +439,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   // ----------------------------------------------------------- batch API --
 
-  uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* Rq, const uint32_t* ID,
+  uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* Rq,
+                     const uint32_t* ID,
                      uint64_t nb, const void* pos, const void* rot, const uint8_t* active, int gx, int gy,
                      int gz) override {
     CDX_REQUIRE(gx > 0 && gy > 0 && gz > 0, ERR_INVALID, "bins per axis must be positive");
@@ -463,7 +464,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     ensure_body_capacity(sc.body_begin + nb);
     scenes_.push_back(sc);
     const uint32_t s = (uint32_t)(scenes_.size() - 1);
-    upload_shapes(sc.shape_begin, n, T, A, B, Rq, ID, sc.body_begin);
+    upload_shapes(sc.shape_begin, n, T, A, B, C, Rq, ID, sc.body_begin);
     upload_bodies(sc.body_begin, nb, pos, rot, active);
     update_desc(s);
     shapes_set_ = bodies_set_ = true;
@@ -680,7 +681,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   uint64_t resident_bytes_estimate(uint64_t n, uint64_t nb) const override {
     // inputs in SoA plus the per-shape records each step rewrites
-    const uint64_t per_shape = 1 + 4 + 10 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
+    const uint64_t per_shape = 1 + 4 + 13 * sizeof(R) + 6 * sizeof(R) + sizeof(ShapeRec<R>) + sizeof(BpRec<R>) +
                                sizeof(ShapeHdr) + sizeof(ShapeRecF);
     const uint64_t per_body = 7 * sizeof(R) + 1;
     return n * per_shape + nb * per_body;
@@ -704,9 +705,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     CDX_REQUIRE(n < (1ull << 31), ERR_INVALID, "too many shapes (limit 2^31 - 1)");
     for (uint64_t i = 0; i < n; ++i) {
       int t = T[i];
-      CDX_REQUIRE(t == SPHERE || t == ELLIPSOID || t == BOX || t == CYLINDER || t == CAPSULE, ERR_UNSUPPORTED_SHAPE,
+      CDX_REQUIRE(t >= SPHERE && t <= CAPSULE, ERR_UNSUPPORTED_SHAPE,
                   "shape " + std::to_string(i) + " has type " + std::to_string(t) +
-                      ": only SPHERE, ELLIPSOID, BOX, CYLINDER and CAPSULE are supported");
+                      ": shape types are SPHERE..TRIANGLEMESH (0-6) and CAPSULE (7)");
     }
   }
 
@@ -736,6 +737,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     grow_soa(body_, 4, 1, nc, keep);
     grow_soa(lpos_, sizeof(R), 3, nc, keep);
     grow_soa(dims_, sizeof(R), 3, nc, keep);
+    grow_soa(cpos_, sizeof(R), 3, nc, keep);
     grow_soa(lrot_, sizeof(R), 4, nc, keep);
     aabb_.release();
     aabb_.reserve(nc * 6 * sizeof(R));
@@ -783,8 +785,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     CDX_HIP_CHECK(hipStreamSynchronize(s_));
   }
 
-  void upload_shapes(uint64_t at, uint64_t n, const int32_t* T, const void* A, const void* B, const void* Rq,
-                     const uint32_t* ID, uint64_t body_base) {
+  void upload_shapes(uint64_t at, uint64_t n, const int32_t* T, const void* A, const void* B, const void* C,
+                     const void* Rq, const uint32_t* ID, uint64_t body_base) {
     if (!n) return;
     std::vector<uint8_t> t8(n);
     std::vector<uint32_t> ids(n);
@@ -796,6 +798,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     CDX_HIP_CHECK(hipMemcpy(body_.as<uint32_t>() + at, ids.data(), n * 4, hipMemcpyHostToDevice));
     upload_soa(A, n, 3, lpos_.as<R>(), NSC_, at);
     upload_soa(B, n, 3, dims_.as<R>(), NSC_, at);
+    if (C) {
+      upload_soa(C, n, 3, cpos_.as<R>(), NSC_, at);
+    } else {
+      for (int c = 0; c < 3; ++c) CDX_HIP_CHECK(hipMemset(cpos_.as<R>() + c * NSC_ + at, 0, n * sizeof(R)));
+    }
     upload_soa(Rq, n, 4, lrot_.as<R>(), NSC_, at);
   }
 
@@ -944,7 +951,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_AABB, true);
     hipLaunchKernelGGL(k_aabb<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, type_.as<uint8_t>(), body_.as<uint32_t>(), lpos_.as<R>(),
-                       dims_.as<R>(), lrot_.as<R>(), NBC_, bpos_.as<R>(), brot_.as<R>(), active_.as<uint8_t>(),
+                       dims_.as<R>(), cpos_.as<R>(), lrot_.as<R>(), NBC_, bpos_.as<R>(), brot_.as<R>(), active_.as<uint8_t>(),
                        aabb_.as<R>(), rec_.as<ShapeRec<R>>(), hdr_.as<ShapeHdr>(), frec_.as<ShapeRecF>(),
                        bpart_.as<uint32_t>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), st());
     mark(ST_AABB, false);
@@ -1117,14 +1124,21 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mpr_in_tmp_ = mode_ != NP_NATIVE;  // the filter's selection went to queue_tmp_
     mpr_q_ = exact_ ? (mode_ != NP_NATIVE ? hstat_->Q2 : hstat_->Q) : 0;
     mark(ST_NP_MPR, true);
-    if (mode_ == NP_NATIVE)
-      hipLaunchKernelGGL((k_np_resolve<R, R>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q, dq,
-                         (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
-                         o, rq);
-    else
-      hipLaunchKernelGGL((k_np_resolve<R, double>), dim3(blocks_for(q, 128)), dim3(128), 0, s_, q, dq,
-                         (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_,
-                         o, rq);
+    // two launches over the class-sorted queue: the pairs of the five
+    // filtered types, then those with a rectangle, cone or triangle (kept at
+    // the queue's end), each kernel holding only the support maps it meets
+    auto resolve = [&](auto kern, unsigned grid) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(128), 0, s_, q, dq, (const uint64_t*)pairs_.as<uint64_t>(),
+                         (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, o, rq);
+    };
+    const unsigned ext_grid = std::min<unsigned>(blocks_for(q, 128), EXT_RESOLVE_GRID);
+    if (mode_ == NP_NATIVE) {
+      resolve(k_np_resolve<R, R, false>, blocks_for(q, 128));
+      resolve(k_np_resolve<R, R, true>, ext_grid);
+    } else {
+      resolve(k_np_resolve<R, double, false>, blocks_for(q, 128));
+      resolve(k_np_resolve<R, double, true>, ext_grid);
+    }
     mark(ST_NP_MPR, false);
     mark(ST_NP_RESOLVE, false);
     mark(ST_NP_COMPACT, true);
@@ -1345,7 +1359,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void* souts_pinned_ = nullptr;
   size_t souts_pinned_bytes_ = 0;
   bool desc_dirty_ = true;
-  DevBuf type_, body_, lpos_, dims_, lrot_, bpos_, brot_, active_;
+  DevBuf type_, body_, lpos_, dims_, cpos_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
   DevBuf chunks_, sel_, desc_, souts_;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;

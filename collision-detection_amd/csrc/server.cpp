@@ -211,7 +211,7 @@ int Server::class_of(uint64_t n) const {
   return kNumClasses - 1;
 }
 
-uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R,
+uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* R,
                            const uint32_t* ID, uint64_t nb, const void* pos, const void* rot, const uint8_t* active,
                            int bx, int by, int bz) {
   CDX_REQUIRE(n > 0 && nb > 0, ERR_INVALID, "a scene needs shapes and bodies");
@@ -246,7 +246,7 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
     }
   } reservation{this, need};
   if (bx == 0 && by == 0 && bz == 0) {
-    TuneChoice t = tuner_.choose(n, T, A, B, R, ID, nb, pos, rot, active);
+    TuneChoice t = tuner_.choose(n, T, A, B, C, R, ID, nb, pos, rot, active);
     for (int i = 0; i < 3; ++i) g[i] = t.grid[i];
     if (t.m_per_shape > k.m_per_shape || t.p_per_shape > k.p_per_shape) {
       k.m_per_shape = std::max(k.m_per_shape, t.m_per_shape);
@@ -254,7 +254,7 @@ uint32_t Server::add_scene(uint64_t n, const int32_t* T, const void* A, const vo
       k.eng->set_capacity_factors(k.m_per_shape * 1.25 + 1.0, k.p_per_shape * 1.25 + 1.0);
     }
   }
-  const uint32_t local = k.eng->add_scene(n, T, A, B, R, ID, nb, pos, rot, active, g[0], g[1], g[2]);
+  const uint32_t local = k.eng->add_scene(n, T, A, B, C, R, ID, nb, pos, rot, active, g[0], g[1], g[2]);
   // step buffers for the largest step this class can form, sized now
   const uint64_t nsc = k.eng->scene_count();
   k.min_n = std::min(k.min_n, n);  // a step holds at most max_shapes / min_n scenes

@@ -65,7 +65,8 @@ class Server {
   // Resident scene (host arrays in the reference layouts; body ids local).
   // bins = 0,0,0 autotunes.  Refuses (ERR_CAPACITY) a scene whose estimated
   // footprint does not fit the HBM budget.
-  uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* R, const uint32_t* ID,
+  uint32_t add_scene(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* R,
+                     const uint32_t* ID,
                      uint64_t nb, const void* pos, const void* rot, const uint8_t* active, int bx, int by, int bz);
   uint64_t scene_count() const;
   void scene_info(uint32_t scene, uint64_t* n_shapes, uint64_t* n_bodies, int* size_class, int bins[3]) const;

@@ -9,7 +9,9 @@ position 0 and local rotation identity.
 
 Dimension convention (B): sphere (r, 0, 0); box and ellipsoid semi-axes;
 cylinder (radius, half-height, radius) about local y; capsule (radius,
-half-length of the core segment, radius) about local y.
+half-length of the core segment, radius) about local y; rectangle (half
+extents in x and y, height z of its plane); cone (radius, full height, -)
+about local y, apex up.  A triangle's vertices are A, B and C.
 """
 from __future__ import annotations
 
@@ -18,7 +20,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 SPHERE, ELLIPSOID, BOX, CYLINDER, RECT, CONE, TRIANGLEMESH, CAPSULE = range(8)
-TYPE_NAMES = {SPHERE: "sphere", ELLIPSOID: "ellipsoid", BOX: "box", CYLINDER: "cylinder", CAPSULE: "capsule"}
+TYPE_NAMES = {SPHERE: "sphere", ELLIPSOID: "ellipsoid", BOX: "box", CYLINDER: "cylinder", RECT: "rect", CONE: "cone",
+              TRIANGLEMESH: "triangle", CAPSULE: "capsule"}
 
 # 1M mixed-body workload: 40 % sphere, 20 % box, 15 % ellipsoid, 15 % capsule, 10 % cylinder.
 MIX_TYPES = np.array([SPHERE, BOX, ELLIPSOID, CAPSULE, CYLINDER], dtype=np.int32)
@@ -278,3 +281,55 @@ def pose_step(s: Scene, rng, sigma=0.05, angle=0.05):
     rot = quat_mult(dq, s.rot.astype(np.float64))
     rot /= np.linalg.norm(rot, axis=1, keepdims=True)
     return pos.astype(s.dtype), rot.astype(s.dtype)
+
+
+def all_shape_types(n=4000, seed=0x5EED0010, dtype=np.float64, neighbours=6.0, grid=None):
+    """Every shape type of the reference enum plus the capsule, in one cube.
+
+    Placement follows how function_MPR_Store (Narrowphase.cu:356-377) reads
+    each type: rectangles and cones sit at their local position A (the
+    narrowphase applies no body pose to them) with a random local rotation;
+    triangles have vertices A, B, C around the origin and bodies with
+    identity rotation at the triangle's place (the reference forms the world
+    vertex as R (v + body position)); the other types as in mixed_bodies.
+    """
+    rng = np.random.default_rng(seed)
+    T = rng.choice(np.arange(8, dtype=np.int32), size=n)
+    s = np.exp(rng.normal(np.log(0.01), 0.3, size=n))
+    B = np.zeros((n, 3))
+    A = np.zeros((n, 3))
+    Cv = np.zeros((n, 3))
+    R = np.zeros((n, 4))
+    R[:, 0] = 1.0
+    rot = random_quats(rng, n)
+    for t in (BOX, ELLIPSOID):
+        m = T == t
+        B[m] = s[m, None] * rng.uniform(0.5, 1.5, size=(m.sum(), 3))
+    m = T == SPHERE
+    B[m, 0] = s[m]
+    m = T == CAPSULE
+    B[m] = np.stack([0.5 * s[m], s[m], 0.5 * s[m]], axis=1)
+    m = T == CYLINDER
+    B[m] = np.stack([s[m], s[m] * rng.uniform(0.5, 1.5, m.sum()), s[m]], axis=1)
+    m = T == RECT
+    B[m] = np.stack([s[m] * rng.uniform(0.7, 1.5, m.sum()), s[m] * rng.uniform(0.7, 1.5, m.sum()), np.zeros(m.sum())],
+                    axis=1)
+    m = T == CONE
+    B[m] = np.stack([s[m], 2.0 * s[m] * rng.uniform(0.7, 1.5, m.sum()), np.zeros(m.sum())], axis=1)
+    h = np.full((n, 3), 1.2) * s[:, None] * 1.5
+    L = box_side_for_neighbours(h, n, neighbours, rng)
+    pos = rng.uniform(0.0, L, (n, 3))
+    rt = (T == RECT) | (T == CONE)
+    A[rt] = pos[rt]
+    R[rt] = random_quats(rng, int(rt.sum()))
+    tri = T == TRIANGLEMESH
+    k = int(tri.sum())
+    verts = rng.normal(0.0, 1.0, (k, 3, 3)) * s[tri, None, None] * 1.2
+    verts -= verts.mean(axis=1, keepdims=True)
+    A[tri], B[tri], Cv[tri] = verts[:, 0], verts[:, 1], verts[:, 2]
+    rot[tri] = [1.0, 0.0, 0.0, 0.0]
+    if grid is None:
+        grid = grid_for(L, 2.0 * float(np.mean(2 * h.max(axis=1))))
+    c = lambda a: np.ascontiguousarray(a, dtype=dtype)
+    return Scene(T, c(A), c(B), c(Cv), c(R), np.arange(n, dtype=np.uint32), c(pos), c(rot), np.ones(n, np.uint8),
+                 tuple(grid), "all-types", {"n": n, "seed": seed, "L": L})

@@ -9,8 +9,15 @@
  *   real3  = 3 packed reals (x,y,z); real4 = quaternion (w,x,y,z);
  *   shape types are the reference enum (src/collision_detection.h:57)
  *   SPHERE=0 ELLIPSOID=1 BOX=2 CYLINDER=3 RECT=4 CONE=5 TRIANGLEMESH=6 plus
- *   CDX_CAPSULE=7 (no reference counterpart).  RECT, CONE and TRIANGLEMESH
- *   are rejected with CDX_ERR_UNSUPPORTED_SHAPE;
+ *   CDX_CAPSULE=7 (no reference counterpart); other values are rejected with
+ *   CDX_ERR_UNSUPPORTED_SHAPE.  A triangle's world vertices are
+ *   R (A + p), R (B + p), R (C + p) with p the body position and R the body
+ *   rotation times the local rotation (Narrowphase.cu:363-367); RECT and
+ *   CONE are centred at their local position A, rotated by R, without the
+ *   body position (Narrowphase.cu:359-377 applies none to them).  Their
+ *   AABBs, which the reference leaves unwritten (RECT, TRIANGLEMESH) or
+ *   computes by the triangle formula (CONE, AABBGenerator.cu:103-112), are
+ *   the bounds of the shape as the narrowphase places it;
  *   aabb_data = 2n real3, the n minimum corners followed by the n maximum
  *   corners (AABBGenerator.cu:114-115);
  *   pairs = int64 (a << 32 | b), a < b shape indices (Broadphase.cu:302-308);
@@ -110,9 +117,9 @@ const char* cdx_ctx_last_error(const cdx_ctx* ctx);
 
 /* Replaces AABBGenerator::GenerateAABB (AABBGenerator.cuh:21-30,
  * AABBGenerator.cu:135-188).  aabb_out receives 2*n_shapes real3.
- * C (obj_data_C) is accepted for signature parity; no in-scope shape reads
- * it.  Unlike the reference, which leaves the AABB slot of an unsupported
- * shape unwritten, unsupported types fail with CDX_ERR_UNSUPPORTED_SHAPE. */
+ * C (obj_data_C) holds the triangles' third vertices (NULL: zeros).  Types
+ * outside the enum fail with CDX_ERR_UNSUPPORTED_SHAPE (the reference
+ * leaves their AABB slots unwritten). */
 int cdx_generate_aabb(cdx_ctx* ctx, int real_kind, uint64_t n_shapes, const int32_t* T, const void* A,
                       const void* B, const void* C, const void* R, const uint32_t* ID, uint64_t n_bodies,
                       const void* body_pos, const void* body_rot, void* aabb_out);

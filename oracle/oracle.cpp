@@ -53,6 +53,7 @@ struct Fl {
   double v;
   Fl() = default;
   Fl(double x) : v(x) {}
+  explicit operator double() const { return v; }
 };
 inline thread_local uint64_t t_flops = 0;
 inline Fl operator+(Fl a, Fl b) { ++t_flops; return Fl(a.v + b.v); }
@@ -102,12 +103,26 @@ template <class R> void ComputeAABBBox(r3<R> dim, r3<R> lpos, r3<R> pos, r4<R> l
   mn = p - temp;
   mx = p + temp;
 }
-// AABBGenerator.cu:84-116.  Types 4, 5 and 6 are out of scope (5 takes the
-// reference's triangle path, AABBGenerator.cu:103-107); like the reference,
-// the slot is left unwritten for them.
+// AABBGenerator.cu:16-23
+template <class R> void ComputeAABBTriangle(r3<R> A, r3<R> B, r3<R> C, r3<R>& mn, r3<R>& mx) {
+  mn = R3<R>(std::min(A.x, std::min(B.x, C.x)), std::min(A.y, std::min(B.y, C.y)), std::min(A.z, std::min(B.z, C.z)));
+  mx = R3<R>(std::max(A.x, std::max(B.x, C.x)), std::max(A.y, std::max(B.y, C.y)), std::max(A.z, std::max(B.z, C.z)));
+}
+
+// AABBGenerator.cu:84-116, with three reference quirks fixed (FIX):
+//  * TRIANGLEMESH (6) gets the triangle bounds of :103-107, which the
+//    reference computes under type 5 instead;
+//  * CONE (5) gets the loose box (r, h/2, r) of its support set instead of
+//    the triangle path;
+//  * RECT (4), for which the reference writes nothing, gets the box of its
+//    rectangle (half extents B.x, B.y at height B.z).
+// RECT and CONE are bounded where function_MPR_Store places them: at their
+// local position, without the body pose (Narrowphase.cu:359-377 transforms
+// only SPHERE, ELLIPSOID, BOX, CYLINDER and TRIANGLEMESH), rotated by
+// body (x) local, so the box holds exactly the shape the narrowphase tests.
 template <class R>
-void function_ComputeAABB(size_t i, const int32_t* T, const R* A, const R* B, const R* Rq, const uint32_t* ID,
-                          const R* bpos, const R* brot, size_t n, R* aabb) {
+void function_ComputeAABB(size_t i, const int32_t* T, const R* A, const R* B, const R* Cv, const R* Rq,
+                          const uint32_t* ID, const R* bpos, const R* brot, size_t n, R* aabb) {
   int type = T[i];
   uint32_t id = ID[i];
   r3<R> a = ld3(A, i), b = ld3(B, i);
@@ -125,6 +140,22 @@ void function_ComputeAABB(size_t i, const int32_t* T, const R* A, const R* B, co
     // treatment the reference gives cylinders.
     r3<R> d = R3<R>(b.x, b.y + b.x, b.z);
     ComputeAABBBox(d, a, position, ld4(Rq, i), rotation, mn, mx);
+  } else if (type == TRIANGLEMESH) {  // FIX: the reference's type-5 branch
+    r4<R> q1 = mult(rotation, ld4(Rq, i));
+    r3<R> c = Cv ? ld3(Cv, i) : R3<R>(0, 0, 0);
+    ComputeAABBTriangle(quatRotate(a + position, q1), quatRotate(b + position, q1), quatRotate(c + position, q1), mn,
+                        mx);
+  } else if (type == CONE) {  // FIX: the support set's box, at the narrowphase's placement
+    r4<R> q1 = mult(rotation, ld4(Rq, i));
+    r3<R> h = MatMult(AbsMat(AMat(q1)), R3<R>(b.x, b.y * R(.5), b.x));
+    mn = a - h;
+    mx = a + h;
+  } else if (type == RECT) {  // FIX: the rectangle's box, at the narrowphase's placement
+    r4<R> q1 = mult(rotation, ld4(Rq, i));
+    r3<R> ctr = a + quatRotateMat(R3<R>(0, 0, b.z), q1);
+    r3<R> h = MatMult(AbsMat(AMat(q1)), R3<R>(b.x, b.y, 0));
+    mn = ctr - h;
+    mx = ctr + h;
   } else {
     return;
   }
@@ -369,13 +400,55 @@ template <class R> inline r3<R> sup_capsule(r3<R> B, r3<R> n) {
   return sign(n.y) * B.y * u + B.x * normalize(n);
 }
 
+// SupportFunctions.cuh:65-73 (RECT: the rectangle |x| <= B.x, |y| <= B.y at z = B.z)
+template <class R> inline r3<R> sup_plane(r3<R> B, r3<R> n) {
+  r3<R> r = B;
+  if (n.x < R(0)) r.x = -r.x;
+  if (n.y < R(0)) r.y = -r.y;
+  return r;
+}
+// SupportFunctions.cuh:74-99 (CONE: radius B.x, height B.y, apex at +B.y/2)
+template <class R> inline r3<R> sup_cone(r3<R> B, r3<R> n) {
+  R radius = B.x;
+  R height = B.y;
+  R sin_angle = radius / sqrt(radius * radius + height * height);
+  if (n.y > length(n) * sin_angle) return R3<R>(0, height / R(2.0), 0);
+  R s = sqrt(n.x * n.x + n.z * n.z);
+  if ((double)s > 1e-9) {  // a double literal in the reference: compared in double
+    R d = radius / s;
+    return R3<R>(n.x * d, -height / R(2.0), n.z * d);
+  }
+  return R3<R>(0, -height / R(2.0), 0);
+}
+// SupportFunctions.cuh:14-29 (world-placed vertices; the first maximum wins)
+template <class R> inline r3<R> sup_triangle(r3<R> A, r3<R> B, r3<R> C, r3<R> n) {
+  R dist = dot(A, n);
+  r3<R> point = A;
+  if (dot(B, n) > dist) {
+    dist = dot(B, n);
+    point = B;
+  }
+  if (dot(C, n) > dist) {
+    dist = dot(C, n);
+    point = C;
+  }
+  return point;
+}
+// Narrowphase.cu:11-17
+template <class R> inline r3<R> GetCenter(int type, r3<R> A, r3<R> B, r3<R> C) {
+  if (type == TRIANGLEMESH)  // SupportFunctions.cuh:119-121
+    return R3<R>((A.x + B.x + C.x) / R(3.0f), (A.y + B.y + C.y) / R(3.0f), (A.z + B.z + C.z) / R(3.0f));
+  return R3<R>(0, 0, 0) + A;
+}
+
 // Support-map evaluations made by this thread (for the MPR step histogram).
 thread_local uint64_t t_support_calls = 0;
 
-// Narrowphase.cu:19-52 (in-scope shapes)
-template <class R> inline r3<R> TransformSupportVert(int type, r3<R> A, r3<R> B, r4<R> Rq, r3<R> b) {
+// Narrowphase.cu:19-52
+template <class R> inline r3<R> TransformSupportVert(int type, r3<R> A, r3<R> B, r3<R> C, r4<R> Rq, r3<R> b) {
   ++t_support_calls;
   r3<R> n = normalize(b);
+  if (type == TRIANGLEMESH) return sup_triangle(A, B, C, n);  // :46-48, world vertices
   r3<R> rn = quatRotateMatT(n, Rq);
   r3<R> ls;
   switch (type) {
@@ -383,6 +456,8 @@ template <class R> inline r3<R> TransformSupportVert(int type, r3<R> A, r3<R> B,
     case ELLIPSOID: ls = sup_ellipsoid(B, rn); break;
     case BOX: ls = sup_box(B, rn); break;
     case CYLINDER: ls = sup_cylinder(B, rn); break;
+    case RECT: ls = sup_plane(B, rn); break;
+    case CONE: ls = sup_cone(B, rn); break;
     case CAPSULE: ls = sup_capsule(B, rn); break;
     default: ls = R3<R>(0, 0, 0); break;
   }
@@ -411,16 +486,16 @@ template <class R> struct MprPortal {
 // Narrowphase.cu:122-314 (XenoCollide-derived MPR).  GetCenter (:11-17) is
 // the shape position for every in-scope type.
 template <class R>
-bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r4<R> AR, int tB, r3<R> BX, r3<R> BY, r4<R> BR, r3<R>& rnormal,
-                         r3<R>& point, R& depth, MprPortal<R>* portal = nullptr) {
+bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r3<R> AZ, r4<R> AR, int tB, r3<R> BX, r3<R> BY, r3<R> BZ,
+                         r4<R> BR, r3<R>& rnormal, r3<R>& point, R& depth, MprPortal<R>* portal = nullptr) {
   r3<R> v01, v02, v0, n, v11, v12, v1, v21, v22, v2;
-  v01 = R3<R>(0, 0, 0) + AX;
-  v02 = R3<R>(0, 0, 0) + BX;
+  v01 = GetCenter(tA, AX, AY, AZ);
+  v02 = GetCenter(tB, BX, BY, BZ);
   v0 = v02 - v01;
   if (IsZero(v0)) v0 = R3<R>(1, 0, 0);
   n = -v0;
-  v11 = TransformSupportVert(tA, AX, AY, AR, -n);
-  v12 = TransformSupportVert(tB, BX, BY, BR, n);
+  v11 = TransformSupportVert(tA, AX, AY, AZ, AR, -n);
+  v12 = TransformSupportVert(tB, BX, BY, BZ, BR, n);
   v1 = v12 - v11;
   if (dot(v1, n) <= 0.0) return false;
   n = cross(v1, v0);
@@ -432,8 +507,8 @@ bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r4<R> AR, int tB, r3<R> BX,
     depth = dot((v12 - v11), n);
     return true;
   }
-  v21 = TransformSupportVert(tA, AX, AY, AR, -n);
-  v22 = TransformSupportVert(tB, BX, BY, BR, n);
+  v21 = TransformSupportVert(tA, AX, AY, AZ, AR, -n);
+  v22 = TransformSupportVert(tB, BX, BY, BZ, BR, n);
   v2 = v22 - v21;
   if (dot(v2, n) <= 0.0) return false;
   n = cross((v1 - v0), (v2 - v0));
@@ -450,8 +525,8 @@ bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r4<R> AR, int tB, r3<R> BX,
   while (true) {
     if (phase1 > max_iterations) return false;
     phase1++;
-    v31 = TransformSupportVert(tA, AX, AY, AR, -n);
-    v32 = TransformSupportVert(tB, BX, BY, BR, n);
+    v31 = TransformSupportVert(tA, AX, AY, AZ, AR, -n);
+    v32 = TransformSupportVert(tB, BX, BY, BZ, BR, n);
     v3 = v32 - v31;
     if (dot(v3, n) <= 0.0) return false;
     if (dot(cross(v1, v3), v0) < 0.0) {
@@ -469,8 +544,8 @@ bool CollideAndFindPoint(int tA, r3<R> AX, r3<R> AY, r4<R> AR, int tB, r3<R> BX,
       n = cross((v2 - v1), (v3 - v1));
       n = normalize(n);
       if (dot(n, v1) >= 0.0 && !hit) hit = true;
-      r3<R> v41 = TransformSupportVert(tA, AX, AY, AR, -n);
-      r3<R> v42 = TransformSupportVert(tB, BX, BY, BR, n);
+      r3<R> v41 = TransformSupportVert(tA, AX, AY, AZ, AR, -n);
+      r3<R> v42 = TransformSupportVert(tB, BX, BY, BZ, BR, n);
       r3<R> v4 = v42 - v41;
       R delta = dot((v4 - v3), n);
       depth = dot(v4, n);
@@ -537,12 +612,15 @@ template <class R> struct NPOut {
   size_t c = 0;
 };
 
+// Shapes function_MPR_Store places by the body pose (Narrowphase.cu:359-367);
+// RECT and CONE keep their local position as their centre there.
 inline bool body_attached(int t) { return t == SPHERE || t == ELLIPSOID || t == BOX || t == CYLINDER || t == CAPSULE; }
 
 // Narrowphase.cu:316-423 (function_MPR_Store); returns true on contact.
 template <class R>
-bool mpr_store(int64_t p, const int32_t* T, const R* A, const R* B, const R* Rq, const uint32_t* ID, const uint8_t* active,
-               const R* bpos, const R* brot, R envelope, r3<R>& N, r3<R>& p1, r3<R>& p2, R& depth, int32_t ids[2]) {
+bool mpr_store(int64_t p, const int32_t* T, const R* A, const R* B, const R* Cv, const R* Rq, const uint32_t* ID,
+               const uint8_t* active, const R* bpos, const R* brot, R envelope, r3<R>& N, r3<R>& p1, r3<R>& p2,
+               R& depth, int32_t ids[2]) {
   int ia = (int)(p >> 32), ib = (int)(p & 0xffffffff);
   int tA = T[ia], tB = T[ib];
   uint32_t idA = ID[ia], idB = ID[ib];
@@ -552,10 +630,25 @@ bool mpr_store(int64_t p, const int32_t* T, const R* A, const R* B, const R* Rq,
   r4<R> rotA = ld4(brot, idA), rotB = ld4(brot, idB);
   r3<R> AX = ld3(A, ia), BX = ld3(A, ib);
   r3<R> AY = ld3(B, ia), BY = ld3(B, ib);
+  r3<R> AZ = Cv ? ld3(Cv, ia) : R3<R>(0, 0, 0), BZ = Cv ? ld3(Cv, ib) : R3<R>(0, 0, 0);
   r4<R> AR = mult(rotA, ld4(Rq, ia));
   r4<R> BR = mult(rotB, ld4(Rq, ib));
-  if (body_attached(tA)) AX = quatRotate(AX, rotA) + posA;
-  if (body_attached(tB)) BX = quatRotate(BX, rotB) + posB;
+  if (body_attached(tA)) {
+    AX = quatRotate(AX, rotA) + posA;
+  } else if (tA == TRIANGLEMESH) {  // Narrowphase.cu:363-367
+    envelope = 0;
+    AX = quatRotate(AX + posA, AR);
+    AY = quatRotate(AY + posA, AR);
+    AZ = quatRotate(AZ + posA, AR);
+  }
+  if (body_attached(tB)) {
+    BX = quatRotate(BX, rotB) + posB;
+  } else if (tB == TRIANGLEMESH) {  // Narrowphase.cu:372-376
+    envelope = 0;
+    BX = quatRotate(BX + posB, BR);
+    BY = quatRotate(BY + posB, BR);
+    BZ = quatRotate(BZ + posB, BR);
+  }
   mark_placed<R>();
   N = R3<R>(1, 0, 0);
   p1 = R3<R>(0, 0, 0);
@@ -574,9 +667,9 @@ bool mpr_store(int64_t p, const int32_t* T, const R* A, const R* B, const R* Rq,
       return false;
     }
   } else {
-    if (!CollideAndFindPoint(tA, AX, AY, AR, tB, BX, BY, BR, N, p0, depth)) return false;
-    p1 = dot((TransformSupportVert(tA, AX, AY, AR, -N) - p0), N) * N + p0;
-    p2 = dot((TransformSupportVert(tB, BX, BY, BR, N) - p0), N) * N + p0;
+    if (!CollideAndFindPoint(tA, AX, AY, AZ, AR, tB, BX, BY, BZ, BR, N, p0, depth)) return false;
+    p1 = dot((TransformSupportVert(tA, AX, AY, AZ, AR, -N) - p0), N) * N + p0;
+    p2 = dot((TransformSupportVert(tB, BX, BY, BZ, BR, N) - p0), N) * N + p0;
     N = -N;
   }
   p1 = p1 - N * envelope;
@@ -590,8 +683,8 @@ bool mpr_store(int64_t p, const int32_t* T, const R* A, const R* B, const R* Rq,
 // Narrowphase.cu:511-595 (DoNarrowphase): per-pair loop (:481-503) then a
 // stable partition of hits to the front (:580-593).
 template <class R>
-void narrowphase(size_t P, const int64_t* pairs, const int32_t* T, const R* A, const R* B, const R* Rq, const uint32_t* ID,
-                 const uint8_t* active, const R* bpos, const R* brot, R envelope, NPOut<R>& o) {
+void narrowphase(size_t P, const int64_t* pairs, const int32_t* T, const R* A, const R* B, const R* Cv, const R* Rq,
+                 const uint32_t* ID, const uint8_t* active, const R* bpos, const R* brot, R envelope, NPOut<R>& o) {
   o = NPOut<R>();
   std::vector<R> nrm(3 * P), pa(3 * P), pb(3 * P), dp(P);
   std::vector<int32_t> ids(2 * P);
@@ -601,7 +694,7 @@ void narrowphase(size_t P, const int64_t* pairs, const int32_t* T, const R* A, c
     r3<R> N, p1, p2;
     R d;
     int32_t id2[2];
-    if (mpr_store(pairs[i], T, A, B, Rq, ID, active, bpos, brot, envelope, N, p1, p2, d, id2)) {
+    if (mpr_store(pairs[i], T, A, B, Cv, Rq, ID, active, bpos, brot, envelope, N, p1, p2, d, id2)) {
       hit[i] = 1;
       st3(nrm.data(), i, N);
       st3(pa.data(), i, p1);
@@ -679,9 +772,8 @@ static BPFlags flags_from(const int32_t* f) {
 #define ORACLE_AABB(NAME, R)                                                                                   \
   int NAME(uint64_t n, const int32_t* T, const R* A, const R* B, const R* C, const R* Rq, const uint32_t* ID,  \
            const R* bpos, const R* brot, R* aabb) {                                                          \
-    (void)C;                                                                                                   \
     _Pragma("omp parallel for") for (int64_t i = 0; i < (int64_t)n; ++i)                                      \
-        function_ComputeAABB<R>((size_t)i, T, A, B, Rq, ID, bpos, brot, (size_t)n, aabb);                     \
+        function_ComputeAABB<R>((size_t)i, T, A, B, C, Rq, ID, bpos, brot, (size_t)n, aabb);                  \
     return 0;                                                                                                  \
   }
 ORACLE_AABB(oracle_aabb_f64, double)
@@ -718,9 +810,8 @@ void oracle_bp_free(oracle_bp_result* r) {
   oracle_np_result* NAME(uint64_t P, const int64_t* pairs, const int32_t* T, const R* A, const R* B, const R* C, \
                          const R* Rq, const uint32_t* ID, const uint8_t* active, const R* bpos, const R* brot,  \
                          R envelope) {                                                                        \
-    (void)C;                                                                                                    \
     NPOut<R> o;                                                                                                 \
-    narrowphase<R>((size_t)P, pairs, T, A, B, Rq, ID, active, bpos, brot, envelope, o);                        \
+    narrowphase<R>((size_t)P, pairs, T, A, B, C, Rq, ID, active, bpos, brot, envelope, o);                        \
     auto* r = (oracle_np_result*)std::calloc(1, sizeof(oracle_np_result));                                     \
     r->p = P; r->c = o.c;                                                                                       \
     r->norm = dup_vec(o.norm); r->cpta = dup_vec(o.cpta); r->cptb = dup_vec(o.cptb); r->dpth = dup_vec(o.dpth); \
@@ -741,15 +832,15 @@ void oracle_np_free(oracle_np_result* r) {
 // function_MPR_Store (Narrowphase.cu:316-423): 0 for pairs that end before
 // the MPR (inactive, same body, sphere-sphere).
 void oracle_mpr_steps_f64(uint64_t P, const int64_t* pairs, const int32_t* T, const double* A, const double* B,
-                          const double* Rq, const uint32_t* ID, const uint8_t* active, const double* bpos,
-                          const double* brot, int32_t* steps) {
+                          const double* C, const double* Rq, const uint32_t* ID, const uint8_t* active,
+                          const double* bpos, const double* brot, int32_t* steps) {
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t i = 0; i < (int64_t)P; ++i) {
     r3<double> N, p1, p2;
     double d;
     int32_t id2[2];
     t_support_calls = 0;
-    mpr_store<double>(pairs[i], T, A, B, Rq, ID, active, bpos, brot, 0.0, N, p1, p2, d, id2);
+    mpr_store<double>(pairs[i], T, A, B, C, Rq, ID, active, bpos, brot, 0.0, N, p1, p2, d, id2);
     steps[i] = (int32_t)(t_support_calls / 2);
   }
 }
@@ -761,8 +852,8 @@ void oracle_mpr_steps_f64(uint64_t P, const int64_t* pairs, const int32_t* T, co
 // contact points; 0 for pairs that end before the placement (inactive,
 // same body).  The pair runs on a two-shape copy of its inputs.
 void oracle_mpr_flops_f64(uint64_t P, const int64_t* pairs, const int32_t* T, const double* A, const double* B,
-                          const double* Rq, const uint32_t* ID, const uint8_t* active, const double* bpos,
-                          const double* brot, double envelope, uint64_t* flops) {
+                          const double* C, const double* Rq, const uint32_t* ID, const uint8_t* active,
+                          const double* bpos, const double* brot, double envelope, uint64_t* flops) {
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t i = 0; i < (int64_t)P; ++i) {
     const uint64_t ia = (uint64_t)pairs[i] >> 32, ib = (uint64_t)pairs[i] & 0xffffffffu;
@@ -770,12 +861,16 @@ void oracle_mpr_flops_f64(uint64_t P, const int64_t* pairs, const int32_t* T, co
     int32_t t2[2];
     uint32_t id2[2] = {0, ID[ia] == ID[ib] ? 0u : 1u};
     uint8_t act2[2];
-    Fl a2[6], b2[6], r2[8], p2[6], q2[8];
+    Fl a2[6], b2[6], c2[6], r2[8], p2[6], q2[8];
     for (int k = 0; k < 2; ++k) {
       const uint64_t s = sh[k], b = ID[s];
       t2[k] = T[s];
       act2[id2[k]] = active[b];
-      for (int c = 0; c < 3; ++c) { a2[3 * k + c] = A[3 * s + c]; b2[3 * k + c] = B[3 * s + c]; }
+      for (int c = 0; c < 3; ++c) {
+        a2[3 * k + c] = A[3 * s + c];
+        b2[3 * k + c] = B[3 * s + c];
+        c2[3 * k + c] = C ? C[3 * s + c] : 0.0;
+      }
       for (int c = 0; c < 4; ++c) r2[4 * k + c] = Rq[4 * s + c];
       for (int c = 0; c < 3; ++c) p2[3 * id2[k] + c] = bpos[3 * b + c];
       for (int c = 0; c < 4; ++c) q2[4 * id2[k] + c] = brot[4 * b + c];
@@ -785,7 +880,7 @@ void oracle_mpr_flops_f64(uint64_t P, const int64_t* pairs, const int32_t* T, co
     int32_t ids[2];
     t_flops = 0;
     t_flops_placed = 0;
-    mpr_store<Fl>((int64_t)1, t2, a2, b2, r2, id2, act2, p2, q2, Fl(envelope), N, pa, pb, d, ids);
+    mpr_store<Fl>((int64_t)1, t2, a2, b2, c2, r2, id2, act2, p2, q2, Fl(envelope), N, pa, pb, d, ids);
     flops[i] = t_flops_placed ? t_flops - t_flops_placed : 0;
   }
 }
@@ -796,8 +891,9 @@ int oracle_mpr_f64(int tA, const double* AX, const double* AY, const double* AR,
                    const double* BY, const double* BR, double* normal, double* point, double* depth) {
   r3<double> n{0, 0, 0}, p{0, 0, 0};
   double d = 0;
-  bool hit = CollideAndFindPoint<double>(tA, ld3(AX, 0), ld3(AY, 0), ld4(AR, 0), tB, ld3(BX, 0), ld3(BY, 0), ld4(BR, 0),
-                                         n, p, d);
+  const r3<double> z{0, 0, 0};
+  bool hit = CollideAndFindPoint<double>(tA, ld3(AX, 0), ld3(AY, 0), z, ld4(AR, 0), tB, ld3(BX, 0), ld3(BY, 0), z,
+                                         ld4(BR, 0), n, p, d);
   st3(normal, 0, n);
   st3(point, 0, p);
   *depth = d;
@@ -814,8 +910,9 @@ int oracle_mpr_portal_f64(int tA, const double* AX, const double* AY, const doub
   r3<double> n{0, 0, 0}, p{0, 0, 0};
   double d = 0;
   MprPortal<double> pt;
-  bool hit = CollideAndFindPoint<double>(tA, ld3(AX, 0), ld3(AY, 0), ld4(AR, 0), tB, ld3(BX, 0), ld3(BY, 0), ld4(BR, 0),
-                                         n, p, d, &pt);
+  const r3<double> z{0, 0, 0};
+  bool hit = CollideAndFindPoint<double>(tA, ld3(AX, 0), ld3(AY, 0), z, ld4(AR, 0), tB, ld3(BX, 0), ld3(BY, 0), z,
+                                         ld4(BR, 0), n, p, d, &pt);
   st3(normal, 0, n);
   st3(point, 0, p);
   *depth = d;
@@ -834,7 +931,7 @@ int oracle_mpr_portal_f64(int tA, const double* AX, const double* AY, const doub
 // Support point of one world-placed shape (TransformSupportVert,
 // Narrowphase.cu:19-52), for closed-form pin tests.
 void oracle_support_f64(int t, const double* X, const double* Y, const double* Rq, const double* dir, double* out) {
-  st3(out, 0, TransformSupportVert<double>(t, ld3(X, 0), ld3(Y, 0), ld4(Rq, 0), ld3(dir, 0)));
+  st3(out, 0, TransformSupportVert<double>(t, ld3(X, 0), ld3(Y, 0), r3<double>{0, 0, 0}, ld4(Rq, 0), ld3(dir, 0)));
 }
 
 void oracle_quat_rotate_f64(const double* v, const double* q, double* out) {

@@ -93,9 +93,9 @@ def _configure(L):
         L.oracle_support_f64.argtypes = [C.c_int, vp, vp, vp, vp, vp]
         L.oracle_quat_rotate_f64.argtypes = [vp, vp, vp]
         L.oracle_quat_mult_f64.argtypes = [vp, vp, vp]
-        L.oracle_mpr_steps_f64.argtypes = [C.c_uint64] + [vp] * 10
+        L.oracle_mpr_steps_f64.argtypes = [C.c_uint64] + [vp] * 11
         L.oracle_max_threads.restype = C.c_int
-        L.oracle_mpr_flops_f64.argtypes = [C.c_uint64] + [vp] * 9 + [C.c_double, vp]
+        L.oracle_mpr_flops_f64.argtypes = [C.c_uint64] + [vp] * 10 + [C.c_double, vp]
         L.oracle_mpr_portal_f64.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_mpr_portal_f64.restype = C.c_int
     return L
@@ -225,8 +225,8 @@ def mpr_steps(scene, pairs: np.ndarray) -> np.ndarray:
     assert s.dtype == np.float64
     pr = np.ascontiguousarray(pairs, dtype=np.int64)
     out = np.zeros(len(pr), dtype=np.int32)
-    lib().oracle_mpr_steps_f64(len(pr), _p(pr), _p(s.T), _p(s.A), _p(s.B), _p(s.R), _p(s.ID), _p(s.active),
-                               _p(s.pos), _p(s.rot), _p(out))
+    lib().oracle_mpr_steps_f64(len(pr), _p(pr), _p(s.T), _p(s.A), _p(s.B), _p(s.C), _p(s.R), _p(s.ID),
+                               _p(s.active), _p(s.pos), _p(s.rot), _p(out))
     return out
 
 
@@ -239,8 +239,8 @@ def mpr_flops(scene, pairs: np.ndarray, envelope: float = 0.0) -> np.ndarray:
     assert s.dtype == np.float64
     pr = np.ascontiguousarray(pairs, dtype=np.int64)
     out = np.zeros(len(pr), dtype=np.uint64)
-    lib().oracle_mpr_flops_f64(len(pr), _p(pr), _p(s.T), _p(s.A), _p(s.B), _p(s.R), _p(s.ID), _p(s.active),
-                               _p(s.pos), _p(s.rot), float(envelope), _p(out))
+    lib().oracle_mpr_flops_f64(len(pr), _p(pr), _p(s.T), _p(s.A), _p(s.B), _p(s.C), _p(s.R), _p(s.ID),
+                               _p(s.active), _p(s.pos), _p(s.rot), float(envelope), _p(out))
     return out
 
 

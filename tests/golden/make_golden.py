@@ -34,6 +34,7 @@ def cases(scenes):
     e.active = e.active.copy()
     e.active[::7] = 0
     out["mixed300_env_inactive_f64"] = e
+    out["alltypes400_f64"] = s.all_shape_types(n=400, seed=0x5EED0105)
     return out
 
 

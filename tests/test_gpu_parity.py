@@ -19,6 +19,10 @@ def _scene_cases(scenes):
         "mixed20k_f64": scenes.mixed_bodies(n=20_000, seed=11),
         "mixed20k_f32": scenes.mixed_bodies(n=20_000, seed=12, dtype=np.float32),
         "lattice16_f32": scenes.lattice_spheres(side=16, seed=13),
+        # every reference shape type plus the capsule (rectangles, cones and
+        # triangles take the fixed AABB paths and the extended MPR launch)
+        "alltypes4k_f64": scenes.all_shape_types(n=4000, seed=14),
+        "alltypes4k_f32": scenes.all_shape_types(n=4000, seed=15, dtype=np.float32),
     }
 
 
@@ -27,7 +31,7 @@ def cases(scenes):
     return _scene_cases(scenes)
 
 
-CASE_NAMES = ["spheres8k_f64", "mixed20k_f64", "mixed20k_f32", "lattice16_f32"]
+CASE_NAMES = ["spheres8k_f64", "mixed20k_f64", "mixed20k_f32", "lattice16_f32", "alltypes4k_f64", "alltypes4k_f32"]
 
 
 @pytest.mark.parametrize("name", CASE_NAMES)
@@ -306,4 +310,32 @@ def test_mpr_queue_is_the_mpr_work(cdx, ctx, oracle, scenes):
     q = sc.read_mpr_queue()
     assert len(q) == sc.counts().resolved and len(np.unique(q)) == len(q)
     assert np.isin(q, want).all()
+    sc.close()
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fast"])
+def test_all_shape_types_with_envelope(cdx, ctx, oracle, scenes, mode):
+    """Rectangles, cones and triangles skip the FP32 filter and are resolved
+    on the reference path in every mode; triangle pairs ignore the envelope
+    (Narrowphase.cu:364, :373).  Contacts equal the FP64 oracle's."""
+    s = scenes.all_shape_types(n=6000, seed=16)
+    pairs = oracle.broadphase(oracle.aabb(s), s.grid).pairs
+    ref = oracle.narrowphase(s, pairs, envelope=0.0005)
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.set_mode(cdx.NP_MIXED if mode == "mixed" else cdx.NP_FAST, 1e-2)
+    sc.set_envelope(0.0005)
+    sc.step()
+    sc.sync()
+    norm, pa, pb, d, ids = sc.read_contacts()
+    print(f"all types mode={mode}: {len(d)} contacts, oracle {ref.count}", flush=True)
+    assert len(d) == ref.count
+    np.testing.assert_array_equal(ids, ref.bids)
+    T = s.T
+    ext = np.isin(T[ids[:, 0]], [4, 5, 6]) | np.isin(T[ids[:, 1]], [4, 5, 6])
+    assert ext.sum() > 100
+    t = _tol(s)
+    sel = slice(None) if mode == "mixed" else ext  # FAST keeps FP32 contacts for the other types
+    for g, r in ((norm, ref.norm), (pa, ref.cpta), (pb, ref.cptb)):
+        np.testing.assert_allclose(g[sel], r[sel], rtol=0, atol=t)
+    np.testing.assert_allclose(d[sel], ref.dpth[sel], rtol=0, atol=t)
     sc.close()

@@ -590,3 +590,138 @@ def test_flop_census_sphere_pair_closed_form(oracle, scenes):
     assert (f[m] > 0).all()
     per_step = f[m] / st[m]
     assert 150 < np.median(per_step) < 600
+
+
+# ----------------------------------------- RECT, CONE, TRIANGLEMESH shapes --
+
+RECT, CONE, TRIANGLEMESH = 4, 5, 6
+
+
+def _cone_points(B, k=3600):
+    r, h = B[0], B[1]
+    t = np.linspace(0, 2 * np.pi, k, endpoint=False)
+    rim = np.stack([r * np.cos(t), np.full(k, -h / 2), r * np.sin(t)], axis=1)
+    return np.vstack([[0.0, h / 2, 0.0], rim])
+
+
+def test_support_maps_rect_cone_triangle(oracle):
+    """SupportFunctions.cuh:14-29, :65-99 against brute force: the rectangle's
+    four corners at height B.z, the cone's apex and base rim, the triangle's
+    vertices (first maximum wins)."""
+    rng = np.random.default_rng(41)
+    for _ in range(100):
+        q = rand_quat(rng)
+        Rm = quat_to_mat(q)
+        c = rng.standard_normal(3)
+        d = rng.standard_normal(3)
+        B = np.array([rng.uniform(0.3, 1.0), rng.uniform(0.3, 1.0), rng.uniform(-0.2, 0.2)])
+        corners = np.array([[sx * B[0], sy * B[1], B[2]] for sx in (-1, 1) for sy in (-1, 1)]) @ Rm.T + c
+        np.testing.assert_allclose(oracle.support(RECT, c, B, q, d), corners[np.argmax(corners @ d)], atol=1e-12)
+        Bc = np.array([rng.uniform(0.2, 0.8), rng.uniform(0.3, 1.5), 0.0])
+        pts = _cone_points(Bc) @ Rm.T + c
+        got = oracle.support(CONE, c, Bc, q, d)
+        assert got @ d >= (pts @ d).max() - 1e-9 * np.linalg.norm(d)
+        loc = Rm.T @ (got - c)
+        on_apex = np.allclose(loc, [0, Bc[1] / 2, 0], atol=1e-12)
+        on_rim = abs(loc[1] + Bc[1] / 2) < 1e-12 and abs(np.hypot(loc[0], loc[2]) - Bc[0]) < 1e-12
+        assert on_apex or on_rim
+        # triangle: world vertices X, Y and (0, 0, 0) through the test entry point
+        Y = rng.standard_normal(3)
+        verts = np.array([c, Y, np.zeros(3)])
+        dn = d / np.linalg.norm(d)
+        np.testing.assert_array_equal(oracle.support(TRIANGLEMESH, c, Y, q, d), verts[np.argmax(verts @ dn)])
+
+
+def _typed_scene(scenes, T, A, B, C, R, pos, rot, grid=(4, 4, 4)):
+    n = len(T)
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    return scenes.Scene(np.asarray(T, np.int32), f(A), f(B), f(C), f(R), np.arange(n, dtype=np.uint32), f(pos),
+                        f(rot), np.ones(n, np.uint8), grid)
+
+
+def test_aabb_rect_cone_triangle(oracle, scenes):
+    """The bounds written for the types the reference leaves unwritten (RECT)
+    or sends down the wrong path (CONE, TRIANGLEMESH; AABBGenerator.cu:103-112):
+    the rectangle's corners and the cone's box at the narrowphase's placement
+    (local position, rotation body x local), and the triangle's world
+    vertices R (v + body position) (AABBGenerator.cu:16-23, :103-107)."""
+    rng = np.random.default_rng(42)
+    n = 60
+    T = np.repeat([RECT, CONE, TRIANGLEMESH], n // 3)
+    A, B, C = rng.standard_normal((n, 3)), rng.uniform(0.2, 1.0, (n, 3)), rng.standard_normal((n, 3))
+    R = np.array([rand_quat(rng) for _ in range(n)])
+    pos, rot = rng.standard_normal((n, 3)), np.array([rand_quat(rng) for _ in range(n)])
+    s = _typed_scene(scenes, T, A, B, C, R, pos, rot)
+    ab = oracle.aabb(s)
+    for i in range(n):
+        Rm = quat_to_mat(hamilton(rot[i], R[i]))
+        if T[i] == RECT:
+            pts = np.array([[sx * B[i, 0], sy * B[i, 1], B[i, 2]] for sx in (-1, 1) for sy in (-1, 1)]) @ Rm.T + A[i]
+        elif T[i] == CONE:
+            box = np.array([B[i, 0], B[i, 1] / 2, B[i, 0]])
+            pts = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]) * box @ Rm.T + A[i]
+            inner = _cone_points(B[i], 360) @ Rm.T + A[i]
+            assert np.all(inner >= ab[i] - 1e-12) and np.all(inner <= ab[n + i] + 1e-12)
+        else:
+            pts = np.array([A[i], B[i], C[i]]) + pos[i]
+            pts = pts @ Rm.T
+        np.testing.assert_allclose(ab[i], pts.min(axis=0), atol=1e-13)
+        np.testing.assert_allclose(ab[n + i], pts.max(axis=0), atol=1e-13)
+
+
+def _pair(oracle, scenes, T, A, B, C, R, pos, rot, envelope=0.0):
+    s = _typed_scene(scenes, T, A, B, C, R, pos, rot)
+    return oracle.narrowphase(s, np.array([(0 << 32) | 1], np.int64), envelope=envelope)
+
+
+I4 = [1.0, 0.0, 0.0, 0.0]
+Z3 = [0.0, 0.0, 0.0]
+
+
+@pytest.mark.parametrize("case", ["rect-face", "cone-apex", "cone-base", "triangle-face"])
+def test_mpr_new_shapes_vs_sphere_closed_form(oracle, scenes, case):
+    """A sphere pressed onto a rectangle, a cone's apex and base, and a
+    triangle: depth = -(penetration), normal along the face normal / axis."""
+    rs = 0.2
+    if case == "rect-face":
+        shape = (RECT, Z3, [0.5, 0.4, 0.0], Z3)
+        c, want_n, pen = [0.1, -0.05, 0.15], [0, 0, 1], 0.05
+    elif case == "cone-apex":  # vertex contact: along the apex-to-centre line
+        shape = (CONE, Z3, [0.3, 0.8, 0.0], Z3)
+        c = [1e-3, 0.55, -2e-3]
+        d = np.array(c) - [0.0, 0.4, 0.0]
+        want_n, pen = d / np.linalg.norm(d), rs - np.linalg.norm(d)
+    elif case == "cone-base":
+        shape = (CONE, Z3, [0.3, 0.8, 0.0], Z3)
+        c, want_n, pen = [0.05, -0.55, 0.02], [0, -1, 0], 0.05
+    else:
+        shape = (TRIANGLEMESH, [-0.5, -0.4, 0.0], [0.5, -0.4, 0.0], [0.0, 0.6, 0.0])
+        c, want_n, pen = [0.02, 0.01, 0.15], [0, 0, 1], 0.05
+    t, a, b, cc = shape
+    r = _pair(oracle, scenes, [t, SPHERE], [a, Z3], [b, [rs, 0, 0]], [cc, Z3], [I4, I4], [Z3, c], [I4, I4])
+    assert r.count == 1, case
+    # the MPR normal at a vertex contact is the portal's after the 34-step
+    # cap (Narrowphase.cu:263): close to the closed form, not equal
+    np.testing.assert_allclose(r.norm[0], want_n, atol=5e-3 if case == "cone-apex" else 1e-6)
+    np.testing.assert_allclose(r.dpth[0], -pen, atol=1e-4 if case == "cone-apex" else 1e-6)
+
+
+def test_new_shape_placement_and_envelope_quirks(oracle, scenes):
+    """function_MPR_Store (Narrowphase.cu:356-377, :412-415): RECT and CONE are
+    placed at their local position (the body position is not applied; the
+    body rotation is, through body x local), and a pair with a triangle
+    ignores the collision envelope."""
+    c = [1e-3, 0.55, -2e-3]
+    args = ([CONE, SPHERE], [Z3, Z3], [[0.3, 0.8, 0.0], [0.2, 0, 0]], [Z3, Z3], [I4, I4])
+    base = _pair(oracle, scenes, *args, [Z3, c], [I4, I4])
+    moved = _pair(oracle, scenes, *args, [[5.0, -3.0, 2.0], c], [I4, I4])
+    assert base.count == moved.count == 1
+    np.testing.assert_array_equal(base.norm, moved.norm)
+    np.testing.assert_array_equal(base.dpth, moved.dpth)
+    env = _pair(oracle, scenes, *args, [Z3, c], [I4, I4], envelope=0.01)
+    np.testing.assert_allclose(env.dpth, base.dpth + 0.02, atol=1e-14)
+    tri = ([TRIANGLEMESH, SPHERE], [[-0.5, -0.4, 0.0], Z3], [[0.5, -0.4, 0.0], [0.2, 0, 0]],
+           [[0.0, 0.6, 0.0], Z3], [I4, I4], [Z3, [0.02, 0.01, 0.15]], [I4, I4])
+    t0 = _pair(oracle, scenes, *tri)
+    t1 = _pair(oracle, scenes, *tri, envelope=0.01)
+    np.testing.assert_array_equal(t0.dpth, t1.dpth)
