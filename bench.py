@@ -353,17 +353,26 @@ def dropin_leg(pkg, ctx, s, mode, steps):
 def lattice_leg(pkg, scenes, ctx, side, steps):
     """16,777,216-sphere FP32 granular pack (side 256): AABB + broadphase, then the full step.
 
-    HBM fraction uses the broadphase's algorithmic bytes
-    N*(24+4+8+28) + M*(8+16+4+24) + 8A + 8P (count, scan, key emit; key sort,
-    bin starts, pair emit) over the summed broadphase stage time.
+    The scene's broadphase takes the cell search (near-uniform sizes).  Two
+    byte counts are set against its time:
+    * the reference decomposition's bytes at the reference grid,
+      N*(24+4+8+28) + M*(8+16+4+24) + 8A + 8P (count, scan, key emit; key
+      sort, bin starts, pair emit): the work the bin path moves, so the
+      fraction states how fast the broadphase delivers the reference's result;
+    * the cell search's own algorithmic bytes (_cell_algo_bytes), each array
+      counted once per kernel that reads or writes it.
+    The bin search at the reference grid and at the autotuned grid are timed
+    beside it.
     """
     log(f"lattice leg: generating {side}^3 spheres")
     s = scenes.lattice_spheres(side=side)
     sc = pkg.Scene(ctx, np.float32).load(s)
     avg, wall, c = _bp_measure(sc, steps)
+    cells = _cell_grid_cells(sc.read_aabb(shifted=True), s.n)
     bp_ms = sum(avg[k] for k in BP_STAGES)
     n, m, a, p = s.n, c.bin_entries, c.active_bins, c.pairs
     algo = _bp_algo_bytes(n, m, a, p)
+    own = _cell_algo_bytes(n, p, cells, 4)
     gbs = algo / (bp_ms * 1e-3) / 1e9
     sc.set_mode(pkg.NP_NATIVE)
     t0 = time.perf_counter()
@@ -374,26 +383,47 @@ def lattice_leg(pkg, scenes, ctx, side, steps):
     cc = sc.counts()
     out = {
         "workload": f"lattice-{side}^3: {n} FP32 spheres, jittered cubic lattice, r in {{0.005, 0.0075}}",
+        "pair_search": "cells" if c.flags & pkg.FLAG_CELL_SEARCH else "bins",
         "bins": list(s.grid), "bin_entries": int(m), "active_bins": int(a), "pairs": int(p),
-        "contacts": int(cc.contacts),
+        "cell_grid_cells": int(cells), "contacts": int(cc.contacts),
         "aabb_ms": round(avg["aabb"], 4), "broadphase_ms": round(bp_ms, 4),
         "broadphase_wall_ms_p50": round(float(np.median(wall)) * 1e3, 3),
         "broadphase_stage_ms": {k: round(avg[k], 4) for k in BP_STAGES},
         "broadphase_algorithmic_bytes": int(algo), "broadphase_GBs": round(gbs, 1),
         "broadphase_hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "broadphase_bytes_basis": "the reference bin decomposition at the reference grid "
+                                  "(N*64 + M*52 + 8A + 8P bytes)",
+        "cell_search_algorithmic_bytes": int(own),
+        "cell_search_GBs": round(own / (bp_ms * 1e-3) / 1e9, 1),
+        "cell_search_hbm_frac": round(own / (bp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "full_step_ms": round(step_ms, 3), "narrowphase_ms": round(np_ms, 3),
         "bodies_per_s_full_step": round(n / (step_ms * 1e-3), 1),
     }
     sc.close()
-    # The same scene at the grid the per-size-class autotuner picks (the pair
-    # list does not depend on the grid; bin counts and keys do).
-    log("lattice leg: autotuned grid")
+    # The bin search, at the reference grid and at the grid the per-size-class
+    # autotuner picks (the pair list does not depend on the grid).
+    log("lattice leg: bin search")
     bt = pkg.Batch(ctx, np.float32)
     tuned = bt.scene_bins(bt.add_scene(s, bins=(0, 0, 0)))
     bt.close()
-    sc = pkg.Scene(ctx, np.float32).load(s, bins=tuned)
-    tavg, twall, tc = _bp_measure(sc, steps)
-    sc.close()
+    for key, grid in (("bin_search_reference_grid", s.grid), ("bin_search_autotuned_grid", tuned)):
+        sc = pkg.Scene(ctx, np.float32).load(s, bins=grid)
+        sc.set_pair_search(pkg.PAIRS_BINS)
+        tavg, twall, tc = _bp_measure(sc, steps)
+        sc.close()
+        if tc.pairs != p:
+            raise RuntimeError(f"the bin search found {tc.pairs} pairs, the cell search {p}")
+        t_ms = sum(tavg[k] for k in BP_STAGES)
+        t_algo = _bp_algo_bytes(n, tc.bin_entries, tc.active_bins, tc.pairs)
+        out[key] = {
+            "bins": list(grid), "bin_entries": int(tc.bin_entries), "active_bins": int(tc.active_bins),
+            "pairs": int(tc.pairs), "broadphase_ms": round(t_ms, 4),
+            "broadphase_wall_ms_p50": round(float(np.median(twall)) * 1e3, 3),
+            "broadphase_stage_ms": {k: round(tavg[k], 4) for k in BP_STAGES},
+            "broadphase_algorithmic_bytes": int(t_algo),
+            "broadphase_GBs": round(t_algo / (t_ms * 1e-3) / 1e9, 1),
+            "broadphase_hbm_frac": round(t_algo / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        }
     del s
     # a 64^3 block of the same lattice, full step in NATIVE FP32, for the
     # FP32 contact error against the FP64 oracle
@@ -406,20 +436,43 @@ def lattice_leg(pkg, scenes, ctx, side, steps):
              "workload": "lattice-64^3 (262,144 FP32 spheres, same spacing and radii)",
              "gpu_broadphase_bodies_per_s": round(n / (bp_ms * 1e-3), 1)}
     sc.close()
-    if tc.pairs != p:
-        raise RuntimeError(f"autotuned grid changed the pair count: {tc.pairs} != {p}")
-    t_ms = sum(tavg[k] for k in BP_STAGES)
-    t_algo = _bp_algo_bytes(n, tc.bin_entries, tc.active_bins, tc.pairs)
-    out["autotuned_grid"] = {
-        "bins": list(tuned), "bin_entries": int(tc.bin_entries), "active_bins": int(tc.active_bins),
-        "pairs": int(tc.pairs), "broadphase_ms": round(t_ms, 4),
-        "broadphase_wall_ms_p50": round(float(np.median(twall)) * 1e3, 3),
-        "broadphase_stage_ms": {k: round(tavg[k], 4) for k in BP_STAGES},
-        "broadphase_algorithmic_bytes": int(t_algo),
-        "broadphase_GBs": round(t_algo / (t_ms * 1e-3) / 1e9, 1),
-        "broadphase_hbm_frac": round(t_algo / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-    }
     return out, check
+
+
+CELL_KX = 2  # fine cells along x per coarse cell of the library's cell search
+
+
+def _cell_grid_cells(shifted_aabb, n):
+    """Cells of the cell search's grid, as the library's scene broadphase plans
+    it: coarse edge (1 + 1e-3) x the widest extent (at most 2N + 64 coarse
+    cells), CELL_KX times finer along x."""
+    lo, hi = shifted_aabb[:n].astype(np.float64), shifted_aabb[n:].astype(np.float64)
+    w = float(np.float32((hi - lo).max()))
+    ext = np.maximum(hi.max(axis=0), 0.0)
+    target = 2.0 * n + 64.0
+    edge = w * (1.0 + 1e-3)
+    for _ in range(64):
+        g = np.floor(ext / edge) + 1.0
+        if float(np.prod(g)) <= target:
+            break
+        edge *= max(float(np.cbrt(np.prod(g) / target)), 1.0 + 1e-3)
+    return int((np.floor(ext[0] * CELL_KX / edge) + 1.0) * g[1] * g[2])
+
+
+def _cell_algo_bytes(n, p, cells, rs):
+    """Bytes the cell search moves, each array once per kernel touching it:
+    bin count pass (AABB 6rs, count 4) and its scan (4 + 4); cell keys (AABB,
+    entry 8, histogram read and write 8), table clear and scan (4 + 8 per
+    cell); radix sort (8 read twice, 8 written, per pass); gather (entry,
+    AABB, the two entry halves 2h, last index 4 per cell after its clear);
+    window maxima (4 + 4 per cell); search (own halves, count 4, 4 per
+    partner into the slab); count scan (4 + 4); placement (count, offset,
+    4 per partner read back, 8 per pair written)."""
+    h = 16 if rs == 4 else 32
+    bits = max(1, int(np.ceil(np.log2(max(cells, 2)))))
+    passes = -(-bits // 11)
+    per_n = (6 * rs + 4) + 8 + (6 * rs + 16) + 24 * passes + (8 + 6 * rs + 2 * h) + (2 * h + 4) + 8 + 8
+    return n * per_n + 16 * p + 24 * cells
 
 
 def _bp_algo_bytes(n, m, a, p):

@@ -26,6 +26,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 REAL_F64, REAL_F32 = 0, 1
 NP_NATIVE, NP_MIXED, NP_FAST = 0, 1, 2
+PAIRS_AUTO, PAIRS_BINS, PAIRS_CELLS = 0, 1, 2
+FLAG_CELL_SEARCH, FLAG_NO_ACTIVE_BINS = 0x100, 0x200
 STAGES = ["aabb", "bp_bounds", "bp_count", "bp_scan", "bp_emit", "bp_sort", "bp_runs", "bp_pairs",
           "bp_pairsort", "np_main", "np_resolve", "np_compact", "bp_pairscan", "bp_pairemit", "np_mpr"]
 
@@ -37,7 +39,7 @@ EXPORTS = [
     "cdx_do_narrowphase", "cdx_read_contacts", "cdx_mpr_store", "cdx_scene_create", "cdx_scene_destroy", "cdx_scene_last_error",
     "cdx_scene_set_shapes", "cdx_scene_set_bodies", "cdx_scene_update_bodies", "cdx_scene_set_aabbs",
     "cdx_scene_set_pairs", "cdx_scene_set_bins", "cdx_scene_set_envelope", "cdx_scene_set_narrowphase_mode",
-    "cdx_scene_set_profiling", "cdx_scene_aabb", "cdx_scene_broadphase", "cdx_scene_narrowphase",
+    "cdx_scene_set_profiling", "cdx_scene_set_pair_search", "cdx_scene_aabb", "cdx_scene_broadphase", "cdx_scene_narrowphase",
     "cdx_scene_step", "cdx_scene_sync", "cdx_scene_counts", "cdx_scene_read_aabb", "cdx_scene_read_bins",
     "cdx_scene_read_pairs", "cdx_scene_read_pairs_permuted", "cdx_scene_read_mpr_queue", "cdx_scene_read_contacts",
     "cdx_scene_stage_ms",
@@ -135,6 +137,7 @@ def lib():
         "cdx_scene_set_envelope": ([vp, C.c_double], i32),
         "cdx_scene_set_narrowphase_mode": ([vp, i32, C.c_double], i32),
         "cdx_scene_set_profiling": ([vp, i32], i32),
+        "cdx_scene_set_pair_search": ([vp, i32], i32),
         "cdx_scene_aabb": ([vp], i32),
         "cdx_scene_broadphase": ([vp], i32),
         "cdx_scene_narrowphase": ([vp], i32),
@@ -334,6 +337,10 @@ class Scene:
 
     def set_profiling(self, on=True):
         self._chk(lib().cdx_scene_set_profiling(self.h, 1 if on else 0))
+
+    def set_pair_search(self, mode):
+        """PAIRS_AUTO, PAIRS_BINS or PAIRS_CELLS (cdx_scene_set_pair_search)."""
+        self._chk(lib().cdx_scene_set_pair_search(self.h, int(mode)))
 
     def aabb(self):
         self._chk(lib().cdx_scene_aabb(self.h))

@@ -269,21 +269,29 @@ template <class R> __device__ __forceinline__ Cells cells_of(const BpRec<R>& r, 
 // Bins spanned per AABB (Broadphase.cu:150-159), on coordinates shifted by
 // the scene origin (Broadphase.cu:382-386) and hashed by truncation
 // (HashMin, :125-133); plus the largest key (the max corner's, keys do not
-// wrap in the dense key range).
+// wrap in the dense key range).  With wpart, also the widest shifted AABB
+// extent per chunk (as float bits: non-negative floats order as unsigned
+// integers), which sizes the cell search's grid.
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                  uint64_t NS, const R* __restrict__ aabb,
                                                  const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ counts,
-                                                 uint32_t* __restrict__ kpart, DevStatus* st) {
+                                                 uint32_t* __restrict__ kpart, uint32_t* __restrict__ wpart,
+                                                 DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
-  __shared__ uint32_t skm[CHUNK / WAVE];
+  __shared__ uint32_t skm[CHUNK / WAVE], swm[CHUNK / WAVE];
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  uint32_t km = 0;
+  uint32_t km = 0, wm = 0;
   if (threadIdx.x < ch.len) {
     const SceneDesc<R>& d = desc[ch.scene];
-    const Cells c = cells_of(shifted_rec(aabb, NS, i, d, ch.scene), d);
+    const BpRec<R> r = shifted_rec(aabb, NS, i, d, ch.scene);
+    const Cells c = cells_of(r, d);
+    if (wpart) {
+      const float w = fmaxf(fmaxf((float)(r.hi[0] - r.lo[0]), (float)(r.hi[1] - r.lo[1])), (float)(r.hi[2] - r.lo[2]));
+      wm = w > 0.f ? __float_as_uint(w) : 0u;  // NaN extents come with a reported AABB
+    }
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
       counts[slot] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = d.key_base + hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, d.gx, d.gy);
@@ -294,26 +302,45 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
   } else {
     counts[slot] = 0;
   }
-  for (int o = 32; o > 0; o >>= 1) km = max(km, (uint32_t)__shfl_xor((int)km, o));
-  if (threadIdx.x % WAVE == 0) skm[threadIdx.x / WAVE] = km;
+  for (int o = 32; o > 0; o >>= 1) {
+    km = max(km, (uint32_t)__shfl_xor((int)km, o));
+    wm = max(wm, (uint32_t)__shfl_xor((int)wm, o));
+  }
+  if (threadIdx.x % WAVE == 0) {
+    skm[threadIdx.x / WAVE] = km;
+    swm[threadIdx.x / WAVE] = wm;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t m = 0;
-    for (int w = 0; w < CHUNK / WAVE; ++w) m = max(m, skm[w]);
+    uint32_t m = 0, mw = 0;
+    for (int w = 0; w < CHUNK / WAVE; ++w) {
+      m = max(m, skm[w]);
+      mw = max(mw, swm[w]);
+    }
     kpart[blockIdx.x] = m;
+    if (wpart) wpart[blockIdx.x] = mw;
   }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
-  __shared__ uint32_t red[256];
+__global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart,
+                                                       const uint32_t* __restrict__ wpart, DevStatus* st) {
+  __shared__ uint32_t red[256], redw[256];
   const uint64_t npart = st->nslots / CHUNK;
-  uint32_t m = 0;
-  for (uint64_t b = threadIdx.x; b < npart; b += 256) m = max(m, kpart[b]);
+  uint32_t m = 0, mw = 0;
+  for (uint64_t b = threadIdx.x; b < npart; b += 256) {
+    m = max(m, kpart[b]);
+    if (wpart) mw = max(mw, wpart[b]);
+  }
   red[threadIdx.x] = m;
+  redw[threadIdx.x] = mw;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int t = 1; t < 256; ++t) m = max(m, red[t]);
+    for (int t = 1; t < 256; ++t) {
+      m = max(m, red[t]);
+      mw = max(mw, redw[t]);
+    }
     st->key_max = m;
+    st->wmax = mw;
   }
 }
 

@@ -75,7 +75,7 @@ template <class F> int guarded(std::string* err, F&& f, bool gate = true) {
 
 void fill_counts(const cdx::StepCounts& c, cdx_counts* out) {
   out->n_shapes = c.n; out->bin_entries = c.m; out->active_bins = c.a; out->pairs = c.p;
-  out->contacts = c.c; out->resolved = c.resolved; out->key_max = c.key_max; out->flags = c.bad;
+  out->contacts = c.c; out->resolved = c.resolved; out->key_max = c.key_max; out->flags = c.bad | c.info;
   for (int i = 0; i < 3; ++i) { out->bmin[i] = c.bmin[i]; out->bmax[i] = c.bmax[i]; }
 }
 
@@ -328,13 +328,14 @@ int cdx_scene_set_narrowphase_mode(cdx_scene* scene, int mode, double tau) {
   SCENE_CALL(CDX_REQUIRE(tau >= 0.0, CDX_ERR_INVALID, "tau must be non-negative"); s.set_np_mode(mode, tau));
 }
 int cdx_scene_set_profiling(cdx_scene* scene, int on) { SCENE_CALL(s.set_profiling(on != 0)); }
+int cdx_scene_set_pair_search(cdx_scene* scene, int mode) { SCENE_CALL(s.set_pair_search(mode)); }
 int cdx_scene_aabb(cdx_scene* scene) { SCENE_CALL(s.run_aabb()); }
 int cdx_scene_broadphase(cdx_scene* scene) { SCENE_CALL(s.run_broadphase()); }
 int cdx_scene_narrowphase(cdx_scene* scene) { SCENE_CALL(s.run_narrowphase()); }
 int cdx_scene_step(cdx_scene* scene) { SCENE_CALL(s.run_aabb(); s.run_broadphase(); s.run_narrowphase()); }
 int cdx_scene_sync(cdx_scene* scene) { SCENE_CALL(s.sync()); }
 int cdx_scene_counts(cdx_scene* scene, cdx_counts* out) {
-  SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); fill_counts(s.counts(), out));
+  SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); fill_counts(s.counts_full(), out));
 }
 int cdx_scene_read_aabb(cdx_scene* scene, int shifted, void* out) {
   SCENE_CALL(CDX_REQUIRE(out, CDX_ERR_INVALID, "null output"); s.read_aabb(out, shifted != 0));
@@ -352,6 +353,10 @@ int cdx_scene_read_mpr_queue(cdx_scene* scene, uint32_t* pair_index, uint64_t ca
   SCENE_CALL(CDX_REQUIRE(count, CDX_ERR_INVALID, "null count"); *count = s.read_mpr_queue(pair_index, capacity));
 }
 static_assert(cdx::ST_COUNT == CDX_STAGE_COUNT, "stage list of cdx.h and the pipeline differ");
+static_assert(cdx::PAIRS_AUTO == CDX_PAIRS_AUTO && cdx::PAIRS_BINS == CDX_PAIRS_BINS && cdx::PAIRS_CELLS == CDX_PAIRS_CELLS,
+              "pair search modes");
+static_assert(cdx::INFO_CELL_SEARCH == CDX_FLAG_CELL_SEARCH && cdx::INFO_NO_ACTIVE_BINS == CDX_FLAG_NO_ACTIVE_BINS,
+              "count flags");
 static_assert(cdx::ST_NP_MPR == CDX_STAGE_NP_MPR && cdx::ST_BP_PAIREMIT == CDX_STAGE_BP_PAIREMIT, "stage numbers");
 
 int cdx_scene_stage_ms(cdx_scene* scene, double* ms) {

@@ -40,7 +40,15 @@ struct StepCounts {
   uint32_t key_max;
   uint32_t bad;  // bit 0: unsupported shape type, bit 1: body id out of range
   float bmin[3], bmax[3];
+  uint32_t info = 0;  // INFO_* bits
 };
+
+// StepCounts::info (reported with the flags of cdx_counts).
+constexpr uint32_t INFO_CELL_SEARCH = 0x100;     // pairs found by the cell search
+constexpr uint32_t INFO_NO_ACTIVE_BINS = 0x200;  // active bins not computed (yet)
+
+// Pair search of a scene's broadphase.
+enum : int { PAIRS_AUTO = 0, PAIRS_BINS = 1, PAIRS_CELLS = 2 };
 
 class SceneBase {
  public:
@@ -61,6 +69,10 @@ class SceneBase {
   virtual void set_envelope(double e) = 0;
   virtual void set_np_mode(int mode, double tau) = 0;
   virtual void set_profiling(bool on) = 0;
+  // PAIRS_AUTO: the cell search for near-uniform shape sizes, else the
+  // bins; PAIRS_BINS / PAIRS_CELLS force one (the cell search only where its
+  // grid fits).
+  virtual void set_pair_search(int mode) = 0;
 
   // Stages.  Each enqueues on the scene stream; the broadphase reads its
   // entry and pair totals back to size buffers, as the reference does.
@@ -71,6 +83,8 @@ class SceneBase {
 
   // Results (after sync()).
   virtual const StepCounts& counts() const = 0;
+  // counts(), with the active bins of a cell-search broadphase computed.
+  virtual const StepCounts& counts_full() = 0;
   virtual void read_aabb(void* out, bool shifted) = 0;
   virtual void read_bins(uint32_t* bin_counts, uint32_t* sorted_keys, uint32_t* sorted_bodies, uint32_t* active_keys,
                          uint32_t* active_counts) = 0;

@@ -648,7 +648,6 @@ __global__ __launch_bounds__(COMPACT_BS) void k_np_compact(uint64_t cap, const u
   __shared__ R lv[10][COMPACT_BS];
   __shared__ int32_t lid[2 * COMPACT_BS];
   __shared__ uint32_t s_k0, s_nh;
-  const uint64_t P = o.P;
   const uint64_t n = dev_count(cap, d_np);
   const uint64_t i0 = (uint64_t)blockIdx.x * COMPACT_BS;
   if (i0 >= n) return;

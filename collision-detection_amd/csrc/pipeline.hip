@@ -31,6 +31,7 @@
 
 #include "bp_kernels.hpp"
 #include "cdx_pipeline.hpp"
+#include "cell_kernels.hpp"
 #include "cdx_util.hpp"
 #include "np_kernels.hpp"
 
@@ -49,7 +50,7 @@ __global__ void k_stage_begin(const StepHdr* __restrict__ hdr, DevStatus* st, in
   st->nslots = hdr->nslots;
   st->nsel = hdr->nsel;
   if (what & ZERO_AABB) st->bad = 0;
-  if (what & ZERO_BP) { st->M = st->A = st->P = 0; st->overflow = 0; st->key_max = 0; st->nbig = 0; }
+  if (what & ZERO_BP) { st->M = st->A = st->P = 0; st->overflow = 0; st->key_max = 0; st->nbig = 0; st->wmax = 0; st->novf = 0; }
   if (what & ZERO_NP) { st->Q = st->Q2 = st->C = 0; }
 }
 
@@ -233,6 +234,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void set_aabbs(uint64_t n, const void* aabb) override {
     CDX_REQUIRE(n < (1ull << 31), ERR_INVALID, "too many AABBs (limit 2^31 - 1)");
     sync_internal();
+    aabbs_changed();
     single_ensure_scene0();
     ensure_shape_capacity(n);
     scenes_[0].n_shapes = n;
@@ -280,6 +282,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     tau_ = (float)tau;
   }
   void set_profiling(bool on) override { prof_ = on; }
+  void set_pair_search(int mode) override {
+    CDX_REQUIRE(mode == PAIRS_AUTO || mode == PAIRS_BINS || mode == PAIRS_CELLS, ERR_INVALID,
+                "unknown pair search mode");
+    pair_search_ = mode;
+  }
 
   void run_aabb() override {
     CDX_REQUIRE(shapes_set_ && bodies_set_, ERR_NOT_READY, "the AABB stage needs shapes and bodies");
@@ -301,6 +308,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     if (scenes_[0].n_shapes == 0) {
       M_ = A_ = P_ = 0;
       counts_.n = 0; counts_.m = counts_.a = counts_.p = 0;
+      counts_.info = 0;
+      cell_ = bins_pending_ = bins_lost_ = false;
       return;
     }
     stage_broadphase();
@@ -354,38 +363,66 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     // scene 0 starts at shape 0, so its slots are its shape indices
     if (bin_counts && n) CDX_HIP_CHECK(hipMemcpy(bin_counts, counts_buf_.p, n * 4, hipMemcpyDeviceToHost));
     if ((sorted_keys || sorted_bodies || active_keys || active_counts) && M_) {
-      // reduce_by_key on the device (Broadphase.cu:439-443), only for callers
-      // that read the bins; the host copies the results
       const uint64_t m = M_;
-      const size_t tmp_words = std::max<size_t>(scan_tmp_words(m), 16);
-      bins_.reserve(m * 4 * 6 + 64 + tmp_words * 4);
-      uint32_t* k = bins_.as<uint32_t>();
-      uint32_t* b = k + m;
-      uint32_t* heads = b + m;
-      uint32_t* run_of = heads + m;
-      uint32_t* akeys = run_of + m;
-      uint32_t* rstart = akeys + m;
-      uint64_t* d_runs = reinterpret_cast<uint64_t*>(rstart + m);  // 24m bytes in: 8-byte aligned
-      uint32_t* tmp = reinterpret_cast<uint32_t*>(d_runs + 8);
-      hipLaunchKernelGGL(k_bins_split, dim3(blocks_for(m, 256)), dim3(256), 0, s_, m,
-                         (const uint64_t*)keys_.as<uint64_t>(), k, b, heads);
-      scan_excl_u32(heads, run_of, m, d_runs, tmp, s_);
-      hipLaunchKernelGGL(k_bins_runs, dim3(blocks_for(m, 256)), dim3(256), 0, s_, m, (const uint32_t*)k,
-                         (const uint32_t*)heads, (const uint32_t*)run_of, akeys, rstart);
-      uint64_t runs = 0;
-      CDX_HIP_CHECK(hipMemcpyAsync(&runs, d_runs, 8, hipMemcpyDeviceToHost, s_));
-      sync();
-      CDX_REQUIRE(runs == A_, ERR_HIP, "internal error: active bins of the pair search and reduce_by_key differ");
-      if (runs)
-        hipLaunchKernelGGL(k_bins_occupancy, dim3(blocks_for(runs, 256)), dim3(256), 0, s_, runs, m,
-                           (const uint32_t*)rstart, heads);  // heads no longer needed: occupancy goes there
+      const uint64_t runs = reduce_bins();
+      const uint32_t* k = bins_.as<uint32_t>();
+      const uint32_t* b = k + m;
+      const uint32_t* occupancy = b + m;
+      const uint32_t* akeys = occupancy + 2 * m;
       if (sorted_keys) CDX_HIP_CHECK(hipMemcpyAsync(sorted_keys, k, m * 4, hipMemcpyDeviceToHost, s_));
       if (sorted_bodies) CDX_HIP_CHECK(hipMemcpyAsync(sorted_bodies, b, m * 4, hipMemcpyDeviceToHost, s_));
       if (active_keys && runs) CDX_HIP_CHECK(hipMemcpyAsync(active_keys, akeys, runs * 4, hipMemcpyDeviceToHost, s_));
       if (active_counts && runs)
-        CDX_HIP_CHECK(hipMemcpyAsync(active_counts, heads, runs * 4, hipMemcpyDeviceToHost, s_));
+        CDX_HIP_CHECK(hipMemcpyAsync(active_counts, occupancy, runs * 4, hipMemcpyDeviceToHost, s_));
       sync();
     }
+  }
+
+  // reduce_by_key over the sorted bin entries on the device
+  // (Broadphase.cu:439-443), only for callers that read the bins: bins_
+  // then holds the sorted keys, the sorted shapes, the occupancy of each
+  // active bin, a scratch block and the active keys, m words each.
+  // Returns the number of active bins.
+  uint64_t reduce_bins() {
+    ensure_bins();
+    const uint64_t m = M_;
+    const size_t tmp_words = std::max<size_t>(scan_tmp_words(m), 16);
+    bins_.reserve(m * 4 * 6 + 64 + tmp_words * 4);
+    uint32_t* k = bins_.as<uint32_t>();
+    uint32_t* b = k + m;
+    uint32_t* heads = b + m;
+    uint32_t* run_of = heads + m;
+    uint32_t* akeys = run_of + m;
+    uint32_t* rstart = akeys + m;
+    uint64_t* d_runs = reinterpret_cast<uint64_t*>(rstart + m);  // 24m bytes in: 8-byte aligned
+    uint32_t* tmp = reinterpret_cast<uint32_t*>(d_runs + 8);
+    hipLaunchKernelGGL(k_bins_split, dim3(blocks_for(m, 256)), dim3(256), 0, s_, m,
+                       (const uint64_t*)keys_.as<uint64_t>(), k, b, heads);
+    scan_excl_u32(heads, run_of, m, d_runs, tmp, s_);
+    hipLaunchKernelGGL(k_bins_runs, dim3(blocks_for(m, 256)), dim3(256), 0, s_, m, (const uint32_t*)k,
+                       (const uint32_t*)heads, (const uint32_t*)run_of, akeys, rstart);
+    uint64_t runs = 0;
+    CDX_HIP_CHECK(hipMemcpyAsync(&runs, d_runs, 8, hipMemcpyDeviceToHost, s_));
+    sync();
+    if (cell_) {
+      A_ = runs;
+      counts_.a = runs;
+      counts_.info &= ~INFO_NO_ACTIVE_BINS;
+    }
+    CDX_REQUIRE(runs == A_, ERR_HIP, "internal error: active bins of the pair search and reduce_by_key differ");
+    if (runs)  // heads are no longer needed: the occupancy goes there
+      hipLaunchKernelGGL(k_bins_occupancy, dim3(blocks_for(runs, 256)), dim3(256), 0, s_, runs, m,
+                         (const uint32_t*)rstart, heads);
+    return runs;
+  }
+
+  const StepCounts& counts_full() override {
+    if (cell_ && (counts_.info & INFO_NO_ACTIVE_BINS) && M_ && !bins_lost_) {
+      sync();
+      reduce_bins();
+      sync();
+    }
+    return counts_;
   }
 
   void read_pairs(int64_t* out) override {
@@ -431,7 +468,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
-                           &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &bins_};
+                           &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &bins_,
+                           &wpart_, &cell_tab_, &cell_ckr_, &cell_tmp_, &cell_lo_, &cell_hi_, &cell_slab_, &cell_last_, &cell_wmax_, &cell_ovf_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -944,7 +982,16 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t chunk_grid() const { return exact_ ? std::max<uint64_t>(nchunks_, 1) : std::max<uint64_t>(chunk_cap_, 1); }
   uint64_t sel_grid() const { return exact_ ? std::max<uint64_t>(nsel_, 1) : std::max<uint64_t>(sel_cap_, 1); }
 
+  // The AABBs change: the bins of a cell-search broadphase can no longer be built.
+  void aabbs_changed() {
+    if (bins_pending_) {
+      bins_pending_ = false;
+      bins_lost_ = true;
+    }
+  }
+
   void stage_aabb() {
+    aabbs_changed();
     if (!opened_)
       hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
                          (int)ZERO_AABB);
@@ -977,40 +1024,33 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(),
-                       kpart_.as<uint32_t>(), st());
-    hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
+                       kpart_.as<uint32_t>(), exact_ ? wpart_.as<uint32_t>() : nullptr, st());
+    hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(),
+                       exact_ ? (const uint32_t*)wpart_.as<uint32_t>() : nullptr, st());
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
     scan_excl_u32(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, scan_tmp_.as<uint32_t>(), s_,
                   &st()->nslots);
     mark(ST_BP_SCAN, false);
     int key_bits = key_bits_;
+    cell_ = false;
+    bins_pending_ = false;
+    bins_lost_ = false;
     if (exact_) {
       read_status();
       M_ = hstat_->M;
       CDX_REQUIRE(M_ < (1ull << 32), ERR_CAPACITY, "more than 2^32 bin entries: use fewer bins per axis");
       mcap_ = std::max<uint64_t>(M_, 1);
       key_bits = bits_for(hstat_->key_max);
+      exact_key_bits_ = key_bits;
+      if (pair_search_ != PAIRS_BINS && plan_cells()) {
+        stage_cells();
+        return;
+      }
     }
-    keys_.reserve(mcap_ * 8);
-    keys_tmp_.reserve(mcap_ * 8);
-    pcount_.reserve((mcap_ + 1) * 4);
+    emit_and_sort_bins(key_bits);
     meta_.reserve(mcap_ * sizeof(PairMeta));
     runs_part_.reserve((mcap_ / PAIR_BS + 1) * 4);
-    mark(ST_BP_EMIT, true);
-    hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
-                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (const uint32_t*)offs_.as<uint32_t>(),
-                       (const uint32_t*)counts_buf_.as<uint32_t>(), mcap_, keys_.as<uint64_t>(),
-                       pcount_.as<uint32_t>(), brec_.as<BpRec<R>>(), st());
-    mark(ST_BP_EMIT, false);
-    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits), 16) * 4);
-    mark(ST_BP_SORT, true);
-    if (radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, 32, key_bits, radix_tmp_.as<uint32_t>(),
-                       s_, &st()->M))
-      swap_bufs(keys_, keys_tmp_);
-    mark(ST_BP_SORT, false);
-
     mark(ST_BP_PAIRS, true);
     hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
@@ -1057,12 +1097,183 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_BP_PAIRSORT, false);
     if (exact_) {
       counts_.n = scenes_[0].n_shapes; counts_.m = M_; counts_.a = A_; counts_.p = P_;
+      counts_.info = 0;
       counts_.key_max = hstat_->key_max;
       counts_.bad = hstat_->bad;
       SceneDesc<R> d;
       CDX_HIP_CHECK(hipMemcpy(&d, desc_.p, sizeof(d), hipMemcpyDeviceToHost));
       for (int c = 0; c < 3; ++c) { counts_.bmin[c] = ord2f(d.bmin_enc[c]); counts_.bmax[c] = ord2f(d.bmax_enc[c]); }
     }
+  }
+
+  // (bin key << 32 | shape) entries (Broadphase.cu:172-209) sorted by key
+  // (:428-432), plus each shape's broadphase record.
+  void emit_and_sort_bins(int key_bits) {
+    keys_.reserve(mcap_ * 8);
+    keys_tmp_.reserve(mcap_ * 8);
+    pcount_.reserve((mcap_ + 1) * 4);
+    mark(ST_BP_EMIT, true);
+    hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
+                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (const uint32_t*)offs_.as<uint32_t>(),
+                       (const uint32_t*)counts_buf_.as<uint32_t>(), mcap_, keys_.as<uint64_t>(),
+                       pcount_.as<uint32_t>(), brec_.as<BpRec<R>>(), st());
+    mark(ST_BP_EMIT, false);
+    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits), 16) * 4);
+    mark(ST_BP_SORT, true);
+    if (radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, 32, key_bits, radix_tmp_.as<uint32_t>(),
+                       s_, &st()->M))
+      swap_bufs(keys_, keys_tmp_);
+    mark(ST_BP_SORT, false);
+  }
+
+  // Grid of the cell search for scene 0 (exact mode, after the bin count
+  // pass): cells of edge (1 + 1e-3) * the widest AABB extent along y and z
+  // (enlarged until there are at most about two per shape) and CELL_KX times
+  // finer along x.  Taken when the mean occupancy of the coarse cells stays
+  // at most CELL_AUTO_DENSITY shapes (always, when the caller asks for the
+  // cell search).
+  bool plan_cells() {
+    const uint64_t n = scenes_[0].n_shapes;
+    if (n < 2) return false;
+    SceneDesc<R> d;
+    CDX_HIP_CHECK(hipMemcpy(&d, desc_.p, sizeof(d), hipMemcpyDeviceToHost));
+    float w;
+    std::memcpy(&w, &hstat_->wmax, 4);
+    if (!(w >= 0.f) || !std::isfinite(w)) return false;
+    double ext[3];
+    for (int c = 0; c < 3; ++c) {
+      // shifted coordinates run from about 0 to bmax + origin
+      const double hi = (double)ord2f(d.bmax_enc[c]) + (double)d.origin[c];
+      if (!std::isfinite(hi)) return false;
+      ext[c] = std::max(hi, 0.0);
+    }
+    const double target = 2.0 * (double)n + 64.0;
+    double edge = (double)w * (1.0 + 1e-3);
+    if (!(edge > 0.0)) edge = std::max(std::max(ext[0], ext[1]), std::max(ext[2], 1e-300));
+    double g[3], cells = 0;
+    for (int it = 0; it < 64; ++it) {
+      cells = 1;
+      for (int c = 0; c < 3; ++c) {
+        g[c] = std::floor(ext[c] / edge) + 1.0;
+        cells *= g[c];
+      }
+      if (cells <= target) break;
+      edge *= std::max(std::cbrt(cells / target), 1.0 + 1e-3);
+    }
+    if (!(cells <= target)) return false;
+    if (pair_search_ == PAIRS_AUTO && (double)n / cells > CELL_AUTO_DENSITY) return false;
+    const double gxf = std::floor(ext[0] * CELL_KX / edge) + 1.0;
+    if (gxf * g[1] * g[2] >= 4294967295.0) return false;
+    cg_.g[0] = (uint32_t)gxf;
+    cg_.g[1] = (uint32_t)g[1];
+    cg_.g[2] = (uint32_t)g[2];
+    cg_.kx = (uint32_t)CELL_KX;
+    cg_.inv[0] = CELL_KX / edge;
+    cg_.inv[1] = cg_.inv[2] = 1.0 / edge;
+    ncells_ = (uint64_t)cg_.g[0] * cg_.g[1] * cg_.g[2];
+    return true;
+  }
+
+  // Pair search by cells (the bin entries are counted and scanned, but
+  // emitted and sorted only when a caller reads the bins).
+  void stage_cells() {
+    const uint64_t n = scenes_[0].n_shapes;
+    const uint64_t slots = chunk_grid() * CHUNK;
+    const uint64_t ntab = ncells_ + 1;
+    const int kbits = bits_for(ncells_ - 1);
+    cell_tab_.reserve(ntab * 4);
+    cell_ckr_.reserve(n * 8);
+    cell_tmp_.reserve(n * 8);
+    cell_lo_.reserve(n * sizeof(CellLo<R>));
+    cell_hi_.reserve(n * sizeof(CellHi<R>));
+    cell_slab_.reserve(n * CELL_SLAB * 4);
+    cell_last_.reserve(ncells_ * 4);
+    cell_wmax_.reserve(ncells_ * 4);
+    cell_ovf_.reserve(n * 4);
+    scan_tmp_.reserve(std::max<size_t>(std::max(scan_tmp_words(ntab), scan_tmp_words(slots)), 16) * 4);
+    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(n, kbits), 16) * 4);
+    uint32_t* tab = cell_tab_.as<uint32_t>();
+    mark(ST_BP_EMIT, true);
+    fill_u32(tab, 0u, ntab, s_);
+    hipLaunchKernelGGL(k_cell_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
+                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cg_, tab, cell_ckr_.as<uint64_t>());
+    scan_excl_u32(tab, tab, ntab, &st()->cells, scan_tmp_.as<uint32_t>(), s_);
+    mark(ST_BP_EMIT, false);
+    mark(ST_BP_SORT, true);
+    if (radix_sort_u64(cell_ckr_.as<uint64_t>(), cell_tmp_.as<uint64_t>(), n, 32, kbits, radix_tmp_.as<uint32_t>(), s_))
+      swap_bufs(cell_ckr_, cell_tmp_);
+    fill_u32(cell_last_.as<uint32_t>(), 0u, ncells_, s_);
+    hipLaunchKernelGGL(k_cell_gather<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s_, n,
+                       (const uint64_t*)cell_ckr_.as<uint64_t>(), NSC_, (const R*)aabb_.as<R>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cell_lo_.as<CellLo<R>>(), cell_hi_.as<CellHi<R>>(),
+                       cell_last_.as<uint32_t>());
+    const CellLo<R>* clo = cell_lo_.as<CellLo<R>>();
+    const CellHi<R>* chi = cell_hi_.as<CellHi<R>>();
+    hipLaunchKernelGGL(k_cell_window, dim3(blocks_for(ncells_, 256)), dim3(256), 0, s_, ncells_, cg_,
+                       (const uint32_t*)cell_last_.as<uint32_t>(), cell_wmax_.as<uint32_t>());
+    mark(ST_BP_SORT, false);
+    const uint32_t* wmx = cell_wmax_.as<uint32_t>();
+    mark(ST_BP_PAIRS, true);
+    // slots past the last shape take part in the scan: their counts are 0
+    if (slots > n) fill_u32(scount_.as<uint32_t>() + n, 0u, slots - n, s_);
+    hipLaunchKernelGGL(k_cell_find<R>, dim3(blocks_for(n, CELL_BS)), dim3(CELL_BS), 0, s_, n, clo, chi,
+                       (const uint32_t*)tab, wmx, cg_, scount_.as<uint32_t>(), cell_slab_.as<uint32_t>(),
+                       cell_ovf_.as<uint32_t>(), st());
+    mark(ST_BP_PAIRS, false);
+    mark(ST_BP_PAIRSCAN, true);
+    scan_excl_u32(scount_.as<uint32_t>(), soff_.as<uint32_t>(), slots, &st()->P, scan_tmp_.as<uint32_t>(), s_,
+                  &st()->nslots);
+    mark(ST_BP_PAIRSCAN, false);
+    read_status();
+    P_ = hstat_->P;
+    CDX_REQUIRE(hstat_->cells == n, ERR_HIP, "internal error: the cell table does not hold every shape");
+    CDX_REQUIRE(P_ < (1ull << 32), ERR_CAPACITY, "more than 2^32 candidate pairs");
+    pcap_ = std::max<uint64_t>(P_, 1);
+    pairs_.reserve(pcap_ * 8);
+    const uint64_t long_cap = pcap_ / (GROUP_LONG + 1) + 1;
+    longs_.reserve(long_cap * 4);
+    mark(ST_BP_PAIREMIT, true);
+    hipLaunchKernelGGL(k_cell_place, dim3(blocks_for(n, CELL_BS)), dim3(CELL_BS), 0, s_, n,
+                       (const uint32_t*)scount_.as<uint32_t>(), (const uint32_t*)soff_.as<uint32_t>(),
+                       (const uint32_t*)cell_slab_.as<uint32_t>(), pcap_, pairs_.as<uint64_t>(), st());
+    mark(ST_BP_PAIREMIT, false);
+    mark(ST_BP_PAIRSORT, true);
+    hipLaunchKernelGGL(k_cell_overflow<R>, dim3(256), dim3(CELL_BS), 0, s_, (const uint32_t*)cell_ovf_.as<uint32_t>(),
+                       clo, chi, (const uint32_t*)tab, wmx, cg_, (const uint32_t*)scount_.as<uint32_t>(),
+                       (const uint32_t*)soff_.as<uint32_t>(), pcap_, pairs_.as<uint64_t>(), longs_.as<uint32_t>(),
+                       long_cap, st());
+    hipLaunchKernelGGL(k_pair_groups_long, dim3(256), dim3(256), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
+                       (const DevStatus*)st(), (const uint32_t*)scount_.as<uint32_t>(),
+                       (const uint32_t*)soff_.as<uint32_t>(), pairs_.as<uint64_t>(),
+                       (const uint32_t*)longs_.as<uint32_t>(), long_cap);
+    mark(ST_BP_PAIRSORT, false);
+    cell_ = true;
+    bins_pending_ = true;
+    A_ = 0;
+    counts_.n = n; counts_.m = M_; counts_.a = 0; counts_.p = P_;
+    counts_.info = INFO_CELL_SEARCH | INFO_NO_ACTIVE_BINS;
+    counts_.key_max = hstat_->key_max;
+    counts_.bad = hstat_->bad;
+    SceneDesc<R> d;
+    CDX_HIP_CHECK(hipMemcpy(&d, desc_.p, sizeof(d), hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; ++c) { counts_.bmin[c] = ord2f(d.bmin_enc[c]); counts_.bmax[c] = ord2f(d.bmax_enc[c]); }
+  }
+
+  // Bin entries of a cell-search broadphase, built on request (they
+  // depend on the AABBs of that broadphase, so not after they change).
+  void ensure_bins() {
+    if (!bins_pending_) {
+      CDX_REQUIRE(!bins_lost_, ERR_NOT_READY,
+                  "the AABBs changed after a cell-search broadphase: run the broadphase again to read its bins");
+      return;
+    }
+    const bool prof = prof_;
+    prof_ = false;  // the stage times stay those of the broadphase
+    emit_and_sort_bins(exact_key_bits_);
+    prof_ = prof;
+    bins_pending_ = false;
   }
 
   void stage_narrowphase() {
@@ -1176,6 +1387,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     chunks_.reserve(chunk_cap_ * sizeof(Chunk));
     bpart_.reserve(chunk_cap_ * 6 * 4);
     kpart_.reserve(chunk_cap_ * 4);
+    wpart_.reserve(chunk_cap_ * 4);
     counts_buf_.reserve(chunk_cap_ * CHUNK * 4);
     offs_.reserve(chunk_cap_ * CHUNK * 4);
     scount_.reserve(chunk_cap_ * CHUNK * 4);
@@ -1369,6 +1581,15 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   bool opened_ = false;  // the step's totals were zeroed by one k_stage_begin
   bool mpr_in_tmp_ = false;  // the last reference-order MPR launch read queue_tmp_ (else queue_)
   uint64_t mpr_q_ = 0;       // and this many pairs of it
+  // cell search (exact mode)
+  int pair_search_ = PAIRS_AUTO;
+  bool cell_ = false;          // the last broadphase found its pairs by cells
+  bool bins_pending_ = false;  // its bin entries are not built yet
+  bool bins_lost_ = false;     // and can no longer be: the AABBs changed
+  int exact_key_bits_ = 0;
+  CellGrid cg_{};
+  uint64_t ncells_ = 0;
+  DevBuf wpart_, cell_tab_, cell_ckr_, cell_tmp_, cell_lo_, cell_hi_, cell_slab_, cell_last_, cell_wmax_, cell_ovf_;
 };
 
 }  // namespace

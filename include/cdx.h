@@ -89,6 +89,18 @@ extern "C" {
 #define CDX_STAGE_NP_MPR 14 /* the reference-order MPR launch alone (part of NP_RESOLVE) */
 #define CDX_STAGE_COUNT 15
 
+/* cdx_counts.flags */
+#define CDX_FLAG_UNSUPPORTED_SHAPE 0x1u
+#define CDX_FLAG_BAD_BODY_ID 0x2u
+#define CDX_FLAG_CELL_SEARCH 0x100u    /* the broadphase found its pairs by the cell search */
+#define CDX_FLAG_NO_ACTIVE_BINS 0x200u /* active_bins not computed: the AABBs changed after a
+                                          cell-search broadphase before its counts were read */
+
+/* Pair search of a scene's broadphase (cdx_scene_set_pair_search). */
+#define CDX_PAIRS_AUTO 0  /* cell search for near-uniform shape sizes, else bins (default) */
+#define CDX_PAIRS_BINS 1  /* the reference's bin decomposition always */
+#define CDX_PAIRS_CELLS 2 /* cell search wherever its grid fits */
+
 typedef struct cdx_ctx cdx_ctx;
 typedef struct cdx_scene cdx_scene;
 
@@ -100,7 +112,7 @@ typedef struct cdx_counts {
   uint64_t contacts;      /* number_of_contacts (Narrowphase.cu:580) */
   uint64_t resolved;      /* pairs re-solved in FP64 (CDX_NP_MIXED) */
   uint32_t key_max;       /* largest bin key emitted */
-  uint32_t flags;
+  uint32_t flags;         /* CDX_FLAG_* bits */
   float bmin[3], bmax[3]; /* scene bounds, float (Broadphase.cu:368-376) */
 } cdx_counts;
 
@@ -204,6 +216,16 @@ int cdx_scene_set_bins(cdx_scene* scene, int bins_x, int bins_y, int bins_z);
 int cdx_scene_set_envelope(cdx_scene* scene, double envelope);
 int cdx_scene_set_narrowphase_mode(cdx_scene* scene, int np_mode, double tau);
 int cdx_scene_set_profiling(cdx_scene* scene, int on);
+/* How the broadphase finds candidate pairs; the pair list is the same
+ * either way (every pair of overlapping shifted AABBs, ascending, as
+ * Broadphase.cu:484-535 leaves it).  The bins (Broadphase.cu:393-480) walk
+ * every bin each AABB spans.  The cell search keys each shape once, by the
+ * cell of its AABB's minimum corner in an internal grid of cells at least as
+ * wide as the widest AABB, and tests the 27 cells around it; its bin counts
+ * and entry total are still computed, while the sorted entries, active
+ * bins and occupancy are built when cdx_scene_read_bins or
+ * cdx_scene_counts asks for them (before the AABBs change). */
+int cdx_scene_set_pair_search(cdx_scene* scene, int mode);
 
 int cdx_scene_aabb(cdx_scene* scene);
 int cdx_scene_broadphase(cdx_scene* scene);

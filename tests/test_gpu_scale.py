@@ -29,17 +29,29 @@ def _lattice(scenes, n):
 
 
 def test_unit_lattice_closed_forms(cdx, ctx, scenes):
+    """Both pair searches at full size: closed-form totals, identical pair lists."""
     n = 210
     s = _lattice(scenes, n)
-    sc = cdx.Scene(ctx, s.dtype).load(s)
-    sc.step()
-    sc.sync()
-    c = sc.counts()
     m = n - 1
     pairs = 3 * m * n * n + 6 * m * m * n + 4 * m ** 3
     contacts = 3 * m * n * n
-    print(f"n={n}: {c.pairs} pairs (closed form {pairs}), {c.contacts} contacts (closed form {contacts}), "
-          f"{c.bin_entries} bin entries", flush=True)
-    assert c.pairs == pairs
-    assert c.contacts == contacts
-    sc.close()
+    lists = {}
+    for mode, name in ((cdx.PAIRS_CELLS, "cells"), (cdx.PAIRS_BINS, "bins")):
+        sc = cdx.Scene(ctx, s.dtype).load(s)
+        sc.set_pair_search(mode)
+        sc.step()
+        sc.sync()
+        c = sc.counts()
+        print(f"n={n} {name}: {c.pairs} pairs (closed form {pairs}), {c.contacts} contacts (closed form {contacts}), "
+              f"{c.bin_entries} bin entries, {c.active_bins} active bins", flush=True)
+        assert bool(c.flags & cdx.FLAG_CELL_SEARCH) == (name == "cells")
+        assert c.pairs == pairs
+        assert c.contacts == contacts
+        lists[name] = sc.read_pairs()
+        if name == "bins":
+            assert c.active_bins == cells_active
+        else:
+            cells_active = c.active_bins  # built on request from the cell search's AABBs
+        sc.close()
+    assert np.array_equal(lists["cells"], lists["bins"])
+    assert np.all(np.diff(lists["cells"]) > 0)
