@@ -595,7 +595,7 @@ def served_leg(pkg, scenes, ctx, args):
     latency runs from the submit call at the arrival time to the result being
     ready on the host.
     The plain per-scene path runs the same requests one at a time on
-    resident cdx_scene objects (pose upload + step + contact count).
+    resident cdx_scene objects (pose upload + step + contact readback).
     """
     import threading
     count = args.served_scenes
@@ -610,7 +610,8 @@ def served_leg(pkg, scenes, ctx, args):
         return s, p1, r1
 
     srv = pkg.Server(ctx, np.float64, mode=pkg.NP_MIXED, max_batch_shapes=args.served_batch_shapes,
-                     max_pending=args.served_max_pending, use_graph=not args.served_eager)
+                     max_pending=args.served_max_pending, use_graph=not args.served_eager,
+                     lean_shapes=args.lean_shapes)
     poses = [None] * count
     t0 = time.perf_counter()
     from concurrent.futures import ThreadPoolExecutor
@@ -649,6 +650,7 @@ def served_leg(pkg, scenes, ctx, args):
         sc.update_bodies(idx, p1, r1)
         sc.step()
         sc.sync()
+        sc.read_contacts()  # a request returns its contacts, on both paths
         plain.append(time.perf_counter() - t1)
         sc.close()
     plain = np.array(plain)
@@ -763,6 +765,10 @@ def served_leg(pkg, scenes, ctx, args):
               "steps": int(st["steps"]), "requests_per_step": round(st["stepped_requests"] / max(st["steps"], 1), 2),
               "device_busy_frac": round(st["busy_ms"] / 1e3 / wall, 3),
               "worker_host_ms_per_step": {k: round(v / max(st["steps"], 1), 3) for k, v in st["host_ms"].items()},
+              "classes": [{"completed": int(k["completed"]), "steps": int(k["steps"]),
+                           "requests_per_step": round(k["completed"] / max(k["steps"], 1), 2),
+                           "p50_ms": round(k["p50_ms"], 3), "p99_ms": round(k["p99_ms"], 3),
+                           "busy_ms_per_step": round(k["busy_ms"] / max(k["steps"], 1), 3)} for k in st["classes"]],
               "submit_lag_p99_ms": round(float(np.percentile(lag, 99)) * 1e3, 3),
               "submit_end_lag_s": round(t_submit_end - float(arrivals[-1]), 3)}
         log(f"served rate {rate}: {pt}")
@@ -831,6 +837,8 @@ def main():
     ap.add_argument("--served-max-pending", type=int, default=4096,
                     help="queued requests before the server refuses new ones")
     ap.add_argument("--served-eager", action="store_true", help="launch served steps without HIP graphs")
+    ap.add_argument("--lean-shapes", type=int, default=-1,
+                    help="largest lean step in shapes for the graph path and the server (-1: library default, 0: never)")
     ap.add_argument("--plain-sample", type=int, default=256, help="scenes timed on the plain per-scene path")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (calibrates HBM byte counters)")
@@ -886,6 +894,8 @@ def main():
     b1 = pkg.Batch(ctx, s.dtype)
     b1.add_scene(s)
     b1.set_mode(mode, 1e-2)
+    if args.lean_shapes >= 0:
+        b1.set_lean_shapes(args.lean_shapes)
     for _ in range(max(args.warmup, 2)):
         b1.step(True)
         gc = b1.finish()

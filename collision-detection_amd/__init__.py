@@ -45,7 +45,8 @@ EXPORTS = [
     "cdx_scene_stage_ms",
     "cdx_scene_device_bytes", "cdx_batch_create", "cdx_batch_destroy", "cdx_batch_last_error",
     "cdx_batch_add_scene", "cdx_batch_scene_bins", "cdx_batch_scene_count", "cdx_batch_update_bodies",
-    "cdx_batch_set_envelope", "cdx_batch_set_narrowphase_mode", "cdx_batch_select", "cdx_batch_step",
+    "cdx_batch_set_envelope", "cdx_batch_set_narrowphase_mode", "cdx_batch_set_lean_shapes", "cdx_batch_select",
+    "cdx_batch_step",
     "cdx_batch_finish", "cdx_batch_scene_counts", "cdx_batch_read_pairs", "cdx_batch_read_contacts",
     "cdx_batch_step_ms", "cdx_batch_device_bytes", "cdx_batch_scene_bytes", "cdx_server_create",
     "cdx_server_destroy", "cdx_server_last_error", "cdx_server_add_scene", "cdx_server_scene_info",
@@ -59,7 +60,7 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("real_kind", C.c_int), ("np_mode", C.c_int), ("tau", C.c_double), ("envelope", C.c_double),
         ("hbm_budget", C.c_uint64), ("max_batch_shapes", C.c_uint64), ("max_pending", C.c_uint64),
-        ("use_graph", C.c_int),
+        ("use_graph", C.c_int), ("lean_shapes", C.c_int64),
     ]
 
 
@@ -68,11 +69,16 @@ class ServerStats(C.Structure):
         ("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("steps", C.c_uint64),
         ("stepped_requests", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
         ("max_ms", C.c_double), ("busy_ms", C.c_double), ("window_s", C.c_double), ("host_ms", C.c_double * 6),
+        ("class_completed", C.c_uint64 * 4), ("class_steps", C.c_uint64 * 4), ("class_p50_ms", C.c_double * 4),
+        ("class_p99_ms", C.c_double * 4), ("class_busy_ms", C.c_double * 4),
     ]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "host_ms"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "host_ms" and not k.startswith("class_")}
         d["host_ms"] = dict(zip(["upload", "select", "launch", "wait", "block", "readback"], list(self.host_ms)))
+        d["classes"] = [{"completed": self.class_completed[c], "steps": self.class_steps[c],
+                         "p50_ms": self.class_p50_ms[c], "p99_ms": self.class_p99_ms[c],
+                         "busy_ms": self.class_busy_ms[c]} for c in range(4)]
         return d
 
 
@@ -161,6 +167,7 @@ def lib():
         "cdx_batch_update_bodies": ([vp, C.c_uint32, u64, vp, vp, vp], i32),
         "cdx_batch_set_envelope": ([vp, C.c_double], i32),
         "cdx_batch_set_narrowphase_mode": ([vp, i32, C.c_double], i32),
+        "cdx_batch_set_lean_shapes": ([vp, C.c_uint64], i32),
         "cdx_batch_select": ([vp, vp, u64], i32),
         "cdx_batch_step": ([vp, i32], i32),
         "cdx_batch_finish": ([vp, vp], i32),
@@ -471,6 +478,10 @@ class Batch:
     def set_mode(self, mode, tau=1e-2):
         self._chk(lib().cdx_batch_set_narrowphase_mode(self.h, int(mode), float(tau)))
 
+    def set_lean_shapes(self, n):
+        """Steps over at most n shapes run lean (fewer launches, same results); 0: never."""
+        self._chk(lib().cdx_batch_set_lean_shapes(self.h, int(n)))
+
     def select(self, scenes=None):
         if scenes is None:
             self._chk(lib().cdx_batch_select(self.h, None, 0))
@@ -527,11 +538,11 @@ class Server:
     """Served request loop over device-resident scenes (cdx_server)."""
 
     def __init__(self, ctx: Context, dtype=np.float64, mode=NP_MIXED, tau=1e-2, envelope=0.0, hbm_budget=0,
-                 max_batch_shapes=0, max_pending=0, use_graph=True):
+                 max_batch_shapes=0, max_pending=0, use_graph=True, lean_shapes=-1):
         self.ctx = ctx
         self.dtype = np.dtype(dtype)
         cfg = ServerConfig(_kind(dtype), int(mode), float(tau), float(envelope), int(hbm_budget),
-                           int(max_batch_shapes), int(max_pending), 1 if use_graph else 0)
+                           int(max_batch_shapes), int(max_pending), 1 if use_graph else 0, int(lean_shapes))
         self.h = C.c_void_p()
         ctx._chk(lib().cdx_server_create(ctx.h, C.byref(cfg), C.byref(self.h)))
 

@@ -433,6 +433,7 @@ int cdx_batch_set_envelope(cdx_batch* batch, double envelope) { BATCH_CALL(b.set
 int cdx_batch_set_narrowphase_mode(cdx_batch* batch, int np_mode, double tau) {
   BATCH_CALL(CDX_REQUIRE(tau >= 0.0, CDX_ERR_INVALID, "tau must be non-negative"); b.set_np_mode(np_mode, tau));
 }
+int cdx_batch_set_lean_shapes(cdx_batch* batch, uint64_t max_shapes) { BATCH_CALL(b.set_lean_shapes(max_shapes)); }
 int cdx_batch_select(cdx_batch* batch, const uint32_t* scenes, uint64_t k) {
   BATCH_CALL(CDX_REQUIRE(scenes || k == 0, CDX_ERR_INVALID, "null scene list"); b.select(scenes, k));
 }
@@ -484,6 +485,7 @@ int cdx_server_create(cdx_ctx* ctx, const cdx_server_config* cfg, cdx_server** o
     if (cfg->max_batch_shapes) c.max_batch_shapes = cfg->max_batch_shapes;
     if (cfg->max_pending) c.max_pending = cfg->max_pending;
     c.use_graph = cfg->use_graph != 0;
+    if (cfg->lean_shapes >= 0) c.lean_shapes = (uint64_t)cfg->lean_shapes;
     auto sv = std::make_unique<cdx_server>();
     sv->srv = std::make_unique<cdx::Server>(ctx->device, c);
     *out = sv.release();
@@ -550,6 +552,11 @@ int cdx_server_stats(cdx_server* server, cdx_server_stats_t* out, int reset) {
               out->steps = s.batches; out->stepped_requests = s.batched_requests; out->p50_ms = s.p50_ms;
               out->p99_ms = s.p99_ms; out->mean_ms = s.mean_ms; out->max_ms = s.max_ms; out->busy_ms = s.busy_ms;
               out->window_s = s.window_s; for (int q = 0; q < 6; ++q) out->host_ms[q] = s.host_ms[q];
+              for (int c = 0; c < 4; ++c) {
+                out->class_completed[c] = s.class_completed[c]; out->class_steps[c] = s.class_steps[c];
+                out->class_p50_ms[c] = s.class_p50_ms[c]; out->class_p99_ms[c] = s.class_p99_ms[c];
+                out->class_busy_ms[c] = s.class_busy_ms[c];
+              }
               if (reset) sv.reset_stats());
 }
 int cdx_server_memory(cdx_server* server, uint64_t* hbm_budget, uint64_t* hbm_committed, uint64_t* device_bytes) {

@@ -450,21 +450,17 @@ template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(
 // to the reference-order MPR.  In those modes the front pass already applies
 // the FP32 MPR's first portal test, so pairs separated along the centre
 // line (most candidate pairs) never enter the queue.
+// One candidate pair of the front pass: its hit flag cleared, the early
+// exits and the sphere-sphere contact applied, and (return value) whether it
+// needs the MPR.  ta / tb receive the shape types.
 template <class R>
-__global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* __restrict__ d_np,
-                                                  const uint64_t* __restrict__ pairs,
-                                                  const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
-                                                  const ShapeRecF* __restrict__ frec, float tau_rel, int prefilter,
-                                                  R envelope, PairOut<R> o, uint32_t* __restrict__ qsel,
-                                                  uint8_t* __restrict__ qcls) {
+__device__ __forceinline__ bool np_front_one(uint64_t i, uint32_t ia, uint32_t ib,
+                                             const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
+                                             const ShapeRecF* __restrict__ frec, float tau_rel, int prefilter,
+                                             R envelope, const PairOut<R>& o, uint32_t& ta, uint32_t& tb) {
   const uint64_t P = o.P;
-  uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
-  if (i >= dev_count(cap, d_np)) return;
-  uint64_t pr = pairs[i];
-  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
   o.hit[i] = 0u;
   bool enq = false;
-  uint32_t ta, tb;
   auto spheres = [&]() {
     Contact<R> c;
     ShapeRec<R> ra = rec[ia], rb = rec[ib];
@@ -508,6 +504,22 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
       else enq = true;
     }
   }
+  return enq;
+}
+
+template <class R>
+__global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* __restrict__ d_np,
+                                                  const uint64_t* __restrict__ pairs,
+                                                  const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
+                                                  const ShapeRecF* __restrict__ frec, float tau_rel, int prefilter,
+                                                  R envelope, PairOut<R> o, uint32_t* __restrict__ qsel,
+                                                  uint8_t* __restrict__ qcls) {
+  uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (i >= dev_count(cap, d_np)) return;
+  uint64_t pr = pairs[i];
+  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+  uint32_t ta, tb;
+  const bool enq = np_front_one(i, ia, ib, rec, hdr, frec, tau_rel, prefilter, envelope, o, ta, tb);
   qsel[i] = enq ? 1u : 0u;
   qcls[i] = (uint8_t)pair_class(ta, tb);
 }
@@ -515,24 +527,13 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
 // FP32 MPR over the class-sorted queue (MIXED / FAST), in the frame of
 // shape A: clear misses end here; FAST keeps clear FP32 hits; everything
 // else is selected for the reference-order MPR.
+// FP32 MPR of one queued pair of the five filtered types: returns whether
+// it goes on to the reference-order MPR (KEEP: FAST, which stores clear FP32
+// hits itself).
 template <class R, bool KEEP>
-__global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t* __restrict__ d_q,
-                                                   const uint64_t* __restrict__ queue,
-                                                   const uint64_t* __restrict__ pairs,
-                                                   const ShapeHdr* __restrict__ hdr,
-                                                   const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
-                                                   PairOut<R> o, uint32_t* __restrict__ sel) {
-  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
-  if (j >= dev_count(cap, d_q)) return;
+__device__ __forceinline__ bool np_filter_one(uint64_t i, const ShapeRecF& fa, const ShapeRecF& fb, R envelope,
+                                              float tau_rel, const PairOut<R>& o) {
   const uint64_t P = o.P;
-  const uint32_t i = (uint32_t)queue[j];
-  const uint64_t pr = pairs[i];
-  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
-  const ShapeRecF fa = frec[ia], fb = frec[ib];
-  if (!fp32_filtered(fa.type) || !fp32_filtered(fb.type)) {
-    sel[j] = 1u;  // rectangles, cones and triangles: the reference-order MPR decides
-    return;
-  }
   PlacedF A = placed_local(fa, fa), B = placed_local(fb, fa);
   float tau = tau_rel * (bound_radius(A.type, A.B) + bound_radius(B.type, B.B));
   v3<float> N, p0;
@@ -563,7 +564,27 @@ __global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t*
   }
   // MIXED: FP32 settles clear misses only; hits take the reference path,
   // whose capped iteration sequence FP32 cannot reproduce.
-  sel[j] = (r == MPR_UNSURE || (r == MPR_HIT && !KEEP)) ? 1u : 0u;
+  return r == MPR_UNSURE || (r == MPR_HIT && !KEEP);
+}
+
+template <class R, bool KEEP>
+__global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t* __restrict__ d_q,
+                                                   const uint64_t* __restrict__ queue,
+                                                   const uint64_t* __restrict__ pairs,
+                                                   const ShapeHdr* __restrict__ hdr,
+                                                   const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
+                                                   PairOut<R> o, uint32_t* __restrict__ sel) {
+  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (j >= dev_count(cap, d_q)) return;
+  const uint32_t i = (uint32_t)queue[j];
+  const uint64_t pr = pairs[i];
+  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+  const ShapeRecF fa = frec[ia], fb = frec[ib];
+  if (!fp32_filtered(fa.type) || !fp32_filtered(fb.type)) {
+    sel[j] = 1u;  // rectangles, cones and triangles: the reference-order MPR decides
+    return;
+  }
+  sel[j] = np_filter_one<R, KEEP>(i, fa, fb, envelope, tau_rel, o) ? 1u : 0u;
 }
 
 // Stable compaction of the selected queue entries (keeps the class order).
@@ -585,6 +606,24 @@ __global__ void k_queue_compact(uint64_t cap, const uint64_t* __restrict__ d_np,
   queue[qpos[i]] = ((uint64_t)qcls[i] << 32) | i;
 }
 
+// MPR on the reference path for one pair, in compute type CT; a hit is
+// stored in the scene's real type R.
+template <class R, class CT, bool EXT>
+__device__ __forceinline__ void np_resolve_one(uint64_t i, uint32_t ia, uint32_t ib,
+                                               const ShapeRec<R>* __restrict__ rec, R envelope, const PairOut<R>& o) {
+  ShapeRec<CT> ra = as_compute<CT, R>(rec[ia]);
+  ShapeRec<CT> rb = as_compute<CT, R>(rec[ib]);
+  Contact<CT> c;
+  if (contact_native<EXT>(ra, rb, (CT)envelope, c)) {
+    Contact<R> cr;
+    cr.N = mk3((R)c.N.x, (R)c.N.y, (R)c.N.z);
+    cr.p1 = mk3((R)c.p1.x, (R)c.p1.y, (R)c.p1.z);
+    cr.p2 = mk3((R)c.p2.x, (R)c.p2.y, (R)c.p2.z);
+    cr.depth = (R)c.depth;
+    store_pair(o, o.P, i, cr, ra.body, rb.body);
+  }
+}
+
 // MPR on the reference path for the queued pairs, in compute type CT
 // (the scene's real type for NATIVE, FP64 for MIXED / FAST).
 // EXT: the launch for the pairs of class EXT_CLASS and above (a rectangle,
@@ -597,21 +636,10 @@ __global__ __launch_bounds__(128) void k_np_resolve(uint64_t cap, const uint64_t
                                                     const ShapeRec<R>* __restrict__ rec, R envelope, PairOut<R> o,
                                                     const uint64_t* __restrict__ queue) {
   const uint64_t q = dev_count(cap, d_q);
-  const uint64_t P = o.P;
   auto one = [&](uint64_t e) {
     const uint32_t i = (uint32_t)e;
     const uint64_t pr = pairs[i];
-    ShapeRec<CT> ra = as_compute<CT, R>(rec[(uint32_t)(pr >> 32)]);
-    ShapeRec<CT> rb = as_compute<CT, R>(rec[(uint32_t)(pr & 0xffffffffu)]);
-    Contact<CT> c;
-    if (contact_native<EXT>(ra, rb, (CT)envelope, c)) {
-      Contact<R> cr;
-      cr.N = mk3((R)c.N.x, (R)c.N.y, (R)c.N.z);
-      cr.p1 = mk3((R)c.p1.x, (R)c.p1.y, (R)c.p1.z);
-      cr.p2 = mk3((R)c.p2.x, (R)c.p2.y, (R)c.p2.z);
-      cr.depth = (R)c.depth;
-      store_pair(o, P, i, cr, ra.body, rb.body);
-    }
+    np_resolve_one<R, CT, EXT>(i, (uint32_t)(pr >> 32), (uint32_t)(pr & 0xffffffffu), rec, envelope, o);
   };
   const uint64_t t = (uint64_t)blockIdx.x * 128 + threadIdx.x;
   if constexpr (!EXT) {
@@ -627,6 +655,100 @@ __global__ __launch_bounds__(128) void k_np_resolve(uint64_t cap, const uint64_t
       if ((uint32_t)(e >> 32) < EXT_CLASS) return;
       one(e);
     }
+  }
+}
+
+// The whole narrowphase of a step with few pairs (a lean step) in one
+// kernel.  Each workgroup takes FUSED_BS consecutive pairs through the front
+// pass, then the FP32 filter (MIXED / FAST) over its queued pairs and the
+// reference-order MPR over the survivors, each phase in shape-pair class
+// order (an LDS counting sort of the workgroup's pairs), so a wavefront
+// mostly runs one pair of support maps, as the class-sorted queue of the full
+// step arranges device-wide.  Every pair goes through the same per-pair
+// functions as in the queued kernels, so its result is the same bit for bit;
+// only which lane does the work differs.  KEEP: FAST.  The queue totals
+// (DevStatus Q: pairs past the front pass; Q2: pairs re-solved on the
+// reference path) are added once per workgroup.
+constexpr int FUSED_BS = 256;
+constexpr uint32_t NUM_CLASSES = 128;  // pair_class() < 128
+
+// Orders the threads whose pred holds by class: s_list[0..returned count)
+// are their thread indices, classes ascending.  Every thread must call it.
+__device__ __forceinline__ uint32_t block_class_order(bool pred, uint32_t cls, uint32_t* s_cnt, uint16_t* s_list,
+                                                      uint32_t* s_total) {
+  for (uint32_t c = threadIdx.x; c < NUM_CLASSES; c += FUSED_BS) s_cnt[c] = 0u;
+  __syncthreads();
+  const uint32_t rank = pred ? atomicAdd(&s_cnt[cls], 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x < WAVE) {
+    const uint32_t l = threadIdx.x;
+    const uint32_t a = s_cnt[2 * l], b = s_cnt[2 * l + 1];
+    const uint32_t inc = wave_incl_scan(a + b);
+    s_cnt[2 * l] = inc - a - b;
+    s_cnt[2 * l + 1] = inc - b;
+    if (l == WAVE - 1) *s_total = inc;
+  }
+  __syncthreads();
+  if (pred) s_list[s_cnt[cls] + rank] = (uint16_t)threadIdx.x;
+  __syncthreads();
+  return *s_total;
+}
+
+template <class R, class CT, bool FILTER, bool KEEP>
+__global__ __launch_bounds__(FUSED_BS) void k_np_fused(uint64_t cap, const uint64_t* __restrict__ d_np,
+                                                       const uint64_t* __restrict__ pairs,
+                                                       const ShapeRec<R>* __restrict__ rec,
+                                                       const ShapeHdr* __restrict__ hdr,
+                                                       const ShapeRecF* __restrict__ frec, float tau_rel, R envelope,
+                                                       PairOut<R> o, DevStatus* __restrict__ st) {
+  static_assert(2 * WAVE == NUM_CLASSES, "one wavefront scans the class counts, two per lane");
+  __shared__ uint32_t s_cnt[NUM_CLASSES];
+  __shared__ uint16_t s_list[FUSED_BS];
+  __shared__ uint8_t s_cls[FUSED_BS];
+  __shared__ uint8_t s_go[FUSED_BS];  // 0 done, 1 FP32 filter next, 2 reference-order MPR next
+  __shared__ uint32_t s_total;
+  const uint64_t n = dev_count(cap, d_np);
+  const uint64_t base = (uint64_t)blockIdx.x * FUSED_BS;
+  if (base >= n) return;  // the whole workgroup
+  const uint32_t t = threadIdx.x;
+  const uint64_t i = base + t;
+  uint32_t cls = 0, go = 0;
+  if (i < n) {
+    const uint64_t pr = pairs[i];
+    uint32_t ta, tb;
+    const bool enq = np_front_one(i, (uint32_t)(pr >> 32), (uint32_t)(pr & 0xffffffffu), rec, hdr, frec, tau_rel,
+                                  FILTER ? 1 : 0, envelope, o, ta, tb);
+    cls = pair_class(ta, tb);
+    go = enq ? (FILTER && cls < EXT_CLASS ? 1u : 2u) : 0u;
+  }
+  s_cls[t] = (uint8_t)cls;
+  s_go[t] = (uint8_t)go;
+  const int queued = __syncthreads_count(go != 0);
+  if constexpr (FILTER) {
+    const uint32_t m = block_class_order(go == 1, cls, s_cnt, s_list, &s_total);
+    if (t < m) {
+      const uint32_t li = s_list[t];
+      const uint64_t pi = base + li;
+      const uint64_t pr = pairs[pi];
+      s_go[li] = np_filter_one<R, KEEP>(pi, frec[(uint32_t)(pr >> 32)], frec[(uint32_t)(pr & 0xffffffffu)], envelope,
+                                        tau_rel, o)
+                     ? 2u
+                     : 0u;
+    }
+    __syncthreads();
+  }
+  const uint32_t m2 = block_class_order(s_go[t] == 2, s_cls[t], s_cnt, s_list, &s_total);
+  if (t == 0) {
+    if (queued) atomicAdd((unsigned long long*)&st->Q, (unsigned long long)queued);
+    if (m2) atomicAdd((unsigned long long*)&st->Q2, (unsigned long long)m2);
+  }
+  if (t < m2) {
+    const uint32_t li = s_list[t];
+    const uint64_t pi = base + li;
+    const uint64_t pr = pairs[pi];
+    const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+    if (s_cls[li] >= EXT_CLASS) np_resolve_one<R, CT, true>(pi, ia, ib, rec, envelope, o);
+    else np_resolve_one<R, CT, false>(pi, ia, ib, rec, envelope, o);
   }
 }
 

@@ -186,7 +186,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     GateShared gate;
     (void)hipSetDevice(dev_);
     (void)hipStreamSynchronize(s_);
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    for (hipGraphExec_t& g : graph_exec_)
+      if (g) (void)hipGraphExecDestroy(g);
     for (int i = 0; i < 2 * ST_COUNT + 2; ++i) (void)hipEventDestroy(ev_[i]);
     (void)hipHostFree(hstat_);
     (void)hipHostFree(hhdr_);
@@ -280,6 +281,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     if (mode != mode_ || (float)tau != tau_) graph_valid_ = false;
     mode_ = mode;
     tau_ = (float)tau;
+  }
+  void set_lean_shapes(uint64_t n) override {
+    if (n != lean_shapes_) graph_valid_ = false;
+    lean_shapes_ = n;
   }
   void set_profiling(bool on) override { prof_ = on; }
   void set_pair_search(int mode) override {
@@ -990,6 +995,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
   }
 
+  // Exclusive scan on the step stream: one workgroup in a lean step.
+  void scan(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, const uint64_t* d_n) {
+    if (lean_) scan_excl_u32_one(in, out, n, d_total, s_, d_n);
+    else scan_excl_u32(in, out, n, d_total, scan_tmp_.as<uint32_t>(), s_, d_n);
+  }
+
   void stage_aabb() {
     aabbs_changed();
     if (!opened_)
@@ -1029,8 +1040,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        exact_ ? (const uint32_t*)wpart_.as<uint32_t>() : nullptr, st());
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
-    scan_excl_u32(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, scan_tmp_.as<uint32_t>(), s_,
-                  &st()->nslots);
+    scan(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, &st()->nslots);
     mark(ST_BP_SCAN, false);
     int key_bits = key_bits_;
     cell_ = false;
@@ -1066,8 +1076,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
                        (const uint32_t*)counts_buf_.as<uint32_t>(), (const uint32_t*)offs_.as<uint32_t>(),
                        (const uint32_t*)pcount_.as<uint32_t>(), mcap_, scount_.as<uint32_t>());
-    scan_excl_u32(scount_.as<uint32_t>(), soff_.as<uint32_t>(), slots, &st()->P, scan_tmp_.as<uint32_t>(), s_,
-                  &st()->nslots);
+    scan(scount_.as<uint32_t>(), soff_.as<uint32_t>(), slots, &st()->P, &st()->nslots);
     mark(ST_BP_PAIRSCAN, false);
     if (exact_) {
       read_status();
@@ -1122,7 +1131,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits), 16) * 4);
     mark(ST_BP_SORT, true);
     if (radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, 32, key_bits, radix_tmp_.as<uint32_t>(),
-                       s_, &st()->M))
+                       s_, &st()->M, lean_))
       swap_bufs(keys_, keys_tmp_);
     mark(ST_BP_SORT, false);
   }
@@ -1292,6 +1301,29 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(P, CLASS_BITS), 16) * 4);
     PairOut<R> o{np_v_.as<PairRec<R>>(), np_hit_.as<uint32_t>(), P};
     const uint64_t* dP = &st()->P;
+    if (lean_) {
+      // a lean step: every pair's narrowphase in one kernel, no queue
+      auto fused = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(blocks_for(P, FUSED_BS)), dim3(FUSED_BS), 0, s_, P, dP,
+                           (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(),
+                           (const ShapeHdr*)hdr_.as<ShapeHdr>(), (const ShapeRecF*)frec_.as<ShapeRecF>(), tau_,
+                           envelope_, o, st());
+      };
+      mark(ST_NP_RESOLVE, true);
+      if (mode_ == NP_NATIVE) fused(k_np_fused<R, R, false, false>);
+      else if (mode_ == NP_FAST) fused(k_np_fused<R, double, true, true>);
+      else fused(k_np_fused<R, double, true, false>);
+      mark(ST_NP_RESOLVE, false);
+      mark(ST_NP_COMPACT, true);
+      scan(np_hit_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->C, dP);
+      out_.reserve(P * (10 * sizeof(R) + 8));
+      perm_.reserve(P * 8);
+      hipLaunchKernelGGL(k_np_compact<R>, dim3(blocks_for(P, COMPACT_BS)), dim3(COMPACT_BS), 0, s_, P, dP, o,
+                         (const uint32_t*)np_pos_.as<uint32_t>(), (const uint64_t*)pairs_.as<uint64_t>(),
+                         (const DevStatus*)st(), out_.as<R>(), perm_.as<uint64_t>());
+      mark(ST_NP_COMPACT, false);
+      return;
+    }
     mark(ST_NP_MAIN, true);
     hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, dP,
                        (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(),
@@ -1428,17 +1460,31 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   void launch_capacity_step(bool use_graph) {
     prepare_capacity_buffers();  // every buffer is sized before recording
+    const int v = total_selected_shapes() <= lean_shapes_ ? 1 : 0;
+    struct Lean {  // also on an exception
+      bool& f;
+      Lean(bool& f_, bool on) : f(f_) { f = on; }
+      ~Lean() { f = false; }
+    };
     CDX_HIP_CHECK(hipEventRecord(ev_[2 * ST_COUNT], s_));
     if (use_graph) {
-      if (!graph_valid_) {
-        if (graph_exec_) {
-          CDX_HIP_CHECK(hipGraphExecDestroy(graph_exec_));
-          graph_exec_ = nullptr;
+      if (!graph_valid_ || !graph_exec_[v]) {
+        // Destroy, record and instantiate with no other library thread inside
+        // HIP: other size classes launch and synchronise their own graphs
+        // concurrently otherwise, and this stream's last launch of an old
+        // graph must have ended before its executable is freed.
+        GateExclusive gate;
+        CDX_HIP_CHECK(hipStreamSynchronize(s_));
+        if (!graph_valid_) {
+          for (hipGraphExec_t& g : graph_exec_) {
+            if (g) CDX_HIP_CHECK(hipGraphExecDestroy(g));
+            g = nullptr;
+          }
         }
         hipGraph_t g = nullptr;
-        GateExclusive gate;  // no other thread calls HIP while this stream records
         CDX_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         try {
+          Lean lean(lean_, v == 1);
           enqueue_capacity_step();
         } catch (...) {
           // close the capture so the stream stays usable, then report
@@ -1446,14 +1492,15 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
           throw;
         }
         CDX_HIP_CHECK(hipStreamEndCapture(s_, &g));
-        CDX_HIP_CHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+        CDX_HIP_CHECK(hipGraphInstantiate(&graph_exec_[v], g, nullptr, nullptr, 0));
         CDX_HIP_CHECK(hipGraphDestroy(g));
         graph_valid_ = true;
         graph_key_bits_ = key_bits_;
         graph_shape_bits_ = shape_bits_;
       }
-      CDX_HIP_CHECK(hipGraphLaunch(graph_exec_, s_));
+      CDX_HIP_CHECK(hipGraphLaunch(graph_exec_[v], s_));
     } else {
+      Lean lean(lean_, v == 1);
       enqueue_capacity_step();
     }
     CDX_HIP_CHECK(hipEventRecord(ev_[2 * ST_COUNT + 1], s_));
@@ -1554,7 +1601,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   std::vector<SceneOut> souts_host_;
 
   // graph of a capacity step
-  hipGraphExec_t graph_exec_ = nullptr;
+  // recorded steps: [0] the full step, [1] the lean step (one-workgroup
+  // scans, fused narrowphase) for selections of at most lean_shapes_ shapes
+  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+  bool lean_ = false;  // the step being enqueued is lean
+  uint64_t lean_shapes_ = kLeanShapes;
   bool graph_valid_ = false;
   int graph_key_bits_ = -1, graph_shape_bits_ = -1;
   bool pending_step_ = false, pending_graph_ = false;

@@ -32,6 +32,7 @@ namespace {
 // steps on its own stream; four match the device's hardware queues.
 constexpr uint64_t kClassHi[] = {8192, 49152, 131072, ~0ull};
 constexpr int kNumClasses = 4;
+static_assert(kNumClasses == 4, "the per-class statistics arrays hold four classes");
 // A class's steps hold at most this many scenes of its largest size (and
 // never more than max_batch_shapes shapes): step buffers, and so every
 // launch grid of a recorded step, are sized by it, and a step of a few
@@ -181,6 +182,7 @@ Server::Server(int device, const ServerConfig& cfg) : dev_(device), cfg_(cfg), t
     k.eng = make_batch(cfg.real_kind, dev_);
     k.eng->set_np_mode(cfg.np_mode, cfg.tau);
     k.eng->set_envelope(cfg.envelope);
+    k.eng->set_lean_shapes(cfg.lean_shapes);
   }
   st_t0_ = now_ms();
   for (int c = 0; c < kNumClasses; ++c) classes_[c]->worker = std::thread(&Server::worker_loop, this, c);
@@ -532,6 +534,7 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
         r->state = 2;
         r->latency_ms = t - r->t_submit;
         lat_.push_back(r->latency_ms);
+        lat_cls_[c].push_back(r->latency_ms);
         ++st_done_;
       } else {
         r->state = 3;
@@ -540,8 +543,10 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
       }
     }
     ++st_batches_;
+    ++st_steps_cls_[c];
     st_breqs_ += batch.size();
     st_busy_ += step_ms;
+    st_busy_cls_[c] += step_ms;
     for (int q = 0; q < 6; ++q) st_phase_[q] += ph[q];
   }
   cv_done_.notify_all();
@@ -625,19 +630,32 @@ ServerStats Server::stats() {
   s.busy_ms = st_busy_;
   for (int q = 0; q < 6; ++q) s.host_ms[q] = st_phase_[q];
   s.window_s = (now_ms() - st_t0_) / 1000.0;
+  // nearest-rank quantile of a sorted sample
+  auto q = [](const std::vector<double>& v, double f) {
+    size_t i = (size_t)std::ceil(f * (double)v.size()) - 1;
+    return v[std::min(i, v.size() - 1)];
+  };
   if (!lat_.empty()) {
     std::vector<double> v = lat_;
     std::sort(v.begin(), v.end());
-    auto q = [&](double f) {
-      size_t i = (size_t)std::ceil(f * (double)v.size()) - 1;
-      return v[std::min(i, v.size() - 1)];
-    };
-    s.p50_ms = q(0.50);
-    s.p99_ms = q(0.99);
+    s.p50_ms = q(v, 0.50);
+    s.p99_ms = q(v, 0.99);
     s.max_ms = v.back();
     double sum = 0.0;
     for (double x : v) sum += x;
     s.mean_ms = sum / (double)v.size();
+  }
+  for (int c = 0; c < kNumClasses; ++c) {
+    s.class_completed[c] = lat_cls_[c].size();
+    s.class_steps[c] = st_steps_cls_[c];
+    s.class_busy_ms[c] = st_busy_cls_[c];
+    s.class_p50_ms[c] = s.class_p99_ms[c] = 0.0;
+    if (!lat_cls_[c].empty()) {
+      std::vector<double> v = lat_cls_[c];
+      std::sort(v.begin(), v.end());
+      s.class_p50_ms[c] = q(v, 0.50);
+      s.class_p99_ms[c] = q(v, 0.99);
+    }
   }
   return s;
 }
@@ -645,6 +663,11 @@ ServerStats Server::stats() {
 void Server::reset_stats() {
   std::lock_guard<std::mutex> lk(mu_);
   lat_.clear();
+  for (int c = 0; c < kNumClasses; ++c) {
+    lat_cls_[c].clear();
+    st_steps_cls_[c] = 0;
+    st_busy_cls_[c] = 0.0;
+  }
   st_sub_ = st_done_ = st_fail_ = st_batches_ = st_breqs_ = 0;
   st_busy_ = 0.0;
   for (int q = 0; q < 6; ++q) st_phase_[q] = 0.0;

@@ -32,6 +32,7 @@ struct ServerConfig {
   uint64_t max_batch_shapes = 1u << 19;  // shapes per batched step (bounds step latency)
   uint64_t max_pending = 1u << 16;       // queued requests before submit refuses
   bool use_graph = true;
+  uint64_t lean_shapes = kLeanShapes;  // largest lean step (BatchBase::set_lean_shapes)
 };
 
 struct ServerStats {
@@ -42,6 +43,9 @@ struct ServerStats {
   // host time of the workers' step phases, summed: pose uploads, selection,
   // launch, wait for the step, result block, contact readback
   double host_ms[6];
+  // per size class
+  uint64_t class_completed[4], class_steps[4];
+  double class_p50_ms[4], class_p99_ms[4], class_busy_ms[4];
 };
 
 struct PoseBlock;
@@ -163,6 +167,9 @@ class Server {
   uint64_t next_buffer_ = 1;
   // window statistics
   std::vector<double> lat_;
+  std::vector<double> lat_cls_[4];
+  uint64_t st_steps_cls_[4] = {0, 0, 0, 0};
+  double st_busy_cls_[4] = {0, 0, 0, 0};
   uint64_t st_sub_ = 0, st_done_ = 0, st_fail_ = 0, st_batches_ = 0, st_breqs_ = 0;
   double st_busy_ = 0.0, st_t0_ = 0.0;
   double st_phase_[6] = {0, 0, 0, 0, 0, 0};

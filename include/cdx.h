@@ -287,6 +287,11 @@ int cdx_batch_update_bodies(cdx_batch* batch, uint32_t scene, uint64_t k, const 
                             const void* body_rot);
 int cdx_batch_set_envelope(cdx_batch* batch, double envelope);
 int cdx_batch_set_narrowphase_mode(cdx_batch* batch, int np_mode, double tau);
+/* Steps over at most max_shapes shapes run lean (default 16384): scans of
+ * one workgroup and each pair's whole narrowphase in one kernel, about half
+ * the launches of a full step.  Results are identical either way; 0 makes
+ * every step a full one. */
+int cdx_batch_set_lean_shapes(cdx_batch* batch, uint64_t max_shapes);
 /* Scenes of the following steps (scenes = NULL: every resident scene). */
 int cdx_batch_select(cdx_batch* batch, const uint32_t* scenes, uint64_t k);
 /* Enqueues one step on the batch's stream (use_graph: record / replay). */
@@ -332,6 +337,8 @@ typedef struct cdx_server_config {
   uint64_t max_batch_shapes; /* shapes per step and class (bounds step latency); 0 = 524288 */
   uint64_t max_pending;      /* queued requests before submit returns CDX_ERR_CAPACITY; 0 = 65536 */
   int use_graph;             /* replay recorded HIP graphs */
+  int64_t lean_shapes;       /* steps of at most this many shapes run lean (cdx_batch_set_lean_shapes);
+                                < 0: the default (16384), 0: never */
 } cdx_server_config;
 
 typedef struct cdx_server_stats_t {
@@ -340,6 +347,10 @@ typedef struct cdx_server_stats_t {
   double busy_ms;                         /* summed device time of the steps */
   double window_s;                        /* wall time of the statistics window */
   double host_ms[6]; /* workers' host time per phase: pose upload, selection, launch, wait, result block, readback */
+  /* per size class (< 8192, < 49152, < 131072, >= 131072 shapes) */
+  uint64_t class_completed[4], class_steps[4];
+  double class_p50_ms[4], class_p99_ms[4]; /* submit -> result ready */
+  double class_busy_ms[4];                 /* summed device time of the class's steps */
 } cdx_server_stats_t;
 
 int cdx_server_create(cdx_ctx* ctx, const cdx_server_config* cfg, cdx_server** out);
