@@ -16,8 +16,9 @@
 // minimum corners at most one cell apart along y and z and kx cells apart
 // along x, so a search of 9 rows of 2 kx + 1 consecutive cells around a
 // shape finds all its partners.  The cell table is a histogram of the dense
-// cell keys and its scan; the entries are put in cell order by a stable
-// radix sort of N keys.  Each shape keeps its partners of higher index, so
+// cell keys and its scan; the entries are put in cell order by a counting
+// sort (each shape's rank in its cell comes from the histogram's atomic,
+// so no sort pass runs).  Each shape keeps its partners of higher index, so
 // each pair is found once, by its first shape, and the pair list comes out
 // grouped by first shape; each group is sorted, which leaves the list
 // ascending as sort + unique leaves it.
@@ -72,14 +73,20 @@ __device__ __forceinline__ uint32_t cell_key(const uint32_t c[3], const CellGrid
   return (c[2] * cg.g[1] + c[1]) * cg.g[0] + c[0];
 }
 
-// Per shape: the entry (cell key << 32 | shape) and one count in its cell's
-// histogram slot (ccount zero beforehand).
+// Per shape: one count in its cell's histogram slot (ccount zero
+// beforehand), the shape's rank among the shapes of its cell (the count
+// before its own), and the largest shape index of the cell (last, zero
+// beforehand).  After a scan of the histogram, cell start + rank is the
+// shape's place in cell order: a counting sort with no sort pass.  The
+// order within a cell is the order of the atomics; nothing depends on it
+// (every shape's partner group is sorted, and the walk reads whole cells).
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ chunks,
                                                      const DevStatus* __restrict__ st, uint64_t NS,
                                                      const R* __restrict__ aabb,
                                                      const SceneDesc<R>* __restrict__ desc, CellGrid cg,
-                                                     uint32_t* __restrict__ ccount, uint64_t* __restrict__ ent) {
+                                                     uint32_t* __restrict__ ccount, uint32_t* __restrict__ rank,
+                                                     uint32_t* __restrict__ last) {
   if ((uint64_t)blockIdx.x * CHUNK >= st->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
   if (threadIdx.x >= ch.len) return;
@@ -88,27 +95,24 @@ __global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ c
   uint32_t c[3];
   cell_of_box(r.lo, cg, c);
   const uint32_t key = cell_key(c, cg);
-  atomicAdd(&ccount[key], 1u);
-  ent[i] = ((uint64_t)key << 32) | (uint32_t)i;
+  rank[i] = atomicAdd(&ccount[key], 1u);
+  atomicMax(&last[key], (uint32_t)i);
 }
 
-// Sorted entries to search entries in cell order (scene 0, whose shapes
-// start at 0).
-// Also the largest shape index of each non-empty cell (its last entry: the
-// radix sort keeps a cell's shapes in index order) into last, which is
-// zero beforehand.
+// Search entries in cell order (scene 0, whose shapes start at 0): shape
+// i goes to cell start + rank.
 template <class R>
-__global__ __launch_bounds__(256) void k_cell_gather(uint64_t n, const uint64_t* __restrict__ sorted, uint64_t NS,
-                                                     const R* __restrict__ aabb,
-                                                     const SceneDesc<R>* __restrict__ desc,
-                                                     CellLo<R>* __restrict__ slo, CellHi<R>* __restrict__ shi,
-                                                     uint32_t* __restrict__ last) {
-  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= n) return;
-  const uint64_t e = sorted[p];
-  const uint32_t i = (uint32_t)e;
-  if (p + 1 == n || (sorted[p + 1] >> 32) != (e >> 32)) last[e >> 32] = i;
+__global__ __launch_bounds__(256) void k_cell_scatter(uint64_t n, const uint32_t* __restrict__ rank,
+                                                      const uint32_t* __restrict__ cstart, uint64_t NS,
+                                                      const R* __restrict__ aabb,
+                                                      const SceneDesc<R>* __restrict__ desc, CellGrid cg,
+                                                      CellLo<R>* __restrict__ slo, CellHi<R>* __restrict__ shi) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
   const BpRec<R> r = shifted_rec(aabb, NS, i, desc[0], 0u);
+  uint32_t c[3];
+  cell_of_box(r.lo, cg, c);
+  const uint64_t p = (uint64_t)cstart[cell_key(c, cg)] + rank[i];
   CellLo<R> l{};
   CellHi<R> h{};
 #pragma unroll
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(256) void k_cell_gather(uint64_t n, const uint64_t*
     l.lo[k] = r.lo[k];
     h.hi[k] = r.hi[k];
   }
-  l.id = i;
+  l.id = (uint32_t)i;
   slo[p] = l;
   shi[p] = h;
 }

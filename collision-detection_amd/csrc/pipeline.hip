@@ -474,7 +474,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
                            &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &bins_,
-                           &wpart_, &cell_tab_, &cell_ckr_, &cell_tmp_, &cell_lo_, &cell_hi_, &cell_slab_, &cell_last_, &cell_wmax_, &cell_ovf_};
+                           &wpart_, &cell_tab_, &cell_rank_, &cell_lo_, &cell_hi_, &cell_slab_, &cell_last_, &cell_wmax_, &cell_ovf_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -1190,10 +1190,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     const uint64_t n = scenes_[0].n_shapes;
     const uint64_t slots = chunk_grid() * CHUNK;
     const uint64_t ntab = ncells_ + 1;
-    const int kbits = bits_for(ncells_ - 1);
     cell_tab_.reserve(ntab * 4);
-    cell_ckr_.reserve(n * 8);
-    cell_tmp_.reserve(n * 8);
+    cell_rank_.reserve(n * 4);
     cell_lo_.reserve(n * sizeof(CellLo<R>));
     cell_hi_.reserve(n * sizeof(CellHi<R>));
     cell_slab_.reserve(n * CELL_SLAB * 4);
@@ -1201,23 +1199,21 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     cell_wmax_.reserve(ncells_ * 4);
     cell_ovf_.reserve(n * 4);
     scan_tmp_.reserve(std::max<size_t>(std::max(scan_tmp_words(ntab), scan_tmp_words(slots)), 16) * 4);
-    radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(n, kbits), 16) * 4);
     uint32_t* tab = cell_tab_.as<uint32_t>();
+    uint32_t* rank = cell_rank_.as<uint32_t>();
     mark(ST_BP_EMIT, true);
     fill_u32(tab, 0u, ntab, s_);
+    fill_u32(cell_last_.as<uint32_t>(), 0u, ncells_, s_);
     hipLaunchKernelGGL(k_cell_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cg_, tab, cell_ckr_.as<uint64_t>());
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cg_, tab, rank, cell_last_.as<uint32_t>());
     scan_excl_u32(tab, tab, ntab, &st()->cells, scan_tmp_.as<uint32_t>(), s_);
     mark(ST_BP_EMIT, false);
     mark(ST_BP_SORT, true);
-    if (radix_sort_u64(cell_ckr_.as<uint64_t>(), cell_tmp_.as<uint64_t>(), n, 32, kbits, radix_tmp_.as<uint32_t>(), s_))
-      swap_bufs(cell_ckr_, cell_tmp_);
-    fill_u32(cell_last_.as<uint32_t>(), 0u, ncells_, s_);
-    hipLaunchKernelGGL(k_cell_gather<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s_, n,
-                       (const uint64_t*)cell_ckr_.as<uint64_t>(), NSC_, (const R*)aabb_.as<R>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cell_lo_.as<CellLo<R>>(), cell_hi_.as<CellHi<R>>(),
-                       cell_last_.as<uint32_t>());
+    hipLaunchKernelGGL(k_cell_scatter<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s_, n, (const uint32_t*)rank,
+                       (const uint32_t*)tab, NSC_, (const R*)aabb_.as<R>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cg_, cell_lo_.as<CellLo<R>>(),
+                       cell_hi_.as<CellHi<R>>());
     const CellLo<R>* clo = cell_lo_.as<CellLo<R>>();
     const CellHi<R>* chi = cell_hi_.as<CellHi<R>>();
     hipLaunchKernelGGL(k_cell_window, dim3(blocks_for(ncells_, 256)), dim3(256), 0, s_, ncells_, cg_,
@@ -1640,7 +1636,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   int exact_key_bits_ = 0;
   CellGrid cg_{};
   uint64_t ncells_ = 0;
-  DevBuf wpart_, cell_tab_, cell_ckr_, cell_tmp_, cell_lo_, cell_hi_, cell_slab_, cell_last_, cell_wmax_, cell_ovf_;
+  DevBuf wpart_, cell_tab_, cell_rank_, cell_lo_, cell_hi_, cell_slab_, cell_last_, cell_wmax_, cell_ovf_;
 };
 
 }  // namespace
