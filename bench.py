@@ -577,7 +577,8 @@ def batched_leg(pkg, scenes, ctx, count, steps):
         "speedup_vs_plain": round(sps / plain_sps, 2),
         "per_scene_counts_match_plain": mismatch == 0,
         "add_scenes_s": round(t_add, 2),
-        "autotuned_grids_by_size_class": {str(c): g[0] for c, g in sorted(grids.items())},
+        "autotuned_grids_by_size_class": {str(c): {"scenes": len(g), "distinct_grids": len({tuple(x) for x in g}),
+                                                   "first": g[0]} for c, g in sorted(grids.items())},
         "device_bytes": int(b.device_bytes()),
     }
     b.close()

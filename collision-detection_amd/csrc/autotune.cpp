@@ -17,30 +17,43 @@ namespace {
 // (entries grow as the cube of the span); above ~3 the bins hold many
 // shapes and the per-bin pair search grows quadratically.
 constexpr double kFactors[] = {0.75, 1.0, 1.5, 2.0, 3.0};
+constexpr double kRefine[] = {0.85, 1.18};  // around the coarse winner
 constexpr int kReps = 3;
 
 struct Bounds {
-  double lo[3], hi[3], mean_width;
+  double lo[3], hi[3], mean_width, volume_sum;
 };
 
 template <class R> Bounds bounds_of(const std::vector<R>& ab, uint64_t n) {
   Bounds b;
   for (int k = 0; k < 3; ++k) { b.lo[k] = INFINITY; b.hi[k] = -INFINITY; }
-  double w = 0.0;
+  double w = 0.0, vs = 0.0;
   for (uint64_t i = 0; i < n; ++i) {
+    double v = 1.0;
     for (int k = 0; k < 3; ++k) {
       double lo = (double)ab[3 * i + k], hi = (double)ab[3 * (n + i) + k];
       b.lo[k] = std::min(b.lo[k], lo);
       b.hi[k] = std::max(b.hi[k], hi);
       w += hi - lo;
+      v *= hi - lo;
     }
+    vs += v;
   }
+  b.volume_sum = vs;
   b.mean_width = n ? w / (3.0 * (double)n) : 1.0;
   if (!(b.mean_width > 0.0)) b.mean_width = 1.0;
   return b;
 }
 
 }  // namespace
+
+int Autotuner::density_bucket(double aabb_volume_sum, const double lo[3], const double hi[3]) {
+  double vol = 1.0;
+  for (int k = 0; k < 3; ++k) vol *= std::max(hi[k] - lo[k], 0.0);
+  if (!(vol > 0.0) || !(aabb_volume_sum > 0.0) || !std::isfinite(aabb_volume_sum / vol)) return 0;
+  const double neighbours = 8.0 * aabb_volume_sum / vol;
+  return (int)std::lround(std::min(std::max(std::log2(neighbours), -8.0), 16.0));
+}
 
 int Autotuner::size_class(uint64_t n) {
   int c = 0;
@@ -87,10 +100,14 @@ TuneChoice Autotuner::choose(uint64_t n, const int32_t* T, const void* A, const 
   const uint64_t max_bins = std::max<uint64_t>(64, std::min<uint64_t>(4 * n, 1ull << 24));
 
   const int cls = size_class(n);
+  const int dens = density_bucket(bd.volume_sum, bd.lo, bd.hi);
+  const std::pair<int, int> key{cls, dens};
   TuneChoice out{};
+  out.size_class = cls;
+  out.density = dens;
   {
     std::lock_guard<std::mutex> lk(mu_);
-    auto it = classes_.find(cls);
+    auto it = classes_.find(key);
     if (it != classes_.end()) {
       out.cell_factor = it->second.cell_factor;
       out.m_per_shape = it->second.m_per_shape;
@@ -103,8 +120,9 @@ TuneChoice Autotuner::choose(uint64_t n, const int32_t* T, const void* A, const 
 
   double best_ms = INFINITY;
   s->set_profiling(true);
+  s->set_pair_search(PAIRS_BINS);  // batched steps search by bins
   std::vector<double> ms(ST_COUNT);
-  for (double f : kFactors) {
+  auto trial = [&](double f) {
     int g[3];
     grid_for(bd.lo, bd.hi, f * bd.mean_width, max_bins, g);
     s->set_grid(g[0], g[1], g[2]);
@@ -128,10 +146,14 @@ TuneChoice Autotuner::choose(uint64_t n, const int32_t* T, const void* A, const 
       out.p_per_shape = (double)p / (double)n;
       for (int k = 0; k < 3; ++k) out.grid[k] = g[k];
     }
-  }
+  };
+  for (double f : kFactors) trial(f);
+  // second round: between the winner and its neighbours on the coarse list
+  const double f0 = out.cell_factor;
+  for (double r : kRefine) trial(f0 * r);
   out.measured = true;
   std::lock_guard<std::mutex> lk(mu_);
-  classes_[cls] = ClassInfo{out.cell_factor, out.m_per_shape, out.p_per_shape};
+  classes_[key] = ClassInfo{out.cell_factor, out.m_per_shape, out.p_per_shape};
   return out;
 }
 
