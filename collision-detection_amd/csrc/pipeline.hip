@@ -33,6 +33,7 @@
 #include "cdx_pipeline.hpp"
 #include "cell_kernels.hpp"
 #include "cdx_util.hpp"
+#include "hostcopy.hpp"
 #include "np_kernels.hpp"
 
 namespace cdx {
@@ -241,7 +242,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     scenes_[0].n_shapes = n;
     if (n) {
       stage_.reserve(n * 6 * sizeof(R));
-      CDX_HIP_CHECK(hipMemcpy(stage_.p, aabb, n * 6 * sizeof(R), hipMemcpyHostToDevice));
+      stager_.h2d(stage_.p, aabb, n * 6 * sizeof(R), s_);
       hipLaunchKernelGGL(k_aabb_from_aos<R>, dim3(blocks_for(2 * n, 256)), dim3(256), 0, s_, n, NSC_,
                          (const R*)stage_.as<R>(), aabb_.as<R>());
       sync_internal();
@@ -261,7 +262,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
     pcap_ = std::max<uint64_t>(p, 1);
     pairs_.reserve(pcap_ * 8);
-    if (p) CDX_HIP_CHECK(hipMemcpy(pairs_.p, pairs, p * 8, hipMemcpyHostToDevice));
+    if (p) stager_.h2d(pairs_.p, pairs, p * 8, s_);
     given_pairs_ = p;
     pairs_given_ = true;
   }
@@ -357,8 +358,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_aabb_aos<R>, dim3(blocks_for(2 * n, 256)), dim3(256), 0, s_, n, NSC_,
                        (const R*)aabb_.as<R>(), (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), shifted ? 1 : 0,
                        tmp_out_.as<R>());
-    CDX_HIP_CHECK(hipMemcpyAsync(out, tmp_out_.p, n * 6 * sizeof(R), hipMemcpyDeviceToHost, s_));
-    sync();
+    stager_.d2h(out, tmp_out_.p, n * 6 * sizeof(R), s_);
   }
 
   void read_bins(uint32_t* bin_counts, uint32_t* sorted_keys, uint32_t* sorted_bodies, uint32_t* active_keys,
@@ -432,11 +432,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   void read_pairs(int64_t* out) override {
     sync();
-    if (P_) CDX_HIP_CHECK(hipMemcpy(out, pairs_.p, P_ * 8, hipMemcpyDeviceToHost));
+    if (P_) stager_.d2h(out, pairs_.p, P_ * 8, s_);
   }
   void read_pairs_permuted(int64_t* out) override {
     sync();
-    if (P_) CDX_HIP_CHECK(hipMemcpy(out, perm_.p, P_ * 8, hipMemcpyDeviceToHost));
+    if (P_) stager_.d2h(out, perm_.p, P_ * 8, s_);
   }
   uint64_t read_mpr_queue(uint32_t* out, uint64_t cap) override {
     sync();
@@ -822,7 +822,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   void upload_soa(const void* host, uint64_t n, int k, R* dst, uint64_t stride, uint64_t offset) {
     stage_.reserve(n * k * sizeof(R));
-    CDX_HIP_CHECK(hipMemcpy(stage_.p, host, n * k * sizeof(R), hipMemcpyHostToDevice));
+    stager_.h2d(stage_.p, host, n * k * sizeof(R), s_);
     hipLaunchKernelGGL(k_deinterleave<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s_, n, k, (const R*)stage_.as<R>(),
                        dst, stride, offset);
     CDX_HIP_CHECK(hipStreamSynchronize(s_));
@@ -837,8 +837,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       t8[i] = (uint8_t)T[i];
       ids[i] = (uint32_t)(ID[i] + body_base);
     }
-    CDX_HIP_CHECK(hipMemcpy(type_.as<uint8_t>() + at, t8.data(), n, hipMemcpyHostToDevice));
-    CDX_HIP_CHECK(hipMemcpy(body_.as<uint32_t>() + at, ids.data(), n * 4, hipMemcpyHostToDevice));
+    stager_.h2d(type_.as<uint8_t>() + at, t8.data(), n, s_);
+    stager_.h2d(body_.as<uint32_t>() + at, ids.data(), n * 4, s_);
+    CDX_HIP_CHECK(hipStreamSynchronize(s_));  // t8 and ids are freed on return
     upload_soa(A, n, 3, lpos_.as<R>(), NSC_, at);
     upload_soa(B, n, 3, dims_.as<R>(), NSC_, at);
     if (C) {
@@ -853,7 +854,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     if (!nb) return;
     upload_soa(pos, nb, 3, bpos_.as<R>(), NBC_, at);
     upload_soa(rot, nb, 4, brot_.as<R>(), NBC_, at);
-    if (active) CDX_HIP_CHECK(hipMemcpy(active_.as<uint8_t>() + at, active, nb, hipMemcpyHostToDevice));
+    if (active) {
+      stager_.h2d(active_.as<uint8_t>() + at, active, nb, s_);
+      CDX_HIP_CHECK(hipStreamSynchronize(s_));
+    }
     else CDX_HIP_CHECK(hipMemset(active_.as<uint8_t>() + at, 1, nb));
   }
 
@@ -1533,13 +1537,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     // the step's contact arrays lie one after the other, each C_ rows long
     const R* o = out_.as<R>();
     const uint64_t C = C_;
-    if (norm) CDX_HIP_CHECK(hipMemcpy(norm, o + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
-    if (cpta) CDX_HIP_CHECK(hipMemcpy(cpta, o + 3 * C + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
-    if (cptb) CDX_HIP_CHECK(hipMemcpy(cptb, o + 6 * C + 3 * begin, n * 3 * sizeof(R), hipMemcpyDeviceToHost));
-    if (dpth) CDX_HIP_CHECK(hipMemcpy(dpth, o + 9 * C + begin, n * sizeof(R), hipMemcpyDeviceToHost));
+    if (norm) stager_.d2h(norm, o + 3 * begin, n * 3 * sizeof(R), s_);
+    if (cpta) stager_.d2h(cpta, o + 3 * C + 3 * begin, n * 3 * sizeof(R), s_);
+    if (cptb) stager_.d2h(cptb, o + 6 * C + 3 * begin, n * 3 * sizeof(R), s_);
+    if (dpth) stager_.d2h(dpth, o + 9 * C + begin, n * sizeof(R), s_);
     if (bids) {
-      CDX_HIP_CHECK(hipMemcpy(bids, reinterpret_cast<const int32_t*>(o + 10 * C) + 2 * begin, n * 8,
-                              hipMemcpyDeviceToHost));
+      stager_.d2h(bids, reinterpret_cast<const int32_t*>(o + 10 * C) + 2 * begin, n * 8, s_);
       if (body_base)
         for (uint64_t i = 0; i < 2 * n; ++i) bids[i] -= (int32_t)body_base;
     }
@@ -1624,6 +1627,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;
   DevBuf np_v_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_, perm_;  // out_: the step's contact arrays back to back, C rows each
+  HostStager stager_;  // large copies to and from caller memory
   DevBuf scan_tmp_, radix_tmp_, runs_part_;
   bool opened_ = false;  // the step's totals were zeroed by one k_stage_begin
   bool mpr_in_tmp_ = false;  // the last reference-order MPR launch read queue_tmp_ (else queue_)
