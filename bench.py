@@ -612,7 +612,7 @@ def served_leg(pkg, scenes, ctx, args):
 
     srv = pkg.Server(ctx, np.float64, mode=pkg.NP_MIXED, max_batch_shapes=args.served_batch_shapes,
                      max_pending=args.served_max_pending, use_graph=not args.served_eager,
-                     lean_shapes=args.lean_shapes)
+                     lean_shapes=None if args.lean_shapes < 0 else args.lean_shapes)
     poses = [None] * count
     t0 = time.perf_counter()
     from concurrent.futures import ThreadPoolExecutor

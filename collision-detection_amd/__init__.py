@@ -538,11 +538,13 @@ class Server:
     """Served request loop over device-resident scenes (cdx_server)."""
 
     def __init__(self, ctx: Context, dtype=np.float64, mode=NP_MIXED, tau=1e-2, envelope=0.0, hbm_budget=0,
-                 max_batch_shapes=0, max_pending=0, use_graph=True, lean_shapes=-1):
+                 max_batch_shapes=0, max_pending=0, use_graph=True, lean_shapes=None):
+        """lean_shapes: largest lean step in shapes (None: the library default, 0: never)."""
         self.ctx = ctx
         self.dtype = np.dtype(dtype)
+        lean = 0 if lean_shapes is None else (-1 if int(lean_shapes) == 0 else int(lean_shapes))
         cfg = ServerConfig(_kind(dtype), int(mode), float(tau), float(envelope), int(hbm_budget),
-                           int(max_batch_shapes), int(max_pending), 1 if use_graph else 0, int(lean_shapes))
+                           int(max_batch_shapes), int(max_pending), 1 if use_graph else 0, lean)
         self.h = C.c_void_p()
         ctx._chk(lib().cdx_server_create(ctx.h, C.byref(cfg), C.byref(self.h)))
 

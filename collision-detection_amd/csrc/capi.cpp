@@ -485,7 +485,8 @@ int cdx_server_create(cdx_ctx* ctx, const cdx_server_config* cfg, cdx_server** o
     if (cfg->max_batch_shapes) c.max_batch_shapes = cfg->max_batch_shapes;
     if (cfg->max_pending) c.max_pending = cfg->max_pending;
     c.use_graph = cfg->use_graph != 0;
-    if (cfg->lean_shapes >= 0) c.lean_shapes = (uint64_t)cfg->lean_shapes;
+    if (cfg->lean_shapes > 0) c.lean_shapes = (uint64_t)cfg->lean_shapes;
+    if (cfg->lean_shapes < 0) c.lean_shapes = 0;  // never lean
     auto sv = std::make_unique<cdx_server>();
     sv->srv = std::make_unique<cdx::Server>(ctx->device, c);
     *out = sv.release();

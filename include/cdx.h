@@ -338,7 +338,7 @@ typedef struct cdx_server_config {
   uint64_t max_pending;      /* queued requests before submit returns CDX_ERR_CAPACITY; 0 = 65536 */
   int use_graph;             /* replay recorded HIP graphs */
   int64_t lean_shapes;       /* steps of at most this many shapes run lean (cdx_batch_set_lean_shapes);
-                                < 0: the default (16384), 0: never */
+                                0: the default (16384), < 0: never */
 } cdx_server_config;
 
 typedef struct cdx_server_stats_t {
