@@ -725,3 +725,51 @@ def test_new_shape_placement_and_envelope_quirks(oracle, scenes):
     t0 = _pair(oracle, scenes, *tri)
     t1 = _pair(oracle, scenes, *tri, envelope=0.01)
     np.testing.assert_array_equal(t0.dpth, t1.dpth)
+
+
+def _segment_closest(p1, q1, p2, q2):
+    """Closest points of segments p1-q1 and p2-q2 (the clamped
+    parametric solution, as in Ericson, Real-Time Collision Detection 5.1.9)."""
+    d1, d2, r = q1 - p1, q2 - p2, p1 - p2
+    a, e, f = d1 @ d1, d2 @ d2, d2 @ r
+    c, b = d1 @ r, d1 @ d2
+    den = a * e - b * b
+    s = np.clip((b * f - c * e) / den, 0.0, 1.0) if den > 1e-14 else 0.0
+    t = (b * s + f) / e
+    if t < 0.0:
+        t, s = 0.0, np.clip(-c / a, 0.0, 1.0)
+    elif t > 1.0:
+        t, s = 1.0, np.clip((b - c) / a, 0.0, 1.0)
+    return p1 + d1 * s, p2 + d2 * t
+
+
+def test_mpr_capsule_capsule_vs_segment_distance(oracle):
+    """Two capsules (local axis y, half-length h, radius r) whose axes cross
+    at their midpoints, offset along the common perpendicular w: the axis
+    segments' closest points (computed independently) are the centres, so
+    the overlap r_a + r_b - |c_b - c_a| lies along the centre line, the ray
+    MPR walks (Narrowphase.cu:135-140), and MPR's depth must equal it.  200
+    random orientations, sizes and overlaps."""
+    rng = np.random.default_rng(0x5EED0C0C)
+    checked = 0
+    for _ in range(200):
+        ra, rb = rng.uniform(0.1, 0.3, 2)
+        ha, hb = rng.uniform(0.2, 0.8, 2)
+        qa, qb = (q / np.linalg.norm(q) for q in rng.normal(size=(2, 4)))
+        ua, ub = quat_to_mat(qa)[:, 1], quat_to_mat(qb)[:, 1]
+        w = np.cross(ua, ub)
+        if np.linalg.norm(w) < 0.2:  # nearly parallel axes: the closest points are not unique
+            continue
+        w /= np.linalg.norm(w)
+        overlap = rng.uniform(0.02, 0.5) * min(ra, rb)
+        XA = rng.normal(size=3)
+        XB = XA + w * (ra + rb - overlap)
+        ca, cb = _segment_closest(XA - ha * ua, XA + ha * ua, XB - hb * ub, XB + hb * ub)
+        want = ra + rb - np.linalg.norm(cb - ca)
+        hit, n, p, depth = _mpr_case(CAPSULE, np.array([ra, ha, ra]), XA, qa, CAPSULE, np.array([rb, hb, rb]), XB, qb)
+        assert hit
+        np.testing.assert_allclose(want, overlap, rtol=1e-9)  # the closest points are the centres
+        np.testing.assert_allclose(depth, want, atol=2e-3 * min(ra, rb))
+        assert abs(n @ w) > 0.999
+        checked += 1
+    assert checked > 100
