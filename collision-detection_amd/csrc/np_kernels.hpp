@@ -243,6 +243,7 @@ template <class R> struct PairOut {
   PairRec<R>* rec;  // P records
   uint32_t* hit;    // P hit flags
   uint64_t P;       // capacity
+  uint8_t* kind;    // per hit: 0 its contact is in rec, 1 sphere-sphere (recomputed at compaction)
 };
 
 __device__ __forceinline__ uint64_t dev_count(uint64_t cap, const uint64_t* d_n) {
@@ -263,6 +264,23 @@ __device__ __forceinline__ void store_pair(const PairOut<R>& o, uint64_t P, uint
   r.ids[1] = (int32_t)idb;
   o.rec[i] = r;
   o.hit[i] = 1u;
+  o.kind[i] = 0u;
+}
+
+// A sphere-sphere hit: no record; the compaction recomputes its contact from
+// the shape records with contact_spheres (the same inputs and operations,
+// so the same bits), which saves a record write and read per hit.
+template <class R> __device__ __forceinline__ void mark_sphere_hit(const PairOut<R>& o, uint64_t i) {
+  o.hit[i] = 1u;
+  o.kind[i] = 1u;
+}
+
+// The hit test of contact_spheres on its own (same operations).
+template <class R> __device__ __forceinline__ bool spheres_touch(v3<R> AX, R rad_a, v3<R> BX, R rad_b) {
+  const v3<R> relpos = BX - AX;
+  const R d2 = dot(relpos, relpos);
+  const R cd = rad_a + rad_b;
+  return d2 <= cd * cd;
 }
 
 // ---- mixed precision: FP32 MPR in the frame of shape A, FP64 re-solve ----
@@ -458,13 +476,11 @@ __device__ __forceinline__ bool np_front_one(uint64_t i, uint32_t ia, uint32_t i
                                              const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
                                              const ShapeRecF* __restrict__ frec, float tau_rel, int prefilter,
                                              R envelope, const PairOut<R>& o, uint32_t& ta, uint32_t& tb) {
-  const uint64_t P = o.P;
   o.hit[i] = 0u;
   bool enq = false;
   auto spheres = [&]() {
-    Contact<R> c;
-    ShapeRec<R> ra = rec[ia], rb = rec[ib];
-    if (contact_spheres(ra, rb, envelope, c)) store_pair(o, P, i, c, ra.body, rb.body);
+    const ShapeRec<R> ra = rec[ia], rb = rec[ib];
+    if (spheres_touch(mk3(ra.x, ra.y, ra.z), ra.bx, mk3(rb.x, rb.y, rb.z), rb.bx)) mark_sphere_hit(o, i);
   };
   if (prefilter) {
     // the FP32 records carry type, body and activity: two gathers per pair
@@ -484,9 +500,11 @@ __device__ __forceinline__ bool np_front_one(uint64_t i, uint32_t ia, uint32_t i
           rad_a = rec[ia].bx;
           rad_b = rec[ib].bx;
         }
-        Contact<R> c;
-        if (contact_spheres(mk3((R)fa.x, (R)fa.y, (R)fa.z), rad_a, mk3((R)fb.x, (R)fb.y, (R)fb.z), rad_b, envelope, c))
-          store_pair(o, P, i, c, fa.body, fb.body);
+        // the FP32 record's centre is the scene-precision centre of the
+        // full record, exactly: the compaction's recomputation from the full
+        // records sees the same values
+        if (spheres_touch(mk3((R)fa.x, (R)fa.y, (R)fa.z), rad_a, mk3((R)fb.x, (R)fb.y, (R)fb.z), rad_b))
+          mark_sphere_hit(o, i);
       } else if (!fp32_filtered(ta) || !fp32_filtered(tb)) {
         enq = true;  // no FP32 model of rectangles, cones and triangles: FP64 decides
       } else {
@@ -765,6 +783,7 @@ template <class R>
 __global__ __launch_bounds__(COMPACT_BS) void k_np_compact(uint64_t cap, const uint64_t* __restrict__ d_np,
                                                            PairOut<R> o, const uint32_t* __restrict__ pos,
                                                            const uint64_t* __restrict__ pairs,
+                                                           const ShapeRec<R>* __restrict__ rec, R envelope,
                                                            const DevStatus* __restrict__ st, R* __restrict__ out,
                                                            uint64_t* __restrict__ perm) {
   __shared__ R lv[10][COMPACT_BS];
@@ -790,12 +809,25 @@ __global__ __launch_bounds__(COMPACT_BS) void k_np_compact(uint64_t cap, const u
     const uint32_t k = pos[i];
     if (o.hit[i]) {
       const uint32_t sl = k - k0;
-      const PairRec<R> r = o.rec[i];
+      const uint64_t pr = pairs[i];
+      if (o.kind[i]) {
+        const ShapeRec<R> ra = rec[(uint32_t)(pr >> 32)], rb = rec[(uint32_t)(pr & 0xffffffffu)];
+        Contact<R> c;
+        contact_spheres(ra, rb, envelope, c);  // a hit by the front pass's test
+        lv[0][sl] = c.N.x; lv[1][sl] = c.N.y; lv[2][sl] = c.N.z;
+        lv[3][sl] = c.p1.x; lv[4][sl] = c.p1.y; lv[5][sl] = c.p1.z;
+        lv[6][sl] = c.p2.x; lv[7][sl] = c.p2.y; lv[8][sl] = c.p2.z;
+        lv[9][sl] = c.depth;
+        lid[2 * sl] = (int32_t)ra.body;
+        lid[2 * sl + 1] = (int32_t)rb.body;
+      } else {
+        const PairRec<R> r = o.rec[i];
 #pragma unroll
-      for (int c = 0; c < 10; ++c) lv[c][sl] = r.v[c];
-      lid[2 * sl] = r.ids[0];
-      lid[2 * sl + 1] = r.ids[1];
-      perm[k] = pairs[i];
+        for (int c = 0; c < 10; ++c) lv[c][sl] = r.v[c];
+        lid[2 * sl] = r.ids[0];
+        lid[2 * sl + 1] = r.ids[1];
+      }
+      perm[k] = pr;
     } else {
       perm[C + (i - k)] = pairs[i];
     }

@@ -471,7 +471,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t device_bytes() const override {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &cpos_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
-                           &pcount_, &pairs_, &np_v_, &np_hit_, &np_pos_, &queue_,
+                           &pcount_, &pairs_, &np_v_, &np_hit_, &np_kind_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
                            &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &bins_,
                            &wpart_, &cell_tab_, &cell_rank_, &cell_lo_, &cell_hi_, &cell_slab_, &cell_last_, &cell_wmax_, &cell_ovf_};
@@ -734,7 +734,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     // keys + ping-pong and per-entry pair counts/offsets; pairs, narrowphase
     // records, queues and contact outputs per pair; slot counts/offsets
     const double per_m = 2 * 8 + 4 + 16;
-    const double per_p = 2 * 8 + sizeof(PairRec<R>) + 4 + 4 + 2 * 8 + 4 + 1 + 10 * sizeof(R) + 8 + 8;
+    const double per_p = 2 * 8 + sizeof(PairRec<R>) + 4 + 1 + 4 + 2 * 8 + 4 + 1 + 10 * sizeof(R) + 8 + 8;
     return (uint64_t)((double)n * (cap_m_per_shape_ * per_m + cap_p_per_shape_ * per_p + 8.0)) + (1u << 20);
   }
 
@@ -1299,7 +1299,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     qcls_.reserve(P);
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(P), 16) * 4);
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(P, CLASS_BITS), 16) * 4);
-    PairOut<R> o{np_v_.as<PairRec<R>>(), np_hit_.as<uint32_t>(), P};
+    np_kind_.reserve(P);
+    PairOut<R> o{np_v_.as<PairRec<R>>(), np_hit_.as<uint32_t>(), P, np_kind_.as<uint8_t>()};
     const uint64_t* dP = &st()->P;
     if (lean_) {
       // a lean step: every pair's narrowphase in one kernel, no queue
@@ -1320,7 +1321,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       perm_.reserve(P * 8);
       hipLaunchKernelGGL(k_np_compact<R>, dim3(blocks_for(P, COMPACT_BS)), dim3(COMPACT_BS), 0, s_, P, dP, o,
                          (const uint32_t*)np_pos_.as<uint32_t>(), (const uint64_t*)pairs_.as<uint64_t>(),
-                         (const DevStatus*)st(), out_.as<R>(), perm_.as<uint64_t>());
+                         (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, (const DevStatus*)st(), out_.as<R>(),
+                         perm_.as<uint64_t>());
       mark(ST_NP_COMPACT, false);
       return;
     }
@@ -1391,7 +1393,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     perm_.reserve(P * 8);
     hipLaunchKernelGGL(k_np_compact<R>, dim3(blocks_for(P, COMPACT_BS)), dim3(COMPACT_BS), 0, s_, P, dP, o,
                        (const uint32_t*)np_pos_.as<uint32_t>(), (const uint64_t*)pairs_.as<uint64_t>(),
-                       (const DevStatus*)st(), out_.as<R>(), perm_.as<uint64_t>());
+                       (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, (const DevStatus*)st(), out_.as<R>(),
+                       perm_.as<uint64_t>());
     mark(ST_NP_COMPACT, false);
     if (exact_) {
       read_status();
@@ -1521,6 +1524,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     longs_.reserve((P / (GROUP_LONG + 1) + 1) * 4);
     np_v_.reserve(P * sizeof(PairRec<R>));
     np_hit_.reserve(P * 4);
+    np_kind_.reserve(P);
     np_pos_.reserve(P * 4);
     queue_.reserve(P * 8);
     queue_tmp_.reserve(P * 8);
@@ -1625,7 +1629,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;
   DevBuf chunks_, sel_, desc_, souts_;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;
-  DevBuf np_v_, np_hit_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
+  DevBuf np_v_, np_hit_, np_kind_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
   DevBuf out_, perm_;  // out_: the step's contact arrays back to back, C rows each
   HostStager stager_;  // large copies to and from caller memory
   DevBuf scan_tmp_, radix_tmp_, runs_part_;
