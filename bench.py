@@ -611,8 +611,7 @@ def served_leg(pkg, scenes, ctx, args):
         return s, p1, r1
 
     srv = pkg.Server(ctx, np.float64, mode=pkg.NP_MIXED, max_batch_shapes=args.served_batch_shapes,
-                     max_pending=args.served_max_pending, use_graph=not args.served_eager,
-                     lean_shapes=None if args.lean_shapes < 0 else args.lean_shapes)
+                     max_pending=args.served_max_pending, use_graph=not args.served_eager)
     poses = [None] * count
     t0 = time.perf_counter()
     from concurrent.futures import ThreadPoolExecutor
@@ -838,8 +837,6 @@ def main():
     ap.add_argument("--served-max-pending", type=int, default=4096,
                     help="queued requests before the server refuses new ones")
     ap.add_argument("--served-eager", action="store_true", help="launch served steps without HIP graphs")
-    ap.add_argument("--lean-shapes", type=int, default=-1,
-                    help="largest lean step in shapes for the graph path and the server (-1: library default, 0: never)")
     ap.add_argument("--plain-sample", type=int, default=256, help="scenes timed on the plain per-scene path")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (calibrates HBM byte counters)")
@@ -895,8 +892,6 @@ def main():
     b1 = pkg.Batch(ctx, s.dtype)
     b1.add_scene(s)
     b1.set_mode(mode, 1e-2)
-    if args.lean_shapes >= 0:
-        b1.set_lean_shapes(args.lean_shapes)
     for _ in range(max(args.warmup, 2)):
         b1.step(True)
         gc = b1.finish()

@@ -45,7 +45,7 @@ EXPORTS = [
     "cdx_scene_stage_ms",
     "cdx_scene_device_bytes", "cdx_batch_create", "cdx_batch_destroy", "cdx_batch_last_error",
     "cdx_batch_add_scene", "cdx_batch_scene_bins", "cdx_batch_scene_count", "cdx_batch_update_bodies",
-    "cdx_batch_set_envelope", "cdx_batch_set_narrowphase_mode", "cdx_batch_set_lean_shapes", "cdx_batch_select",
+    "cdx_batch_set_envelope", "cdx_batch_set_narrowphase_mode", "cdx_batch_select",
     "cdx_batch_step",
     "cdx_batch_finish", "cdx_batch_scene_counts", "cdx_batch_read_pairs", "cdx_batch_read_contacts",
     "cdx_batch_step_ms", "cdx_batch_device_bytes", "cdx_batch_scene_bytes", "cdx_server_create",
@@ -60,7 +60,7 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("real_kind", C.c_int), ("np_mode", C.c_int), ("tau", C.c_double), ("envelope", C.c_double),
         ("hbm_budget", C.c_uint64), ("max_batch_shapes", C.c_uint64), ("max_pending", C.c_uint64),
-        ("use_graph", C.c_int), ("lean_shapes", C.c_int64),
+        ("use_graph", C.c_int),
     ]
 
 
@@ -167,7 +167,6 @@ def lib():
         "cdx_batch_update_bodies": ([vp, C.c_uint32, u64, vp, vp, vp], i32),
         "cdx_batch_set_envelope": ([vp, C.c_double], i32),
         "cdx_batch_set_narrowphase_mode": ([vp, i32, C.c_double], i32),
-        "cdx_batch_set_lean_shapes": ([vp, C.c_uint64], i32),
         "cdx_batch_select": ([vp, vp, u64], i32),
         "cdx_batch_step": ([vp, i32], i32),
         "cdx_batch_finish": ([vp, vp], i32),
@@ -478,10 +477,6 @@ class Batch:
     def set_mode(self, mode, tau=1e-2):
         self._chk(lib().cdx_batch_set_narrowphase_mode(self.h, int(mode), float(tau)))
 
-    def set_lean_shapes(self, n):
-        """Steps over at most n shapes run lean (fewer launches, same results); 0: never."""
-        self._chk(lib().cdx_batch_set_lean_shapes(self.h, int(n)))
-
     def select(self, scenes=None):
         if scenes is None:
             self._chk(lib().cdx_batch_select(self.h, None, 0))
@@ -538,13 +533,11 @@ class Server:
     """Served request loop over device-resident scenes (cdx_server)."""
 
     def __init__(self, ctx: Context, dtype=np.float64, mode=NP_MIXED, tau=1e-2, envelope=0.0, hbm_budget=0,
-                 max_batch_shapes=0, max_pending=0, use_graph=True, lean_shapes=None):
-        """lean_shapes: largest lean step in shapes (None: the library default, 0: never)."""
+                 max_batch_shapes=0, max_pending=0, use_graph=True):
         self.ctx = ctx
         self.dtype = np.dtype(dtype)
-        lean = 0 if lean_shapes is None else (-1 if int(lean_shapes) == 0 else int(lean_shapes))
         cfg = ServerConfig(_kind(dtype), int(mode), float(tau), float(envelope), int(hbm_budget),
-                           int(max_batch_shapes), int(max_pending), 1 if use_graph else 0, lean)
+                           int(max_batch_shapes), int(max_pending), 1 if use_graph else 0)
         self.h = C.c_void_p()
         ctx._chk(lib().cdx_server_create(ctx.h, C.byref(cfg), C.byref(self.h)))
 

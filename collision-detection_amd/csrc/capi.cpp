@@ -433,7 +433,6 @@ int cdx_batch_set_envelope(cdx_batch* batch, double envelope) { BATCH_CALL(b.set
 int cdx_batch_set_narrowphase_mode(cdx_batch* batch, int np_mode, double tau) {
   BATCH_CALL(CDX_REQUIRE(tau >= 0.0, CDX_ERR_INVALID, "tau must be non-negative"); b.set_np_mode(np_mode, tau));
 }
-int cdx_batch_set_lean_shapes(cdx_batch* batch, uint64_t max_shapes) { BATCH_CALL(b.set_lean_shapes(max_shapes)); }
 int cdx_batch_select(cdx_batch* batch, const uint32_t* scenes, uint64_t k) {
   BATCH_CALL(CDX_REQUIRE(scenes || k == 0, CDX_ERR_INVALID, "null scene list"); b.select(scenes, k));
 }
@@ -485,8 +484,6 @@ int cdx_server_create(cdx_ctx* ctx, const cdx_server_config* cfg, cdx_server** o
     if (cfg->max_batch_shapes) c.max_batch_shapes = cfg->max_batch_shapes;
     if (cfg->max_pending) c.max_pending = cfg->max_pending;
     c.use_graph = cfg->use_graph != 0;
-    if (cfg->lean_shapes > 0) c.lean_shapes = (uint64_t)cfg->lean_shapes;
-    if (cfg->lean_shapes < 0) c.lean_shapes = 0;  // never lean
     auto sv = std::make_unique<cdx_server>();
     sv->srv = std::make_unique<cdx::Server>(ctx->device, c);
     *out = sv.release();

@@ -105,8 +105,6 @@ class SceneBase {
 // Many resident scenes stepped together: each step covers a selection of
 // scenes with one launch sequence (recorded as a HIP graph and replayed),
 // buffers sized by capacities, totals read back once at finish().
-// Default largest lean step (shapes).
-constexpr uint64_t kLeanShapes = 16384;
 
 class BatchBase {
  public:
@@ -122,11 +120,6 @@ class BatchBase {
                                    const void* rot) = 0;
   virtual void set_envelope(double e) = 0;
   virtual void set_np_mode(int mode, double tau) = 0;
-  // Steps over at most n shapes run lean: scans of one workgroup and the
-  // whole narrowphase of a pair in one kernel (about half the launches of
-  // the full step, whose class-sorted queue pays off on many pairs).
-  // Results are the same either way; 0 makes every step a full one.
-  virtual void set_lean_shapes(uint64_t n) = 0;
   // Scenes of the following steps (null = all resident scenes).
   virtual void select(const uint32_t* scenes, uint64_t k) = 0;
   virtual uint64_t selected_count() const = 0;

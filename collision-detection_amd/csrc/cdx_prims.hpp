@@ -61,10 +61,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratc
 size_t scan_tmp_words(uint64_t n);
 void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s,
                    const uint64_t* d_n = nullptr);
-// The same scan in one launch of one workgroup (no tmp): for inputs of a
-// few tiles, where launches cost more than the work.
-void scan_excl_u32_one(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, hipStream_t s,
-                       const uint64_t* d_n = nullptr);
 
 // Stable LSD radix sort of n 64-bit values by bits [lo_bit, lo_bit + nbits),
 // in the fewest passes of 8 to 11-bit digits whose per-tile digit counts
@@ -72,9 +68,8 @@ void scan_excl_u32_one(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* 
 // result ends in tmp (an odd number of passes): the caller swaps the buffers
 // instead of copying.  hist_tmp must hold radix_tmp_words(n, nbits).
 size_t radix_tmp_words(uint64_t n, int nbits);
-// one_wg_scans: the digit histograms are scanned by scan_excl_u32_one.
 bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, int lo_bit, int nbits, uint32_t* hist_tmp,
-                    hipStream_t s, const uint64_t* d_n = nullptr, bool one_wg_scans = false);
+                    hipStream_t s, const uint64_t* d_n = nullptr);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 

@@ -676,100 +676,6 @@ __global__ __launch_bounds__(128) void k_np_resolve(uint64_t cap, const uint64_t
   }
 }
 
-// The whole narrowphase of a step with few pairs (a lean step) in one
-// kernel.  Each workgroup takes FUSED_BS consecutive pairs through the front
-// pass, then the FP32 filter (MIXED / FAST) over its queued pairs and the
-// reference-order MPR over the survivors, each phase in shape-pair class
-// order (an LDS counting sort of the workgroup's pairs), so a wavefront
-// mostly runs one pair of support maps, as the class-sorted queue of the full
-// step arranges device-wide.  Every pair goes through the same per-pair
-// functions as in the queued kernels, so its result is the same bit for bit;
-// only which lane does the work differs.  KEEP: FAST.  The queue totals
-// (DevStatus Q: pairs past the front pass; Q2: pairs re-solved on the
-// reference path) are added once per workgroup.
-constexpr int FUSED_BS = 256;
-constexpr uint32_t NUM_CLASSES = 128;  // pair_class() < 128
-
-// Orders the threads whose pred holds by class: s_list[0..returned count)
-// are their thread indices, classes ascending.  Every thread must call it.
-__device__ __forceinline__ uint32_t block_class_order(bool pred, uint32_t cls, uint32_t* s_cnt, uint16_t* s_list,
-                                                      uint32_t* s_total) {
-  for (uint32_t c = threadIdx.x; c < NUM_CLASSES; c += FUSED_BS) s_cnt[c] = 0u;
-  __syncthreads();
-  const uint32_t rank = pred ? atomicAdd(&s_cnt[cls], 1u) : 0u;
-  __syncthreads();
-  if (threadIdx.x < WAVE) {
-    const uint32_t l = threadIdx.x;
-    const uint32_t a = s_cnt[2 * l], b = s_cnt[2 * l + 1];
-    const uint32_t inc = wave_incl_scan(a + b);
-    s_cnt[2 * l] = inc - a - b;
-    s_cnt[2 * l + 1] = inc - b;
-    if (l == WAVE - 1) *s_total = inc;
-  }
-  __syncthreads();
-  if (pred) s_list[s_cnt[cls] + rank] = (uint16_t)threadIdx.x;
-  __syncthreads();
-  return *s_total;
-}
-
-template <class R, class CT, bool FILTER, bool KEEP>
-__global__ __launch_bounds__(FUSED_BS) void k_np_fused(uint64_t cap, const uint64_t* __restrict__ d_np,
-                                                       const uint64_t* __restrict__ pairs,
-                                                       const ShapeRec<R>* __restrict__ rec,
-                                                       const ShapeHdr* __restrict__ hdr,
-                                                       const ShapeRecF* __restrict__ frec, float tau_rel, R envelope,
-                                                       PairOut<R> o, DevStatus* __restrict__ st) {
-  static_assert(2 * WAVE == NUM_CLASSES, "one wavefront scans the class counts, two per lane");
-  __shared__ uint32_t s_cnt[NUM_CLASSES];
-  __shared__ uint16_t s_list[FUSED_BS];
-  __shared__ uint8_t s_cls[FUSED_BS];
-  __shared__ uint8_t s_go[FUSED_BS];  // 0 done, 1 FP32 filter next, 2 reference-order MPR next
-  __shared__ uint32_t s_total;
-  const uint64_t n = dev_count(cap, d_np);
-  const uint64_t base = (uint64_t)blockIdx.x * FUSED_BS;
-  if (base >= n) return;  // the whole workgroup
-  const uint32_t t = threadIdx.x;
-  const uint64_t i = base + t;
-  uint32_t cls = 0, go = 0;
-  if (i < n) {
-    const uint64_t pr = pairs[i];
-    uint32_t ta, tb;
-    const bool enq = np_front_one(i, (uint32_t)(pr >> 32), (uint32_t)(pr & 0xffffffffu), rec, hdr, frec, tau_rel,
-                                  FILTER ? 1 : 0, envelope, o, ta, tb);
-    cls = pair_class(ta, tb);
-    go = enq ? (FILTER && cls < EXT_CLASS ? 1u : 2u) : 0u;
-  }
-  s_cls[t] = (uint8_t)cls;
-  s_go[t] = (uint8_t)go;
-  const int queued = __syncthreads_count(go != 0);
-  if constexpr (FILTER) {
-    const uint32_t m = block_class_order(go == 1, cls, s_cnt, s_list, &s_total);
-    if (t < m) {
-      const uint32_t li = s_list[t];
-      const uint64_t pi = base + li;
-      const uint64_t pr = pairs[pi];
-      s_go[li] = np_filter_one<R, KEEP>(pi, frec[(uint32_t)(pr >> 32)], frec[(uint32_t)(pr & 0xffffffffu)], envelope,
-                                        tau_rel, o)
-                     ? 2u
-                     : 0u;
-    }
-    __syncthreads();
-  }
-  const uint32_t m2 = block_class_order(s_go[t] == 2, s_cls[t], s_cnt, s_list, &s_total);
-  if (t == 0) {
-    if (queued) atomicAdd((unsigned long long*)&st->Q, (unsigned long long)queued);
-    if (m2) atomicAdd((unsigned long long*)&st->Q2, (unsigned long long)m2);
-  }
-  if (t < m2) {
-    const uint32_t li = s_list[t];
-    const uint64_t pi = base + li;
-    const uint64_t pr = pairs[pi];
-    const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
-    if (s_cls[li] >= EXT_CLASS) np_resolve_one<R, CT, true>(pi, ia, ib, rec, envelope, o);
-    else np_resolve_one<R, CT, false>(pi, ia, ib, rec, envelope, o);
-  }
-}
-
 // Grid of the extended-class resolve launch (it strides over its entries).
 constexpr unsigned EXT_RESOLVE_GRID = 1024;
 

@@ -187,8 +187,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     GateShared gate;
     (void)hipSetDevice(dev_);
     (void)hipStreamSynchronize(s_);
-    for (hipGraphExec_t& g : graph_exec_)
-      if (g) (void)hipGraphExecDestroy(g);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (int i = 0; i < 2 * ST_COUNT + 2; ++i) (void)hipEventDestroy(ev_[i]);
     (void)hipHostFree(hstat_);
     (void)hipHostFree(hhdr_);
@@ -282,10 +281,6 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     if (mode != mode_ || (float)tau != tau_) graph_valid_ = false;
     mode_ = mode;
     tau_ = (float)tau;
-  }
-  void set_lean_shapes(uint64_t n) override {
-    if (n != lean_shapes_) graph_valid_ = false;
-    lean_shapes_ = n;
   }
   void set_profiling(bool on) override { prof_ = on; }
   void set_pair_search(int mode) override {
@@ -999,10 +994,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
   }
 
-  // Exclusive scan on the step stream: one workgroup in a lean step.
+  // Exclusive scan on the step stream.
   void scan(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, const uint64_t* d_n) {
-    if (lean_) scan_excl_u32_one(in, out, n, d_total, s_, d_n);
-    else scan_excl_u32(in, out, n, d_total, scan_tmp_.as<uint32_t>(), s_, d_n);
+    scan_excl_u32(in, out, n, d_total, scan_tmp_.as<uint32_t>(), s_, d_n);
   }
 
   void stage_aabb() {
@@ -1135,7 +1129,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits), 16) * 4);
     mark(ST_BP_SORT, true);
     if (radix_sort_u64(keys_.as<uint64_t>(), keys_tmp_.as<uint64_t>(), mcap_, 32, key_bits, radix_tmp_.as<uint32_t>(),
-                       s_, &st()->M, lean_))
+                       s_, &st()->M))
       swap_bufs(keys_, keys_tmp_);
     mark(ST_BP_SORT, false);
   }
@@ -1302,36 +1296,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     np_kind_.reserve(P);
     PairOut<R> o{np_v_.as<PairRec<R>>(), np_hit_.as<uint32_t>(), P, np_kind_.as<uint8_t>()};
     const uint64_t* dP = &st()->P;
-    if (lean_) {
-      // a lean step: every pair's narrowphase in one kernel, no queue
-      auto fused = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(blocks_for(P, FUSED_BS)), dim3(FUSED_BS), 0, s_, P, dP,
-                           (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(),
-                           (const ShapeHdr*)hdr_.as<ShapeHdr>(), (const ShapeRecF*)frec_.as<ShapeRecF>(), tau_,
-                           envelope_, o, st());
-      };
-      mark(ST_NP_RESOLVE, true);
-      if (mode_ == NP_NATIVE) fused(k_np_fused<R, R, false, false>);
-      else if (mode_ == NP_FAST) fused(k_np_fused<R, double, true, true>);
-      else fused(k_np_fused<R, double, true, false>);
-      mark(ST_NP_RESOLVE, false);
-      mark(ST_NP_COMPACT, true);
-      scan(np_hit_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->C, dP);
-      out_.reserve(P * (10 * sizeof(R) + 8));
-      perm_.reserve(P * 8);
-      hipLaunchKernelGGL(k_np_compact<R>, dim3(blocks_for(P, COMPACT_BS)), dim3(COMPACT_BS), 0, s_, P, dP, o,
-                         (const uint32_t*)np_pos_.as<uint32_t>(), (const uint64_t*)pairs_.as<uint64_t>(),
-                         (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, (const DevStatus*)st(), out_.as<R>(),
-                         perm_.as<uint64_t>());
-      mark(ST_NP_COMPACT, false);
-      return;
-    }
     mark(ST_NP_MAIN, true);
     hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, dP,
                        (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(),
                        (const ShapeHdr*)hdr_.as<ShapeHdr>(), (const ShapeRecF*)frec_.as<ShapeRecF>(), tau_,
                        (int)(mode_ != NP_NATIVE), envelope_, o, qsel_.as<uint32_t>(), qcls_.as<uint8_t>());
-    scan_excl_u32(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->Q, scan_tmp_.as<uint32_t>(), s_, dP);
+    scan(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->Q, dP);
     hipLaunchKernelGGL(k_queue_compact, dim3(blocks_for(P, 256)), dim3(256), 0, s_, P, dP,
                        (const uint32_t*)qsel_.as<uint32_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
                        (const uint8_t*)qcls_.as<uint8_t>(), queue_.as<uint64_t>());
@@ -1355,7 +1325,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       };
       if (mode_ == NP_FAST) filt(k_np_filter<R, true>);
       else filt(k_np_filter<R, false>);
-      scan_excl_u32(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), q, &st()->Q2, scan_tmp_.as<uint32_t>(), s_, dq);
+      scan(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), q, &st()->Q2, dq);
       hipLaunchKernelGGL(k_queue_select, dim3(blocks_for(q, 256)), dim3(256), 0, s_, q, dq,
                          (const uint64_t*)queue_.as<uint64_t>(), (const uint32_t*)qsel_.as<uint32_t>(),
                          (const uint32_t*)np_pos_.as<uint32_t>(), queue_tmp_.as<uint64_t>());
@@ -1387,7 +1357,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_NP_MPR, false);
     mark(ST_NP_RESOLVE, false);
     mark(ST_NP_COMPACT, true);
-    scan_excl_u32(np_hit_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->C, scan_tmp_.as<uint32_t>(), s_, dP);
+    scan(np_hit_.as<uint32_t>(), np_pos_.as<uint32_t>(), P, &st()->C, dP);
     // contacts are at most the pairs: the outputs are sized by the pair capacity
     out_.reserve(P * (10 * sizeof(R) + 8));
     perm_.reserve(P * 8);
@@ -1463,31 +1433,22 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   void launch_capacity_step(bool use_graph) {
     prepare_capacity_buffers();  // every buffer is sized before recording
-    const int v = total_selected_shapes() <= lean_shapes_ ? 1 : 0;
-    struct Lean {  // also on an exception
-      bool& f;
-      Lean(bool& f_, bool on) : f(f_) { f = on; }
-      ~Lean() { f = false; }
-    };
     CDX_HIP_CHECK(hipEventRecord(ev_[2 * ST_COUNT], s_));
     if (use_graph) {
-      if (!graph_valid_ || !graph_exec_[v]) {
+      if (!graph_valid_) {
         // Destroy, record and instantiate with no other library thread inside
         // HIP: other size classes launch and synchronise their own graphs
         // concurrently otherwise, and this stream's last launch of an old
         // graph must have ended before its executable is freed.
         GateExclusive gate;
         CDX_HIP_CHECK(hipStreamSynchronize(s_));
-        if (!graph_valid_) {
-          for (hipGraphExec_t& g : graph_exec_) {
-            if (g) CDX_HIP_CHECK(hipGraphExecDestroy(g));
-            g = nullptr;
-          }
+        if (graph_exec_) {
+          CDX_HIP_CHECK(hipGraphExecDestroy(graph_exec_));
+          graph_exec_ = nullptr;
         }
         hipGraph_t g = nullptr;
         CDX_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         try {
-          Lean lean(lean_, v == 1);
           enqueue_capacity_step();
         } catch (...) {
           // close the capture so the stream stays usable, then report
@@ -1495,15 +1456,14 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
           throw;
         }
         CDX_HIP_CHECK(hipStreamEndCapture(s_, &g));
-        CDX_HIP_CHECK(hipGraphInstantiate(&graph_exec_[v], g, nullptr, nullptr, 0));
+        CDX_HIP_CHECK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
         CDX_HIP_CHECK(hipGraphDestroy(g));
         graph_valid_ = true;
         graph_key_bits_ = key_bits_;
         graph_shape_bits_ = shape_bits_;
       }
-      CDX_HIP_CHECK(hipGraphLaunch(graph_exec_[v], s_));
+      CDX_HIP_CHECK(hipGraphLaunch(graph_exec_, s_));
     } else {
-      Lean lean(lean_, v == 1);
       enqueue_capacity_step();
     }
     CDX_HIP_CHECK(hipEventRecord(ev_[2 * ST_COUNT + 1], s_));
@@ -1604,11 +1564,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   std::vector<SceneOut> souts_host_;
 
   // graph of a capacity step
-  // recorded steps: [0] the full step, [1] the lean step (one-workgroup
-  // scans, fused narrowphase) for selections of at most lean_shapes_ shapes
-  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
-  bool lean_ = false;  // the step being enqueued is lean
-  uint64_t lean_shapes_ = kLeanShapes;
+  hipGraphExec_t graph_exec_ = nullptr;
   bool graph_valid_ = false;
   int graph_key_bits_ = -1, graph_shape_bits_ = -1;
   bool pending_step_ = false, pending_graph_ = false;

@@ -210,40 +210,6 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* _
   if (blockIdx.x == nb - 1 && threadIdx.x == 0) *d_total = offset + part[blockIdx.x];
 }
 
-// Exclusive scan by one workgroup, tile after tile with a running carry: a
-// single launch for small inputs, where the two launches of the
-// reduce-then-scan cost more than the work.
-constexpr int SCAN1_BS = 1024;
-constexpr uint64_t SCAN1_TILE = (uint64_t)SCAN1_BS * SCAN_IPT;
-
-__global__ __launch_bounds__(SCAN1_BS) void k_scan_one(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                       uint64_t cap, const uint64_t* __restrict__ d_n,
-                                                       uint64_t* __restrict__ d_total) {
-  __shared__ uint32_t scratch[SCAN1_BS / WAVE + 1];
-  const uint64_t n = eff_n(cap, d_n);
-  uint64_t carry = 0;
-  for (uint64_t t0 = 0; t0 < n; t0 += SCAN1_TILE) {
-    const uint64_t base = t0 + (uint64_t)threadIdx.x * SCAN_IPT;
-    uint32_t v[SCAN_IPT];
-    load_run(in, base, n, v);
-    uint32_t s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_IPT; ++i) s += v[i];
-    uint32_t tot;
-    const uint32_t pre = block_excl_scan<SCAN1_BS>(s, scratch, &tot);
-    uint32_t run = (uint32_t)carry + pre;
-#pragma unroll
-    for (int i = 0; i < SCAN_IPT; ++i) {
-      const uint32_t x = v[i];
-      v[i] = run;
-      run += x;
-    }
-    store_run(out, base, n, v);
-    carry += tot;
-  }
-  if (threadIdx.x == 0) *d_total = carry;
-}
-
 __global__ void k_copy_u64(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, uint64_t cap,
                            const uint64_t* __restrict__ d_n) {
   const uint64_t n = eff_n(cap, d_n);
@@ -413,21 +379,15 @@ RadixPlan radix_plan(uint64_t n, int nbits) {
 
 template <int DB>
 void radix_pass(const uint64_t* src, uint64_t* dst, uint64_t n, const uint64_t* d_n, int shift, uint32_t* hist,
-                uint64_t* total, uint32_t* stmp, uint32_t nb, hipStream_t s, bool one_wg_scan) {
+                uint64_t* total, uint32_t* stmp, uint32_t nb, hipStream_t s) {
   const uint64_t h = (uint64_t)(1u << DB) * nb;
   hipLaunchKernelGGL(k_radix_upsweep<DB>, dim3(nb), dim3(RS_BS), 0, s, src, n, d_n, shift, hist, nb);
-  if (one_wg_scan) scan_excl_u32_one(hist, hist, h, total, s, nullptr);
-  else scan_excl_u32(hist, hist, h, total, stmp, s, nullptr);
+  scan_excl_u32(hist, hist, h, total, stmp, s, nullptr);
   hipLaunchKernelGGL(k_radix_downsweep<DB>, dim3(nb), dim3(RS_BS), 0, s, src, dst, n, d_n, shift,
                      (const uint32_t*)hist, nb);
 }
 
 }  // namespace
-
-void scan_excl_u32_one(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, hipStream_t s,
-                       const uint64_t* d_n) {
-  hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(SCAN1_BS), 0, s, in, out, n, d_n, d_total);
-}
 
 size_t scan_tmp_words(uint64_t n) {
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -460,7 +420,7 @@ size_t radix_tmp_words(uint64_t n, int nbits) {
 }
 
 bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, int lo_bit, int nbits, uint32_t* hist_tmp,
-                    hipStream_t s, const uint64_t* d_n, bool one_wg_scans) {
+                    hipStream_t s, const uint64_t* d_n) {
   const RadixPlan pl = radix_plan(n, nbits);
   if (n <= 1 || pl.passes == 0) return false;
   const uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
@@ -473,10 +433,10 @@ bool radix_sort_u64(uint64_t* data, uint64_t* tmp, uint64_t n, int lo_bit, int n
   for (int p = 0; p < pl.passes; ++p) {
     const int shift = lo_bit + p * pl.db;
     switch (pl.db) {
-      case 8: radix_pass<8>(src, dst, n, d_n, shift, hist, total, stmp, nb, s, one_wg_scans); break;
-      case 9: radix_pass<9>(src, dst, n, d_n, shift, hist, total, stmp, nb, s, one_wg_scans); break;
-      case 10: radix_pass<10>(src, dst, n, d_n, shift, hist, total, stmp, nb, s, one_wg_scans); break;
-      default: radix_pass<11>(src, dst, n, d_n, shift, hist, total, stmp, nb, s, one_wg_scans); break;
+      case 8: radix_pass<8>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
+      case 9: radix_pass<9>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
+      case 10: radix_pass<10>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
+      default: radix_pass<11>(src, dst, n, d_n, shift, hist, total, stmp, nb, s); break;
     }
     uint64_t* t = src; src = dst; dst = t;
   }

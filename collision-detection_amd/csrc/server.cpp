@@ -182,7 +182,6 @@ Server::Server(int device, const ServerConfig& cfg) : dev_(device), cfg_(cfg), t
     k.eng = make_batch(cfg.real_kind, dev_);
     k.eng->set_np_mode(cfg.np_mode, cfg.tau);
     k.eng->set_envelope(cfg.envelope);
-    k.eng->set_lean_shapes(cfg.lean_shapes);
   }
   st_t0_ = now_ms();
   for (int c = 0; c < kNumClasses; ++c) classes_[c]->worker = std::thread(&Server::worker_loop, this, c);

@@ -32,7 +32,6 @@ struct ServerConfig {
   uint64_t max_batch_shapes = 1u << 19;  // shapes per batched step (bounds step latency)
   uint64_t max_pending = 1u << 16;       // queued requests before submit refuses
   bool use_graph = true;
-  uint64_t lean_shapes = kLeanShapes;  // largest lean step (BatchBase::set_lean_shapes)
 };
 
 struct ServerStats {

@@ -287,11 +287,6 @@ int cdx_batch_update_bodies(cdx_batch* batch, uint32_t scene, uint64_t k, const 
                             const void* body_rot);
 int cdx_batch_set_envelope(cdx_batch* batch, double envelope);
 int cdx_batch_set_narrowphase_mode(cdx_batch* batch, int np_mode, double tau);
-/* Steps over at most max_shapes shapes run lean (default 16384): scans of
- * one workgroup and each pair's whole narrowphase in one kernel, about half
- * the launches of a full step.  Results are identical either way; 0 makes
- * every step a full one. */
-int cdx_batch_set_lean_shapes(cdx_batch* batch, uint64_t max_shapes);
 /* Scenes of the following steps (scenes = NULL: every resident scene). */
 int cdx_batch_select(cdx_batch* batch, const uint32_t* scenes, uint64_t k);
 /* Enqueues one step on the batch's stream (use_graph: record / replay). */
@@ -337,8 +332,6 @@ typedef struct cdx_server_config {
   uint64_t max_batch_shapes; /* shapes per step and class (bounds step latency); 0 = 524288 */
   uint64_t max_pending;      /* queued requests before submit returns CDX_ERR_CAPACITY; 0 = 65536 */
   int use_graph;             /* replay recorded HIP graphs */
-  int64_t lean_shapes;       /* steps of at most this many shapes run lean (cdx_batch_set_lean_shapes);
-                                0: the default (16384), < 0: never */
 } cdx_server_config;
 
 typedef struct cdx_server_stats_t {

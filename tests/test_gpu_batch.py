@@ -175,36 +175,33 @@ def test_batch_errors(cdx, ctx, scenes):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("mode", ["mixed", "native", "fast"])
-def test_lean_step_equals_full_step(cdx, ctx, scenes, mode, dtype):
-    """A lean step (one-workgroup scans, each pair's narrowphase in one
-    kernel) gives the same pairs, contacts and re-solve counts as the full
-    step (class-sorted queue) on the same selection, every shape type
-    included, with graphs and eagerly."""
+def test_graph_step_equals_eager_step_all_types(cdx, ctx, scenes, mode, dtype):
+    """A replayed graph step gives the same pairs, contacts and re-solve
+    counts as the same step launched eagerly, on every selection in turn
+    (the graph is recorded once), every shape type included."""
     m = {"mixed": cdx.NP_MIXED, "native": cdx.NP_NATIVE, "fast": cdx.NP_FAST}[mode]
     ss = [scenes.all_shape_types(n=3000, seed=40, dtype=dtype),
           scenes.mixed_scene_of_size(5000, 6.0, 41, dtype),
           scenes.two_shapes_per_body(scenes.mixed_scene_of_size(1200, 4.0, 42, dtype)),
           scenes.all_shape_types(n=800, seed=43, dtype=dtype)]
-    print(f"lean vs full mode={mode} dtype={np.dtype(dtype).name}", flush=True)
+    print(f"graph vs eager mode={mode} dtype={np.dtype(dtype).name}", flush=True)
     res = {}
-    for lean in (0, 1 << 20):
-        for use_graph in (True, False):
-            b = cdx.Batch(ctx, dtype)
-            for s in ss:
-                b.add_scene(s)
-            b.set_mode(m, 1e-2)
-            b.set_lean_shapes(lean)
-            out = []
-            for sel in (None, [2], [3, 0]):  # all scenes, then two sub-selections
-                b.select(sel)
-                b.step(use_graph)
-                tot = b.finish()
-                js = range(len(ss)) if sel is None else range(len(sel))
-                out.append((tot.pairs, tot.contacts, tot.resolved,
-                            [(b.read_pairs(j), b.read_contacts(j)) for j in js]))
-            b.close()
-            res[(lean, use_graph)] = out
-    ref = res[(0, True)]
+    for use_graph in (True, False):
+        b = cdx.Batch(ctx, dtype)
+        for s in ss:
+            b.add_scene(s)
+        b.set_mode(m, 1e-2)
+        out = []
+        for sel in (None, [2], [3, 0], None):  # all scenes, two sub-selections, all again
+            b.select(sel)
+            b.step(use_graph)
+            tot = b.finish()
+            js = range(len(ss)) if sel is None else range(len(sel))
+            out.append((tot.pairs, tot.contacts, tot.resolved,
+                        [(b.read_pairs(j), b.read_contacts(j)) for j in js]))
+        b.close()
+        res[use_graph] = out
+    ref = res[False]
     for key, got in res.items():
         for (p0, c0, r0, sc0), (p1, c1, r1, sc1) in zip(ref, got):
             assert (p1, c1, r1) == (p0, c0, r0), f"{key}: totals {(p1, c1, r1)} vs {(p0, c0, r0)}"
