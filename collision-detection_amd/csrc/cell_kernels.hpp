@@ -174,6 +174,21 @@ __device__ __forceinline__ void cell_ranges(const uint32_t c[3], const CellGrid&
   }
 }
 
+// Candidate range of one search row r (as in cell_ranges).
+__device__ __forceinline__ void cell_row_range(const uint32_t c[3], const CellGrid& cg,
+                                               const uint32_t* __restrict__ cstart,
+                                               const uint32_t* __restrict__ wmax, uint32_t a, int r, uint32_t& s,
+                                               uint32_t& len) {
+  const uint32_t gx = cg.g[0], gy = cg.g[1], gz = cg.g[2];
+  const uint32_t x0 = c[0] > cg.kx ? c[0] - cg.kx : 0, x1 = min(c[0] + cg.kx, gx - 1);
+  const int zz = (int)c[2] + r / 3 - 1, yy = (int)c[1] + r % 3 - 1;
+  const bool in = zz >= 0 && zz < (int)gz && yy >= 0 && yy < (int)gy;
+  const uint32_t row = in ? ((uint32_t)zz * gy + (uint32_t)yy) * gx : 0u;
+  const uint32_t b = cstart[row + x0], e = cstart[row + x1 + 1], m = wmax[row + x0];
+  s = b;
+  len = in && m > a ? e - b : 0u;
+}
+
 // Partners b > a of shape A, in walk order: hit(b) for each (closed
 // overlap test, Broadphase.cu:237).  Row by row, CELL_ROUND candidates per
 // round with all their loads issued first.  Going row by row keeps the
@@ -232,9 +247,56 @@ __device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t nb) {
   return x * q + min(x, rem) + k;
 }
 
+// Wavefront-wide minimum and maximum (every lane active).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+
+// LDS written by one lane and read by another of the same wavefront: the
+// wavefront's LDS operations run in order; this keeps the compiler from
+// moving them across each other.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The closed overlap test of candidate (L, H) against shape A with L's
+// index above A's (Broadphase.cu:237), on the active lanes: the seven
+// comparisons are combined as wavefront lane masks (scalar ANDs), which
+// the compiler otherwise packs into per-lane bit vectors at twice the
+// vector instructions.
+template <class R>
+__device__ __forceinline__ bool cell_hit(const CellSelf<R>& A, const CellLo<R>& L, const CellHi<R>& H) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(L.id > A.id) & __builtin_amdgcn_ballot_w64(L.lo[0] <= A.hi[0]) &
+                     __builtin_amdgcn_ballot_w64(L.lo[1] <= A.hi[1]) & __builtin_amdgcn_ballot_w64(L.lo[2] <= A.hi[2]) &
+                     __builtin_amdgcn_ballot_w64(A.lo[0] <= H.hi[0]) & __builtin_amdgcn_ballot_w64(A.lo[1] <= H.hi[1]) &
+                     __builtin_amdgcn_ballot_w64(A.lo[2] <= H.hi[2]);
+  return __builtin_amdgcn_inverse_ballot_w64(m);
+}
+
+// Entries a wavefront stages in LDS per window.
+constexpr int CELL_WIN = 2 * WAVE;
+
 // Search pass: per sorted position, the shape's partner count (scount)
 // and its first CELL_SLAB partners in walk order (slab row of the shape);
 // shapes with more are listed (ovf) to be searched again at placement.
+//
+// The 64 lanes of a wavefront hold neighbouring shapes in cell order, so
+// their candidate ranges in a search row overlap.  Per row the wavefront
+// loads the union of its lanes' ranges once, coalesced, in windows of
+// CELL_WIN entries starting at the lowest position some lane still needs
+// (lanes whose ranges lie further on wait for a later window: a wavefront
+// that spans two grid rows takes two windows), and each lane tests its
+// own range from LDS.  Each candidate record is then read from L2 once per
+// wavefront instead of once per lane.
 template <class R>
 __global__ __launch_bounds__(CELL_BS) void k_cell_find(uint64_t n, const CellLo<R>* __restrict__ slo,
                                                        const CellHi<R>* __restrict__ shi,
@@ -242,19 +304,101 @@ __global__ __launch_bounds__(CELL_BS) void k_cell_find(uint64_t n, const CellLo<
                                                        const uint32_t* __restrict__ wmax, CellGrid cg,
                                                        uint32_t* __restrict__ scount, uint32_t* __restrict__ slab,
                                                        uint32_t* __restrict__ ovf, DevStatus* st) {
+  // records move as 16-byte words (both record types are multiples of 16 bytes)
+  constexpr int QL = sizeof(CellLo<R>) / sizeof(uint4), QH = sizeof(CellHi<R>) / sizeof(uint4);
+  static_assert(sizeof(CellLo<R>) % sizeof(uint4) == 0 && sizeof(CellHi<R>) % sizeof(uint4) == 0, "record size");
+  __shared__ uint4 l_lo[CELL_BS / WAVE][CELL_WIN * QL];
+  __shared__ uint4 l_hi[CELL_BS / WAVE][CELL_WIN * QH];
   const uint64_t p = xcd_tile(blockIdx.x, gridDim.x) * CELL_BS + threadIdx.x;
-  if (p >= n) return;
-  const CellSelf<R> A = cell_self(slo, shi, p);
-  uint32_t c[3];
-  cell_of_box(A.lo, cg, c);
-  uint32_t s[CELL_ROWS], len[CELL_ROWS];
-  cell_ranges(c, cg, cstart, wmax, A.id, s, len);
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  uint4* wlo = l_lo[wv];
+  uint4* whi = l_hi[wv];
+  const uint4* glo = reinterpret_cast<const uint4*>(slo);
+  const uint4* ghi = reinterpret_cast<const uint4*>(shi);
+  const bool valid = p < n;  // no early exit: every lane takes part in the wavefront reductions
+  CellSelf<R> A{};
+  uint32_t c[3] = {0, 0, 0};
+  uint32_t sn = 0, ln = 0;  // range of the next row, loaded one row ahead
+  if (valid) {
+    A = cell_self(slo, shi, p);
+    cell_of_box(A.lo, cg, c);
+    cell_row_range(c, cg, cstart, wmax, A.id, 0, sn, ln);
+  }
+  // partners go to the slab row four at a time (hb holds those not yet stored)
   uint32_t cnt = 0;
-  uint32_t* row = slab + (uint64_t)A.id * CELL_SLAB;
-  cell_find<R>(A, s, len, slo, shi, [&](uint32_t b) {
-    if (cnt < (uint32_t)CELL_SLAB) row[cnt] = b;
-    ++cnt;
-  });
+  uint4* row4 = reinterpret_cast<uint4*>(slab + (uint64_t)A.id * CELL_SLAB);
+  uint4 hb = make_uint4(0u, 0u, 0u, 0u);
+  for (int r = 0; r < CELL_ROWS; ++r) {
+    uint32_t cur = sn;
+    const uint32_t end = sn + ln;
+    const uint64_t busy = __builtin_amdgcn_ballot_w64(ln != 0u);
+    if (valid && r + 1 < CELL_ROWS) cell_row_range(c, cg, cstart, wmax, A.id, r + 1, sn, ln);
+    if (!busy) continue;  // wave-uniform
+    // Lanes hold consecutive positions in cell order and a row's start and
+    // end are monotone in the cell key, so the busy lanes' ranges do not
+    // decrease with the lane: the union ends at the last busy lane's end and
+    // the lowest position still needed is the first unfinished lane's.  A
+    // wavefront reduction backs both up (never taken when the order holds).
+    uint32_t last = (uint32_t)__builtin_amdgcn_readlane((int)end, 63 - __builtin_clzll(busy));
+    if (__builtin_amdgcn_ballot_w64(ln != 0u && end > last)) last = wave_max_u32(ln ? end : 0u);
+    for (;;) {
+      const uint64_t act = __builtin_amdgcn_ballot_w64(cur < end);
+      if (!act) break;  // wave-uniform
+      uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)cur, __builtin_ctzll(act));
+      if (__builtin_amdgcn_ballot_w64(cur < end && cur < w0)) w0 = wave_min_u32(cur < end ? cur : ~0u);
+      const uint32_t w1 = min(w0 + (uint32_t)CELL_WIN, last);
+      const uint32_t k0 = w0 + lane, k1 = k0 + WAVE;
+      uint4 L0[QL], L1[QL], H0[QH], H1[QH];
+      if (k0 < w1) {
+#pragma unroll
+        for (int q = 0; q < QL; ++q) L0[q] = glo[(uint64_t)k0 * QL + q];
+#pragma unroll
+        for (int q = 0; q < QH; ++q) H0[q] = ghi[(uint64_t)k0 * QH + q];
+      }
+      if (k1 < w1) {
+#pragma unroll
+        for (int q = 0; q < QL; ++q) L1[q] = glo[(uint64_t)k1 * QL + q];
+#pragma unroll
+        for (int q = 0; q < QH; ++q) H1[q] = ghi[(uint64_t)k1 * QH + q];
+      }
+      wave_lds_sync();  // the previous window's reads are done
+      if (k0 < w1) {
+#pragma unroll
+        for (int q = 0; q < QL; ++q) wlo[lane * QL + q] = L0[q];
+#pragma unroll
+        for (int q = 0; q < QH; ++q) whi[lane * QH + q] = H0[q];
+      }
+      if (k1 < w1) {
+#pragma unroll
+        for (int q = 0; q < QL; ++q) wlo[(lane + WAVE) * QL + q] = L1[q];
+#pragma unroll
+        for (int q = 0; q < QH; ++q) whi[(lane + WAVE) * QH + q] = H1[q];
+      }
+      wave_lds_sync();
+      if (cur < end && cur < w1) {
+        const uint32_t e = min(end, w1);
+        const CellLo<R>* tlo = reinterpret_cast<const CellLo<R>*>(wlo) - w0;
+        const CellHi<R>* thi = reinterpret_cast<const CellHi<R>*>(whi) - w0;
+        for (uint32_t j = cur; j < e; ++j) {
+          const CellLo<R>& L = tlo[j];
+          const CellHi<R>& H = thi[j];
+          if (cell_hit(A, L, H)) {
+            const uint32_t q = cnt & 3u;
+            hb.x = q == 0u ? L.id : hb.x;
+            hb.y = q == 1u ? L.id : hb.y;
+            hb.z = q == 2u ? L.id : hb.z;
+            hb.w = q == 3u ? L.id : hb.w;
+            if (q == 3u && cnt < (uint32_t)CELL_SLAB) row4[cnt >> 2] = hb;
+            ++cnt;
+          }
+        }
+        cur = e;
+      }
+    }
+  }
+  // the last partial group (slots past cnt are never read)
+  if ((cnt & 3u) && cnt < (uint32_t)CELL_SLAB) row4[cnt >> 2] = hb;
+  if (!valid) return;
   scount[A.id] = cnt;
   if (cnt > (uint32_t)CELL_SLAB) ovf[atomicAdd(&st->novf, 1u)] = (uint32_t)p;
 }
