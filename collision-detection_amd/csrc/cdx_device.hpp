@@ -28,6 +28,7 @@ struct DevStatus {
   uint64_t cells;    // shapes counted into the cell search's table (its scan total)
   uint32_t novf;     // cell search: shapes with more partners than the slab holds
   uint32_t pad;
+  uint64_t qhn;      // entries of the MPR queue's per-workgroup class histograms
 };
 
 // What the host sets before a step (copied into DevStatus by k_begin_step,
