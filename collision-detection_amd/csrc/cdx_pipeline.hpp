@@ -157,6 +157,14 @@ class BatchBase {
   virtual void scene_contact_range(uint64_t j, uint64_t* begin, uint64_t* end) const = 0;
   virtual uint64_t contacts_packed_bytes() const = 0;
   virtual void read_all_contacts_packed(void* dst) = 0;
+  // The next step() also copies the first `rows` contact rows of that
+  // packed layout (as laid out for the step's own count) to dst, pinned
+  // host memory of at least rows rows untouched until finish() returns,
+  // right behind the step on its stream.  After finish(),
+  // contacts_prefetched() is true when the step's contacts all fit those
+  // rows: dst then holds what read_all_contacts_packed would copy.
+  virtual void prefetch_contacts(void* dst, uint64_t rows) = 0;
+  virtual bool contacts_prefetched() const = 0;
   virtual uint64_t scene_bytes_estimate(uint64_t n_shapes, uint64_t n_bodies) const = 0;
   // Bytes a scene keeps resident, and the step buffers of a step over n shapes.
   virtual uint64_t resident_bytes_estimate(uint64_t n_shapes, uint64_t n_bodies) const = 0;

@@ -577,7 +577,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     sync();
     if (!pending_step_) return;
     pending_step_ = false;
-    read_status();
+    check_status();  // the totals were copied behind the step
     // Re-run with larger capacities until every total fits.
     for (int attempt = 0; attempt < 4; ++attempt) {
       const DevStatus& st = *hstat_;
@@ -586,9 +586,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       mcap_ = std::max<uint64_t>(mcap_, st.M + st.M / 4 + 1024);
       pcap_ = std::max<uint64_t>(pcap_, st.P + st.P / 4 + 1024);
       graph_valid_ = false;
-      launch_capacity_step(pending_graph_);
+      launch_capacity_step(pending_graph_);  // queues its copies again
       sync();
-      read_status();
+      check_status();
     }
     CDX_REQUIRE(hstat_->overflow == 0 && hstat_->M <= mcap_ && hstat_->P <= pcap_, ERR_CAPACITY,
                 "step totals exceed the buffer capacities");
@@ -596,17 +596,11 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     m_seen_ = std::max(m_seen_, M_);
     p_seen_ = std::max(p_seen_, P_);
     souts_host_.resize(selection_.size());
-    if (!selection_.empty()) {
-      const size_t bytes = selection_.size() * sizeof(SceneOut);
-      if (bytes > souts_pinned_bytes_) {
-        if (souts_pinned_) CDX_HIP_CHECK(hipHostFree(souts_pinned_));
-        souts_pinned_bytes_ = std::max<size_t>(bytes * 2, 4096);
-        CDX_HIP_CHECK(hipHostMalloc(&souts_pinned_, souts_pinned_bytes_, hipHostMallocDefault));
-      }
-      CDX_HIP_CHECK(hipMemcpyAsync(souts_pinned_, souts_.p, bytes, hipMemcpyDeviceToHost, s_));
-      CDX_HIP_CHECK(hipStreamSynchronize(s_));
-      std::memcpy(souts_host_.data(), souts_pinned_, bytes);
-    }
+    if (!selection_.empty())  // copied behind the step
+      std::memcpy(souts_host_.data(), souts_pinned_, selection_.size() * sizeof(SceneOut));
+    pf_ok_ = pf_dst_ != nullptr && C_ <= pf_rows_;
+    pf_dst_ = nullptr;  // one step
+    pf_rows_ = 0;
     counts_.n = total_selected_shapes();
     counts_.m = M_; counts_.a = A_; counts_.p = P_; counts_.c = C_;
     counts_.resolved = mode_ == NP_NATIVE ? 0 : hstat_->Q2;
@@ -698,6 +692,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   }
 
   uint64_t contacts_packed_bytes() const override { return C_ * (10 * sizeof(R) + 8); }
+
+  void prefetch_contacts(void* dst, uint64_t rows) override {
+    pf_dst_ = rows ? dst : nullptr;
+    pf_rows_ = dst ? rows : 0;
+  }
+  bool contacts_prefetched() const override { return pf_ok_; }
 
   void read_all_contacts_packed(void* dst) override {
     if (!C_) return;
@@ -1468,6 +1468,30 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       enqueue_capacity_step();
     }
     CDX_HIP_CHECK(hipEventRecord(ev_[2 * ST_COUNT + 1], s_));
+    enqueue_step_readback();
+  }
+
+  // Copies queued right behind a capacity step on its stream, so finish()
+  // waits once: the totals, the per-scene outputs and (when the caller set
+  // a prefetch target) the first rows of the packed contacts.
+  void enqueue_step_readback() {
+    CDX_HIP_CHECK(hipMemcpyAsync(hstat_, status_.p, sizeof(DevStatus), hipMemcpyDeviceToHost, s_));
+    const size_t sb = selection_.size() * sizeof(SceneOut);
+    if (sb) {
+      if (sb > souts_pinned_bytes_) {
+        CDX_HIP_CHECK(hipStreamSynchronize(s_));  // a previous copy may still target the old block
+        if (souts_pinned_) CDX_HIP_CHECK(hipHostFree(souts_pinned_));
+        souts_pinned_ = nullptr;
+        souts_pinned_bytes_ = 0;
+        CDX_HIP_CHECK(hipHostMalloc(&souts_pinned_, std::max<size_t>(sb * 2, 4096), hipHostMallocDefault));
+        souts_pinned_bytes_ = std::max<size_t>(sb * 2, 4096);
+      }
+      CDX_HIP_CHECK(hipMemcpyAsync(souts_pinned_, souts_.p, sb, hipMemcpyDeviceToHost, s_));
+    }
+    if (pf_dst_ && pf_rows_) {
+      const uint64_t bytes = std::min<uint64_t>(pf_rows_ * (10 * sizeof(R) + 8), out_.bytes);
+      CDX_HIP_CHECK(hipMemcpyAsync(pf_dst_, out_.p, bytes, hipMemcpyDeviceToHost, s_));
+    }
   }
 
   void prepare_capacity_buffers() {
@@ -1516,7 +1540,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void read_status() {
     CDX_HIP_CHECK(hipMemcpyAsync(hstat_, status_.p, sizeof(DevStatus), hipMemcpyDeviceToHost, s_));
     CDX_HIP_CHECK(hipStreamSynchronize(s_));
-    uint32_t bad = hstat_->bad;
+    check_status();
+  }
+  void check_status() const {
+    const uint32_t bad = hstat_->bad;
     CDX_REQUIRE(!(bad & 1u), ERR_UNSUPPORTED_SHAPE, "unsupported shape type");
     CDX_REQUIRE(!(bad & 2u), ERR_INVALID, "shape body id out of range");
     CDX_REQUIRE(!(bad & 4u), ERR_INVALID,
@@ -1580,6 +1607,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   bool stage_pending_ = false;
   void* souts_pinned_ = nullptr;
   size_t souts_pinned_bytes_ = 0;
+  void* pf_dst_ = nullptr;  // contacts prefetch target of the next step
+  uint64_t pf_rows_ = 0;
+  bool pf_ok_ = false;      // every contact of the last finished step is there
   bool desc_dirty_ = true;
   DevBuf type_, body_, lpos_, dims_, cpos_, lrot_, bpos_, brot_, active_;
   DevBuf aabb_, rec_, brec_, hdr_, frec_, bpart_, kpart_;

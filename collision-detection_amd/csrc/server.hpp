@@ -129,6 +129,9 @@ class Server {
     std::thread worker;
     std::vector<std::shared_ptr<ResultBlock>> blocks;  // pool, used by the worker only
     uint64_t max_contacts = 0;
+    // contacts of each scene (by engine index) at its last step, ~0 before
+    // its first (worker only, under eng_mu): sizes the contacts prefetch
+    std::vector<uint64_t> last_contacts;
   };
   struct SceneRef {
     int cls;
