@@ -426,7 +426,9 @@ __device__ __forceinline__ bool fp32_filtered(uint32_t t) { return t <= CYLINDER
 constexpr uint32_t EXT_CLASS = 64;
 constexpr int NQC = 128;            // queue classes (pair_class < NQC)
 constexpr uint32_t NO_CLASS = 255;  // not queued
-constexpr int FRONT_BS = 128;
+constexpr int QUEUE_BS = 256;       // class histograms and placement: a tile of
+constexpr int QUEUE_IPT = 8;        // QUEUE_BS * QUEUE_IPT pairs per workgroup
+constexpr uint64_t QUEUE_TILE = (uint64_t)QUEUE_BS * QUEUE_IPT;
 __device__ __forceinline__ uint32_t pair_class(uint32_t ta, uint32_t tb) {
   const uint32_t c = type_slot(ta) * 8u + type_slot(tb);
   return fp32_filtered(ta) && fp32_filtered(tb) ? c : EXT_CLASS + c;
@@ -461,13 +463,13 @@ template <> __device__ __forceinline__ ShapeRec<float> as_compute<float, float>(
 
 // Front pass over all candidate pairs (function_MPR_Store's early exits and
 // sphere-sphere closed form, Narrowphase.cu:338-396).  Pairs that need the
-// MPR get their shape-pair class (qcls, NO_CLASS for the others), and each
-// workgroup stores its class histogram class-major (hist[c * nb + b], nb
-// workgroups holding pairs); one scan of those histograms gives every
-// (class, workgroup) its first queue slot, and k_queue_scatter writes the
-// queue of (class << 32 | pair index) grouped by class: a one-digit
-// counting sort whose count pass is the front pass itself, so every
-// wavefront of the MPR kernels runs one pair of support maps.
+// MPR get their shape-pair class (qcls, NO_CLASS for the others).  The queue
+// of (class << 32 | pair index) is then grouped by class with a one-digit
+// counting sort over tiles of QUEUE_TILE pairs: k_queue_hist stores each
+// tile's class histogram class-major (hist[c * nt + t], nt tiles), one scan
+// gives every (class, tile) its first queue slot, and k_queue_scatter places
+// the tile's pairs, so every wavefront of the MPR kernels runs one pair of
+// support maps.
 // MIXED / FAST then run an FP32 MPR in the frame of shape A (k_np_filter):
 // clear misses end there, FAST keeps clear FP32 hits, everything else goes
 // to the reference-order MPR.  In those modes the front pass already applies
@@ -535,50 +537,65 @@ __global__ __launch_bounds__(128) void k_np_front(uint64_t cap, const uint64_t* 
                                                   const uint64_t* __restrict__ pairs,
                                                   const ShapeRec<R>* __restrict__ rec, const ShapeHdr* __restrict__ hdr,
                                                   const ShapeRecF* __restrict__ frec, float tau_rel, int prefilter,
-                                                  R envelope, PairOut<R> o, uint8_t* __restrict__ qcls,
-                                                  uint32_t* __restrict__ hist, uint64_t* __restrict__ hist_n) {
-  static_assert(NQC == FRONT_BS, "one histogram slot per thread");
+                                                  R envelope, PairOut<R> o, uint8_t* __restrict__ qcls) {
+  const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  if (i >= dev_count(cap, d_np)) return;
+  const uint64_t pr = pairs[i];
+  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+  uint32_t ta, tb;
+  const bool enq = np_front_one(i, ia, ib, rec, hdr, frec, tau_rel, prefilter, envelope, o, ta, tb);
+  qcls[i] = (uint8_t)(enq ? pair_class(ta, tb) : NO_CLASS);
+}
+
+// Class histogram of one tile of QUEUE_TILE pairs (pair k * QUEUE_BS + t of
+// the tile on thread t), stored class-major: hist[c * nt + tile].  Tile 0
+// writes the number of histogram entries (the scan's length).
+__global__ __launch_bounds__(QUEUE_BS) void k_queue_hist(uint64_t cap, const uint64_t* __restrict__ d_np,
+                                                         const uint8_t* __restrict__ qcls,
+                                                         uint32_t* __restrict__ hist, uint64_t* __restrict__ hist_n) {
   __shared__ uint32_t h[NQC];
   const uint64_t n = dev_count(cap, d_np);
-  const uint64_t nb = (n + FRONT_BS - 1) / FRONT_BS;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *hist_n = (uint64_t)NQC * nb;
-  if (blockIdx.x >= nb) return;  // workgroup-uniform
-  h[threadIdx.x] = 0;
+  const uint64_t nt = (n + QUEUE_TILE - 1) / QUEUE_TILE;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hist_n = (uint64_t)NQC * nt;
+  if (blockIdx.x >= nt) return;  // workgroup-uniform
+  if (threadIdx.x < NQC) h[threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * FRONT_BS + threadIdx.x;
-  if (i < n) {
-    const uint64_t pr = pairs[i];
-    const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
-    uint32_t ta, tb;
-    const bool enq = np_front_one(i, ia, ib, rec, hdr, frec, tau_rel, prefilter, envelope, o, ta, tb);
-    const uint32_t c = enq ? pair_class(ta, tb) : NO_CLASS;
-    qcls[i] = (uint8_t)c;
+  const uint64_t t0 = (uint64_t)blockIdx.x * QUEUE_TILE + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < QUEUE_IPT; ++k) {
+    const uint64_t i = t0 + (uint64_t)k * QUEUE_BS;
+    const uint32_t c = i < n ? qcls[i] : NO_CLASS;
     if (c != NO_CLASS) atomicAdd(&h[c], 1u);
   }
   __syncthreads();
-  hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+  if (threadIdx.x < NQC) hist[(uint64_t)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
 }
 
 // Queue of the pairs the front pass kept, grouped by class: a pair of class
-// c in workgroup b takes a slot of [base, base + h) with base the scanned
-// hist[c * nb + b].  Within a workgroup's run of a class, slots follow an
-// LDS atomic; order within a class carries no meaning (each pair's result is
+// c in tile t takes a slot of the run that starts at the scanned
+// hist[c * nt + t].  Within a tile's run of a class, slots follow an LDS
+// atomic; order within a class carries no meaning (each pair's result is
 // written at the pair's own index).
-__global__ __launch_bounds__(FRONT_BS) void k_queue_scatter(uint64_t cap, const uint64_t* __restrict__ d_np,
+__global__ __launch_bounds__(QUEUE_BS) void k_queue_scatter(uint64_t cap, const uint64_t* __restrict__ d_np,
                                                             const uint8_t* __restrict__ qcls,
                                                             const uint32_t* __restrict__ base,
                                                             uint64_t* __restrict__ queue) {
-  __shared__ uint32_t lcnt[NQC];
+  __shared__ uint32_t lcnt[NQC], lbase[NQC];
   const uint64_t n = dev_count(cap, d_np);
-  const uint64_t nb = (n + FRONT_BS - 1) / FRONT_BS;
-  if (blockIdx.x >= nb) return;  // workgroup-uniform
-  lcnt[threadIdx.x] = 0;
+  const uint64_t nt = (n + QUEUE_TILE - 1) / QUEUE_TILE;
+  if (blockIdx.x >= nt) return;  // workgroup-uniform
+  if (threadIdx.x < NQC) {
+    lcnt[threadIdx.x] = 0;
+    lbase[threadIdx.x] = base[(uint64_t)threadIdx.x * nt + blockIdx.x];
+  }
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * FRONT_BS + threadIdx.x;
-  const uint32_t c = i < n ? qcls[i] : NO_CLASS;
-  if (c == NO_CLASS) return;
-  const uint32_t r = atomicAdd(&lcnt[c], 1u);
-  queue[base[(uint64_t)c * nb + blockIdx.x] + r] = ((uint64_t)c << 32) | i;
+  const uint64_t t0 = (uint64_t)blockIdx.x * QUEUE_TILE + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < QUEUE_IPT; ++k) {
+    const uint64_t i = t0 + (uint64_t)k * QUEUE_BS;
+    const uint32_t c = i < n ? qcls[i] : NO_CLASS;
+    if (c != NO_CLASS) queue[lbase[c] + atomicAdd(&lcnt[c], 1u)] = ((uint64_t)c << 32) | i;
+  }
 }
 
 // FP32 MPR over the class-sorted queue (MIXED / FAST), in the frame of

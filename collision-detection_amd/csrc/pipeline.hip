@@ -994,9 +994,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
   }
 
-  // Entries of the front pass's class histograms for up to P pairs (at
-  // least P: the buffer also takes scans of P flags).
-  static uint64_t queue_hist_words(uint64_t P) { return (uint64_t)NQC * blocks_for(P, FRONT_BS); }
+  // Entries of the MPR queue's class histograms for up to P pairs.
+  static uint64_t queue_hist_words(uint64_t P) { return (uint64_t)NQC * ((P + QUEUE_TILE - 1) / QUEUE_TILE); }
 
   // Exclusive scan on the step stream.
   void scan(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, const uint64_t* d_n) {
@@ -1290,25 +1289,27 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     const uint64_t P = std::max<uint64_t>(pcap_, 1);
     np_v_.reserve(P * sizeof(PairRec<R>));
     np_hit_.reserve(P * 4);
-    np_pos_.reserve(queue_hist_words(P) * 4);
+    np_pos_.reserve(std::max(queue_hist_words(P), P) * 4);
     queue_.reserve(P * 8);
     queue_tmp_.reserve(P * 8);
     qsel_.reserve(P * 4);
     qcls_.reserve(P);
-    scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(queue_hist_words(P)), 16) * 4);
+    scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(std::max(queue_hist_words(P), P)), 16) * 4);
     np_kind_.reserve(P);
     PairOut<R> o{np_v_.as<PairRec<R>>(), np_hit_.as<uint32_t>(), P, np_kind_.as<uint8_t>()};
     const uint64_t* dP = &st()->P;
     mark(ST_NP_MAIN, true);
-    hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, FRONT_BS)), dim3(FRONT_BS), 0, s_, P, dP,
+    hipLaunchKernelGGL(k_np_front<R>, dim3(blocks_for(P, 128)), dim3(128), 0, s_, P, dP,
                        (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(),
                        (const ShapeHdr*)hdr_.as<ShapeHdr>(), (const ShapeRecF*)frec_.as<ShapeRecF>(), tau_,
-                       (int)(mode_ != NP_NATIVE), envelope_, o, qcls_.as<uint8_t>(), np_pos_.as<uint32_t>(), &st()->qhn);
-    // class histograms -> first slot per (class, workgroup); the total is the queue length
+                       (int)(mode_ != NP_NATIVE), envelope_, o, qcls_.as<uint8_t>());
+    // class histograms per tile -> first slot per (class, tile); the total is the queue length
+    const unsigned qtiles = (unsigned)((P + QUEUE_TILE - 1) / QUEUE_TILE);
+    hipLaunchKernelGGL(k_queue_hist, dim3(qtiles), dim3(QUEUE_BS), 0, s_, P, dP, (const uint8_t*)qcls_.as<uint8_t>(),
+                       np_pos_.as<uint32_t>(), &st()->qhn);
     scan(np_pos_.as<uint32_t>(), np_pos_.as<uint32_t>(), queue_hist_words(P), &st()->Q, &st()->qhn);
-    hipLaunchKernelGGL(k_queue_scatter, dim3(blocks_for(P, FRONT_BS)), dim3(FRONT_BS), 0, s_, P, dP,
-                       (const uint8_t*)qcls_.as<uint8_t>(), (const uint32_t*)np_pos_.as<uint32_t>(),
-                       queue_.as<uint64_t>());
+    hipLaunchKernelGGL(k_queue_scatter, dim3(qtiles), dim3(QUEUE_BS), 0, s_, P, dP, (const uint8_t*)qcls_.as<uint8_t>(),
+                       (const uint32_t*)np_pos_.as<uint32_t>(), queue_.as<uint64_t>());
     uint64_t q = P;
     if (exact_) {
       read_status();
@@ -1497,7 +1498,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void prepare_capacity_buffers() {
     const uint64_t slots = std::max<uint64_t>(chunk_cap_, 1) * CHUNK;
     const uint64_t P = pcap_;
-    scan_tmp_.reserve(std::max<size_t>(std::max(scan_tmp_words(slots), scan_tmp_words(queue_hist_words(P))), 16) * 4);
+    scan_tmp_.reserve(std::max<size_t>(std::max(scan_tmp_words(slots), scan_tmp_words(std::max(queue_hist_words(P), P))), 16) * 4);
     keys_.reserve(mcap_ * 8);
     keys_tmp_.reserve(mcap_ * 8);
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits_), 16) * 4);
@@ -1509,7 +1510,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     np_v_.reserve(P * sizeof(PairRec<R>));
     np_hit_.reserve(P * 4);
     np_kind_.reserve(P);
-    np_pos_.reserve(queue_hist_words(P) * 4);
+    np_pos_.reserve(std::max(queue_hist_words(P), P) * 4);
     queue_.reserve(P * 8);
     queue_tmp_.reserve(P * 8);
     qsel_.reserve(P * 4);
