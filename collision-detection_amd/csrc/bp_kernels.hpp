@@ -271,19 +271,20 @@ template <class R> __device__ __forceinline__ Cells cells_of(const BpRec<R>& r, 
 // (HashMin, :125-133); plus the largest key (the max corner's, keys do not
 // wrap in the dense key range).  With wpart, also the widest shifted AABB
 // extent per chunk (as float bits: non-negative floats order as unsigned
-// integers), which sizes the cell search's grid.
+// integers), which sizes the cell search's grid.  With tile_part, also the
+// counts' tile sums for scan_excl_u32_tiled.
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                  uint64_t NS, const R* __restrict__ aabb,
                                                  const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ counts,
                                                  uint32_t* __restrict__ kpart, uint32_t* __restrict__ wpart,
-                                                 DevStatus* st) {
+                                                 uint64_t* __restrict__ tile_part, DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   __shared__ uint32_t skm[CHUNK / WAVE], swm[CHUNK / WAVE];
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  uint32_t km = 0, wm = 0;
+  uint32_t km = 0, wm = 0, cv = 0;
   if (threadIdx.x < ch.len) {
     const SceneDesc<R>& d = desc[ch.scene];
     const BpRec<R> r = shifted_rec(aabb, NS, i, d, ch.scene);
@@ -293,14 +294,16 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
       wm = w > 0.f ? __float_as_uint(w) : 0u;  // NaN extents come with a reported AABB
     }
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
-      counts[slot] = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
+      cv = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = d.key_base + hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, d.gx, d.gy);
     } else {
-      counts[slot] = 0;
       atomicOr(&st->bad, 4u);
     }
-  } else {
-    counts[slot] = 0;
+  }
+  counts[slot] = cv;
+  if (tile_part) {
+    __shared__ uint32_t red[CHUNK / WAVE];
+    add_tile_sum<CHUNK>(tile_part, slot, cv, red);
   }
   for (int o = 32; o > 0; o >>= 1) {
     km = max(km, (uint32_t)__shfl_xor((int)km, o));
@@ -871,7 +874,7 @@ __global__ __launch_bounds__(CHUNK) void k_shape_pairs(const Chunk* __restrict__
                                                        const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ offs,
                                                        const uint32_t* __restrict__ pcount, uint64_t mcap,
-                                                       uint32_t* __restrict__ scount) {
+                                                       uint32_t* __restrict__ scount, uint64_t* __restrict__ tile_part) {
   __shared__ uint32_t sp[SHAPE_PAIRS_LDS];
   __shared__ uint64_t s_e0, s_e1;
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
@@ -898,6 +901,10 @@ __global__ __launch_bounds__(CHUNK) void k_shape_pairs(const Chunk* __restrict__
     for (uint64_t e = e0; e < e1; ++e) sum += pcount[e];
   }
   scount[slot] = sum;
+  if (tile_part) {  // a chunk with no shapes adds nothing
+    __shared__ uint32_t red[CHUNK / WAVE];
+    add_tile_sum<CHUNK>(tile_part, slot, sum, red);
+  }
 }
 
 // In-register bitonic sort of N values (compile-time indices only).

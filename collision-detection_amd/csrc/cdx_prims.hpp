@@ -62,6 +62,35 @@ size_t scan_tmp_words(uint64_t n);
 void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint32_t* tmp, hipStream_t s,
                    const uint64_t* d_n = nullptr);
 
+// The same scan when the kernel that produced `in` has already summed its
+// tiles: part[t] holds the sum of elements [t * SCAN_TILE_ELEMS, (t + 1) *
+// SCAN_TILE_ELEMS) (zero for tiles past the input), so the pass that reads
+// the input only to sum it is skipped.  part holds scan_tile_count(n)
+// uint64 and is scanned in place.
+constexpr uint64_t SCAN_TILE_ELEMS = 4096;
+inline uint64_t scan_tile_count(uint64_t n) { return (n + SCAN_TILE_ELEMS - 1) / SCAN_TILE_ELEMS + 1; }
+void scan_excl_u32_tiled(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint64_t* part,
+                         hipStream_t s, const uint64_t* d_n = nullptr);
+
+// For producers of such an input: adds the values the workgroup's threads
+// hold (element index i on each thread; one workgroup's elements lie in
+// one tile) to the tile's partial with one atomic.  Every thread of the
+// workgroup calls it (it synchronises the workgroup); red holds BS / WAVE
+// words of LDS.
+template <int BS>
+__device__ __forceinline__ void add_tile_sum(uint64_t* part, uint64_t i, uint32_t v, uint32_t* red) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+#pragma unroll
+    for (int w = 0; w < BS / WAVE; ++w) t += red[w];
+    if (t) atomicAdd((unsigned long long*)&part[i / SCAN_TILE_ELEMS], (unsigned long long)t);
+  }
+}
+
 // Stable LSD radix sort of n 64-bit values by bits [lo_bit, lo_bit + nbits),
 // in the fewest passes of 8 to 11-bit digits whose per-tile digit counts
 // stay small, ping-ponging between data and tmp.  Returns true when the

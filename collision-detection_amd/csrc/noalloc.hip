@@ -192,7 +192,7 @@ uint64_t broadphase(int dev, void* aabb_aos, uint64_t n, int gx, int gy, int gz,
   hipLaunchKernelGGL(k_params<R>, dim3(1), dim3(PARAMS_BS), 0, s, (const uint32_t*)sel, (const DevStatus*)st,
                      (const uint32_t*)bpart, desc);
   hipLaunchKernelGGL(k_count<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
-                     (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, counts, kpart, (uint32_t*)nullptr, st);
+                     (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, counts, kpart, (uint32_t*)nullptr, (uint64_t*)nullptr, st);
   hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s, (const uint32_t*)kpart, (const uint32_t*)nullptr, st);
   scan_excl_u32(counts, offs, slots, &st->M, scan_tmp, s, &st->nslots);
   hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
@@ -204,7 +204,7 @@ uint64_t broadphase(int dev, void* aabb_aos, uint64_t n, int gx, int gy, int gz,
   hipLaunchKernelGGL(k_sum_runs, dim3(RUNS_GRID), dim3(RUNS_BS), 0, s, mcap, (const uint32_t*)runs_part, st);
   hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, (const uint32_t*)counts, (const uint32_t*)offs, (const uint32_t*)pcount,
-                     mcap, scount);
+                     mcap, scount, (uint64_t*)nullptr);
   scan_excl_u32(scount, soff, slots, &st->P, scan_tmp, s, &st->nslots);
   hipLaunchKernelGGL(k_pair_emit<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, (const uint32_t*)counts, (const uint32_t*)offs, (const uint32_t*)pcount,

@@ -647,18 +647,21 @@ __global__ __launch_bounds__(128) void k_np_filter(uint64_t cap, const uint64_t*
                                                    const uint64_t* __restrict__ pairs,
                                                    const ShapeHdr* __restrict__ hdr,
                                                    const ShapeRecF* __restrict__ frec, R envelope, float tau_rel,
-                                                   PairOut<R> o, uint32_t* __restrict__ sel) {
-  uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
-  if (j >= dev_count(cap, d_q)) return;
-  const uint32_t i = (uint32_t)queue[j];
-  const uint64_t pr = pairs[i];
-  const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
-  const ShapeRecF fa = frec[ia], fb = frec[ib];
-  if (!fp32_filtered(fa.type) || !fp32_filtered(fb.type)) {
-    sel[j] = 1u;  // rectangles, cones and triangles: the reference-order MPR decides
-    return;
+                                                   PairOut<R> o, uint32_t* __restrict__ sel,
+                                                   uint64_t* __restrict__ tile_part) {
+  const uint64_t j = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+  uint32_t keep = 0;
+  if (j < dev_count(cap, d_q)) {
+    const uint32_t i = (uint32_t)queue[j];
+    const uint64_t pr = pairs[i];
+    const uint32_t ia = (uint32_t)(pr >> 32), ib = (uint32_t)(pr & 0xffffffffu);
+    const ShapeRecF fa = frec[ia], fb = frec[ib];
+    // rectangles, cones and triangles: the reference-order MPR decides
+    keep = !fp32_filtered(fa.type) || !fp32_filtered(fb.type) || np_filter_one<R, KEEP>(i, fa, fb, envelope, tau_rel, o);
+    sel[j] = keep;
   }
-  sel[j] = np_filter_one<R, KEEP>(i, fa, fb, envelope, tau_rel, o) ? 1u : 0u;
+  __shared__ uint32_t red[128 / WAVE];
+  add_tile_sum<128>(tile_part, j, keep, red);  // the selection's tile sums, for scan_excl_u32_tiled
 }
 
 // Stable compaction of the selected queue entries (keeps the class order).

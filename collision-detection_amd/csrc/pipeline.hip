@@ -46,7 +46,14 @@ enum : int { ZERO_AABB = 1, ZERO_BP = 2, ZERO_NP = 4 };
 
 // Opens a stage: copies the step header, zeroes the totals the stage
 // produces (a kernel, so the sequence can be recorded as a graph).
-__global__ void k_stage_begin(const StepHdr* __restrict__ hdr, DevStatus* st, int what) {
+// Also zeroes the tile sums the stage's producers add to (zbp: the
+// broadphase's, znp: the narrowphase's).
+__global__ void k_stage_begin(const StepHdr* __restrict__ hdr, DevStatus* st, uint64_t* __restrict__ zbp,
+                              uint32_t nbp, uint64_t* __restrict__ znp, uint32_t nnp, int what) {
+  if (what & ZERO_BP)
+    for (uint32_t k = threadIdx.x; k < nbp; k += blockDim.x) zbp[k] = 0;
+  if (what & ZERO_NP)
+    for (uint32_t k = threadIdx.x; k < nnp; k += blockDim.x) znp[k] = 0;
   if (threadIdx.x != 0) return;
   st->nslots = hdr->nslots;
   st->nsel = hdr->nsel;
@@ -467,7 +474,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     const DevBuf* all[] = {&type_, &body_, &lpos_, &dims_, &cpos_, &lrot_, &bpos_, &brot_, &active_, &aabb_, &rec_,
                            &brec_, &hdr_, &frec_, &bpart_, &kpart_, &counts_buf_, &offs_, &keys_, &keys_tmp_,
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_kind_, &np_pos_, &queue_,
-                           &queue_tmp_, &qsel_, &qcls_, &out_, &perm_,
+                           &queue_tmp_, &qsel_, &qcls_, &tile_parts_, &out_, &perm_,
                            &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &bins_,
                            &wpart_, &cell_tab_, &cell_rank_, &cell_lo_, &cell_hi_, &cell_slab_, &cell_last_, &cell_wmax_, &cell_ovf_};
     uint64_t t = 0;
@@ -1002,10 +1009,26 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     scan_excl_u32(in, out, n, d_total, scan_tmp_.as<uint32_t>(), s_, d_n);
   }
 
+  // Tile sums the producers of three scans add to (bin counts, pairs per
+  // shape, filter selections), zeroed by k_stage_begin: tp_T_ tiles each for
+  // the two per-slot scans, then tp_F_ for the per-pair one.
+  uint64_t* tp_counts() { return tile_parts_.as<uint64_t>(); }
+  uint64_t* tp_spairs() { return tile_parts_.as<uint64_t>() + tp_T_; }
+  uint64_t* tp_filter() { return tile_parts_.as<uint64_t>() + 2 * tp_T_; }
+  void reserve_tile_parts(uint64_t slots, uint64_t P) {
+    const uint64_t T = scan_tile_count(slots), F = scan_tile_count(P);
+    if (T <= tp_T_ && F <= tp_F_ && tile_parts_.p) return;
+    tp_T_ = std::max(T, tp_T_);
+    tp_F_ = std::max(F, tp_F_);
+    tile_parts_.reserve((2 * tp_T_ + tp_F_) * 8);
+    graph_valid_ = false;  // the layout moved
+  }
+
   void stage_aabb() {
     aabbs_changed();
     if (!opened_)
       hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                       tp_counts(), (uint32_t)(2 * tp_T_), tp_filter(), (uint32_t)tp_F_,
                          (int)ZERO_AABB);
     mark(ST_AABB, true);
     hipLaunchKernelGGL(k_aabb<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
@@ -1018,8 +1041,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   // Broadphase from the AABBs (exact mode reads totals back between steps).
   void stage_broadphase() {
+    reserve_tile_parts(chunk_grid() * CHUNK, std::max<uint64_t>(pcap_, 1));
     if (!opened_)  // AABBs from the caller: no AABB stage cleared the error flags
       hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                       tp_counts(), (uint32_t)(2 * tp_T_), tp_filter(), (uint32_t)tp_F_,
                          (int)(ZERO_BP | (aabb_from_caller_ ? ZERO_AABB : 0)));
     mark(ST_BP_BOUNDS, true);
     if (aabb_from_caller_)
@@ -1036,12 +1061,13 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(),
-                       kpart_.as<uint32_t>(), exact_ ? wpart_.as<uint32_t>() : nullptr, st());
+                       kpart_.as<uint32_t>(), exact_ ? wpart_.as<uint32_t>() : nullptr, tp_counts(), st());
     hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(),
                        exact_ ? (const uint32_t*)wpart_.as<uint32_t>() : nullptr, st());
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
-    scan(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, &st()->nslots);
+    scan_excl_u32_tiled(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, tp_counts(), s_,
+                        &st()->nslots);
     mark(ST_BP_SCAN, false);
     int key_bits = key_bits_;
     cell_ = false;
@@ -1076,8 +1102,9 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
                        (const uint32_t*)counts_buf_.as<uint32_t>(), (const uint32_t*)offs_.as<uint32_t>(),
-                       (const uint32_t*)pcount_.as<uint32_t>(), mcap_, scount_.as<uint32_t>());
-    scan(scount_.as<uint32_t>(), soff_.as<uint32_t>(), slots, &st()->P, &st()->nslots);
+                       (const uint32_t*)pcount_.as<uint32_t>(), mcap_, scount_.as<uint32_t>(), tp_spairs());
+    scan_excl_u32_tiled(scount_.as<uint32_t>(), soff_.as<uint32_t>(), slots, &st()->P, tp_spairs(), s_,
+                        &st()->nslots);
     mark(ST_BP_PAIRSCAN, false);
     if (exact_) {
       read_status();
@@ -1283,8 +1310,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   }
 
   void stage_narrowphase() {
+    reserve_tile_parts(chunk_grid() * CHUNK, std::max<uint64_t>(pcap_, 1));
     if (!opened_)
       hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                       tp_counts(), (uint32_t)(2 * tp_T_), tp_filter(), (uint32_t)tp_F_,
                          (int)ZERO_NP);
     const uint64_t P = std::max<uint64_t>(pcap_, 1);
     np_v_.reserve(P * sizeof(PairRec<R>));
@@ -1323,11 +1352,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       auto filt = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks_for(q, 128)), dim3(128), 0, s_, q, dq, (const uint64_t*)queue_.as<uint64_t>(),
                            (const uint64_t*)pairs_.as<uint64_t>(), (const ShapeHdr*)hdr_.as<ShapeHdr>(),
-                           (const ShapeRecF*)frec_.as<ShapeRecF>(), envelope_, tau_, o, qsel_.as<uint32_t>());
+                           (const ShapeRecF*)frec_.as<ShapeRecF>(), envelope_, tau_, o, qsel_.as<uint32_t>(),
+                           tp_filter());
       };
       if (mode_ == NP_FAST) filt(k_np_filter<R, true>);
       else filt(k_np_filter<R, false>);
-      scan(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), q, &st()->Q2, dq);
+      scan_excl_u32_tiled(qsel_.as<uint32_t>(), np_pos_.as<uint32_t>(), q, &st()->Q2, tp_filter(), s_, dq);
       hipLaunchKernelGGL(k_queue_select, dim3(blocks_for(q, 256)), dim3(256), 0, s_, q, dq,
                          (const uint64_t*)queue_.as<uint64_t>(), (const uint32_t*)qsel_.as<uint32_t>(),
                          (const uint32_t*)np_pos_.as<uint32_t>(), queue_tmp_.as<uint64_t>());
@@ -1425,6 +1455,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     prof_ = false;  // no event records inside a recorded step
     // one stage opening for the whole step (the stages' totals are disjoint)
     hipLaunchKernelGGL(k_stage_begin, dim3(1), dim3(64), 0, s_, (const StepHdr*)stephdr_.as<StepHdr>(), st(),
+                       tp_counts(), (uint32_t)(2 * tp_T_), tp_filter(), (uint32_t)tp_F_,
                        (int)(ZERO_AABB | ZERO_BP | ZERO_NP));
     opened_ = true;
     stage_aabb();
@@ -1498,6 +1529,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void prepare_capacity_buffers() {
     const uint64_t slots = std::max<uint64_t>(chunk_cap_, 1) * CHUNK;
     const uint64_t P = pcap_;
+    reserve_tile_parts(slots, std::max<uint64_t>(P, 1));
     scan_tmp_.reserve(std::max<size_t>(std::max(scan_tmp_words(slots), scan_tmp_words(std::max(queue_hist_words(P), P))), 16) * 4);
     keys_.reserve(mcap_ * 8);
     keys_tmp_.reserve(mcap_ * 8);
@@ -1617,6 +1649,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   DevBuf chunks_, sel_, desc_, souts_;
   DevBuf counts_buf_, offs_, keys_, keys_tmp_, pcount_, pairs_;
   DevBuf np_v_, np_hit_, np_kind_, np_pos_, queue_, queue_tmp_, qsel_, qcls_;
+  DevBuf tile_parts_;
+  uint64_t tp_T_ = 0, tp_F_ = 0;
   DevBuf out_, perm_;  // out_: the step's contact arrays back to back, C rows each
   HostStager stager_;  // large copies to and from caller memory
   DevBuf scan_tmp_, radix_tmp_, runs_part_;

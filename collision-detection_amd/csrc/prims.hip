@@ -11,6 +11,7 @@ namespace {
 constexpr int SCAN_BS = 256;
 constexpr int SCAN_IPT = 16;
 constexpr int SCAN_TILE = SCAN_BS * SCAN_IPT;
+static_assert((uint64_t)SCAN_TILE == SCAN_TILE_ELEMS, "producers sum the scan's own tiles");
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -403,6 +404,22 @@ void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_to
     return;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, n, d_n, part);
+  if (nb <= SCAN_DIRECT_TILES) {
+    hipLaunchKernelGGL(k_scan_tiles_direct, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, d_n,
+                       (const uint64_t*)part, d_total);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, nb, d_total);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, d_n, (const uint64_t*)part);
+}
+
+void scan_excl_u32_tiled(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, uint64_t* part,
+                         hipStream_t s, const uint64_t* d_n) {
+  const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (n == 0) {
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, (uint64_t)0, d_total);
+    return;
+  }
   if (nb <= SCAN_DIRECT_TILES) {
     hipLaunchKernelGGL(k_scan_tiles_direct, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, out, n, d_n,
                        (const uint64_t*)part, d_total);
