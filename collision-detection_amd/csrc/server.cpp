@@ -165,7 +165,8 @@ Server::Server(int device, const ServerConfig& cfg) : dev_(device), cfg_(cfg), t
   CDX_REQUIRE(cfg.real_kind == REAL_F64 || cfg.real_kind == REAL_F32, ERR_INVALID, "unknown real kind");
   CDX_REQUIRE(cfg.max_batch_shapes > 0 && cfg.max_pending > 0, ERR_INVALID, "batch and queue limits must be positive");
   CDX_HIP_CHECK(hipSetDevice(dev_));
-  CDX_HIP_CHECK(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+  static_assert(kNumClasses == 4, "one upload stream per size class");
+  for (hipStream_t& u : up_) CDX_HIP_CHECK(hipStreamCreateWithFlags(&u, hipStreamNonBlocking));
   pools_ = std::make_shared<DevPools>();
   size_t free_b = 0, total_b = 0;
   CDX_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
@@ -197,13 +198,13 @@ Server::~Server() {
     if (k->worker.joinable()) k->worker.join();
   {
     GateShared g;
-    (void)hipStreamSynchronize(up_);
+    for (hipStream_t u : up_) (void)hipStreamSynchronize(u);
   }
   reqs_.clear();
   buffers_.clear();
   pools_.reset();
   GateShared g;
-  (void)hipStreamDestroy(up_);
+  for (hipStream_t u : up_) (void)hipStreamDestroy(u);
 }
 
 int Server::class_of(uint64_t n) const {
@@ -328,7 +329,7 @@ uint64_t Server::submit(uint32_t s, uint64_t k, const uint32_t* idx, const void*
   return enqueue(std::move(req), ref.cls, t_enter);
 }
 
-void Server::upload_poses(Request& r) {
+void Server::upload_poses(Request& r, int cls) {
   if (!r.k || !r.poses) return;
   PoseBlock& pb = *r.poses;
   const uint64_t bytes = pose_block_bytes(r.k, cfg_.real_kind);
@@ -338,12 +339,12 @@ void Server::upload_poses(Request& r) {
   pb.dev = pools_->get_block(bytes, &pb.dev_bucket);
   pb.ev = pools_->get_event();
   CDX_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(pb.dev) + from, static_cast<const char*>(pb.p) + from,
-                               bytes - from, hipMemcpyHostToDevice, up_));
-  CDX_HIP_CHECK(hipEventRecord(pb.ev, up_));
+                               bytes - from, hipMemcpyHostToDevice, up_[cls]));
+  CDX_HIP_CHECK(hipEventRecord(pb.ev, up_[cls]));
 }
 
 uint64_t Server::enqueue(std::shared_ptr<Request> req, int cls, double t_enter) {
-  upload_poses(*req);
+  upload_poses(*req, cls);
   uint64_t id;
   {
     std::lock_guard<std::mutex> lk(mu_);

@@ -149,10 +149,12 @@ class Server {
   static double now_ms();
   // Starts the copy of a request's poses to device memory on up_ (at submit,
   // so the copy overlaps the queueing and the steps before it).
-  void upload_poses(Request& r);
+  void upload_poses(Request& r, int cls);
 
   int dev_;
-  hipStream_t up_ = nullptr;        // pose copies at submit
+  // pose copies at submit, one stream per size class: a small scene's copy
+  // does not wait behind a large scene's
+  hipStream_t up_[4] = {nullptr, nullptr, nullptr, nullptr};
   std::shared_ptr<DevPools> pools_;  // device pose blocks and their copy events
   ServerConfig cfg_;
   uint64_t budget_ = 0, committed_ = 0;
