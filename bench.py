@@ -830,7 +830,7 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=512, help="0 skips the batched leg")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--served-scenes", type=int, default=4096, help="0 skips the served leg")
-    ap.add_argument("--served-rates", type=str, default="500,1000,2000,2500,3000,3500,4000,5000,6000",
+    ap.add_argument("--served-rates", type=str, default="500,1000,2000,3000,3500,4000,4500,5000,6000",
                     help="Poisson arrival rates (requests/s) swept by the served leg")
     ap.add_argument("--served-seconds", type=float, default=3.0, help="seconds of arrivals per rate")
     ap.add_argument("--served-batch-shapes", type=int, default=1 << 19)
