@@ -458,19 +458,18 @@ void Server::worker_loop(int c) {
   }
 }
 
+// Result blocks come in power-of-two capacities of at least 4,096 contacts.
+// A step takes an idle block whose capacity is its need rounded up to a
+// power of two, or up to twice that: blocks stay near the size of the steps
+// they serve, and a new one is pinned (about a millisecond per 50 MB, with
+// the worker waiting) only while every block of the size is held by results
+// the caller has not released.  The pool keeps what it allocated.
 std::shared_ptr<ResultBlock> Server::get_block(Class& k, uint64_t contacts) {
+  uint64_t cap = 4096;
+  while (cap < contacts) cap <<= 1;
   for (auto& b : k.blocks)
-    if (b.use_count() == 1 && b->cap >= contacts) return b;
-  // Replace an idle block that is too small, or add one: capacity for twice
-  // the largest step seen, so growth stops after the first large steps.
-  for (auto it = k.blocks.begin(); it != k.blocks.end(); ++it)
-    if (it->use_count() == 1) {
-      k.blocks.erase(it);
-      break;
-    }
-  k.max_contacts = std::max(k.max_contacts, contacts);
+    if (b.use_count() == 1 && b->cap >= cap && b->cap <= 2 * cap) return b;
   auto b = std::make_shared<ResultBlock>();
-  const uint64_t cap = std::max<uint64_t>(2 * k.max_contacts, 1 << 16);
   const uint64_t rs = cfg_.real_kind == REAL_F64 ? 8 : 4;
   {
     GateShared g;

@@ -128,7 +128,6 @@ class Server {
     std::deque<std::shared_ptr<Request>> queue;
     std::thread worker;
     std::vector<std::shared_ptr<ResultBlock>> blocks;  // pool, used by the worker only
-    uint64_t max_contacts = 0;
     // contacts of each scene (by engine index) at its last step, ~0 before
     // its first (worker only, under eng_mu): sizes the contacts prefetch
     std::vector<uint64_t> last_contacts;
