@@ -522,13 +522,6 @@ void Server::run_batch(int c, std::vector<std::shared_ptr<Request>>& batch) {
     double t2 = now_ms();
     e.step(cfg_.use_graph);
     double t3 = now_ms();
-    // Poll the class stream for up to a few milliseconds before finish()
-    // blocks: a serving worker trades a host core for the wake-up latency
-    // of a blocking wait.
-    {
-      const double until = t3 + 5.0;
-      while (hipStreamQuery(e.stream()) == hipErrorNotReady && now_ms() < until) std::this_thread::yield();
-    }
     e.finish();
     double t4 = now_ms();
     step_ms = e.last_step_ms();
