@@ -17,6 +17,7 @@ restatement of the reference's shipped OpenMP path) timed on a bounded
 sample on this host's cores.
 """
 import argparse
+import gc
 import importlib.util
 import json
 import os
@@ -723,7 +724,7 @@ def served_leg(pkg, scenes, ctx, args):
                 srv.wait(item, 120_000)
                 srv.release(item)
 
-        prods = [threading.Thread(target=producer, args=(k, 8)) for k in range(8)]
+        prods = [threading.Thread(target=producer, args=(k, 4)) for k in range(4)]
         for t in prods:
             t.start()
         th = threading.Thread(target=reaper)
@@ -733,6 +734,7 @@ def served_leg(pkg, scenes, ctx, args):
         srv.stats(reset=True)
         refused_q = 0
         lag = []
+        gc.disable()  # no collector pauses in the load generator while arrivals run
         t_start = time.perf_counter()
         for r in range(n_req):
             now = time.perf_counter() - t_start
@@ -757,6 +759,8 @@ def served_leg(pkg, scenes, ctx, args):
         th.join()
         for t in prods:
             t.join()
+        gc.enable()
+        gc.collect()
         st = srv.stats(reset=True)
         wall = time.perf_counter() - t_start
         pt = {"rate_rps": rate, "requests": int(st["completed"]), "queue_refused": refused_q,
