@@ -62,14 +62,16 @@ __global__ void k_stage_begin(const StepHdr* __restrict__ hdr, DevStatus* st, ui
   if (what & ZERO_NP) { st->Q = st->Q2 = st->C = 0; }
 }
 
-// Chunk ranges of the selected scenes, written into their descriptors.
+// Chunk ranges and step-local key bases of the selected scenes, written
+// into their descriptors (rng: scene, first chunk, end chunk, key base).
 template <class R>
 __global__ void k_apply_selection(uint64_t nsel, const uint32_t* __restrict__ rng, SceneDesc<R>* __restrict__ desc) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nsel) return;
-  SceneDesc<R>& d = desc[rng[3 * j]];
-  d.chunk_begin = rng[3 * j + 1];
-  d.chunk_end = rng[3 * j + 2];
+  SceneDesc<R>& d = desc[rng[4 * j]];
+  d.chunk_begin = rng[4 * j + 1];
+  d.chunk_end = rng[4 * j + 2];
+  d.key_base = rng[4 * j + 3];
 }
 
 template <class R>
@@ -902,23 +904,33 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
         ch.push_back(c);
       }
       d.chunk_end = (uint32_t)ch.size();
-      ks_total = std::max<uint64_t>(ks_total, sc.key_base + sc.key_space);
+      // step-local key bases: the selected scenes' key ranges follow one
+      // another from 0, so the keys a step sorts span only its own scenes
+      d.key_base = (uint32_t)ks_total;
+      ks_total += sc.key_space;
+      CDX_REQUIRE(ks_total <= (1ull << 32), ERR_INVALID, "bin key space of a step exceeds 2^32: use fewer bins per axis");
       nshapes = std::max<uint64_t>(nshapes, sc.shape_begin + sc.n_shapes);
     }
-    CDX_REQUIRE(ks_total <= (1ull << 32), ERR_INVALID, "bin key space exceeds 2^32: use fewer bins per axis");
     nchunks_ = ch.size();
     nslots_ = nchunks_ * CHUNK;
     nsel_ = (uint32_t)sel.size();
-    // radix digit counts from every resident scene, so a recorded step
-    // stays valid whichever scenes a later step selects
-    for (const HostScene& sc : scenes_) {
-      ks_total = std::max<uint64_t>(ks_total, sc.key_base + sc.key_space);
-      nshapes = std::max<uint64_t>(nshapes, sc.shape_begin + sc.n_shapes);
-    }
-    key_bits_ = bits_for(ks_total ? ks_total - 1 : 0);
+    for (const HostScene& sc : scenes_) nshapes = std::max<uint64_t>(nshapes, sc.shape_begin + sc.n_shapes);
     shape_bits_ = bits_for(nshapes ? nshapes - 1 : 0);
     if (nchunks_ > chunk_cap_ || !chunks_.p)
       grow_chunk_cap(std::max<uint64_t>(std::max<uint64_t>(nchunks_, 1), chunk_cap_ * 2));
+    // Key width of the radix sort: a recorded step stays valid whichever
+    // scenes a later step selects, so it covers any selection the buffers
+    // hold: at most chunk_cap_ * CHUNK shapes, each scene holding at most
+    // `ratio` keys per shape (and never more than every resident scene).
+    {
+      uint64_t ratio = 1, ks_all = 0;
+      for (const HostScene& sc : scenes_) {
+        ratio = std::max<uint64_t>(ratio, (sc.key_space + sc.n_shapes - 1) / std::max<uint64_t>(sc.n_shapes, 1));
+        ks_all += sc.key_space;
+      }
+      const uint64_t bound = std::max(ks_total, std::min(ks_all, ratio * chunk_cap_ * CHUNK));
+      key_bits_ = bits_for(bound ? bound - 1 : 0);
+    }
     if (nsel_ > sel_cap_ || !sel_.p) grow_sel_cap(std::max<uint64_t>(std::max<uint64_t>(nsel_, 1), sel_cap_ * 2));
     if (desc_host_.size() > desc_cap_ || !desc_.p) {
       desc_cap_ = std::max<uint64_t>(std::max<uint64_t>(desc_host_.size(), 1), desc_cap_ * 2);
@@ -941,7 +953,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       stage_pending_ = false;
     }
     const size_t sel_bytes = ((nsel_ * 4 + 15) & ~size_t(15));
-    const size_t rng_bytes = ((nsel_ * 12 + 15) & ~size_t(15));
+    const size_t rng_bytes = ((nsel_ * 16 + 15) & ~size_t(15));
     const size_t need = sizeof(StepHdr) + sel_bytes + rng_bytes + nchunks_ * sizeof(Chunk) + 64;
     if (need > stage_host_bytes_) {
       if (stage_host_) CDX_HIP_CHECK(hipHostFree(stage_host_));
@@ -958,16 +970,17 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     Chunk* sch = reinterpret_cast<Chunk*>(hs + sizeof(StepHdr) + sel_bytes + rng_bytes);
     for (uint32_t j = 0; j < nsel_; ++j) {
       ssel[j] = sel[j];
-      srng[3 * j] = sel[j];
-      srng[3 * j + 1] = desc_host_[sel[j]].chunk_begin;
-      srng[3 * j + 2] = desc_host_[sel[j]].chunk_end;
+      srng[4 * j] = sel[j];
+      srng[4 * j + 1] = desc_host_[sel[j]].chunk_begin;
+      srng[4 * j + 2] = desc_host_[sel[j]].chunk_end;
+      srng[4 * j + 3] = desc_host_[sel[j]].key_base;
     }
     if (!ch.empty()) std::memcpy(sch, ch.data(), ch.size() * sizeof(Chunk));
     selrng_.reserve(std::max<size_t>(rng_bytes, 16));
     CDX_HIP_CHECK(hipMemcpyAsync(stephdr_.p, shdr, sizeof(StepHdr), hipMemcpyHostToDevice, s_));
     if (nsel_) {
       CDX_HIP_CHECK(hipMemcpyAsync(sel_.p, ssel, nsel_ * 4, hipMemcpyHostToDevice, s_));
-      CDX_HIP_CHECK(hipMemcpyAsync(selrng_.p, srng, nsel_ * 12, hipMemcpyHostToDevice, s_));
+      CDX_HIP_CHECK(hipMemcpyAsync(selrng_.p, srng, nsel_ * 16, hipMemcpyHostToDevice, s_));
       hipLaunchKernelGGL(k_apply_selection<R>, dim3(blocks_for(nsel_, 64)), dim3(64), 0, s_, (uint64_t)nsel_,
                          (const uint32_t*)selrng_.as<uint32_t>(), desc_.as<SceneDesc<R>>());
     }
