@@ -19,10 +19,15 @@ __device__ __forceinline__ uint64_t cap_count(uint64_t cap, const uint64_t* d_n)
   return v < cap ? v : cap;
 }
 
-// Per-workgroup bounds partials (reduced per scene by k_params): the float
-// min / max over the workgroup's AABB corners, as order-preserving uints.
-__device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[3], uint32_t* part) {
-  __shared__ uint32_t smn[3][CHUNK / WAVE], smx[3][CHUNK / WAVE];
+// Per-workgroup bounds partials (reduced per scene by k_params), BPART_WORDS
+// per workgroup: the float min / max over the workgroup's AABB corners, as
+// order-preserving uints, then the widest AABB extent of the workgroup (float
+// bits: non-negative floats order as unsigned integers), which sizes the cell
+// search's grid.
+constexpr int BPART_WORDS = 7;
+
+__device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[3], uint32_t wm, uint32_t* part) {
+  __shared__ uint32_t smn[3][CHUNK / WAVE], smx[3][CHUNK / WAVE], swm[CHUNK / WAVE];
   const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -33,18 +38,27 @@ __device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[
     }
     if (l == 0) { smn[c][w] = a; smx[c][w] = b; }
   }
+  for (int o = 32; o > 0; o >>= 1) wm = max(wm, (uint32_t)__shfl_xor((int)wm, o));
+  if (l == 0) swm[w] = wm;
   __syncthreads();
+  uint32_t* out = part + (uint64_t)BPART_WORDS * blockIdx.x;
   if (threadIdx.x < 3) {
     int c = threadIdx.x;
     uint32_t a = 0xffffffffu, b = 0u;
     for (int i = 0; i < CHUNK / WAVE; ++i) { a = min(a, smn[c][i]); b = max(b, smx[c][i]); }
-    part[6 * (uint64_t)blockIdx.x + c] = a;
-    part[6 * (uint64_t)blockIdx.x + 3 + c] = b;
+    out[c] = a;
+    out[3 + c] = b;
+  } else if (threadIdx.x == 3) {
+    uint32_t m = 0u;
+    for (int i = 0; i < CHUNK / WAVE; ++i) m = max(m, swm[i]);
+    out[6] = m;
   }
 }
 
-__device__ __forceinline__ void bounds_fold(uint32_t mn[3], uint32_t mx[3], float lx, float ly, float lz, float hx,
-                                            float hy, float hz) {
+// Folds one AABB into the corner bounds and the widest extent (NaN extents
+// count as 0: such AABBs make the scene bounds non-finite and are reported).
+__device__ __forceinline__ void bounds_fold(uint32_t mn[3], uint32_t mx[3], uint32_t& wm, float lx, float ly,
+                                            float lz, float hx, float hy, float hz) {
   uint32_t u;
   u = f2ord(lx); mn[0] = min(mn[0], u); mx[0] = max(mx[0], u);
   u = f2ord(ly); mn[1] = min(mn[1], u); mx[1] = max(mx[1], u);
@@ -52,6 +66,8 @@ __device__ __forceinline__ void bounds_fold(uint32_t mn[3], uint32_t mx[3], floa
   u = f2ord(hx); mn[0] = min(mn[0], u); mx[0] = max(mx[0], u);
   u = f2ord(hy); mn[1] = min(mn[1], u); mx[1] = max(mx[1], u);
   u = f2ord(hz); mn[2] = min(mn[2], u); mx[2] = max(mx[2], u);
+  const float w = fmaxf(fmaxf(hx - lx, hy - ly), hz - lz);
+  if (w > 0.f) wm = max(wm, __float_as_uint(w));
 }
 
 // One pass over shapes (SoA) and body poses: world AABB (AABBGenerator.cu:
@@ -75,7 +91,7 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+  uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u}, wm = 0u;
   if (threadIdx.x < ch.len) {
     int t = type[i];
     uint32_t id = body[i];
@@ -150,10 +166,10 @@ __global__ __launch_bounds__(CHUNK) void k_aabb(const Chunk* __restrict__ chunks
       fr.body = id;
       fr.active = r.active;
       lfrec[threadIdx.x] = fr;
-      bounds_fold(mn, mx, (float)lo.x, (float)lo.y, (float)lo.z, (float)hi.x, (float)hi.y, (float)hi.z);
+      bounds_fold(mn, mx, wm, (float)lo.x, (float)lo.y, (float)lo.z, (float)hi.x, (float)hi.y, (float)hi.z);
     }
   }
-  block_bounds_commit(mn, mx, bpart);  // synchronises: the LDS records are complete
+  block_bounds_commit(mn, mx, wm, bpart);  // synchronises: the LDS records are complete
   {
     const uint32_t nr = ch.len * (uint32_t)(sizeof(ShapeRec<R>) / 16);
     const uint4* src = reinterpret_cast<const uint4*>(lrec);
@@ -175,48 +191,56 @@ __global__ __launch_bounds__(CHUNK) void k_bounds(const Chunk* __restrict__ chun
   if ((uint64_t)blockIdx.x * CHUNK >= st->nslots) return;
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+  uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u}, wm = 0u;
   if (threadIdx.x < ch.len)
-    bounds_fold(mn, mx, (float)aabb[i], (float)aabb[2 * NS + i], (float)aabb[4 * NS + i], (float)aabb[NS + i],
+    bounds_fold(mn, mx, wm, (float)aabb[i], (float)aabb[2 * NS + i], (float)aabb[4 * NS + i], (float)aabb[NS + i],
                 (float)aabb[3 * NS + i], (float)aabb[5 * NS + i]);
-  block_bounds_commit(mn, mx, bpart);
+  block_bounds_commit(mn, mx, wm, bpart);
 }
 
 // Per stepped scene: bounds from its chunks' partials, then origin and
-// inverse bin size (Broadphase.cu:377-381).  One workgroup per scene.
+// inverse bin size (Broadphase.cu:377-381).  One workgroup per scene; the
+// workgroup of the first stepped scene also stores that scene's widest AABB
+// extent in st->wmax.
 constexpr int PARAMS_BS = 1024;
 
 template <class R>
-__global__ __launch_bounds__(PARAMS_BS) void k_params(const uint32_t* __restrict__ sel,
-                                                      const DevStatus* __restrict__ st,
+__global__ __launch_bounds__(PARAMS_BS) void k_params(const uint32_t* __restrict__ sel, DevStatus* __restrict__ st,
                                                       const uint32_t* __restrict__ bpart,
                                                       SceneDesc<R>* __restrict__ desc) {
   if (blockIdx.x >= st->nsel) return;
-  __shared__ uint32_t red[6][PARAMS_BS / WAVE];
+  __shared__ uint32_t red[BPART_WORDS][PARAMS_BS / WAVE];
   SceneDesc<R>& d = desc[sel[blockIdx.x]];
-  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  uint32_t v[BPART_WORDS] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u};
 #pragma unroll 4
   for (uint64_t b = d.chunk_begin + threadIdx.x; b < d.chunk_end; b += PARAMS_BS) {
+    const uint32_t* p = bpart + BPART_WORDS * b;
     for (int c = 0; c < 3; ++c) {
-      v[c] = min(v[c], bpart[6 * b + c]);
-      v[3 + c] = max(v[3 + c], bpart[6 * b + 3 + c]);
+      v[c] = min(v[c], p[c]);
+      v[3 + c] = max(v[3 + c], p[3 + c]);
     }
+    v[6] = max(v[6], p[6]);
   }
   const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
-  for (int o = 32; o > 0; o >>= 1)
+  for (int o = 32; o > 0; o >>= 1) {
     for (int c = 0; c < 3; ++c) {
       v[c] = min(v[c], (uint32_t)__shfl_xor((int)v[c], o));
       v[3 + c] = max(v[3 + c], (uint32_t)__shfl_xor((int)v[3 + c], o));
     }
+    v[6] = max(v[6], (uint32_t)__shfl_xor((int)v[6], o));
+  }
   if (l == 0)
-    for (int c = 0; c < 6; ++c) red[c][w] = v[c];
+    for (int c = 0; c < BPART_WORDS; ++c) red[c][w] = v[c];
   __syncthreads();
   if (threadIdx.x != 0) return;
-  for (int t = 1; t < PARAMS_BS / WAVE; ++t)
+  for (int t = 1; t < PARAMS_BS / WAVE; ++t) {
     for (int c = 0; c < 3; ++c) {
       v[c] = min(v[c], red[c][t]);
       v[3 + c] = max(v[3 + c], red[3 + c][t]);
     }
+    v[6] = max(v[6], red[6][t]);
+  }
+  if (blockIdx.x == 0) st->wmax = v[6];
   int g[3] = {d.gx, d.gy, d.gz};
   for (int c = 0; c < 3; ++c) {
     d.bmin_enc[c] = v[c];
@@ -268,31 +292,25 @@ template <class R> __device__ __forceinline__ Cells cells_of(const BpRec<R>& r, 
 
 // Bins spanned per AABB (Broadphase.cu:150-159), on coordinates shifted by
 // the scene origin (Broadphase.cu:382-386) and hashed by truncation
-// (HashMin, :125-133); plus the largest key (the max corner's, keys do not
-// wrap in the dense key range).  With wpart, also the widest shifted AABB
-// extent per chunk (as float bits: non-negative floats order as unsigned
-// integers), which sizes the cell search's grid.  With tile_part, also the
-// counts' tile sums for scan_excl_u32_tiled.
+// (HashMin, :125-133); plus the largest key per chunk (the max corner's, keys
+// do not wrap in the dense key range).  With tile_part, also the counts'
+// tile sums for scan_excl_u32_tiled.
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                  uint64_t NS, const R* __restrict__ aabb,
                                                  const SceneDesc<R>* __restrict__ desc, uint32_t* __restrict__ counts,
-                                                 uint32_t* __restrict__ kpart, uint32_t* __restrict__ wpart,
-                                                 uint64_t* __restrict__ tile_part, DevStatus* st) {
+                                                 uint32_t* __restrict__ kpart, uint64_t* __restrict__ tile_part,
+                                                 DevStatus* st) {
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
-  __shared__ uint32_t skm[CHUNK / WAVE], swm[CHUNK / WAVE];
+  __shared__ uint32_t skm[CHUNK / WAVE];
   const Chunk ch = chunks[blockIdx.x];
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  uint32_t km = 0, wm = 0, cv = 0;
+  uint32_t km = 0, cv = 0;
   if (threadIdx.x < ch.len) {
     const SceneDesc<R>& d = desc[ch.scene];
     const BpRec<R> r = shifted_rec(aabb, NS, i, d, ch.scene);
     const Cells c = cells_of(r, d);
-    if (wpart) {
-      const float w = fmaxf(fmaxf((float)(r.hi[0] - r.lo[0]), (float)(r.hi[1] - r.lo[1])), (float)(r.hi[2] - r.lo[2]));
-      wm = w > 0.f ? __float_as_uint(w) : 0u;  // NaN extents come with a reported AABB
-    }
     if (cells_ok(c, d.gx, d.gy, d.gz)) {
       cv = (uint32_t)((c.x1 - c.x0 + 1) * (c.y1 - c.y0 + 1) * (c.z1 - c.z0 + 1));
       km = d.key_base + hash_index((uint32_t)c.x1, (uint32_t)c.y1, (uint32_t)c.z1, d.gx, d.gy);
@@ -305,45 +323,26 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
     __shared__ uint32_t red[CHUNK / WAVE];
     add_tile_sum<CHUNK>(tile_part, slot, cv, red);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    km = max(km, (uint32_t)__shfl_xor((int)km, o));
-    wm = max(wm, (uint32_t)__shfl_xor((int)wm, o));
-  }
-  if (threadIdx.x % WAVE == 0) {
-    skm[threadIdx.x / WAVE] = km;
-    swm[threadIdx.x / WAVE] = wm;
-  }
+  for (int o = 32; o > 0; o >>= 1) km = max(km, (uint32_t)__shfl_xor((int)km, o));
+  if (threadIdx.x % WAVE == 0) skm[threadIdx.x / WAVE] = km;
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t m = 0, mw = 0;
-    for (int w = 0; w < CHUNK / WAVE; ++w) {
-      m = max(m, skm[w]);
-      mw = max(mw, swm[w]);
-    }
+    uint32_t m = 0;
+    for (int w = 0; w < CHUNK / WAVE; ++w) m = max(m, skm[w]);
     kpart[blockIdx.x] = m;
-    if (wpart) wpart[blockIdx.x] = mw;
   }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart,
-                                                       const uint32_t* __restrict__ wpart, DevStatus* st) {
-  __shared__ uint32_t red[256], redw[256];
+__global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
+  __shared__ uint32_t red[256];
   const uint64_t npart = st->nslots / CHUNK;
-  uint32_t m = 0, mw = 0;
-  for (uint64_t b = threadIdx.x; b < npart; b += 256) {
-    m = max(m, kpart[b]);
-    if (wpart) mw = max(mw, wpart[b]);
-  }
+  uint32_t m = 0;
+  for (uint64_t b = threadIdx.x; b < npart; b += 256) m = max(m, kpart[b]);
   red[threadIdx.x] = m;
-  redw[threadIdx.x] = mw;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int t = 1; t < 256; ++t) {
-      m = max(m, red[t]);
-      mw = max(mw, redw[t]);
-    }
+    for (int t = 1; t < 256; ++t) m = max(m, red[t]);
     st->key_max = m;
-    st->wmax = mw;
   }
 }
 

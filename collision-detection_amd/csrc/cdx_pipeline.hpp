@@ -45,7 +45,7 @@ struct StepCounts {
 
 // StepCounts::info (reported with the flags of cdx_counts).
 constexpr uint32_t INFO_CELL_SEARCH = 0x100;     // pairs found by the cell search
-constexpr uint32_t INFO_NO_ACTIVE_BINS = 0x200;  // active bins not computed (yet)
+constexpr uint32_t INFO_NO_ACTIVE_BINS = 0x200;  // bin totals and active bins not computed (yet)
 
 // Pair search of a scene's broadphase.
 enum : int { PAIRS_AUTO = 0, PAIRS_BINS = 1, PAIRS_CELLS = 2 };

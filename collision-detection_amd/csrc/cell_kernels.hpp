@@ -24,12 +24,17 @@
 // ascending as sort + unique leaves it.
 //
 // Rounding: cells are floor(x * inv) in double, inv = 1 / edge along y and
-// z and kx / edge along x, edge = (1 + 1e-3) * the widest extent (read as
-// float).  For overlapping boxes lo_b - lo_a <= hi_a - lo_a, so the two
-// products differ by less than (1 + 1e-6) / (1 + 1e-3) < 1 (kx times that
-// along x) plus the double rounding of the products (below 1e-6 for any
-// grid here): their floors differ by at most 1 (kx along x).  Clamping to
-// the grid keeps that.
+// z and kx / edge along x, edge = (1 + 1e-3) * (W + 2^-20 * C), W the widest
+// AABB extent as the bounds pass computes it (float differences of the
+// unshifted corners) and C the largest coordinate magnitude of the scene,
+// shifted or not.  The widest shifted extent, in exact arithmetic on the
+// stored shifted corners, is below W * (1 + 2^-23) + 2^-22 * C (the float
+// rounding of the corners, of their difference and of the shift), so
+// below edge / (1 + 1e-3) * (1 + 1e-6).  For overlapping boxes lo_b - lo_a
+// <= hi_a - lo_a, so the two products differ by less than (1 + 1e-6) /
+// (1 + 1e-3) < 1 (kx times that along x) plus the double rounding of the
+// products (below 1e-6 for any grid here): their floors differ by at most 1
+// (kx along x).  Clamping to the grid keeps that.
 #pragma once
 #include "bp_kernels.hpp"
 
@@ -80,10 +85,11 @@ __device__ __forceinline__ uint32_t cell_key(const uint32_t c[3], const CellGrid
 // shape's place in cell order: a counting sort with no sort pass.  The
 // order within a cell is the order of the atomics; nothing depends on it
 // (every shape's partner group is sorted, and the walk reads whole cells).
+// The bin count does not run before this search, so the pass also reports
+// (st->bad bit 2) the AABBs the reference grid cannot bin, as k_count does.
 template <class R>
-__global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ chunks,
-                                                     const DevStatus* __restrict__ st, uint64_t NS,
-                                                     const R* __restrict__ aabb,
+__global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ chunks, DevStatus* __restrict__ st,
+                                                     uint64_t NS, const R* __restrict__ aabb,
                                                      const SceneDesc<R>* __restrict__ desc, CellGrid cg,
                                                      uint32_t* __restrict__ ccount, uint32_t* __restrict__ rank,
                                                      uint32_t* __restrict__ last) {
@@ -91,7 +97,9 @@ __global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ c
   const Chunk ch = chunks[blockIdx.x];
   if (threadIdx.x >= ch.len) return;
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  const BpRec<R> r = shifted_rec(aabb, NS, i, desc[ch.scene], ch.scene);
+  const SceneDesc<R>& d = desc[ch.scene];
+  const BpRec<R> r = shifted_rec(aabb, NS, i, d, ch.scene);
+  if (!cells_ok(cells_of(r, d), d.gx, d.gy, d.gz)) atomicOr(&st->bad, 4u);
   uint32_t c[3];
   cell_of_box(r.lo, cg, c);
   const uint32_t key = cell_key(c, cg);

@@ -157,7 +157,7 @@ uint64_t broadphase(int dev, void* aabb_aos, uint64_t n, int gx, int gy, int gz,
   Chunk* chunks = c.take<Chunk>(nchunks);
   R* aabb = c.take<R>(6 * NS);
   BpRec<R>* brec = c.take<BpRec<R>>(NS);
-  uint32_t* bpart = c.take<uint32_t>(6 * nchunks);
+  uint32_t* bpart = c.take<uint32_t>(BPART_WORDS * nchunks);
   uint32_t* kpart = c.take<uint32_t>(nchunks);
   uint32_t* counts = c.take<uint32_t>(slots);
   uint32_t* offs = c.take<uint32_t>(slots);
@@ -189,11 +189,11 @@ uint64_t broadphase(int dev, void* aabb_aos, uint64_t n, int gx, int gy, int gz,
   hipLaunchKernelGGL(k_na_soa<R>, dim3(blocks_for(2 * n, 256)), dim3(256), 0, s, n, NS, (const R*)aabb_aos, aabb);
   hipLaunchKernelGGL(k_bounds<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, NS, (const R*)aabb, bpart);
-  hipLaunchKernelGGL(k_params<R>, dim3(1), dim3(PARAMS_BS), 0, s, (const uint32_t*)sel, (const DevStatus*)st,
+  hipLaunchKernelGGL(k_params<R>, dim3(1), dim3(PARAMS_BS), 0, s, (const uint32_t*)sel, st,
                      (const uint32_t*)bpart, desc);
   hipLaunchKernelGGL(k_count<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
-                     (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, counts, kpart, (uint32_t*)nullptr, (uint64_t*)nullptr, st);
-  hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s, (const uint32_t*)kpart, (const uint32_t*)nullptr, st);
+                     (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, counts, kpart, (uint64_t*)nullptr, st);
+  hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s, (const uint32_t*)kpart, st);
   scan_excl_u32(counts, offs, slots, &st->M, scan_tmp, s, &st->nslots);
   hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, (const uint32_t*)offs,

@@ -25,7 +25,9 @@
 //    step whose totals exceed the capacities is re-run with larger ones.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <vector>
 
@@ -279,6 +281,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     CDX_REQUIRE(gx > 0 && gy > 0 && gz > 0, ERR_INVALID, "bins per axis must be positive");
     single_ensure_scene0();
     scenes_[0].g[0] = gx; scenes_[0].g[1] = gy; scenes_[0].g[2] = gz;
+    aabbs_changed();  // deferred bins of a cell-search broadphase would mix the two grids
     update_desc(0);
   }
   void set_envelope(double e) override {
@@ -320,6 +323,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       counts_.n = 0; counts_.m = counts_.a = counts_.p = 0;
       counts_.info = 0;
       cell_ = bins_pending_ = bins_lost_ = false;
+      bins_counted_ = true;
       return;
     }
     stage_broadphase();
@@ -368,6 +372,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void read_bins(uint32_t* bin_counts, uint32_t* sorted_keys, uint32_t* sorted_bodies, uint32_t* active_keys,
                  uint32_t* active_counts) override {
     sync();
+    ensure_bin_counts();
     const uint64_t n = scenes_.empty() ? 0 : scenes_[0].n_shapes;
     // scene 0 starts at shape 0, so its slots are its shape indices
     if (bin_counts && n) CDX_HIP_CHECK(hipMemcpy(bin_counts, counts_buf_.p, n * 4, hipMemcpyDeviceToHost));
@@ -426,9 +431,15 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   }
 
   const StepCounts& counts_full() override {
-    if (cell_ && (counts_.info & INFO_NO_ACTIVE_BINS) && M_ && !bins_lost_) {
+    if (cell_ && (counts_.info & INFO_NO_ACTIVE_BINS) && !bins_lost_) {
       sync();
-      reduce_bins();
+      ensure_bin_counts();
+      if (M_) {
+        reduce_bins();
+      } else {
+        counts_.a = A_ = 0;
+        counts_.info &= ~INFO_NO_ACTIVE_BINS;
+      }
       sync();
     }
     return counts_;
@@ -478,7 +489,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                            &pcount_, &pairs_, &np_v_, &np_hit_, &np_kind_, &np_pos_, &queue_,
                            &queue_tmp_, &qsel_, &qcls_, &tile_parts_, &out_, &perm_,
                            &scan_tmp_, &radix_tmp_, &runs_part_, &chunks_, &sel_, &desc_, &souts_, &longs_, &meta_, &scount_, &soff_, &bins_,
-                           &wpart_, &cell_tab_, &cell_rank_, &cell_lo_, &cell_hi_, &cell_slab_, &cell_last_, &cell_wmax_, &cell_ovf_};
+                           &cell_tab_, &cell_rank_, &cell_lo_, &cell_hi_, &cell_slab_, &cell_last_, &cell_wmax_, &cell_ovf_};
     uint64_t t = 0;
     for (auto* b : all) t += b->bytes;
     return t;
@@ -1006,7 +1017,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   uint64_t chunk_grid() const { return exact_ ? std::max<uint64_t>(nchunks_, 1) : std::max<uint64_t>(chunk_cap_, 1); }
   uint64_t sel_grid() const { return exact_ ? std::max<uint64_t>(nsel_, 1) : std::max<uint64_t>(sel_cap_, 1); }
 
-  // The AABBs change: the bins of a cell-search broadphase can no longer be built.
+  // The AABBs (or the scene grid) change: the bins of a cell-search
+  // broadphase can no longer be counted or built.
   void aabbs_changed() {
     if (bins_pending_) {
       bins_pending_ = false;
@@ -1065,38 +1077,41 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                          (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                          bpart_.as<uint32_t>());
     hipLaunchKernelGGL(k_params<R>, dim3((unsigned)sel_grid()), dim3(PARAMS_BS), 0, s_, (const uint32_t*)sel_.as<uint32_t>(),
-                       (const DevStatus*)st(), (const uint32_t*)bpart_.as<uint32_t>(), desc_.as<SceneDesc<R>>());
+                       st(), (const uint32_t*)bpart_.as<uint32_t>(), desc_.as<SceneDesc<R>>());
     mark(ST_BP_BOUNDS, false);
 
     const uint64_t slots = chunk_grid() * CHUNK;
     scan_tmp_.reserve(std::max<size_t>(scan_tmp_words(slots), 16) * 4);
-    mark(ST_BP_COUNT, true);
-    hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
-                       (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
-                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(),
-                       kpart_.as<uint32_t>(), exact_ ? wpart_.as<uint32_t>() : nullptr, tp_counts(), st());
-    hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(),
-                       exact_ ? (const uint32_t*)wpart_.as<uint32_t>() : nullptr, st());
-    mark(ST_BP_COUNT, false);
-    mark(ST_BP_SCAN, true);
-    scan_excl_u32_tiled(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, tp_counts(), s_,
-                        &st()->nslots);
-    mark(ST_BP_SCAN, false);
-    int key_bits = key_bits_;
     cell_ = false;
     bins_pending_ = false;
     bins_lost_ = false;
-    if (exact_) {
+    bins_counted_ = true;
+    // The cell search needs only the bounds.  A scene whose last broadphase
+    // took it (or that asks for it) decides right after them and counts
+    // the reference grid's bins only when a caller reads bin totals or bins;
+    // any other scene counts first, so the bin search waits for one status
+    // read, not two.
+    const bool cells_first = exact_ && (pair_search_ == PAIRS_CELLS || (pair_search_ == PAIRS_AUTO && last_cells_));
+    if (cells_first) {
       read_status();
-      M_ = hstat_->M;
-      CDX_REQUIRE(M_ < (1ull << 32), ERR_CAPACITY, "more than 2^32 bin entries: use fewer bins per axis");
-      mcap_ = std::max<uint64_t>(M_, 1);
-      key_bits = bits_for(hstat_->key_max);
-      exact_key_bits_ = key_bits;
-      if (pair_search_ != PAIRS_BINS && plan_cells()) {
+      if (plan_cells()) {
+        bins_counted_ = false;
+        M_ = 0;
         stage_cells();
         return;
       }
+    }
+    enqueue_bin_count();
+    int key_bits = key_bits_;
+    if (exact_) {
+      read_status();
+      take_bin_totals();
+      key_bits = exact_key_bits_;
+      if (!cells_first && pair_search_ != PAIRS_BINS && plan_cells()) {
+        stage_cells();
+        return;
+      }
+      last_cells_ = false;
     }
     emit_and_sort_bins(key_bits);
     meta_.reserve(mcap_ * sizeof(PairMeta));
@@ -1156,6 +1171,52 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     }
   }
 
+  // Bins spanned per AABB at the reference grid (Broadphase.cu:150-166),
+  // the largest key, and the scan of the counts (bin entry total st->M).
+  void enqueue_bin_count() {
+    const uint64_t slots = chunk_grid() * CHUNK;
+    mark(ST_BP_COUNT, true);
+    hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
+                       (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
+                       (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(),
+                       kpart_.as<uint32_t>(), tp_counts(), st());
+    hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
+    mark(ST_BP_COUNT, false);
+    mark(ST_BP_SCAN, true);
+    scan_excl_u32_tiled(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, tp_counts(), s_,
+                        &st()->nslots);
+    mark(ST_BP_SCAN, false);
+  }
+
+  // Bin entry total and key width from the status just read (exact mode).
+  void take_bin_totals() {
+    M_ = hstat_->M;
+    CDX_REQUIRE(M_ < (1ull << 32), ERR_CAPACITY, "more than 2^32 bin entries: use fewer bins per axis");
+    mcap_ = std::max<uint64_t>(M_, 1);
+    exact_key_bits_ = bits_for(hstat_->key_max);
+    bins_counted_ = true;
+  }
+
+  // The bin count of a cell-search broadphase that skipped it, run when a
+  // caller reads bin totals or bins: the AABBs and the scene grid are that
+  // broadphase's, so the counts are the ones a count before the search
+  // would have given.
+  void ensure_bin_counts() {
+    if (bins_counted_) return;
+    CDX_REQUIRE(!bins_lost_, ERR_NOT_READY,
+                "the AABBs changed after a cell-search broadphase: run the broadphase again to read its bins");
+    const bool prof = prof_;
+    prof_ = false;  // the stage times stay those of the broadphase
+    // the count adds to the scan's tile sums: clear them (a narrowphase may have moved them since)
+    fill_u32(reinterpret_cast<uint32_t*>(tp_counts()), 0u, 2 * tp_T_, s_);
+    enqueue_bin_count();
+    prof_ = prof;
+    read_status();
+    take_bin_totals();
+    counts_.m = M_;
+    counts_.key_max = hstat_->key_max;
+  }
+
   // (bin key << 32 | shape) entries (Broadphase.cu:172-209) sorted by key
   // (:428-432), plus each shape's broadphase record.
   void emit_and_sort_bins(int key_bits) {
@@ -1177,9 +1238,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_BP_SORT, false);
   }
 
-  // Grid of the cell search for scene 0 (exact mode, after the bin count
-  // pass): cells of edge (1 + 1e-3) * the widest AABB extent along y and z
-  // (enlarged until there are at most about two per shape) and CELL_KX times
+  // Grid of the cell search for scene 0 (exact mode, after the bounds):
+  // cells of edge (1 + 1e-3) * (W + 2^-20 C) along y and z, W the widest
+  // AABB extent and C the largest coordinate magnitude (the allowance for
+  // the rounding of the shift and of W itself; the search's rounding
+  // argument needs the edge to bound the widest shifted extent), enlarged
+  // until there are at most about two cells per shape, and CELL_KX times
   // finer along x.  Taken when the mean occupancy of the coarse cells stays
   // at most CELL_AUTO_DENSITY shapes (always, when the caller asks for the
   // cell search).
@@ -1191,16 +1255,20 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     float w;
     std::memcpy(&w, &hstat_->wmax, 4);
     if (!(w >= 0.f) || !std::isfinite(w)) return false;
-    double ext[3];
+    double ext[3], cmax = 0.0;
     for (int c = 0; c < 3; ++c) {
       // shifted coordinates run from about 0 to bmax + origin
-      const double hi = (double)ord2f(d.bmax_enc[c]) + (double)d.origin[c];
-      if (!std::isfinite(hi)) return false;
+      const double lo = (double)ord2f(d.bmin_enc[c]), up = (double)ord2f(d.bmax_enc[c]);
+      const double hi = up + (double)d.origin[c];
+      if (!std::isfinite(hi) || !std::isfinite(lo)) return false;
       ext[c] = std::max(hi, 0.0);
+      cmax = std::max(cmax, std::max(std::max(std::fabs(lo), std::fabs(up)), ext[c]));
     }
     const double target = 2.0 * (double)n + 64.0;
-    double edge = (double)w * (1.0 + 1e-3);
-    if (!(edge > 0.0)) edge = std::max(std::max(ext[0], ext[1]), std::max(ext[2], 1e-300));
+    // plus 8 of the smallest float denormals: the relative bounds end there
+    double edge = ((double)w + std::ldexp(cmax, -20) + 8.0 * (double)std::numeric_limits<float>::denorm_min()) *
+                  (1.0 + 1e-3);
+    if (!(edge > 0.0)) edge = std::max(std::max(ext[0], ext[1]), std::max(ext[2], 1e-300));  // not reached
     double g[3], cells = 0;
     for (int it = 0; it < 64; ++it) {
       cells = 1;
@@ -1225,8 +1293,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     return true;
   }
 
-  // Pair search by cells (the bin entries are counted and scanned, but
-  // emitted and sorted only when a caller reads the bins).
+  // Pair search by cells (the reference grid's bin entries are counted,
+  // emitted and sorted only when a caller reads bin totals or bins).
   void stage_cells() {
     const uint64_t n = scenes_[0].n_shapes;
     const uint64_t slots = chunk_grid() * CHUNK;
@@ -1246,7 +1314,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     fill_u32(tab, 0u, ntab, s_);
     fill_u32(cell_last_.as<uint32_t>(), 0u, ncells_, s_);
     hipLaunchKernelGGL(k_cell_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
-                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
+                       (const Chunk*)chunks_.as<Chunk>(), st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cg_, tab, rank, cell_last_.as<uint32_t>());
     scan_excl_u32(tab, tab, ntab, &st()->cells, scan_tmp_.as<uint32_t>(), s_);
     mark(ST_BP_EMIT, false);
@@ -1296,11 +1364,13 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        (const uint32_t*)longs_.as<uint32_t>(), long_cap);
     mark(ST_BP_PAIRSORT, false);
     cell_ = true;
+    last_cells_ = true;
     bins_pending_ = true;
     A_ = 0;
+    // bin totals: zero until counted (counts_full, read_bins)
     counts_.n = n; counts_.m = M_; counts_.a = 0; counts_.p = P_;
     counts_.info = INFO_CELL_SEARCH | INFO_NO_ACTIVE_BINS;
-    counts_.key_max = hstat_->key_max;
+    counts_.key_max = bins_counted_ ? hstat_->key_max : 0u;
     counts_.bad = hstat_->bad;
     SceneDesc<R> d;
     CDX_HIP_CHECK(hipMemcpy(&d, desc_.p, sizeof(d), hipMemcpyDeviceToHost));
@@ -1310,6 +1380,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   // Bin entries of a cell-search broadphase, built on request (they
   // depend on the AABBs of that broadphase, so not after they change).
   void ensure_bins() {
+    ensure_bin_counts();
     if (!bins_pending_) {
       CDX_REQUIRE(!bins_lost_, ERR_NOT_READY,
                   "the AABBs changed after a cell-search broadphase: run the broadphase again to read its bins");
@@ -1435,9 +1506,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   void grow_chunk_cap(uint64_t cap) {
     chunk_cap_ = cap;
     chunks_.reserve(chunk_cap_ * sizeof(Chunk));
-    bpart_.reserve(chunk_cap_ * 6 * 4);
+    bpart_.reserve(chunk_cap_ * BPART_WORDS * 4);
     kpart_.reserve(chunk_cap_ * 4);
-    wpart_.reserve(chunk_cap_ * 4);
     counts_buf_.reserve(chunk_cap_ * CHUNK * 4);
     offs_.reserve(chunk_cap_ * CHUNK * 4);
     scount_.reserve(chunk_cap_ * CHUNK * 4);
@@ -1675,10 +1745,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   bool cell_ = false;          // the last broadphase found its pairs by cells
   bool bins_pending_ = false;  // its bin entries are not built yet
   bool bins_lost_ = false;     // and can no longer be: the AABBs changed
+  bool bins_counted_ = true;   // its bin counts, their scan and the key maximum are there
+  bool last_cells_ = false;    // the last exact broadphase took the cell search
   int exact_key_bits_ = 0;
   CellGrid cg_{};
   uint64_t ncells_ = 0;
-  DevBuf wpart_, cell_tab_, cell_rank_, cell_lo_, cell_hi_, cell_slab_, cell_last_, cell_wmax_, cell_ovf_;
+  DevBuf cell_tab_, cell_rank_, cell_lo_, cell_hi_, cell_slab_, cell_last_, cell_wmax_, cell_ovf_;
 };
 
 }  // namespace

@@ -93,7 +93,8 @@ extern "C" {
 #define CDX_FLAG_UNSUPPORTED_SHAPE 0x1u
 #define CDX_FLAG_BAD_BODY_ID 0x2u
 #define CDX_FLAG_CELL_SEARCH 0x100u    /* the broadphase found its pairs by the cell search */
-#define CDX_FLAG_NO_ACTIVE_BINS 0x200u /* active_bins not computed: the AABBs changed after a
+#define CDX_FLAG_NO_ACTIVE_BINS 0x200u /* bin_entries, active_bins and key_max not computed (0):
+                                          the AABBs or the bins per axis changed after a
                                           cell-search broadphase before its counts were read */
 
 /* Pair search of a scene's broadphase (cdx_scene_set_pair_search). */
@@ -221,10 +222,10 @@ int cdx_scene_set_profiling(cdx_scene* scene, int on);
  * Broadphase.cu:484-535 leaves it).  The bins (Broadphase.cu:393-480) walk
  * every bin each AABB spans.  The cell search keys each shape once, by the
  * cell of its AABB's minimum corner in an internal grid of cells at least as
- * wide as the widest AABB, and tests the 27 cells around it; its bin counts
- * and entry total are still computed, while the sorted entries, active
- * bins and occupancy are built when cdx_scene_read_bins or
- * cdx_scene_counts asks for them (before the AABBs change). */
+ * wide as the widest AABB, and tests the cells around it.  After it, the
+ * reference grid's bin counts, entry total, sorted entries, active bins and
+ * occupancy are computed when cdx_scene_counts or cdx_scene_read_bins asks
+ * for them, as long as the AABBs and the bins per axis have not changed. */
 int cdx_scene_set_pair_search(cdx_scene* scene, int mode);
 
 int cdx_scene_aabb(cdx_scene* scene);

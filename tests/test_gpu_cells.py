@@ -126,6 +126,36 @@ def test_auto_takes_cells_for_uniform_spheres(cdx, ctx, oracle, scenes):
     _check(cdx, ctx, oracle, s, cdx.PAIRS_AUTO, True)
 
 
+def test_repeated_cell_steps_count_bins_on_request(cdx, ctx, oracle, scenes):
+    """After a cell-search broadphase the next one decides from the bounds
+    alone and counts the reference grid's bins only when they are read; full
+    steps (a narrowphase between the broadphase and the read) leave the
+    totals and bins the oracle's."""
+    s = scenes.lattice_spheres(side=30, seed=25)
+    print(f"lattice-30 n={s.n}", flush=True)
+    ref = oracle.broadphase(oracle.aabb(s), s.grid)
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.set_pair_search(cdx.PAIRS_AUTO)
+    for it in range(3):
+        print(f"step {it}", flush=True)
+        sc.step()
+        sc.sync()
+        np.testing.assert_array_equal(sc.read_pairs(), ref.pairs)
+        c = sc.counts()
+        assert c.flags & cdx.FLAG_CELL_SEARCH and not (c.flags & cdx.FLAG_NO_ACTIVE_BINS)
+        assert c.pairs == len(ref.pairs)
+        assert c.bin_entries == len(ref.sorted_keys)
+        assert c.active_bins == len(ref.active_keys)
+        assert c.key_max == int(ref.sorted_keys.max())
+        bc, sk, sb, ak, ac = sc.read_bins()
+        np.testing.assert_array_equal(bc, ref.bin_counts)
+        np.testing.assert_array_equal(sk, ref.sorted_keys)
+        np.testing.assert_array_equal(sb, ref.sorted_bodies)
+        np.testing.assert_array_equal(ak, ref.active_keys)
+        np.testing.assert_array_equal(ac, ref.active_counts)
+    sc.close()
+
+
 def test_auto_takes_bins_for_mixed_sizes(cdx, ctx, oracle, scenes):
     s = scenes.mixed_bodies(n=20_000, seed=22)
     _check(cdx, ctx, oracle, s, cdx.PAIRS_AUTO, False)
