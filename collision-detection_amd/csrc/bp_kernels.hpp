@@ -232,14 +232,22 @@ __global__ __launch_bounds__(PARAMS_BS) void k_params(const uint32_t* __restrict
   if (l == 0)
     for (int c = 0; c < BPART_WORDS; ++c) red[c][w] = v[c];
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int t = 1; t < PARAMS_BS / WAVE; ++t) {
-    for (int c = 0; c < 3; ++c) {
-      v[c] = min(v[c], red[c][t]);
-      v[3 + c] = max(v[3 + c], red[3 + c][t]);
-    }
-    v[6] = max(v[6], red[6][t]);
+  if (threadIdx.x >= WAVE) return;
+  // the first wavefront reduces the per-wavefront values by shuffles
+  const bool has = threadIdx.x < PARAMS_BS / WAVE;
+  for (int c = 0; c < 3; ++c) {
+    v[c] = has ? red[c][threadIdx.x] : 0xffffffffu;
+    v[3 + c] = has ? red[3 + c][threadIdx.x] : 0u;
   }
+  v[6] = has ? red[6][threadIdx.x] : 0u;
+  for (int o = 32; o > 0; o >>= 1) {
+    for (int c = 0; c < 3; ++c) {
+      v[c] = min(v[c], (uint32_t)__shfl_xor((int)v[c], o));
+      v[3 + c] = max(v[3 + c], (uint32_t)__shfl_xor((int)v[3 + c], o));
+    }
+    v[6] = max(v[6], (uint32_t)__shfl_xor((int)v[6], o));
+  }
+  if (threadIdx.x != 0) return;
   if (blockIdx.x == 0) st->wmax = v[6];
   int g[3] = {d.gx, d.gy, d.gz};
   for (int c = 0; c < 3; ++c) {
@@ -290,11 +298,19 @@ template <class R> __device__ __forceinline__ Cells cells_of(const BpRec<R>& r, 
   return c;
 }
 
+// Launches of at most this many workgroups fold their per-workgroup totals
+// into the step status by atomics: that many same-address atomics cost less
+// than the extra reduction kernel (a few microseconds per launch); larger
+// grids leave per-workgroup partials to a reduction kernel, where the
+// atomics would serialise.
+constexpr uint64_t FOLD_WGS = 128;
+
 // Bins spanned per AABB (Broadphase.cu:150-159), on coordinates shifted by
 // the scene origin (Broadphase.cu:382-386) and hashed by truncation
 // (HashMin, :125-133); plus the largest key per chunk (the max corner's, keys
-// do not wrap in the dense key range).  With tile_part, also the counts'
-// tile sums for scan_excl_u32_tiled.
+// do not wrap in the dense key range), into kpart for k_reduce_keymax or,
+// with kpart null, into st->key_max by atomicMax.  With tile_part, also the
+// counts' tile sums for scan_excl_u32_tiled.
 template <class R>
 __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunks, const DevStatus* __restrict__ st_in,
                                                  uint64_t NS, const R* __restrict__ aabb,
@@ -329,20 +345,25 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
   if (threadIdx.x == 0) {
     uint32_t m = 0;
     for (int w = 0; w < CHUNK / WAVE; ++w) m = max(m, skm[w]);
-    kpart[blockIdx.x] = m;
+    if (kpart) kpart[blockIdx.x] = m;
+    else if (m) atomicMax(&st->key_max, m);
   }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
-  __shared__ uint32_t red[256];
+constexpr int KEYMAX_BS = 1024;
+
+__global__ __launch_bounds__(KEYMAX_BS) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
+  __shared__ uint32_t red[KEYMAX_BS / WAVE];
   const uint64_t npart = st->nslots / CHUNK;
   uint32_t m = 0;
-  for (uint64_t b = threadIdx.x; b < npart; b += 256) m = max(m, kpart[b]);
-  red[threadIdx.x] = m;
+  for (uint64_t b = threadIdx.x; b < npart; b += KEYMAX_BS) m = max(m, kpart[b]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = m;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int t = 1; t < 256; ++t) m = max(m, red[t]);
-    st->key_max = m;
+  if (threadIdx.x < WAVE) {
+    m = threadIdx.x < KEYMAX_BS / WAVE ? red[threadIdx.x] : 0u;
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    if (threadIdx.x == 0) st->key_max = m;
   }
 }
 
@@ -470,7 +491,9 @@ __device__ __forceinline__ uint32_t local_cell_index(uint32_t kk, const Cells& c
 // the entry d+1 places later is a partner; bit 63: partners beyond 62
 // places) at the entry's emission position j (the first entry of its shape,
 // carried in the shape's record, + its cell's index among the shape's
-// cells), so pairs can be gathered per first shape.
+// cells), so pairs can be gathered per first shape.  The run starts among
+// its own entries (the active bins) go to runs_part for k_sum_runs or, with
+// runs_part null, straight into st->A.
 // k_shape_pairs sums each shape's counts, a scan gives each shape's pair
 // range, and k_pair_emit writes each shape's partners from the masks and
 // sorts them in registers: the pair list comes out ascending per scene
@@ -689,7 +712,8 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     uint32_t r = 0;
 #pragma unroll
     for (int w = 0; w < PAIR_BS / WAVE; ++w) r += sruns[w];
-    runs_part[blockIdx.x] = r;
+    if (runs_part) runs_part[blockIdx.x] = r;
+    else if (r) atomicAdd((unsigned long long*)&st->A, (unsigned long long)r);
   }
   smask[tid] = 0;
   __syncthreads();

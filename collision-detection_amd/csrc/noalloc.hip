@@ -193,7 +193,7 @@ uint64_t broadphase(int dev, void* aabb_aos, uint64_t n, int gx, int gy, int gz,
                      (const uint32_t*)bpart, desc);
   hipLaunchKernelGGL(k_count<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, counts, kpart, (uint64_t*)nullptr, st);
-  hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s, (const uint32_t*)kpart, st);
+  hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(KEYMAX_BS), 0, s, (const uint32_t*)kpart, st);
   scan_excl_u32(counts, offs, slots, &st->M, scan_tmp, s, &st->nslots);
   hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, (const uint32_t*)offs,

@@ -215,9 +215,12 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
   void set_shapes(uint64_t n, const int32_t* T, const void* A, const void* B, const void* C, const void* Rq,
                   const uint32_t* ID) override {
-    check_types(n, T);
+    const bool ext = check_types(n, T);
     sync_internal();
     single_ensure_scene0();
+    const bool had = has_ext_;
+    has_ext_ = scenes_.size() == 1 ? ext : (has_ext_ || ext);  // sticky over several scenes
+    if (has_ext_ != had) graph_valid_ = false;
     ensure_shape_capacity(n);
     scenes_[0].n_shapes = n;
     upload_shapes(0, n, T, A, B, C, Rq, ID, 0);
@@ -503,7 +506,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                      int gz) override {
     CDX_REQUIRE(gx > 0 && gy > 0 && gz > 0, ERR_INVALID, "bins per axis must be positive");
     CDX_REQUIRE(n > 0 && nb > 0, ERR_INVALID, "a scene needs shapes and bodies");
-    check_types(n, T);
+    const bool ext = check_types(n, T);
     for (uint64_t i = 0; i < n; ++i)
       CDX_REQUIRE(ID[i] < nb, ERR_INVALID, "shape " + std::to_string(i) + " has a body id out of range");
     sync_internal();
@@ -526,6 +529,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     upload_bodies(sc.body_begin, nb, pos, rot, active);
     update_desc(s);
     shapes_set_ = bodies_set_ = true;
+    has_ext_ |= ext;
     graph_valid_ = false;
     return s;
   }
@@ -759,14 +763,18 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
 
  private:
   // ------------------------------------------------------------- uploads --
-  static void check_types(uint64_t n, const int32_t* T) {
+  // Returns whether a rectangle, cone or triangle is among the shapes.
+  static bool check_types(uint64_t n, const int32_t* T) {
     CDX_REQUIRE(n < (1ull << 31), ERR_INVALID, "too many shapes (limit 2^31 - 1)");
+    bool ext = false;
     for (uint64_t i = 0; i < n; ++i) {
       int t = T[i];
       CDX_REQUIRE(t >= SPHERE && t <= CAPSULE, ERR_UNSUPPORTED_SHAPE,
                   "shape " + std::to_string(i) + " has type " + std::to_string(t) +
                       ": shape types are SPHERE..TRIANGLEMESH (0-6) and CAPSULE (7)");
+      ext |= t >= RECT && t <= TRIANGLEMESH;
     }
+    return ext;
   }
 
   uint64_t total_selected_shapes() const {
@@ -1116,16 +1124,19 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     emit_and_sort_bins(key_bits);
     meta_.reserve(mcap_ * sizeof(PairMeta));
     runs_part_.reserve((mcap_ / PAIR_BS + 1) * 4);
+    const bool fold_runs = blocks_for(mcap_, PAIR_BS) <= FOLD_WGS;  // active bins by atomics, no k_sum_runs
     mark(ST_BP_PAIRS, true);
     hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(),
-                       runs_part_.as<uint32_t>(), st());
+                       fold_runs ? nullptr : runs_part_.as<uint32_t>(), st());
     mark(ST_BP_PAIRS, false);
-    mark(ST_BP_RUNS, true);
-    hipLaunchKernelGGL(k_sum_runs, dim3(RUNS_GRID), dim3(RUNS_BS), 0, s_, mcap_,
-                       (const uint32_t*)runs_part_.as<uint32_t>(), st());
-    mark(ST_BP_RUNS, false);
+    if (!fold_runs) {
+      mark(ST_BP_RUNS, true);
+      hipLaunchKernelGGL(k_sum_runs, dim3(RUNS_GRID), dim3(RUNS_BS), 0, s_, mcap_,
+                         (const uint32_t*)runs_part_.as<uint32_t>(), st());
+      mark(ST_BP_RUNS, false);
+    }
     mark(ST_BP_PAIRSCAN, true);
     hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
@@ -1175,12 +1186,14 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   // the largest key, and the scan of the counts (bin entry total st->M).
   void enqueue_bin_count() {
     const uint64_t slots = chunk_grid() * CHUNK;
+    const bool fold = chunk_grid() <= FOLD_WGS;  // the key maximum by atomics, no reduction kernel
     mark(ST_BP_COUNT, true);
     hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(),
-                       kpart_.as<uint32_t>(), tp_counts(), st());
-    hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(256), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
+                       fold ? nullptr : kpart_.as<uint32_t>(), tp_counts(), st());
+    if (!fold)
+      hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(KEYMAX_BS), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
     scan_excl_u32_tiled(counts_buf_.as<uint32_t>(), offs_.as<uint32_t>(), slots, &st()->M, tp_counts(), s_,
@@ -1457,7 +1470,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     mark(ST_NP_MPR, true);
     // two launches over the class-sorted queue: the pairs of the five
     // filtered types, then those with a rectangle, cone or triangle (kept at
-    // the queue's end), each kernel holding only the support maps it meets
+    // the queue's end), each kernel holding only the support maps it meets;
+    // the second only when a resident scene holds such a shape
     auto resolve = [&](auto kern, unsigned grid) {
       hipLaunchKernelGGL(kern, dim3(grid), dim3(128), 0, s_, q, dq, (const uint64_t*)pairs_.as<uint64_t>(),
                          (const ShapeRec<R>*)rec_.as<ShapeRec<R>>(), envelope_, o, rq);
@@ -1465,10 +1479,10 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     const unsigned ext_grid = std::min<unsigned>(blocks_for(q, 128), EXT_RESOLVE_GRID);
     if (mode_ == NP_NATIVE) {
       resolve(k_np_resolve<R, R, false>, blocks_for(q, 128));
-      resolve(k_np_resolve<R, R, true>, ext_grid);
+      if (has_ext_) resolve(k_np_resolve<R, R, true>, ext_grid);
     } else {
       resolve(k_np_resolve<R, double, false>, blocks_for(q, 128));
-      resolve(k_np_resolve<R, double, true>, ext_grid);
+      if (has_ext_) resolve(k_np_resolve<R, double, true>, ext_grid);
     }
     mark(ST_NP_MPR, false);
     mark(ST_NP_RESOLVE, false);
@@ -1709,6 +1723,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   // graph of a capacity step
   hipGraphExec_t graph_exec_ = nullptr;
   bool graph_valid_ = false;
+  bool has_ext_ = false;  // a resident scene holds a rectangle, cone or triangle (sticky)
   int graph_key_bits_ = -1, graph_shape_bits_ = -1;
   bool pending_step_ = false, pending_graph_ = false;
 

@@ -168,7 +168,8 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const uint32_t* __restri
 // Tile scan that also forms its own tile offset: the sum of the preceding
 // tile sums, read by the whole workgroup (a few KB from L2 while the tile
 // count is small), so no separate kernel scans the tile sums.  The last
-// tile writes the grand total.
+// tile writes the grand total.  With no tile sums (part null) the input is
+// one tile, whose 64-bit total the workgroup forms itself.
 constexpr uint64_t SCAN_DIRECT_TILES = 2048;
 
 __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* __restrict__ in,
@@ -192,7 +193,12 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* _
 #pragma unroll
   for (int i = 0; i < SCAN_IPT; ++i) s += v[i];
   uint64_t pre_tiles = 0;
-  for (uint64_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BS) pre_tiles += part[i];
+  if (part) {
+    for (uint64_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BS) pre_tiles += part[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) pre_tiles += v[i];
+  }
   pre_tiles = wave_sum_u64(pre_tiles);
   if (threadIdx.x % WAVE == 0) ws[threadIdx.x / WAVE] = pre_tiles;
   uint32_t tot;
@@ -200,6 +206,8 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* _
   uint64_t offset = 0;
 #pragma unroll
   for (int w = 0; w < SCAN_BS / WAVE; ++w) offset += ws[w];
+  const uint64_t total = part ? 0 : offset;  // one tile: ws held its total
+  if (!part) offset = 0;
   uint32_t run = (uint32_t)offset + pre;
 #pragma unroll
   for (int i = 0; i < SCAN_IPT; ++i) {
@@ -208,7 +216,7 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles_direct(const uint32_t* _
     run += x;
   }
   store_run(out, base, n, v);
-  if (blockIdx.x == nb - 1 && threadIdx.x == 0) *d_total = offset + part[blockIdx.x];
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) *d_total = part ? offset + part[blockIdx.x] : total;
 }
 
 __global__ void k_copy_u64(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, uint64_t cap,
@@ -401,6 +409,11 @@ void scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_to
   uint64_t* part = reinterpret_cast<uint64_t*>(tmp);
   if (n == 0) {
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part, (uint64_t)0, d_total);
+    return;
+  }
+  if (nb == 1) {  // one tile: one kernel, no tile sums
+    hipLaunchKernelGGL(k_scan_tiles_direct, dim3(1), dim3(SCAN_BS), 0, s, in, out, n, d_n, (const uint64_t*)nullptr,
+                       d_total);
     return;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(SCAN_BS), 0, s, in, n, d_n, part);
