@@ -29,6 +29,19 @@ def _single(cdx, ctx, s, mode, bins=None):
     return sc.read_pairs(), sc.read_contacts()
 
 
+def _single_counts(cdx, ctx, s, mode):
+    """Bin entries and active bins of the one-scene path (its broadphase
+    searches by bins, as batched steps do)."""
+    sc = cdx.Scene(ctx, s.dtype).load(s)
+    sc.set_pair_search(cdx.PAIRS_BINS)
+    sc.set_mode(mode, 1e-2)
+    sc.step()
+    sc.sync()
+    c = sc.counts()
+    sc.close()
+    return c.bin_entries, c.active_bins
+
+
 def _check_scene(b, j, ref_pairs, ref_contacts):
     got = b.read_pairs(j)
     np.testing.assert_array_equal(got, ref_pairs)
@@ -59,6 +72,29 @@ def test_batch_equals_single(cdx, ctx, scenes, mode, use_graph):
         csum += c.contacts
         _check_scene(b, j, rp, rc)
     assert tot.pairs == psum and tot.contacts == csum
+    # bin totals of the step: the sums of the one-scene totals (every scene
+    # keys its bins from its own key base, so no run spans two scenes)
+    ms, as_ = zip(*[_single_counts(cdx, ctx, s, m) for s in ss])
+    assert tot.bin_entries == sum(ms) and tot.active_bins == sum(as_)
+
+
+@pytest.mark.parametrize("sizes", [[300, 700, 1000], [20000, 30000]])
+def test_batch_bin_totals(cdx, ctx, scenes, sizes):
+    """Steps small enough to fold the active-bin count and the key maximum
+    by atomics, and steps that reduce per-workgroup partials instead: the
+    same totals as the one-scene path."""
+    ss = _scenes(scenes, sizes=sizes, seed=300)
+    print(f"batch sizes={sizes}", flush=True)
+    b = cdx.Batch(ctx, np.float64)
+    for s in ss:
+        b.add_scene(s)
+    for rep in range(2):
+        b.step(True)
+        tot = b.finish()
+        ms, as_ = zip(*[_single_counts(cdx, ctx, s, cdx.NP_MIXED) for s in ss])
+        assert tot.bin_entries == sum(ms) and tot.active_bins == sum(as_)
+        for j, s in enumerate(ss):
+            assert b.scene_counts(j).bin_entries == ms[j]
 
 
 def test_batch_small_scenes_match_oracle(cdx, ctx, oracle, scenes):
