@@ -352,19 +352,24 @@ __global__ __launch_bounds__(CHUNK) void k_count(const Chunk* __restrict__ chunk
 
 constexpr int KEYMAX_BS = 1024;
 
-__global__ __launch_bounds__(KEYMAX_BS) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
-  __shared__ uint32_t red[KEYMAX_BS / WAVE];
-  const uint64_t npart = st->nslots / CHUNK;
+// The largest key over the chunk maxima, by one workgroup of BS threads.
+template <int BS>
+__device__ __forceinline__ void block_keymax(const uint32_t* __restrict__ kpart, uint64_t npart, DevStatus* st) {
+  __shared__ uint32_t red[BS / WAVE];
   uint32_t m = 0;
-  for (uint64_t b = threadIdx.x; b < npart; b += KEYMAX_BS) m = max(m, kpart[b]);
+  for (uint64_t b = threadIdx.x; b < npart; b += BS) m = max(m, kpart[b]);
   for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
   if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = m;
   __syncthreads();
   if (threadIdx.x < WAVE) {
-    m = threadIdx.x < KEYMAX_BS / WAVE ? red[threadIdx.x] : 0u;
+    m = threadIdx.x < BS / WAVE ? red[threadIdx.x] : 0u;
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
     if (threadIdx.x == 0) st->key_max = m;
   }
+}
+
+__global__ __launch_bounds__(KEYMAX_BS) void k_reduce_keymax(const uint32_t* __restrict__ kpart, DevStatus* st) {
+  block_keymax<KEYMAX_BS>(kpart, st->nslots / CHUNK, st);
 }
 
 // (bin, body) entries in the reference's store order (Broadphase.cu:
@@ -384,11 +389,16 @@ __global__ __launch_bounds__(CHUNK) void k_emit_keys(const Chunk* __restrict__ c
                                                      const uint32_t* __restrict__ offs,
                                                      const uint32_t* __restrict__ counts, uint64_t mcap,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ pcount,
-                                                     BpRec<R>* __restrict__ brec, DevStatus* st) {
+                                                     BpRec<R>* __restrict__ brec, const uint32_t* __restrict__ kpart,
+                                                     DevStatus* st) {
   static_assert(sizeof(BpRec<R>) * CHUNK <= sizeof(uint64_t) * EMIT_LDS, "records fit the key staging area");
   __shared__ alignas(64) uint64_t sk[EMIT_LDS];  // the chunk's entries, then its records
   __shared__ uint64_t s_total;
   if ((uint64_t)blockIdx.x * CHUNK >= st_in->nslots) return;
+  // with kpart, workgroup 0 also reduces the count pass's chunk maxima
+  // (steps whose key width is fixed beforehand need the maximum only as a
+  // reported total, so no kernel of its own runs for it)
+  if (kpart && blockIdx.x == 0) block_keymax<CHUNK>(kpart, st_in->nslots / CHUNK, st);
   const Chunk ch = chunks[blockIdx.x];
   if (ch.len == 0) return;
   const uint64_t slot0 = (uint64_t)blockIdx.x * CHUNK;
@@ -492,8 +502,7 @@ __device__ __forceinline__ uint32_t local_cell_index(uint32_t kk, const Cells& c
 // places) at the entry's emission position j (the first entry of its shape,
 // carried in the shape's record, + its cell's index among the shape's
 // cells), so pairs can be gathered per first shape.  The run starts among
-// its own entries (the active bins) go to runs_part for k_sum_runs or, with
-// runs_part null, straight into st->A.
+// its own entries (the active bins) go to runs_part, summed by sum_runs.
 // k_shape_pairs sums each shape's counts, a scan gives each shape's pair
 // range, and k_pair_emit writes each shape's partners from the masks and
 // sorts them in registers: the pair list comes out ascending per scene
@@ -712,8 +721,7 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
     uint32_t r = 0;
 #pragma unroll
     for (int w = 0; w < PAIR_BS / WAVE; ++w) r += sruns[w];
-    if (runs_part) runs_part[blockIdx.x] = r;
-    else if (r) atomicAdd((unsigned long long*)&st->A, (unsigned long long)r);
+    runs_part[blockIdx.x] = r;
   }
   smask[tid] = 0;
   __syncthreads();
@@ -825,27 +833,26 @@ __global__ __launch_bounds__(PAIR_BS) void k_pairs(uint64_t mcap, const uint64_t
 }
 
 // Active bins (reduce_by_key, Broadphase.cu:439-443): the pair count pass
-// counts the run starts among each workgroup's own entries; this sums those
-// counts.  A fixed grid of RUNS_GRID workgroups strides over them, so only
-// RUNS_GRID atomics meet at the one counter (same-address atomics are
-// serialised device-wide).
-constexpr int RUNS_BS = 256;
+// counts the run starts among each workgroup's own entries; the first
+// workgroups of the pair offset pass (k_shape_pairs) sum those counts,
+// striding over them, so at most RUNS_GRID atomics meet at the one counter
+// (same-address atomics are serialised device-wide) and no kernel of its
+// own runs for it.
 constexpr unsigned RUNS_GRID = 64;
 
-__global__ __launch_bounds__(RUNS_BS) void k_sum_runs(uint64_t mcap, const uint32_t* __restrict__ runs_part,
-                                                      DevStatus* st) {
-  __shared__ uint32_t red[RUNS_BS / WAVE];
-  const uint64_t m = cap_count(mcap, &st->M);
-  const uint64_t nb = (m + PAIR_BS - 1) / PAIR_BS;
+__device__ __forceinline__ void sum_runs(uint64_t nparts, const uint32_t* __restrict__ runs_part, uint64_t nblocks,
+                                         DevStatus* st) {
+  const uint64_t g = min((uint64_t)RUNS_GRID, nblocks);
+  if (blockIdx.x >= g) return;
+  __shared__ uint64_t red[CHUNK / WAVE];
   uint64_t c = 0;
-  for (uint64_t b = (uint64_t)blockIdx.x * RUNS_BS + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * RUNS_BS)
-    c += runs_part[b];
+  for (uint64_t b = (uint64_t)blockIdx.x * CHUNK + threadIdx.x; b < nparts; b += g * CHUNK) c += runs_part[b];
   for (int o = 32; o > 0; o >>= 1) c += (uint64_t)__shfl_xor((long long)c, o);
-  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = (uint32_t)c;
+  if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint64_t t = 0;
-    for (int w = 0; w < RUNS_BS / WAVE; ++w) t += red[w];
+    for (int w = 0; w < CHUNK / WAVE; ++w) t += red[w];
     if (t) atomicAdd((unsigned long long*)&st->A, (unsigned long long)t);
   }
 }
@@ -892,22 +899,24 @@ __global__ __launch_bounds__(256) void k_bins_occupancy(uint64_t a, uint64_t m, 
 // sweep into LDS (one memory latency), then each shape sums its own.
 constexpr int SHAPE_PAIRS_LDS = 4096;
 
-__global__ __launch_bounds__(CHUNK) void k_shape_pairs(const Chunk* __restrict__ chunks,
-                                                       const DevStatus* __restrict__ st,
+// With runs_part, the first workgroups also sum the active bins (sum_runs).
+__global__ __launch_bounds__(CHUNK) void k_shape_pairs(const Chunk* __restrict__ chunks, DevStatus* __restrict__ st,
                                                        const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ offs,
                                                        const uint32_t* __restrict__ pcount, uint64_t mcap,
-                                                       uint32_t* __restrict__ scount, uint64_t* __restrict__ tile_part) {
+                                                       uint32_t* __restrict__ scount, uint64_t* __restrict__ tile_part,
+                                                       const uint32_t* __restrict__ runs_part) {
   __shared__ uint32_t sp[SHAPE_PAIRS_LDS];
   __shared__ uint64_t s_e0, s_e1;
   const uint64_t slot = (uint64_t)blockIdx.x * CHUNK + threadIdx.x;
   if ((uint64_t)blockIdx.x * CHUNK >= st->nslots) return;
+  const uint64_t m = cap_count(mcap, &st->M);
+  if (runs_part) sum_runs((m + PAIR_BS - 1) / PAIR_BS, runs_part, st->nslots / CHUNK, st);
   const Chunk ch = chunks[blockIdx.x];
   if (ch.len == 0) {
     scount[slot] = 0;
     return;
   }
-  const uint64_t m = cap_count(mcap, &st->M);
   const bool mine = threadIdx.x < ch.len;
   const uint64_t e0 = mine ? offs[slot] : 0, e1 = mine ? min(e0 + counts[slot], m) : 0;
   if (threadIdx.x == 0) s_e0 = min(e0, m);

@@ -22,7 +22,7 @@ enum Stage : int {
   ST_BP_SCAN,
   ST_BP_EMIT,
   ST_BP_SORT,
-  ST_BP_RUNS,
+  ST_BP_RUNS,   // kept for the stage numbering: active bins are summed inside ST_BP_PAIRSCAN (reads 0)
   ST_BP_PAIRS,  // pair counting pass
   ST_BP_PAIRSORT,
   ST_NP_MAIN,

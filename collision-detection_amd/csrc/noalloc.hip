@@ -197,14 +197,13 @@ uint64_t broadphase(int dev, void* aabb_aos, uint64_t n, int gx, int gy, int gz,
   scan_excl_u32(counts, offs, slots, &st->M, scan_tmp, s, &st->nslots);
   hipLaunchKernelGGL(k_emit_keys<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, NS, (const R*)aabb, (const SceneDesc<R>*)desc, (const uint32_t*)offs,
-                     (const uint32_t*)counts, mcap, keys, pcount, brec, st);
+                     (const uint32_t*)counts, mcap, keys, pcount, brec, (const uint32_t*)nullptr, st);
   if (radix_sort_u64(keys, keys_tmp, mcap, 32, key_bits, radix_tmp, s, &st->M)) std::swap(keys, keys_tmp);
   hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap, PAIR_BS)), dim3(PAIR_BS), 0, s, mcap, (const uint64_t*)keys,
                      (const BpRec<R>*)brec, (const SceneDesc<R>*)desc, pcount, meta, runs_part, st);
-  hipLaunchKernelGGL(k_sum_runs, dim3(RUNS_GRID), dim3(RUNS_BS), 0, s, mcap, (const uint32_t*)runs_part, st);
-  hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
-                     (const DevStatus*)st, (const uint32_t*)counts, (const uint32_t*)offs, (const uint32_t*)pcount,
-                     mcap, scount, (uint64_t*)nullptr);
+  hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks, st,
+                     (const uint32_t*)counts, (const uint32_t*)offs, (const uint32_t*)pcount, mcap, scount,
+                     (uint64_t*)nullptr, (const uint32_t*)runs_part);
   scan_excl_u32(scount, soff, slots, &st->P, scan_tmp, s, &st->nslots);
   hipLaunchKernelGGL(k_pair_emit<R>, dim3((unsigned)nchunks), dim3(CHUNK), 0, s, (const Chunk*)chunks,
                      (const DevStatus*)st, (const uint32_t*)counts, (const uint32_t*)offs, (const uint32_t*)pcount,

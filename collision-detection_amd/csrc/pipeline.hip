@@ -8,7 +8,7 @@
 //   scan #1      Broadphase.cu:403-407     -> scan_excl_u32 over chunk slots
 //   key store    Broadphase.cu:172-209     -> k_emit_keys
 //   sort_by_key  Broadphase.cu:428-432     -> radix_sort_u64 on (key<<32 | shape)
-//   reduce_by_key + scan #2  :439-480      -> run starts counted in k_pairs, summed by k_sum_runs
+//   reduce_by_key + scan #2  :439-480      -> run starts counted in k_pairs, summed in k_shape_pairs
 //   pair count / store / sort / unique :484-535
 //                                          -> k_pairs (owner-bin dedup), k_shape_pairs + scan,
 //                                             k_pair_emit (per-shape sorted groups), k_pair_groups_long
@@ -1124,24 +1124,18 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     emit_and_sort_bins(key_bits);
     meta_.reserve(mcap_ * sizeof(PairMeta));
     runs_part_.reserve((mcap_ / PAIR_BS + 1) * 4);
-    const bool fold_runs = blocks_for(mcap_, PAIR_BS) <= FOLD_WGS;  // active bins by atomics, no k_sum_runs
     mark(ST_BP_PAIRS, true);
     hipLaunchKernelGGL(k_pairs<R>, dim3(blocks_for(mcap_, PAIR_BS)), dim3(PAIR_BS), 0, s_, mcap_,
                        (const uint64_t*)keys_.as<uint64_t>(), (const BpRec<R>*)brec_.as<BpRec<R>>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), pcount_.as<uint32_t>(), meta_.as<PairMeta>(),
-                       fold_runs ? nullptr : runs_part_.as<uint32_t>(), st());
+                       runs_part_.as<uint32_t>(), st());
     mark(ST_BP_PAIRS, false);
-    if (!fold_runs) {
-      mark(ST_BP_RUNS, true);
-      hipLaunchKernelGGL(k_sum_runs, dim3(RUNS_GRID), dim3(RUNS_BS), 0, s_, mcap_,
-                         (const uint32_t*)runs_part_.as<uint32_t>(), st());
-      mark(ST_BP_RUNS, false);
-    }
     mark(ST_BP_PAIRSCAN, true);
     hipLaunchKernelGGL(k_shape_pairs, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
-                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(),
+                       (const Chunk*)chunks_.as<Chunk>(), st(),
                        (const uint32_t*)counts_buf_.as<uint32_t>(), (const uint32_t*)offs_.as<uint32_t>(),
-                       (const uint32_t*)pcount_.as<uint32_t>(), mcap_, scount_.as<uint32_t>(), tp_spairs());
+                       (const uint32_t*)pcount_.as<uint32_t>(), mcap_, scount_.as<uint32_t>(), tp_spairs(),
+                       (const uint32_t*)runs_part_.as<uint32_t>());
     scan_excl_u32_tiled(scount_.as<uint32_t>(), soff_.as<uint32_t>(), slots, &st()->P, tp_spairs(), s_,
                         &st()->nslots);
     mark(ST_BP_PAIRSCAN, false);
@@ -1186,13 +1180,17 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
   // the largest key, and the scan of the counts (bin entry total st->M).
   void enqueue_bin_count() {
     const uint64_t slots = chunk_grid() * CHUNK;
-    const bool fold = chunk_grid() <= FOLD_WGS;  // the key maximum by atomics, no reduction kernel
+    // The key maximum: a step with a fixed key width leaves the chunk maxima
+    // to the key emission's first workgroup; an exact step needs it before
+    // the emission (for the key width), from atomics in small launches or
+    // from k_reduce_keymax.
+    const bool fold = exact_ && chunk_grid() <= FOLD_WGS;
     mark(ST_BP_COUNT, true);
     hipLaunchKernelGGL(k_count<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_, (const Chunk*)chunks_.as<Chunk>(),
                        (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), counts_buf_.as<uint32_t>(),
                        fold ? nullptr : kpart_.as<uint32_t>(), tp_counts(), st());
-    if (!fold)
+    if (exact_ && !fold)
       hipLaunchKernelGGL(k_reduce_keymax, dim3(1), dim3(KEYMAX_BS), 0, s_, (const uint32_t*)kpart_.as<uint32_t>(), st());
     mark(ST_BP_COUNT, false);
     mark(ST_BP_SCAN, true);
@@ -1241,7 +1239,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
                        (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), (const uint32_t*)offs_.as<uint32_t>(),
                        (const uint32_t*)counts_buf_.as<uint32_t>(), mcap_, keys_.as<uint64_t>(),
-                       pcount_.as<uint32_t>(), brec_.as<BpRec<R>>(), st());
+                       pcount_.as<uint32_t>(), brec_.as<BpRec<R>>(),
+                       exact_ ? (const uint32_t*)nullptr : (const uint32_t*)kpart_.as<uint32_t>(), st());
     mark(ST_BP_EMIT, false);
     radix_tmp_.reserve(std::max<size_t>(radix_tmp_words(mcap_, key_bits), 16) * 4);
     mark(ST_BP_SORT, true);

@@ -78,7 +78,7 @@ extern "C" {
 #define CDX_STAGE_BP_SCAN 3
 #define CDX_STAGE_BP_EMIT 4
 #define CDX_STAGE_BP_SORT 5
-#define CDX_STAGE_BP_RUNS 6
+#define CDX_STAGE_BP_RUNS 6 /* 0: the active bins are summed inside CDX_STAGE_BP_PAIRSCAN */
 #define CDX_STAGE_BP_PAIRS 7 /* pair counting pass; the scan and the emission pass are 12 and 13 */
 #define CDX_STAGE_BP_PAIRSORT 8
 #define CDX_STAGE_NP_MAIN 9
