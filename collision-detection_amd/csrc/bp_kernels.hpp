@@ -23,8 +23,11 @@ __device__ __forceinline__ uint64_t cap_count(uint64_t cap, const uint64_t* d_n)
 // per workgroup: the float min / max over the workgroup's AABB corners, as
 // order-preserving uints, then the widest AABB extent of the workgroup (float
 // bits: non-negative floats order as unsigned integers), which sizes the cell
-// search's grid.
+// search's grid, with bit 31 (WIDTH_INVERTED) set when an AABB's max corner
+// is not at or above its min corner on some axis (such a scene takes the bin
+// search, whose count reports the AABBs the reference grid cannot bin).
 constexpr int BPART_WORDS = 7;
+constexpr uint32_t WIDTH_INVERTED = 0x80000000u;
 
 __device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[3], uint32_t wm, uint32_t* part) {
   __shared__ uint32_t smn[3][CHUNK / WAVE], smx[3][CHUNK / WAVE], swm[CHUNK / WAVE];
@@ -56,7 +59,8 @@ __device__ __forceinline__ void block_bounds_commit(uint32_t mn[3], uint32_t mx[
 }
 
 // Folds one AABB into the corner bounds and the widest extent (NaN extents
-// count as 0: such AABBs make the scene bounds non-finite and are reported).
+// count as 0: such AABBs make the scene bounds non-finite and are reported),
+// flagging an inverted or NaN AABB (WIDTH_INVERTED).
 __device__ __forceinline__ void bounds_fold(uint32_t mn[3], uint32_t mx[3], uint32_t& wm, float lx, float ly,
                                             float lz, float hx, float hy, float hz) {
   uint32_t u;
@@ -68,6 +72,7 @@ __device__ __forceinline__ void bounds_fold(uint32_t mn[3], uint32_t mx[3], uint
   u = f2ord(hz); mn[2] = min(mn[2], u); mx[2] = max(mx[2], u);
   const float w = fmaxf(fmaxf(hx - lx, hy - ly), hz - lz);
   if (w > 0.f) wm = max(wm, __float_as_uint(w));
+  if (!(hx >= lx) || !(hy >= ly) || !(hz >= lz)) wm |= WIDTH_INVERTED;
 }
 
 // One pass over shapes (SoA) and body poses: world AABB (AABBGenerator.cu:

@@ -24,7 +24,7 @@ struct DevStatus {
   uint32_t key_max;
   uint32_t nsel;     // scenes in this step
   uint32_t nbig;     // pair groups too long for the in-register sort
-  uint32_t wmax;     // widest AABB extent of the first stepped scene (float bits): the cell grid
+  uint32_t wmax;     // widest AABB extent of the first stepped scene (float bits | WIDTH_INVERTED)
   uint64_t cells;    // shapes counted into the cell search's table (its scan total)
   uint32_t novf;     // cell search: shapes with more partners than the slab holds
   uint32_t pad;

@@ -85,11 +85,10 @@ __device__ __forceinline__ uint32_t cell_key(const uint32_t c[3], const CellGrid
 // shape's place in cell order: a counting sort with no sort pass.  The
 // order within a cell is the order of the atomics; nothing depends on it
 // (every shape's partner group is sorted, and the walk reads whole cells).
-// The bin count does not run before this search, so the pass also reports
-// (st->bad bit 2) the AABBs the reference grid cannot bin, as k_count does.
 template <class R>
-__global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ chunks, DevStatus* __restrict__ st,
-                                                     uint64_t NS, const R* __restrict__ aabb,
+__global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ chunks,
+                                                     const DevStatus* __restrict__ st, uint64_t NS,
+                                                     const R* __restrict__ aabb,
                                                      const SceneDesc<R>* __restrict__ desc, CellGrid cg,
                                                      uint32_t* __restrict__ ccount, uint32_t* __restrict__ rank,
                                                      uint32_t* __restrict__ last) {
@@ -97,9 +96,7 @@ __global__ __launch_bounds__(CHUNK) void k_cell_keys(const Chunk* __restrict__ c
   const Chunk ch = chunks[blockIdx.x];
   if (threadIdx.x >= ch.len) return;
   const uint64_t i = (uint64_t)ch.begin + threadIdx.x;
-  const SceneDesc<R>& d = desc[ch.scene];
-  const BpRec<R> r = shifted_rec(aabb, NS, i, d, ch.scene);
-  if (!cells_ok(cells_of(r, d), d.gx, d.gy, d.gz)) atomicOr(&st->bad, 4u);
+  const BpRec<R> r = shifted_rec(aabb, NS, i, desc[ch.scene], ch.scene);
   uint32_t c[3];
   cell_of_box(r.lo, cg, c);
   const uint32_t key = cell_key(c, cg);

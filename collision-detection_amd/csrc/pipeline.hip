@@ -1264,9 +1264,17 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     if (n < 2) return false;
     SceneDesc<R> d;
     CDX_HIP_CHECK(hipMemcpy(&d, desc_.p, sizeof(d), hipMemcpyDeviceToHost));
+    // The bin count, which reports AABBs the reference grid cannot bin, does
+    // not run before the cell search.  With no inverted AABB, finite scene
+    // bounds and finite, positive inverse bin sizes every AABB is finite and
+    // maps into the grid (cells_ok), so none goes unreported; any other scene
+    // counts first.
+    if (hstat_->wmax & WIDTH_INVERTED) return false;
     float w;
     std::memcpy(&w, &hstat_->wmax, 4);
     if (!(w >= 0.f) || !std::isfinite(w)) return false;
+    for (int c = 0; c < 3; ++c)
+      if (!(d.inv[c] > (R)0) || !std::isfinite((double)d.inv[c])) return false;
     double ext[3], cmax = 0.0;
     for (int c = 0; c < 3; ++c) {
       // shifted coordinates run from about 0 to bmax + origin
@@ -1326,7 +1334,7 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
     fill_u32(tab, 0u, ntab, s_);
     fill_u32(cell_last_.as<uint32_t>(), 0u, ncells_, s_);
     hipLaunchKernelGGL(k_cell_keys<R>, dim3((unsigned)chunk_grid()), dim3(CHUNK), 0, s_,
-                       (const Chunk*)chunks_.as<Chunk>(), st(), NSC_, (const R*)aabb_.as<R>(),
+                       (const Chunk*)chunks_.as<Chunk>(), (const DevStatus*)st(), NSC_, (const R*)aabb_.as<R>(),
                        (const SceneDesc<R>*)desc_.as<SceneDesc<R>>(), cg_, tab, rank, cell_last_.as<uint32_t>());
     scan_excl_u32(tab, tab, ntab, &st()->cells, scan_tmp_.as<uint32_t>(), s_);
     mark(ST_BP_EMIT, false);

@@ -181,6 +181,38 @@ def test_bins_unavailable_after_the_aabbs_change(cdx, ctx, oracle, scenes):
     sc.close()
 
 
+def test_bins_unavailable_after_the_grid_changes(cdx, ctx, oracle, scenes):
+    """The bins of a cell-search broadphase are counted on request at the
+    grid of that broadphase; after new bins per axis they are unavailable."""
+    s = scenes.lattice_spheres(side=12, seed=26)
+    sc = _run(cdx, ctx, s, cdx.PAIRS_CELLS)
+    sc.set_bins(*[g + 1 for g in s.grid])
+    c = sc.counts()
+    assert c.flags & cdx.FLAG_CELL_SEARCH and c.flags & cdx.FLAG_NO_ACTIVE_BINS
+    assert c.bin_entries == 0 and c.active_bins == 0
+    with pytest.raises(cdx.CdxError):
+        sc.read_bins()
+    ref = oracle.broadphase(oracle.aabb(s), s.grid)
+    np.testing.assert_array_equal(sc.read_pairs(), ref.pairs)
+    sc.close()
+
+
+def test_inverted_aabb_reported_after_cell_steps(cdx, ctx, scenes):
+    """A scene whose last broadphase took the cell search still reports an
+    AABB whose max corner lies below its min corner (the bounds pass flags
+    it, so the bin count runs and reports it)."""
+    s = scenes.lattice_spheres(side=12, seed=27)
+    sc = _run(cdx, ctx, s, cdx.PAIRS_AUTO)
+    assert sc.counts().flags & cdx.FLAG_CELL_SEARCH
+    ab = sc.read_aabb()
+    ab[s.n + 5] = ab[5] - 1.0  # shape 5: max corner one unit below its min corner
+    sc.set_aabbs(ab)
+    with pytest.raises(cdx.CdxError):
+        sc.broadphase()
+        sc.sync()
+    sc.close()
+
+
 def test_step_contacts_equal_across_searches(cdx, ctx, oracle, scenes):
     """Full steps (AABB, broadphase, NATIVE narrowphase) through each search."""
     s = scenes.lattice_spheres(side=24, seed=24, dtype=np.float64)
