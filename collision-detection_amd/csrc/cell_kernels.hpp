@@ -329,7 +329,8 @@ __global__ __launch_bounds__(CELL_BS) void k_cell_find(uint64_t n, const CellLo<
     cell_of_box(A.lo, cg, c);
     cell_row_range(c, cg, cstart, wmax, A.id, 0, sn, ln);
   }
-  // partners go to the slab row four at a time (hb holds those not yet stored)
+  // partners go to the slab row four at a time: hb is a shift register whose
+  // last (cnt & 3) words hold the partners not yet stored, oldest first
   uint32_t cnt = 0;
   uint4* row4 = reinterpret_cast<uint4*>(slab + (uint64_t)A.id * CELL_SLAB);
   uint4 hb = make_uint4(0u, 0u, 0u, 0u);
@@ -388,12 +389,8 @@ __global__ __launch_bounds__(CELL_BS) void k_cell_find(uint64_t n, const CellLo<
           const CellLo<R>& L = tlo[j];
           const CellHi<R>& H = thi[j];
           if (cell_hit(A, L, H)) {
-            const uint32_t q = cnt & 3u;
-            hb.x = q == 0u ? L.id : hb.x;
-            hb.y = q == 1u ? L.id : hb.y;
-            hb.z = q == 2u ? L.id : hb.z;
-            hb.w = q == 3u ? L.id : hb.w;
-            if (q == 3u && cnt < (uint32_t)CELL_SLAB) row4[cnt >> 2] = hb;
+            hb = make_uint4(hb.y, hb.z, hb.w, L.id);
+            if ((cnt & 3u) == 3u && cnt < (uint32_t)CELL_SLAB) row4[cnt >> 2] = hb;
             ++cnt;
           }
         }
@@ -401,8 +398,11 @@ __global__ __launch_bounds__(CELL_BS) void k_cell_find(uint64_t n, const CellLo<
       }
     }
   }
-  // the last partial group (slots past cnt are never read)
-  if ((cnt & 3u) && cnt < (uint32_t)CELL_SLAB) row4[cnt >> 2] = hb;
+  // the last partial group: its k = cnt & 3 partners are hb's last k words,
+  // moved to the front (slots past cnt are never read)
+  const uint32_t k = cnt & 3u;
+  if (k && cnt < (uint32_t)CELL_SLAB)
+    row4[cnt >> 2] = make_uint4(k == 1u ? hb.w : (k == 2u ? hb.z : hb.y), k == 2u ? hb.w : hb.z, hb.w, 0u);
   if (!valid) return;
   scount[A.id] = cnt;
   if (cnt > (uint32_t)CELL_SLAB) ovf[atomicAdd(&st->novf, 1u)] = (uint32_t)p;
