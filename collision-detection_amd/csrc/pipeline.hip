@@ -1277,7 +1277,8 @@ template <class R> class Engine final : public SceneBase, public BatchBase {
       if (!(d.inv[c] > (R)0) || !std::isfinite((double)d.inv[c])) return false;
     double ext[3], cmax = 0.0;
     for (int c = 0; c < 3; ++c) {
-      // shifted coordinates run from about 0 to bmax + origin
+      // shifted coordinates lie in [0, bmax + origin] up to rounding (from
+      // 2 bmin when bmin > 0: the origin is |bmin|, Broadphase.cu:377-381)
       const double lo = (double)ord2f(d.bmin_enc[c]), up = (double)ord2f(d.bmax_enc[c]);
       const double hi = up + (double)d.origin[c];
       if (!std::isfinite(hi) || !std::isfinite(lo)) return false;
